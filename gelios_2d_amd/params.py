@@ -1,0 +1,114 @@
+"""Sweep parameters: the ctypes mirror of ``gel_params`` (include/gelios.h).
+
+The reference keeps these as compile-time macros (Help.h:39-64) and literals inside
+Gran::Get_par (Gran.cpp:234-282); here they are run-time fields so that the
+synthetic heliosphere workloads can set their own inflow while the parity runs use
+the reference's values (:func:`reference_params`).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+GEL_F_USUALY, GEL_F_INNER_SPHERE, GEL_F_EXTERN, GEL_F_INPUT, GEL_F_AXIS, GEL_F_UPPER_WALL = range(6)
+GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_C_NO = range(7)
+GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
+GEL_PHYS_GAS = 0
+GEL_SRC_NONE, GEL_SRC_MK1 = 0, 1
+GEL_MK1_NFIELDS = 19
+NVAR_GAS = 5
+
+# Reference constants (Help.h).
+RR_ = 390.222
+CHI_REAL = 36.1275
+VELOSITY_INF = -2.54338
+A_2 = 0.1307345665
+KN_ = 6.0
+N_P_LISM = 3.0
+GA = 5.0 / 3.0
+
+
+class GelParams(ctypes.Structure):
+    """Field-for-field mirror of ``struct gel_params``."""
+
+    _fields_ = [
+        ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("dt0", ctypes.c_double),
+        ("r_skip", ctypes.c_double),
+        ("flux", ctypes.c_int),
+        ("physics", ctypes.c_int),
+        ("source_mode", ctypes.c_int),
+        ("reserved0", ctypes.c_int),
+        ("in_rho_div", ctypes.c_double),
+        ("in_p_a", ctypes.c_double),
+        ("in_p_den", ctypes.c_double),
+        ("in_u", ctypes.c_double),
+        ("in_q", ctypes.c_double),
+        ("inf_rho", ctypes.c_double),
+        ("inf_p", ctypes.c_double),
+        ("inf_u", ctypes.c_double),
+        ("inf_v", ctypes.c_double),
+        ("inf_q", ctypes.c_double),
+        ("extern_u", ctypes.c_double),
+        ("src_coef", ctypes.c_double),
+        ("sig_a", ctypes.c_double),
+        ("sig_b", ctypes.c_double),
+    ]
+
+    def copy(self) -> "GelParams":
+        p = GelParams()
+        ctypes.pointer(p)[0] = self
+        return p
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def reference_params() -> GelParams:
+    """The reference's own constants: Help.h macros, Gran::Get_par literals, HLL flux
+    as in Setka::Go_5_komponent__MK2 (``met = 0``, Setka.cpp:10211)."""
+    p = GelParams()
+    p.gamma = GA
+    p.cfl = 0.1                       # kurant, Help.h:57
+    p.dt0 = 0.00000001                # T[0] = T[1], Setka.cpp:10128-10129
+    p.r_skip = 38.0 / RR_             # R11_, Help.h:81
+    p.flux = GEL_FLUX_HLL
+    p.physics = GEL_PHYS_GAS
+    p.source_mode = GEL_SRC_NONE
+    p.in_rho_div = CHI_REAL * CHI_REAL          # kv(chi_real), Gran.cpp:240
+    p.in_p_a = CHI_REAL                         # ro*chi*chi/(ggg*10*10), Gran.cpp:241
+    p.in_p_den = GA * 10.0 * 10.0
+    p.in_u = CHI_REAL
+    p.in_q = 1.0
+    p.inf_rho, p.inf_p, p.inf_u, p.inf_v, p.inf_q = 1.0, 1.0, VELOSITY_INF, 0.0, 100.0
+    p.extern_u = -0.1                           # Gran.cpp:216-219
+    p.src_coef = N_P_LISM / KN_
+    p.sig_a = 1.0
+    p.sig_b = A_2
+    return p
+
+
+def heliosphere_params(*, flux: int = GEL_FLUX_HLLC, cfl: float = 0.4, rho_in: float = 1.0,
+                       u_in: float = 10.0, p_in: float = 0.01, rho_inf: float = 0.01,
+                       u_inf: float = 2.5, mach_inf: float = 2.0, gamma: float = GA) -> GelParams:
+    """Synthetic single-fluid heliosphere: supersonic wind from the inner sphere r = 1
+    (ro = rho_in/r^2, p = p_in/r^2 at r, |u| = u_in radial) against a uniform inflow
+    along -x of density rho_inf, speed u_inf and Mach number mach_inf."""
+    p = reference_params()
+    p.gamma = gamma
+    p.cfl = cfl
+    p.flux = flux
+    p.r_skip = 0.0
+    p.in_rho_div = 1.0 / rho_in
+    p.in_p_a = 1.0
+    p.in_p_den = rho_in / p_in
+    p.in_u = u_in
+    p.in_q = 1.0
+    p_inf = rho_inf * u_inf * u_inf / (gamma * mach_inf * mach_inf)
+    p.inf_rho, p.inf_p, p.inf_u, p.inf_v, p.inf_q = rho_inf, p_inf, -u_inf, 0.0, 0.0
+    return p
+
+
+def pi_() -> float:
+    return math.pi

@@ -1,0 +1,182 @@
+/*
+ * gelios.h - C-ABI of the MI355X Godunov flux-and-update sweep.
+ *
+ * The reference (SergeyMSU/Gelios_2D) has no plugin or FFI layer: its sweep is the
+ * C++ member Setka::Go_5_komponent__MK2 (Setka.h:186, Setka.cpp:10121-10482), which
+ * loops over cells with OpenMP and calls Gran/Cell helpers and the face flux
+ * Setka::HLLC_2d_Korolkov_b_s (Setka.h:187-189, Setka.cpp:10484-10700).  The entry
+ * points below are what a host that keeps the reference's Setka/Cell/Gran objects
+ * calls instead of that OpenMP loop.  Each declaration names the reference code it
+ * replaces.
+ *
+ * Conventions
+ *   - Every call returns an int status, GEL_OK (0) on success; nothing calls exit().
+ *     gel_strerror() turns a status into text; gel_last_error() gives the detail of
+ *     the last failure on the calling thread.
+ *   - A context owns its device buffers; the caller owns every host array passed in.
+ *   - One context is driven by one host thread at a time.
+ *   - Results are deterministic: fixed summation order, no floating-point atomics
+ *     (the global time step is reduced with an integer min on the bit pattern of a
+ *     positive double).
+ *
+ * Data layout (host side and device side alike)
+ *   - Grid: logically structured polar grid of ni (along a ray, index i) by nj
+ *     (across rays, index j) quadrilateral cells.  Node (i,j), i in [0,ni],
+ *     j in [0,nj], lives at index i*(nj+1)+j.  x is the symmetry-axis coordinate,
+ *     y >= 0 the distance from the axis (the reference's Point::x, Point::y).
+ *     Cell (i,j) has contour (i,j),(i+1,j),(i+1,j+1),(i,j+1), counter-clockwise.
+ *     Faces with j = 0 and j = nj lie on the axis (Gran_type Axis), the faces at
+ *     i = 0 are the inner sphere (Inner_sphere), the faces at i = ni are Input or
+ *     Extern as given by outer_type.
+ *   - State: primitive variables, structure of arrays, variable-major:
+ *     prim[v*ncells + i*nj + j].  Gas (5 variables): ro, p, u, v, Q in the order of
+ *     the reference struct Parametr (Cell.h:19-24).  MHD (9 variables): ro, p, u, v,
+ *     w, bx, by, bz, Q (w and bz are the azimuthal velocity and field).
+ */
+#ifndef GELIOS_H
+#define GELIOS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum gel_status {
+    GEL_OK = 0,
+    GEL_E_ARG = 1,          /* invalid argument (null pointer, bad size, bad enum) */
+    GEL_E_HIP = 2,          /* a HIP runtime call failed */
+    GEL_E_NOMEM = 3,        /* device allocation failed */
+    GEL_E_UNSUPPORTED = 4,  /* combination of options not built */
+    GEL_E_STATE = 5,        /* call out of order (e.g. stepping before upload) */
+    GEL_E_NUMERIC = 6       /* non-positive reconstructed density/pressure where the
+                               reference would exit(-1) (Gran.cpp:444-455) */
+};
+
+/* Face boundary types: same enumerators, same order as Gran_type (Gran.h:7-16). */
+enum gel_face_type {
+    GEL_F_USUALY = 0, GEL_F_INNER_SPHERE = 1, GEL_F_EXTERN = 2,
+    GEL_F_INPUT = 3, GEL_F_AXIS = 4, GEL_F_UPPER_WALL = 5
+};
+
+/* Cell types: same enumerators, same order as Cell_type (Cell.h:8-16).  C_1 cells
+ * get the radially rescaled reconstruction (Setka.cpp:10219-10249), C_centr cells
+ * are frozen (Setka.cpp:10461-10468). */
+enum gel_cell_type {
+    GEL_C_1 = 0, GEL_C_2 = 1, GEL_C_3 = 2, GEL_C_4 = 3, GEL_C_5 = 4,
+    GEL_C_CENTR = 5, GEL_C_NO = 6
+};
+
+/* Face flux: the `metod` argument of HLLC_2d_Korolkov_b_s (0 = HLL, 1 = HLLC). */
+enum gel_flux { GEL_FLUX_HLL = 0, GEL_FLUX_HLLC = 1 };
+
+enum gel_physics { GEL_PHYS_GAS = 0 };
+
+enum gel_source_mode {
+    GEL_SRC_NONE = 0,
+    /* Charge exchange with 4 frozen hydrogen populations, Cell::Get_Sourse_MK1
+       (Cell.cpp:498-665); moments n,u,v,T per population and k_u,k_v,k_T per cell. */
+    GEL_SRC_MK1 = 1
+};
+
+/* Number of per-cell source inputs for GEL_SRC_MK1, variable-major like the state:
+ * H_n[4], H_u[4], H_v[4], H_T[4], k_u, k_v, k_T. */
+#define GEL_MK1_NFIELDS 19
+
+typedef struct gel_params {
+    double gamma;       /* adiabatic index (reference macro ga/ggg = 5/3, Help.h:39-40) */
+    double cfl;         /* Courant number (reference macro kurant = 0.1, Help.h:57) */
+    double dt0;         /* time step of the first step (T0 = 1e-8, Setka.cpp:10128-10129) */
+    double r_skip;      /* cells whose centre radius is below it are skipped (R11_,
+                           Setka.cpp:10169-10172); 0 disables */
+    int flux;           /* enum gel_flux */
+    int physics;        /* enum gel_physics */
+    int source_mode;    /* enum gel_source_mode */
+    int reserved0;
+    /* Inner_sphere ghost state at face centre distance d (Gran.cpp:234-265):
+       ro = 1/(in_rho_div*d*d), p = ro*in_p_a*in_p_a/in_p_den,
+       (u,v) = in_u*(x,y)/d, Q = ro*in_q. */
+    double in_rho_div, in_p_a, in_p_den, in_u, in_q;
+    /* Input ghost state (Gran.cpp:276-282): ro, p, u, v, Q. */
+    double inf_rho, inf_p, inf_u, inf_v, inf_q;
+    /* Extern: outflow copy whose normal... u >= 0 is replaced by this (Gran.cpp:212-220). */
+    double extern_u;
+    /* Charge-exchange constants (Help.h:60-64, Cell.cpp:620-659):
+       coefficient n_p/Kn, and sigma(U) = (sig_a - sig_b*ln U)^2. */
+    double src_coef, sig_a, sig_b;
+} gel_params;
+
+/* Fill *p with the reference's own constants (Help.h, Gran.cpp) and HLL flux. */
+void gel_params_reference(gel_params* p);
+
+typedef struct gel_grid {
+    int ni, nj;                 /* cells along and across the rays */
+    const double* x;            /* (ni+1)*(nj+1) node axis coordinates */
+    const double* y;            /* (ni+1)*(nj+1) node distances from the axis (>= 0) */
+    const int8_t* cell_type;    /* ni*nj enum gel_cell_type, or NULL for all C_2 */
+    const int8_t* outer_type;   /* nj face types of the i = ni boundary, or NULL for
+                                   Input where the face centre has x > 0, else Extern */
+} gel_grid;
+
+typedef struct gel_stats {
+    double dt_last;         /* time step used by the last step taken */
+    double dt_next;         /* time step the next step will use (global CFL minimum) */
+    long long steps;        /* steps taken by this context */
+    long long n_rho_clamp;  /* cumulative density clamps (Setka.cpp:10432-10436) */
+    long long n_p_clamp;    /* cumulative pressure clamps (Setka.cpp:10450-10453) */
+    long long n_numeric;    /* faces where the reference would have exited */
+} gel_stats;
+
+typedef struct gel_ctx gel_ctx;
+
+/* Create a context for one grid: uploads node coordinates and the grid tables.
+ * Replaces the set-up the reference does before its sweep (Setka::TVD_prepare,
+ * Setka.cpp:1294-1492, builds the face stencils that are implicit here). */
+int gel_ctx_create(gel_ctx** out, const gel_grid* grid, const gel_params* prm);
+void gel_ctx_destroy(gel_ctx* ctx);
+
+/* Copy the primitive state (variable-major SoA) to the device; resets the time step
+ * to prm->dt0 and the step counter.  Replaces Cell::par[now1] as the sweep input. */
+int gel_upload_state(gel_ctx* ctx, const double* prim);
+/* Copy the current primitive state back (Cell::par[now2] after the sweep). */
+int gel_download_state(gel_ctx* ctx, double* prim);
+
+/* Per-cell source inputs, GEL_MK1_NFIELDS variable-major arrays of ncells doubles
+ * (Parametr::H_n/H_u/H_v/H_T/k_u/k_v/k_T, Cell.h:55-66). */
+int gel_set_sources(gel_ctx* ctx, const double* fields);
+
+/* Advance nsteps steps: replaces the body of the step loop of
+ * Setka::Go_5_komponent__MK2 (Setka.cpp:10131-10481) on a static grid.
+ * Steps run back to back on the context's stream without host round trips; the
+ * lagged CFL step (Setka.cpp:10388-10393) stays on the device.  out may be NULL. */
+int gel_step(gel_ctx* ctx, int nsteps, gel_stats* out);
+
+/* Block until queued work is done and fill *out (may be NULL). */
+int gel_sync(gel_ctx* ctx, gel_stats* out);
+
+/* Average device duration in milliseconds of the sweep kernel over the launches
+ * timed since the last call (HIP events on the context's stream); resets the
+ * accumulator.  *n_launches gets the number of launches averaged. */
+int gel_kernel_timing(gel_ctx* ctx, double* avg_ms, long long* n_launches);
+/* Turn per-launch event timing on or off (off by default). */
+int gel_enable_timing(gel_ctx* ctx, int on);
+
+/* Batched device evaluation of the face flux, for unit tests against fixtures:
+ * same arguments and outputs as Setka::HLLC_2d_Korolkov_b_s (Setka.h:187-189),
+ * one face per index.  Host arrays in and out; L and R are [n][5] as
+ * (ro, Q, p, u, v); out P is [n][4], PQ, Vl, Vc, Vp, dt are [n]. */
+int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double* W,
+                          const double* n1, const double* n2, const double* rad,
+                          double cfl, double gamma, int metod, int nul_potok,
+                          double* P, double* PQ, double* Vl, double* Vc, double* Vp,
+                          double* dt);
+
+const char* gel_strerror(int status);
+const char* gel_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GELIOS_H */

@@ -1,0 +1,529 @@
+/*
+ * gelios_oracle.c - CPU restatement of the reference sweep.  TEST INFRASTRUCTURE:
+ * loaded only by tests, the smoke check and the CPU-baseline leg of the benchmark,
+ * never by the product path.
+ *
+ * Restates Setka::Go_5_komponent__MK2 (Setka.cpp:10121-10482), the face flux
+ * Setka::HLLC_2d_Korolkov_b_s (Setka.cpp:10484-10700), the boundary states
+ * Gran::Get_par (Gran.cpp:205-289), the reconstructions Gran::Get_par_TVD
+ * (Gran.cpp:292-559) and Gran::Get_par_TVD_radial (Gran.cpp:562-675), the cell and
+ * face geometry (Cell.cpp:66-275, Gran.cpp:159-198), the limiter helpers
+ * (Help.cpp:3-120) and the charge-exchange source Cell::Get_Sourse_MK1
+ * (Cell.cpp:498-665).  Expressions keep the reference's operation order.
+ *
+ * Not restated here (no call on the polar grids this build runs): the fitted-surface
+ * branch of the sweep (exact Godunov solver on faces whose nodes are P_Contact /
+ * P_Inner_shock / P_Outer_shock, Setka.cpp:10287-10370) and the radial-chain
+ * re-interpolation of the MC moments in Get_Sourse_MK1 (Cell.cpp:528-589), which on a
+ * static grid returns the cell's own moments.
+ */
+#include "gelios_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define PI_ 3.14159265358979323846   /* Help.h:50 */
+
+/* std::min / std::max semantics (return the first argument on ties). */
+static inline double mn(double a, double b) { return (b < a) ? b : a; }
+static inline double mx(double a, double b) { return (a < b) ? b : a; }
+static inline double sq(double a) { return a * a; }
+
+/* ------------------------------------------------------------------ helpers --- */
+
+static double sgn(double x)   /* Help.cpp:80-94 */
+{
+    if (x > 0) return 1.0;
+    if (x < 0) return -1.0;
+    return 0.0;
+}
+
+double ora_minmod(double x, double y)   /* Help.cpp:67-78 */
+{
+    const double s = sgn(x) + sgn(y);
+    if (s == 0) return 0.0;
+    return (s / 2.0) * mn(fabs(x), fabs(y));
+}
+
+double ora_linear3(double x1, double t1, double x2, double t2, double x3, double t3, double y)
+{   /* Help.cpp:96-113, the minmod branch */
+    const double d = ora_minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+    return d * (y - x2) + t2;
+}
+
+double ora_linear2(double x1, double t1, double x2, double t2, double y)  /* Help.cpp:115-120 */
+{
+    const double d = (t1 - t2) / (x1 - x2);
+    return d * (y - x2) + t2;
+}
+
+double ora_polar_angle(double x, double y)   /* Help.cpp:37-65 */
+{
+    if (fabs(x) + fabs(y) < 0.000001 / 390.222) return 0.0;   /* RR_, Help.h:79 */
+    if (x < 0) return atan(y / x) + 1.0 * PI_;
+    if (x > 0 && y >= 0) return atan(y / x);
+    if (x > 0 && y < 0) return atan(y / x) + 2.0 * PI_;
+    if (y > 0) return PI_ / 2.0;
+    if (y < 0) return 3.0 * PI_ / 2.0;
+    return 0.0;
+}
+
+void ora_polar_perenos(double x1, double y1, double x2, double y2, double* u, double* v)
+{   /* Help.cpp:3-14: rotate (u,v) from the polar frame at (x1,y1) to the one at (x2,y2) */
+    const double a1 = ora_polar_angle(x1, y1);
+    const double a2 = ora_polar_angle(x2, y2);
+    const double fr = *u * cos(a1) + *v * sin(a1);
+    const double ff = -*u * sin(a1) + *v * cos(a1);
+    *u = fr * cos(a2) - ff * sin(a2);
+    *v = fr * sin(a2) + ff * cos(a2);
+}
+
+/* ----------------------------------------------------------------- geometry --- */
+
+void ora_cell_center(const double* px, const double* py, const int* q, double* cx, double* cy)
+{   /* Cell::Get_Center, polygon barycentre (Cell.cpp:66-88) */
+    double A = 0.0, Gx = 0.0, Gy = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const int a = q[k], b = q[(k + 1) & 3];
+        const double cr = px[a] * py[b] - py[a] * px[b];
+        A = A + cr;
+        Gx = Gx + (px[a] + px[b]) * cr;
+        Gy = Gy + (py[a] + py[b]) * cr;
+    }
+    A = A * 0.5;
+    *cx = Gx / (6.0 * A);
+    *cy = Gy / (6.0 * A);
+}
+
+double ora_cell_area(const double* px, const double* py, const int* q)
+{   /* Cell::Get_Volume for quads: half product of the diagonals times sin (Cell.cpp:146-166) */
+    double a1 = px[q[0]] - px[q[2]], a2 = py[q[0]] - py[q[2]];
+    const double a = sqrt(sq(a1) + sq(a2));
+    double b1 = px[q[1]] - px[q[3]], b2 = py[q[1]] - py[q[3]];
+    const double b = sqrt(sq(b1) + sq(b2));
+    a1 = a1 / a; a2 = a2 / a;
+    b1 = b1 / b; b2 = b2 / b;
+    const double c = a1 * b1 + a2 * b2;
+    const double s = sqrt(1 - sq(c));
+    return 0.5 * a * b * s;
+}
+
+double ora_cell_volume_rotate(const double* px, const double* py, const int* q, double angle)
+{   /* Cell::Get_Volume_rotate (Cell.cpp:188-206): Pappus volume of the swept polygon */
+    double A = 0.0, G = 0.0;
+    for (int k = 0; k < 4; k++) {
+        const int a = q[k], b = q[(k + 1) & 3];
+        const double cr = px[a] * py[b] - py[a] * px[b];
+        A = A + cr;
+        G = G + (py[a] + py[b]) * cr;
+    }
+    A = A * 0.5;
+    return fabs(A) * (1.0 / (6.0 * A)) * G * PI_ * angle / 180.0;
+}
+
+/* ---------------------------------------------------------------- face flux --- */
+
+double ora_hllc2d(const ora_gas* L, const ora_gas* R, double W, double n1, double n2,
+                  double rad, double cfl, double gam, int metod, int nul_potok,
+                  double P[4], double* PQ, double* Vl, double* Vc, double* Vp)
+{
+    /* Setka.cpp:10496-10503: normal/tangential components, tangent t = (n2, -n1) */
+    const double t1 = n2, t2 = -n1;
+    const double u1 = L->u * n1 + L->v * n2, v1 = L->u * t1 + L->v * t2;
+    const double u2 = R->u * n1 + R->v * n2, v2 = R->u * t1 + R->v * t2;
+    const double g1 = gam - 1.0;
+
+    const double cL = sqrt(gam * L->p / L->ro);   /* :10507-10508 */
+    const double cR = sqrt(gam * R->p / R->ro);
+    const double kL = (sq(L->u) + sq(L->v)) / 2.0;
+    const double kR = (sq(R->u) + sq(R->v)) / 2.0;
+
+    const double SL = mn(u1, u2) - mx(cL, cR);    /* Davis speeds, :10514-10515 */
+    const double SR = mx(u1, u2) + mx(cL, cR);
+    *Vl = SL;
+    *Vp = SR;
+    const double suR = SR - u2, suL = SL - u1;
+    const double SM = (suR * R->ro * u2 - suL * L->ro * u1 - R->p + L->p)
+                      / (suR * R->ro - suL * L->ro);          /* :10526-10527 */
+    *Vc = SM;
+    const double time = cfl * rad / mx(fabs(SL), fabs(SR)); /* :10532-10536 */
+
+    const double eL = L->p / g1 + L->ro * kL;
+    const double eR = R->p / g1 + R->ro * kR;
+    const double FL[5] = { L->ro * u1, L->ro * u1 * u1 + L->p, L->ro * u1 * v1, (eL + L->p) * u1, L->Q * u1 };
+    const double FR[5] = { R->ro * u2, R->ro * u2 * u2 + R->p, R->ro * u2 * v2, (eR + R->p) * u2, R->Q * u2 };
+    const double UL[5] = { L->ro, L->ro * u1, L->ro * v1, eL, L->Q };
+    const double UR[5] = { R->ro, R->ro * u2, R->ro * v2, eR, R->Q };
+
+    /* supersonic sides, flux minus W*U (ALE), :10569-10589 */
+    if (SL >= W || SR <= W) {
+        const double* F = (SL >= W) ? FL : FR;
+        const double* U = (SL >= W) ? UL : UR;
+        P[1] = n1 * (F[1] - W * U[1]) + t1 * (F[2] - W * U[2]);
+        P[2] = n2 * (F[1] - W * U[1]) + t2 * (F[2] - W * U[2]);
+        P[0] = F[0] - W * U[0];
+        P[3] = F[3] - W * U[3];
+        *PQ = F[4] - W * U[4];
+        return time;
+    }
+
+    const double roLL = L->ro * (SL - u1) / (SL - SM);
+    const double roRR = R->ro * (SR - u2) / (SR - SM);
+    const double QLL = L->Q * (SL - u1) / (SL - SM);
+    const double QRR = R->Q * (SR - u2) / (SR - SM);
+    double UZ[5];                                    /* HLL average state, :10601-10605 */
+    for (int k = 0; k < 5; k++) UZ[k] = (SR * UR[k] - SL * UL[k] + FL[k] - FR[k]) / (SR - SL);
+
+    double PO[5];
+    if (metod == 0) {                                 /* HLL, :10606-10618 */
+        for (int k = 0; k < 5; k++)
+            PO[k] = (SR * FL[k] - SL * FR[k] + SR * SL * (UR[k] - UL[k])) / (SR - SL);
+        P[1] = n1 * (PO[1] - W * UZ[1]) + t1 * (PO[2] - W * UZ[2]);
+        P[2] = n2 * (PO[1] - W * UZ[1]) + t2 * (PO[2] - W * UZ[2]);
+        P[0] = PO[0] - W * UZ[0];
+        P[3] = PO[3] - W * UZ[3];
+        *PQ = PO[4] - W * UZ[4];
+        return time;
+    }
+
+    /* HLLC star states, :10621-10672 */
+    const double suRm = suR / (SR - SM), suLm = suL / (SL - SM);
+    const double ptzR = R->p + R->ro * suR * (SM - u2);
+    const double ptzL = L->p + L->ro * suL * (SM - u1);
+    const double ptz = (ptzR + ptzL) / 2.0;
+    double vLL, vRR;
+    if (nul_potok) { vRR = UZ[2] / UZ[0]; vLL = vRR; }   /* :10644-10653 */
+    else { vLL = v1; vRR = v2; }
+    const double ee2 = eR * suRm + (ptz * SM - R->p * u2) / (SR - SM);
+    const double ee1 = eL * suLm + (ptz * SM - L->p * u1) / (SL - SM);
+    const double ULL[5] = { roLL, roLL * SM, roLL * vLL, ee1, QLL };
+    const double URR[5] = { roRR, roRR * SM, roRR * vRR, ee2, QRR };
+    if (SL < W && SM >= W) {
+        for (int k = 0; k < 5; k++) PO[k] = FL[k] + SL * ULL[k] - SL * UL[k] - W * ULL[k];
+    } else if (SR > W && SM < W) {
+        for (int k = 0; k < 5; k++) PO[k] = FR[k] + SR * URR[k] - SR * UR[k] - W * URR[k];
+    } else {
+        for (int k = 0; k < 5; k++) PO[k] = NAN;    /* unreachable unless SM is NaN */
+    }
+    P[1] = n1 * PO[1] + t1 * PO[2];                  /* back rotation, :10693-10697 */
+    P[2] = n2 * PO[1] + t2 * PO[2];
+    P[0] = PO[0];
+    P[3] = PO[3];
+    *PQ = PO[4];
+    return time;
+}
+
+/* ------------------------------------------------------------------ sources --- */
+
+void ora_source_mk1(const gel_params* prm, const double* f, double u, double v,
+                    double ro, double p, double* q1, double* q2, double* q3)
+{   /* Cell::Get_Sourse_MK1, own-cell branch (Cell.cpp:590-595, :598-664);
+       f is strided by the caller: f[k] is field k of this cell. */
+    double Mu = 0.0, Mv = 0.0, MT = 0.0;
+    double nu[4], uH[4], vH[4], roH[4], pH[4], UM[4], UU[4];
+    for (int k = 0; k < 4; k++) {
+        uH[k] = f[4 + k];
+        vH[k] = f[8 + k];
+        roH[k] = f[k];
+        pH[k] = 0.5 * f[12 + k] * f[k];
+        if (roH[k] <= 0.0) { roH[k] = 0.0000001; pH[k] = 0.0; }
+    }
+    for (int k = 0; k < 4; k++) {
+        UM[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (64.0 / (9.0 * PI_)) * (p / ro + 2.0 * pH[k] / roH[k]));
+        UU[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (4.0 / PI_) * (p / ro + 2.0 * pH[k] / roH[k]));
+        const double sig = sq(prm->sig_a - prm->sig_b * log(UM[k]));
+        nu[k] = ro * roH[k] * UM[k] * sig;
+    }
+    /* sums in the reference's left-to-right order, :645-659 */
+    Mu = nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u);
+    Mv = nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v);
+    {
+        double t[4];
+        for (int k = 0; k < 4; k++)
+            t[k] = nu[k] * ((sq(uH[k]) + sq(vH[k]) - sq(u) - sq(v)) / 2.0 +
+                            (UU[k] / UM[k]) * (2.0 * pH[k] / roH[k] - p / ro));
+        MT = t[0] + t[1] + t[2] + t[3];
+    }
+    Mu = prm->src_coef * Mu;
+    Mv = prm->src_coef * Mv;
+    MT = prm->src_coef * MT;
+    *q1 = Mu * f[16];
+    *q2 = Mv * f[17];
+    *q3 = MT * f[18];
+}
+
+/* ------------------------------------------------------------------- sweep --- */
+
+typedef struct sweep_ctx {
+    const ora_mesh* m;
+    const gel_params* prm;
+    const double* now;
+    int nc;
+    const double* cx;   /* cell centres at the old node positions */
+    const double* cy;
+} sweep_ctx;
+
+static inline ora_gas load(const sweep_ctx* S, int c)
+{
+    const int nc = S->nc;
+    ora_gas g = { S->now[c], S->now[nc + c], S->now[2 * nc + c], S->now[3 * nc + c], S->now[4 * nc + c] };
+    return g;
+}
+
+static int cell_has_axis(const ora_mesh* m, int c)
+{
+    for (int k = 0; k < m->nface[c]; k++)
+        if (m->ftype[c * ORA_MAXF + k] == GEL_F_AXIS) return 1;
+    return 0;
+}
+
+static inline double dist2d(double ax, double ay, double bx, double by)
+{
+    return sqrt(sq(ax - bx) + sq(ay - by));
+}
+
+/* Gran::Get_par (Gran.cpp:205-289): state on the far side of the face. */
+static ora_gas get_par(const sweep_ctx* S, int M, int slot, double fx, double fy)
+{
+    const gel_params* p = S->prm;
+    const int t = S->m->ftype[slot];
+    ora_gas g = load(S, M);
+    switch (t) {
+    case GEL_F_USUALY:
+        return load(S, S->m->fnbr[slot]);
+    case GEL_F_EXTERN:
+        if (g.u >= 0.0) g.u = p->extern_u;
+        return g;
+    case GEL_F_AXIS:
+        g.v = -g.v;
+        return g;
+    case GEL_F_INNER_SPHERE: {
+        const double d = sqrt(fx * fx + fy * fy);
+        const double ro = 1.0 / (p->in_rho_div * d * d);
+        ora_gas w = { ro, ro * p->in_p_a * p->in_p_a / p->in_p_den, p->in_u * fx / d,
+                      p->in_u * fy / d, ro * p->in_q };
+        return w;
+    }
+    case GEL_F_INPUT: {
+        ora_gas w = { p->inf_rho, p->inf_p, p->inf_u, p->inf_v, p->inf_q };
+        return w;
+    }
+    default: /* Upper_wall */
+        return g;
+    }
+}
+
+/* Gran::Get_par_TVD for an Usualy face (Gran.cpp:294-457): value of master M at the
+ * face centre (fx,fy) from the line (down D, M, sosed N).  *bad set where the
+ * reference exits. */
+static ora_gas tvd_usualy(const sweep_ctx* S, int M, int N, int D, double fx, double fy, int* bad)
+{
+    ora_gas pm = load(S, M);
+    if (D < 0) {
+        if (cell_has_axis(S->m, M)) {          /* mirrored ghost across the axis, :297-339 */
+            ora_gas pn = load(S, N);
+            const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+            const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+            const double d1 = dist2d(fx, fy, S->cx[M], -S->cy[M]);
+            ora_gas r = pm;
+            r.v = ora_linear3(-d1, -pm.v, -d2, pm.v, d3, pn.v, 0.0);
+            return r;
+        }
+        return pm;                             /* :340-343 */
+    }
+    ora_gas pd = load(S, D), pn = load(S, N);
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+    const double d1 = dist2d(fx, fy, S->cx[D], S->cy[D]);
+    ora_gas r;                                  /* :427-456 */
+    r.ro = ora_linear3(-d1, pd.ro, -d2, pm.ro, d3, pn.ro, 0.0);
+    r.p = ora_linear3(-d1, pd.p, -d2, pm.p, d3, pn.p, 0.0);
+    r.u = ora_linear3(-d1, pd.u, -d2, pm.u, d3, pn.u, 0.0);
+    r.v = ora_linear3(-d1, pd.v, -d2, pm.v, d3, pn.v, 0.0);
+    r.Q = ora_linear3(-d1, pd.Q, -d2, pm.Q, d3, pn.Q, 0.0);
+    if (r.ro <= 0.0 || r.p <= 0.0) *bad = 1;
+    return r;
+}
+
+/* Gran::Get_par_TVD_radial, density and pressure only (Gran.cpp:562-675): the line
+ * fit is done on ro*r^2 and p*r^(2*gamma). */
+static void tvd_radial(const sweep_ctx* S, int M, int N, int D, double fx, double fy,
+                       double* ro, double* pr)
+{
+    const double g2 = 2.0 * S->prm->gamma;
+    ora_gas pm = load(S, M), pn = load(S, N);
+    double xd, yd;
+    ora_gas pd;
+    if (D < 0) {
+        if (!cell_has_axis(S->m, M)) { *ro = pm.ro; *pr = pm.p; return; }
+        xd = S->cx[M]; yd = -S->cy[M]; pd = pm;      /* mirrored master, :575-604 */
+    } else {
+        xd = S->cx[D]; yd = S->cy[D]; pd = load(S, D);
+    }
+    const double r = sqrt(sq(fx) + sq(fy));
+    const double r2 = sqrt(sq(S->cx[M]) + sq(S->cy[M]));
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double r3 = sqrt(sq(S->cx[N]) + sq(S->cy[N]));
+    const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+    const double r1 = sqrt(sq(xd) + sq(yd));
+    const double d1 = dist2d(fx, fy, xd, yd);
+    *ro = ora_linear3(-d1, pd.ro * sq(r1), -d2, pm.ro * sq(r2), d3, pn.ro * sq(r3), 0.0) / sq(r);
+    *pr = ora_linear3(-d1, pd.p * pow(r1, g2), -d2, pm.p * pow(r2, g2), d3, pn.p * pow(r3, g2), 0.0) / pow(r, g2);
+}
+
+/* Gran::Get_par_TVD for an Axis face (Gran.cpp:486-519). */
+static ora_gas tvd_axis(const sweep_ctx* S, int M, int D, double fx, double fy)
+{
+    ora_gas pm = load(S, M), pd = load(S, D);
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double d1 = dist2d(fx, fy, S->cx[D], S->cy[D]);
+    ora_gas r = pm;
+    r.v = ora_linear3(-d1, pd.v, -d2, pm.v, d2, -pm.v, 0.0);
+    return r;
+}
+
+/* radial rescaling of a C_1 cell value from its centre radius rc to the face
+ * radius dis, and rotation of the velocity to the face's polar frame
+ * (Setka.cpp:10221-10226). */
+static void rescale_c1(ora_gas* g, double rc, double dis, double gam, double cx, double cy,
+                       double fx, double fy, int rotate)
+{
+    g->ro = g->ro * sq(rc) / sq(dis);
+    g->Q = g->Q * sq(rc) / sq(dis);
+    g->p = g->p * pow(rc / dis, 2 * gam);
+    if (rotate) ora_polar_perenos(cx, cy, fx, fy, &g->u, &g->v);
+}
+
+int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
+                 double* next, const double* src, double dt_prev, double* dt_new,
+                 long long* n_rho_clamp, long long* n_p_clamp, int nthreads)
+{
+    const int nc = m->n_cells;
+    const double gam = prm->gamma;
+    double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
+    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    sweep_ctx S = { m, prm, now, nc, cx, cy };
+
+    double Tmin = 100000000;          /* T[now2], Setka.cpp:10142 */
+    long long nrc = 0, npc = 0;
+    int bad_any = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
+#endif
+    for (int c = 0; c < nc; c++) {
+        const int* q = m->contour + 4 * c;
+        const double x = cx[c], y = cy[c];
+        const double vs1 = ora_cell_area(m->x, m->y, q);
+        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double radius = sqrt(sq(x) + sq(y));
+        if (radius < prm->r_skip) continue;
+        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
+        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        const ora_gas par1 = load(&S, c);
+        const int K1 = (m->cell_type[c] == GEL_C_1);
+        double Pot[5] = { 0, 0, 0, 0, 0 };
+        double time = 10000000;
+        int bad = 0;
+
+        for (int k = 0; k < m->nface[c]; k++) {
+            const int s = c * ORA_MAXF + k;
+            const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
+            /* Gran geometry at the old and new node positions (Gran.cpp:159-198) */
+            const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+            const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
+            const double fx2 = (m->x2[A] + m->x2[B]) / 2.0, fy2 = (m->y2[A] + m->y2[B]) / 2.0;
+            const double dist = sqrt(sq(x - fx) + sq(y - fy));
+            double n1, n2;
+            {
+                double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
+                const double nn = sqrt(sq(t1) + sq(t2));
+                t1 = t1 / nn; t2 = t2 / nn;
+                n1 = t2; n2 = -t1;
+            }
+            const double W = ((fx2 - fx) * n1 + (fy2 - fy) * n2) / dt_prev;   /* :10209 */
+            const double dis = sqrt(sq(fx) + sq(fy));
+            ora_gas p11 = par1;
+            ora_gas p2 = get_par(&S, c, s, fx, fy);
+
+            if (t == GEL_F_USUALY) {                    /* :10213-10251 */
+                const int N = m->fnbr[s];
+                const int N1 = (m->cell_type[N] == GEL_C_1);
+                if (K1) {
+                    rescale_c1(&p11, radius, dis, gam, x, y, fx, fy, 1);
+                    double ro, pr;
+                    tvd_radial(&S, c, N, m->fdown[s], fx, fy, &ro, &pr);
+                    p11.ro = ro; p11.p = pr;
+                } else {
+                    p11 = tvd_usualy(&S, c, N, m->fdown[s], fx, fy, &bad);
+                }
+                if (N1) {
+                    const double dd = sqrt(sq(cx[N]) + sq(cy[N]));
+                    rescale_c1(&p2, dd, dis, gam, cx[N], cy[N], fx, fy, 1);
+                    double ro, pr;
+                    tvd_radial(&S, N, c, m->fup[s], fx, fy, &ro, &pr);
+                    p2.ro = ro; p2.p = pr;
+                } else {
+                    p2 = tvd_usualy(&S, N, c, m->fup[s], fx, fy, &bad);
+                }
+            } else if (t == GEL_F_INNER_SPHERE) {      /* :10252-10257 */
+                rescale_c1(&p11, radius, dis, gam, x, y, fx, fy, 0);
+            } else if (K1 && t == GEL_F_AXIS) {        /* :10258-10268 */
+                rescale_c1(&p11, radius, dis, gam, x, y, fx, fy, 1);
+                p2 = p11;
+                p2.v = -p2.v;
+            } else if (t == GEL_F_AXIS) {              /* :10269-10274 */
+                p11 = tvd_axis(&S, c, m->fdown[s], fx, fy);
+                p2 = p11;
+                p2.v = -p2.v;
+            }
+
+            double P[4], PQ, Vl, Vc, Vp;
+            const double tf = ora_hllc2d(&p11, &p2, W, n1, n2, dist, prm->cfl, gam, prm->flux, 0,
+                                         P, &PQ, &Vl, &Vc, &Vp);
+            time = mn(time, tf);
+            for (int v = 0; v < 4; v++) Pot[v] = Pot[v] + P[v] * S_;   /* :10372-10376 */
+            Pot[4] = Pot[4] + PQ * S_;
+        }
+        if (bad) bad_any = 1;
+        if (time < Tmin) Tmin = time;
+
+        const double u = par1.u, v = par1.v, ro = par1.ro, p = par1.p, Q = par1.Q;
+        double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        if (prm->source_mode == GEL_SRC_MK1 && src) {
+            double f[GEL_MK1_NFIELDS];
+            for (int k = 0; k < GEL_MK1_NFIELDS; k++) f[k] = src[(size_t)k * nc + c];
+            ora_source_mk1(prm, f, u, v, ro, p, &q1, &q2, &q3);
+        }
+        const double T1 = dt_prev;
+        if (m->cell_type[c] != GEL_C_CENTR) {           /* :10423-10460 */
+            double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+            const double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+            if (ro3 <= 0.0) { ro3 = 0.00001; nrc++; }
+            const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+            const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2)) / ro3;
+            double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3)) -
+                         0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+            if (p3 <= 0) { p3 = 0.000001; npc++; }
+            next[c] = ro3;
+            next[nc + c] = p3;
+            next[2 * nc + c] = u3;
+            next[3 * nc + c] = v3;
+            next[4 * nc + c] = Q33;
+        } else {
+            for (int k = 0; k < 5; k++) next[(size_t)k * nc + c] = now[(size_t)k * nc + c];
+        }
+    }
+    free(cx);
+    free(cy);
+    *dt_new = Tmin;
+    if (n_rho_clamp) *n_rho_clamp += nrc;
+    if (n_p_clamp) *n_p_clamp += npc;
+    return bad_any ? GEL_E_NUMERIC : GEL_OK;
+}

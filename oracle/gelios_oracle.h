@@ -1,0 +1,84 @@
+/*
+ * gelios_oracle.h - CPU restatement of the reference sweep.  TEST INFRASTRUCTURE.
+ *
+ * This library is the checker for the HIP path: only tests, the smoke check and the
+ * CPU-baseline leg of the benchmark load it.  The product path (gelios_2d_amd) never
+ * links or calls it.
+ *
+ * It restates, in plain C and FP64, the cell loop of Setka::Go_5_komponent__MK2
+ * (Setka.cpp:10121-10482) on a general mesh given as per-cell face slots, the way the
+ * reference walks Cell::Grans.  Each function cites the reference lines it follows.
+ * Floating-point expressions keep the reference's operation order so that the
+ * restatement agrees with the compiled reference to the last bits where the
+ * compilers agree.
+ *
+ * Pinning (see tests/test_oracle_*.py): the face flux, geometry, limiter and the
+ * whole sweep are compared with vectors written by the compiled reference
+ * (oracle/_ref, tests/golden/); the flux is also checked against closed forms
+ * (consistency F(U,U), mirror antisymmetry, supersonic upwinding).
+ */
+#ifndef GELIOS_ORACLE_H
+#define GELIOS_ORACLE_H
+
+#include "../include/gelios.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORA_MAXF 8   /* face slots per cell */
+
+/* General mesh: per-cell face slots in Cell::Grans order.  Slot s = c*ORA_MAXF + k.
+ * fa/fb: nodes A,B of the face as seen from the cell (normal to the right of A->B,
+ * pointing out of the cell, Gran.h:22).  fnbr: Gran::Sosed (-1 if none),
+ * fdown: Gran::Sosed_down, fup: Gran::Sosed_up (-1 if none), ftype: Gran_type. */
+typedef struct ora_mesh {
+    int n_nodes, n_cells;
+    const double *x, *y;      /* node positions now (Point::x, y) */
+    const double *x2, *y2;    /* node positions after the step (Point::x2, y2) */
+    const int *contour;       /* 4 node indices per cell, counter-clockwise */
+    const int *cell_type;     /* enum gel_cell_type */
+    const int *nface;         /* faces per cell */
+    const int *fa, *fb, *ftype, *fnbr, *fdown, *fup;
+} ora_mesh;
+
+typedef struct ora_gas { double ro, p, u, v, Q; } ora_gas;
+
+/* Setka::HLLC_2d_Korolkov_b_s (Setka.cpp:10484-10700). Returns cfl*rad/max|S|. */
+double ora_hllc2d(const ora_gas* L, const ora_gas* R, double W, double n1, double n2,
+                  double rad, double cfl, double gam, int metod, int nul_potok,
+                  double P[4], double* PQ, double* Vl, double* Vc, double* Vp);
+
+/* Help.cpp helpers. */
+double ora_minmod(double x, double y);                                  /* :67-78 */
+double ora_linear3(double x1, double t1, double x2, double t2,
+                   double x3, double t3, double y);                    /* :96-113 */
+double ora_linear2(double x1, double t1, double x2, double t2, double y); /* :115-120 */
+double ora_polar_angle(double x, double y);                             /* :37-65 */
+void ora_polar_perenos(double x1, double y1, double x2, double y2,
+                       double* u, double* v);                           /* :3-14 */
+
+/* Cell geometry on 4 nodes (Cell.cpp:66-88, :146-186, :188-206). */
+void ora_cell_center(const double* px, const double* py, const int* q, double* cx, double* cy);
+double ora_cell_area(const double* px, const double* py, const int* q);
+double ora_cell_volume_rotate(const double* px, const double* py, const int* q, double angle);
+
+/* Cell::Get_Sourse_MK1 (Cell.cpp:498-665) with the cell's own moments.
+ * f points at the 19 fields of one cell (H_n[4],H_u[4],H_v[4],H_T[4],k_u,k_v,k_T). */
+void ora_source_mk1(const gel_params* prm, const double* f, double u, double v,
+                    double ro, double p, double* q1, double* q2, double* q3);
+
+/* One step of the cell loop of Setka::Go_5_komponent__MK2 (Setka.cpp:10148-10470).
+ * now/next: 5 x n_cells primitive SoA (ro,p,u,v,Q).  src: 19 x n_cells or NULL.
+ * dt_prev is T[now1]; *dt_new receives T[now2].  Cells skipped by r_skip keep
+ * whatever next held (Setka.cpp:10169-10172).  Returns GEL_OK or GEL_E_NUMERIC
+ * where the reference would exit(-1) (Gran.cpp:444-455). */
+int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
+                 double* next, const double* src, double dt_prev, double* dt_new,
+                 long long* n_rho_clamp, long long* n_p_clamp, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

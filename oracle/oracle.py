@@ -1,0 +1,154 @@
+"""ctypes front end of the CPU oracle (oracle/libgelios_oracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, the smoke check and the CPU-baseline
+leg of bench.py.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+from gelios_2d_amd.grid import MAXF, GeneralMesh, PolarGrid, pack_general
+from gelios_2d_amd.params import GelParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgelios_oracle.so")
+REF_HARNESS = os.path.join(HERE, "_ref", "ref_harness")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class OraMesh(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int), ("n_cells", ctypes.c_int),
+                ("x", _dp), ("y", _dp), ("x2", _dp), ("y2", _dp),
+                ("contour", _ip), ("cell_type", _ip), ("nface", _ip),
+                ("fa", _ip), ("fb", _ip), ("ftype", _ip), ("fnbr", _ip), ("fdown", _ip), ("fup", _ip)]
+
+
+class OraGas(ctypes.Structure):
+    _fields_ = [("ro", ctypes.c_double), ("p", ctypes.c_double), ("u", ctypes.c_double),
+                ("v", ctypes.c_double), ("Q", ctypes.c_double)]
+
+
+def build(quiet: bool = True) -> None:
+    """Compile the restatement (and the reference harness where the reference
+    sources exist) with oracle/Makefile."""
+    subprocess.run(["make", "-C", HERE, "-j8"], check=True,
+                   stdout=subprocess.DEVNULL if quiet else None)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.ora_hllc2d.restype = ctypes.c_double
+        L.ora_hllc2d.argtypes = [ctypes.POINTER(OraGas), ctypes.POINTER(OraGas), ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        for name, n in (("ora_minmod", 2), ("ora_linear3", 7), ("ora_linear2", 5), ("ora_polar_angle", 2)):
+            f = getattr(L, name)
+            f.restype = ctypes.c_double
+            f.argtypes = [ctypes.c_double] * n
+        L.ora_polar_perenos.restype = None
+        L.ora_polar_perenos.argtypes = [ctypes.c_double] * 4 + [_dp, _dp]
+        L.ora_cell_center.argtypes = [_dp, _dp, _ip, _dp, _dp]
+        L.ora_cell_area.restype = ctypes.c_double
+        L.ora_cell_area.argtypes = [_dp, _dp, _ip]
+        L.ora_cell_volume_rotate.restype = ctypes.c_double
+        L.ora_cell_volume_rotate.argtypes = [_dp, _dp, _ip, ctypes.c_double]
+        L.ora_source_mk1.argtypes = [ctypes.POINTER(GelParams), _dp, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.ora_step_mk2.restype = ctypes.c_int
+        L.ora_step_mk2.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, _dp,
+                                   ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def hllc2d(L, R, W, n1, n2, rad, cfl, gamma, metod, nul_potok):
+    """Point-wise flux. L, R = (ro, Q, p, u, v) (argument order of
+    HLLC_2d_Korolkov_b_s). Returns (P[4], PQ, Vl, Vc, Vp, dt)."""
+    gl = OraGas(L[0], L[2], L[3], L[4], L[1])
+    gr = OraGas(R[0], R[2], R[3], R[4], R[1])
+    P = np.zeros(4)
+    o = [ctypes.c_double() for _ in range(4)]
+    t = lib().ora_hllc2d(ctypes.byref(gl), ctypes.byref(gr), W, n1, n2, rad, cfl, gamma, int(metod),
+                         int(nul_potok), _d(P), *[ctypes.byref(v) for v in o])
+    return P, o[0].value, o[1].value, o[2].value, o[3].value, t
+
+
+class OracleSweep:
+    """Holds a general mesh and steps the restated Go_5_komponent__MK2 cell loop."""
+
+    def __init__(self, mesh: GeneralMesh, x, y, prm: GelParams, x2=None, y2=None):
+        self.mesh = mesh
+        self.prm = prm.copy()
+        self.x = np.ascontiguousarray(x, np.float64)
+        self.y = np.ascontiguousarray(y, np.float64)
+        self.x2 = np.ascontiguousarray(self.x if x2 is None else x2, np.float64)
+        self.y2 = np.ascontiguousarray(self.y if y2 is None else y2, np.float64)
+        self.nc = len(mesh.cell_type)
+        self._keep = [np.ascontiguousarray(a, np.int32) for a in
+                      (mesh.contour.ravel(), mesh.cell_type, mesh.nface, mesh.fa, mesh.fb, mesh.ftype,
+                       mesh.fnbr, mesh.fdown, mesh.fup)]
+        self.n_rho_clamp = ctypes.c_longlong(0)
+        self.n_p_clamp = ctypes.c_longlong(0)
+        self.dt = prm.dt0
+        self.src = None
+
+    @classmethod
+    def from_polar(cls, g: PolarGrid, prm: GelParams):
+        return cls(pack_general(g), g.x, g.y, prm)
+
+    def _mesh(self):
+        k = self._keep
+        return OraMesh(len(self.x), self.nc, _d(self.x), _d(self.y), _d(self.x2), _d(self.y2),
+                       *[_i(a) for a in k])
+
+    def set_nodes(self, x, y, x2, y2):
+        self.x = np.ascontiguousarray(x, np.float64)
+        self.y = np.ascontiguousarray(y, np.float64)
+        self.x2 = np.ascontiguousarray(x2, np.float64)
+        self.y2 = np.ascontiguousarray(y2, np.float64)
+
+    def step(self, prim, nthreads=0, dt_prev=None):
+        """One step from prim (5 x nc); returns (next, status).  Uses and updates the
+        lagged time step unless dt_prev is given."""
+        now = np.ascontiguousarray(prim, np.float64).reshape(5, self.nc)
+        nxt = now.copy()
+        dt_new = ctypes.c_double()
+        m = self._mesh()
+        src = _d(self.src) if self.src is not None else None
+        st = lib().ora_step_mk2(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src,
+                                self.dt if dt_prev is None else dt_prev, ctypes.byref(dt_new),
+                                ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
+        if dt_prev is None:
+            self.dt = dt_new.value
+        self.last_dt_new = dt_new.value
+        return nxt, st
+
+    def run(self, prim, steps, nthreads=0):
+        cur = np.ascontiguousarray(prim, np.float64).reshape(5, self.nc)
+        for _ in range(steps):
+            cur, st = self.step(cur, nthreads)
+            if st != 0:
+                raise RuntimeError(f"oracle step failed with status {st}")
+        return cur

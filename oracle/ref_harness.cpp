@@ -1,0 +1,322 @@
+// ref_harness.cpp - drives the compiled reference (SergeyMSU/Gelios_2D sources under
+// /root/reference, built by oracle/Makefile into oracle/_ref/) to write golden vectors.
+// TEST INFRASTRUCTURE: runs only in the build container, never on the GPU box.
+//
+// It builds a polar quad grid out of the reference's own Point/Cell/Gran objects in a
+// Setka, lets the reference's Setka::TVD_prepare (Setka.cpp:1294-1492) build the face
+// stencils, and runs the reference's sweep Setka::Go_5_komponent__MK2
+// (Setka.cpp:10121-10482).  Point-wise modes call the flux, geometry, limiter and
+// source functions directly.
+//
+// Binary I/O: little-endian int32 / float64 arrays, laid out as documented per mode.
+#include "Setka.h"
+#include "Cell.h"
+#include "Gran.h"
+#include "Point.h"
+#include "Help.h"
+#include "Solvers.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <cstdint>
+#include <vector>
+
+static FILE* xopen(const char* p, const char* m)
+{
+    FILE* f = std::fopen(p, m);
+    if (!f) { std::fprintf(stderr, "cannot open %s\n", p); std::exit(2); }
+    return f;
+}
+template <class T> static void rd(FILE* f, T* v, size_t n)
+{
+    if (std::fread(v, sizeof(T), n, f) != n) { std::fprintf(stderr, "short read\n"); std::exit(2); }
+}
+template <class T> static void wr(FILE* f, const T* v, size_t n) { std::fwrite(v, sizeof(T), n, f); }
+
+// ---- point-wise: HLLC_2d_Korolkov_b_s -------------------------------------------
+// in : int32 n; float64 [n][16] = roL QL pL uL vL roR QR pR uR vR W n1 n2 rad metod nul
+// out: float64 [n][9] = P0 P1 P2 P3 PQ Vl Vc Vp time
+static int mode_hllc(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t n; rd(f, &n, 1);
+    std::vector<double> a((size_t)n * 16); rd(f, a.data(), a.size()); std::fclose(f);
+    Setka* S = new Setka();
+    std::vector<double> o((size_t)n * 9);
+    for (int k = 0; k < n; k++) {
+        const double* r = &a[(size_t)k * 16];
+        double P[4], PQ, Vl, Vc, Vp;
+        double t = S->HLLC_2d_Korolkov_b_s(r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10],
+                                           P, PQ, r[11], r[12], r[13], (int)r[14], Vl, Vc, Vp, r[15] != 0.0);
+        double* q = &o[(size_t)k * 9];
+        q[0] = P[0]; q[1] = P[1]; q[2] = P[2]; q[3] = P[3]; q[4] = PQ; q[5] = Vl; q[6] = Vc; q[7] = Vp; q[8] = t;
+    }
+    f = xopen(out, "wb"); wr(f, o.data(), o.size()); std::fclose(f);
+    return 0;
+}
+
+// ---- point-wise: Help.cpp helpers --------------------------------------------------
+// in : int32 n; float64 [n][7] = x1 t1 x2 t2 x3 t3 y
+// out: float64 [n][8] = minmod(x1,t1) linear3 linear2(x1..t2,y) polar_angle(x1,t1)
+//                       perenos(x1,t1 -> x2,t2 applied to (x3,t3)) -> u v, sign(x1)
+static int mode_helpers(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t n; rd(f, &n, 1);
+    std::vector<double> a((size_t)n * 7); rd(f, a.data(), a.size()); std::fclose(f);
+    std::vector<double> o((size_t)n * 8);
+    for (int k = 0; k < n; k++) {
+        const double* r = &a[(size_t)k * 7];
+        double* q = &o[(size_t)k * 8];
+        q[0] = minmod(r[0], r[1]);
+        q[1] = linear(r[0], r[1], r[2], r[3], r[4], r[5], r[6]);
+        q[2] = linear(r[0], r[1], r[2], r[3], r[6]);
+        q[3] = polar_angle(r[0], r[1]);
+        double u = r[4], v = r[5];
+        polar_perenos(r[0], r[1], r[2], r[3], u, v);
+        q[4] = u; q[5] = v;
+        q[6] = sign(r[0]);
+        q[7] = 0.0;
+    }
+    f = xopen(out, "wb"); wr(f, o.data(), o.size()); std::fclose(f);
+    return 0;
+}
+
+// ---- point-wise: cell geometry and Get_Sourse_MK1 ----------------------------------
+// in : int32 n; float64 [n][8+4+19] = quad x0 y0 .. x3 y3, (u v ro p), 19 MK1 fields
+// out: float64 [n][7] = center x y, Get_Volume, Get_Volume_rotate(360), q1 q2 q3
+static int mode_cell(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t n; rd(f, &n, 1);
+    const int W = 8 + 4 + 19;
+    std::vector<double> a((size_t)n * W); rd(f, a.data(), a.size()); std::fclose(f);
+    std::vector<double> o((size_t)n * 7);
+    for (int k = 0; k < n; k++) {
+        const double* r = &a[(size_t)k * W];
+        Point* P0 = new Point(r[0], r[1]);
+        Point* P1 = new Point(r[2], r[3]);
+        Point* P2 = new Point(r[4], r[5]);
+        Point* P3 = new Point(r[6], r[7]);
+        Cell* C = new Cell(P0, P1, P2, P3);
+        const double* fld = r + 12;
+        for (int h = 0; h < 4; h++) {
+            C->par[0].H_n[h] = fld[h];
+            C->par[0].H_u[h] = fld[4 + h];
+            C->par[0].H_v[h] = fld[8 + h];
+            C->par[0].H_T[h] = fld[12 + h];
+        }
+        C->par[0].k_u = fld[16]; C->par[0].k_v = fld[17]; C->par[0].k_T = fld[18];
+        double* q = &o[(size_t)k * 7];
+        C->Get_Center(q[0], q[1]);
+        q[2] = C->Get_Volume();
+        q[3] = C->Get_Volume_rotate(360.0);
+        C->Get_Sourse_MK1(q[4], q[5], q[6], r[8], r[9], r[10], r[11]);
+    }
+    f = xopen(out, "wb"); wr(f, o.data(), o.size()); std::fclose(f);
+    return 0;
+}
+
+// ---- whole sweep -----------------------------------------------------------------
+// in : int32 ni nj steps moving has_src
+//      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
+//      each step (x2 then y2)
+//      int32 cell_type[nc] outer_type[nj]
+//      float64 prim[5][nc] (ro p u v Q); if has_src: float64 src[19][nc]
+// Static: Go_5_komponent__MK2(steps).  Moving: for each step set Point::x2,y2 and
+// call Go_5_komponent__MK2(1), so every step uses the reference's first-step dt.
+// out: float64 prim[5][nc]; int32 contour[nc][4]; int32 nface[nc];
+//      int32 slots[nc][8][6] = A B type sosed sosed_down sosed_up (-1 when absent)
+static int mode_sweep(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[5]; rd(f, h, 5);
+    const int ni = h[0], nj = h[1], steps = h[2], moving = h[3], has_src = h[4];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn);
+    rd(f, x.data(), nn); rd(f, y.data(), nn);
+    std::vector<double> traj;
+    if (moving) { traj.resize((size_t)steps * 2 * nn); rd(f, traj.data(), traj.size()); }
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)5 * nc); rd(f, prim.data(), prim.size());
+    std::vector<double> src;
+    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
+    std::fclose(f);
+
+    Setka* S = new Setka();
+    auto nid = [&](int i, int j) { return i * (nj + 1) + j; };
+    auto cid = [&](int i, int j) { return i * nj + j; };
+    for (int k = 0; k < nn; k++) {
+        Point* P = new Point(x[k], y[k]);
+        P->number = k;
+        S->All_Points.push_back(P);
+    }
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            Cell* C = new Cell(S->All_Points[nid(i, j)], S->All_Points[nid(i + 1, j)],
+                               S->All_Points[nid(i + 1, j + 1)], S->All_Points[nid(i, j + 1)]);
+            C->number = cid(i, j);
+            C->type = (Cell_type)ctype[cid(i, j)];
+            S->All_Cells.push_back(C);
+        }
+    int gnum = 0;
+    // Face of edge k of cell (i,j): contour[k] -> contour[k+1] (normal out of the cell).
+    // Edges 1 (outer, i+1) and 2 (towards j+1) are owned by the cell; edges 0 and 3 are
+    // the copies of the neighbours' faces.  Boundary copies have no master, as the
+    // reference builds its axis faces (Setka.cpp:711-741).
+    std::vector<Gran*> own1(nc, nullptr), own2(nc, nullptr);
+    auto mk_boundary = [&](Cell* K, Point* A, Point* B, Gran_type t) {
+        Gran* G = new Gran(A, B, t);
+        Gran* G2 = new Gran(B, A, t);
+        G->main_gran = true; G2->main_gran = false;
+        G->Master = K; G2->Master = nullptr; G2->Sosed = K;
+        G->Gran_copy = G2; G2->Gran_copy = G;
+        G->number = gnum++; G2->number = gnum++;
+        S->All_Gran.push_back(G); S->All_Gran_copy.push_back(G2);
+        return G;
+    };
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            Cell* K = S->All_Cells[cid(i, j)];
+            Point* c1 = K->contour[1]; Point* c2 = K->contour[2]; Point* c3 = K->contour[3];
+            if (i + 1 < ni) {
+                Cell* N = S->All_Cells[cid(i + 1, j)];
+                Gran* G = new Gran(c1, c2, Usualy);
+                Gran* G2 = new Gran(c2, c1, Usualy);
+                G->Master = K; G->Sosed = N; G2->Master = N; G2->Sosed = K;
+                G->Gran_copy = G2; G2->Gran_copy = G; G2->main_gran = false;
+                G->number = gnum++; G2->number = gnum++;
+                S->All_Gran.push_back(G); S->All_Gran_copy.push_back(G2);
+                own1[cid(i, j)] = G;
+            } else {
+                own1[cid(i, j)] = mk_boundary(K, c1, c2, (Gran_type)otype[j]);
+            }
+            if (j + 1 < nj) {
+                Cell* N = S->All_Cells[cid(i, j + 1)];
+                Gran* G = new Gran(c2, c3, Usualy);
+                Gran* G2 = new Gran(c3, c2, Usualy);
+                G->Master = K; G->Sosed = N; G2->Master = N; G2->Sosed = K;
+                G->Gran_copy = G2; G2->Gran_copy = G; G2->main_gran = false;
+                G->number = gnum++; G2->number = gnum++;
+                S->All_Gran.push_back(G); S->All_Gran_copy.push_back(G2);
+                own2[cid(i, j)] = G;
+            } else {
+                own2[cid(i, j)] = mk_boundary(K, c2, c3, Axis);
+            }
+        }
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            Cell* K = S->All_Cells[cid(i, j)];
+            Gran* e0 = (j > 0) ? own2[cid(i, j - 1)]->Gran_copy : mk_boundary(K, K->contour[0], K->contour[1], Axis);
+            Gran* e3 = (i > 0) ? own1[cid(i - 1, j)]->Gran_copy : mk_boundary(K, K->contour[3], K->contour[0], Inner_sphere);
+            K->Grans.push_back(e0);
+            K->Grans.push_back(own1[cid(i, j)]);
+            K->Grans.push_back(own2[cid(i, j)]);
+            K->Grans.push_back(e3);
+        }
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        for (int b = 0; b < 2; b++) {
+            Parametr& p = K->par[b];
+            p.ro = prim[c]; p.p = prim[nc + c]; p.u = prim[2 * nc + c]; p.v = prim[3 * nc + c]; p.Q = prim[4 * nc + c];
+            for (int hh = 0; hh < 4; hh++) {
+                p.H_n[hh] = has_src ? src[(size_t)hh * nc + c] : 0.0;
+                p.H_u[hh] = has_src ? src[(size_t)(4 + hh) * nc + c] : 0.0;
+                p.H_v[hh] = has_src ? src[(size_t)(8 + hh) * nc + c] : 0.0;
+                p.H_T[hh] = has_src ? src[(size_t)(12 + hh) * nc + c] : 0.0;
+            }
+            p.k_u = has_src ? src[(size_t)16 * nc + c] : 0.0;
+            p.k_v = has_src ? src[(size_t)17 * nc + c] : 0.0;
+            p.k_T = has_src ? src[(size_t)18 * nc + c] : 0.0;
+        }
+        K->Back = nullptr;
+        K->Next = nullptr;
+    }
+    S->TVD_prepare();
+
+    // Setka::Move_surface (called by every step of Go_5_komponent__MK2) indexes the
+    // fitted-surface face lists without checking that they are non-empty
+    // (Setka.cpp:3532, :3734, :3840).  The polar grid has no fitted surfaces, so each
+    // list gets one stand-in face between two detached cells far from the grid.  Its
+    // nodes are not in All_Points and there are no rails, so Move_Setka_Calculate_2
+    // moves nothing and the stand-in never touches the grid.
+    {
+        S->N1 = S->N2 = S->N3 = S->N4 = S->M1 = S->M2 = S->M3 = S->M4 = 0;
+        std::vector<Point*> dp;
+        for (int k = 0; k <= 4; k++) { dp.push_back(new Point(1000.0 + k, 1000.0)); dp.push_back(new Point(1000.0 + k, 1001.0)); }
+        std::vector<Cell*> dc;
+        for (int k = 0; k < 4; k++) {
+            Cell* C = new Cell(dp[2 * k], dp[2 * k + 2], dp[2 * k + 3], dp[2 * k + 1]);
+            C->type = C_2;
+            for (int b = 0; b < 2; b++) { C->par[b].ro = 1.0; C->par[b].p = 1.0; C->par[b].u = 0.0; C->par[b].v = 0.0; C->par[b].Q = 1.0; }
+            dc.push_back(C);
+        }
+        Gran* G = new Gran(dp[4], dp[5], Usualy);
+        Gran* G2 = new Gran(dp[5], dp[4], Usualy);
+        G->Master = dc[1]; G->Sosed = dc[2]; G->Sosed_down = dc[0]; G->Sosed_up = dc[3];
+        G2->Master = dc[2]; G2->Sosed = dc[1]; G2->Sosed_down = dc[3]; G2->Sosed_up = dc[0];
+        G->Gran_copy = G2; G2->Gran_copy = G;
+        G->number = gnum++; G2->number = gnum++;
+        S->Line_Contact.push_back(G);
+        S->Line_Inner.push_back(G);
+        S->Line_Outer.push_back(G);
+    }
+
+    int final_buf;
+    if (!moving) {
+        S->Go_5_komponent__MK2(steps);
+        final_buf = steps % 2;
+    } else {
+        for (int s = 0; s < steps; s++) {
+            const double* t = &traj[(size_t)s * 2 * nn];
+            for (int k = 0; k < nn; k++) { S->All_Points[k]->x2 = t[k]; S->All_Points[k]->y2 = t[nn + k]; }
+            S->Go_5_komponent__MK2(1);
+            // the single step wrote par[1]; carry it to par[0], the next call's input
+            for (auto* K : S->All_Cells) {
+                K->par[0].ro = K->par[1].ro; K->par[0].p = K->par[1].p; K->par[0].u = K->par[1].u;
+                K->par[0].v = K->par[1].v; K->par[0].Q = K->par[1].Q;
+            }
+        }
+        final_buf = 0;
+    }
+
+    std::vector<double> res((size_t)5 * nc);
+    for (int c = 0; c < nc; c++) {
+        const Parametr& p = S->All_Cells[c]->par[final_buf];
+        res[c] = p.ro; res[nc + c] = p.p; res[2 * nc + c] = p.u; res[3 * nc + c] = p.v; res[4 * nc + c] = p.Q;
+    }
+    std::vector<int32_t> contour((size_t)nc * 4), nface(nc), slots((size_t)nc * 8 * 6, -1);
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        for (int k = 0; k < 4; k++) contour[(size_t)c * 4 + k] = K->contour[k]->number;
+        nface[c] = (int32_t)K->Grans.size();
+        for (size_t k = 0; k < K->Grans.size(); k++) {
+            Gran* G = K->Grans[k];
+            int32_t* s = &slots[((size_t)c * 8 + k) * 6];
+            s[0] = G->A->number; s[1] = G->B->number; s[2] = (int32_t)G->type;
+            s[3] = G->Sosed ? G->Sosed->number : -1;
+            s[4] = G->Sosed_down ? G->Sosed_down->number : -1;
+            s[5] = G->Sosed_up ? G->Sosed_up->number : -1;
+        }
+    }
+    f = xopen(out, "wb");
+    wr(f, res.data(), res.size());
+    wr(f, contour.data(), contour.size());
+    wr(f, nface.data(), nface.size());
+    wr(f, slots.data(), slots.size());
+    std::fclose(f);
+    return 0;
+}
+
+int main(int argc, char** argv)
+{
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep in out\n"); return 2; }
+    if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "sweep")) return mode_sweep(argv[2], argv[3]);
+    std::fprintf(stderr, "unknown mode %s\n", argv[1]);
+    return 2;
+}

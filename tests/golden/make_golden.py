@@ -1,0 +1,210 @@
+"""Write the golden fixtures of tests/golden/ by running the compiled reference.
+
+Runs only in the build container, where /root/reference exists: oracle/Makefile
+compiles the reference sources plus oracle/ref_harness.cpp into oracle/_ref/, and this
+script feeds the harness seeded inputs and stores inputs and outputs as .npz files.
+The reference's sweep has races on its shared time step (Setka.cpp:10141,
+:10388-10393), so the harness runs with OMP_NUM_THREADS=1.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from gelios_2d_amd.grid import polar_grid  # noqa: E402
+from gelios_2d_amd.params import (GEL_C_1, GEL_C_2, GEL_F_EXTERN, GEL_F_INPUT,  # noqa: E402
+                                  reference_params)
+from oracle import oracle as ora  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def harness(mode, blob: bytes, tmp):
+    ora.build()
+    fin = os.path.join(tmp, f"{mode}.in")
+    fout = os.path.join(tmp, f"{mode}.out")
+    with open(fin, "wb") as f:
+        f.write(blob)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([ora.REF_HARNESS, mode, fin, fout], check=True, cwd=tmp, env=env,
+                   stdout=subprocess.DEVNULL)
+    with open(fout, "rb") as f:
+        return f.read()
+
+
+def gen_hllc(rng, tmp, n=2000):
+    roL = rng.uniform(0.05, 3.0, n)
+    roR = rng.uniform(0.05, 3.0, n)
+    pL = rng.uniform(0.01, 3.0, n)
+    pR = rng.uniform(0.01, 3.0, n)
+    QL = rng.uniform(0.0, 5.0, n)
+    QR = rng.uniform(0.0, 5.0, n)
+    # a quarter of the faces carry a strong common drift (supersonic branches)
+    drift = np.where(rng.random(n) < 0.25, rng.normal(0, 6.0, (2, n)), 0.0)
+    uL, vL = rng.normal(0, 1.0, (2, n)) + drift
+    uR, vR = rng.normal(0, 1.0, (2, n)) + drift
+    ang = rng.uniform(0, 2 * np.pi, n)
+    n1, n2 = np.cos(ang), np.sin(ang)
+    W = np.where(rng.random(n) < 0.5, 0.0, rng.normal(0, 1.5, n))
+    rad = rng.uniform(0.01, 1.0, n)
+    metod = (np.arange(n) % 2).astype(np.float64)
+    nul = ((np.arange(n) // 2) % 2).astype(np.float64)
+    inp = np.stack([roL, QL, pL, uL, vL, roR, QR, pR, uR, vR, W, n1, n2, rad, metod, nul], axis=1)
+    out = harness("hllc", np.int32(n).tobytes() + inp.tobytes(), tmp)
+    res = np.frombuffer(out, np.float64).reshape(n, 9)
+    np.savez_compressed(os.path.join(OUT, "hllc_vectors.npz"), inp=inp, out=res)
+
+
+def gen_helpers(rng, tmp, n=1000):
+    x1, x3 = rng.normal(0, 1, (2, n))
+    x2 = x1 + rng.uniform(0.1, 1.0, n) * np.sign(rng.normal(size=n))
+    t1, t2, t3 = rng.normal(0, 1, (3, n))
+    # exact ties and zeros exercise the sign/minmod branches
+    t2[::7] = t1[::7]
+    t3[::11] = t2[::11]
+    x1[::13] = 0.0
+    y = rng.normal(0, 1, n)
+    inp = np.stack([x1, t1, x2, t2, x3, t3, y], axis=1)
+    out = harness("helpers", np.int32(n).tobytes() + inp.tobytes(), tmp)
+    res = np.frombuffer(out, np.float64).reshape(n, 8)
+    np.savez_compressed(os.path.join(OUT, "helpers_vectors.npz"), inp=inp, out=res)
+
+
+def gen_cell(rng, tmp, n=300):
+    rows = []
+    for k in range(n):
+        # perturbed polar quads; centres at x < 0 or r > 1500/RR_ keep Get_Sourse_MK1 on
+        # its own-cell branch (Cell.cpp:590-595)
+        r0 = rng.uniform(4.0, 20.0)
+        th0 = rng.uniform(0.05, np.pi - 0.3)
+        dr, dth = rng.uniform(0.05, 1.0), rng.uniform(0.02, 0.25)
+        pts = [(r0, th0), (r0 + dr, th0), (r0 + dr, th0 + dth), (r0, th0 + dth)]
+        q = []
+        for (r, t) in pts:
+            r *= 1 + 0.02 * rng.normal()
+            q += [r * np.cos(t), r * np.sin(t)]
+        uvrp = [rng.normal(), rng.normal(), rng.uniform(0.1, 3), rng.uniform(0.1, 3)]
+        fld = list(rng.uniform(0.01, 1.0, 4)) + list(rng.normal(0, 2, 4)) + list(rng.normal(0, 2, 4)) + \
+            list(rng.uniform(0.1, 3.0, 4)) + list(rng.uniform(0.33, 3.0, 3))
+        if k % 10 == 0:
+            fld[k % 4] = 0.0      # population with no density: clamp branch
+        rows.append(q + uvrp + fld)
+    inp = np.array(rows, np.float64)
+    out = harness("cell", np.int32(n).tobytes() + inp.tobytes(), tmp)
+    res = np.frombuffer(out, np.float64).reshape(n, 7)
+    np.savez_compressed(os.path.join(OUT, "cell_vectors.npz"), inp=inp, out=res)
+
+
+def harness_grid(ni=16, nj=10, n_c1=3):
+    g = polar_grid(ni, nj, 4.0, 12.0, spacing="log")
+    ct = np.full(ni * nj, GEL_C_2, np.int8)
+    ct[: n_c1 * nj] = GEL_C_1
+    g.cell_type = ct
+    return g
+
+
+def initial_state(g, rng, prm):
+    """Wind from the inner sphere (the Inner_sphere ghost formula, Gran.cpp:234-265)
+    blended into the reference's Input state, with a seeded 1% density noise."""
+    cx, cy = g.cell_centers()
+    r = np.hypot(cx, cy)
+    ro_w = 1.0 / (prm.in_rho_div * r * r)
+    p_w = ro_w * prm.in_p_a * prm.in_p_a / prm.in_p_den
+    u_w, v_w = prm.in_u * cx / r, prm.in_u * cy / r
+    s = 0.5 * (1 - np.tanh((r - 7.0) / 1.5))
+    ro = s * ro_w + (1 - s) * prm.inf_rho
+    p = s * p_w + (1 - s) * prm.inf_p
+    u = s * u_w + (1 - s) * prm.inf_u
+    v = s * v_w + (1 - s) * prm.inf_v
+    Q = s * ro_w + (1 - s) * prm.inf_q
+    ro = ro * (1 + 0.01 * rng.uniform(-1, 1, ro.shape))
+    return np.stack([ro, p, u, v, Q])
+
+
+def sweep_blob(g, steps, prim, traj=None, src=None):
+    h = np.array([g.ni, g.nj, steps, 1 if traj is not None else 0, 1 if src is not None else 0], np.int32)
+    parts = [h.tobytes(), g.x.tobytes(), g.y.tobytes()]
+    if traj is not None:
+        parts.append(np.ascontiguousarray(traj, np.float64).tobytes())
+    parts += [g.cell_type.astype(np.int32).tobytes(), g.outer_type.astype(np.int32).tobytes(),
+              np.ascontiguousarray(prim, np.float64).tobytes()]
+    if src is not None:
+        parts.append(np.ascontiguousarray(src, np.float64).tobytes())
+    return b"".join(parts)
+
+
+def parse_sweep(out, g):
+    nc = g.ncells
+    o = 0
+    prim = np.frombuffer(out, np.float64, 5 * nc, o).reshape(5, nc); o += 5 * nc * 8
+    contour = np.frombuffer(out, np.int32, 4 * nc, o).reshape(nc, 4); o += 4 * nc * 4
+    nface = np.frombuffer(out, np.int32, nc, o); o += nc * 4
+    slots = np.frombuffer(out, np.int32, nc * 8 * 6, o).reshape(nc, 8, 6)
+    return prim.copy(), contour.copy(), nface.copy(), slots.copy()
+
+
+def gen_sweeps(rng, tmp):
+    prm = reference_params()
+    # static grid, 40 steps of Go_5_komponent__MK2
+    g = harness_grid()
+    prim0 = initial_state(g, rng, prm)
+    steps = 40
+    out = harness("sweep", sweep_blob(g, steps, prim0), tmp)
+    prim, contour, nface, slots = parse_sweep(out, g)
+    np.savez_compressed(os.path.join(OUT, "sweep_static.npz"), ni=g.ni, nj=g.nj, x=g.x, y=g.y,
+                        cell_type=g.cell_type, outer_type=g.outer_type, steps=steps, prim0=prim0,
+                        prim=prim, contour=contour, nface=nface, slots=slots)
+
+    # moving grid: every step a seeded radial displacement of the nodes
+    steps = 6
+    traj = []
+    x, y = g.x.copy(), g.y.copy()
+    for s in range(steps):
+        f = 1.0 + 1e-9 * rng.uniform(-1, 1, x.shape)
+        traj.append(np.stack([x * f, y * f]))
+        x, y = x * f, y * f
+    traj = np.array(traj)
+    prim0 = initial_state(g, rng, prm)
+    out = harness("sweep", sweep_blob(g, steps, prim0, traj=traj), tmp)
+    prim, contour, nface, slots = parse_sweep(out, g)
+    np.savez_compressed(os.path.join(OUT, "sweep_moving.npz"), ni=g.ni, nj=g.nj, x=g.x, y=g.y,
+                        cell_type=g.cell_type, outer_type=g.outer_type, steps=steps, prim0=prim0,
+                        traj=traj, prim=prim, contour=contour, nface=nface, slots=slots)
+
+    # static grid with charge-exchange sources from 4 populations
+    steps = 25
+    nc = g.ncells
+    src = np.concatenate([rng.uniform(0.05, 0.5, (4, nc)), rng.normal(-2.0, 0.5, (4, nc)),
+                          rng.normal(0.0, 0.5, (4, nc)), rng.uniform(0.5, 2.0, (4, nc)),
+                          rng.uniform(0.5, 2.0, (3, nc))])
+    prim0 = initial_state(g, rng, prm)
+    out = harness("sweep", sweep_blob(g, steps, prim0, src=src), tmp)
+    prim, contour, nface, slots = parse_sweep(out, g)
+    np.savez_compressed(os.path.join(OUT, "sweep_sources.npz"), ni=g.ni, nj=g.nj, x=g.x, y=g.y,
+                        cell_type=g.cell_type, outer_type=g.outer_type, steps=steps, prim0=prim0,
+                        src=src, prim=prim, contour=contour, nface=nface, slots=slots)
+
+
+def main():
+    rng = np.random.default_rng(20261203)
+    with tempfile.TemporaryDirectory() as tmp:
+        gen_hllc(rng, tmp)
+        gen_helpers(rng, tmp)
+        gen_cell(rng, tmp)
+        gen_sweeps(rng, tmp)
+    for f in sorted(os.listdir(OUT)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(OUT, f)))
+
+
+if __name__ == "__main__":
+    main()
