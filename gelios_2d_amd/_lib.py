@@ -1,0 +1,87 @@
+"""Loader of the HIP library libgelios.so (built in-tree by ``gelios_2d_amd.build``).
+
+There is no CPU fallback: every product call goes through this library, and
+:func:`lib` raises if it is missing or fails to load.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+from .params import GelParams
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgelios.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_i8p = ctypes.POINTER(ctypes.c_int8)
+
+
+class GelGrid(ctypes.Structure):
+    _fields_ = [("ni", ctypes.c_int), ("nj", ctypes.c_int), ("x", _dp), ("y", _dp),
+                ("cell_type", _i8p), ("outer_type", _i8p)]
+
+
+class GelStats(ctypes.Structure):
+    _fields_ = [("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double), ("steps", ctypes.c_longlong),
+                ("n_rho_clamp", ctypes.c_longlong), ("n_p_clamp", ctypes.c_longlong),
+                ("n_numeric", ctypes.c_longlong)]
+
+
+class GelError(RuntimeError):
+    pass
+
+
+# Every symbol include/gelios.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "gel_params_reference": (None, [ctypes.POINTER(GelParams)]),
+    "gel_ctx_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GelGrid),
+                                      ctypes.POINTER(GelParams)]),
+    "gel_ctx_destroy": (None, [ctypes.c_void_p]),
+    "gel_upload_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_download_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_set_sources": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(GelStats)]),
+    "gel_step_moving": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.POINTER(GelStats)]),
+    "gel_sync": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelStats)]),
+    "gel_set_dt": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "gel_set_rows_per_wave": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gel_enable_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "gel_kernel_timing": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.POINTER(ctypes.c_longlong)]),
+    "gel_flux_hllc2d_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                             _dp, _dp]),
+    "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "gel_last_error": (ctypes.c_char_p, []),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libgelios.so; raise if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GelError(f"{LIB_PATH} is not built; run gelios_2d_amd.build.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != 0:
+        L = lib()
+        raise GelError(f"{L.gel_strerror(status).decode()}: {L.gel_last_error().decode()}")
+
+
+def dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def i8ptr(a):
+    return a.ctypes.data_as(_i8p)

@@ -1,0 +1,114 @@
+"""Host-side mirror of the reference's sweep interface, over the C-ABI.
+
+The reference drives its sweep through the C++ object ``Setka`` (Setka.h:21-235):
+cells carry ``par[2]`` double buffers (Cell.h:98), ``Go_5_komponent__MK2(step)``
+advances ``step`` steps (Setka.h:186, Setka.cpp:10121-10482) and
+``HLLC_2d_Korolkov_b_s`` evaluates one face flux (Setka.h:187-189).  :class:`Setka`
+keeps those names; the state lives in device memory of one MI355X and every step
+runs in libgelios.so.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .grid import PolarGrid
+from .params import GEL_MK1_NFIELDS, NVAR_GAS, GelParams
+
+
+class Setka:
+    """A polar grid, its state on the GPU, and the sweep.
+
+    ``prim`` arrays are variable-major (ro, p, u, v, Q) x ncells, cell (i, j) at
+    ``i*nj + j`` -- the fields of Parametr (Cell.h:21-25) as structure of arrays.
+    """
+
+    def __init__(self, grid: PolarGrid, params: GelParams):
+        self.grid = grid
+        self.params = params.copy()
+        self._ctx = ctypes.c_void_p()
+        g = _lib.GelGrid(grid.ni, grid.nj, _lib.dptr(grid.x), _lib.dptr(grid.y),
+                         _lib.i8ptr(grid.cell_type), _lib.i8ptr(grid.outer_type))
+        _lib.check(_lib.lib().gel_ctx_create(ctypes.byref(self._ctx), ctypes.byref(g), ctypes.byref(self.params)))
+        self.stats = _lib.GelStats()
+
+    def close(self):
+        if self._ctx:
+            _lib.lib().gel_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def ncells(self) -> int:
+        return self.grid.ncells
+
+    # -- state ------------------------------------------------------------------
+    def upload(self, prim: np.ndarray) -> None:
+        a = np.ascontiguousarray(prim, np.float64).reshape(NVAR_GAS, self.ncells)
+        _lib.check(_lib.lib().gel_upload_state(self._ctx, _lib.dptr(a)))
+
+    def download(self) -> np.ndarray:
+        out = np.empty((NVAR_GAS, self.ncells), np.float64)
+        _lib.check(_lib.lib().gel_download_state(self._ctx, _lib.dptr(out)))
+        return out
+
+    def set_sources(self, fields: np.ndarray) -> None:
+        a = np.ascontiguousarray(fields, np.float64).reshape(GEL_MK1_NFIELDS, self.ncells)
+        _lib.check(_lib.lib().gel_set_sources(self._ctx, _lib.dptr(a)))
+
+    def set_rows_per_wave(self, rows: int) -> None:
+        _lib.check(_lib.lib().gel_set_rows_per_wave(self._ctx, int(rows)))
+
+    # -- sweep ------------------------------------------------------------------
+    def Go_5_komponent__MK2(self, step: int) -> _lib.GelStats:
+        """Advance ``step`` steps on the static grid (Setka.cpp:10121-10482)."""
+        _lib.check(_lib.lib().gel_step(self._ctx, int(step), ctypes.byref(self.stats)))
+        return self.stats
+
+    def step_moving(self, x2: np.ndarray, y2: np.ndarray) -> _lib.GelStats:
+        """One step with node positions after the step (Point::x2, y2, as
+        Move_Setka_Calculate_2 leaves them); commits x <- x2 (Setka.cpp:10473-10480)."""
+        a = np.ascontiguousarray(x2, np.float64)
+        b = np.ascontiguousarray(y2, np.float64)
+        _lib.check(_lib.lib().gel_step_moving(self._ctx, _lib.dptr(a), _lib.dptr(b), ctypes.byref(self.stats)))
+        return self.stats
+
+    def set_dt(self, dt: float) -> None:
+        _lib.check(_lib.lib().gel_set_dt(self._ctx, float(dt)))
+
+    def sync(self) -> _lib.GelStats:
+        _lib.check(_lib.lib().gel_sync(self._ctx, ctypes.byref(self.stats)))
+        return self.stats
+
+    def enable_timing(self, on: bool = True) -> None:
+        _lib.check(_lib.lib().gel_enable_timing(self._ctx, 1 if on else 0))
+
+    def kernel_timing(self):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        _lib.check(_lib.lib().gel_kernel_timing(self._ctx, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+
+def HLLC_2d_Korolkov_b_s(L, R, W, n1, n2, rad, metod, nul_potok=False, cfl=0.1, gamma=5.0 / 3.0):
+    """Batched device evaluation of the face flux (Setka.cpp:10484-10700).
+
+    L, R: [n, 5] as (ro, Q, p, u, v) -- the argument order of the reference.
+    Returns P [n, 4], PQ, Vl, Vc, Vp, dt [n] each."""
+    L = np.ascontiguousarray(L, np.float64).reshape(-1, 5)
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 5)
+    n = L.shape[0]
+    vec = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,))) for a in (W, n1, n2, rad)]
+    P = np.empty((n, 4))
+    outs = [np.empty(n) for _ in range(5)]
+    _lib.check(_lib.lib().gel_flux_hllc2d_batch(n, _lib.dptr(L), _lib.dptr(R), *[_lib.dptr(v) for v in vec],
+                                                float(cfl), float(gamma), int(metod), int(bool(nul_potok)),
+                                                _lib.dptr(P), *[_lib.dptr(o) for o in outs]))
+    return (P, *outs)

@@ -100,7 +100,7 @@ typedef struct gel_params {
     double in_rho_div, in_p_a, in_p_den, in_u, in_q;
     /* Input ghost state (Gran.cpp:276-282): ro, p, u, v, Q. */
     double inf_rho, inf_p, inf_u, inf_v, inf_q;
-    /* Extern: outflow copy whose normal... u >= 0 is replaced by this (Gran.cpp:212-220). */
+    /* Extern: copy of the cell, with u >= 0 replaced by this value (Gran.cpp:212-220). */
     double extern_u;
     /* Charge-exchange constants (Help.h:60-64, Cell.cpp:620-659):
        coefficient n_p/Kn, and sigma(U) = (sig_a - sig_b*ln U)^2. */
@@ -151,6 +151,18 @@ int gel_set_sources(gel_ctx* ctx, const double* fields);
  * Steps run back to back on the context's stream without host round trips; the
  * lagged CFL step (Setka.cpp:10388-10393) stays on the device.  out may be NULL. */
 int gel_step(gel_ctx* ctx, int nsteps, gel_stats* out);
+
+/* One step on a moving grid: x2,y2 are the host node positions after the step, as
+ * Setka::Move_Setka_Calculate_2 leaves them in Point::x2, y2 (Setka.cpp:10145); face
+ * speeds and swept volumes come from old and new positions (Setka.cpp:10187-10209),
+ * and the step ends with Point::x = x2 (Setka.cpp:10473-10480). */
+int gel_step_moving(gel_ctx* ctx, const double* x2, const double* y2, gel_stats* out);
+
+/* Override the time step the next step uses (T[now1], Setka.cpp:10143-10145). */
+int gel_set_dt(gel_ctx* ctx, double dt);
+
+/* Rows of a ray each wavefront marches over (launch shape; the autotuner sets it). */
+int gel_set_rows_per_wave(gel_ctx* ctx, int rows);
 
 /* Block until queued work is done and fill *out (may be NULL). */
 int gel_sync(gel_ctx* ctx, gel_stats* out);
