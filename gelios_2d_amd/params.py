@@ -13,7 +13,8 @@ import math
 GEL_F_USUALY, GEL_F_INNER_SPHERE, GEL_F_EXTERN, GEL_F_INPUT, GEL_F_AXIS, GEL_F_UPPER_WALL = range(6)
 GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_C_NO = range(7)
 GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
-GEL_PHYS_GAS = 0
+GEL_PHYS_GAS, GEL_PHYS_MHD = 0, 1
+NVAR_MHD = 9
 GEL_SRC_NONE, GEL_SRC_MK1 = 0, 1
 GEL_MK1_NFIELDS = 19
 NVAR_GAS = 5
@@ -54,6 +55,10 @@ class GelParams(ctypes.Structure):
         ("src_coef", ctypes.c_double),
         ("sig_a", ctypes.c_double),
         ("sig_b", ctypes.c_double),
+        ("inf_w", ctypes.c_double),
+        ("inf_bx", ctypes.c_double),
+        ("inf_by", ctypes.c_double),
+        ("inf_bz", ctypes.c_double),
     ]
 
     def copy(self) -> "GelParams":
@@ -109,6 +114,3 @@ def heliosphere_params(*, flux: int = GEL_FLUX_HLLC, cfl: float = 0.4, rho_in: f
     p.inf_rho, p.inf_p, p.inf_u, p.inf_v, p.inf_q = rho_inf, p_inf, -u_inf, 0.0, 0.0
     return p
 
-
-def pi_() -> float:
-    return math.pi

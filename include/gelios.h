@@ -71,7 +71,11 @@ enum gel_cell_type {
 /* Face flux: the `metod` argument of HLLC_2d_Korolkov_b_s (0 = HLL, 1 = HLLC). */
 enum gel_flux { GEL_FLUX_HLL = 0, GEL_FLUX_HLLC = 1 };
 
-enum gel_physics { GEL_PHYS_GAS = 0 };
+/* GAS: 5 primitives (ro, p, u, v, Q), face flux of HLLC_2d_Korolkov_b_s.
+ * MHD: 9 primitives (ro, p, u, v, w, bx, by, bz, Q), HLLD face flux after
+ * Solvers::HLLDQ_Korolkov metod 3 (Solvers.cpp:1980-2185) for any in-plane normal and
+ * moving faces; B in Gaussian units as there (b = B/sqrt(4 pi)). */
+enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1 };
 
 enum gel_source_mode {
     GEL_SRC_NONE = 0,
@@ -105,6 +109,9 @@ typedef struct gel_params {
     /* Charge-exchange constants (Help.h:60-64, Cell.cpp:620-659):
        coefficient n_p/Kn, and sigma(U) = (sig_a - sig_b*ln U)^2. */
     double src_coef, sig_a, sig_b;
+    /* MHD Input ghost: azimuthal velocity and field (bx, by, bphi); the inner-sphere
+       wind carries no field. */
+    double inf_w, inf_bx, inf_by, inf_bz;
 } gel_params;
 
 /* Fill *p with the reference's own constants (Help.h, Gran.cpp) and HLL flux. */

@@ -527,3 +527,370 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
     if (n_p_clamp) *n_p_clamp += npc;
     return bad_any ? GEL_E_NUMERIC : GEL_OK;
 }
+
+/* ------------------------------------------------------------------ MHD HLLD --- */
+
+static double sgn_ref(double x) { return sgn(x); }
+
+/* HLLD flux of Solvers::HLLDQ_Korolkov, metod 3 (Solvers.cpp:1642-1740 set-up,
+ * :1980-2185 HLLD), extended to any normal n = (n1, n2, 0) in the meridional plane and
+ * to a face moving with speed W along n: the fan is sampled at xi = W and every
+ * region's flux is taken minus W*U, as the gas flux does (Setka.cpp:10569-10618).
+ * Basis: t = (-n2, n1, 0), m = (0, 0, 1) (the reference's choice for n = (1,0,0),
+ * Solvers.cpp:1664-1668).  B is Gaussian; b = B/sqrt(4 pi) inside.
+ * P: [0] mass, [1..3] momentum x,y,phi, [4] energy, [5..7] B x,y,phi, [8] tracer.
+ * Returns max(|SL|,|SR|). */
+double ora_hlld(const ora_mhd* L, const ora_mhd* R, double W, double n1, double n2, double gam, double P[9])
+{
+    const double spi4 = sqrt(4 * 3.14159265358979323846);   /* Solvers.h:9-12 */
+    const double g1 = gam - 1.0;
+    const double n3 = 0.0;
+    const double t1 = -n2, t2 = n1, t3 = 0.0;
+    const double m1 = 0.0, m2 = 0.0, m3 = 1.0;
+    const double bx_L = L->bx / spi4, by_L = L->by / spi4, bz_L = L->bz / spi4;
+    const double bx_R = R->bx / spi4, by_R = R->by / spi4, bz_R = R->bz / spi4;
+
+    const double u1 = L->u * n1 + L->v * n2 + L->w * n3;
+    const double v1 = L->u * t1 + L->v * t2 + L->w * t3;
+    const double w1 = L->u * m1 + L->v * m2 + L->w * m3;
+    const double u2 = R->u * n1 + R->v * n2 + R->w * n3;
+    const double v2 = R->u * t1 + R->v * t2 + R->w * t3;
+    const double w2 = R->u * m1 + R->v * m2 + R->w * m3;
+    const double bn1 = bx_L * n1 + by_L * n2 + bz_L * n3;
+    const double bt1 = bx_L * t1 + by_L * t2 + bz_L * t3;
+    const double bm1 = bx_L * m1 + by_L * m2 + bz_L * m3;
+    const double bn2 = bx_R * n1 + by_R * n2 + bz_R * n3;
+    const double bt2 = bx_R * t1 + by_R * t2 + bz_R * t3;
+    const double bm2 = bx_R * m1 + by_R * m2 + bz_R * m3;
+
+    const double ro_L = L->ro, ro_R = R->ro, p_L = L->p, p_R = R->p, Q_L = L->Q, Q_R = R->Q;
+    const double bb_L = sq(bx_L) + sq(by_L) + sq(bz_L);
+    const double bb_R = sq(bx_R) + sq(by_R) + sq(bz_R);
+    const double uu_L = (sq(L->u) + sq(L->v) + sq(L->w)) / 2.0;
+    const double uu_R = (sq(R->u) + sq(R->v) + sq(R->w)) / 2.0;
+    /* fast magnetosonic speeds, :1734-1737 */
+    const double cfL = sqrt((gam * p_L + bb_L + sqrt(sq(gam * p_L + bb_L) - 4.0 * gam * p_L * sq(bn1))) / (2.0 * ro_L));
+    const double cfR = sqrt((gam * p_R + bb_R + sqrt(sq(gam * p_R + bb_R) - 4.0 * gam * p_R * sq(bn2))) / (2.0 * ro_R));
+    const double SL = mn(u1, u2) - mx(cfL, cfR);
+    const double SR = mx(u1, u2) + mx(cfL, cfR);
+    const double pTL = p_L + bb_L / 2.0;
+    const double pTR = p_R + bb_R / 2.0;
+    const double suR = SR - u2, suL = SL - u1;
+    const double SM = (suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL) / (suR * ro_R - suL * ro_L);
+    const double PTT = (suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) /
+                       (suR * ro_R - suL * ro_L);
+    const double UU = mx(fabs(SL), fabs(SR));
+
+    const double e1 = p_L / g1 + ro_L * uu_L + bb_L / 2.0;
+    const double e2 = p_R / g1 + ro_R * uu_R + bb_R / 2.0;
+    double FL[9], FR[9], UL[9], UR[9];
+    FL[0] = ro_L * u1;
+    FL[1] = ro_L * u1 * u1 + pTL - sq(bn1);
+    FL[2] = ro_L * u1 * v1 - bn1 * bt1;
+    FL[3] = ro_L * u1 * w1 - bn1 * bm1;
+    FL[4] = (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1);
+    FL[5] = 0.0;
+    FL[6] = u1 * bt1 - v1 * bn1;
+    FL[7] = u1 * bm1 - w1 * bn1;
+    FL[8] = Q_L * u1;
+    FR[0] = ro_R * u2;
+    FR[1] = ro_R * u2 * u2 + pTR - sq(bn2);
+    FR[2] = ro_R * u2 * v2 - bn2 * bt2;
+    FR[3] = ro_R * u2 * w2 - bn2 * bm2;
+    FR[4] = (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2);
+    FR[5] = 0.0;
+    FR[6] = u2 * bt2 - v2 * bn2;
+    FR[7] = u2 * bm2 - w2 * bn2;
+    FR[8] = Q_R * u2;
+    UL[0] = ro_L; UL[1] = ro_L * u1; UL[2] = ro_L * v1; UL[3] = ro_L * w1; UL[4] = e1;
+    UL[5] = bn1; UL[6] = bt1; UL[7] = bm1; UL[8] = Q_L;
+    UR[0] = ro_R; UR[1] = ro_R * u2; UR[2] = ro_R * v2; UR[3] = ro_R * w2; UR[4] = e2;
+    UR[5] = bn2; UR[6] = bt2; UR[7] = bm2; UR[8] = Q_R;
+
+    const double bn = (SR * UR[5] - SL * UL[5] + FL[5] - FR[5]) / (SR - SL);
+    const double bbn = bn * bn;
+    const double ro_LL = ro_L * (SL - u1) / (SL - SM);
+    const double ro_RR = ro_R * (SR - u2) / (SR - SM);
+    const double Q_LL = Q_L * (SL - u1) / (SL - SM);
+    const double Q_RR = Q_R * (SR - u2) / (SR - SM);
+
+    /* :1983-2017 */
+    const double ttL = ro_L * suL * (SL - SM) - bbn;
+    const double ttR = ro_R * suR * (SR - SM) - bbn;
+    double vLL, wLL, vRR, wRR, btt1, bmm1, btt2, bmm2;
+    if (fabs(ttL) >= 0.000001) {
+        vLL = v1 - bn * bt1 * (SM - u1) / ttL;
+        wLL = w1 - bn * bm1 * (SM - u1) / ttL;
+        btt1 = bt1 * (ro_L * suL * suL - bbn) / ttL;
+        bmm1 = bm1 * (ro_L * suL * suL - bbn) / ttL;
+    } else {
+        vLL = v1; wLL = w1; btt1 = 0.0; bmm1 = 0.0;
+    }
+    if (fabs(ttR) >= 0.000001) {
+        vRR = v2 - bn * bt2 * (SM - u2) / ttR;
+        wRR = w2 - bn * bm2 * (SM - u2) / ttR;
+        btt2 = bt2 * (ro_R * suR * suR - bbn) / ttR;
+        bmm2 = bm2 * (ro_R * suR * suR - bbn) / ttR;
+    } else {
+        vRR = v2; wRR = w2; btt2 = 0.0; bmm2 = 0.0;
+    }
+    /* :2019-2024 */
+    const double eLL = (e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) / (SL - SM);
+    const double eRR = (e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) / (SR - SM);
+    const double sqrtroLL = sqrt(ro_LL), sqrtroRR = sqrt(ro_RR);
+    double SLL = SM - fabs(bn) / sqrtroLL;
+    double SRR = SM + fabs(bn) / sqrtroRR;
+    double idbn;
+    if (fabs(bn) > 0.001) {
+        idbn = 1.0 * sgn_ref(bn);
+    } else {
+        idbn = 0.0; SLL = SM; SRR = SM;
+    }
+    const double vLLL = (sqrtroLL * vLL + sqrtroRR * vRR + idbn * (btt2 - btt1)) / (sqrtroLL + sqrtroRR);
+    const double wLLL = (sqrtroLL * wLL + sqrtroRR * wRR + idbn * (bmm2 - bmm1)) / (sqrtroLL + sqrtroRR);
+    const double bttt = (sqrtroLL * btt2 + sqrtroRR * btt1 + idbn * sqrtroLL * sqrtroRR * (vRR - vLL)) / (sqrtroLL + sqrtroRR);
+    const double bmmm = (sqrtroLL * bmm2 + sqrtroRR * bmm1 + idbn * sqrtroLL * sqrtroRR * (wRR - wLL)) / (sqrtroLL + sqrtroRR);
+    const double eLLL = eLL - idbn * sqrtroLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+    const double eRRR = eRR + idbn * sqrtroRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+    const double ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
+    const double URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+    const double ULLL[9] = { ro_LL, ro_LL * SM, ro_LL * vLLL, ro_LL * wLLL, eLLL, bn, bttt, bmmm, Q_LL };
+    const double URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
+
+    double PO[9];
+    double bnW = 0.0;   /* normal field of the sampled region, for the -W*U term */
+    /* regions of the fan at xi = W, :2106-2156 */
+    if (SL >= W) {
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] - W * UL[i];
+        bnW = UL[5];
+    } else if (SL < W && SLL >= W) {
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
+        bnW = bn;
+    } else if (SLL <= W && SM >= W) {
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
+        bnW = bn;
+    } else if (SM < W && SRR > W) {
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
+        bnW = bn;
+    } else if (SR > W && SRR <= W) {
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
+        bnW = bn;
+    } else {
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] - W * UR[i];
+        bnW = UR[5];
+    }
+    /* normal-field flux replaced by the jump times the fastest speed, :2161-2163 */
+    const double SN = mx(fabs(SL), fabs(SR));
+    PO[5] = -SN * (bn2 - bn1) - W * bnW;
+
+    P[0] = PO[0];
+    P[1] = n1 * PO[1] + t1 * PO[2] + m1 * PO[3];
+    P[2] = n2 * PO[1] + t2 * PO[2] + m2 * PO[3];
+    P[3] = n3 * PO[1] + t3 * PO[2] + m3 * PO[3];
+    P[4] = PO[4];
+    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6] + m1 * PO[7]);
+    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6] + m2 * PO[7]);
+    P[7] = spi4 * (n3 * PO[5] + t3 * PO[6] + m3 * PO[7]);
+    P[8] = PO[8];
+    return UU;
+}
+
+/* ------------------------------------------------------------------ MHD sweep --- */
+/* The MHD extension of the cell loop of Go_5_komponent__MK2 (Setka.cpp:10148-10470):
+ * same gather over Cell::Grans, same boundary states and minmod reconstruction
+ * (extended to the 9 primitives), HLLD faces, and the r-weighted ALE update with the
+ * geometric sources of axisymmetric MHD with swirl, per unit planar area:
+ *   r-momentum   + (ro w^2 + p + B^2/8pi - Bphi^2/4pi)   (hoop stress; gas: + p)
+ *   phi-momentum - (ro v w - By Bphi/4pi)
+ *   Bphi         + (v Bphi - w By)
+ * Variables: 0 ro, 1 p, 2 u, 3 v, 4 w, 5 bx, 6 by, 7 bz (= Bphi), 8 Q. */
+
+#define MV 9
+static const int mirror9[MV] = { 0, 0, 0, 1, 1, 0, 1, 1, 0 };   /* odd across the axis */
+
+typedef struct msw { const ora_mesh* m; const gel_params* prm; const double* now; int nc;
+                     const double* cx; const double* cy; } msw;
+
+static void mload(const msw* S, int c, double* q) { for (int k = 0; k < MV; k++) q[k] = S->now[(size_t)k * S->nc + c]; }
+
+static void mget_par(const msw* S, int M, int slot, double fx, double fy, double* g)
+{
+    const gel_params* p = S->prm;
+    const int t = S->m->ftype[slot];
+    mload(S, M, g);
+    if (t == GEL_F_USUALY) { mload(S, S->m->fnbr[slot], g); return; }
+    if (t == GEL_F_EXTERN) { if (g[2] >= 0.0) g[2] = p->extern_u; return; }
+    if (t == GEL_F_AXIS) { for (int k = 0; k < MV; k++) if (mirror9[k]) g[k] = -g[k]; return; }
+    if (t == GEL_F_INNER_SPHERE) {
+        const double d = sqrt(fx * fx + fy * fy);
+        const double ro = 1.0 / (p->in_rho_div * d * d);
+        g[0] = ro; g[1] = ro * p->in_p_a * p->in_p_a / p->in_p_den;
+        g[2] = p->in_u * fx / d; g[3] = p->in_u * fy / d; g[4] = 0.0;
+        g[5] = 0.0; g[6] = 0.0; g[7] = 0.0; g[8] = ro * p->in_q;
+        return;
+    }
+    if (t == GEL_F_INPUT) {
+        g[0] = p->inf_rho; g[1] = p->inf_p; g[2] = p->inf_u; g[3] = p->inf_v; g[4] = p->inf_w;
+        g[5] = p->inf_bx; g[6] = p->inf_by; g[7] = p->inf_bz; g[8] = p->inf_q;
+    }
+}
+
+static void mtvd_usualy(const msw* S, int M, int N, int D, double fx, double fy, double* r, int* bad)
+{
+    double pm[MV], pn[MV];
+    mload(S, M, pm);
+    mload(S, N, pn);
+    if (D < 0) {
+        for (int k = 0; k < MV; k++) r[k] = pm[k];
+        if (cell_has_axis(S->m, M)) {
+            const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+            const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+            const double d1 = dist2d(fx, fy, S->cx[M], -S->cy[M]);
+            for (int k = 0; k < MV; k++)
+                if (mirror9[k]) r[k] = ora_linear3(-d1, -pm[k], -d2, pm[k], d3, pn[k], 0.0);
+        }
+        return;
+    }
+    double pd[MV];
+    mload(S, D, pd);
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+    const double d1 = dist2d(fx, fy, S->cx[D], S->cy[D]);
+    for (int k = 0; k < MV; k++) r[k] = ora_linear3(-d1, pd[k], -d2, pm[k], d3, pn[k], 0.0);
+    if (r[0] <= 0.0 || r[1] <= 0.0) *bad = 1;
+}
+
+static void mtvd_axis(const msw* S, int M, int D, double fx, double fy, double* r)
+{
+    double pm[MV], pd[MV];
+    mload(S, M, pm);
+    mload(S, D, pd);
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double d1 = dist2d(fx, fy, S->cx[D], S->cy[D]);
+    for (int k = 0; k < MV; k++) r[k] = pm[k];
+    for (int k = 0; k < MV; k++)
+        if (mirror9[k]) r[k] = ora_linear3(-d1, pd[k], -d2, pm[k], d2, -pm[k], 0.0);
+}
+
+static ora_mhd as_mhd(const double* q)
+{
+    ora_mhd s = { q[0], q[1], q[2], q[3], q[4], q[5], q[6], q[7], q[8] };
+    return s;
+}
+
+int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, double* next,
+                 const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
+                 long long* n_p_clamp, int nthreads)
+{
+    const int nc = m->n_cells;
+    const double gam = prm->gamma;
+    const double pi4 = 4.0 * PI_, pi8 = 8.0 * PI_;
+    for (int c = 0; c < nc; c++)
+        if (m->cell_type[c] == GEL_C_1) return GEL_E_UNSUPPORTED;
+    double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
+    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    msw S = { m, prm, now, nc, cx, cy };
+    double Tmin = 100000000;
+    long long nrc = 0, npc = 0;
+    int bad_any = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
+#endif
+    for (int c = 0; c < nc; c++) {
+        const int* q = m->contour + 4 * c;
+        const double x = cx[c], y = cy[c];
+        const double vs1 = ora_cell_area(m->x, m->y, q);
+        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double radius = sqrt(sq(x) + sq(y));
+        if (radius < prm->r_skip) continue;
+        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
+        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        double par1[MV];
+        mload(&S, c, par1);
+        double Pot[MV] = { 0 };
+        double time = 10000000;
+        int bad = 0;
+        for (int k = 0; k < m->nface[c]; k++) {
+            const int s = c * ORA_MAXF + k;
+            const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
+            const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+            const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
+            const double fx2 = (m->x2[A] + m->x2[B]) / 2.0, fy2 = (m->y2[A] + m->y2[B]) / 2.0;
+            const double dist = sqrt(sq(x - fx) + sq(y - fy));
+            double n1, n2;
+            {
+                double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
+                const double nn = sqrt(sq(t1) + sq(t2));
+                t1 = t1 / nn; t2 = t2 / nn;
+                n1 = t2; n2 = -t1;
+            }
+            const double W = ((fx2 - fx) * n1 + (fy2 - fy) * n2) / dt_prev;
+            const double dis = sqrt(sq(fx) + sq(fy));
+            double p11[MV], p2[MV];
+            for (int v = 0; v < MV; v++) p11[v] = par1[v];
+            mget_par(&S, c, s, fx, fy, p2);
+            if (t == GEL_F_USUALY) {
+                const int N = m->fnbr[s];
+                mtvd_usualy(&S, c, N, m->fdown[s], fx, fy, p11, &bad);
+                mtvd_usualy(&S, N, c, m->fup[s], fx, fy, p2, &bad);
+            } else if (t == GEL_F_INNER_SPHERE) {
+                p11[0] = p11[0] * sq(radius) / sq(dis);
+                p11[8] = p11[8] * sq(radius) / sq(dis);
+                p11[1] = p11[1] * pow(radius / dis, 2 * gam);
+            } else if (t == GEL_F_AXIS) {
+                mtvd_axis(&S, c, m->fdown[s], fx, fy, p11);
+                for (int v = 0; v < MV; v++) p2[v] = mirror9[v] ? -p11[v] : p11[v];
+            }
+            const ora_mhd Lm = as_mhd(p11), Rm = as_mhd(p2);
+            double P[MV];
+            const double UU = ora_hlld(&Lm, &Rm, W, n1, n2, gam, P);
+            time = mn(time, prm->cfl * dist / UU);
+            for (int v = 0; v < MV; v++) Pot[v] = Pot[v] + P[v] * S_;
+        }
+        if (bad) bad_any = 1;
+        if (time < Tmin) Tmin = time;
+
+        const double ro = par1[0], p = par1[1], u = par1[2], v = par1[3], w = par1[4];
+        const double bx = par1[5], by = par1[6], bz = par1[7], Q = par1[8];
+        double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        if (prm->source_mode == GEL_SRC_MK1 && src) {
+            double f[GEL_MK1_NFIELDS];
+            for (int k = 0; k < GEL_MK1_NFIELDS; k++) f[k] = src[(size_t)k * nc + c];
+            ora_source_mk1(prm, f, u, v, ro, p, &q1, &q2, &q3);
+        }
+        const double T1 = dt_prev;
+        const double Ar = 0.5 * (vs1 + vs2);
+        const double BB = bx * bx + by * by + bz * bz;
+        const double Sv = ro * w * w + p + BB / pi8 - bz * bz / pi4;
+        const double Sw = -(ro * v * w - by * bz / pi4);
+        const double Sb = v * bz - w * by;
+        const double E = p / (gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + BB / pi8;
+        if (m->cell_type[c] != GEL_C_CENTR) {
+            double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+            if (ro3 <= 0.0) { ro3 = 0.00001; nrc++; }
+            const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+            const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - Sv * Ar) / Volume2 - q2)) / ro3;
+            const double w3 = (ro * w * Volume / Volume2 - T1 * ((Pot[3] - Sw * Ar) / Volume2)) / ro3;
+            const double E3 = E * Volume / Volume2 - T1 * (Pot[4] / Volume2 - q3);
+            const double bx3 = bx * Volume / Volume2 - T1 * (Pot[5] / Volume2);
+            const double by3 = by * Volume / Volume2 - T1 * (Pot[6] / Volume2);
+            const double bz3 = bz * Volume / Volume2 - T1 * ((Pot[7] - Sb * Ar) / Volume2);
+            const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2);
+            double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / pi8) * (gam - 1);
+            if (p3 <= 0) { p3 = 0.000001; npc++; }
+            const double o[MV] = { ro3, p3, u3, v3, w3, bx3, by3, bz3, Q3 };
+            for (int k = 0; k < MV; k++) next[(size_t)k * nc + c] = o[k];
+        } else {
+            for (int k = 0; k < MV; k++) next[(size_t)k * nc + c] = now[(size_t)k * nc + c];
+        }
+    }
+    free(cx);
+    free(cy);
+    *dt_new = Tmin;
+    if (n_rho_clamp) *n_rho_clamp += nrc;
+    if (n_p_clamp) *n_p_clamp += npc;
+    return bad_any ? GEL_E_NUMERIC : GEL_OK;
+}

@@ -43,6 +43,13 @@ typedef struct ora_mesh {
 } ora_mesh;
 
 typedef struct ora_gas { double ro, p, u, v, Q; } ora_gas;
+/* MHD primitive state: velocity (u, v, w) and field (bx, by, bz) in (x, y, phi). */
+typedef struct ora_mhd { double ro, p, u, v, w, bx, by, bz, Q; } ora_mhd;
+
+/* HLLD of Solvers::HLLDQ_Korolkov metod 3 (Solvers.cpp:1642-2185) for any in-plane
+ * normal and face speed W.  P: mass, momentum x/y/phi, energy, B x/y/phi, tracer.
+ * Returns max(|SL|,|SR|). */
+double ora_hlld(const ora_mhd* L, const ora_mhd* R, double W, double n1, double n2, double gam, double P[9]);
 
 /* Setka::HLLC_2d_Korolkov_b_s (Setka.cpp:10484-10700). Returns cfl*rad/max|S|. */
 double ora_hllc2d(const ora_gas* L, const ora_gas* R, double W, double n1, double n2,
@@ -76,6 +83,12 @@ void ora_source_mk1(const gel_params* prm, const double* f, double u, double v,
 int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
                  double* next, const double* src, double dt_prev, double* dt_new,
                  long long* n_rho_clamp, long long* n_p_clamp, int nthreads);
+
+/* MHD form of the same cell loop: 9 x n_cells primitives (ro,p,u,v,w,bx,by,bz,Q),
+ * HLLD faces, axisymmetric MHD geometric sources.  C_1 cells are not supported. */
+int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, double* next,
+                 const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
+                 long long* n_p_clamp, int nthreads);
 
 #ifdef __cplusplus
 }

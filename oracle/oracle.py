@@ -29,6 +29,10 @@ class OraMesh(ctypes.Structure):
                 ("fa", _ip), ("fb", _ip), ("ftype", _ip), ("fnbr", _ip), ("fdown", _ip), ("fup", _ip)]
 
 
+class OraMhd(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ro", "p", "u", "v", "w", "bx", "by", "bz", "Q")]
+
+
 class OraGas(ctypes.Structure):
     _fields_ = [("ro", ctypes.c_double), ("p", ctypes.c_double), ("u", ctypes.c_double),
                 ("v", ctypes.c_double), ("Q", ctypes.c_double)]
@@ -67,10 +71,15 @@ def lib():
         L.ora_cell_volume_rotate.argtypes = [_dp, _dp, _ip, ctypes.c_double]
         L.ora_source_mk1.argtypes = [ctypes.POINTER(GelParams), _dp, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
-        L.ora_step_mk2.restype = ctypes.c_int
-        L.ora_step_mk2.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, _dp,
-                                   ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_longlong),
-                                   ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        for name in ("ora_step_mk2", "ora_step_mhd"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, _dp,
+                          ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_longlong),
+                          ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.ora_hlld.restype = ctypes.c_double
+        L.ora_hlld.argtypes = [ctypes.POINTER(OraMhd), ctypes.POINTER(OraMhd), ctypes.c_double, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, _dp]
         _lib = L
     return _lib
 
@@ -95,6 +104,15 @@ def hllc2d(L, R, W, n1, n2, rad, cfl, gamma, metod, nul_potok):
     return P, o[0].value, o[1].value, o[2].value, o[3].value, t
 
 
+def hlld(L, R, W, n1, n2, gamma=5.0 / 3.0):
+    """Point-wise HLLD. L, R = 9 primitives (ro, p, u, v, w, bx, by, bz, Q).
+    Returns (P[9] = mass, momentum x/y/phi, energy, B x/y/phi, tracer; max|S|)."""
+    P = np.zeros(9)
+    a, b = OraMhd(*map(float, L)), OraMhd(*map(float, R))
+    uu = lib().ora_hlld(ctypes.byref(a), ctypes.byref(b), W, n1, n2, gamma, _d(P))
+    return P, uu
+
+
 class OracleSweep:
     """Holds a general mesh and steps the restated Go_5_komponent__MK2 cell loop."""
 
@@ -106,6 +124,7 @@ class OracleSweep:
         self.x2 = np.ascontiguousarray(self.x if x2 is None else x2, np.float64)
         self.y2 = np.ascontiguousarray(self.y if y2 is None else y2, np.float64)
         self.nc = len(mesh.cell_type)
+        self.nv = 9 if prm.physics == 1 else 5
         self._keep = [np.ascontiguousarray(a, np.int32) for a in
                       (mesh.contour.ravel(), mesh.cell_type, mesh.nface, mesh.fa, mesh.fb, mesh.ftype,
                        mesh.fnbr, mesh.fdown, mesh.fup)]
@@ -132,12 +151,13 @@ class OracleSweep:
     def step(self, prim, nthreads=0, dt_prev=None):
         """One step from prim (5 x nc); returns (next, status).  Uses and updates the
         lagged time step unless dt_prev is given."""
-        now = np.ascontiguousarray(prim, np.float64).reshape(5, self.nc)
+        now = np.ascontiguousarray(prim, np.float64).reshape(self.nv, self.nc)
         nxt = now.copy()
         dt_new = ctypes.c_double()
         m = self._mesh()
         src = _d(self.src) if self.src is not None else None
-        st = lib().ora_step_mk2(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src,
+        fn = lib().ora_step_mhd if self.nv == 9 else lib().ora_step_mk2
+        st = fn(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src,
                                 self.dt if dt_prev is None else dt_prev, ctypes.byref(dt_new),
                                 ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
         if dt_prev is None:
@@ -146,7 +166,7 @@ class OracleSweep:
         return nxt, st
 
     def run(self, prim, steps, nthreads=0):
-        cur = np.ascontiguousarray(prim, np.float64).reshape(5, self.nc)
+        cur = np.ascontiguousarray(prim, np.float64).reshape(self.nv, self.nc)
         for _ in range(steps):
             cur, st = self.step(cur, nthreads)
             if st != 0:

@@ -118,6 +118,31 @@ static int mode_cell(const char* in, const char* out)
     return 0;
 }
 
+// ---- point-wise: Solvers::HLLDQ_Korolkov (MHD, W = 0) ------------------------------
+// in : int32 n; float64 [n][22] = roL QL pL v1L v2L v3L BxL ByL BzL roR QR pR v1R v2R v3R
+//                                 BxR ByR BzR n1 n2 n3 metod
+// out: float64 [n][10] = P0..P7 PQ time
+static int mode_hlld(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t n; rd(f, &n, 1);
+    std::vector<double> a((size_t)n * 22); rd(f, a.data(), a.size()); std::fclose(f);
+    Solvers S;
+    std::vector<double> o((size_t)n * 10);
+    for (int k = 0; k < n; k++) {
+        const double* r = &a[(size_t)k * 22];
+        double P[8] = {0, 0, 0, 0, 0, 0, 0, 0}, PQ = 0.0;
+        double t = S.HLLDQ_Korolkov(r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10], r[11],
+                                    r[12], r[13], r[14], r[15], r[16], r[17], P, PQ, r[18], r[19], r[20], 1.0,
+                                    (int)r[21]);
+        double* q = &o[(size_t)k * 10];
+        for (int i = 0; i < 8; i++) q[i] = P[i];
+        q[8] = PQ; q[9] = t;
+    }
+    f = xopen(out, "wb"); wr(f, o.data(), o.size()); std::fclose(f);
+    return 0;
+}
+
 // ---- whole sweep -----------------------------------------------------------------
 // in : int32 ni nj steps moving has_src
 //      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
@@ -312,11 +337,12 @@ static int mode_sweep(const char* in, const char* out)
 
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "sweep")) return mode_sweep(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "hlld")) return mode_hlld(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

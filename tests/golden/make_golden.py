@@ -104,6 +104,34 @@ def gen_cell(rng, tmp, n=300):
     np.savez_compressed(os.path.join(OUT, "cell_vectors.npz"), inp=inp, out=res)
 
 
+def gen_hlld(rng, tmp, n=1200):
+    """Solvers::HLLDQ_Korolkov, metod 3, on the normals it supports (axis-aligned,
+    Solvers.cpp:1664-1697) and a face at rest (:1643)."""
+    roL, roR = rng.uniform(0.05, 3.0, (2, n))
+    pL, pR = rng.uniform(0.01, 3.0, (2, n))
+    QL, QR = rng.uniform(0.0, 5.0, (2, n))
+    drift = np.where(rng.random(n) < 0.25, rng.normal(0, 5.0, (3, n)), 0.0)
+    vL = rng.normal(0, 1.0, (3, n)) + drift
+    vR = rng.normal(0, 1.0, (3, n)) + drift
+    bscale = np.where(rng.random(n) < 0.2, 0.0, rng.uniform(0.1, 4.0, n))   # some B = 0 faces
+    BL = rng.normal(0, 1.0, (3, n)) * bscale
+    BR = rng.normal(0, 1.0, (3, n)) * bscale
+    # a common normal field on half of the faces (div B = 0 across the face)
+    k = np.arange(n)
+    nsel = k % 4
+    nrm = np.array([[1.0, 0, 0], [-1.0, 0, 0], [0, 1.0, 0], [0, -1.0, 0]])[nsel]
+    same = rng.random(n) < 0.5
+    for i in range(n):
+        if same[i]:
+            a = int(np.argmax(np.abs(nrm[i])))
+            BR[a, i] = BL[a, i]
+    inp = np.concatenate([roL[:, None], QL[:, None], pL[:, None], vL.T, BL.T,
+                          roR[:, None], QR[:, None], pR[:, None], vR.T, BR.T, nrm, np.full((n, 1), 3.0)], axis=1)
+    out = harness("hlld", np.int32(n).tobytes() + np.ascontiguousarray(inp).tobytes(), tmp)
+    res = np.frombuffer(out, np.float64).reshape(n, 10)
+    np.savez_compressed(os.path.join(OUT, "hlld_vectors.npz"), inp=inp, out=res)
+
+
 def harness_grid(ni=16, nj=10, n_c1=3):
     g = polar_grid(ni, nj, 4.0, 12.0, spacing="log")
     ct = np.full(ni * nj, GEL_C_2, np.int8)
@@ -200,6 +228,7 @@ def main():
         gen_hllc(rng, tmp)
         gen_helpers(rng, tmp)
         gen_cell(rng, tmp)
+        gen_hlld(np.random.default_rng(20261204), tmp)
         gen_sweeps(rng, tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):

@@ -1,0 +1,129 @@
+"""The MHD part of the oracle: HLLD against the reference's HLLDQ_Korolkov vectors and
+against closed forms, and the MHD sweep against the gas sweep and invariants."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from gelios_2d_amd.configs import heliosphere
+from gelios_2d_amd.params import GEL_FLUX_HLLC, GEL_PHYS_MHD
+
+
+def exact_flux(q, W, n1, n2, gamma=5.0 / 3.0):
+    """Physical MHD flux along n minus W*U (Gaussian B), written out directly."""
+    ro, p, u, v, w, bx, by, bz, Q = q
+    pi4 = 4 * np.pi
+    un = u * n1 + v * n2
+    bn = bx * n1 + by * n2
+    B2 = bx * bx + by * by + bz * bz
+    pT = p + B2 / (2 * pi4)
+    E = p / (gamma - 1) + 0.5 * ro * (u * u + v * v + w * w) + B2 / (2 * pi4)
+    ub = u * bx + v * by + w * bz
+    F = np.array([ro * un, ro * un * u + pT * n1 - bn * bx / pi4, ro * un * v + pT * n2 - bn * by / pi4,
+                  ro * un * w - bn * bz / pi4, (E + pT) * un - bn * ub / pi4,
+                  0.0, un * by - v * bn, un * bz - w * bn, Q * un])
+    F[5] = un * bx - u * bn
+    U = np.array([ro, ro * u, ro * v, ro * w, E, bx, by, bz, Q])
+    return F - W * U
+
+
+def test_hlld_matches_reference_vectors(oracle_lib):
+    d = golden("hlld_vectors.npz")
+    inp, out = d["inp"], d["out"]
+    for r, o in zip(inp, out):
+        L = [r[0], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[1]]
+        R = [r[9], r[11], r[12], r[13], r[14], r[15], r[16], r[17], r[10]]
+        P, uu = oracle_lib.hlld(L, R, 0.0, r[18], r[19])
+        mine = np.array([P[0], P[1], P[2], P[3], P[5], P[6], P[7], P[4], P[8], 0.1 / uu])
+        scale = max(np.abs(o[:9]).max(), 1e-12)
+        assert np.max(np.abs(mine - o)) / scale <= 4e-15
+
+
+def test_hlld_without_field_is_hllc(oracle_lib):
+    rng = np.random.default_rng(5)
+    for _ in range(400):
+        roL, roR, pL, pR = rng.uniform(0.1, 2.0, 4)
+        uL, vL, uR, vR = rng.normal(0, 1.0, 4)
+        QL, QR = rng.uniform(0, 2, 2)
+        a = rng.uniform(0, 2 * np.pi)
+        n1, n2 = np.cos(a), np.sin(a)
+        W = rng.normal(0, 0.7)
+        P, uu = oracle_lib.hlld([roL, pL, uL, vL, 0, 0, 0, 0, QL], [roR, pR, uR, vR, 0, 0, 0, 0, QR], W, n1, n2)
+        Pg, PQ, Vl, Vc, Vp, t = oracle_lib.hllc2d([roL, QL, pL, uL, vL], [roR, QR, pR, uR, vR], W, n1, n2,
+                                                  1.0, 1.0, 5.0 / 3.0, 1, 0)
+        gas = np.array([Pg[0], Pg[1], Pg[2], 0.0, Pg[3], 0, 0, 0, PQ])
+        assert np.allclose(P, gas, rtol=1e-12, atol=1e-12 * np.abs(gas).max())
+        assert abs(1.0 / uu - t) <= 1e-14 * t
+
+
+def test_hlld_consistency_and_mirror(oracle_lib):
+    """F(U, U) is the physical flux minus W U; F(L, R, n) = -F(R, L, -n)."""
+    rng = np.random.default_rng(6)
+    for _ in range(300):
+        q = [rng.uniform(0.2, 2), rng.uniform(0.2, 2), *rng.normal(0, 1, 3), *rng.normal(0, 1.5, 3),
+             rng.uniform(0, 1)]
+        a = rng.uniform(0, 2 * np.pi)
+        n1, n2 = np.cos(a), np.sin(a)
+        W = rng.normal(0, 0.5)
+        P, _ = oracle_lib.hlld(q, q, W, n1, n2)
+        F = exact_flux(q, W, n1, n2)
+        # field: tangential part from the physical flux; the normal part is the
+        # jump term -max|S| (bn_R - bn_L) = 0 for equal states, minus W*bn
+        bn = q[5] * n1 + q[6] * n2
+        Fb_t = (F[5] * (-n2) + F[6] * n1)    # component along t = (-n2, n1)
+        Fb = np.array([-W * bn * n1 + Fb_t * (-n2), -W * bn * n2 + Fb_t * n1, F[7]])
+        got = np.concatenate([P[:5], P[8:9]])
+        want = np.concatenate([F[:5], F[8:9]])
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12)
+        assert np.allclose(P[5:8], Fb, rtol=1e-11, atol=1e-12)
+        L = q
+        R = [rng.uniform(0.2, 2), rng.uniform(0.2, 2), *rng.normal(0, 1, 3), *rng.normal(0, 1.5, 3),
+             rng.uniform(0, 1)]
+        P1, u1 = oracle_lib.hlld(L, R, W, n1, n2)
+        P2, u2 = oracle_lib.hlld(R, L, -W, -n1, -n2)
+        assert np.allclose(P1, -P2, rtol=1e-11, atol=1e-11 * np.abs(P1).max())
+        assert abs(u1 - u2) <= 1e-14 * u1
+
+
+def _mhd_from_gas(w):
+    prm = w.params.copy()
+    prm.physics = GEL_PHYS_MHD
+    ro, p, u, v, Q = w.prim0
+    z = np.zeros_like(ro)
+    return prm, np.stack([ro, p, u, v, z, z, z, z, Q])
+
+
+def test_mhd_sweep_without_field_is_gas_sweep(oracle_lib):
+    w = heliosphere(30, 24, 40, seed=9, flux=GEL_FLUX_HLLC)
+    gas = oracle_lib.OracleSweep.from_polar(w.grid, w.params).run(w.prim0, w.steps)
+    prm, q0 = _mhd_from_gas(w)
+    mhd = oracle_lib.OracleSweep.from_polar(w.grid, prm).run(q0, w.steps)
+    sel = [0, 1, 2, 3, 8]
+    scale = np.abs(gas).max(axis=1)
+    # pressure comes out of total energy minus kinetic energy (Mach 10 wind): its
+    # rounding scale is the kinetic energy density, not p
+    scale[1] = np.max(gas[0] * (gas[2] ** 2 + gas[3] ** 2))
+    assert np.all(np.abs(mhd[sel] - gas).max(axis=1) <= 1e-12 * scale)
+    assert np.all(mhd[[4, 5, 6, 7]] == 0.0)
+
+
+def test_mhd_free_stream_with_axial_field(oracle_lib):
+    """A uniform state at rest in a uniform axial field is steady: the hoop stress
+    p + B^2/8pi balances the r-weighted normal stress of the faces."""
+    from gelios_2d_amd.grid import polar_grid
+    from gelios_2d_amd.params import heliosphere_params
+    ni, nj, steps = 64, 30, 8
+    g = polar_grid(ni, nj, 1.0, 50.0)
+    prm = heliosphere_params()
+    prm.physics = GEL_PHYS_MHD
+    prm.inf_rho, prm.inf_p, prm.inf_u, prm.inf_v, prm.inf_q = 1.0, 1.0, 0.0, 0.0, 0.0
+    prm.inf_bx = 0.7
+    q = np.zeros((9, ni * nj))
+    q[0] = 1.0
+    q[1] = 1.0
+    q[5] = 0.7
+    out = oracle_lib.OracleSweep.from_polar(g, prm).run(q, steps).reshape(9, ni, nj)
+    far = out[:, 2 * steps + 4: ni - 2 * steps - 4, :]
+    assert np.max(np.abs(far[0] - 1.0)) < 1e-12
+    assert np.max(np.abs(far[1] - 1.0)) < 1e-12
+    assert np.max(np.abs(far[5] - 0.7)) < 1e-12
+    assert np.max(np.abs(far[[2, 3, 4, 6, 7]])) < 1e-12
