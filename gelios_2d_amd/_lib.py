@@ -51,6 +51,8 @@ SIGNATURES = {
     "gel_flux_hllc2d_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                              _dp, _dp]),
+    "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+    "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
     "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gel_last_error": (ctypes.c_char_p, []),
 }

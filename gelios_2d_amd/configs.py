@@ -9,12 +9,13 @@ carries a seeded +-1% density perturbation.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
 
 from .grid import PolarGrid, polar_grid
-from .params import GA, GEL_FLUX_HLLC, GelParams, heliosphere_params
+from .params import GA, GEL_FLUX_HLLC, GEL_PHYS_MHD, GEL_PREC_FP64, GelParams, heliosphere_params
 
 
 @dataclass
@@ -24,6 +25,7 @@ class Workload:
     params: GelParams
     prim0: np.ndarray
     steps: int
+    motion: tuple = None          # (amp, period, phase) of the prescribed node motion
 
 
 WIND_RHO, WIND_U, WIND_P = 1.0, 10.0, 0.01
@@ -67,3 +69,47 @@ def large(ni: int = 4096, nj: int = 2048, steps: int = 20) -> Workload:
     """Same physics on a grid whose state (5 x 8 B x 2 buffers per cell) is far
     larger than the 256 MB Infinity Cache, for the HBM roofline."""
     return heliosphere(ni, nj, steps, seed=7, name=f"hllc-{ni}x{nj}")
+
+
+B_ISM, B_ANGLE_DEG = 0.05, 60.0
+MOTION_AMP, MOTION_PERIOD = 1e-3, 50
+
+
+def motion_phase(seed: int) -> float:
+    """Global phase of the prescribed motion, fixed by the seed."""
+    return 2.0 * math.pi * ((seed * 0.6180339887498949) % 1.0)
+
+
+def motion_scales(amp: float, period: int, phase: float) -> list:
+    """s_k = 1 + amp sin(2 pi k/period + phase): the same expression, in the same
+    order, as the library's motion table (glibc sin on the host)."""
+    return [1.0 + amp * math.sin(2.0 * math.pi * k / period + phase) for k in range(period)]
+
+
+def mhd_heliosphere(ni: int, nj: int, steps: int, seed: int, moving: bool = True, cfl: float = 0.4,
+                    precision: int = GEL_PREC_FP64, b_ism: float = B_ISM, b_angle_deg: float = B_ANGLE_DEG,
+                    name: str = "mhd-heliosphere") -> Workload:
+    """Ideal MHD (HLLD), 9 variables, same geometry and flows as the gas workload;
+    the interstellar inflow carries a field |B| = b_ism at b_angle_deg to the axis
+    (in the meridional plane); the grid breathes radially by 1e-3 r with period 50
+    steps (global phase fixed by the seed)."""
+    w = heliosphere(ni, nj, steps, seed, cfl=cfl, name=name)
+    prm = w.params
+    prm.physics = GEL_PHYS_MHD
+    prm.precision = precision
+    a = math.radians(b_angle_deg)
+    prm.inf_w, prm.inf_bx, prm.inf_by, prm.inf_bz = 0.0, b_ism * math.cos(a), b_ism * math.sin(a), 0.0
+    ro, p, u, v, Q = w.prim0
+    cx, cy = w.grid.cell_centers()
+    r = np.hypot(cx, cy)
+    s = 0.5 * (1.0 - np.tanh((r - R_BLEND) / D_BLEND))
+    z = np.zeros_like(ro)
+    q = np.stack([ro, p, u, v, z, (1 - s) * prm.inf_bx, (1 - s) * prm.inf_by, z, Q])
+    motion = (MOTION_AMP, MOTION_PERIOD, motion_phase(seed)) if moving else None
+    return Workload(name, w.grid, prm, np.ascontiguousarray(q), steps, motion)
+
+
+def config2(precision: int = GEL_PREC_FP64) -> Workload:
+    """HLLD MHD, 800 x 400 cells, moving grid, 1000 steps, CFL 0.4, seed 2."""
+    tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
+    return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"cfg2-hlld-{tag}-800x400-moving")

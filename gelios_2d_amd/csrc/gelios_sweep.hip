@@ -35,16 +35,30 @@ constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kOut = kWave - 2 * kHalo;   // output columns per wavefront
 
+// State vectors.  GAS: (ro, p, u, v, Q), the fields of Parametr (Cell.h:21-25).
+// MHD: (ro, p, u, v, w, bx, by, bz, Q).  odd(k): the primitive changes sign across the
+// symmetry axis (the reference mirrors v, Gran.cpp:225-233; with swirl and field the
+// azimuthal velocity and the radial and azimuthal field change sign as well).
+struct GasPhys {
+    static constexpr int NV = 5;
+    __host__ __device__ static constexpr bool odd(int k) { return k == 3; }
+};
+struct MhdPhys {
+    static constexpr int NV = 9;
+    __host__ __device__ static constexpr bool odd(int k) { return k == 3 || k == 4 || k == 6 || k == 7; }
+};
+
 struct SweepArgs {
     int ni, nj;
     int rows;                     // rows per wavefront chunk
     int dt_rd, dt_wr, dt_clr;     // slots of the rotating time-step buffer
-    const double* __restrict__ x;
+    double s_old, s_new;          // node scale factors before / after the step (moving)
+    const double* __restrict__ x; // node positions before the step are x*s_old, y*s_old
     const double* __restrict__ y;
-    const double* __restrict__ x2;
+    const double* __restrict__ x2;  // after the step: x2*s_new, y2*s_new
     const double* __restrict__ y2;
-    const double* __restrict__ now;
-    double* __restrict__ next;
+    const void* __restrict__ now; // [NV][ncells] of the storage type
+    void* __restrict__ next;
     const int8_t* __restrict__ ctype;
     const int8_t* __restrict__ otype;
     const double* __restrict__ src;
@@ -53,70 +67,68 @@ struct SweepArgs {
     gel_params prm;
 };
 
-struct Cell {          // one cell of the stencil as seen by a lane
-    Gas<double> s;
+template <class Ph, class TS>
+struct CellT {          // one cell of the stencil as seen by a lane
+    TS s[Ph::NV];
     double cx, cy;
     int type;
-    int ok;            // exists (inside the grid)
+    int ok;             // exists (inside the grid)
 };
 
-struct Nodes {         // nodes (r, j) and (r, j+1) of a lane
+struct Nodes {          // nodes (r, j) and (r, j+1) of a lane
     double xa, ya, xb, yb;
 };
 
-__device__ __forceinline__ double shfl_dn(double v, int d) { return __shfl_down(v, d, kWave); }
-__device__ __forceinline__ double shfl_up(double v, int d) { return __shfl_up(v, d, kWave); }
-__device__ __forceinline__ int shfl_dn(int v, int d) { return __shfl_down(v, d, kWave); }
-__device__ __forceinline__ int shfl_up(int v, int d) { return __shfl_up(v, d, kWave); }
+template <class T> __device__ __forceinline__ T shfl_dn(T v, int d) { return __shfl_down(v, d, kWave); }
+template <class T> __device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, d, kWave); }
 
-__device__ __forceinline__ Cell cell_dn(const Cell& c, int d)
+template <bool DOWN, class Ph, class TS>
+__device__ __forceinline__ CellT<Ph, TS> cell_shfl(const CellT<Ph, TS>& c, int d)
 {
-    Cell r;
-    r.s.ro = shfl_dn(c.s.ro, d); r.s.p = shfl_dn(c.s.p, d); r.s.u = shfl_dn(c.s.u, d);
-    r.s.v = shfl_dn(c.s.v, d); r.s.Q = shfl_dn(c.s.Q, d);
-    r.cx = shfl_dn(c.cx, d); r.cy = shfl_dn(c.cy, d);
-    r.type = shfl_dn(c.type, d); r.ok = shfl_dn(c.ok, d);
-    return r;
-}
-__device__ __forceinline__ Cell cell_up(const Cell& c, int d)
-{
-    Cell r;
-    r.s.ro = shfl_up(c.s.ro, d); r.s.p = shfl_up(c.s.p, d); r.s.u = shfl_up(c.s.u, d);
-    r.s.v = shfl_up(c.s.v, d); r.s.Q = shfl_up(c.s.Q, d);
-    r.cx = shfl_up(c.cx, d); r.cy = shfl_up(c.cy, d);
-    r.type = shfl_up(c.type, d); r.ok = shfl_up(c.ok, d);
+    CellT<Ph, TS> r;
+#pragma unroll
+    for (int k = 0; k < Ph::NV; k++) r.s[k] = DOWN ? shfl_dn(c.s[k], d) : shfl_up(c.s[k], d);
+    r.cx = DOWN ? shfl_dn(c.cx, d) : shfl_up(c.cx, d);
+    r.cy = DOWN ? shfl_dn(c.cy, d) : shfl_up(c.cy, d);
+    r.type = DOWN ? shfl_dn(c.type, d) : shfl_up(c.type, d);
+    r.ok = DOWN ? shfl_dn(c.ok, d) : shfl_up(c.ok, d);
     return r;
 }
 
+template <bool MOVING>
 __device__ __forceinline__ Nodes load_nodes(const SweepArgs& a, const double* __restrict__ X,
-                                            const double* __restrict__ Y, int r, int j, bool col)
+                                            const double* __restrict__ Y, double sc, int r, int j, bool col)
 {
     Nodes n{1.0, 1.0, 1.0, 1.0};
     if (col && r >= 0 && r <= a.ni) {
         const size_t k = (size_t)r * (a.nj + 1) + j;
         n.xa = X[k]; n.ya = Y[k]; n.xb = X[k + 1]; n.yb = Y[k + 1];
+        if (MOVING) { n.xa *= sc; n.ya *= sc; n.xb *= sc; n.yb *= sc; }
     }
     return n;
 }
 
-template <bool TYPES>
-__device__ __forceinline__ Cell load_cell(const SweepArgs& a, int r, int j, bool col,
-                                          const Nodes& lo, const Nodes& hi)
+template <class Ph, class TS, bool TYPES>
+__device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, int j, bool col,
+                                                   const Nodes& lo, const Nodes& hi)
 {
-    Cell c;
+    CellT<Ph, TS> c;
     c.ok = (col && r >= 0 && r < a.ni) ? 1 : 0;
     c.type = GEL_C_2;
     if (c.ok) {
         const size_t nc = (size_t)a.ni * a.nj;
         const size_t k = (size_t)r * a.nj + j;
-        c.s.ro = a.now[k]; c.s.p = a.now[nc + k]; c.s.u = a.now[2 * nc + k];
-        c.s.v = a.now[3 * nc + k]; c.s.Q = a.now[4 * nc + k];
+        const TS* __restrict__ now = (const TS*)a.now;
+#pragma unroll
+        for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)v * nc + k];
         if (TYPES) c.type = a.ctype[k];
         const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
         const double py[4] = { lo.ya, hi.ya, hi.yb, lo.yb };
         quad_center(px, py, c.cx, c.cy);
     } else {
-        c.s.ro = 1.0; c.s.p = 1.0; c.s.u = 0.0; c.s.v = 0.0; c.s.Q = 0.0;
+#pragma unroll
+        for (int v = 0; v < Ph::NV; v++) c.s[v] = TS(0);
+        c.s[0] = TS(1); c.s[1] = TS(1);
         c.cx = 1.0; c.cy = 1.0;
     }
     return c;
@@ -129,44 +141,45 @@ __device__ __forceinline__ double dist2d(double ax, double ay, double bx, double
 
 // Gran::Get_par_TVD, Usualy face (Gran.cpp:294-457): master M's value at face centre
 // (fx,fy) from the line (D, M, N).  D.ok == 0: mirrored ghost across the axis when M
-// touches it (:297-339), else M itself (:340-343).
-__device__ __forceinline__ Gas<double> tvd_usualy(const Cell& M, const Cell& N, const Cell& D, bool m_axis,
-                                                  double fx, double fy, int& bad)
+// touches it (:297-339), else M itself (:340-343).  Distances are FP64 geometry; the
+// limiter runs in the flux type TF.
+template <class Ph, class TS, class TF>
+__device__ __forceinline__ void tvd_usualy(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
+                                           bool m_axis, double fx, double fy, TF* r, int& bad)
 {
     if (!D.ok) {
-        Gas<double> r = M.s;
+#pragma unroll
+        for (int k = 0; k < Ph::NV; k++) r[k] = TF(M.s[k]);
         if (m_axis) {
-            const double d2 = dist2d(fx, fy, M.cx, M.cy);
-            const double d3 = dist2d(fx, fy, N.cx, N.cy);
-            const double d1 = dist2d(fx, fy, M.cx, -M.cy);
-            r.v = linear3(-d1, -M.s.v, -d2, M.s.v, d3, N.s.v, 0.0);
+            const TF d2 = TF(dist2d(fx, fy, M.cx, M.cy));
+            const TF d3 = TF(dist2d(fx, fy, N.cx, N.cy));
+            const TF d1 = TF(dist2d(fx, fy, M.cx, -M.cy));
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++)
+                if (Ph::odd(k)) r[k] = linear3<TF>(-d1, -TF(M.s[k]), -d2, TF(M.s[k]), d3, TF(N.s[k]), TF(0));
         }
-        return r;
+        return;
     }
-    const double d2 = dist2d(fx, fy, M.cx, M.cy);
-    const double d3 = dist2d(fx, fy, N.cx, N.cy);
-    const double d1 = dist2d(fx, fy, D.cx, D.cy);
-    Gas<double> r;
-    r.ro = linear3(-d1, D.s.ro, -d2, M.s.ro, d3, N.s.ro, 0.0);
-    r.p = linear3(-d1, D.s.p, -d2, M.s.p, d3, N.s.p, 0.0);
-    r.u = linear3(-d1, D.s.u, -d2, M.s.u, d3, N.s.u, 0.0);
-    r.v = linear3(-d1, D.s.v, -d2, M.s.v, d3, N.s.v, 0.0);
-    r.Q = linear3(-d1, D.s.Q, -d2, M.s.Q, d3, N.s.Q, 0.0);
-    if (r.ro <= 0.0 || r.p <= 0.0) bad = 1;
-    return r;
+    const TF d2 = TF(dist2d(fx, fy, M.cx, M.cy));
+    const TF d3 = TF(dist2d(fx, fy, N.cx, N.cy));
+    const TF d1 = TF(dist2d(fx, fy, D.cx, D.cy));
+#pragma unroll
+    for (int k = 0; k < Ph::NV; k++) r[k] = linear3<TF>(-d1, TF(D.s[k]), -d2, TF(M.s[k]), d3, TF(N.s[k]), TF(0));
+    if (r[0] <= TF(0) || r[1] <= TF(0)) bad = 1;
 }
 
-// Gran::Get_par_TVD_radial, density and pressure (Gran.cpp:562-675).
-__device__ __forceinline__ void tvd_radial(const Cell& M, const Cell& N, const Cell& D, bool m_axis,
-                                           double fx, double fy, double gam, double& ro, double& pr)
+// Gran::Get_par_TVD_radial, density and pressure (Gran.cpp:562-675); FP64 gas only.
+template <class Ph>
+__device__ __forceinline__ void tvd_radial(const CellT<Ph, double>& M, const CellT<Ph, double>& N,
+                                           const CellT<Ph, double>& D, bool m_axis, double fx, double fy,
+                                           double gam, double& ro, double& pr)
 {
-    double xd, yd;
-    Gas<double> sd;
+    double xd, yd, dro, dp;
     if (!D.ok) {
-        if (!m_axis) { ro = M.s.ro; pr = M.s.p; return; }
-        xd = M.cx; yd = -M.cy; sd = M.s;
+        if (!m_axis) { ro = M.s[0]; pr = M.s[1]; return; }
+        xd = M.cx; yd = -M.cy; dro = M.s[0]; dp = M.s[1];
     } else {
-        xd = D.cx; yd = D.cy; sd = D.s;
+        xd = D.cx; yd = D.cy; dro = D.s[0]; dp = D.s[1];
     }
     const double g2 = 2.0 * gam;
     const double r = sqrt(sq(fx) + sq(fy));
@@ -176,36 +189,43 @@ __device__ __forceinline__ void tvd_radial(const Cell& M, const Cell& N, const C
     const double d3 = dist2d(fx, fy, N.cx, N.cy);
     const double r1 = sqrt(sq(xd) + sq(yd));
     const double d1 = dist2d(fx, fy, xd, yd);
-    ro = linear3(-d1, sd.ro * sq(r1), -d2, M.s.ro * sq(r2), d3, N.s.ro * sq(r3), 0.0) / sq(r);
-    pr = linear3(-d1, sd.p * pow(r1, g2), -d2, M.s.p * pow(r2, g2), d3, N.s.p * pow(r3, g2), 0.0) / pow(r, g2);
+    ro = linear3(-d1, dro * sq(r1), -d2, M.s[0] * sq(r2), d3, N.s[0] * sq(r3), 0.0) / sq(r);
+    pr = linear3(-d1, dp * pow(r1, g2), -d2, M.s[1] * pow(r2, g2), d3, N.s[1] * pow(r3, g2), 0.0) / pow(r, g2);
 }
 
-// Radial rescaling of a C_1 value to the face radius (Setka.cpp:10221-10226).
-__device__ __forceinline__ void rescale_c1(Gas<double>& g, double rc, double dis, double gam,
+// Radial rescaling of a cell value to the face radius (Setka.cpp:10221-10226):
+// ro and the tracer by (rc/dis)^2, p by (rc/dis)^(2 gamma); C_1 cells also rotate u,v.
+template <int NV>
+__device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, double gam,
                                            double cx, double cy, double fx, double fy, bool rotate)
 {
-    g.ro = g.ro * sq(rc) / sq(dis);
-    g.Q = g.Q * sq(rc) / sq(dis);
-    g.p = g.p * pow(rc / dis, 2 * gam);
-    if (rotate) polar_perenos(cx, cy, fx, fy, g.u, g.v);
+    g[0] = g[0] * sq(rc) / sq(dis);
+    g[NV - 1] = g[NV - 1] * sq(rc) / sq(dis);
+    g[1] = g[1] * pow(rc / dis, 2 * gam);
+    if (rotate) polar_perenos(cx, cy, fx, fy, g[2], g[3]);
 }
 
-// Value of cell M on its side of an interior face (both cell kinds of the sweep,
-// Setka.cpp:10213-10251).
-template <bool TYPES>
-__device__ __forceinline__ Gas<double> side_value(const Cell& M, const Cell& N, const Cell& D, bool m_axis,
-                                                  double fx, double fy, double dis, double gam, int& bad)
+// Value of cell M on its side of an interior face (Setka.cpp:10213-10251).
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
+                                           bool m_axis, double fx, double fy, double dis, double gam, TF* r, int& bad)
 {
-    if (TYPES && M.type == GEL_C_1) {
-        Gas<double> g = M.s;
-        const double rc = sqrt(sq(M.cx) + sq(M.cy));
-        rescale_c1(g, rc, dis, gam, M.cx, M.cy, fx, fy, true);
-        double ro, pr;
-        tvd_radial(M, N, D, m_axis, fx, fy, gam, ro, pr);
-        g.ro = ro; g.p = pr;
-        return g;
+    if constexpr (TYPES) {
+        if (M.type == GEL_C_1) {
+            double g[Ph::NV];
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++) g[k] = M.s[k];
+            const double rc = sqrt(sq(M.cx) + sq(M.cy));
+            rescale_c1<Ph::NV>(g, rc, dis, gam, M.cx, M.cy, fx, fy, true);
+            double ro, pr;
+            tvd_radial<Ph>(M, N, D, m_axis, fx, fy, gam, ro, pr);
+            g[0] = ro; g[1] = pr;
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++) r[k] = g[k];
+            return;
+        }
     }
-    return tvd_usualy(M, N, D, m_axis, fx, fy, bad);
+    tvd_usualy<Ph, TS, TF>(M, N, D, m_axis, fx, fy, r, bad);
 }
 
 struct Face {          // geometry of a face as seen from its owning cell
@@ -234,63 +254,147 @@ __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, doubl
     return f;
 }
 
-template <bool HLLC>
-__device__ __forceinline__ double flux(const Gas<double>& L, const Gas<double>& R, const Face& f,
-                                       double gam, double P[5])
+// Face flux from left/right primitives; P in the order of the cell accumulators.
+template <class Ph, class TF, int FLUX>
+__device__ __forceinline__ TF face_flux(const TF* L, const TF* R, const Face& f, TF gam, TF* P)
 {
-    double PQ;
-    const double UU = hllc2d<double, HLLC>(L, R, f.W, f.n1, f.n2, gam, P, PQ);
-    P[4] = PQ;
-    return UU;
+    if constexpr (Ph::NV == 5) {
+        const Gas<TF> gl{ L[0], L[1], L[2], L[3], L[4] };
+        const Gas<TF> gr{ R[0], R[1], R[2], R[3], R[4] };
+        TF PQ;
+        const TF UU = hllc2d<TF, FLUX == GEL_FLUX_HLLC>(gl, gr, TF(f.W), TF(f.n1), TF(f.n2), gam, P, PQ);
+        P[4] = PQ;
+        return UU;
+    } else {
+        return hlld<TF>(L, R, TF(f.W), TF(f.n1), TF(f.n2), gam, P);
+    }
 }
 
-// Boundary face of cell M (not Usualy): Gran::Get_par ghost (Gran.cpp:205-289) and the
-// sweep's treatment of the cell side (Setka.cpp:10252-10274).  D is the cell across
-// M from the face (Sosed_down); used on the axis.
-template <bool TYPES, bool HLLC>
-__device__ __forceinline__ double boundary_flux(const SweepArgs& a, int ftype, const Cell& M, const Cell& D,
-                                                const Face& f, double P[5])
+// Boundary face of cell M: Gran::Get_par ghost (Gran.cpp:205-289) and the sweep's
+// treatment of the cell side (Setka.cpp:10252-10274).  D is the cell across M from
+// the face (Sosed_down), used on the axis.
+template <class Ph, class TS, class TF, bool TYPES, int FLUX>
+__device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const CellT<Ph, TS>& M,
+                                            const CellT<Ph, TS>& D, const Face& f, TF* P)
 {
+    constexpr int NV = Ph::NV;
     const gel_params& p = a.prm;
     const double gam = p.gamma;
-    Gas<double> L = M.s, R = M.s;
+    TF L[NV], R[NV];
     const double radius = sqrt(sq(M.cx) + sq(M.cy));
     if (ftype == GEL_F_INNER_SPHERE) {
-        rescale_c1(L, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, false);
+        double g[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) g[k] = M.s[k];
+        rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, false);
         const double d = sqrt(f.fx * f.fx + f.fy * f.fy);
         const double ro = 1.0 / (p.in_rho_div * d * d);
-        R.ro = ro;
-        R.p = ro * p.in_p_a * p.in_p_a / p.in_p_den;
-        R.u = p.in_u * f.fx / d;
-        R.v = p.in_u * f.fy / d;
-        R.Q = ro * p.in_q;
+        double w[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) w[k] = 0.0;
+        w[0] = ro;
+        w[1] = ro * p.in_p_a * p.in_p_a / p.in_p_den;
+        w[2] = p.in_u * f.fx / d;
+        w[3] = p.in_u * f.fy / d;
+        w[NV - 1] = ro * p.in_q;
+#pragma unroll
+        for (int k = 0; k < NV; k++) { L[k] = TF(g[k]); R[k] = TF(w[k]); }
     } else if (ftype == GEL_F_AXIS) {
-        if (TYPES && M.type == GEL_C_1) {
-            rescale_c1(L, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
-        } else {                                             // Gran.cpp:486-519
-            const double d2 = dist2d(f.fx, f.fy, M.cx, M.cy);
-            const double d1 = dist2d(f.fx, f.fy, D.cx, D.cy);
-            L.v = linear3(-d1, D.s.v, -d2, M.s.v, d2, -M.s.v, 0.0);
+        bool done = false;
+        if constexpr (TYPES) {
+            if (M.type == GEL_C_1) {
+                double g[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++) g[k] = M.s[k];
+                rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
+#pragma unroll
+                for (int k = 0; k < NV; k++) L[k] = TF(g[k]);
+                done = true;
+            }
         }
-        R = L;
-        R.v = -R.v;
-    } else if (ftype == GEL_F_INPUT) {
-        R.ro = p.inf_rho; R.p = p.inf_p; R.u = p.inf_u; R.v = p.inf_v; R.Q = p.inf_q;
-    } else if (ftype == GEL_F_EXTERN) {
-        if (R.u >= 0.0) R.u = p.extern_u;
+        if (!done) {                                          // Gran.cpp:486-519
+            const TF d2 = TF(dist2d(f.fx, f.fy, M.cx, M.cy));
+            const TF d1 = TF(dist2d(f.fx, f.fy, D.cx, D.cy));
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                L[k] = TF(M.s[k]);
+                if (Ph::odd(k)) L[k] = linear3<TF>(-d1, TF(D.s[k]), -d2, TF(M.s[k]), d2, -TF(M.s[k]), TF(0));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NV; k++) R[k] = Ph::odd(k) ? -L[k] : L[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NV; k++) { L[k] = TF(M.s[k]); R[k] = L[k]; }
+        if (ftype == GEL_F_INPUT) {
+            R[0] = TF(p.inf_rho); R[1] = TF(p.inf_p); R[2] = TF(p.inf_u); R[3] = TF(p.inf_v);
+            R[NV - 1] = TF(p.inf_q);
+            if constexpr (NV == 9) {
+                R[4] = TF(p.inf_w); R[5] = TF(p.inf_bx); R[6] = TF(p.inf_by); R[7] = TF(p.inf_bz);
+            }
+        } else if (ftype == GEL_F_EXTERN) {
+            if (R[2] >= TF(0)) R[2] = TF(p.extern_u);
+        }
     }
-    return flux<HLLC>(L, R, f, gam, P);
+    return face_flux<Ph, TF, FLUX>(L, R, f, TF(gam), P);
 }
 
-
-template <bool TYPES, bool HLLC>
-__device__ __forceinline__ double interior_flux(const Cell& M, const Cell& N, const Cell& D, const Cell& U,
-                                                bool m_axis, bool n_axis, const Face& f, double gam,
-                                                double P[5], int& bad)
+template <class Ph, class TS, class TF, bool TYPES, int FLUX>
+__device__ __forceinline__ TF interior_flux(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
+                                            const CellT<Ph, TS>& U, bool m_axis, bool n_axis, const Face& f,
+                                            double gam, TF* P, int& bad)
 {
-    const Gas<double> L = side_value<TYPES>(M, N, D, m_axis, f.fx, f.fy, f.dis, gam, bad);
-    const Gas<double> R = side_value<TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, bad);
-    return flux<HLLC>(L, R, f, gam, P);
+    TF L[Ph::NV], R[Ph::NV];
+    side_value<Ph, TS, TF, TYPES>(M, N, D, m_axis, f.fx, f.fy, f.dis, gam, L, bad);
+    side_value<Ph, TS, TF, TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, R, bad);
+    return face_flux<Ph, TF, FLUX>(L, R, f, TF(gam), P);
+}
+
+// Conservative ALE update of one cell and primitive recovery (Setka.cpp:10423-10460),
+// r-weighted: Volume = int y dA before, Volume2 after the step; Ar = planar area.
+// MHD adds the axisymmetric geometric sources (hoop stress with field and swirl,
+// angular momentum, azimuthal induction) per unit planar area.
+template <class Ph>
+__device__ __forceinline__ void cell_update(const double* s, const double* Pot, double Volume, double Volume2,
+                                            double vs1, double vs2, double T1, double gam, double q1, double q2,
+                                            double q3, double* o, int& n_rho, int& n_p)
+{
+    if constexpr (Ph::NV == 5) {
+        const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+        const double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2)) / ro3;
+        double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3)) -
+                     0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = Q33;
+    } else {
+        const double pi4 = 4.0 * kPi, pi8 = 8.0 * kPi;
+        const double ro = s[0], p = s[1], u = s[2], v = s[3], w = s[4];
+        const double bx = s[5], by = s[6], bz = s[7], Q = s[8];
+        const double Ar = 0.5 * (vs1 + vs2);
+        const double BB = bx * bx + by * by + bz * bz;
+        const double Sv = ro * w * w + p + BB / pi8 - bz * bz / pi4;
+        const double Sw = -(ro * v * w - by * bz / pi4);
+        const double Sb = v * bz - w * by;
+        const double E = p / (gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + BB / pi8;
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - Sv * Ar) / Volume2 - q2)) / ro3;
+        const double w3 = (ro * w * Volume / Volume2 - T1 * ((Pot[3] - Sw * Ar) / Volume2)) / ro3;
+        const double E3 = E * Volume / Volume2 - T1 * (Pot[4] / Volume2 - q3);
+        const double bx3 = bx * Volume / Volume2 - T1 * (Pot[5] / Volume2);
+        const double by3 = by * Volume / Volume2 - T1 * (Pot[6] / Volume2);
+        const double bz3 = bz * Volume / Volume2 - T1 * ((Pot[7] - Sb * Ar) / Volume2);
+        const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2);
+        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / pi8) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = w3;
+        o[5] = bx3; o[6] = by3; o[7] = bz3; o[8] = Q3;
+    }
 }
 
 __device__ __forceinline__ double wave_min(double v)
@@ -307,10 +411,15 @@ __device__ __forceinline__ int wave_sum(int v)
 }
 
 // One step of the sweep over a chunk of rows for a strip of columns.
-template <bool MOVING, bool TYPES, bool SRC, bool HLLC>
+// Ph: state vector; TS: storage type of the state in HBM; TF: type of the face
+// arithmetic (reconstruction and Riemann solver); geometry, accumulation and the
+// update are FP64 throughout.
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, bool SRC, int FLUX>
 __global__ void __launch_bounds__(kWave)
 sweep_kernel(SweepArgs a)
 {
+    constexpr int NV = Ph::NV;
+    using C = CellT<Ph, TS>;
     const int lane = threadIdx.x;
     const int j = blockIdx.x * kOut + lane - kHalo;
     const bool col = (j >= 0 && j < a.nj);
@@ -330,33 +439,35 @@ sweep_kernel(SweepArgs a)
 
     // window: cells of rows i-1, i, i+1, i+2 and node rows i .. i+3
     Nodes nr[4];
-    Cell w[4];
-    const Nodes nm2 = load_nodes(a, a.x, a.y, i0 - 2, j, col);
-    const Nodes nm1 = load_nodes(a, a.x, a.y, i0 - 1, j, col);
+    C w[4];
+    const Nodes nm2 = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 - 2, j, col);
+    const Nodes nm1 = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 - 1, j, col);
 #pragma unroll
-    for (int k = 0; k < 4; k++) nr[k] = load_nodes(a, a.x, a.y, i0 + k, j, col);
-    const Cell cm2 = load_cell<TYPES>(a, i0 - 2, j, col, nm2, nm1);
-    w[0] = load_cell<TYPES>(a, i0 - 1, j, col, nm1, nr[0]);
+    for (int k = 0; k < 4; k++) nr[k] = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 + k, j, col);
+    const C cm2 = load_cell<Ph, TS, TYPES>(a, i0 - 2, j, col, nm2, nm1);
+    w[0] = load_cell<Ph, TS, TYPES>(a, i0 - 1, j, col, nm1, nr[0]);
 #pragma unroll
-    for (int k = 1; k < 4; k++) w[k] = load_cell<TYPES>(a, i0 + k - 1, j, col, nr[k - 1], nr[k]);
+    for (int k = 1; k < 4; k++) w[k] = load_cell<Ph, TS, TYPES>(a, i0 + k - 1, j, col, nr[k - 1], nr[k]);
 
     int bad = 0;
     // flux through the radial face below row i0, oriented along +i
-    double flo[5], uulo;
+    TF flo[NV];
+    TF uulo;
     {
-        Nodes q = nr[0], qn = nr[0];
-        if (MOVING) qn = load_nodes(a, a.x2, a.y2, i0, j, col);
-        double P[5];
+        const Nodes q = nr[0];
+        Nodes qn = q;
+        if (MOVING) qn = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i0, j, col);
         if (i0 == 0) {
             // inner sphere, edge 3 of cell (0, j): (0, j+1) -> (0, j)
             const Face f = face_geom<MOVING>(q.xb, q.yb, q.xa, q.ya, qn.xb, qn.yb, qn.xa, qn.ya, dt);
-            uulo = boundary_flux<TYPES, HLLC>(a, GEL_F_INNER_SPHERE, w[1], w[2], f, P);
+            TF P[NV];
+            uulo = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_INNER_SPHERE, w[1], w[2], f, P);
 #pragma unroll
-            for (int k = 0; k < 5; k++) flo[k] = -P[k];
+            for (int k = 0; k < NV; k++) flo[k] = -P[k];
         } else {
             // edge 1 of cell (i0-1, j): (i0, j) -> (i0, j+1)
             const Face f = face_geom<MOVING>(q.xa, q.ya, q.xb, q.yb, qn.xa, qn.ya, qn.xb, qn.yb, dt);
-            uulo = interior_flux<TYPES, HLLC>(w[0], w[1], cm2, w[2], m_axis, m_axis, f, gam, flo, bad);
+            uulo = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[0], w[1], cm2, w[2], m_axis, m_axis, f, gam, flo, bad);
         }
     }
 
@@ -366,37 +477,37 @@ sweep_kernel(SweepArgs a)
         const Nodes q0 = nr[0], q1 = nr[1];
         Nodes q0n = q0, q1n = q1;
         if (MOVING) {
-            q0n = load_nodes(a, a.x2, a.y2, i, j, col);
-            q1n = load_nodes(a, a.x2, a.y2, i + 1, j, col);
+            q0n = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i, j, col);
+            q1n = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i + 1, j, col);
         }
         // edge 1: radial face (i, i+1), (i+1, j) -> (i+1, j+1)
         const Face f1 = face_geom<MOVING>(q1.xa, q1.ya, q1.xb, q1.yb, q1n.xa, q1n.ya, q1n.xb, q1n.yb, dt);
-        double P1[5], uu1;
+        TF P1[NV], uu1;
         if (i + 1 < a.ni) {
-            uu1 = interior_flux<TYPES, HLLC>(w[1], w[2], w[0], w[3], m_axis, m_axis, f1, gam, P1, bad);
+            uu1 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], w[2], w[0], w[3], m_axis, m_axis, f1, gam, P1, bad);
         } else {
             const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
-            uu1 = boundary_flux<TYPES, HLLC>(a, ot, w[1], w[0], f1, P1);
+            uu1 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, ot, w[1], w[0], f1, P1);
         }
         // edge 2: polar face (j, j+1), (i+1, j+1) -> (i, j+1)
         const Face f2 = face_geom<MOVING>(q1.xb, q1.yb, q0.xb, q0.yb, q1n.xb, q1n.yb, q0n.xb, q0n.yb, dt);
-        const Cell cN = cell_dn(w[1], 1), cD = cell_up(w[1], 1), cU = cell_dn(w[1], 2);
-        double P2[5], uu2;
-        if (axis_hi) uu2 = boundary_flux<TYPES, HLLC>(a, GEL_F_AXIS, w[1], cD, f2, P2);
-        else uu2 = interior_flux<TYPES, HLLC>(w[1], cN, cD, cU, m_axis, n_axis, f2, gam, P2, bad);
+        const C cN = cell_shfl<true>(w[1], 1), cD = cell_shfl<false>(w[1], 1), cU = cell_shfl<true>(w[1], 2);
+        TF P2[NV], uu2;
+        if (axis_hi) uu2 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_AXIS, w[1], cD, f2, P2);
+        else uu2 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], cN, cD, cU, m_axis, n_axis, f2, gam, P2, bad);
         // edge 0: polar face (j-1, j), (i, j) -> (i+1, j); solved by the lane below
         const Face f0 = face_geom<MOVING>(q0.xa, q0.ya, q1.xa, q1.ya, q0n.xa, q0n.ya, q1n.xa, q1n.ya, dt);
-        double P0[5], uu0;
+        TF P0[NV], uu0;
         {
-            double Pl[5];
+            TF Pl[NV];
 #pragma unroll
-            for (int k = 0; k < 5; k++) Pl[k] = shfl_up(P2[k], 1);
-            const double uul = shfl_up(uu2, 1);
+            for (int k = 0; k < NV; k++) Pl[k] = shfl_up(P2[k], 1);
+            const TF uul = shfl_up(uu2, 1);
             if (axis_lo) {
-                uu0 = boundary_flux<TYPES, HLLC>(a, GEL_F_AXIS, w[1], cN, f0, P0);
+                uu0 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_AXIS, w[1], cN, f0, P0);
             } else {
 #pragma unroll
-                for (int k = 0; k < 5; k++) P0[k] = -Pl[k];
+                for (int k = 0; k < NV; k++) P0[k] = -Pl[k];
                 uu0 = uul;
             }
         }
@@ -404,31 +515,35 @@ sweep_kernel(SweepArgs a)
         const Face f3 = face_geom<MOVING>(q0.xb, q0.yb, q0.xa, q0.ya, q0n.xb, q0n.yb, q0n.xa, q0n.ya, dt);
 
         // cell update (Setka.cpp:10372-10468)
-        const Cell& K = w[1];
+        const C& K = w[1];
         const double x = K.cx, y = K.cy;
         const double radius = sqrt(sq(x) + sq(y));
         if (out_lane && !(radius < a.prm.r_skip)) {
-            double Pot[5];
+            double Pot[NV];
 #pragma unroll
-            for (int k = 0; k < 5; k++) {
+            for (int k = 0; k < NV; k++) {
                 double s = 0.0;
-                s = s + P0[k] * f0.S;
-                s = s + P1[k] * f1.S;
-                s = s + P2[k] * f2.S;
-                s = s + (-flo[k]) * f3.S;
+                s = s + double(P0[k]) * f0.S;
+                s = s + double(P1[k]) * f1.S;
+                s = s + double(P2[k]) * f2.S;
+                s = s + double(-flo[k]) * f3.S;
                 Pot[k] = s;
             }
             double time = 10000000;
-            time = mn(time, cfl * sqrt(sq(x - f0.fx) + sq(y - f0.fy)) / uu0);
-            time = mn(time, cfl * sqrt(sq(x - f1.fx) + sq(y - f1.fy)) / uu1);
-            time = mn(time, cfl * sqrt(sq(x - f2.fx) + sq(y - f2.fy)) / uu2);
-            time = mn(time, cfl * sqrt(sq(x - f3.fx) + sq(y - f3.fy)) / uulo);
+            time = mn(time, cfl * sqrt(sq(x - f0.fx) + sq(y - f0.fy)) / double(uu0));
+            time = mn(time, cfl * sqrt(sq(x - f1.fx) + sq(y - f1.fy)) / double(uu1));
+            time = mn(time, cfl * sqrt(sq(x - f2.fx) + sq(y - f2.fy)) / double(uu2));
+            time = mn(time, cfl * sqrt(sq(x - f3.fx) + sq(y - f3.fy)) / double(uulo));
             tmin = fmin(tmin, time);
 
             const size_t kc = (size_t)i * a.nj + j;
+            TS* __restrict__ nxt = (TS*)a.next;
+            double sd[NV];
+#pragma unroll
+            for (int k = 0; k < NV; k++) sd[k] = double(K.s[k]);
             if (TYPES && K.type == GEL_C_CENTR) {
-                a.next[kc] = K.s.ro; a.next[nc + kc] = K.s.p; a.next[2 * nc + kc] = K.s.u;
-                a.next[3 * nc + kc] = K.s.v; a.next[4 * nc + kc] = K.s.Q;
+#pragma unroll
+                for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = K.s[k];
             } else {
                 const double px[4] = { q0.xa, q1.xa, q1.xb, q0.xb };
                 const double py[4] = { q0.ya, q1.ya, q1.yb, q0.yb };
@@ -441,37 +556,27 @@ sweep_kernel(SweepArgs a)
                     vs2 = quad_area(px2, py2);
                     Volume2 = quad_volume_r(px2, py2);
                 }
-                const double u = K.s.u, v = K.s.v, ro = K.s.ro, p = K.s.p, Q = K.s.Q;
                 double q1s = 0.0, q2s = 0.0, q3s = 0.0;
                 if (SRC) {
                     const double* __restrict__ sp = a.src + kc;
-                    source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, u, v, ro, p, q1s, q2s, q3s);
+                    source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
+                               q1s, q2s, q3s);
                 }
-                const double T1 = dt;
-                double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
-                const double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
-                if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
-                const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1s)) / ro3;
-                const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2s)) / ro3;
-                double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3s)) -
-                             0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
-                if (p3 <= 0) { p3 = 0.000001; n_p++; }
-                a.next[kc] = ro3;
-                a.next[nc + kc] = p3;
-                a.next[2 * nc + kc] = u3;
-                a.next[3 * nc + kc] = v3;
-                a.next[4 * nc + kc] = Q33;
+                double o[NV];
+                cell_update<Ph>(sd, Pot, Volume, Volume2, vs1, vs2, dt, gam, q1s, q2s, q3s, o, n_rho, n_p);
+#pragma unroll
+                for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
             }
         }
 
         // slide the window one row up
 #pragma unroll
-        for (int k = 0; k < 5; k++) flo[k] = P1[k];
+        for (int k = 0; k < NV; k++) flo[k] = P1[k];
         uulo = uu1;
         w[0] = w[1]; w[1] = w[2]; w[2] = w[3];
         nr[0] = nr[1]; nr[1] = nr[2]; nr[2] = nr[3];
-        nr[3] = load_nodes(a, a.x, a.y, i + 4, j, col);
-        w[3] = load_cell<TYPES>(a, i + 3, j, col, nr[2], nr[3]);
+        nr[3] = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i + 4, j, col);
+        w[3] = load_cell<Ph, TS, TYPES>(a, i + 3, j, col, nr[2], nr[3]);
     }
 
     // the numeric flag counts on lanes whose faces are used (edge 0 of lane 2 is lane 1's)
@@ -503,6 +608,18 @@ __global__ void flux_batch_kernel(int n, const double* __restrict__ L, const dou
     o[0] = P[0]; o[1] = P[1]; o[2] = P[2]; o[3] = P[3]; o[4] = PQ; o[5] = Vl; o[6] = Vc; o[7] = Vp; o[8] = t;
 }
 
+// Batched HLLD for tests (gel_flux_hlld_batch): primitives [n][9] in, fluxes [n][9]
+// and max wave speed [n] out.
+__global__ void hlld_batch_kernel(int n, const double* __restrict__ L, const double* __restrict__ R,
+                                  const double* __restrict__ W, const double* __restrict__ n1,
+                                  const double* __restrict__ n2, double gam, double* __restrict__ P,
+                                  double* __restrict__ uu)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    uu[k] = hlld<double>(L + 9 * (size_t)k, R + 9 * (size_t)k, W[k], n1[k], n2[k], gam, P + 9 * (size_t)k);
+}
+
 }  // namespace gel
 
 // ------------------------------------------------------------------ C-ABI ------
@@ -527,12 +644,14 @@ static int fail(int st, const std::string& msg)
 struct gel_ctx {
     int ni = 0, nj = 0;
     size_t nc = 0, nn = 0;
+    int nv = 5;                   // primitives per cell
+    size_t ts = 8;                // bytes per stored primitive (8 FP64, 4 FP32)
     gel_params prm{};
-    double* d_x = nullptr;
+    double* d_x = nullptr;        // node positions (base positions under prescribed motion)
     double* d_y = nullptr;
-    double* d_x2 = nullptr;       // next node positions (moving steps)
+    double* d_x2 = nullptr;       // next node positions (explicit moving steps)
     double* d_y2 = nullptr;
-    double* d_state[2] = { nullptr, nullptr };
+    void* d_state[2] = { nullptr, nullptr };
     int cur = 0;
     int8_t* d_ctype = nullptr;
     int8_t* d_otype = nullptr;
@@ -543,6 +662,9 @@ struct gel_ctx {
     long long steps = 0;
     bool uploaded = false;
     int rows = 8;
+    // prescribed motion: nodes at base*s[k], k = step mod period
+    std::vector<double> motion;
+    long long motion_step = 0;
     hipStream_t stream = nullptr;
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -578,6 +700,7 @@ void gel_params_reference(gel_params* p)
     p->flux = GEL_FLUX_HLL;
     p->physics = GEL_PHYS_GAS;
     p->source_mode = GEL_SRC_NONE;
+    p->precision = GEL_PREC_FP64;
     p->in_rho_div = chi * chi;
     p->in_p_a = chi;
     p->in_p_den = ga * 10.0 * 10.0;
@@ -593,12 +716,12 @@ void gel_params_reference(gel_params* p)
 static void free_ctx(gel_ctx* c)
 {
     if (!c) return;
-    hipFree(c->d_x); hipFree(c->d_y); hipFree(c->d_x2); hipFree(c->d_y2);
-    hipFree(c->d_state[0]); hipFree(c->d_state[1]);
-    hipFree(c->d_ctype); hipFree(c->d_otype); hipFree(c->d_src);
-    hipFree(c->d_dt); hipFree(c->d_cnt);
-    for (auto& e : c->events) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-    if (c->stream) hipStreamDestroy(c->stream);
+    (void)hipFree(c->d_x); (void)hipFree(c->d_y); (void)hipFree(c->d_x2); (void)hipFree(c->d_y2);
+    (void)hipFree(c->d_state[0]); (void)hipFree(c->d_state[1]);
+    (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src);
+    (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
+    for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -620,14 +743,19 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if (!out || !g || !prm || !g->x || !g->y) return fail(GEL_E_ARG, "null argument");
     *out = nullptr;
     if (g->ni < 3 || g->nj < 3) return fail(GEL_E_ARG, "grid needs at least 3x3 cells");
-    if (prm->physics != GEL_PHYS_GAS) return fail(GEL_E_UNSUPPORTED, "physics");
-    if (prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC) return fail(GEL_E_UNSUPPORTED, "flux");
+    if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD) return fail(GEL_E_UNSUPPORTED, "physics");
+    if (prm->physics == GEL_PHYS_GAS && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
+        return fail(GEL_E_UNSUPPORTED, "flux");
     if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1) return fail(GEL_E_UNSUPPORTED, "source_mode");
+    if (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK) return fail(GEL_E_UNSUPPORTED, "precision");
+    if (!(prm->dt0 > 0.0) || !(prm->cfl > 0.0) || !(prm->gamma > 1.0)) return fail(GEL_E_ARG, "dt0, cfl, gamma");
     gel_ctx* c = new (std::nothrow) gel_ctx();
     if (!c) return fail(GEL_E_NOMEM, "host allocation");
     c->ni = g->ni; c->nj = g->nj;
     c->nc = (size_t)g->ni * g->nj;
     c->nn = (size_t)(g->ni + 1) * (g->nj + 1);
+    c->nv = prm->physics == GEL_PHYS_MHD ? 9 : 5;
+    c->ts = prm->precision == GEL_PREC_FP32_BULK ? 4 : 8;
     c->prm = *prm;
     c->rows = choose_rows(g->ni, g->nj);
     auto bad = [&](hipError_t e, const char* what) {
@@ -642,7 +770,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if ((e = hipMalloc(&c->d_x2, nb)) != hipSuccess) return bad(e, "nodes");
     if ((e = hipMalloc(&c->d_y2, nb)) != hipSuccess) return bad(e, "nodes");
     for (int b = 0; b < 2; b++)
-        if ((e = hipMalloc(&c->d_state[b], 5 * c->nc * sizeof(double))) != hipSuccess) return bad(e, "state");
+        if ((e = hipMalloc(&c->d_state[b], (size_t)c->nv * c->nc * c->ts)) != hipSuccess) return bad(e, "state");
     if ((e = hipMalloc(&c->d_otype, (size_t)g->nj)) != hipSuccess) return bad(e, "outer types");
     if ((e = hipMalloc(&c->d_dt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
     if ((e = hipMalloc(&c->d_cnt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "counters");
@@ -670,6 +798,10 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
             if (t == GEL_C_1 || t == GEL_C_CENTR) any = true;
         }
         if (any) {
+            if (prm->physics != GEL_PHYS_GAS || prm->precision != GEL_PREC_FP64) {
+                free_ctx(c);
+                return fail(GEL_E_UNSUPPORTED, "C_1 / C_centr cells need FP64 gas physics");
+            }
             if ((e = hipMalloc(&c->d_ctype, c->nc)) != hipSuccess) return bad(e, "cell types");
             if ((e = hipMemcpy(c->d_ctype, g->cell_type, c->nc, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
             c->has_types = true;
@@ -693,16 +825,34 @@ static double bitsd(unsigned long long b)
     return v;
 }
 
+static int copy_state_in(gel_ctx* c, void* dst, const double* prim)
+{
+    const size_t n = (size_t)c->nv * c->nc;
+    if (c->ts == 8) {
+        HIP_OK(hipMemcpyAsync(dst, prim, n * 8, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> f(n);
+        for (size_t k = 0; k < n; k++) f[k] = (float)prim[k];
+        HIP_OK(hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    }
+    return GEL_OK;
+}
+
 int gel_upload_state(gel_ctx* c, const double* prim)
 {
     if (!c || !prim) return fail(GEL_E_ARG, "null argument");
-    HIP_OK(hipMemcpyAsync(c->d_state[0], prim, 5 * c->nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIP_OK(hipMemcpyAsync(c->d_state[1], prim, 5 * c->nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int st = copy_state_in(c, c->d_state[0], prim);
+    if (st != GEL_OK) return st;
+    st = copy_state_in(c, c->d_state[1], prim);
+    if (st != GEL_OK) return st;
     const unsigned long long dt[3] = { dbits(c->prm.dt0), dbits(__builtin_huge_val()), dbits(__builtin_huge_val()) };
     HIP_OK(hipMemcpyAsync(c->d_dt, dt, sizeof dt, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->steps = 0;
+    c->motion_step = 0;
     c->uploaded = true;
     return GEL_OK;
 }
@@ -711,8 +861,16 @@ int gel_download_state(gel_ctx* c, double* prim)
 {
     if (!c || !prim) return fail(GEL_E_ARG, "null argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
-    HIP_OK(hipMemcpyAsync(prim, c->d_state[c->cur], 5 * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)c->nv * c->nc;
+    if (c->ts == 8) {
+        HIP_OK(hipMemcpyAsync(prim, c->d_state[c->cur], n * 8, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> f(n);
+        HIP_OK(hipMemcpyAsync(f.data(), c->d_state[c->cur], n * 4, hipMemcpyDeviceToHost, c->stream));
+        HIP_OK(hipStreamSynchronize(c->stream));
+        for (size_t k = 0; k < n; k++) prim[k] = (double)f[k];
+    }
     return GEL_OK;
 }
 
@@ -735,37 +893,75 @@ int gel_set_rows_per_wave(gel_ctx* c, int rows)
     return GEL_OK;
 }
 
+int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
+{
+    if (!c || period < 1 || !(amp >= 0.0) || !(amp < 0.5)) return fail(GEL_E_ARG, "motion: amp in [0,0.5), period >= 1");
+    c->motion.assign((size_t)period, 1.0);
+    const double pi = 3.14159265358979323846;
+    for (int k = 0; k < period; k++) c->motion[k] = 1.0 + amp * std::sin(2.0 * pi * k / period + phase);
+    c->motion_step = 0;
+    return GEL_OK;
+}
+
 }  // extern "C"
 
-template <bool MOVING, bool TYPES, bool SRC, bool HLLC>
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, bool SRC, int FLUX>
 static hipError_t launch_t(const SweepArgs& a, dim3 grid, hipStream_t s)
 {
-    hipLaunchKernelGGL((sweep_kernel<MOVING, TYPES, SRC, HLLC>), grid, dim3(kWave), 0, s, a);
+    hipLaunchKernelGGL((sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX>), grid, dim3(kWave), 0, s, a);
     return hipGetLastError();
 }
 
-static hipError_t launch(const SweepArgs& a, dim3 grid, hipStream_t s, bool moving, bool types, bool src, bool hllc)
+template <class Ph, class TS, class TF, bool TYPES, int FLUX>
+static hipError_t launch_ms(const SweepArgs& a, dim3 grid, hipStream_t s, bool moving, bool src)
 {
-    const int key = (moving ? 8 : 0) | (types ? 4 : 0) | (src ? 2 : 0) | (hllc ? 1 : 0);
-    switch (key) {
-#define K(m, t, sr, h) case ((m) * 8 + (t) * 4 + (sr) * 2 + (h)): return launch_t<m, t, sr, h>(a, grid, s);
-    K(0, 0, 0, 0) K(0, 0, 0, 1) K(0, 0, 1, 0) K(0, 0, 1, 1) K(0, 1, 0, 0) K(0, 1, 0, 1) K(0, 1, 1, 0) K(0, 1, 1, 1)
-    K(1, 0, 0, 0) K(1, 0, 0, 1) K(1, 0, 1, 0) K(1, 0, 1, 1) K(1, 1, 0, 0) K(1, 1, 0, 1) K(1, 1, 1, 0) K(1, 1, 1, 1)
-#undef K
+    if (moving) {
+        return src ? launch_t<Ph, TS, TF, true, TYPES, true, FLUX>(a, grid, s)
+                   : launch_t<Ph, TS, TF, true, TYPES, false, FLUX>(a, grid, s);
     }
-    return hipErrorInvalidValue;
+    return src ? launch_t<Ph, TS, TF, false, TYPES, true, FLUX>(a, grid, s)
+               : launch_t<Ph, TS, TF, false, TYPES, false, FLUX>(a, grid, s);
 }
 
-static int one_step(gel_ctx* c, bool moving)
+static hipError_t launch(const gel_ctx* c, const SweepArgs& a, dim3 grid, bool moving)
+{
+    const bool src = (c->prm.source_mode == GEL_SRC_MK1);
+    const bool f32 = (c->prm.precision == GEL_PREC_FP32_BULK);
+    hipStream_t s = c->stream;
+    if (c->prm.physics == GEL_PHYS_MHD) {
+        return f32 ? launch_ms<MhdPhys, float, float, false, 3>(a, grid, s, moving, src)
+                   : launch_ms<MhdPhys, double, double, false, 3>(a, grid, s, moving, src);
+    }
+    const bool hllc = (c->prm.flux == GEL_FLUX_HLLC);
+    if (f32) {
+        return hllc ? launch_ms<GasPhys, float, float, false, GEL_FLUX_HLLC>(a, grid, s, moving, src)
+                    : launch_ms<GasPhys, float, float, false, GEL_FLUX_HLL>(a, grid, s, moving, src);
+    }
+    if (c->has_types) {
+        return hllc ? launch_ms<GasPhys, double, double, true, GEL_FLUX_HLLC>(a, grid, s, moving, src)
+                    : launch_ms<GasPhys, double, double, true, GEL_FLUX_HLL>(a, grid, s, moving, src);
+    }
+    return hllc ? launch_ms<GasPhys, double, double, false, GEL_FLUX_HLLC>(a, grid, s, moving, src)
+                : launch_ms<GasPhys, double, double, false, GEL_FLUX_HLL>(a, grid, s, moving, src);
+}
+
+// moving: 0 static, 1 explicit next positions in d_x2/d_y2, 2 prescribed motion
+static int one_step(gel_ctx* c, int moving)
 {
     SweepArgs a;
     a.ni = c->ni; a.nj = c->nj; a.rows = c->rows;
     a.dt_rd = (int)(c->steps % 3);
     a.dt_wr = (int)((c->steps + 1) % 3);
     a.dt_clr = (int)((c->steps + 2) % 3);
+    a.s_old = 1.0; a.s_new = 1.0;
     a.x = c->d_x; a.y = c->d_y;
-    a.x2 = moving ? c->d_x2 : c->d_x;
-    a.y2 = moving ? c->d_y2 : c->d_y;
+    a.x2 = c->d_x; a.y2 = c->d_y;
+    if (moving == 1) { a.x2 = c->d_x2; a.y2 = c->d_y2; }
+    if (moving == 2) {
+        const size_t P = c->motion.size();
+        a.s_old = c->motion[(size_t)(c->motion_step % (long long)P)];
+        a.s_new = c->motion[(size_t)((c->motion_step + 1) % (long long)P)];
+    }
     a.now = c->d_state[c->cur];
     a.next = c->d_state[c->cur ^ 1];
     a.ctype = c->d_ctype; a.otype = c->d_otype;
@@ -773,8 +969,7 @@ static int one_step(gel_ctx* c, bool moving)
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
     a.prm = c->prm;
-    const bool src = (c->prm.source_mode == GEL_SRC_MK1);
-    if (src && !c->d_src) return fail(GEL_E_STATE, "sources not set");
+    if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
     dim3 grid((c->nj + kOut - 1) / kOut, (c->ni + c->rows - 1) / c->rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -782,7 +977,7 @@ static int one_step(gel_ctx* c, bool moving)
         HIP_OK(hipEventCreate(&e1));
         HIP_OK(hipEventRecord(e0, c->stream));
     }
-    hipError_t e = launch(a, grid, c->stream, moving, c->has_types, src, c->prm.flux == GEL_FLUX_HLLC);
+    hipError_t e = launch(c, a, grid, moving != 0);
     if (e != hipSuccess) return fail(GEL_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
     if (c->timing) {
         HIP_OK(hipEventRecord(e1, c->stream));
@@ -790,6 +985,7 @@ static int one_step(gel_ctx* c, bool moving)
     }
     c->cur ^= 1;
     c->steps++;
+    if (moving == 2) c->motion_step++;
     return GEL_OK;
 }
 
@@ -809,14 +1005,47 @@ static int fill_stats(gel_ctx* c, gel_stats* out)
     return GEL_OK;
 }
 
+template <class F>
+static int batch_run(int n, std::initializer_list<std::pair<const double*, size_t>> ins,
+                     std::initializer_list<std::pair<double*, size_t>> outs, F&& launch_fn)
+{
+    std::vector<double*> din, dout;
+    auto cleanup = [&]() { for (auto p : din) (void)hipFree(p); for (auto p : dout) (void)hipFree(p); };
+    for (auto& in : ins) {
+        double* d = nullptr;
+        if (hipMalloc(&d, in.second * n * sizeof(double)) != hipSuccess) { cleanup(); return fail(GEL_E_NOMEM, "batch"); }
+        din.push_back(d);
+        if (hipMemcpy(d, in.first, in.second * n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            cleanup();
+            return fail(GEL_E_HIP, "batch copy");
+        }
+    }
+    for (auto& o : outs) {
+        double* d = nullptr;
+        if (hipMalloc(&d, o.second * n * sizeof(double)) != hipSuccess) { cleanup(); return fail(GEL_E_NOMEM, "batch"); }
+        dout.push_back(d);
+    }
+    launch_fn(din, dout);
+    hipError_t e = hipGetLastError();
+    size_t k = 0;
+    for (auto& o : outs) {
+        if (e == hipSuccess) e = hipMemcpy(o.first, dout[k], o.second * n * sizeof(double), hipMemcpyDeviceToHost);
+        k++;
+    }
+    cleanup();
+    if (e != hipSuccess) return fail(GEL_E_HIP, std::string("batch: ") + hipGetErrorString(e));
+    return GEL_OK;
+}
+
 extern "C" {
 
 int gel_step(gel_ctx* c, int nsteps, gel_stats* out)
 {
     if (!c || nsteps < 0) return fail(GEL_E_ARG, "bad argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
+    const int mode = c->motion.empty() ? 0 : 2;
     for (int s = 0; s < nsteps; s++) {
-        const int st = one_step(c, false);
+        const int st = one_step(c, mode);
         if (st != GEL_OK) return st;
     }
     return fill_stats(c, out);
@@ -826,10 +1055,11 @@ int gel_step_moving(gel_ctx* c, const double* x2, const double* y2, gel_stats* o
 {
     if (!c || !x2 || !y2) return fail(GEL_E_ARG, "null argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
+    if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
     const size_t nb = c->nn * sizeof(double);
     HIP_OK(hipMemcpyAsync(c->d_x2, x2, nb, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->d_y2, y2, nb, hipMemcpyHostToDevice, c->stream));
-    const int st = one_step(c, true);
+    const int st = one_step(c, 1);
     if (st != GEL_OK) return st;
     std::swap(c->d_x, c->d_x2);       // Point::x = x2 after the step (Setka.cpp:10473-10480)
     std::swap(c->d_y, c->d_y2);
@@ -868,8 +1098,8 @@ int gel_kernel_timing(gel_ctx* c, double* avg_ms, long long* n)
         float ms = 0.0f;
         HIP_OK(hipEventElapsedTime(&ms, e.first, e.second));
         tot += ms;
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
     }
     *n = (long long)c->events.size();
     *avg_ms = c->events.empty() ? 0.0 : tot / (double)c->events.size();
@@ -884,37 +1114,31 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
     if (n < 0 || (n > 0 && (!L || !R || !W || !n1 || !n2 || !rad || !P || !PQ || !Vl || !Vc || !Vp || !dt)))
         return fail(GEL_E_ARG, "null argument");
     if (n == 0) return GEL_OK;
-    double *dL = nullptr, *dR = nullptr, *dW = nullptr, *d1 = nullptr, *d2 = nullptr, *dr = nullptr, *dout = nullptr;
-    auto cleanup = [&]() { hipFree(dL); hipFree(dR); hipFree(dW); hipFree(d1); hipFree(d2); hipFree(dr); hipFree(dout); };
-    const size_t s = (size_t)n * sizeof(double);
-    if (hipMalloc(&dL, 5 * s) != hipSuccess || hipMalloc(&dR, 5 * s) != hipSuccess || hipMalloc(&dW, s) != hipSuccess ||
-        hipMalloc(&d1, s) != hipSuccess || hipMalloc(&d2, s) != hipSuccess || hipMalloc(&dr, s) != hipSuccess ||
-        hipMalloc(&dout, 9 * s) != hipSuccess) {
-        cleanup();
-        return fail(GEL_E_NOMEM, "flux batch buffers");
-    }
-    hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = hipMemcpy(dL, L, 5 * s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dR, R, 5 * s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dW, W, s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d1, n1, s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d2, n2, s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dr, rad, s, hipMemcpyHostToDevice);
-    if (e == hipSuccess) {
-        hipLaunchKernelGGL(flux_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, dL, dR, dW, d1, d2, dr,
-                           cfl, gamma, metod, nul_potok, dout);
-        e = hipGetLastError();
-    }
     std::vector<double> o((size_t)9 * n);
-    if (e == hipSuccess) e = hipMemcpy(o.data(), dout, 9 * s, hipMemcpyDeviceToHost);
-    cleanup();
-    if (e != hipSuccess) return fail(GEL_E_HIP, std::string("flux batch: ") + hipGetErrorString(e));
+    const int st = batch_run(n, { { L, 5 }, { R, 5 }, { W, 1 }, { n1, 1 }, { n2, 1 }, { rad, 1 } }, { { o.data(), 9 } },
+                             [&](std::vector<double*>& i, std::vector<double*>& d) {
+                                 hipLaunchKernelGGL(flux_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, i[0],
+                                                    i[1], i[2], i[3], i[4], i[5], cfl, gamma, metod, nul_potok, d[0]);
+                             });
+    if (st != GEL_OK) return st;
     for (int k = 0; k < n; k++) {
         const double* q = &o[(size_t)9 * k];
         P[4 * k] = q[0]; P[4 * k + 1] = q[1]; P[4 * k + 2] = q[2]; P[4 * k + 3] = q[3];
         PQ[k] = q[4]; Vl[k] = q[5]; Vc[k] = q[6]; Vp[k] = q[7]; dt[k] = q[8];
     }
     return GEL_OK;
+}
+
+int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
+                        const double* n2, double gamma, double* P, double* smax)
+{
+    if (n < 0 || (n > 0 && (!L || !R || !W || !n1 || !n2 || !P || !smax))) return fail(GEL_E_ARG, "null argument");
+    if (n == 0) return GEL_OK;
+    return batch_run(n, { { L, 9 }, { R, 9 }, { W, 1 }, { n1, 1 }, { n2, 1 } }, { { P, 9 }, { smax, 1 } },
+                     [&](std::vector<double*>& i, std::vector<double*>& d) {
+                         hipLaunchKernelGGL(hlld_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, i[0], i[1],
+                                            i[2], i[3], i[4], gamma, d[0], d[1]);
+                     });
 }
 
 }  // extern "C"

@@ -274,3 +274,146 @@ GEL_HD void source_mk1(const gel_params& prm, F f, double u, double v, double ro
 }
 
 }  // namespace gel
+
+namespace gel {
+
+// HLLD of Solvers::HLLDQ_Korolkov metod 3 (Solvers.cpp:1642-2185) for an in-plane
+// normal n = (n1, n2, 0) and a face moving with speed W along n: the fan is sampled
+// at xi = W and each region's flux is taken minus W*U, as the gas flux does
+// (Setka.cpp:10569-10618).  Basis t = (-n2, n1, 0), m = (0, 0, 1).  Primitive input
+// q = (ro, p, u, v, w, bx, by, bz, Q), B Gaussian.  P: mass, momentum x/y/phi,
+// energy, B x/y/phi, tracer.  Returns max(|SL|,|SR|).
+template <class T>
+GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
+{
+    const T spi4 = T(3.5449077018110318);    // sqrt(4 pi), Solvers.h:9-12
+    const T g1 = gam - T(1);
+    const T t1 = -n2, t2 = n1;
+    const T ro_L = L[0], p_L = L[1], Q_L = L[8], ro_R = R[0], p_R = R[1], Q_R = R[8];
+    const T bx_L = L[5] / spi4, by_L = L[6] / spi4, bz_L = L[7] / spi4;
+    const T bx_R = R[5] / spi4, by_R = R[6] / spi4, bz_R = R[7] / spi4;
+    // three-term dot products with the zero components kept, as the reference forms them
+    const T z = T(0), o = T(1);
+    const T u1 = L[2] * n1 + L[3] * n2 + L[4] * z;
+    const T v1 = L[2] * t1 + L[3] * t2 + L[4] * z;
+    const T w1 = L[2] * z + L[3] * z + L[4] * o;
+    const T u2 = R[2] * n1 + R[3] * n2 + R[4] * z;
+    const T v2 = R[2] * t1 + R[3] * t2 + R[4] * z;
+    const T w2 = R[2] * z + R[3] * z + R[4] * o;
+    const T bn1 = bx_L * n1 + by_L * n2 + bz_L * z;
+    const T bt1 = bx_L * t1 + by_L * t2 + bz_L * z;
+    const T bm1 = bx_L * z + by_L * z + bz_L * o;
+    const T bn2 = bx_R * n1 + by_R * n2 + bz_R * z;
+    const T bt2 = bx_R * t1 + by_R * t2 + bz_R * z;
+    const T bm2 = bx_R * z + by_R * z + bz_R * o;
+    const T bb_L = sq(bx_L) + sq(by_L) + sq(bz_L);
+    const T bb_R = sq(bx_R) + sq(by_R) + sq(bz_R);
+    const T uu_L = (sq(L[2]) + sq(L[3]) + sq(L[4])) / T(2);
+    const T uu_R = (sq(R[2]) + sq(R[3]) + sq(R[4])) / T(2);
+    const T cfL = sqrt((gam * p_L + bb_L + sqrt(sq(gam * p_L + bb_L) - T(4) * gam * p_L * sq(bn1))) / (T(2) * ro_L));
+    const T cfR = sqrt((gam * p_R + bb_R + sqrt(sq(gam * p_R + bb_R) - T(4) * gam * p_R * sq(bn2))) / (T(2) * ro_R));
+    const T SL = mn(u1, u2) - mx(cfL, cfR);
+    const T SR = mx(u1, u2) + mx(cfL, cfR);
+    const T pTL = p_L + bb_L / T(2);
+    const T pTR = p_R + bb_R / T(2);
+    const T suR = SR - u2, suL = SL - u1;
+    const T SM = (suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL) / (suR * ro_R - suL * ro_L);
+    const T PTT = (suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) / (suR * ro_R - suL * ro_L);
+    const T UU = mx(fabs(SL), fabs(SR));
+    const T e1 = p_L / g1 + ro_L * uu_L + bb_L / T(2);
+    const T e2 = p_R / g1 + ro_R * uu_R + bb_R / T(2);
+    const T FL[9] = { ro_L * u1, ro_L * u1 * u1 + pTL - sq(bn1), ro_L * u1 * v1 - bn1 * bt1, ro_L * u1 * w1 - bn1 * bm1,
+                      (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1), T(0), u1 * bt1 - v1 * bn1,
+                      u1 * bm1 - w1 * bn1, Q_L * u1 };
+    const T FR[9] = { ro_R * u2, ro_R * u2 * u2 + pTR - sq(bn2), ro_R * u2 * v2 - bn2 * bt2, ro_R * u2 * w2 - bn2 * bm2,
+                      (e2 + pTR) * u2 - bn2 * (u2 * bn2 + v2 * bt2 + w2 * bm2), T(0), u2 * bt2 - v2 * bn2,
+                      u2 * bm2 - w2 * bn2, Q_R * u2 };
+    const T UL[9] = { ro_L, ro_L * u1, ro_L * v1, ro_L * w1, e1, bn1, bt1, bm1, Q_L };
+    const T UR[9] = { ro_R, ro_R * u2, ro_R * v2, ro_R * w2, e2, bn2, bt2, bm2, Q_R };
+    const T bn = (SR * UR[5] - SL * UL[5] + FL[5] - FR[5]) / (SR - SL);
+    const T bbn = bn * bn;
+    const T ro_LL = ro_L * (SL - u1) / (SL - SM);
+    const T ro_RR = ro_R * (SR - u2) / (SR - SM);
+    const T Q_LL = Q_L * (SL - u1) / (SL - SM);
+    const T Q_RR = Q_R * (SR - u2) / (SR - SM);
+    const T ttL = ro_L * suL * (SL - SM) - bbn;
+    const T ttR = ro_R * suR * (SR - SM) - bbn;
+    T vLL = v1, wLL = w1, btt1 = T(0), bmm1 = T(0);
+    T vRR = v2, wRR = w2, btt2 = T(0), bmm2 = T(0);
+    if (fabs(ttL) >= T(0.000001)) {
+        vLL = v1 - bn * bt1 * (SM - u1) / ttL;
+        wLL = w1 - bn * bm1 * (SM - u1) / ttL;
+        btt1 = bt1 * (ro_L * suL * suL - bbn) / ttL;
+        bmm1 = bm1 * (ro_L * suL * suL - bbn) / ttL;
+    }
+    if (fabs(ttR) >= T(0.000001)) {
+        vRR = v2 - bn * bt2 * (SM - u2) / ttR;
+        wRR = w2 - bn * bm2 * (SM - u2) / ttR;
+        btt2 = bt2 * (ro_R * suR * suR - bbn) / ttR;
+        bmm2 = bm2 * (ro_R * suR * suR - bbn) / ttR;
+    }
+    const T eLL = (e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) / (SL - SM);
+    const T eRR = (e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) / (SR - SM);
+    const T sLL = sqrt(ro_LL), sRR = sqrt(ro_RR);
+    T SLL = SM - fabs(bn) / sLL;
+    T SRR = SM + fabs(bn) / sRR;
+    T idbn;
+    if (fabs(bn) > T(0.001)) {
+        idbn = T(1) * sgn(bn);
+    } else {
+        idbn = T(0); SLL = SM; SRR = SM;
+    }
+    const T vLLL = (sLL * vLL + sRR * vRR + idbn * (btt2 - btt1)) / (sLL + sRR);
+    const T wLLL = (sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1)) / (sLL + sRR);
+    const T bttt = (sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL)) / (sLL + sRR);
+    const T bmmm = (sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL)) / (sLL + sRR);
+    const T eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+    const T eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+    T PO[9];
+    T bnW;
+    if (SL >= W) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] - W * UL[i];
+        bnW = UL[5];
+    } else if (SLL >= W) {
+        const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
+        bnW = bn;
+    } else if (SM >= W) {
+        const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
+        const T ULLL[9] = { ro_LL, ro_LL * SM, ro_LL * vLLL, ro_LL * wLLL, eLLL, bn, bttt, bmmm, Q_LL };
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
+        bnW = bn;
+    } else if (SRR > W) {
+        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+        const T URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
+        bnW = bn;
+    } else if (SR > W) {
+        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
+        bnW = bn;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] - W * UR[i];
+        bnW = UR[5];
+    }
+    const T SN = mx(fabs(SL), fabs(SR));
+    PO[5] = -SN * (bn2 - bn1) - W * bnW;
+    P[0] = PO[0];
+    P[1] = n1 * PO[1] + t1 * PO[2] + z * PO[3];
+    P[2] = n2 * PO[1] + t2 * PO[2] + z * PO[3];
+    P[3] = z * PO[1] + z * PO[2] + o * PO[3];
+    P[4] = PO[4];
+    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6] + z * PO[7]);
+    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6] + z * PO[7]);
+    P[7] = spi4 * (z * PO[5] + z * PO[6] + o * PO[7]);
+    P[8] = PO[8];
+    return UU;
+}
+
+}  // namespace gel

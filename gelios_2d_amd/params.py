@@ -16,6 +16,7 @@ GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
 GEL_PHYS_GAS, GEL_PHYS_MHD = 0, 1
 NVAR_MHD = 9
 GEL_SRC_NONE, GEL_SRC_MK1 = 0, 1
+GEL_PREC_FP64, GEL_PREC_FP32_BULK = 0, 1
 GEL_MK1_NFIELDS = 19
 NVAR_GAS = 5
 
@@ -40,7 +41,7 @@ class GelParams(ctypes.Structure):
         ("flux", ctypes.c_int),
         ("physics", ctypes.c_int),
         ("source_mode", ctypes.c_int),
-        ("reserved0", ctypes.c_int),
+        ("precision", ctypes.c_int),
         ("in_rho_div", ctypes.c_double),
         ("in_p_a", ctypes.c_double),
         ("in_p_den", ctypes.c_double),

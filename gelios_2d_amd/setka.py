@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .grid import PolarGrid
-from .params import GEL_MK1_NFIELDS, NVAR_GAS, GelParams
+from .params import GEL_MK1_NFIELDS, GEL_PHYS_MHD, NVAR_GAS, NVAR_MHD, GelParams
 
 
 class Setka:
@@ -33,6 +33,7 @@ class Setka:
                          _lib.i8ptr(grid.cell_type), _lib.i8ptr(grid.outer_type))
         _lib.check(_lib.lib().gel_ctx_create(ctypes.byref(self._ctx), ctypes.byref(g), ctypes.byref(self.params)))
         self.stats = _lib.GelStats()
+        self.nvar = NVAR_MHD if self.params.physics == GEL_PHYS_MHD else NVAR_GAS
 
     def close(self):
         if self._ctx:
@@ -51,17 +52,22 @@ class Setka:
 
     # -- state ------------------------------------------------------------------
     def upload(self, prim: np.ndarray) -> None:
-        a = np.ascontiguousarray(prim, np.float64).reshape(NVAR_GAS, self.ncells)
+        a = np.ascontiguousarray(prim, np.float64).reshape(self.nvar, self.ncells)
         _lib.check(_lib.lib().gel_upload_state(self._ctx, _lib.dptr(a)))
 
     def download(self) -> np.ndarray:
-        out = np.empty((NVAR_GAS, self.ncells), np.float64)
+        out = np.empty((self.nvar, self.ncells), np.float64)
         _lib.check(_lib.lib().gel_download_state(self._ctx, _lib.dptr(out)))
         return out
 
     def set_sources(self, fields: np.ndarray) -> None:
         a = np.ascontiguousarray(fields, np.float64).reshape(GEL_MK1_NFIELDS, self.ncells)
         _lib.check(_lib.lib().gel_set_sources(self._ctx, _lib.dptr(a)))
+
+    def set_motion(self, amp: float, period: int, phase: float) -> None:
+        """Prescribed breathing of the grid: nodes at base * (1 + amp sin(2 pi k/period
+        + phase)), k = step mod period; steps become moving steps."""
+        _lib.check(_lib.lib().gel_set_motion(self._ctx, float(amp), int(period), float(phase)))
 
     def set_rows_per_wave(self, rows: int) -> None:
         _lib.check(_lib.lib().gel_set_rows_per_wave(self._ctx, int(rows)))
@@ -95,6 +101,22 @@ class Setka:
         n = ctypes.c_longlong()
         _lib.check(_lib.lib().gel_kernel_timing(self._ctx, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def HLLD_flux(L, R, W, n1, n2, gamma=5.0 / 3.0):
+    """Batched device MHD face flux (HLLD after Solvers::HLLDQ_Korolkov metod 3,
+    Solvers.cpp:1642-2185, any in-plane normal, face speed W).  L, R: [n, 9]
+    primitives (ro, p, u, v, w, bx, by, bz, Q).  Returns P [n, 9] (mass, momentum
+    x/y/phi, energy, B x/y/phi, tracer) and max|S| [n]."""
+    L = np.ascontiguousarray(L, np.float64).reshape(-1, 9)
+    R = np.ascontiguousarray(R, np.float64).reshape(-1, 9)
+    n = L.shape[0]
+    vec = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,))) for a in (W, n1, n2)]
+    P = np.empty((n, 9))
+    smax = np.empty(n)
+    _lib.check(_lib.lib().gel_flux_hlld_batch(n, _lib.dptr(L), _lib.dptr(R), *[_lib.dptr(v) for v in vec],
+                                              float(gamma), _lib.dptr(P), _lib.dptr(smax)))
+    return P, smax
 
 
 def HLLC_2d_Korolkov_b_s(L, R, W, n1, n2, rad, metod, nul_potok=False, cfl=0.1, gamma=5.0 / 3.0):

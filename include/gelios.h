@@ -77,6 +77,12 @@ enum gel_flux { GEL_FLUX_HLL = 0, GEL_FLUX_HLLC = 1 };
  * moving faces; B in Gaussian units as there (b = B/sqrt(4 pi)). */
 enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1 };
 
+/* FP64: every operation in FP64 (the reference's arithmetic).
+ * FP32_BULK: state stored in FP32 and face reconstruction + Riemann solver in FP32;
+ * geometry (normals, areas, swept volumes), the accumulation of face fluxes into cells
+ * and the update with primitive recovery stay FP64. */
+enum gel_precision { GEL_PREC_FP64 = 0, GEL_PREC_FP32_BULK = 1 };
+
 enum gel_source_mode {
     GEL_SRC_NONE = 0,
     /* Charge exchange with 4 frozen hydrogen populations, Cell::Get_Sourse_MK1
@@ -97,7 +103,7 @@ typedef struct gel_params {
     int flux;           /* enum gel_flux */
     int physics;        /* enum gel_physics */
     int source_mode;    /* enum gel_source_mode */
-    int reserved0;
+    int precision;      /* enum gel_precision */
     /* Inner_sphere ghost state at face centre distance d (Gran.cpp:234-265):
        ro = 1/(in_rho_div*d*d), p = ro*in_p_a*in_p_a/in_p_den,
        (u,v) = in_u*(x,y)/d, Q = ro*in_q. */
@@ -165,6 +171,13 @@ int gel_step(gel_ctx* ctx, int nsteps, gel_stats* out);
  * and the step ends with Point::x = x2 (Setka.cpp:10473-10480). */
 int gel_step_moving(gel_ctx* ctx, const double* x2, const double* y2, gel_stats* out);
 
+/* Prescribed node motion (no counterpart in the reference, whose Move_Setka_Calculate_2
+ * redistributes rails, Setka.cpp:4324-4759): every node sits at its grid position
+ * times s_k = 1 + amp*sin(2 pi k/period + phase), k = step mod period, so radii
+ * breathe by amp; face speeds and swept volumes follow (Setka.cpp:10187-10209).
+ * gel_step then runs moving steps. */
+int gel_set_motion(gel_ctx* ctx, double amp, int period, double phase);
+
 /* Override the time step the next step uses (T[now1], Setka.cpp:10143-10145). */
 int gel_set_dt(gel_ctx* ctx, double dt);
 
@@ -190,6 +203,13 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
                           double cfl, double gamma, int metod, int nul_potok,
                           double* P, double* PQ, double* Vl, double* Vc, double* Vp,
                           double* dt);
+
+/* Batched device evaluation of the MHD face flux (HLLD after Solvers::HLLDQ_Korolkov
+ * metod 3, Solvers.cpp:1642-2185, any in-plane normal, face speed W).  L, R: [n][9]
+ * primitives (ro, p, u, v, w, bx, by, bz, Q); P: [n][9] = mass, momentum x/y/phi,
+ * energy, B x/y/phi, tracer; smax: [n] max(|SL|,|SR|). */
+int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
+                        const double* n2, double gamma, double* P, double* smax);
 
 const char* gel_strerror(int status);
 const char* gel_last_error(void);

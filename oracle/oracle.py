@@ -165,6 +165,22 @@ class OracleSweep:
         self.last_dt_new = dt_new.value
         return nxt, st
 
+    def run_motion(self, prim, steps, x0, y0, amp, period, phase, nthreads=0):
+        """Steps on the prescribed breathing grid: before step n the nodes are at
+        base * s[n mod period], after it at base * s[(n+1) mod period]."""
+        from gelios_2d_amd.configs import motion_scales
+        s = motion_scales(amp, period, phase)
+        x0 = np.asarray(x0, np.float64)
+        y0 = np.asarray(y0, np.float64)
+        cur = np.ascontiguousarray(prim, np.float64).reshape(self.nv, self.nc)
+        for n in range(steps):
+            so, sn = s[n % period], s[(n + 1) % period]
+            self.set_nodes(x0 * so, y0 * so, x0 * sn, y0 * sn)
+            cur, st = self.step(cur, nthreads)
+            if st != 0:
+                raise RuntimeError(f"oracle step failed with status {st}")
+        return cur
+
     def run(self, prim, steps, nthreads=0):
         cur = np.ascontiguousarray(prim, np.float64).reshape(self.nv, self.nc)
         for _ in range(steps):

@@ -4,7 +4,9 @@ Tolerances: the device rounds +,-,*,/ and sqrt like the host (IEEE, no FMA
 contraction), so flux vectors agree to the last bit except where pow/atan/sin/cos/log
 of the device library differ from glibc by an ulp (radially rescaled C_1 cells,
 charge-exchange log).  Whole-sweep results are held to 1e-12 relative per conserved
-variable after N steps, the bar of the task.
+variable (ro, ro u, ro v, E, Q) after N steps; primitive pressure is not compared
+directly, because p = (gamma-1)(E - ro u^2/2) loses digits by cancellation in the
+Mach-10 wind.
 """
 import numpy as np
 import pytest
@@ -18,11 +20,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def rel_err(a, b):
-    """max over variables of max|a-b| / max|b| (per conserved variable scale)"""
-    a = np.asarray(a).reshape(5, -1)
-    b = np.asarray(b).reshape(5, -1)
-    return float(np.max(np.abs(a - b).max(axis=1) / np.maximum(np.abs(b).max(axis=1), 1e-300)))
+def rel_err(a, b, gamma=5.0 / 3.0):
+    """max over conserved variables of max|a-b| / max|b|"""
+    from gelios_2d_amd.state import rel_err_conserved
+    return rel_err_conserved(a, b, gamma)
 
 
 def test_flux_batch_matches_reference_vectors():
