@@ -139,33 +139,66 @@ __device__ __forceinline__ double dist2d(double ax, double ay, double bx, double
     return sqrt(sq(ax - bx) + sq(ay - by));
 }
 
+// Limited linear value at the face (Help.cpp:96-113 with y = 0, x1 = -d1, x2 = -d2,
+// x3 = d3): t2 + minmod((t1-t2)/(x1-x2), (t2-t3)/(x2-x3)) * (0 - x2).  FP64 keeps the
+// reference's divisions; FP32 bulk multiplies by the reciprocals of the two
+// denominators, which all primitives of a face side share, and assembles the face
+// value in FP64 from the exact cell value t2, so that only the small slope correction
+// carries FP32 rounding.
+struct Lin3 {
+    double den12, den23, d2;   // x1 - x2, x2 - x3, 0 - x2
+};
+__device__ __forceinline__ Lin3 lin3_geom(double d1, double d2, double d3)
+{
+    return Lin3{ -d1 - (-d2), -d2 - d3, 0.0 - (-d2) };
+}
+template <class TF>
+__device__ __forceinline__ double lin3r(double t1, double t2, double t3, const Lin3& g, TF i12, TF i23)
+{
+    if constexpr (sizeof(TF) == 8) {
+        const double d = minmod<double>((t1 - t2) / g.den12, (t2 - t3) / g.den23);
+        return d * g.d2 + t2;
+    } else {
+        const float d = minmod<float>((float(t1) - float(t2)) * i12, (float(t2) - float(t3)) * i23);
+        return double(d * float(g.d2)) + t2;
+    }
+}
+template <class TF> __device__ __forceinline__ TF recip(double x)
+{
+    if constexpr (sizeof(TF) == 8) return 0.0; else return float(1.0 / x);
+}
+
 // Gran::Get_par_TVD, Usualy face (Gran.cpp:294-457): master M's value at face centre
 // (fx,fy) from the line (D, M, N).  D.ok == 0: mirrored ghost across the axis when M
-// touches it (:297-339), else M itself (:340-343).  Distances are FP64 geometry; the
-// limiter runs in the flux type TF.
+// touches it (:297-339), else M itself (:340-343).  Distances are FP64 geometry.
 template <class Ph, class TS, class TF>
 __device__ __forceinline__ void tvd_usualy(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
-                                           bool m_axis, double fx, double fy, TF* r, int& bad)
+                                           bool m_axis, double fx, double fy, double* r, int& bad)
 {
     if (!D.ok) {
 #pragma unroll
-        for (int k = 0; k < Ph::NV; k++) r[k] = TF(M.s[k]);
+        for (int k = 0; k < Ph::NV; k++) r[k] = double(M.s[k]);
         if (m_axis) {
-            const TF d2 = TF(dist2d(fx, fy, M.cx, M.cy));
-            const TF d3 = TF(dist2d(fx, fy, N.cx, N.cy));
-            const TF d1 = TF(dist2d(fx, fy, M.cx, -M.cy));
+            const double d2 = dist2d(fx, fy, M.cx, M.cy);
+            const double d3 = dist2d(fx, fy, N.cx, N.cy);
+            const double d1 = dist2d(fx, fy, M.cx, -M.cy);
+            const Lin3 g = lin3_geom(d1, d2, d3);
+            const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
 #pragma unroll
             for (int k = 0; k < Ph::NV; k++)
-                if (Ph::odd(k)) r[k] = linear3<TF>(-d1, -TF(M.s[k]), -d2, TF(M.s[k]), d3, TF(N.s[k]), TF(0));
+                if (Ph::odd(k)) r[k] = lin3r<TF>(-double(M.s[k]), double(M.s[k]), double(N.s[k]), g, i12, i23);
         }
         return;
     }
-    const TF d2 = TF(dist2d(fx, fy, M.cx, M.cy));
-    const TF d3 = TF(dist2d(fx, fy, N.cx, N.cy));
-    const TF d1 = TF(dist2d(fx, fy, D.cx, D.cy));
+    const double d2 = dist2d(fx, fy, M.cx, M.cy);
+    const double d3 = dist2d(fx, fy, N.cx, N.cy);
+    const double d1 = dist2d(fx, fy, D.cx, D.cy);
+    const Lin3 g = lin3_geom(d1, d2, d3);
+    const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
 #pragma unroll
-    for (int k = 0; k < Ph::NV; k++) r[k] = linear3<TF>(-d1, TF(D.s[k]), -d2, TF(M.s[k]), d3, TF(N.s[k]), TF(0));
-    if (r[0] <= TF(0) || r[1] <= TF(0)) bad = 1;
+    for (int k = 0; k < Ph::NV; k++)
+        r[k] = lin3r<TF>(double(D.s[k]), double(M.s[k]), double(N.s[k]), g, i12, i23);
+    if (r[0] <= 0.0 || r[1] <= 0.0) bad = 1;
 }
 
 // Gran::Get_par_TVD_radial, density and pressure (Gran.cpp:562-675); FP64 gas only.
@@ -208,7 +241,7 @@ __device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, dou
 // Value of cell M on its side of an interior face (Setka.cpp:10213-10251).
 template <class Ph, class TS, class TF, bool TYPES>
 __device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
-                                           bool m_axis, double fx, double fy, double dis, double gam, TF* r, int& bad)
+                                           bool m_axis, double fx, double fy, double dis, double gam, double* r, int& bad)
 {
     if constexpr (TYPES) {
         if (M.type == GEL_C_1) {
@@ -254,19 +287,72 @@ __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, doubl
     return f;
 }
 
-// Face flux from left/right primitives; P in the order of the cell accumulators.
+// Face flux from left/right face primitives (FP64); P in the order of the cell
+// accumulators.  TF = double: the reference arithmetic.  TF = float (FP32 bulk): the
+// wave speeds are estimated in FP32; a supersonic face (the whole solar wind) takes
+// the exact FP64 upwind flux, a subsonic one the FP32 Riemann solver.  Rounding the
+// flux of a steady supersonic stream to FP32 would bias every step the same way.
 template <class Ph, class TF, int FLUX>
-__device__ __forceinline__ TF face_flux(const TF* L, const TF* R, const Face& f, TF gam, TF* P)
+__device__ __forceinline__ TF face_flux(const double* L, const double* R, const Face& f, double gam, double* P)
 {
     if constexpr (Ph::NV == 5) {
-        const Gas<TF> gl{ L[0], L[1], L[2], L[3], L[4] };
-        const Gas<TF> gr{ R[0], R[1], R[2], R[3], R[4] };
-        TF PQ;
-        const TF UU = hllc2d<TF, FLUX == GEL_FLUX_HLLC>(gl, gr, TF(f.W), TF(f.n1), TF(f.n2), gam, P, PQ);
-        P[4] = PQ;
-        return UU;
+        const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
+        const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
+        if constexpr (sizeof(TF) == 8) {
+            double PQ;
+            const double UU = hllc2d<double, FLUX == GEL_FLUX_HLLC>(gl, gr, f.W, f.n1, f.n2, gam, P, PQ);
+            P[4] = PQ;
+            return UU;
+        } else {
+            const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
+            const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
+            const float cL = sqrtf(g * float(L[1]) / float(L[0])), cR = sqrtf(g * float(R[1]) / float(R[0]));
+            const float SL = mn(u1, u2) - mx(cL, cR), SR = mx(u1, u2) + mx(cL, cR);
+            if (SL >= W || SR <= W) {
+                gas_upwind<double>(SL >= W ? gl : gr, f.W, f.n1, f.n2, gam, P);
+                return mx(fabsf(SL), fabsf(SR));
+            }
+            const Gas<float> fl{ float(L[0]), float(L[1]), float(L[2]), float(L[3]), float(L[4]) };
+            const Gas<float> fr{ float(R[0]), float(R[1]), float(R[2]), float(R[3]), float(R[4]) };
+            float Pf[4], PQ;
+            const float UU = hllc2d<float, FLUX == GEL_FLUX_HLLC>(fl, fr, W, n1, n2, g, Pf, PQ);
+#pragma unroll
+            for (int k = 0; k < 4; k++) P[k] = Pf[k];
+            P[4] = PQ;
+            return UU;
+        }
     } else {
-        return hlld<TF>(L, R, TF(f.W), TF(f.n1), TF(f.n2), gam, P);
+        if constexpr (sizeof(TF) == 8) {
+            return hlld<double>(L, R, f.W, f.n1, f.n2, gam, P);
+        } else {
+            const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
+            const float spi4 = 3.5449077018110318f;
+            const float bnL = (float(L[5]) * n1 + float(L[6]) * n2) / spi4;
+            const float bnR = (float(R[5]) * n1 + float(R[6]) * n2) / spi4;
+            const float bbL = (sq(float(L[5])) + sq(float(L[6])) + sq(float(L[7]))) / (spi4 * spi4);
+            const float bbR = (sq(float(R[5])) + sq(float(R[6])) + sq(float(R[7]))) / (spi4 * spi4);
+            const float gpL = g * float(L[1]), gpR = g * float(R[1]);
+            const float cfL = sqrtf((gpL + bbL + sqrtf(fmaxf(sq(gpL + bbL) - 4.0f * gpL * sq(bnL), 0.0f))) / (2.0f * float(L[0])));
+            const float cfR = sqrtf((gpR + bbR + sqrtf(fmaxf(sq(gpR + bbR) - 4.0f * gpR * sq(bnR), 0.0f))) / (2.0f * float(R[0])));
+            const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
+            const float SL = mn(u1, u2) - mx(cfL, cfR), SR = mx(u1, u2) + mx(cfL, cfR);
+            const float SN = mx(fabsf(SL), fabsf(SR));
+            if (SL >= W) {
+                mhd_upwind<double>(L, R, f.W, f.n1, f.n2, gam, double(SN), true, P);
+                return SN;
+            }
+            if (SR <= W) {
+                mhd_upwind<double>(R, L, f.W, f.n1, f.n2, gam, double(SN), false, P);
+                return SN;
+            }
+            float fl[9], fr[9], Pf[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) { fl[k] = float(L[k]); fr[k] = float(R[k]); }
+            const float UU = hlld<float>(fl, fr, W, n1, n2, g, Pf);
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = Pf[k];
+            return UU;
+        }
     }
 }
 
@@ -275,12 +361,12 @@ __device__ __forceinline__ TF face_flux(const TF* L, const TF* R, const Face& f,
 // the face (Sosed_down), used on the axis.
 template <class Ph, class TS, class TF, bool TYPES, int FLUX>
 __device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const CellT<Ph, TS>& M,
-                                            const CellT<Ph, TS>& D, const Face& f, TF* P)
+                                            const CellT<Ph, TS>& D, const Face& f, double* P)
 {
     constexpr int NV = Ph::NV;
     const gel_params& p = a.prm;
     const double gam = p.gamma;
-    TF L[NV], R[NV];
+    double L[NV], R[NV];
     const double radius = sqrt(sq(M.cx) + sq(M.cy));
     if (ftype == GEL_F_INNER_SPHERE) {
         double g[NV];
@@ -298,7 +384,7 @@ __device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const
         w[3] = p.in_u * f.fy / d;
         w[NV - 1] = ro * p.in_q;
 #pragma unroll
-        for (int k = 0; k < NV; k++) { L[k] = TF(g[k]); R[k] = TF(w[k]); }
+        for (int k = 0; k < NV; k++) { L[k] = g[k]; R[k] = w[k]; }
     } else if (ftype == GEL_F_AXIS) {
         bool done = false;
         if constexpr (TYPES) {
@@ -308,46 +394,48 @@ __device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const
                 for (int k = 0; k < NV; k++) g[k] = M.s[k];
                 rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
 #pragma unroll
-                for (int k = 0; k < NV; k++) L[k] = TF(g[k]);
+                for (int k = 0; k < NV; k++) L[k] = g[k];
                 done = true;
             }
         }
         if (!done) {                                          // Gran.cpp:486-519
-            const TF d2 = TF(dist2d(f.fx, f.fy, M.cx, M.cy));
-            const TF d1 = TF(dist2d(f.fx, f.fy, D.cx, D.cy));
+            const double d2 = dist2d(f.fx, f.fy, M.cx, M.cy);
+            const double d1 = dist2d(f.fx, f.fy, D.cx, D.cy);
+            const Lin3 g = lin3_geom(d1, d2, d2);
+            const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
 #pragma unroll
             for (int k = 0; k < NV; k++) {
-                L[k] = TF(M.s[k]);
-                if (Ph::odd(k)) L[k] = linear3<TF>(-d1, TF(D.s[k]), -d2, TF(M.s[k]), d2, -TF(M.s[k]), TF(0));
+                L[k] = double(M.s[k]);
+                if (Ph::odd(k)) L[k] = lin3r<TF>(double(D.s[k]), double(M.s[k]), -double(M.s[k]), g, i12, i23);
             }
         }
 #pragma unroll
         for (int k = 0; k < NV; k++) R[k] = Ph::odd(k) ? -L[k] : L[k];
     } else {
 #pragma unroll
-        for (int k = 0; k < NV; k++) { L[k] = TF(M.s[k]); R[k] = L[k]; }
+        for (int k = 0; k < NV; k++) { L[k] = double(M.s[k]); R[k] = L[k]; }
         if (ftype == GEL_F_INPUT) {
-            R[0] = TF(p.inf_rho); R[1] = TF(p.inf_p); R[2] = TF(p.inf_u); R[3] = TF(p.inf_v);
-            R[NV - 1] = TF(p.inf_q);
+            R[0] = p.inf_rho; R[1] = p.inf_p; R[2] = p.inf_u; R[3] = p.inf_v;
+            R[NV - 1] = p.inf_q;
             if constexpr (NV == 9) {
-                R[4] = TF(p.inf_w); R[5] = TF(p.inf_bx); R[6] = TF(p.inf_by); R[7] = TF(p.inf_bz);
+                R[4] = p.inf_w; R[5] = p.inf_bx; R[6] = p.inf_by; R[7] = p.inf_bz;
             }
         } else if (ftype == GEL_F_EXTERN) {
-            if (R[2] >= TF(0)) R[2] = TF(p.extern_u);
+            if (R[2] >= 0.0) R[2] = p.extern_u;
         }
     }
-    return face_flux<Ph, TF, FLUX>(L, R, f, TF(gam), P);
+    return face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
 }
 
 template <class Ph, class TS, class TF, bool TYPES, int FLUX>
 __device__ __forceinline__ TF interior_flux(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
                                             const CellT<Ph, TS>& U, bool m_axis, bool n_axis, const Face& f,
-                                            double gam, TF* P, int& bad)
+                                            double gam, double* P, int& bad)
 {
-    TF L[Ph::NV], R[Ph::NV];
+    double L[Ph::NV], R[Ph::NV];
     side_value<Ph, TS, TF, TYPES>(M, N, D, m_axis, f.fx, f.fy, f.dis, gam, L, bad);
     side_value<Ph, TS, TF, TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, R, bad);
-    return face_flux<Ph, TF, FLUX>(L, R, f, TF(gam), P);
+    return face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
 }
 
 // Conservative ALE update of one cell and primitive recovery (Setka.cpp:10423-10460),
@@ -451,7 +539,7 @@ sweep_kernel(SweepArgs a)
 
     int bad = 0;
     // flux through the radial face below row i0, oriented along +i
-    TF flo[NV];
+    double flo[NV];
     TF uulo;
     {
         const Nodes q = nr[0];
@@ -460,7 +548,7 @@ sweep_kernel(SweepArgs a)
         if (i0 == 0) {
             // inner sphere, edge 3 of cell (0, j): (0, j+1) -> (0, j)
             const Face f = face_geom<MOVING>(q.xb, q.yb, q.xa, q.ya, qn.xb, qn.yb, qn.xa, qn.ya, dt);
-            TF P[NV];
+            double P[NV];
             uulo = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_INNER_SPHERE, w[1], w[2], f, P);
 #pragma unroll
             for (int k = 0; k < NV; k++) flo[k] = -P[k];
@@ -482,7 +570,8 @@ sweep_kernel(SweepArgs a)
         }
         // edge 1: radial face (i, i+1), (i+1, j) -> (i+1, j+1)
         const Face f1 = face_geom<MOVING>(q1.xa, q1.ya, q1.xb, q1.yb, q1n.xa, q1n.ya, q1n.xb, q1n.yb, dt);
-        TF P1[NV], uu1;
+        double P1[NV];
+        TF uu1;
         if (i + 1 < a.ni) {
             uu1 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], w[2], w[0], w[3], m_axis, m_axis, f1, gam, P1, bad);
         } else {
@@ -492,14 +581,16 @@ sweep_kernel(SweepArgs a)
         // edge 2: polar face (j, j+1), (i+1, j+1) -> (i, j+1)
         const Face f2 = face_geom<MOVING>(q1.xb, q1.yb, q0.xb, q0.yb, q1n.xb, q1n.yb, q0n.xb, q0n.yb, dt);
         const C cN = cell_shfl<true>(w[1], 1), cD = cell_shfl<false>(w[1], 1), cU = cell_shfl<true>(w[1], 2);
-        TF P2[NV], uu2;
+        double P2[NV];
+        TF uu2;
         if (axis_hi) uu2 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_AXIS, w[1], cD, f2, P2);
         else uu2 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], cN, cD, cU, m_axis, n_axis, f2, gam, P2, bad);
         // edge 0: polar face (j-1, j), (i, j) -> (i+1, j); solved by the lane below
         const Face f0 = face_geom<MOVING>(q0.xa, q0.ya, q1.xa, q1.ya, q0n.xa, q0n.ya, q1n.xa, q1n.ya, dt);
-        TF P0[NV], uu0;
+        double P0[NV];
+        TF uu0;
         {
-            TF Pl[NV];
+            double Pl[NV];
 #pragma unroll
             for (int k = 0; k < NV; k++) Pl[k] = shfl_up(P2[k], 1);
             const TF uul = shfl_up(uu2, 1);

@@ -417,3 +417,67 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
 }
 
 }  // namespace gel
+
+namespace gel {
+
+// Upwind (supersonic) gas flux of the state S through a face moving with W: the
+// branch SL >= W / SR <= W of HLLC_2d_Korolkov_b_s (Setka.cpp:10569-10589).
+// P: mass, x-momentum, y-momentum, energy, tracer.
+template <class T>
+GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T gam, T P[5])
+{
+    const T t1 = n2, t2 = -n1;
+    const T u = S.u * n1 + S.v * n2, v = S.u * t1 + S.v * t2;
+    const T k = (sq(S.u) + sq(S.v)) / T(2);
+    const T e = S.p / (gam - T(1)) + S.ro * k;
+    const T F[5] = { S.ro * u, S.ro * u * u + S.p, S.ro * u * v, (e + S.p) * u, S.Q * u };
+    const T U[5] = { S.ro, S.ro * u, S.ro * v, e, S.Q };
+    P[1] = n1 * (F[1] - W * U[1]) + t1 * (F[2] - W * U[2]);
+    P[2] = n2 * (F[1] - W * U[1]) + t2 * (F[2] - W * U[2]);
+    P[0] = F[0] - W * U[0];
+    P[3] = F[3] - W * U[3];
+    P[4] = F[4] - W * U[4];
+}
+
+// Upwind MHD flux of state q (the side the whole fan leaves behind), regions
+// SL >= W / SR <= W of the HLLD above; o is the other side, needed for the
+// normal-field term -SN (bn_R - bn_L) (Solvers.cpp:2161-2163).  q_is_left tells
+// which side q is.
+template <class T>
+GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T gam, T SN, bool q_is_left, T P[9])
+{
+    const T spi4 = T(3.5449077018110318);
+    const T t1 = -n2, t2 = n1;
+    const T z = T(0), one = T(1);
+    const T bx = q[5] / spi4, by = q[6] / spi4, bz = q[7] / spi4;
+    const T u = q[2] * n1 + q[3] * n2 + q[4] * z;
+    const T v = q[2] * t1 + q[3] * t2 + q[4] * z;
+    const T w = q[2] * z + q[3] * z + q[4] * one;
+    const T bn = bx * n1 + by * n2 + bz * z;
+    const T bt = bx * t1 + by * t2 + bz * z;
+    const T bm = bx * z + by * z + bz * one;
+    const T bno = (o[5] / spi4) * n1 + (o[6] / spi4) * n2 + (o[7] / spi4) * z;
+    const T bb = sq(bx) + sq(by) + sq(bz);
+    const T uu = (sq(q[2]) + sq(q[3]) + sq(q[4])) / T(2);
+    const T pT = q[1] + bb / T(2);
+    const T e = q[1] / (gam - T(1)) + q[0] * uu + bb / T(2);
+    const T F[9] = { q[0] * u, q[0] * u * u + pT - sq(bn), q[0] * u * v - bn * bt, q[0] * u * w - bn * bm,
+                     (e + pT) * u - bn * (u * bn + v * bt + w * bm), T(0), u * bt - v * bn, u * bm - w * bn,
+                     q[8] * u };
+    const T U[9] = { q[0], q[0] * u, q[0] * v, q[0] * w, e, bn, bt, bm, q[8] };
+    T PO[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) PO[i] = F[i] - W * U[i];
+    PO[5] = q_is_left ? (-SN * (bno - bn) - W * bn) : (-SN * (bn - bno) - W * bn);
+    P[0] = PO[0];
+    P[1] = n1 * PO[1] + t1 * PO[2] + z * PO[3];
+    P[2] = n2 * PO[1] + t2 * PO[2] + z * PO[3];
+    P[3] = z * PO[1] + z * PO[2] + one * PO[3];
+    P[4] = PO[4];
+    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6] + z * PO[7]);
+    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6] + z * PO[7]);
+    P[7] = spi4 * (z * PO[5] + z * PO[6] + one * PO[7]);
+    P[8] = PO[8];
+}
+
+}  // namespace gel
