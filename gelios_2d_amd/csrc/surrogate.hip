@@ -1,0 +1,583 @@
+// surrogate.hip - learned stand-in for the charge-exchange source terms.
+//
+// A multilayer perceptron 14 -> 128 -> 128 -> 128 -> 128 -> 5 (SiLU after each hidden
+// layer) evaluated and trained on CDNA4 matrix cores: every product is a BF16 MFMA
+// (v_mfma_f32_16x16x16_bf16) with FP32 accumulation; biases, pre-activations, the loss
+// and the Adam moments are FP32; the FP32 master weights are re-quantised to BF16 after
+// each optimiser step.
+//
+// The reference computes these sources by Monte Carlo (Setka::MK_start_new,
+// Setka.cpp:11989-12535) and refreshes them between sweeps; its analytic closure
+// Cell::Get_Sourse_MK1 (Cell.cpp:498-665) is what the sweep then applies.  Here that
+// analytic path generates the training targets, judges the network per cell, and is
+// the fallback where the network misses its tolerance.
+//
+// All matrix products are one kernel shape, C[m][n] = sum_k A[m][k] * Bt[n][k] with
+// both operands contiguous in k (row-major A, row-major B transposed).  Forward:
+// Z = H W^T (A = activations [cells][in], Bt = W [out][in]).  Backprop of activations:
+// dH = dZ W (Bt = W^T kept as a second BF16 copy).  Weight gradients: dW = dZ^T H
+// (A = dZ^T [out][cells], Bt = H^T [in][cells]); the epilogues store the transposed
+// copies these need.  The sum over the batch happens inside one GEMM (K = cells), so
+// gradients are deterministic without atomics.
+#include "sweep_core.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace gel {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+using bf16 = unsigned short;
+
+__host__ __device__ inline bf16 f2bf(float f)
+{   // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)0x7fc0;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16 b)
+{
+    const uint32_t u = (uint32_t)b << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// network shape (inputs and outputs padded to the 16-wide MFMA tile)
+constexpr int kIn = 14, kInP = 16, kHid = 128, kOut = 5, kOutP = 16, kLayers = 5;
+constexpr int kIn_[kLayers] = { kInP, kHid, kHid, kHid, kHid };
+constexpr int kOut_[kLayers] = { kHid, kHid, kHid, kHid, kOutP };
+constexpr int kInReal[kLayers] = { kIn, kHid, kHid, kHid, kHid };
+constexpr int kOutReal[kLayers] = { kHid, kHid, kHid, kHid, kOut };
+
+enum Epi {
+    EPI_F32 = 0,        // C fp32
+    EPI_BIAS_F32 = 1,   // C + bias, fp32 (output layer)
+    EPI_BIAS_SILU = 2,  // Z = C + bias (fp32), H = silu(Z) as bf16 [m][n] and [n][m]
+    EPI_DSILU = 3,      // dZ = C * silu'(Z) as bf16 [m][n] and [n][m]
+};
+
+struct GemmOut {
+    float* cf;          // [M][ldc] fp32 (Z or C)
+    int ldc;
+    const float* bias;  // [N]
+    bf16* cb;           // [M][ldc] bf16
+    bf16* cbt;          // [N][ldt] bf16 transposed
+    int ldt;
+    const float* z;     // [M][ldc] saved pre-activation (EPI_DSILU)
+};
+
+// 256 threads = 4 wavefronts, each a 32x32 output tile of 2x2 MFMA 16x16x16 blocks.
+// M, N, K are multiples of 16; tiles past M or N are skipped whole (wave-uniform).
+template <int EPI>
+__global__ void __launch_bounds__(256)
+gemm_nt(int M, int N, int K, const bf16* __restrict__ A, int lda, const bf16* __restrict__ Bt, int ldb, GemmOut o)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.x * 64 + (wave >> 1) * 32;
+    const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
+    const int r = lane & 15, q = lane >> 4;
+    f4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = f4{ 0.f, 0.f, 0.f, 0.f };
+    const bool mv[2] = { m0 < M, m0 + 16 < M };
+    const bool nv[2] = { n0 < N, n0 + 16 < N };
+    for (int k = 0; k < K; k += 16) {
+        s4 af[2], bfr[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) {
+            if (mv[t]) af[t] = *(const s4*)(A + (size_t)(m0 + t * 16 + r) * lda + k + 4 * q);
+            if (nv[t]) bfr[t] = *(const s4*)(Bt + (size_t)(n0 + t * 16 + r) * ldb + k + 4 * q);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+                if (mv[a] && nv[b]) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+    // D layout: lane holds rows 4q..4q+3 of column r of each 16x16 block
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            if (!(mv[a] && nv[b])) continue;
+            const int n = n0 + b * 16 + r;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int m = m0 + a * 16 + 4 * q + j;
+                float v = acc[a][b][j];
+                const size_t ij = (size_t)m * o.ldc + n;
+                if (EPI == EPI_F32) {
+                    o.cf[ij] = v;
+                } else if (EPI == EPI_BIAS_F32) {
+                    o.cf[ij] = v + o.bias[n];
+                } else if (EPI == EPI_BIAS_SILU) {
+                    const float z = v + o.bias[n];
+                    o.cf[ij] = z;
+                    const bf16 h = f2bf(z * sigmoidf(z));
+                    o.cb[ij] = h;
+                    o.cbt[(size_t)n * o.ldt + m] = h;
+                } else {
+                    const float z = o.z[ij];
+                    const float s = sigmoidf(z);
+                    const bf16 d = f2bf(v * (s * (1.0f + z * (1.0f - s))));
+                    o.cb[ij] = d;
+                    o.cbt[(size_t)n * o.ldt + m] = d;
+                }
+            }
+        }
+}
+
+// dY = 2 (Y - T) / (B * kOut) on the real outputs, 0 on the padding; loss partials.
+__global__ void loss_grad_kernel(int B, const float* __restrict__ Y, const float* __restrict__ T,
+                                 bf16* __restrict__ dy, bf16* __restrict__ dyt, float* __restrict__ partial)
+{
+    __shared__ float red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < B * kOutP) {
+        const int b = i / kOutP, c = i % kOutP;
+        float g = 0.f;
+        if (c < kOut) {
+            const float e = Y[i] - T[(size_t)b * kOut + c];
+            l = e * e;
+            g = 2.0f * e / (float)(B * kOut);
+        }
+        const bf16 gb = f2bf(g);
+        dy[i] = gb;
+        dyt[(size_t)c * B + b] = gb;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// db[n] = sum over the batch of dZ[b][n] (dZ as bf16 [B][ld])
+__global__ void bias_grad_kernel(int B, int N, const bf16* __restrict__ dz, int ld, float* __restrict__ db)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    float s = 0.f;
+    for (int b = 0; b < B; b++) s += bf2f(dz[(size_t)b * ld + n]);
+    db[n] = s;
+}
+
+// Adam on the FP32 master copy, then BF16 copies W [out][in] and W^T [in][out].
+__global__ void adam_kernel(int n, float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+__global__ void quantize_kernel(int rows, int cols, const float* __restrict__ w, bf16* __restrict__ wb,
+                                bf16* __restrict__ wt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * cols) return;
+    const int r = i / cols, c = i % cols;
+    const bf16 b = f2bf(w[i]);
+    wb[i] = b;
+    wt[(size_t)c * rows + r] = b;
+}
+
+// X fp32 [n][14] -> bf16 [B][16] and [16][B] (zero padding)
+__global__ void pack_input_kernel(int n, int B, const float* __restrict__ x, bf16* __restrict__ xb,
+                                  bf16* __restrict__ xt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B * kInP) return;
+    const int b = i / kInP, c = i % kInP;
+    const float v = (b < n && c < kIn) ? x[(size_t)b * kIn + c] : 0.f;
+    const bf16 h = f2bf(v);
+    xb[i] = h;
+    xt[(size_t)c * B + b] = h;
+}
+
+}  // namespace gel
+
+using namespace gel;
+
+static thread_local std::string g_mlp_err;
+static int mfail(int st, const std::string& m)
+{
+    g_mlp_err = m;
+    return st;
+}
+
+struct gel_mlp {
+    // FP32 master parameters, padded layouts: W_l [out_l][in_l], b_l [out_l]
+    float* w[kLayers] = {};
+    float* b[kLayers] = {};
+    float* gw[kLayers] = {};
+    float* gb[kLayers] = {};
+    float* mw[kLayers] = {};
+    float* vw[kLayers] = {};
+    float* mb[kLayers] = {};
+    float* vb[kLayers] = {};
+    bf16* wb[kLayers] = {};   // BF16 W
+    bf16* wt[kLayers] = {};   // BF16 W^T
+    long long adam_t = 0;
+    // activations for a batch of capacity cap
+    int cap = 0;
+    bf16 *x = nullptr, *xt = nullptr;
+    float* z[4] = {};
+    bf16* h[4] = {};
+    bf16* ht[4] = {};
+    float* y = nullptr;
+    float* t = nullptr;
+    bf16 *dy = nullptr, *dyt = nullptr;
+    bf16 *dz = nullptr, *dzt = nullptr, *dz2 = nullptr, *dz2t = nullptr;
+    float* partial = nullptr;
+    float* xin = nullptr;
+    // training set
+    int nset = 0;
+    float* set_x = nullptr;
+    float* set_t = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+static size_t wsz(int l) { return (size_t)kOut_[l] * kIn_[l]; }
+
+static void mlp_free_batch(gel_mlp* m)
+{
+    (void)hipFree(m->x); (void)hipFree(m->xt); (void)hipFree(m->y); (void)hipFree(m->t);
+    (void)hipFree(m->dy); (void)hipFree(m->dyt); (void)hipFree(m->dz); (void)hipFree(m->dzt);
+    (void)hipFree(m->dz2); (void)hipFree(m->dz2t); (void)hipFree(m->partial); (void)hipFree(m->xin);
+    for (int l = 0; l < 4; l++) { (void)hipFree(m->z[l]); (void)hipFree(m->h[l]); (void)hipFree(m->ht[l]); }
+    m->x = m->xt = nullptr; m->y = m->t = nullptr; m->dy = m->dyt = m->dz = m->dzt = m->dz2 = m->dz2t = nullptr;
+    m->partial = m->xin = nullptr;
+    for (int l = 0; l < 4; l++) { m->z[l] = nullptr; m->h[l] = nullptr; m->ht[l] = nullptr; }
+    m->cap = 0;
+}
+
+static int mlp_reserve(gel_mlp* m, int B)
+{
+    if (B <= m->cap) return GEL_OK;
+    mlp_free_batch(m);
+    const size_t b = (size_t)B;
+    bool ok = true;
+    auto al = [&](void** p, size_t bytes) { if (ok && hipMalloc(p, bytes) != hipSuccess) ok = false; };
+    al((void**)&m->x, b * kInP * 2); al((void**)&m->xt, b * kInP * 2);
+    for (int l = 0; l < 4; l++) { al((void**)&m->z[l], b * kHid * 4); al((void**)&m->h[l], b * kHid * 2); al((void**)&m->ht[l], b * kHid * 2); }
+    al((void**)&m->y, b * kOutP * 4); al((void**)&m->t, b * kOut * 4);
+    al((void**)&m->dy, b * kOutP * 2); al((void**)&m->dyt, b * kOutP * 2);
+    al((void**)&m->dz, b * kHid * 2); al((void**)&m->dzt, b * kHid * 2);
+    al((void**)&m->dz2, b * kHid * 2); al((void**)&m->dz2t, b * kHid * 2);
+    al((void**)&m->partial, (b * kOutP / 256 + 2) * 4); al((void**)&m->xin, b * kIn * 4);
+    if (!ok) { mlp_free_batch(m); return mfail(GEL_E_NOMEM, "mlp batch buffers"); }
+    m->cap = B;
+    return GEL_OK;
+}
+
+template <int EPI>
+static void gemm(hipStream_t s, int M, int N, int K, const bf16* A, int lda, const bf16* Bt, int ldb, GemmOut o)
+{
+    dim3 grid((M + 63) / 64, (N + 63) / 64);
+    hipLaunchKernelGGL((gemm_nt<EPI>), grid, dim3(256), 0, s, M, N, K, A, lda, Bt, ldb, o);
+}
+
+// forward over a padded batch B (activations stay on the device)
+static void mlp_forward_dev(gel_mlp* m, int B)
+{
+    const bf16* in = m->x;
+    int ldin = kInP;
+    for (int l = 0; l < 4; l++) {
+        GemmOut o{ m->z[l], kHid, m->b[l], m->h[l], m->ht[l], B, nullptr };
+        gemm<EPI_BIAS_SILU>(m->stream, B, kHid, kIn_[l], in, ldin, m->wb[l], kIn_[l], o);
+        in = m->h[l];
+        ldin = kHid;
+    }
+    GemmOut o{ m->y, kOutP, m->b[4], nullptr, nullptr, 0, nullptr };
+    gemm<EPI_BIAS_F32>(m->stream, B, kOutP, kHid, in, kHid, m->wb[4], kHid, o);
+}
+
+// backward for the batch in m->x / m->t (forward already run); gradients in gw/gb
+static void mlp_backward_dev(gel_mlp* m, int B)
+{
+    hipLaunchKernelGGL(loss_grad_kernel, dim3((B * kOutP + 255) / 256), dim3(256), 0, m->stream, B, m->y, m->t, m->dy,
+                       m->dyt, m->partial);
+    // layer 4 (linear): dW4 = dY^T H3, db4, dH3 = dY W4 -> dZ3
+    {
+        GemmOut o{ m->gw[4], kHid, nullptr, nullptr, nullptr, 0, nullptr };
+        gemm<EPI_F32>(m->stream, kOutP, kHid, B, m->dyt, B, m->ht[3], B, o);
+        hipLaunchKernelGGL(bias_grad_kernel, dim3(1), dim3(64), 0, m->stream, B, kOutP, m->dy, kOutP, m->gb[4]);
+        GemmOut d{ nullptr, kHid, nullptr, m->dz, m->dzt, B, m->z[3] };
+        d.cf = nullptr;
+        d.z = m->z[3];
+        gemm<EPI_DSILU>(m->stream, B, kHid, kOutP, m->dy, kOutP, m->wt[4], kOutP, d);
+    }
+    bf16 *dz = m->dz, *dzt = m->dzt, *dzn = m->dz2, *dznt = m->dz2t;
+    for (int l = 3; l >= 0; l--) {
+        // dW_l = dZ_l^T H_{l-1} (H_{-1} = X)
+        const bf16* hprevT = (l > 0) ? m->ht[l - 1] : m->xt;
+        GemmOut o{ m->gw[l], kIn_[l], nullptr, nullptr, nullptr, 0, nullptr };
+        gemm<EPI_F32>(m->stream, kHid, kIn_[l], B, dzt, B, hprevT, B, o);
+        hipLaunchKernelGGL(bias_grad_kernel, dim3((kHid + 63) / 64), dim3(64), 0, m->stream, B, kHid, dz, kHid, m->gb[l]);
+        if (l > 0) {
+            GemmOut d{ nullptr, kHid, nullptr, dzn, dznt, B, m->z[l - 1] };
+            gemm<EPI_DSILU>(m->stream, B, kHid, kHid, dz, kHid, m->wt[l], kHid, d);
+            std::swap(dz, dzn);
+            std::swap(dzt, dznt);
+        }
+    }
+}
+
+static void mlp_quantize(gel_mlp* m)
+{
+    for (int l = 0; l < kLayers; l++) {
+        const int n = (int)wsz(l);
+        hipLaunchKernelGGL(quantize_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, kOut_[l], kIn_[l], m->w[l],
+                           m->wb[l], m->wt[l]);
+    }
+}
+
+static uint64_t splitmix(uint64_t& s)
+{
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+extern "C" {
+
+const char* gel_mlp_last_error(void) { return g_mlp_err.c_str(); }
+
+int gel_mlp_param_count(void)
+{
+    int n = 0;
+    for (int l = 0; l < kLayers; l++) n += kOutReal[l] * kInReal[l] + kOutReal[l];
+    return n;
+}
+
+void gel_mlp_destroy(gel_mlp* m)
+{
+    if (!m) return;
+    mlp_free_batch(m);
+    for (int l = 0; l < kLayers; l++) {
+        (void)hipFree(m->w[l]); (void)hipFree(m->b[l]); (void)hipFree(m->gw[l]); (void)hipFree(m->gb[l]);
+        (void)hipFree(m->mw[l]); (void)hipFree(m->vw[l]); (void)hipFree(m->mb[l]); (void)hipFree(m->vb[l]);
+        (void)hipFree(m->wb[l]); (void)hipFree(m->wt[l]);
+    }
+    (void)hipFree(m->set_x); (void)hipFree(m->set_t);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int gel_mlp_set_params(gel_mlp* m, const float* p)
+{
+    if (!m || !p) return mfail(GEL_E_ARG, "null argument");
+    size_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        std::vector<float> W(wsz(l), 0.f), b((size_t)kOut_[l], 0.f);
+        for (int r = 0; r < kOutReal[l]; r++)
+            for (int c = 0; c < kInReal[l]; c++) W[(size_t)r * kIn_[l] + c] = p[o++];
+        for (int r = 0; r < kOutReal[l]; r++) b[r] = p[o++];
+        if (hipMemcpy(m->w[l], W.data(), W.size() * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(m->b[l], b.data(), b.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+            return mfail(GEL_E_HIP, "set params");
+    }
+    mlp_quantize(m);
+    return hipStreamSynchronize(m->stream) == hipSuccess ? GEL_OK : mfail(GEL_E_HIP, "quantize");
+}
+
+int gel_mlp_get_params(gel_mlp* m, float* p)
+{
+    if (!m || !p) return mfail(GEL_E_ARG, "null argument");
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return mfail(GEL_E_HIP, "sync");
+    size_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        std::vector<float> W(wsz(l)), b((size_t)kOut_[l]);
+        if (hipMemcpy(W.data(), m->w[l], W.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(b.data(), m->b[l], b.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return mfail(GEL_E_HIP, "get params");
+        for (int r = 0; r < kOutReal[l]; r++)
+            for (int c = 0; c < kInReal[l]; c++) p[o++] = W[(size_t)r * kIn_[l] + c];
+        for (int r = 0; r < kOutReal[l]; r++) p[o++] = b[r];
+    }
+    return GEL_OK;
+}
+
+int gel_mlp_create(gel_mlp** out, uint64_t seed)
+{
+    if (!out) return mfail(GEL_E_ARG, "null argument");
+    *out = nullptr;
+    gel_mlp* m = new (std::nothrow) gel_mlp();
+    if (!m) return mfail(GEL_E_NOMEM, "host");
+    bool ok = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int l = 0; l < kLayers && ok; l++) {
+        const size_t n = wsz(l), nb = (size_t)kOut_[l];
+        float** fs[] = { &m->w[l], &m->gw[l], &m->mw[l], &m->vw[l] };
+        for (auto f : fs) ok = ok && hipMalloc((void**)f, n * 4) == hipSuccess && hipMemset(*f, 0, n * 4) == hipSuccess;
+        float** bs[] = { &m->b[l], &m->gb[l], &m->mb[l], &m->vb[l] };
+        for (auto f : bs) ok = ok && hipMalloc((void**)f, nb * 4) == hipSuccess && hipMemset(*f, 0, nb * 4) == hipSuccess;
+        ok = ok && hipMalloc((void**)&m->wb[l], n * 2) == hipSuccess && hipMalloc((void**)&m->wt[l], n * 2) == hipSuccess;
+    }
+    if (!ok) { gel_mlp_destroy(m); return mfail(GEL_E_NOMEM, "mlp parameters"); }
+    // uniform(-a, a) with a = sqrt(6 / fan_in) for SiLU layers, sqrt(3 / fan_in) for the output
+    std::vector<float> p((size_t)gel_mlp_param_count());
+    uint64_t s = seed;
+    size_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        const float a = std::sqrt((l < 4 ? 6.0f : 3.0f) / (float)kInReal[l]);
+        for (int i = 0; i < kOutReal[l] * kInReal[l]; i++)
+            p[o++] = a * (2.0f * (float)((splitmix(s) >> 11) * (1.0 / 9007199254740992.0)) - 1.0f);
+        for (int i = 0; i < kOutReal[l]; i++) p[o++] = 0.0f;
+    }
+    const int st = gel_mlp_set_params(m, p.data());
+    if (st != GEL_OK) { gel_mlp_destroy(m); return st; }
+    *out = m;
+    return GEL_OK;
+}
+
+// Forward on host arrays x [n][14] (normalised features) -> y [n][5].
+int gel_mlp_forward(gel_mlp* m, int n, const float* x, float* y)
+{
+    if (!m || n < 0 || (n > 0 && (!x || !y))) return mfail(GEL_E_ARG, "null argument");
+    if (n == 0) return GEL_OK;
+    const int B = (n + 15) / 16 * 16;
+    int st = mlp_reserve(m, B);
+    if (st != GEL_OK) return st;
+    if (hipMemcpyAsync(m->xin, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice, m->stream) != hipSuccess)
+        return mfail(GEL_E_HIP, "copy");
+    hipLaunchKernelGGL(pack_input_kernel, dim3((B * kInP + 255) / 256), dim3(256), 0, m->stream, n, B, m->xin, m->x, m->xt);
+    mlp_forward_dev(m, B);
+    std::vector<float> Y((size_t)B * kOutP);
+    if (hipMemcpyAsync(Y.data(), m->y, Y.size() * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return mfail(GEL_E_HIP, "forward");
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < kOut; c++) y[(size_t)i * kOut + c] = Y[(size_t)i * kOutP + c];
+    return GEL_OK;
+}
+
+static int load_batch(gel_mlp* m, int n, const float* x_dev, const float* t_dev, hipMemcpyKind kind)
+{
+    const int B = (n + 15) / 16 * 16;
+    int st = mlp_reserve(m, B);
+    if (st != GEL_OK) return st;
+    if (hipMemsetAsync(m->t, 0, (size_t)B * kOut * 4, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->xin, x_dev, (size_t)n * kIn * 4, kind, m->stream) != hipSuccess ||
+        hipMemcpyAsync(m->t, t_dev, (size_t)n * kOut * 4, kind, m->stream) != hipSuccess)
+        return mfail(GEL_E_HIP, "batch copy");
+    hipLaunchKernelGGL(pack_input_kernel, dim3((B * kInP + 255) / 256), dim3(256), 0, m->stream, n, B, m->xin, m->x, m->xt);
+    return GEL_OK;
+}
+
+// Loss (mean squared error over the n x 5 outputs) and its gradient with respect to
+// every parameter, in the packing order of gel_mlp_get_params.  Rows past n in the
+// padded batch carry zero targets and zero inputs; the gradient divides by n*5 only
+// through the rows that exist (padding rows are excluded by the loss kernel's B).
+int gel_mlp_grad(gel_mlp* m, int n, const float* x, const float* t, float* grad, float* loss)
+{
+    if (!m || n <= 0 || n % 16 || !x || !t || !grad || !loss) return mfail(GEL_E_ARG, "n must be a positive multiple of 16");
+    int st = load_batch(m, n, x, t, hipMemcpyHostToDevice);
+    if (st != GEL_OK) return st;
+    mlp_forward_dev(m, n);
+    mlp_backward_dev(m, n);
+    const int nb = (n * kOutP + 255) / 256;
+    std::vector<float> part((size_t)nb);
+    if (hipMemcpyAsync(part.data(), m->partial, (size_t)nb * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+        return mfail(GEL_E_HIP, "grad");
+    double s = 0.0;
+    for (float v : part) s += v;
+    *loss = (float)(s / (double)(n * kOut));
+    size_t o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        std::vector<float> W(wsz(l)), b((size_t)kOut_[l]);
+        if (hipMemcpy(W.data(), m->gw[l], W.size() * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(b.data(), m->gb[l], b.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return mfail(GEL_E_HIP, "grad copy");
+        for (int r = 0; r < kOutReal[l]; r++)
+            for (int c = 0; c < kInReal[l]; c++) grad[o++] = W[(size_t)r * kIn_[l] + c];
+        for (int r = 0; r < kOutReal[l]; r++) grad[o++] = b[r];
+    }
+    return GEL_OK;
+}
+
+// Training set on the device: x [n][14] normalised features, t [n][5] targets.
+int gel_mlp_set_dataset(gel_mlp* m, int n, const float* x, const float* t)
+{
+    if (!m || n <= 0 || !x || !t) return mfail(GEL_E_ARG, "null argument");
+    (void)hipFree(m->set_x);
+    (void)hipFree(m->set_t);
+    m->set_x = m->set_t = nullptr;
+    if (hipMalloc((void**)&m->set_x, (size_t)n * kIn * 4) != hipSuccess ||
+        hipMalloc((void**)&m->set_t, (size_t)n * kOut * 4) != hipSuccess)
+        return mfail(GEL_E_NOMEM, "dataset");
+    if (hipMemcpy(m->set_x, x, (size_t)n * kIn * 4, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(m->set_t, t, (size_t)n * kOut * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return mfail(GEL_E_HIP, "dataset copy");
+    m->nset = n;
+    return GEL_OK;
+}
+
+// steps minibatch Adam steps over the stored set (batch rows taken in order,
+// wrapping; the caller shuffles the set).  loss_out[steps] receives the batch losses.
+int gel_mlp_train(gel_mlp* m, int steps, int batch, float lr, float* loss_out)
+{
+    if (!m || steps < 0 || batch <= 0 || batch % 16 || !m->set_x) return mfail(GEL_E_ARG, "train arguments");
+    if (batch > m->nset) return mfail(GEL_E_ARG, "batch larger than the set");
+    int st = mlp_reserve(m, batch);
+    if (st != GEL_OK) return st;
+    const int nb = (batch * kOutP + 255) / 256;
+    std::vector<float> part((size_t)nb);
+    long long pos = 0;
+    for (int it = 0; it < steps; it++) {
+        if (pos + batch > m->nset) pos = 0;
+        st = load_batch(m, batch, m->set_x + (size_t)pos * kIn, m->set_t + (size_t)pos * kOut, hipMemcpyDeviceToDevice);
+        if (st != GEL_OK) return st;
+        pos += batch;
+        mlp_forward_dev(m, batch);
+        mlp_backward_dev(m, batch);
+        m->adam_t++;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.0f - std::pow(b1, (float)m->adam_t), c2 = 1.0f - std::pow(b2, (float)m->adam_t);
+        for (int l = 0; l < kLayers; l++) {
+            const int n = (int)wsz(l), nbias = kOut_[l];
+            hipLaunchKernelGGL(adam_kernel, dim3((n + 255) / 256), dim3(256), 0, m->stream, n, m->w[l], m->gw[l], m->mw[l],
+                               m->vw[l], lr, b1, b2, 1e-8f, c1, c2);
+            hipLaunchKernelGGL(adam_kernel, dim3((nbias + 255) / 256), dim3(256), 0, m->stream, nbias, m->b[l], m->gb[l],
+                               m->mb[l], m->vb[l], lr, b1, b2, 1e-8f, c1, c2);
+        }
+        mlp_quantize(m);
+        if (loss_out) {
+            if (hipMemcpyAsync(part.data(), m->partial, (size_t)nb * 4, hipMemcpyDeviceToHost, m->stream) != hipSuccess ||
+                hipStreamSynchronize(m->stream) != hipSuccess)
+                return mfail(GEL_E_HIP, "train");
+            double s = 0.0;
+            for (float v : part) s += v;
+            loss_out[it] = (float)(s / (double)(batch * kOut));
+        }
+    }
+    if (hipStreamSynchronize(m->stream) != hipSuccess) return mfail(GEL_E_HIP, "train sync");
+    if (hipGetLastError() != hipSuccess) return mfail(GEL_E_HIP, "train launch");
+    return GEL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+"""MFMA surrogate kernels against the NumPy restatement (oracle/mlp_oracle.py)."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _mlp(seed=3):
+    from gelios_2d_amd.surrogate import MLP
+    return MLP(seed)
+
+
+def test_param_roundtrip_and_count():
+    m = _mlp()
+    p = m.get_params()
+    assert p.size == mo.n_params() == 52101
+    q = np.random.default_rng(1).normal(0, 0.1, p.size).astype(np.float32)
+    m.set_params(q)
+    assert np.array_equal(m.get_params(), q)
+
+
+def test_forward_matches_bf16_restatement():
+    m = _mlp()
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (1000, 14)).astype(np.float32)
+    y = m.forward(X)
+    p = m.get_params()
+    ref, _ = mo.forward(p, X, "bf16")
+    exact, _ = mo.forward(p, X.astype(np.float64), "exact")
+    scale = np.abs(exact).max()
+    # same BF16 rounding points: differences come from FP32 accumulation order only
+    assert np.max(np.abs(y - ref)) <= 2e-2 * scale
+    # and BF16 compute stays within a few per cent of the FP64 network
+    assert np.sqrt(np.mean((y - exact) ** 2)) <= 2e-2 * np.sqrt(np.mean(exact ** 2))
+
+
+def test_gradients_match_restatement():
+    m = _mlp()
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (256, 14)).astype(np.float32)
+    T = rng.normal(0, 1, (256, 5)).astype(np.float32)
+    loss, g = m.grad(X, T)
+    p = m.get_params().astype(np.float64)
+    lref, gref = mo.loss_and_grad(p, X, T, "bf16")
+    lex, gex = mo.loss_and_grad(p, X.astype(np.float64), T.astype(np.float64), "exact")
+    assert abs(loss - lref) <= 1e-3 * lref
+    o = 0
+    for (i, n) in mo.SIZES:
+        for size in (i * n, n):
+            a, b, c = g[o:o + size], gref[o:o + size], gex[o:o + size]
+            assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-7
+            assert np.linalg.norm(a - c) <= 6e-2 * np.linalg.norm(c) + 1e-7
+            o += size
+
+
+def test_training_fits_a_smooth_map():
+    m = _mlp(7)
+    rng = np.random.default_rng(8)
+    X = rng.uniform(-1, 1, (32768, 14)).astype(np.float32)
+    A = rng.normal(0, 0.5, (14, 5))
+    T = np.sin(X @ A).astype(np.float32)
+    m.set_dataset(X, T)
+    loss = m.train(1500, 1024, 2e-3)
+    assert np.all(np.isfinite(loss))
+    assert loss[-50:].mean() <= 0.05 * loss[:5].mean()
