@@ -14,6 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgelios.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _i8p = ctypes.POINTER(ctypes.c_int8)
 
 
@@ -51,8 +52,24 @@ SIGNATURES = {
     "gel_flux_hllc2d_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_double,
                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                              _dp, _dp]),
+    "gel_set_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
+    "gel_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]),
+    "gel_mlp_destroy": (None, [ctypes.c_void_p]),
+    "gel_mlp_param_count": (ctypes.c_int, []),
+    "gel_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "gel_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "gel_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _fp, _fp]),
+    "gel_mlp_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _fp, _fp, _fp, _fp]),
+    "gel_mlp_set_dataset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _fp, _fp]),
+    "gel_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]),
+    "gel_mlp_set_scaling": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, _fp, _fp]),
+    "gel_mlp_last_error": (ctypes.c_char_p, []),
+    "gel_surrogate_features": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "gel_source_analytic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_surrogate_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_longlong)]),
     "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gel_last_error": (ctypes.c_char_p, []),
 }

@@ -19,6 +19,7 @@
 // once from each side, as the reference's cell gather does (Setka.cpp:10195),
 // changes the flux on the second side only by rounding.
 #include "sweep_core.h"
+#include "gelios_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -62,6 +63,7 @@ struct SweepArgs {
     const int8_t* __restrict__ ctype;
     const int8_t* __restrict__ otype;
     const double* __restrict__ src;
+    const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
     unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
@@ -444,13 +446,15 @@ __device__ __forceinline__ TF interior_flux(const CellT<Ph, TS>& M, const CellT<
 // angular momentum, azimuthal induction) per unit planar area.
 template <class Ph>
 __device__ __forceinline__ void cell_update(const double* s, const double* Pot, double Volume, double Volume2,
-                                            double vs1, double vs2, double T1, double gam, double q1, double q2,
-                                            double q3, double* o, int& n_rho, int& n_p)
+                                            double vs1, double vs2, double T1, double gam, double qm, double q1,
+                                            double q2, double q3, double qQ, bool table, double* o, int& n_rho,
+                                            int& n_p)
 {
     if constexpr (Ph::NV == 5) {
         const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
-        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
-        const double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
+        double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+        if (table) Q33 = Q33 + T1 * qQ;
         if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
         const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
         const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2)) / ro3;
@@ -468,7 +472,7 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
         const double Sw = -(ro * v * w - by * bz / pi4);
         const double Sb = v * bz - w * by;
         const double E = p / (gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + BB / pi8;
-        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
         if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
         const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
         const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - Sv * Ar) / Volume2 - q2)) / ro3;
@@ -477,7 +481,7 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
         const double bx3 = bx * Volume / Volume2 - T1 * (Pot[5] / Volume2);
         const double by3 = by * Volume / Volume2 - T1 * (Pot[6] / Volume2);
         const double bz3 = bz * Volume / Volume2 - T1 * ((Pot[7] - Sb * Ar) / Volume2);
-        const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2);
+        const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2 - qQ);
         double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / pi8) * (gam - 1);
         if (p3 <= 0) { p3 = 0.000001; n_p++; }
         o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = w3;
@@ -502,7 +506,7 @@ __device__ __forceinline__ int wave_sum(int v)
 // Ph: state vector; TS: storage type of the state in HBM; TF: type of the face
 // arithmetic (reconstruction and Riemann solver); geometry, accumulation and the
 // update are FP64 throughout.
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, bool SRC, int FLUX>
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
 __global__ void __launch_bounds__(kWave)
 sweep_kernel(SweepArgs a)
 {
@@ -647,14 +651,21 @@ sweep_kernel(SweepArgs a)
                     vs2 = quad_area(px2, py2);
                     Volume2 = quad_volume_r(px2, py2);
                 }
-                double q1s = 0.0, q2s = 0.0, q3s = 0.0;
-                if (SRC) {
+                double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
+                if (SRC == GEL_SRC_MK1) {
                     const double* __restrict__ sp = a.src + kc;
                     source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
                                q1s, q2s, q3s);
+                } else if (SRC == GEL_SRC_TABLE) {
+                    qms = a.qtab[kc];
+                    q1s = a.qtab[nc + kc];
+                    q2s = a.qtab[2 * nc + kc];
+                    q3s = a.qtab[3 * nc + kc];
+                    qQs = a.qtab[4 * nc + kc];
                 }
                 double o[NV];
-                cell_update<Ph>(sd, Pot, Volume, Volume2, vs1, vs2, dt, gam, q1s, q2s, q3s, o, n_rho, n_p);
+                cell_update<Ph>(sd, Pot, Volume, Volume2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
+                                SRC == GEL_SRC_TABLE, o, n_rho, n_p);
 #pragma unroll
                 for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
             }
@@ -732,35 +743,6 @@ static int fail(int st, const std::string& msg)
             return fail(GEL_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-struct gel_ctx {
-    int ni = 0, nj = 0;
-    size_t nc = 0, nn = 0;
-    int nv = 5;                   // primitives per cell
-    size_t ts = 8;                // bytes per stored primitive (8 FP64, 4 FP32)
-    gel_params prm{};
-    double* d_x = nullptr;        // node positions (base positions under prescribed motion)
-    double* d_y = nullptr;
-    double* d_x2 = nullptr;       // next node positions (explicit moving steps)
-    double* d_y2 = nullptr;
-    void* d_state[2] = { nullptr, nullptr };
-    int cur = 0;
-    int8_t* d_ctype = nullptr;
-    int8_t* d_otype = nullptr;
-    bool has_types = false;
-    double* d_src = nullptr;
-    unsigned long long* d_dt = nullptr;   // [3]
-    unsigned long long* d_cnt = nullptr;  // [3]
-    long long steps = 0;
-    bool uploaded = false;
-    int rows = 8;
-    // prescribed motion: nodes at base*s[k], k = step mod period
-    std::vector<double> motion;
-    long long motion_step = 0;
-    hipStream_t stream = nullptr;
-    bool timing = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
-};
-
 extern "C" {
 
 const char* gel_strerror(int st)
@@ -809,7 +791,7 @@ static void free_ctx(gel_ctx* c)
     if (!c) return;
     (void)hipFree(c->d_x); (void)hipFree(c->d_y); (void)hipFree(c->d_x2); (void)hipFree(c->d_y2);
     (void)hipFree(c->d_state[0]); (void)hipFree(c->d_state[1]);
-    (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src);
+    (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src); (void)hipFree(c->d_qtab);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -837,7 +819,8 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD) return fail(GEL_E_UNSUPPORTED, "physics");
     if (prm->physics == GEL_PHYS_GAS && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
         return fail(GEL_E_UNSUPPORTED, "flux");
-    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1) return fail(GEL_E_UNSUPPORTED, "source_mode");
+    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1 && prm->source_mode != GEL_SRC_TABLE)
+        return fail(GEL_E_UNSUPPORTED, "source_mode");
     if (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK) return fail(GEL_E_UNSUPPORTED, "precision");
     if (!(prm->dt0 > 0.0) || !(prm->cfl > 0.0) || !(prm->gamma > 1.0)) return fail(GEL_E_ARG, "dt0, cfl, gamma");
     gel_ctx* c = new (std::nothrow) gel_ctx();
@@ -977,6 +960,18 @@ int gel_set_sources(gel_ctx* c, const double* f)
     return GEL_OK;
 }
 
+int gel_set_source_table(gel_ctx* c, const double* q)
+{
+    if (!c || !q) return fail(GEL_E_ARG, "null argument");
+    const size_t nb = (size_t)5 * c->nc * sizeof(double);
+    if (!c->d_qtab) {
+        hipError_t e = hipMalloc(&c->d_qtab, nb);
+        if (e != hipSuccess) return fail(GEL_E_NOMEM, "source table");
+    }
+    HIP_OK(hipMemcpy(c->d_qtab, q, nb, hipMemcpyHostToDevice));
+    return GEL_OK;
+}
+
 int gel_set_rows_per_wave(gel_ctx* c, int rows)
 {
     if (!c || rows < 1 || rows > 4096) return fail(GEL_E_ARG, "rows");
@@ -996,7 +991,7 @@ int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
 
 }  // extern "C"
 
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, bool SRC, int FLUX>
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
 static hipError_t launch_t(const SweepArgs& a, dim3 grid, hipStream_t s)
 {
     hipLaunchKernelGGL((sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX>), grid, dim3(kWave), 0, s, a);
@@ -1004,19 +999,21 @@ static hipError_t launch_t(const SweepArgs& a, dim3 grid, hipStream_t s)
 }
 
 template <class Ph, class TS, class TF, bool TYPES, int FLUX>
-static hipError_t launch_ms(const SweepArgs& a, dim3 grid, hipStream_t s, bool moving, bool src)
+static hipError_t launch_ms(const SweepArgs& a, dim3 grid, hipStream_t s, bool moving, int src)
 {
+#define GEL_L(MV, SR) launch_t<Ph, TS, TF, MV, TYPES, SR, FLUX>(a, grid, s)
     if (moving) {
-        return src ? launch_t<Ph, TS, TF, true, TYPES, true, FLUX>(a, grid, s)
-                   : launch_t<Ph, TS, TF, true, TYPES, false, FLUX>(a, grid, s);
+        return src == GEL_SRC_MK1 ? GEL_L(true, GEL_SRC_MK1) : src == GEL_SRC_TABLE ? GEL_L(true, GEL_SRC_TABLE)
+                                                                                    : GEL_L(true, GEL_SRC_NONE);
     }
-    return src ? launch_t<Ph, TS, TF, false, TYPES, true, FLUX>(a, grid, s)
-               : launch_t<Ph, TS, TF, false, TYPES, false, FLUX>(a, grid, s);
+    return src == GEL_SRC_MK1 ? GEL_L(false, GEL_SRC_MK1) : src == GEL_SRC_TABLE ? GEL_L(false, GEL_SRC_TABLE)
+                                                                                 : GEL_L(false, GEL_SRC_NONE);
+#undef GEL_L
 }
 
 static hipError_t launch(const gel_ctx* c, const SweepArgs& a, dim3 grid, bool moving)
 {
-    const bool src = (c->prm.source_mode == GEL_SRC_MK1);
+    const int src = c->prm.source_mode;
     const bool f32 = (c->prm.precision == GEL_PREC_FP32_BULK);
     hipStream_t s = c->stream;
     if (c->prm.physics == GEL_PHYS_MHD) {
@@ -1057,10 +1054,12 @@ static int one_step(gel_ctx* c, int moving)
     a.next = c->d_state[c->cur ^ 1];
     a.ctype = c->d_ctype; a.otype = c->d_otype;
     a.src = c->d_src;
+    a.qtab = c->d_qtab;
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
     a.prm = c->prm;
     if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
+    if (c->prm.source_mode == GEL_SRC_TABLE && !c->d_qtab) return fail(GEL_E_STATE, "source table not set");
     dim3 grid((c->nj + kOut - 1) / kOut, (c->ni + c->rows - 1) / c->rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {

@@ -20,6 +20,7 @@
 // copies these need.  The sum over the batch happens inside one GEMM (K = cells), so
 // gradients are deterministic without atomics.
 #include "sweep_core.h"
+#include "gelios_internal.h"
 
 #include <hip/hip_runtime.h>
 
@@ -217,6 +218,108 @@ __global__ void pack_input_kernel(int n, int B, const float* __restrict__ x, bf1
     xt[(size_t)c * B + b] = h;
 }
 
+
+// ---------------------------------------------------------- source features ---
+// Features of the source model per cell (raw, before normalisation):
+//   log ro, u, v, log(p/ro)  -- plasma (first four primitives of either state vector)
+//   log nA, uA, vA, log TA   -- populations 1+2 (density-weighted velocity, temperature)
+//   log nB, uB, vB, log TB   -- populations 3+4
+//   y, x of the cell centre  -- position (cylindrical radius, axis coordinate)
+// The hydrogen moments are the GEL_MK1_NFIELDS fields of gel_set_sources.
+template <class TS>
+__global__ void features_kernel(int ni, int nj, const double* __restrict__ X, const double* __restrict__ Y,
+                                const TS* __restrict__ st, const double* __restrict__ src,
+                                const float* __restrict__ mu, const float* __restrict__ sd, float* __restrict__ out)
+{
+    const size_t nc = (size_t)ni * nj;
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const int i = (int)(c / nj), j = (int)(c % nj);
+    const size_t a = (size_t)i * (nj + 1) + j, b = a + (nj + 1);
+    const double px[4] = { X[a], X[b], X[b + 1], X[a + 1] };
+    const double py[4] = { Y[a], Y[b], Y[b + 1], Y[a + 1] };
+    double cx, cy;
+    quad_center(px, py, cx, cy);
+    const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
+    double f[kIn];
+    f[0] = log(ro); f[1] = u; f[2] = v; f[3] = log(p / ro);
+    for (int g = 0; g < 2; g++) {
+        const int k0 = 2 * g, k1 = 2 * g + 1;
+        const double n0 = src[k0 * nc + c], n1 = src[k1 * nc + c];
+        const double n = n0 + n1;
+        const double w0 = n > 0 ? n0 / n : 0.5, w1 = n > 0 ? n1 / n : 0.5;
+        f[4 + 4 * g] = log(fmax(n, 1e-30));
+        f[5 + 4 * g] = w0 * src[(4 + k0) * nc + c] + w1 * src[(4 + k1) * nc + c];
+        f[6 + 4 * g] = w0 * src[(8 + k0) * nc + c] + w1 * src[(8 + k1) * nc + c];
+        f[7 + 4 * g] = log(fmax(w0 * src[(12 + k0) * nc + c] + w1 * src[(12 + k1) * nc + c], 1e-30));
+    }
+    f[12] = cy; f[13] = cx;
+#pragma unroll
+    for (int k = 0; k < kIn; k++) out[c * kIn + k] = mu ? (float)((f[k] - mu[k]) / sd[k]) : (float)f[k];
+}
+
+// Analytic charge-exchange source per cell (the judge and fallback): the device
+// closure of Cell::Get_Sourse_MK1 (Cell.cpp:590-664) on the cell's own moments.
+template <class TS>
+__device__ inline void analytic_q(const gel_params& prm, size_t nc, size_t c, const TS* __restrict__ st,
+                                  const double* __restrict__ src, double q[5])
+{
+    const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
+    q[0] = 0.0; q[4] = 0.0;
+    source_mk1(prm, [&](int k) { return src[(size_t)k * nc + c]; }, u, v, ro, p, q[1], q[2], q[3]);
+}
+
+template <class TS>
+__global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __restrict__ st,
+                                      const double* __restrict__ src, double* __restrict__ qtab)
+{
+    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double q[5];
+    analytic_q(prm, nc, c, st, src, q);
+#pragma unroll
+    for (int k = 0; k < 5; k++) qtab[k * nc + c] = q[k];
+}
+
+// Network output -> source table, with the analytic judge: a cell falls back to the
+// analytic source when any component misses by more than tol relative to
+// max(|q_analytic|, sd_k * scale), scale = src_coef * ro * n_H (the per-cell unit
+// of the normalised targets).
+template <class TS>
+__global__ void finalize_kernel(size_t c0, int n, size_t nc, gel_params prm, const TS* __restrict__ st,
+                                const double* __restrict__ src, const float* __restrict__ y,
+                                const float* __restrict__ ymu, const float* __restrict__ ysd, double tol,
+                                int judge, double* __restrict__ qtab, unsigned long long* __restrict__ nfall)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n) return;
+    const size_t c = c0 + b;
+    const double ro = st[c];
+    const double nH = src[c] + src[nc + c] + src[2 * nc + c] + src[3 * nc + c];
+    const double scale = prm.src_coef * ro * nH;
+    double qm[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) qm[k] = ((double)y[(size_t)b * kOutP + k] * ysd[k] + ymu[k]) * scale;
+    bool fall = false;
+    if (judge) {
+        double qa[5];
+        analytic_q(prm, nc, c, st, src, qa);
+        double err = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const double den = fmax(fabs(qa[k]), (double)ysd[k] * scale);
+            if (den > 0.0) err = fmax(err, fabs(qm[k] - qa[k]) / den);
+        }
+        if (!(err <= tol)) {
+            fall = true;
+#pragma unroll
+            for (int k = 0; k < 5; k++) qm[k] = qa[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 5; k++) qtab[k * nc + c] = qm[k];
+    if (fall) atomicAdd(nfall, 1ull);
+}
 }  // namespace gel
 
 using namespace gel;
@@ -257,6 +360,11 @@ struct gel_mlp {
     int nset = 0;
     float* set_x = nullptr;
     float* set_t = nullptr;
+    // normalisation of features and targets (device), set by gel_mlp_set_scaling
+    float* mu = nullptr;
+    float* sd = nullptr;
+    float* ymu = nullptr;
+    float* ysd = nullptr;
     hipStream_t stream = nullptr;
 };
 
@@ -384,6 +492,7 @@ void gel_mlp_destroy(gel_mlp* m)
         (void)hipFree(m->wb[l]); (void)hipFree(m->wt[l]);
     }
     (void)hipFree(m->set_x); (void)hipFree(m->set_t);
+    (void)hipFree(m->mu); (void)hipFree(m->sd); (void)hipFree(m->ymu); (void)hipFree(m->ysd);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -577,6 +686,136 @@ int gel_mlp_train(gel_mlp* m, int steps, int batch, float lr, float* loss_out)
     }
     if (hipStreamSynchronize(m->stream) != hipSuccess) return mfail(GEL_E_HIP, "train sync");
     if (hipGetLastError() != hipSuccess) return mfail(GEL_E_HIP, "train launch");
+    return GEL_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// Feature and target normalisation of the source model: x_n = (x - mu)/sd,
+// y = y_n * ysd + ymu (targets in units of src_coef * ro * n_H).
+int gel_mlp_set_scaling(gel_mlp* m, const float* mu, const float* sd, const float* ymu, const float* ysd)
+{
+    if (!m || !mu || !sd || !ymu || !ysd) return mfail(GEL_E_ARG, "null argument");
+    float** bufs[] = { &m->mu, &m->sd, &m->ymu, &m->ysd };
+    const float* src[] = { mu, sd, ymu, ysd };
+    const int n[] = { kIn, kIn, kOut, kOut };
+    for (int k = 0; k < 4; k++) {
+        if (!*bufs[k] && hipMalloc((void**)bufs[k], 16 * sizeof(float)) != hipSuccess) return mfail(GEL_E_NOMEM, "scaling");
+        if (hipMemcpy(*bufs[k], src[k], n[k] * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return mfail(GEL_E_HIP, "scaling copy");
+    }
+    return GEL_OK;
+}
+
+static int check_ctx(gel_ctx* c)
+{
+    if (!c) return mfail(GEL_E_ARG, "null context");
+    if (!c->uploaded) return mfail(GEL_E_STATE, "no state uploaded");
+    if (!c->d_src) return mfail(GEL_E_STATE, "hydrogen moments not set (gel_set_sources)");
+    return GEL_OK;
+}
+
+// Raw source-model features of every cell of the current state: x [ncells][14].
+int gel_surrogate_features(gel_ctx* c, float* x)
+{
+    int st = check_ctx(c);
+    if (st != GEL_OK) return st;
+    if (!x) return mfail(GEL_E_ARG, "null argument");
+    float* d = nullptr;
+    if (hipMalloc((void**)&d, c->nc * kIn * sizeof(float)) != hipSuccess) return mfail(GEL_E_NOMEM, "features");
+    const dim3 g((unsigned)((c->nc + 255) / 256));
+    if (c->ts == 8)
+        hipLaunchKernelGGL(features_kernel<double>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
+                           (const double*)c->d_state[c->cur], c->d_src, nullptr, nullptr, d);
+    else
+        hipLaunchKernelGGL(features_kernel<float>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
+                           (const float*)c->d_state[c->cur], c->d_src, nullptr, nullptr, d);
+    hipError_t e = hipMemcpyAsync(x, d, c->nc * kIn * sizeof(float), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return e == hipSuccess ? GEL_OK : mfail(GEL_E_HIP, "features");
+}
+
+// Analytic charge-exchange source table of the current state: q [5][ncells].
+int gel_source_analytic(gel_ctx* c, double* q)
+{
+    int st = check_ctx(c);
+    if (st != GEL_OK) return st;
+    if (!q) return mfail(GEL_E_ARG, "null argument");
+    double* d = nullptr;
+    if (hipMalloc((void**)&d, 5 * c->nc * sizeof(double)) != hipSuccess) return mfail(GEL_E_NOMEM, "table");
+    const dim3 g((unsigned)((c->nc + 255) / 256));
+    if (c->ts == 8)
+        hipLaunchKernelGGL(analytic_table_kernel<double>, g, dim3(256), 0, c->stream, c->nc, c->prm,
+                           (const double*)c->d_state[c->cur], c->d_src, d);
+    else
+        hipLaunchKernelGGL(analytic_table_kernel<float>, g, dim3(256), 0, c->stream, c->nc, c->prm,
+                           (const float*)c->d_state[c->cur], c->d_src, d);
+    hipError_t e = hipMemcpyAsync(q, d, 5 * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return e == hipSuccess ? GEL_OK : mfail(GEL_E_HIP, "analytic table");
+}
+
+// Refresh the context's source table from the network on the current state.  judge:
+// compare every cell with the analytic source and fall back where the error exceeds
+// tol; *n_fallback receives the number of such cells.  Runs on the context's stream.
+int gel_surrogate_refresh(gel_ctx* c, gel_mlp* m, double tol, int judge, long long* n_fallback)
+{
+    int st = check_ctx(c);
+    if (st != GEL_OK) return st;
+    if (!m || !m->mu) return mfail(GEL_E_STATE, "network scaling not set");
+    if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * sizeof(double)) != hipSuccess)
+        return mfail(GEL_E_NOMEM, "source table");
+    const int chunk = 65536;
+    st = mlp_reserve(m, chunk);
+    if (st != GEL_OK) return st;
+    float* feat = nullptr;
+    unsigned long long* dcount = nullptr;
+    if (hipMalloc((void**)&feat, c->nc * kIn * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&dcount, sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipFree(feat);
+        return mfail(GEL_E_NOMEM, "refresh buffers");
+    }
+    // the network runs on the context's stream so that the refresh orders with the sweep
+    hipStream_t saved = m->stream;
+    m->stream = c->stream;
+    hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream);
+    const dim3 g((unsigned)((c->nc + 255) / 256));
+    const bool f64 = (c->ts == 8);
+    if (f64)
+        hipLaunchKernelGGL(features_kernel<double>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
+                           (const double*)c->d_state[c->cur], c->d_src, m->mu, m->sd, feat);
+    else
+        hipLaunchKernelGGL(features_kernel<float>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
+                           (const float*)c->d_state[c->cur], c->d_src, m->mu, m->sd, feat);
+    for (size_t c0 = 0; c0 < c->nc && e == hipSuccess; c0 += chunk) {
+        const int n = (int)std::min<size_t>(chunk, c->nc - c0);
+        const int B = (n + 15) / 16 * 16;
+        hipLaunchKernelGGL(pack_input_kernel, dim3((B * kInP + 255) / 256), dim3(256), 0, c->stream, n, B,
+                           feat + c0 * kIn, m->x, m->xt);
+        mlp_forward_dev(m, B);
+        const dim3 gf((unsigned)((n + 255) / 256));
+        if (f64)
+            hipLaunchKernelGGL(finalize_kernel<double>, gf, dim3(256), 0, c->stream, c0, n, c->nc, c->prm,
+                               (const double*)c->d_state[c->cur], c->d_src, m->y, m->ymu, m->ysd, tol, judge,
+                               c->d_qtab, dcount);
+        else
+            hipLaunchKernelGGL(finalize_kernel<float>, gf, dim3(256), 0, c->stream, c0, n, c->nc, c->prm,
+                               (const float*)c->d_state[c->cur], c->d_src, m->y, m->ymu, m->ysd, tol, judge,
+                               c->d_qtab, dcount);
+        e = hipGetLastError();
+    }
+    unsigned long long cnt = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dcount, sizeof cnt, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    m->stream = saved;
+    (void)hipFree(feat);
+    (void)hipFree(dcount);
+    if (e != hipSuccess) return mfail(GEL_E_HIP, std::string("refresh: ") + hipGetErrorString(e));
+    if (n_fallback) *n_fallback = (long long)cnt;
     return GEL_OK;
 }
 

@@ -69,6 +69,11 @@ class Setka:
         + phase)), k = step mod period; steps become moving steps."""
         _lib.check(_lib.lib().gel_set_motion(self._ctx, float(amp), int(period), float(phase)))
 
+    def set_source_table(self, q: np.ndarray) -> None:
+        """Per-cell sources for GEL_SRC_TABLE: (mass, mom x, mom y, energy, tracer) x ncells."""
+        a = np.ascontiguousarray(q, np.float64).reshape(5, self.ncells)
+        _lib.check(_lib.lib().gel_set_source_table(self._ctx, _lib.dptr(a)))
+
     def set_rows_per_wave(self, rows: int) -> None:
         _lib.check(_lib.lib().gel_set_rows_per_wave(self._ctx, int(rows)))
 

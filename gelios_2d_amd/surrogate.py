@@ -21,27 +21,7 @@ def _f(a):
 
 
 def _setup(L):
-    if getattr(L, "_mlp_ready", False):
-        return L
-    L.gel_mlp_create.restype = ctypes.c_int
-    L.gel_mlp_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]
-    L.gel_mlp_destroy.restype = None
-    L.gel_mlp_destroy.argtypes = [ctypes.c_void_p]
-    L.gel_mlp_param_count.restype = ctypes.c_int
-    L.gel_mlp_param_count.argtypes = []
-    for name in ("gel_mlp_get_params", "gel_mlp_set_params"):
-        getattr(L, name).restype = ctypes.c_int
-        getattr(L, name).argtypes = [ctypes.c_void_p, _fp]
-    L.gel_mlp_forward.restype = ctypes.c_int
-    L.gel_mlp_forward.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp]
-    L.gel_mlp_grad.restype = ctypes.c_int
-    L.gel_mlp_grad.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp, _fp, _fp]
-    L.gel_mlp_set_dataset.restype = ctypes.c_int
-    L.gel_mlp_set_dataset.argtypes = [ctypes.c_void_p, ctypes.c_int, _fp, _fp]
-    L.gel_mlp_train.restype = ctypes.c_int
-    L.gel_mlp_train.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]
-    L.gel_mlp_last_error.restype = ctypes.c_char_p
-    L._mlp_ready = True
+    """Signatures live in _lib.SIGNATURES; kept as the single place a handle is made."""
     return L
 
 
@@ -104,3 +84,138 @@ class MLP:
         loss = np.empty(max(steps, 1), np.float32)
         _check(self._L.gel_mlp_train(self._h, int(steps), int(batch), float(lr), _f(loss)))
         return loss[:steps]
+
+
+# ------------------------------------------------------------------ source model ---
+
+def _setup_src(L):
+    return L
+
+
+def features(setka) -> np.ndarray:
+    """Raw source-model features [ncells][14] of a Setka's current state."""
+    L = _setup_src(_setup(_lib.lib()))
+    x = np.empty((setka.ncells, N_IN), np.float32)
+    _check(L.gel_surrogate_features(setka._ctx, _f(x)))
+    return x
+
+
+def analytic_sources(setka) -> np.ndarray:
+    """Analytic charge-exchange sources [5][ncells] (mass, mom x, mom y, energy, tracer)."""
+    L = _setup_src(_setup(_lib.lib()))
+    q = np.empty((5, setka.ncells))
+    _check(L.gel_source_analytic(setka._ctx, q.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return q
+
+
+def target_scale(setka, fields) -> np.ndarray:
+    """Per-cell unit of the targets: src_coef * ro * n_H (n_H summed over populations)."""
+    prim = setka.download()
+    return setka.params.src_coef * prim[0] * fields[0:4].sum(axis=0)
+
+
+class SourceModel:
+    """The MLP plus its feature / target scaling."""
+
+    def __init__(self, mlp: MLP, mu, sd, ymu, ysd):
+        self.mlp = mlp
+        self.mu = np.asarray(mu, np.float32)
+        self.sd = np.asarray(sd, np.float32)
+        self.ymu = np.asarray(ymu, np.float32)
+        self.ysd = np.asarray(ysd, np.float32)
+        L = _setup_src(_setup(_lib.lib()))
+        _check(L.gel_mlp_set_scaling(mlp._h, _f(self.mu), _f(self.sd), _f(self.ymu), _f(self.ysd)))
+
+    def refresh(self, setka, tol: float = 1e-3, judge: bool = True) -> int:
+        """Fill the Setka's source table from the network (GEL_SRC_TABLE); returns the
+        number of cells that fell back to the analytic source."""
+        L = _setup_src(_setup(_lib.lib()))
+        n = ctypes.c_longlong(0)
+        _check(L.gel_surrogate_refresh(setka._ctx, self.mlp._h, float(tol), 1 if judge else 0, ctypes.byref(n)))
+        return n.value
+
+    def save(self, path: str) -> None:
+        np.savez(path, params=self.mlp.get_params(), mu=self.mu, sd=self.sd, ymu=self.ymu, ysd=self.ysd)
+
+    @classmethod
+    def load(cls, path: str) -> "SourceModel":
+        d = np.load(path, allow_pickle=False)
+        m = MLP(0)
+        m.set_params(d["params"])
+        return cls(m, d["mu"], d["sd"], d["ymu"], d["ysd"])
+
+
+def inflow_sets(n_sets: int = 16, seed: int = 4):
+    """Seeded inflow parameter sets: Mach uniform in [1.5, 3], n_H in [0.05, 0.2]."""
+    rng = np.random.default_rng(seed)
+    return [(float(rng.uniform(1.5, 3.0)), float(rng.uniform(0.05, 0.2)), seed * 1000 + k) for k in range(n_sets)]
+
+
+def workload_with_hydrogen(ni: int, nj: int, steps: int, mach: float, nh: float, seed: int, moving: bool = True):
+    """cfg2 geometry and field with interstellar inflow Mach `mach` and hydrogen density
+    `nh`; charge exchange with sigma(U) = (1 - 0.07 ln U)^2.  Returns (workload, fields)."""
+    from .configs import R_BLEND, D_BLEND, mhd_heliosphere
+    from .hydrogen import populations
+    from .params import GEL_SRC_MK1
+    w = mhd_heliosphere(ni, nj, steps, seed=seed, moving=moving)
+    w.params.source_mode = GEL_SRC_MK1
+    w.params.sig_a, w.params.sig_b = 1.0, 0.07
+    u_inf = -w.params.inf_u
+    p_inf = w.params.inf_rho * u_inf ** 2 / (w.params.gamma * mach ** 2)
+    cx, cy = w.grid.cell_centers()
+    s = 0.5 * (1.0 - np.tanh((np.hypot(cx, cy) - R_BLEND) / D_BLEND))
+    w.prim0[1] = w.prim0[1] + (1 - s) * (p_inf - w.params.inf_p)
+    w.params.inf_p = p_inf
+    return w, populations(w.grid, n0=nh, u_inf=u_inf, seed=seed)
+
+
+def training_data(ni: int = 500, nj: int = 250, n_sets: int = 16, seed: int = 4, max_evolve: int = 600):
+    """Features and normalised-unit targets from the analytic closure on the device,
+    over n_sets inflow parameter sets (ni * nj cells each); set k is sampled after
+    k * max_evolve / n_sets steps of the sweep with analytic sources."""
+    from .setka import Setka
+    X, Y = [], []
+    for k, (mach, nh, s) in enumerate(inflow_sets(n_sets, seed)):
+        w, fields = workload_with_hydrogen(ni, nj, 0, mach, nh, s, moving=False)
+        st = Setka(w.grid, w.params)
+        st.upload(w.prim0)
+        st.set_sources(fields)
+        ev = k * max_evolve // max(n_sets, 1)
+        if ev:
+            st.Go_5_komponent__MK2(ev)
+        x = features(st)
+        q = analytic_sources(st)
+        scale = target_scale(st, fields)
+        X.append(x)
+        Y.append((q / scale).T.astype(np.float32))
+        st.close()
+    return np.concatenate(X), np.concatenate(Y)
+
+
+def train_source_model(X, Y, steps: int = 6000, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
+                       shuffle_seed: int = 1) -> SourceModel:
+    """Normalise, shuffle, and train the MLP on the device; returns the model."""
+    mu = X.mean(axis=0)
+    sd = X.std(axis=0)
+    sd = np.where(sd > 0, sd, 1.0)
+    ymu = Y.mean(axis=0)
+    ysd = Y.std(axis=0)
+    const = ~(ysd > 0)
+    ysd_train = np.where(const, 1.0, ysd)
+    rng = np.random.default_rng(shuffle_seed)
+    perm = rng.permutation(X.shape[0])
+    Xn = ((X - mu) / sd)[perm].astype(np.float32)
+    Yn = ((Y - ymu) / ysd_train)[perm].astype(np.float32)
+    m = MLP(seed)
+    m.set_dataset(Xn, Yn)
+    done = 0
+    history = []
+    while done < steps:           # step the learning rate down in three stages
+        k = min(steps - done, max(steps // 3, 1))
+        stage = done * 3 // max(steps, 1)
+        history.append(m.train(k, batch, lr * (0.3 ** stage)))
+        done += k
+    # constant targets are reproduced exactly by the device (ysd = 0)
+    model = SourceModel(m, mu, sd, ymu, np.where(const, 0.0, ysd))
+    model.loss_history = np.concatenate(history)
+    return model

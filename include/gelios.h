@@ -87,7 +87,11 @@ enum gel_source_mode {
     GEL_SRC_NONE = 0,
     /* Charge exchange with 4 frozen hydrogen populations, Cell::Get_Sourse_MK1
        (Cell.cpp:498-665); moments n,u,v,T per population and k_u,k_v,k_T per cell. */
-    GEL_SRC_MK1 = 1
+    GEL_SRC_MK1 = 1,
+    /* Per-cell source table (mass, momentum x, momentum y, energy, tracer per unit
+       r-weighted volume), set with gel_set_source_table or filled on the device by
+       gel_surrogate_refresh. */
+    GEL_SRC_TABLE = 2
 };
 
 /* Number of per-cell source inputs for GEL_SRC_MK1, variable-major like the state:
@@ -159,6 +163,11 @@ int gel_download_state(gel_ctx* ctx, double* prim);
  * (Parametr::H_n/H_u/H_v/H_T/k_u/k_v/k_T, Cell.h:55-66). */
 int gel_set_sources(gel_ctx* ctx, const double* fields);
 
+/* Per-cell source table for GEL_SRC_TABLE: 5 variable-major arrays of ncells doubles
+ * (mass, momentum x, momentum y, energy, tracer), applied like the sources q2_1, q2_2,
+ * q3 of the update (Setka.cpp:10444-10446). */
+int gel_set_source_table(gel_ctx* ctx, const double* q);
+
 /* Advance nsteps steps: replaces the body of the step loop of
  * Setka::Go_5_komponent__MK2 (Setka.cpp:10131-10481) on a static grid.
  * Steps run back to back on the context's stream without host round trips; the
@@ -210,6 +219,34 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
  * energy, B x/y/phi, tracer; smax: [n] max(|SL|,|SR|). */
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
                         const double* n2, double gamma, double* P, double* smax);
+
+/* ---- learned charge-exchange source (no counterpart in the reference, whose
+ * sources come from the Monte Carlo of Setka::MK_start_new, Setka.cpp:11989-12535,
+ * through the closure Cell::Get_Sourse_MK1, Cell.cpp:498-665) ----
+ * A 14 -> 4 x 128 SiLU -> 5 MLP with BF16 weights and activations on MFMA and FP32
+ * accumulation.  Parameters are packed per layer as W [out][in] then b [out]. */
+typedef struct gel_mlp gel_mlp;
+int gel_mlp_create(gel_mlp** out, uint64_t seed);
+void gel_mlp_destroy(gel_mlp* m);
+int gel_mlp_param_count(void);
+int gel_mlp_get_params(gel_mlp* m, float* p);
+int gel_mlp_set_params(gel_mlp* m, const float* p);
+/* y [n][5] = net(x [n][14]) on host arrays (normalised features). */
+int gel_mlp_forward(gel_mlp* m, int n, const float* x, float* y);
+/* Mean squared error and its gradient for a batch (n a multiple of 16). */
+int gel_mlp_grad(gel_mlp* m, int n, const float* x, const float* t, float* grad, float* loss);
+int gel_mlp_set_dataset(gel_mlp* m, int n, const float* x, const float* t);
+/* Adam over the stored set, batch rows in order; loss_out[steps] may be NULL. */
+int gel_mlp_train(gel_mlp* m, int steps, int batch, float lr, float* loss_out);
+int gel_mlp_set_scaling(gel_mlp* m, const float* mu, const float* sd, const float* ymu, const float* ysd);
+const char* gel_mlp_last_error(void);
+/* Raw features [ncells][14] and analytic sources [5][ncells] of a context's current
+ * state and hydrogen moments (gel_set_sources). */
+int gel_surrogate_features(gel_ctx* ctx, float* x);
+int gel_source_analytic(gel_ctx* ctx, double* q);
+/* Fill the context's source table (GEL_SRC_TABLE) from the network; with judge != 0
+ * cells whose error against the analytic source exceeds tol fall back to it. */
+int gel_surrogate_refresh(gel_ctx* ctx, gel_mlp* m, double tol, int judge, long long* n_fallback);
 
 const char* gel_strerror(int status);
 const char* gel_last_error(void);

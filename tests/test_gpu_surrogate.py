@@ -1,0 +1,79 @@
+"""The learned source on the GPU: device features, the source-table sweep mode, the
+per-cell judge/fallback, and what the trained network gains (ablation)."""
+import numpy as np
+import pytest
+
+from gelios_2d_amd.params import GEL_SRC_MK1, GEL_SRC_NONE, GEL_SRC_TABLE
+from gelios_2d_amd.state import l1_rel, rel_err_conserved
+
+pytestmark = pytest.mark.gpu
+
+
+def _setka(w, fields, mode):
+    from gelios_2d_amd.setka import Setka
+    prm = w.params.copy()
+    prm.source_mode = mode
+    s = Setka(w.grid, prm)
+    if w.motion:
+        s.set_motion(*w.motion)
+    s.upload(w.prim0)
+    s.set_sources(fields)
+    return s
+
+
+def test_features_match_host():
+    from gelios_2d_amd import surrogate as sg
+    w, f = sg.workload_with_hydrogen(60, 40, 0, 2.0, 0.1, 3, moving=False)
+    s = _setka(w, f, GEL_SRC_MK1)
+    x = sg.features(s)
+    ro, p, u, v = w.prim0[:4]
+    cx, cy = w.grid.cell_centers()
+    ref = [np.log(ro), u, v, np.log(p / ro)]
+    for g in range(2):
+        a, b = 2 * g, 2 * g + 1
+        n = f[a] + f[b]
+        ref += [np.log(n), (f[a] * f[4 + a] + f[b] * f[4 + b]) / n, (f[a] * f[8 + a] + f[b] * f[8 + b]) / n,
+                np.log((f[a] * f[12 + a] + f[b] * f[12 + b]) / n)]
+    ref += [cy, cx]
+    ref = np.stack(ref, 1)
+    assert np.allclose(x, ref, rtol=1e-5, atol=1e-5)
+
+
+def test_table_mode_is_the_analytic_source_for_one_step():
+    """A table holding the analytic sources of the current state reproduces one step
+    of the sweep with the analytic closure (Cell::Get_Sourse_MK1, Cell.cpp:498-665)."""
+    from gelios_2d_amd import surrogate as sg
+    w, f = sg.workload_with_hydrogen(80, 64, 1, 2.5, 0.15, 5)
+    a = _setka(w, f, GEL_SRC_MK1)
+    b = _setka(w, f, GEL_SRC_TABLE)
+    b.set_source_table(sg.analytic_sources(b))
+    a.Go_5_komponent__MK2(1)
+    b.Go_5_komponent__MK2(1)
+    assert rel_err_conserved(b.download(), a.download()) <= 1e-14
+
+
+def test_trained_network_judge_and_gain():
+    from gelios_2d_amd import surrogate as sg
+    X, Y = sg.training_data(ni=120, nj=60, n_sets=6, seed=4, max_evolve=120)
+    model = sg.train_source_model(X, Y, steps=2400, batch=2048, lr=2e-3)
+    assert model.loss_history[-100:].mean() < 0.05 * model.loss_history[:10].mean()
+
+    # held-out inflow set
+    w, f = sg.workload_with_hydrogen(120, 60, 60, 2.2, 0.12, 99)
+    ana = _setka(w, f, GEL_SRC_MK1)
+    net = _setka(w, f, GEL_SRC_TABLE)
+    nfall = model.refresh(net, tol=1e-3, judge=True)
+    assert 0 <= nfall < net.ncells
+
+    # ablation over 60 steps, sources refreshed every 10 steps from the current state
+    none = _setka(w, f, GEL_SRC_NONE)
+    for k in range(6):
+        ana.Go_5_komponent__MK2(10)
+        model.refresh(net, tol=1e-3, judge=True)
+        net.Go_5_komponent__MK2(10)
+        none.Go_5_komponent__MK2(10)
+    A, N, Z = ana.download(), net.download(), none.download()
+    err_net = rel_err_conserved(N, A)
+    err_none = rel_err_conserved(Z, A)
+    print(f"fallback cells {nfall}/{net.ncells}; error vs analytic: network {err_net:.3e}, no source {err_none:.3e}")
+    assert err_net < 0.1 * err_none
