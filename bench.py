@@ -7,9 +7,15 @@ kernel); a cell-update is one cell advanced by one step.  Inputs are resident in
 HBM when the timed region starts.  Prints one JSON line.
 
 Workloads
-  cfg2 (default)   HLLD MHD, 9 variables, 800 x 400 cells, moving grid, 1000 steps:
-                   the FP32-bulk sweep (value) beside the FP64 plain sweep, with the
-                   density / pressure L1 error of the former against the latter
+  cfg5 (default)   the metric's comparison: 64 heliosphere cases (100x50 .. 800x400
+                   cells, seeded inflow / field / hydrogen), FP32-bulk HLLD with the
+                   learned charge-exchange source, one persistent launch per step,
+                   HIP-graph replay, launch shape from the autotuner, 500 steps (value);
+                   beside it the FP64 plain sweep of cfg2 and the density / pressure L1
+                   error of the batch against an FP64 run of the same cases with the
+                   analytic sources
+  cfg2             HLLD MHD, 9 variables, 800 x 400 cells, moving grid, 1000 steps:
+                   the FP32-bulk sweep (value) beside the FP64 plain sweep
   cfg1             single-fluid HLLC, 200 x 100 cells, 100 steps, FP64
   large-mhd        cfg2 physics on 4096 x 2048 cells (state far beyond the 256 MB
                    Infinity Cache): the HBM roofline of the FP64 sweep
@@ -79,6 +85,67 @@ def run_gpu(w, K, W, label):
             "dt_next": st.dt_next}
 
 
+def run_cfg5(steps, refresh_every, log_prefix="cfg5"):
+    """Build the 64 cases, refresh their sources from the network, autotune, then
+    time `steps` steps (graph replays, with a source refresh every refresh_every
+    steps inside the timed region).  Returns (result dict, final states)."""
+    import numpy as np
+    from gelios_2d_amd.batch import Batch
+    from gelios_2d_amd.configs import config5_cases
+    from gelios_2d_amd.setka import Setka
+    from gelios_2d_amd.surrogate import SourceModel
+    from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    model = SourceModel.load(DEFAULT_OUT)
+    specs = config5_cases(64, seed=5, steps=steps)
+    cases = []
+    for w, f in specs:
+        s = Setka(w.grid, w.params)
+        s.upload(w.prim0)
+        s.set_sources(f)
+        cases.append(s)
+    nfall = sum(model.refresh(s, tol=1e-3, judge=True) for s in cases)
+    ncells = sum(s.ncells for s in cases)
+    b = Batch(cases)
+    rows, wg, ms_trial = b.autotune(6)
+    log(f"bench[{log_prefix}]: {len(cases)} cases, {ncells} cells; autotuner rows {rows} workgroups {wg} "
+        f"({ms_trial * 1e3:.1f} us/step); judge fallback {nfall}/{ncells} cells")
+    b.step(6, True)                               # warm-up (graph capture)
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    done = 0
+    while done < steps:
+        k = min(refresh_every, steps - done)
+        dev_ms += b.step(k, True)
+        done += k
+        if done < steps:
+            for s in cases:
+                model.refresh(s, tol=1e-3, judge=True)
+    wall = time.perf_counter() - t0
+    finals = [s.download() for s in cases]
+    return ({"value": ncells * steps / wall, "wall_s": wall, "cells": ncells, "cases": len(cases),
+             "autotune": {"rows": rows, "workgroups": wg}, "fallback_cells": nfall,
+             "kernel_ms_per_step": dev_ms / steps}, finals, specs)
+
+
+def run_cfg5_reference(specs, steps):
+    """The same cases in FP64 with the analytic sources every step (batched)."""
+    from gelios_2d_amd.batch import Batch
+    from gelios_2d_amd.params import GEL_PREC_FP64, GEL_SRC_MK1
+    from gelios_2d_amd.setka import Setka
+    cases = []
+    for w, f in specs:
+        prm = w.params.copy()
+        prm.precision = GEL_PREC_FP64
+        prm.source_mode = GEL_SRC_MK1
+        s = Setka(w.grid, prm)
+        s.upload(w.prim0)
+        s.set_sources(f)
+        cases.append(s)
+    b = Batch(cases)
+    b.step(steps + 6, True)
+    return [s.download() for s in cases]
+
+
 def cpu_baseline(w, budget_s=12.0):
     """The CPU restatement of the reference sweep (per-cell gather over faces, every
     interior face solved from both sides, FP64, OpenMP over cells, as
@@ -116,7 +183,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg2")
+    ap.add_argument("--workload", default="cfg5")
+    ap.add_argument("--refresh-every", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -131,7 +199,44 @@ def main():
     wl = args.workload
     W = args.warmup
     extra = {}
-    if wl == "cfg2":
+    if wl == "cfg5":
+        import numpy as np
+        K = args.steps if args.steps is not None else 500
+        w64 = configs.config2(GEL_PREC_FP64)
+        log(f"bench: cfg5 batched ensemble, {K} steps; FP64 plain reference workload {w64.name}")
+        r64 = run_gpu(w64, 1000, W, "cfg2-fp64-plain")
+        r5, finals, specs = run_cfg5(K, args.refresh_every)
+        # the timed run did 6 warm-up steps first: the reference runs K + 6 steps
+        ref = run_cfg5_reference(specs, K)
+        num = {0: 0.0, 1: 0.0}
+        den = {0: 0.0, 1: 0.0}
+        for a, b in zip(finals, ref):
+            for k in (0, 1):
+                num[k] += float(np.abs(a[k] - b[k]).sum())
+                den[k] += float(np.abs(b[k]).sum())
+        l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
+        log(f"bench[cfg5]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
+            f"vs FP64 plain cfg2 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
+        B = 2 * 9 * 4 + 32 + 32      # FP32 state, nodes, source / hydrogen moments per cell-update
+        achieved = B * r5["cells"] / (r5["kernel_ms_per_step"] * 1e-3) / 1e9
+        main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
+                  "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
+                               "kernel_avg_us": r5["kernel_ms_per_step"] * 1e3}}
+        main_w = None
+        extra = {
+            "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / 1000,
+                           "roofline": r64["roofline"]},
+            "speedup_vs_fp64_plain": r5["value"] / r64["value"],
+            "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + 6),
+            "batch": {"cases": r5["cases"], "cells": r5["cells"], "autotune": r5["autotune"],
+                      "source_refresh_every": args.refresh_every, "judge_tol": 1e-3,
+                      "judge_fallback_cells_initial": r5["fallback_cells"]},
+        }
+        precision, dtype = "fp32-bulk + learned source (BF16 MFMA), batched, HIP graph", "f32"
+        config = {"workload": "cfg5-batched-64-cases-fp32-surrogate", "cells": r5["cells"], "cases": r5["cases"],
+                  "physics": "mhd-hlld", "moving_grid": False, "precision": precision, "cfl": 0.4}
+    elif wl == "cfg2":
         w64, w32 = configs.config2(GEL_PREC_FP64), configs.config2(GEL_PREC_FP32_BULK)
         K = args.steps if args.steps is not None else w64.steps
         log(f"bench: {w64.name} / {w32.name}, {w64.grid.ncells} cells, warmup {W}, steps {K}")
@@ -145,9 +250,9 @@ def main():
             "l1_vs_fp64": {"density": l1_rel(r32["final"], r64["final"], 0),
                            "pressure": l1_rel(r32["final"], r64["final"], 1),
                            "after_steps": W + K},
-            "surrogate": "not in this line: cfg2 carries no charge-exchange sources",
         }
         precision, dtype = "fp32-bulk (FP32 state and face arithmetic; FP64 geometry, accumulation, update)", "f32"
+        config = None
     else:
         if wl == "cfg1":
             w = configs.config1()
@@ -161,6 +266,7 @@ def main():
         log(f"bench: {w.name}, {w.grid.ncells} cells, warmup {W}, steps {K}")
         main_w, main_r = w, run_gpu(w, K, W, "fp64-plain")
         precision, dtype = "fp64-plain", "f64"
+        config = None
 
     line = {
         "metric": METRIC,
@@ -175,15 +281,15 @@ def main():
         "vs_baseline": None,
         "dtype": dtype,
         "data": "synthetic",
-        "config": {"workload": main_w.name, "cells": main_w.grid.ncells, "grid": [main_w.grid.ni, main_w.grid.nj],
-                   "physics": "mhd-hlld" if main_w.params.physics == 1 else
-                   ("gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
-                   "moving_grid": bool(main_w.motion), "precision": precision, "cfl": main_w.params.cfl},
+        "config": config if config is not None else {
+            "workload": main_w.name, "cells": main_w.grid.ncells, "grid": [main_w.grid.ni, main_w.grid.nj],
+            "physics": "mhd-hlld" if main_w.params.physics == 1 else ("gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
+            "moving_grid": bool(main_w.motion), "precision": precision, "cfl": main_w.params.cfl},
         "roofline": main_r["roofline"],
     }
     line.update(extra)
     if not args.no_cpu:
-        cw = main_w if wl != "cfg2" else w64
+        cw = main_w if wl not in ("cfg2", "cfg5") else w64
         line["cpu_baseline"] = cpu_baseline(cw)
         log(f"bench: cpu baseline {line['cpu_baseline']['value']:.3e} cell-updates/s")
     print(json.dumps(line), flush=True)

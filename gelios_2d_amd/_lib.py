@@ -70,6 +70,13 @@ SIGNATURES = {
     "gel_source_analytic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_surrogate_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
                                              ctypes.POINTER(ctypes.c_longlong)]),
+    "gel_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "gel_batch_destroy": (None, [ctypes.c_void_p]),
+    "gel_batch_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    "gel_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
+    "gel_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int), _dp]),
     "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gel_last_error": (ctypes.c_char_p, []),
 }

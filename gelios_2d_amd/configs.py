@@ -113,3 +113,35 @@ def config2(precision: int = GEL_PREC_FP64) -> Workload:
     """HLLD MHD, 800 x 400 cells, moving grid, 1000 steps, CFL 0.4, seed 2."""
     tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
     return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"cfg2-hlld-{tag}-800x400-moving")
+
+
+def config5_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None):
+    """Batched ensemble: grid sizes log-uniform from 100 x 50 to 800 x 400, inflow Mach
+    uniform in [1.5, 3], field angle uniform in [0, 90] deg, |B| uniform in [0.02, 0.1],
+    hydrogen density uniform in [0.05, 0.2]; FP32-bulk HLLD with the learned source.
+    Returns a list of (workload, hydrogen fields)."""
+    from .params import GEL_PREC_FP32_BULK, GEL_SRC_TABLE
+    from .surrogate import workload_with_hydrogen
+    if precision is None:
+        precision = GEL_PREC_FP32_BULK
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_cases):
+        ni = int(round(math.exp(rng.uniform(math.log(100), math.log(800)))))
+        nj = max(ni // 2, 3)
+        mach = float(rng.uniform(1.5, 3.0))
+        ang = float(rng.uniform(0.0, 90.0))
+        bmag = float(rng.uniform(0.02, 0.1))
+        nh = float(rng.uniform(0.05, 0.2))
+        w, f = workload_with_hydrogen(ni, nj, steps, mach, nh, seed * 1000 + k, moving=False)
+        a = math.radians(ang)
+        w.params.inf_bx, w.params.inf_by = bmag * math.cos(a), bmag * math.sin(a)
+        cx, cy = w.grid.cell_centers()
+        s = 0.5 * (1.0 - np.tanh((np.hypot(cx, cy) - R_BLEND) / D_BLEND))
+        w.prim0[5] = (1 - s) * w.params.inf_bx
+        w.prim0[6] = (1 - s) * w.params.inf_by
+        w.params.precision = precision
+        w.params.source_mode = GEL_SRC_TABLE
+        w.name = f"cfg5-case{k:02d}-{ni}x{nj}"
+        out.append((w, f))
+    return out

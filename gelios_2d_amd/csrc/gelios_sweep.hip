@@ -507,16 +507,15 @@ __device__ __forceinline__ int wave_sum(int v)
 // arithmetic (reconstruction and Riemann solver); geometry, accumulation and the
 // update are FP64 throughout.
 template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
-__global__ void __launch_bounds__(kWave)
-sweep_kernel(SweepArgs a)
+__device__ __forceinline__ void sweep_tile(const SweepArgs& a, int bx, int by)
 {
     constexpr int NV = Ph::NV;
     using C = CellT<Ph, TS>;
     const int lane = threadIdx.x;
-    const int j = blockIdx.x * kOut + lane - kHalo;
+    const int j = bx * kOut + lane - kHalo;
     const bool col = (j >= 0 && j < a.nj);
     const bool out_lane = col && lane >= kHalo && lane < kWave - kHalo;
-    const int i0 = blockIdx.y * a.rows;
+    const int i0 = by * a.rows;
     const int i1 = min(i0 + a.rows, a.ni);
     const double gam = a.prm.gamma;
     const double cfl = a.prm.cfl;
@@ -526,7 +525,7 @@ sweep_kernel(SweepArgs a)
     const bool n_axis = (j + 1 == a.nj - 1);
     const size_t nc = (size_t)a.ni * a.nj;
 
-    if (blockIdx.x == 0 && blockIdx.y == 0 && lane == 0)
+    if (bx == 0 && by == 0 && lane == 0)
         a.dtbuf[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
 
     // window: cells of rows i-1, i, i+1, i+2 and node rows i .. i+3
@@ -691,6 +690,31 @@ sweep_kernel(SweepArgs a)
         if (s_rho) atomicAdd(&a.counters[0], (unsigned long long)s_rho);
         if (s_p) atomicAdd(&a.counters[1], (unsigned long long)s_p);
         if (s_bad) atomicAdd(&a.counters[2], (unsigned long long)s_bad);
+    }
+}
+
+// One step of one case: a wavefront per (strip, chunk) tile.
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
+__global__ void __launch_bounds__(kWave)
+sweep_kernel(SweepArgs a)
+{
+    sweep_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX>(a, blockIdx.x, blockIdx.y);
+}
+
+// One step of many independent cases in one persistent launch: the wavefronts walk
+// the (case, strip, chunk) work list round-robin; args[case] carries each case's
+// grid, buffers, time-step slots and parameters.
+struct BatchItem {
+    int cs, bx, by;
+};
+
+template <class Ph, class TS, class TF, int SRC, int FLUX>
+__global__ void __launch_bounds__(kWave)
+sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
+{
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const BatchItem w = items[it];
+        sweep_tile<Ph, TS, TF, false, false, SRC, FLUX>(args[w.cs], w.bx, w.by);
     }
 }
 
@@ -1229,6 +1253,289 @@ int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W
                          hipLaunchKernelGGL(hlld_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, i[0], i[1],
                                             i[2], i[3], i[4], gamma, d[0], d[1]);
                      });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched cases ---
+
+struct gel_batch {
+    std::vector<gel_ctx*> cases;
+    int rows = 0;                 // rows per chunk for every case; 0 = per-case default
+    int workgroups = 0;           // persistent grid; 0 = min(items, 8 per CU)
+    SweepArgs* d_args = nullptr;  // [6][ncases]: one argument set per phase of the cycle
+    BatchItem* d_items = nullptr;
+    int nitems = 0;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+static constexpr int kCycle = 6;  // two state buffers x three time-step slots
+
+static int batch_rows(const gel_batch* b, const gel_ctx* c) { return b->rows > 0 ? b->rows : choose_rows(c->ni, c->nj); }
+
+static void batch_drop_graph(gel_batch* b)
+{
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->graph_valid = false;
+}
+
+static int batch_build_items(gel_batch* b)
+{
+    std::vector<BatchItem> items;
+    for (int cs = 0; cs < (int)b->cases.size(); cs++) {
+        const gel_ctx* c = b->cases[cs];
+        const int rows = batch_rows(b, c);
+        const int strips = (c->nj + kOut - 1) / kOut, chunks = (c->ni + rows - 1) / rows;
+        for (int by = 0; by < chunks; by++)
+            for (int bx = 0; bx < strips; bx++) items.push_back(BatchItem{ cs, bx, by });
+    }
+    (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    if (hipMalloc(&b->d_items, items.size() * sizeof(BatchItem)) != hipSuccess) return fail(GEL_E_NOMEM, "batch items");
+    HIP_OK(hipMemcpy(b->d_items, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice));
+    b->nitems = (int)items.size();
+    batch_drop_graph(b);
+    return GEL_OK;
+}
+
+// argument sets of the next kCycle steps, from each case's own buffer/slot phase
+static int batch_upload_args(gel_batch* b)
+{
+    const size_t n = b->cases.size();
+    std::vector<SweepArgs> args(kCycle * n);
+    for (int p = 0; p < kCycle; p++)
+        for (size_t cs = 0; cs < n; cs++) {
+            gel_ctx* c = b->cases[cs];
+            SweepArgs& a = args[p * n + cs];
+            const long long st = c->steps + p;
+            const int cur = c->cur ^ (p & 1);
+            a.ni = c->ni; a.nj = c->nj; a.rows = batch_rows(b, c);
+            a.dt_rd = (int)(st % 3); a.dt_wr = (int)((st + 1) % 3); a.dt_clr = (int)((st + 2) % 3);
+            a.s_old = 1.0; a.s_new = 1.0;
+            a.x = c->d_x; a.y = c->d_y; a.x2 = c->d_x; a.y2 = c->d_y;
+            a.now = c->d_state[cur];
+            a.next = c->d_state[cur ^ 1];
+            a.ctype = nullptr; a.otype = c->d_otype;
+            a.src = c->d_src; a.qtab = c->d_qtab;
+            a.dtbuf = c->d_dt; a.counters = c->d_cnt;
+            a.prm = c->prm;
+        }
+    HIP_OK(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(SweepArgs), hipMemcpyHostToDevice, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    return GEL_OK;
+}
+
+template <class Ph, class TS, class TF, int SRC, int FLUX>
+static void batch_launch_t(const gel_batch* b, int phase, int grid)
+{
+    hipLaunchKernelGGL((sweep_batch_kernel<Ph, TS, TF, SRC, FLUX>), dim3(grid), dim3(kWave), 0, b->stream,
+                       b->d_args + (size_t)phase * b->cases.size(), b->d_items, b->nitems);
+}
+
+template <class Ph, class TS, class TF, int FLUX>
+static void batch_launch_s(const gel_batch* b, int phase, int grid, int src)
+{
+    if (src == GEL_SRC_MK1) batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(b, phase, grid);
+    else if (src == GEL_SRC_TABLE) batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(b, phase, grid);
+    else batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(b, phase, grid);
+}
+
+static int batch_grid(const gel_batch* b)
+{
+    int g = b->workgroups > 0 ? b->workgroups : 256 * 8;
+    return g < b->nitems ? g : b->nitems;
+}
+
+static hipError_t batch_launch(const gel_batch* b, int phase)
+{
+    const gel_params& p = b->cases[0]->prm;
+    const int grid = batch_grid(b);
+    const bool f32 = p.precision == GEL_PREC_FP32_BULK;
+    if (p.physics == GEL_PHYS_MHD) {
+        if (f32) batch_launch_s<MhdPhys, float, float, 3>(b, phase, grid, p.source_mode);
+        else batch_launch_s<MhdPhys, double, double, 3>(b, phase, grid, p.source_mode);
+    } else if (p.flux == GEL_FLUX_HLLC) {
+        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLLC>(b, phase, grid, p.source_mode);
+        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLLC>(b, phase, grid, p.source_mode);
+    } else {
+        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLL>(b, phase, grid, p.source_mode);
+        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLL>(b, phase, grid, p.source_mode);
+    }
+    return hipGetLastError();
+}
+
+static void batch_advance(gel_batch* b, int n)
+{
+    for (gel_ctx* c : b->cases) {
+        c->steps += n;
+        if (n & 1) c->cur ^= 1;
+    }
+}
+
+extern "C" {
+
+void gel_batch_destroy(gel_batch* b)
+{
+    if (!b) return;
+    batch_drop_graph(b);
+    (void)hipFree(b->d_args);
+    (void)hipFree(b->d_items);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
+{
+    if (!out || ncases <= 0 || !cases) return fail(GEL_E_ARG, "null argument");
+    *out = nullptr;
+    const gel_params& p0 = cases[0]->prm;
+    for (int k = 0; k < ncases; k++) {
+        const gel_ctx* c = cases[k];
+        if (!c || !c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
+        if (c->has_types || !c->motion.empty()) return fail(GEL_E_UNSUPPORTED, "batched cases are static grids without C_1/C_centr");
+        if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
+            c->prm.source_mode != p0.source_mode)
+            return fail(GEL_E_UNSUPPORTED, "batched cases share physics, precision, flux and source mode");
+        if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
+        if (c->prm.source_mode == GEL_SRC_TABLE && !c->d_qtab) return fail(GEL_E_STATE, "source table not set");
+    }
+    gel_batch* b = new (std::nothrow) gel_batch();
+    if (!b) return fail(GEL_E_NOMEM, "host");
+    b->cases.assign(cases, cases + ncases);
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipMalloc(&b->d_args, (size_t)kCycle * ncases * sizeof(SweepArgs)) != hipSuccess) {
+        gel_batch_destroy(b);
+        return fail(GEL_E_NOMEM, "batch");
+    }
+    const int st = batch_build_items(b);
+    if (st != GEL_OK) { gel_batch_destroy(b); return st; }
+    *out = b;
+    return GEL_OK;
+}
+
+int gel_batch_set_launch(gel_batch* b, int rows, int workgroups)
+{
+    if (!b || rows < 0 || rows > 4096 || workgroups < 0) return fail(GEL_E_ARG, "launch shape");
+    b->rows = rows;
+    b->workgroups = workgroups;
+    return batch_build_items(b);
+}
+
+// Advance every case nsteps steps.  use_graph: whole cycles of kCycle steps replay a
+// captured HIP graph of kCycle launches; the remainder runs as plain launches.
+// *ms (may be NULL) receives the device time of the call (HIP events on the batch stream).
+int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
+{
+    if (!b || nsteps < 0) return fail(GEL_E_ARG, "bad argument");
+    for (gel_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
+    int st = batch_upload_args(b);
+    if (st != GEL_OK) return st;
+    HIP_OK(hipEventRecord(b->ev0, b->stream));
+    int done = 0;
+    if (use_graph && nsteps >= kCycle) {
+        if (!b->graph_valid) {
+            hipGraph_t g = nullptr;
+            HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
+            hipError_t e = hipSuccess;
+            for (int p = 0; p < kCycle && e == hipSuccess; p++) e = batch_launch(b, p);
+            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+            if (e != hipSuccess || e2 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(GEL_E_HIP, "graph capture");
+            }
+            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(GEL_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+            b->graph_valid = true;
+        }
+        while (nsteps - done >= kCycle) {
+            HIP_OK(hipGraphLaunch(b->exec, b->stream));
+            done += kCycle;
+        }
+    }
+    for (int p = 0; done < nsteps; p++, done++) {
+        const hipError_t e = batch_launch(b, p % kCycle);
+        if (e != hipSuccess) return fail(GEL_E_HIP, std::string("batch launch: ") + hipGetErrorString(e));
+    }
+    HIP_OK(hipEventRecord(b->ev1, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    if (ms) {
+        float t = 0.f;
+        HIP_OK(hipEventElapsedTime(&t, b->ev0, b->ev1));
+        *ms = t;
+    }
+    batch_advance(b, nsteps);
+    return GEL_OK;
+}
+
+// Project autotuner: time trial_steps steps for each candidate (rows per chunk,
+// persistent workgroups) on the batch's own cases, restoring every case's state,
+// time-step slots and counters after each trial; keep the fastest.
+int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_wg, double* best_ms)
+{
+    if (!b || trial_steps < 1) return fail(GEL_E_ARG, "bad argument");
+    struct Saved { void* st[2]; unsigned long long dt[3]; int cur; long long steps; };
+    std::vector<Saved> saved(b->cases.size());
+    for (size_t k = 0; k < b->cases.size(); k++) {
+        gel_ctx* c = b->cases[k];
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const size_t bytes = (size_t)c->nv * c->nc * c->ts;
+        for (int q = 0; q < 2; q++) {
+            if (hipMalloc(&saved[k].st[q], bytes) != hipSuccess) return fail(GEL_E_NOMEM, "autotune snapshot");
+            HIP_OK(hipMemcpy(saved[k].st[q], c->d_state[q], bytes, hipMemcpyDeviceToDevice));
+        }
+        HIP_OK(hipMemcpy(saved[k].dt, c->d_dt, sizeof saved[k].dt, hipMemcpyDeviceToHost));
+        saved[k].cur = c->cur;
+        saved[k].steps = c->steps;
+    }
+    auto restore = [&]() -> int {
+        for (size_t k = 0; k < b->cases.size(); k++) {
+            gel_ctx* c = b->cases[k];
+            const size_t bytes = (size_t)c->nv * c->nc * c->ts;
+            for (int q = 0; q < 2; q++) HIP_OK(hipMemcpy(c->d_state[q], saved[k].st[q], bytes, hipMemcpyDeviceToDevice));
+            HIP_OK(hipMemcpy(c->d_dt, saved[k].dt, sizeof saved[k].dt, hipMemcpyHostToDevice));
+            c->cur = saved[k].cur;
+            c->steps = saved[k].steps;
+        }
+        return GEL_OK;
+    };
+    const int rows_c[] = { 1, 2, 4, 8, 16, 32 };
+    const int wg_c[] = { 1024, 2048, 4096, 8192 };
+    double best = 1e300;
+    int br = 0, bw = 0;
+    int st = GEL_OK;
+    for (int r : rows_c) {
+        for (int w : wg_c) {
+            b->rows = r;
+            b->workgroups = w;
+            st = batch_build_items(b);
+            if (st != GEL_OK) break;
+            double ms = 0.0;
+            st = gel_batch_step(b, trial_steps, 0, &ms);
+            if (st != GEL_OK) break;
+            st = restore();
+            if (st != GEL_OK) break;
+            if (ms < best) { best = ms; br = r; bw = w; }
+        }
+        if (st != GEL_OK) break;
+    }
+    for (auto& s : saved) { (void)hipFree(s.st[0]); (void)hipFree(s.st[1]); }
+    if (st != GEL_OK) return st;
+    b->rows = br;
+    b->workgroups = bw;
+    st = batch_build_items(b);
+    if (st != GEL_OK) return st;
+    if (best_rows) *best_rows = br;
+    if (best_wg) *best_wg = bw;
+    if (best_ms) *best_ms = best / trial_steps;
+    return GEL_OK;
 }
 
 }  // extern "C"

@@ -220,6 +220,21 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
                         const double* n2, double gamma, double* P, double* smax);
 
+/* ---- batched cases (no counterpart in the reference, which runs one grid) ----
+ * Many independent contexts (static grids, same physics / precision / flux / source
+ * mode) advanced together: one persistent launch per step walks the (case, strip,
+ * chunk) work list; each case keeps its own state, buffers and CFL time step.
+ * Whole cycles of 6 steps replay a captured HIP graph. */
+typedef struct gel_batch gel_batch;
+int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases);
+void gel_batch_destroy(gel_batch* b);
+/* Rows per chunk (0: each case's default) and persistent workgroups (0: automatic). */
+int gel_batch_set_launch(gel_batch* b, int rows, int workgroups);
+/* Advance all cases nsteps steps; *ms (may be NULL) gets the device time. */
+int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms);
+/* Time candidate launch shapes on the cases (states restored) and keep the fastest. */
+int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_workgroups, double* ms_per_step);
+
 /* ---- learned charge-exchange source (no counterpart in the reference, whose
  * sources come from the Monte Carlo of Setka::MK_start_new, Setka.cpp:11989-12535,
  * through the closure Cell::Get_Sourse_MK1, Cell.cpp:498-665) ----
