@@ -7,19 +7,19 @@ kernel); a cell-update is one cell advanced by one step.  Inputs are resident in
 HBM when the timed region starts.  Prints one JSON line.
 
 Workloads
-  cfg5 (default)   the metric's comparison: 64 heliosphere cases (100x50 .. 800x400
+  ensemble (default)  the metric's comparison: 64 heliosphere cases (100x50 .. 800x400
                    cells, seeded inflow / field / hydrogen), FP32-bulk HLLD with the
                    learned charge-exchange source, one persistent launch per step,
                    HIP-graph replay, launch shape from the autotuner, 500 steps (value);
-                   beside it the FP64 plain sweep of cfg2 and the density / pressure L1
+                   beside it the FP64 plain sweep of hlld-800x400 and the density / pressure L1
                    error of the batch against an FP64 run of the same cases with the
                    analytic sources
-  cfg2             HLLD MHD, 9 variables, 800 x 400 cells, moving grid, 1000 steps:
+  hlld-800x400     HLLD MHD, 9 variables, 800 x 400 cells, moving grid, 1000 steps:
                    the FP32-bulk sweep (value) beside the FP64 plain sweep
-  cfg1             single-fluid HLLC, 200 x 100 cells, 100 steps, FP64
-  large-mhd        cfg2 physics on 4096 x 2048 cells (state far beyond the 256 MB
+  hllc-200x100     single-fluid HLLC, 200 x 100 cells, 100 steps, FP64
+  large-mhd        hlld-800x400 physics on 4096 x 2048 cells (state far beyond the 256 MB
                    Infinity Cache): the HBM roofline of the FP64 sweep
-  large-gas        cfg1 physics on 4096 x 2048 cells, FP64
+  large-gas        hllc-200x100 physics on 4096 x 2048 cells, FP64
 """
 from __future__ import annotations
 
@@ -43,7 +43,7 @@ def log(*a):
 def bytes_per_cell_update(w):
     """Algorithmic HBM bytes per cell-update: state read once and written once, node
     coordinates read (16 B), plus 16 B of written node coordinates on a moving grid
-    (the per-unit figures of the survey's measurement table)."""
+    (per cell-update: 2 x variables x bytes per value + node bytes)."""
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK
     nv = 9 if w.params.physics == 1 else 5
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
@@ -85,18 +85,18 @@ def run_gpu(w, K, W, label):
             "dt_next": st.dt_next}
 
 
-def run_cfg5(steps, refresh_every, log_prefix="cfg5"):
+def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
     """Build the 64 cases, refresh their sources from the network, autotune, then
     time `steps` steps (graph replays, with a source refresh every refresh_every
     steps inside the timed region).  Returns (result dict, final states)."""
     import numpy as np
     from gelios_2d_amd.batch import Batch
-    from gelios_2d_amd.configs import config5_cases
+    from gelios_2d_amd.configs import ensemble_cases
     from gelios_2d_amd.setka import Setka
     from gelios_2d_amd.surrogate import SourceModel
     from gelios_2d_amd.train_surrogate import DEFAULT_OUT
     model = SourceModel.load(DEFAULT_OUT)
-    specs = config5_cases(64, seed=5, steps=steps)
+    specs = ensemble_cases(64, seed=5, steps=steps)
     cases = []
     for w, f in specs:
         s = Setka(w.grid, w.params)
@@ -127,7 +127,7 @@ def run_cfg5(steps, refresh_every, log_prefix="cfg5"):
              "kernel_ms_per_step": dev_ms / steps}, finals, specs)
 
 
-def run_cfg5_reference(specs, steps):
+def run_ensemble_reference(specs, steps):
     """The same cases in FP64 with the analytic sources every step (batched)."""
     from gelios_2d_amd.batch import Batch
     from gelios_2d_amd.params import GEL_PREC_FP64, GEL_SRC_MK1
@@ -183,7 +183,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="cfg5")
+    ap.add_argument("--workload", default="ensemble")
     ap.add_argument("--refresh-every", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -199,15 +199,15 @@ def main():
     wl = args.workload
     W = args.warmup
     extra = {}
-    if wl == "cfg5":
+    if wl == "ensemble":
         import numpy as np
         K = args.steps if args.steps is not None else 500
-        w64 = configs.config2(GEL_PREC_FP64)
-        log(f"bench: cfg5 batched ensemble, {K} steps; FP64 plain reference workload {w64.name}")
-        r64 = run_gpu(w64, 1000, W, "cfg2-fp64-plain")
-        r5, finals, specs = run_cfg5(K, args.refresh_every)
+        w64 = configs.hlld_mhd_800x400(GEL_PREC_FP64)
+        log(f"bench: batched ensemble, {K} steps; FP64 plain reference workload {w64.name}")
+        r64 = run_gpu(w64, 1000, W, "hlld-800x400-fp64-plain")
+        r5, finals, specs = run_ensemble(K, args.refresh_every)
         # the timed run did 6 warm-up steps first: the reference runs K + 6 steps
-        ref = run_cfg5_reference(specs, K)
+        ref = run_ensemble_reference(specs, K)
         num = {0: 0.0, 1: 0.0}
         den = {0: 0.0, 1: 0.0}
         for a, b in zip(finals, ref):
@@ -215,8 +215,8 @@ def main():
                 num[k] += float(np.abs(a[k] - b[k]).sum())
                 den[k] += float(np.abs(b[k]).sum())
         l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
-        log(f"bench[cfg5]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
-            f"vs FP64 plain cfg2 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
+        log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
+            f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
         B = 2 * 9 * 4 + 32 + 32      # FP32 state, nodes, source / hydrogen moments per cell-update
         achieved = B * r5["cells"] / (r5["kernel_ms_per_step"] * 1e-3) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
@@ -234,10 +234,10 @@ def main():
                       "judge_fallback_cells_initial": r5["fallback_cells"]},
         }
         precision, dtype = "fp32-bulk + learned source (BF16 MFMA), batched, HIP graph", "f32"
-        config = {"workload": "cfg5-batched-64-cases-fp32-surrogate", "cells": r5["cells"], "cases": r5["cases"],
+        config = {"workload": "ensemble-64-cases-100x50-to-800x400-hlld-fp32-learned-source", "cells": r5["cells"], "cases": r5["cases"],
                   "physics": "mhd-hlld", "moving_grid": False, "precision": precision, "cfl": 0.4}
-    elif wl == "cfg2":
-        w64, w32 = configs.config2(GEL_PREC_FP64), configs.config2(GEL_PREC_FP32_BULK)
+    elif wl == "hlld-800x400":
+        w64, w32 = configs.hlld_mhd_800x400(GEL_PREC_FP64), configs.hlld_mhd_800x400(GEL_PREC_FP32_BULK)
         K = args.steps if args.steps is not None else w64.steps
         log(f"bench: {w64.name} / {w32.name}, {w64.grid.ncells} cells, warmup {W}, steps {K}")
         r64 = run_gpu(w64, K, W, "fp64-plain")
@@ -254,8 +254,8 @@ def main():
         precision, dtype = "fp32-bulk (FP32 state and face arithmetic; FP64 geometry, accumulation, update)", "f32"
         config = None
     else:
-        if wl == "cfg1":
-            w = configs.config1()
+        if wl == "hllc-200x100":
+            w = configs.hllc_gas_200x100()
         elif wl == "large-mhd":
             w = configs.mhd_heliosphere(4096, 2048, 20, seed=2, name="hlld-fp64-4096x2048-moving")
         elif wl == "large-gas":
@@ -289,7 +289,7 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        cw = main_w if wl not in ("cfg2", "cfg5") else w64
+        cw = main_w if wl not in ("hlld-800x400", "ensemble") else w64
         line["cpu_baseline"] = cpu_baseline(cw)
         log(f"bench: cpu baseline {line['cpu_baseline']['value']:.3e} cell-updates/s")
     print(json.dumps(line), flush=True)

@@ -60,9 +60,9 @@ def heliosphere(ni: int, nj: int, steps: int, seed: int, flux: int = GEL_FLUX_HL
     return Workload(name, g, prm, blend_state(g, seed), steps)
 
 
-def config1() -> Workload:
+def hllc_gas_200x100() -> Workload:
     """Single-fluid HLLC, 200 x 100 cells, CFL 0.4, 100 steps, seed 1."""
-    return heliosphere(200, 100, 100, seed=1, name="cfg1-hllc-200x100")
+    return heliosphere(200, 100, 100, seed=1, name="hllc-gas-200x100-static")
 
 
 def large(ni: int = 4096, nj: int = 2048, steps: int = 20) -> Workload:
@@ -109,13 +109,13 @@ def mhd_heliosphere(ni: int, nj: int, steps: int, seed: int, moving: bool = True
     return Workload(name, w.grid, prm, np.ascontiguousarray(q), steps, motion)
 
 
-def config2(precision: int = GEL_PREC_FP64) -> Workload:
+def hlld_mhd_800x400(precision: int = GEL_PREC_FP64) -> Workload:
     """HLLD MHD, 800 x 400 cells, moving grid, 1000 steps, CFL 0.4, seed 2."""
     tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
-    return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"cfg2-hlld-{tag}-800x400-moving")
+    return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"hlld-mhd-{tag}-800x400-moving")
 
 
-def config5_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None):
+def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None):
     """Batched ensemble: grid sizes log-uniform from 100 x 50 to 800 x 400, inflow Mach
     uniform in [1.5, 3], field angle uniform in [0, 90] deg, |B| uniform in [0.02, 0.1],
     hydrogen density uniform in [0.05, 0.2]; FP32-bulk HLLD with the learned source.
@@ -142,6 +142,6 @@ def config5_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision:
         w.prim0[6] = (1 - s) * w.params.inf_by
         w.params.precision = precision
         w.params.source_mode = GEL_SRC_TABLE
-        w.name = f"cfg5-case{k:02d}-{ni}x{nj}"
+        w.name = f"ensemble-case{k:02d}-{ni}x{nj}"
         out.append((w, f))
     return out

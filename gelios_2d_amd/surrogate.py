@@ -152,7 +152,7 @@ def inflow_sets(n_sets: int = 16, seed: int = 4):
 
 
 def workload_with_hydrogen(ni: int, nj: int, steps: int, mach: float, nh: float, seed: int, moving: bool = True):
-    """cfg2 geometry and field with interstellar inflow Mach `mach` and hydrogen density
+    """The 800 x 400 MHD workload's geometry and field with interstellar inflow Mach `mach` and hydrogen density
     `nh`; charge exchange with sigma(U) = (1 - 0.07 ln U)^2.  Returns (workload, fields)."""
     from .configs import R_BLEND, D_BLEND, mhd_heliosphere
     from .hydrogen import populations

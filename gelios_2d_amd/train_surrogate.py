@@ -19,7 +19,7 @@ from . import surrogate as sg
 from .params import GEL_SRC_TABLE
 from .setka import Setka
 
-DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "surrogate_cfg4.npz")
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "source_mlp_16_inflow_sets.npz")
 
 
 def per_cell_error(model: sg.SourceModel, setka: Setka, fields) -> np.ndarray:

@@ -3,7 +3,7 @@ without HIP-graph replay, for every launch shape; autotuner restores the states.
 import numpy as np
 import pytest
 
-from gelios_2d_amd.configs import config5_cases
+from gelios_2d_amd.configs import ensemble_cases
 from gelios_2d_amd.params import GEL_PREC_FP64
 
 pytestmark = pytest.mark.gpu
@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 def _cases(n, precision=None):
     from gelios_2d_amd.setka import Setka
     out = []
-    for w, f in config5_cases(n, seed=11, steps=0, precision=precision):
+    for w, f in ensemble_cases(n, seed=11, steps=0, precision=precision):
         s = Setka(w.grid, w.params)
         s.upload(w.prim0)
         s.set_sources(f)

@@ -91,11 +91,11 @@ def test_sweep_matches_oracle_heliosphere(oracle_lib, flux, shape):
     assert rel_err(s.download(), ref) <= TOL
 
 
-def test_config1_full_size_matches_oracle(oracle_lib):
-    """cfg1 at its own size: 200 x 100 cells, HLLC, CFL 0.4, 100 steps."""
-    from gelios_2d_amd.configs import config1
+def test_hllc_200x100_full_size_matches_oracle(oracle_lib):
+    """The 200 x 100 gas workload at its own size: 200 x 100 cells, HLLC, CFL 0.4, 100 steps."""
+    from gelios_2d_amd.configs import hllc_gas_200x100
     from gelios_2d_amd.setka import Setka
-    w = config1()
+    w = hllc_gas_200x100()
     sw = oracle_lib.OracleSweep.from_polar(w.grid, w.params)
     ref = sw.run(w.prim0, w.steps)
     s = Setka(w.grid, w.params)
