@@ -35,6 +35,7 @@ namespace gel {
 constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kOut = kWave - 2 * kHalo;   // output columns per wavefront
+constexpr int kNW = 4;                    // wavefronts per work-group, marching together
 
 // State vectors.  GAS: (ro, p, u, v, Q), the fields of Parametr (Cell.h:21-25).
 // MHD: (ro, p, u, v, w, bx, by, bz, Q).  odd(k): the primitive changes sign across the
@@ -360,15 +361,14 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
 
 // Boundary face of cell M: Gran::Get_par ghost (Gran.cpp:205-289) and the sweep's
 // treatment of the cell side (Setka.cpp:10252-10274).  D is the cell across M from
-// the face (Sosed_down), used on the axis.
-template <class Ph, class TS, class TF, bool TYPES, int FLUX>
-__device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const CellT<Ph, TS>& M,
-                                            const CellT<Ph, TS>& D, const Face& f, double* P)
+// the face (Sosed_down), used on the axis.  L is M's side, R the ghost.
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, const CellT<Ph, TS>& M,
+                                                const CellT<Ph, TS>& D, const Face& f, double* L, double* R)
 {
     constexpr int NV = Ph::NV;
     const gel_params& p = a.prm;
     const double gam = p.gamma;
-    double L[NV], R[NV];
     const double radius = sqrt(sq(M.cx) + sq(M.cy));
     if (ftype == GEL_F_INNER_SPHERE) {
         double g[NV];
@@ -426,18 +426,18 @@ __device__ __forceinline__ TF boundary_flux(const SweepArgs& a, int ftype, const
             if (R[2] >= 0.0) R[2] = p.extern_u;
         }
     }
-    return face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
 }
 
-template <class Ph, class TS, class TF, bool TYPES, int FLUX>
-__device__ __forceinline__ TF interior_flux(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
-                                            const CellT<Ph, TS>& U, bool m_axis, bool n_axis, const Face& f,
-                                            double gam, double* P, int& bad)
+// Interior face between M (owner, L side) and N (R side); D lies beyond M and U beyond
+// N on the same line (Setka.cpp:10213-10251 from each side).
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void interior_states(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N,
+                                                const CellT<Ph, TS>& D, const CellT<Ph, TS>& U, bool m_axis,
+                                                bool n_axis, const Face& f, double gam, double* L, double* R,
+                                                int& bad)
 {
-    double L[Ph::NV], R[Ph::NV];
     side_value<Ph, TS, TF, TYPES>(M, N, D, m_axis, f.fx, f.fy, f.dis, gam, L, bad);
     side_value<Ph, TS, TF, TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, R, bad);
-    return face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
 }
 
 // Conservative ALE update of one cell and primitive recovery (Setka.cpp:10423-10460),
@@ -502,219 +502,332 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-// One step of the sweep over a chunk of rows for a strip of columns.
-// Ph: state vector; TS: storage type of the state in HBM; TF: type of the face
-// arithmetic (reconstruction and Riemann solver); geometry, accumulation and the
-// update are FP64 throughout.
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
-__device__ __forceinline__ void sweep_tile(const SweepArgs& a, int bx, int by)
+// ---------------------------------------------------------------- marching tile ---
+//
+// A workgroup of NW wavefronts owns a strip of 64 columns (60 output columns and
+// 2 halo columns each side) and a chunk of rows [i0, i1).  It marches up the
+// chunk NW rows at a time: in a step, wavefront w owns row i = base + w, with
+// base = i0 - 1, i0 - 1 + NW, ...  (row i0 - 1 only contributes the radial face
+// below the chunk).  A step has three phases separated by barriers:
+//   load    each wavefront brings cell row i + 2 (state, centroid, type) and node
+//           row i + 3 from HBM into LDS rings of NW + 3 rows;
+//   faces   each wavefront solves the radial face above its row and the polar
+//           face (j, j+1) of its row, from the LDS rings; the radial face goes to
+//           a ring of NW + 1 LDS slots as flux x area products;
+//   update  each wavefront sums the four faces of its cells (the polar face left
+//           of a column comes from the lane below by a cross-lane shuffle, the
+//           radial faces from the slots of its row and of the row below) and writes
+//           the updated primitives.
+// The update phase reads nothing from the rings, so the next step's loads may
+// overwrite them as soon as a wavefront gets there.  Every face is solved once, at
+// one call site per direction; boundary faces only choose their two states.
+template <class Ph, class TS, bool MOVING, bool TYPES, int NW>
+struct MarchLds {
+    static constexpr int NR = NW + 3;                 // rows held by the rings
+    TS st[NR][Ph::NV][kWave];                         // primitives of cell rows
+    double cx[NR][kWave], cy[NR][kWave];              // cell centroids
+    int8_t ty[TYPES ? NR : 1][kWave];                 // cell types
+    double nx[NR][kWave + 1], ny[NR][kWave + 1];      // node rows before the step
+    double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
+    double ny2[MOVING ? NR : 1][kWave + 1];
+    double fps[NW + 1][Ph::NV][kWave];                // radial flux x area above a row
+    double ftm[NW + 1][kWave];                        // CFL time of the cell above that face
+};
+
+struct MarchAcc {       // per-lane results folded into the case's dt buffer and counters
+    double tmin;
+    int n_rho, n_p, bad;
+};
+
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
+__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, int bx,
+                                           int i0, int i1, MarchAcc& acc)
 {
     constexpr int NV = Ph::NV;
+    constexpr int NR = NW + 3;
     using C = CellT<Ph, TS>;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int ni = a.ni, nj = a.nj;
     const int j = bx * kOut + lane - kHalo;
-    const bool col = (j >= 0 && j < a.nj);
+    const bool col = (j >= 0 && j < nj);
     const bool out_lane = col && lane >= kHalo && lane < kWave - kHalo;
-    const int i0 = by * a.rows;
-    const int i1 = min(i0 + a.rows, a.ni);
+    const bool used_polar = col && lane >= 1 && lane < kWave - kHalo;
     const double gam = a.prm.gamma;
     const double cfl = a.prm.cfl;
     const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
-    const bool axis_lo = (j == 0), axis_hi = (j == a.nj - 1);
+    const bool axis_lo = (j == 0), axis_hi = (j == nj - 1);
     const bool m_axis = axis_lo || axis_hi;
-    const bool n_axis = (j + 1 == a.nj - 1);
-    const size_t nc = (size_t)a.ni * a.nj;
+    const bool n_axis = (j + 1 == nj - 1);
+    const size_t nc = (size_t)ni * nj;
 
-    if (bx == 0 && by == 0 && lane == 0)
+    if (bx == 0 && i0 == 0 && threadIdx.x == 0)
         a.dtbuf[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
 
-    // window: cells of rows i-1, i, i+1, i+2 and node rows i .. i+3
-    Nodes nr[4];
-    C w[4];
-    const Nodes nm2 = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 - 2, j, col);
-    const Nodes nm1 = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 - 1, j, col);
-#pragma unroll
-    for (int k = 0; k < 4; k++) nr[k] = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i0 + k, j, col);
-    const C cm2 = load_cell<Ph, TS, TYPES>(a, i0 - 2, j, col, nm2, nm1);
-    w[0] = load_cell<Ph, TS, TYPES>(a, i0 - 1, j, col, nm1, nr[0]);
-#pragma unroll
-    for (int k = 1; k < 4; k++) w[k] = load_cell<Ph, TS, TYPES>(a, i0 + k - 1, j, col, nr[k - 1], nr[k]);
+    auto slot = [&](int r) { return (r - i0 + 2) % NR; };     // rows and node rows >= i0 - 2
+    auto fslot = [&](int r) { return (r - i0 + 1) % (NW + 1); };  // radial face above row r >= i0 - 1
 
-    int bad = 0;
-    // flux through the radial face below row i0, oriented along +i
-    double flo[NV];
-    TF uulo;
-    {
-        const Nodes q = nr[0];
-        Nodes qn = q;
-        if (MOVING) qn = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i0, j, col);
-        if (i0 == 0) {
-            // inner sphere, edge 3 of cell (0, j): (0, j+1) -> (0, j)
-            const Face f = face_geom<MOVING>(q.xb, q.yb, q.xa, q.ya, qn.xb, qn.yb, qn.xa, qn.ya, dt);
-            double P[NV];
-            uulo = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_INNER_SPHERE, w[1], w[2], f, P);
+    // cell row r and node row r + 1 from HBM into the rings
+    auto load_row = [&](int r) {
+        const Nodes lo = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r, j, col);
+        const Nodes hi = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r + 1, j, col);
+        const C c = load_cell<Ph, TS, TYPES>(a, r, j, col, lo, hi);
+        const int s = slot(r);
 #pragma unroll
-            for (int k = 0; k < NV; k++) flo[k] = -P[k];
-        } else {
-            // edge 1 of cell (i0-1, j): (i0, j) -> (i0, j+1)
-            const Face f = face_geom<MOVING>(q.xa, q.ya, q.xb, q.yb, qn.xa, qn.ya, qn.xb, qn.yb, dt);
-            uulo = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[0], w[1], cm2, w[2], m_axis, m_axis, f, gam, flo, bad);
-        }
-    }
-
-    double tmin = __builtin_huge_val();
-    int n_rho = 0, n_p = 0;
-    for (int i = i0; i < i1; i++) {
-        const Nodes q0 = nr[0], q1 = nr[1];
-        Nodes q0n = q0, q1n = q1;
-        if (MOVING) {
-            q0n = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i, j, col);
-            q1n = load_nodes<MOVING>(a, a.x2, a.y2, a.s_new, i + 1, j, col);
-        }
-        // edge 1: radial face (i, i+1), (i+1, j) -> (i+1, j+1)
-        const Face f1 = face_geom<MOVING>(q1.xa, q1.ya, q1.xb, q1.yb, q1n.xa, q1n.ya, q1n.xb, q1n.yb, dt);
-        double P1[NV];
-        TF uu1;
-        if (i + 1 < a.ni) {
-            uu1 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], w[2], w[0], w[3], m_axis, m_axis, f1, gam, P1, bad);
-        } else {
-            const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
-            uu1 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, ot, w[1], w[0], f1, P1);
-        }
-        // edge 2: polar face (j, j+1), (i+1, j+1) -> (i, j+1)
-        const Face f2 = face_geom<MOVING>(q1.xb, q1.yb, q0.xb, q0.yb, q1n.xb, q1n.yb, q0n.xb, q0n.yb, dt);
-        const C cN = cell_shfl<true>(w[1], 1), cD = cell_shfl<false>(w[1], 1), cU = cell_shfl<true>(w[1], 2);
-        double P2[NV];
-        TF uu2;
-        if (axis_hi) uu2 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_AXIS, w[1], cD, f2, P2);
-        else uu2 = interior_flux<Ph, TS, TF, TYPES, FLUX>(w[1], cN, cD, cU, m_axis, n_axis, f2, gam, P2, bad);
-        // edge 0: polar face (j-1, j), (i, j) -> (i+1, j); solved by the lane below
-        const Face f0 = face_geom<MOVING>(q0.xa, q0.ya, q1.xa, q1.ya, q0n.xa, q0n.ya, q1n.xa, q1n.ya, dt);
-        double P0[NV];
-        TF uu0;
-        {
-            double Pl[NV];
-#pragma unroll
-            for (int k = 0; k < NV; k++) Pl[k] = shfl_up(P2[k], 1);
-            const TF uul = shfl_up(uu2, 1);
-            if (axis_lo) {
-                uu0 = boundary_flux<Ph, TS, TF, TYPES, FLUX>(a, GEL_F_AXIS, w[1], cN, f0, P0);
-            } else {
-#pragma unroll
-                for (int k = 0; k < NV; k++) P0[k] = -Pl[k];
-                uu0 = uul;
+        for (int v = 0; v < NV; v++) S.st[s][v][lane] = c.s[v];
+        S.cx[s][lane] = c.cx;
+        S.cy[s][lane] = c.cy;
+        if constexpr (TYPES) S.ty[s][lane] = (int8_t)c.type;
+        const int q = r + 1, sq1 = slot(q);
+        auto put = [&](int idx, int jj) {
+            const bool v = (q >= 0 && q <= ni && jj >= 0 && jj <= nj);
+            const size_t k = (size_t)q * (nj + 1) + jj;
+            double x = 1.0, y = 1.0;
+            if (v) {
+                x = a.x[k]; y = a.y[k];
+                if (MOVING) { x *= a.s_old; y *= a.s_old; }
             }
-        }
-        // edge 3: radial face (i-1, i), (i, j+1) -> (i, j)
-        const Face f3 = face_geom<MOVING>(q0.xb, q0.yb, q0.xa, q0.ya, q0n.xb, q0n.yb, q0n.xa, q0n.ya, dt);
-
-        // cell update (Setka.cpp:10372-10468)
-        const C& K = w[1];
-        const double x = K.cx, y = K.cy;
-        const double radius = sqrt(sq(x) + sq(y));
-        if (out_lane && !(radius < a.prm.r_skip)) {
-            double Pot[NV];
-#pragma unroll
-            for (int k = 0; k < NV; k++) {
-                double s = 0.0;
-                s = s + double(P0[k]) * f0.S;
-                s = s + double(P1[k]) * f1.S;
-                s = s + double(P2[k]) * f2.S;
-                s = s + double(-flo[k]) * f3.S;
-                Pot[k] = s;
+            S.nx[sq1][idx] = x;
+            S.ny[sq1][idx] = y;
+            if constexpr (MOVING) {
+                double x2 = 1.0, y2 = 1.0;
+                if (v) { x2 = a.x2[k] * a.s_new; y2 = a.y2[k] * a.s_new; }
+                S.nx2[sq1][idx] = x2;
+                S.ny2[sq1][idx] = y2;
             }
-            double time = 10000000;
-            time = mn(time, cfl * sqrt(sq(x - f0.fx) + sq(y - f0.fy)) / double(uu0));
-            time = mn(time, cfl * sqrt(sq(x - f1.fx) + sq(y - f1.fy)) / double(uu1));
-            time = mn(time, cfl * sqrt(sq(x - f2.fx) + sq(y - f2.fy)) / double(uu2));
-            time = mn(time, cfl * sqrt(sq(x - f3.fx) + sq(y - f3.fy)) / double(uulo));
-            tmin = fmin(tmin, time);
+        };
+        put(lane, j);
+        if (lane == kWave - 1) put(kWave, j + 1);
+    };
 
-            const size_t kc = (size_t)i * a.nj + j;
-            TS* __restrict__ nxt = (TS*)a.next;
-            double sd[NV];
+    auto cell = [&](int r, int ln) -> C {
+        ln = min(max(ln, 0), kWave - 1);
+        const int s = slot(r);
+        C c;
 #pragma unroll
-            for (int k = 0; k < NV; k++) sd[k] = double(K.s[k]);
-            if (TYPES && K.type == GEL_C_CENTR) {
+        for (int v = 0; v < NV; v++) c.s[v] = S.st[s][v][ln];
+        c.cx = S.cx[s][ln];
+        c.cy = S.cy[s][ln];
+        c.type = TYPES ? (int)S.ty[s][ln] : (int)GEL_C_2;
+        const int jj = bx * kOut + ln - kHalo;
+        c.ok = (jj >= 0 && jj < nj && r >= 0 && r < ni) ? 1 : 0;
+        return c;
+    };
+
+    struct NodeP { double x, y, x2, y2; };
+    auto node = [&](int q, int idx) -> NodeP {
+        const int s = slot(q);
+        NodeP n;
+        n.x = S.nx[s][idx];
+        n.y = S.ny[s][idx];
+        if constexpr (MOVING) { n.x2 = S.nx2[s][idx]; n.y2 = S.ny2[s][idx]; }
+        else { n.x2 = n.x; n.y2 = n.y; }
+        return n;
+    };
+
+    // initial rows i0 - 2 .. i0 (and node rows i0 - 1 .. i0 + 1)
+    for (int r = i0 - 2 + w; r <= i0; r += NW) load_row(r);
+
+    for (int base = i0 - 1; base < i1; base += NW) {
+        const int i = base + w;
+        const bool act = (i < i1);
+        const bool upd = act && (i >= i0);
+
+        // ---- load
+        if (act) load_row(i + 2);
+        __syncthreads();
+
+        // ---- faces
+        C M;
+        double psp[NV], ps0[NV];
+        double t_dn = 0.0, t_pol = 0.0, t0 = 0.0;
+        double vs1 = 1.0, vol = 1.0, vs2 = 1.0, vol2 = 1.0;
+        if (act) {
+            M = cell(i, lane);
+            const C D = cell(i - 1, lane), N = cell(i + 1, lane), U = cell(i + 2, lane);
+            // radial face above row i, nodes (i+1, j) -> (i+1, j+1); below row 0 it is the
+            // inner sphere, oriented from row 0 inward: (0, j+1) -> (0, j)
+            {
+                const bool inner = (i < 0);
+                NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
+                if (inner) { const NodeP t = na; na = nb; nb = t; }
+                const Face f = face_geom<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                double L[NV], R[NV], P[NV];
+                int b = 0;
+                if (inner) {
+                    boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
+                } else if (i + 1 >= ni) {
+                    const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
+                    boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
+                } else {
+                    interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
+                }
+                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+                const int fs = fslot(i);
 #pragma unroll
-                for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = K.s[k];
-            } else {
-                const double px[4] = { q0.xa, q1.xa, q1.xb, q0.xb };
-                const double py[4] = { q0.ya, q1.ya, q1.yb, q0.yb };
-                const double vs1 = quad_area(px, py);
-                const double Volume = quad_volume_r(px, py);
-                double vs2 = vs1, Volume2 = Volume;
+                for (int k = 0; k < NV; k++) S.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
+                S.ftm[fs][lane] = cfl * sqrt(sq(N.cx - f.fx) + sq(N.cy - f.fy)) / double(uu);
+                t_dn = cfl * sqrt(sq(M.cx - f.fx) + sq(M.cy - f.fy)) / double(uu);
+                if (out_lane) acc.bad += b;
+            }
+            if (upd) {
+                // polar face (j, j+1), nodes (i+1, j+1) -> (i, j+1); the lane left of
+                // column 0 solves the axis face of column 0, oriented (i, 0) -> (i+1, 0)
+                const bool axl = (j == -1);
+                NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
+                if (axl) { const NodeP t = na; na = nb; nb = t; }
+                const Face f = face_geom<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
+                double L[NV], R[NV], P[NV];
+                int b = 0;
+                if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
+                else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
+                else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
+                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+#pragma unroll
+                for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * f.S;
+                t_pol = cfl * sqrt(sq(M.cx - f.fx) + sq(M.cy - f.fy)) / double(uu);
+                const double t_right = cfl * sqrt(sq(Mr.cx - f.fx) + sq(Mr.cy - f.fy)) / double(uu);
+                if (used_polar) acc.bad += b;
+                // the face left of each column is the lane below's face, seen from the other side
+#pragma unroll
+                for (int k = 0; k < NV; k++) ps0[k] = -shfl_up(psp[k], 1);
+                t0 = shfl_up(t_right, 1);
+                // cell geometry before and after the step
+                const NodeP c00 = node(i, lane), c10 = node(i + 1, lane), c11 = node(i + 1, lane + 1),
+                            c01 = node(i, lane + 1);
+                const double px[4] = { c00.x, c10.x, c11.x, c01.x };
+                const double py[4] = { c00.y, c10.y, c11.y, c01.y };
+                vs1 = quad_area(px, py);
+                vol = quad_volume_r(px, py);
+                vs2 = vs1; vol2 = vol;
                 if (MOVING) {
-                    const double px2[4] = { q0n.xa, q1n.xa, q1n.xb, q0n.xb };
-                    const double py2[4] = { q0n.ya, q1n.ya, q1n.yb, q0n.yb };
+                    const double px2[4] = { c00.x2, c10.x2, c11.x2, c01.x2 };
+                    const double py2[4] = { c00.y2, c10.y2, c11.y2, c01.y2 };
                     vs2 = quad_area(px2, py2);
-                    Volume2 = quad_volume_r(px2, py2);
+                    vol2 = quad_volume_r(px2, py2);
                 }
-                double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
-                if (SRC == GEL_SRC_MK1) {
-                    const double* __restrict__ sp = a.src + kc;
-                    source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
-                               q1s, q2s, q3s);
-                } else if (SRC == GEL_SRC_TABLE) {
-                    qms = a.qtab[kc];
-                    q1s = a.qtab[nc + kc];
-                    q2s = a.qtab[2 * nc + kc];
-                    q3s = a.qtab[3 * nc + kc];
-                    qQs = a.qtab[4 * nc + kc];
-                }
-                double o[NV];
-                cell_update<Ph>(sd, Pot, Volume, Volume2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
-                                SRC == GEL_SRC_TABLE, o, n_rho, n_p);
-#pragma unroll
-                for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
             }
         }
+        __syncthreads();
 
-        // slide the window one row up
+        // ---- update (Setka.cpp:10372-10468)
+        if (upd) {
+            const int fa = fslot(i), fb = fslot(i - 1);
+            const double x = M.cx, y = M.cy;
+            const double radius = sqrt(sq(x) + sq(y));
+            if (out_lane && !(radius < a.prm.r_skip)) {
+                // faces in the reference's order: edge 0 (polar, left), 1 (radial, above),
+                // 2 (polar, right), 3 (radial, below)
+                double Pot[NV];
 #pragma unroll
-        for (int k = 0; k < NV; k++) flo[k] = P1[k];
-        uulo = uu1;
-        w[0] = w[1]; w[1] = w[2]; w[2] = w[3];
-        nr[0] = nr[1]; nr[1] = nr[2]; nr[2] = nr[3];
-        nr[3] = load_nodes<MOVING>(a, a.x, a.y, a.s_old, i + 4, j, col);
-        w[3] = load_cell<Ph, TS, TYPES>(a, i + 3, j, col, nr[2], nr[3]);
-    }
+                for (int k = 0; k < NV; k++) {
+                    double s = 0.0;
+                    s = s + ps0[k];
+                    s = s + S.fps[fa][k][lane];
+                    s = s + psp[k];
+                    s = s + (-S.fps[fb][k][lane]);
+                    Pot[k] = s;
+                }
+                double time = 10000000;
+                time = mn(time, t0);
+                time = mn(time, t_dn);
+                time = mn(time, t_pol);
+                time = mn(time, S.ftm[fb][lane]);
+                acc.tmin = fmin(acc.tmin, time);
 
-    // the numeric flag counts on lanes whose faces are used (edge 0 of lane 2 is lane 1's)
-    if (!col || lane < 1 || lane >= kWave - kHalo) bad = 0;
-    tmin = wave_min(tmin);
-    const int s_rho = wave_sum(n_rho), s_p = wave_sum(n_p), s_bad = wave_sum(bad);
-    if (lane == 0) {
+                const size_t kc = (size_t)i * nj + j;
+                TS* __restrict__ nxt = (TS*)a.next;
+                if (TYPES && M.type == GEL_C_CENTR) {
+#pragma unroll
+                    for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = M.s[k];
+                } else {
+                    double sd[NV];
+#pragma unroll
+                    for (int k = 0; k < NV; k++) sd[k] = double(M.s[k]);
+                    double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
+                    if (SRC == GEL_SRC_MK1) {
+                        const double* __restrict__ sp = a.src + kc;
+                        source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
+                                   q1s, q2s, q3s);
+                    } else if (SRC == GEL_SRC_TABLE) {
+                        qms = a.qtab[kc];
+                        q1s = a.qtab[nc + kc];
+                        q2s = a.qtab[2 * nc + kc];
+                        q3s = a.qtab[3 * nc + kc];
+                        qQs = a.qtab[4 * nc + kc];
+                    }
+                    double o[NV];
+                    cell_update<Ph>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
+                                    SRC == GEL_SRC_TABLE, o, acc.n_rho, acc.n_p);
+#pragma unroll
+                    for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
+                }
+            }
+        }
+    }
+}
+
+// Fold a wavefront's results into the case's dt buffer and counters.
+__device__ __forceinline__ void march_flush(const SweepArgs& a, MarchAcc& acc)
+{
+    const double tmin = wave_min(acc.tmin);
+    const int s_rho = wave_sum(acc.n_rho), s_p = wave_sum(acc.n_p), s_bad = wave_sum(acc.bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
         if (tmin < __builtin_huge_val())
             atomicMin(&a.dtbuf[a.dt_wr], (unsigned long long)__double_as_longlong(tmin));
         if (s_rho) atomicAdd(&a.counters[0], (unsigned long long)s_rho);
         if (s_p) atomicAdd(&a.counters[1], (unsigned long long)s_p);
         if (s_bad) atomicAdd(&a.counters[2], (unsigned long long)s_bad);
     }
+    acc = MarchAcc{ __builtin_huge_val(), 0, 0, 0 };
 }
 
-// One step of one case: a wavefront per (strip, chunk) tile.
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
-__global__ void __launch_bounds__(kWave)
-sweep_kernel(SweepArgs a)
+// Work-group index -> work item, so that the work-groups one XCD runs at a time
+// (dispatch deals work-groups to the 8 XCDs round-robin) take neighbouring tiles.
+__device__ __forceinline__ int xcd_item(int g, int ng)
 {
-    sweep_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX>(a, blockIdx.x, blockIdx.y);
+    constexpr int kXcd = 8;
+    if (ng % kXcd != 0) return g;
+    return (g % kXcd) * (ng / kXcd) + g / kXcd;
 }
 
-// One step of many independent cases in one persistent launch: the wavefronts walk
-// the (case, strip, chunk) work list round-robin; args[case] carries each case's
-// grid, buffers, time-step slots and parameters.
+// One step of one case: persistent work-groups over the (strip, chunk) tiles.
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
+__global__ void __launch_bounds__(NW * kWave)
+sweep_kernel(SweepArgs a, int strips, int ntiles)
+{
+    __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
+    MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
+    for (int t = xcd_item(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
+        const int bx = t % strips, by = t / strips;
+        const int i0 = by * a.rows, i1 = min(i0 + a.rows, a.ni);
+        // no barrier between tiles: the update phase of the previous tile reads no ring,
+        // and its faces phase ended at a barrier every wavefront has passed
+        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, bx, i0, i1, acc);
+    }
+    march_flush(a, acc);
+}
+
+// One step of many independent cases in one persistent launch: the work-groups walk
+// the (case, strip, chunk) list round-robin; args[case] carries each case's grid,
+// buffers, time-step slots and parameters.
 struct BatchItem {
     int cs, bx, by;
 };
 
-template <class Ph, class TS, class TF, int SRC, int FLUX>
-__global__ void __launch_bounds__(kWave)
+template <class Ph, class TS, class TF, int SRC, int FLUX, int NW>
+__global__ void __launch_bounds__(NW * kWave)
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const BatchItem w = items[it];
-        sweep_tile<Ph, TS, TF, false, false, SRC, FLUX>(args[w.cs], w.bx, w.by);
+    __shared__ MarchLds<Ph, TS, false, false, NW> S;
+    MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
+    for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
+        const BatchItem b = items[it];
+        const SweepArgs& a = args[b.cs];
+        const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
+        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, b.bx, i0, i1, acc);
+        march_flush(a, acc);
     }
 }
 
@@ -824,15 +937,24 @@ static void free_ctx(gel_ctx* c)
 
 void gel_ctx_destroy(gel_ctx* ctx) { free_ctx(ctx); }
 
-static int choose_rows(int ni, int nj)
+// Rows per chunk for one case run by `groups` resident work-groups: a work-group
+// takes ceil((rows + 1) / kNW) marching steps per tile (the extra row is the face
+// below the chunk), and the tiles run in ceil(tiles / groups) rounds; pick the rows
+// with the fewest steps on the critical path, the longest chunks among equals.
+static int choose_rows(int ni, int nj, int groups)
 {
-    // enough wavefronts to give every SIMD of the 256 CUs several of them
     const int strips = (nj + kOut - 1) / kOut;
-    const long target = 4096;
-    int rows = (int)((long)ni * strips / target);
-    if (rows < 2) rows = 2;
-    if (rows > 64) rows = 64;
-    return rows;
+    long best = -1;
+    int br = kNW - 1;
+    for (int k = 1; k <= 64; k++) {
+        const int rows = k * kNW - 1;
+        const long tiles = (long)strips * ((ni + rows - 1) / rows);
+        const long rounds = (tiles + groups - 1) / groups;
+        const long cost = rounds * k;
+        if (best < 0 || cost <= best) { best = cost; br = rows; }
+        if (rows >= ni) break;
+    }
+    return br;
 }
 
 int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
@@ -855,7 +977,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     c->nv = prm->physics == GEL_PHYS_MHD ? 9 : 5;
     c->ts = prm->precision == GEL_PREC_FP32_BULK ? 4 : 8;
     c->prm = *prm;
-    c->rows = choose_rows(g->ni, g->nj);
+    c->rows = 0;          // chosen at the first launch from the kernel's occupancy
     auto bad = [&](hipError_t e, const char* what) {
         free_ctx(c);
         return fail(e == hipErrorOutOfMemory ? GEL_E_NOMEM : GEL_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -1015,17 +1137,35 @@ int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
 
 }  // extern "C"
 
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
-static hipError_t launch_t(const SweepArgs& a, dim3 grid, hipStream_t s)
+// Resident work-groups of a kernel on the current device (occupancy x CUs).
+static int resident_groups(const void* kern, int block)
 {
-    hipLaunchKernelGGL((sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX>), grid, dim3(kWave), 0, s, a);
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, 0) != hipSuccess || per < 1 || cus < 1)
+        return 256;
+    return per * cus;
+}
+
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
+static hipError_t launch_t(gel_ctx* c, SweepArgs& a, hipStream_t s)
+{
+    const auto kern = sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, kNW>;
+    const int groups = resident_groups((const void*)kern, kNW * kWave);
+    if (c->rows <= 0) c->rows = choose_rows(c->ni, c->nj, groups);
+    a.rows = c->rows;
+    const int strips = (a.nj + kOut - 1) / kOut;
+    const int ntiles = strips * ((a.ni + a.rows - 1) / a.rows);
+    const int grid = ntiles < groups ? ntiles : groups;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, a, strips, ntiles);
     return hipGetLastError();
 }
 
 template <class Ph, class TS, class TF, bool TYPES, int FLUX>
-static hipError_t launch_ms(const SweepArgs& a, dim3 grid, hipStream_t s, bool moving, int src)
+static hipError_t launch_ms(gel_ctx* c, SweepArgs& a, hipStream_t s, bool moving, int src)
 {
-#define GEL_L(MV, SR) launch_t<Ph, TS, TF, MV, TYPES, SR, FLUX>(a, grid, s)
+#define GEL_L(MV, SR) launch_t<Ph, TS, TF, MV, TYPES, SR, FLUX>(c, a, s)
     if (moving) {
         return src == GEL_SRC_MK1 ? GEL_L(true, GEL_SRC_MK1) : src == GEL_SRC_TABLE ? GEL_L(true, GEL_SRC_TABLE)
                                                                                     : GEL_L(true, GEL_SRC_NONE);
@@ -1035,26 +1175,26 @@ static hipError_t launch_ms(const SweepArgs& a, dim3 grid, hipStream_t s, bool m
 #undef GEL_L
 }
 
-static hipError_t launch(const gel_ctx* c, const SweepArgs& a, dim3 grid, bool moving)
+static hipError_t launch(gel_ctx* c, SweepArgs& a, bool moving)
 {
     const int src = c->prm.source_mode;
     const bool f32 = (c->prm.precision == GEL_PREC_FP32_BULK);
     hipStream_t s = c->stream;
     if (c->prm.physics == GEL_PHYS_MHD) {
-        return f32 ? launch_ms<MhdPhys, float, float, false, 3>(a, grid, s, moving, src)
-                   : launch_ms<MhdPhys, double, double, false, 3>(a, grid, s, moving, src);
+        return f32 ? launch_ms<MhdPhys, float, float, false, 3>(c, a, s, moving, src)
+                   : launch_ms<MhdPhys, double, double, false, 3>(c, a, s, moving, src);
     }
     const bool hllc = (c->prm.flux == GEL_FLUX_HLLC);
     if (f32) {
-        return hllc ? launch_ms<GasPhys, float, float, false, GEL_FLUX_HLLC>(a, grid, s, moving, src)
-                    : launch_ms<GasPhys, float, float, false, GEL_FLUX_HLL>(a, grid, s, moving, src);
+        return hllc ? launch_ms<GasPhys, float, float, false, GEL_FLUX_HLLC>(c, a, s, moving, src)
+                    : launch_ms<GasPhys, float, float, false, GEL_FLUX_HLL>(c, a, s, moving, src);
     }
     if (c->has_types) {
-        return hllc ? launch_ms<GasPhys, double, double, true, GEL_FLUX_HLLC>(a, grid, s, moving, src)
-                    : launch_ms<GasPhys, double, double, true, GEL_FLUX_HLL>(a, grid, s, moving, src);
+        return hllc ? launch_ms<GasPhys, double, double, true, GEL_FLUX_HLLC>(c, a, s, moving, src)
+                    : launch_ms<GasPhys, double, double, true, GEL_FLUX_HLL>(c, a, s, moving, src);
     }
-    return hllc ? launch_ms<GasPhys, double, double, false, GEL_FLUX_HLLC>(a, grid, s, moving, src)
-                : launch_ms<GasPhys, double, double, false, GEL_FLUX_HLL>(a, grid, s, moving, src);
+    return hllc ? launch_ms<GasPhys, double, double, false, GEL_FLUX_HLLC>(c, a, s, moving, src)
+                : launch_ms<GasPhys, double, double, false, GEL_FLUX_HLL>(c, a, s, moving, src);
 }
 
 // moving: 0 static, 1 explicit next positions in d_x2/d_y2, 2 prescribed motion
@@ -1084,14 +1224,13 @@ static int one_step(gel_ctx* c, int moving)
     a.prm = c->prm;
     if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
     if (c->prm.source_mode == GEL_SRC_TABLE && !c->d_qtab) return fail(GEL_E_STATE, "source table not set");
-    dim3 grid((c->nj + kOut - 1) / kOut, (c->ni + c->rows - 1) / c->rows);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
         HIP_OK(hipEventRecord(e0, c->stream));
     }
-    hipError_t e = launch(c, a, grid, moving != 0);
+    hipError_t e = launch(c, a, moving != 0);
     if (e != hipSuccess) return fail(GEL_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
     if (c->timing) {
         HIP_OK(hipEventRecord(e1, c->stream));
@@ -1262,7 +1401,7 @@ int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W
 struct gel_batch {
     std::vector<gel_ctx*> cases;
     int rows = 0;                 // rows per chunk for every case; 0 = per-case default
-    int workgroups = 0;           // persistent grid; 0 = min(items, 8 per CU)
+    int workgroups = 0;           // persistent grid; 0 = the resident work-groups of the kernel
     SweepArgs* d_args = nullptr;  // [6][ncases]: one argument set per phase of the cycle
     BatchItem* d_items = nullptr;
     int nitems = 0;
@@ -1274,7 +1413,8 @@ struct gel_batch {
 
 static constexpr int kCycle = 6;  // two state buffers x three time-step slots
 
-static int batch_rows(const gel_batch* b, const gel_ctx* c) { return b->rows > 0 ? b->rows : choose_rows(c->ni, c->nj); }
+static constexpr int kBatchRows = 4 * kNW - 1;   // rows per chunk until the autotuner picks
+static int batch_rows(const gel_batch* b, const gel_ctx*) { return b->rows > 0 ? b->rows : kBatchRows; }
 
 static void batch_drop_graph(gel_batch* b)
 {
@@ -1330,40 +1470,36 @@ static int batch_upload_args(gel_batch* b)
 }
 
 template <class Ph, class TS, class TF, int SRC, int FLUX>
-static void batch_launch_t(const gel_batch* b, int phase, int grid)
+static void batch_launch_t(const gel_batch* b, int phase)
 {
-    hipLaunchKernelGGL((sweep_batch_kernel<Ph, TS, TF, SRC, FLUX>), dim3(grid), dim3(kWave), 0, b->stream,
+    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW>;
+    int grid = b->workgroups > 0 ? b->workgroups : resident_groups((const void*)kern, kNW * kWave);
+    if (grid > b->nitems) grid = b->nitems;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, b->stream,
                        b->d_args + (size_t)phase * b->cases.size(), b->d_items, b->nitems);
 }
 
 template <class Ph, class TS, class TF, int FLUX>
-static void batch_launch_s(const gel_batch* b, int phase, int grid, int src)
+static void batch_launch_s(const gel_batch* b, int phase, int src)
 {
-    if (src == GEL_SRC_MK1) batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(b, phase, grid);
-    else if (src == GEL_SRC_TABLE) batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(b, phase, grid);
-    else batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(b, phase, grid);
-}
-
-static int batch_grid(const gel_batch* b)
-{
-    int g = b->workgroups > 0 ? b->workgroups : 256 * 8;
-    return g < b->nitems ? g : b->nitems;
+    if (src == GEL_SRC_MK1) batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(b, phase);
+    else if (src == GEL_SRC_TABLE) batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(b, phase);
+    else batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(b, phase);
 }
 
 static hipError_t batch_launch(const gel_batch* b, int phase)
 {
     const gel_params& p = b->cases[0]->prm;
-    const int grid = batch_grid(b);
     const bool f32 = p.precision == GEL_PREC_FP32_BULK;
     if (p.physics == GEL_PHYS_MHD) {
-        if (f32) batch_launch_s<MhdPhys, float, float, 3>(b, phase, grid, p.source_mode);
-        else batch_launch_s<MhdPhys, double, double, 3>(b, phase, grid, p.source_mode);
+        if (f32) batch_launch_s<MhdPhys, float, float, 3>(b, phase, p.source_mode);
+        else batch_launch_s<MhdPhys, double, double, 3>(b, phase, p.source_mode);
     } else if (p.flux == GEL_FLUX_HLLC) {
-        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLLC>(b, phase, grid, p.source_mode);
-        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLLC>(b, phase, grid, p.source_mode);
+        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLLC>(b, phase, p.source_mode);
+        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLLC>(b, phase, p.source_mode);
     } else {
-        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLL>(b, phase, grid, p.source_mode);
-        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLL>(b, phase, grid, p.source_mode);
+        if (f32) batch_launch_s<GasPhys, float, float, GEL_FLUX_HLL>(b, phase, p.source_mode);
+        else batch_launch_s<GasPhys, double, double, GEL_FLUX_HLL>(b, phase, p.source_mode);
     }
     return hipGetLastError();
 }
@@ -1506,8 +1642,25 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         }
         return GEL_OK;
     };
-    const int rows_c[] = { 1, 2, 4, 8, 16, 32 };
-    const int wg_c[] = { 1024, 2048, 4096, 8192 };
+    // chunks of k * kNW - 1 rows fill the marching steps; grids of the resident
+    // work-groups and of twice as many (the hardware then balances the tail)
+    const int rows_c[] = { kNW - 1, 2 * kNW - 1, 3 * kNW - 1, 4 * kNW - 1, 6 * kNW - 1, 8 * kNW - 1, 12 * kNW - 1,
+                           16 * kNW - 1 };
+    b->workgroups = 0;
+    int resident = 0;
+    {
+        const gel_params& p = b->cases[0]->prm;
+        const bool f32 = p.precision == GEL_PREC_FP32_BULK;
+        const void* kern = nullptr;
+        if (p.physics == GEL_PHYS_MHD)
+            kern = f32 ? (const void*)sweep_batch_kernel<MhdPhys, float, float, GEL_SRC_TABLE, 3, kNW>
+                       : (const void*)sweep_batch_kernel<MhdPhys, double, double, GEL_SRC_TABLE, 3, kNW>;
+        else
+            kern = f32 ? (const void*)sweep_batch_kernel<GasPhys, float, float, GEL_SRC_TABLE, GEL_FLUX_HLLC, kNW>
+                       : (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_TABLE, GEL_FLUX_HLLC, kNW>;
+        resident = resident_groups(kern, kNW * kWave);
+    }
+    const int wg_c[] = { resident, 2 * resident };
     double best = 1e300;
     int br = 0, bw = 0;
     int st = GEL_OK;
