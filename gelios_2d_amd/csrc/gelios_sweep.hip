@@ -793,8 +793,10 @@ __device__ __forceinline__ int xcd_item(int g, int ng)
 }
 
 // One step of one case: persistent work-groups over the (strip, chunk) tiles.
+// Two work-groups of four wavefronts per CU (the LDS rings of the FP64 MHD kernel
+// take 78 KB of the 160 KB) need at most 256 registers per lane.
 template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
-__global__ void __launch_bounds__(NW * kWave)
+__global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_kernel(SweepArgs a, int strips, int ntiles)
 {
     __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
@@ -817,7 +819,7 @@ struct BatchItem {
 };
 
 template <class Ph, class TS, class TF, int SRC, int FLUX, int NW>
-__global__ void __launch_bounds__(NW * kWave)
+__global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
     __shared__ MarchLds<Ph, TS, false, false, NW> S;
