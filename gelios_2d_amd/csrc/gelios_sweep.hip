@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <type_traits>
 #include <string>
 #include <vector>
 
@@ -143,32 +144,53 @@ __device__ __forceinline__ double dist2d(double ax, double ay, double bx, double
 }
 
 // Limited linear value at the face (Help.cpp:96-113 with y = 0, x1 = -d1, x2 = -d2,
-// x3 = d3): t2 + minmod((t1-t2)/(x1-x2), (t2-t3)/(x2-x3)) * (0 - x2).  FP64 keeps the
-// reference's divisions; FP32 bulk multiplies by the reciprocals of the two
-// denominators, which all primitives of a face side share, and assembles the face
-// value in FP64 from the exact cell value t2, so that only the small slope correction
-// carries FP32 rounding.
+// x3 = d3): t2 + minmod((t1-t2)/(x1-x2), (t2-t3)/(x2-x3)) * (0 - x2), where d1, d2,
+// d3 are the distances of the three cell centres from the face centre.  FP64 keeps
+// the reference's divisions.  FP32 bulk computes the distances in FP32, multiplies by
+// the reciprocals of the two denominators, which all primitives of a face side
+// share, and assembles the face value in FP64 from the exact cell value t2, so that
+// only the small slope correction carries FP32 rounding.
 struct Lin3 {
     double den12, den23, d2;   // x1 - x2, x2 - x3, 0 - x2
 };
-__device__ __forceinline__ Lin3 lin3_geom(double d1, double d2, double d3)
+struct Lin3f {
+    float i12, i23, d2;        // 1 / (x1 - x2), 1 / (x2 - x3), 0 - x2
+};
+template <class TF> using Lin3T = typename std::conditional<sizeof(TF) == 8, Lin3, Lin3f>::type;
+
+__device__ __forceinline__ float dist2f(double ax, double ay, double bx, double by)
 {
-    return Lin3{ -d1 - (-d2), -d2 - d3, 0.0 - (-d2) };
+    const float dx = float(ax - bx), dy = float(ay - by);
+    return sr(dx * dx + dy * dy);
+}
+
+// stencil weights at face centre (fx, fy) for centres 1 (far, behind), 2 (owner), 3 (across)
+template <class TF>
+__device__ __forceinline__ Lin3T<TF> lin3_at(double fx, double fy, double x1, double y1, double x2, double y2,
+                                             double x3, double y3)
+{
+    if constexpr (sizeof(TF) == 8) {
+        const double d2 = dist2d(fx, fy, x2, y2);
+        const double d3 = dist2d(fx, fy, x3, y3);
+        const double d1 = dist2d(fx, fy, x1, y1);
+        return Lin3{ -d1 - (-d2), -d2 - d3, 0.0 - (-d2) };
+    } else {
+        const float d2 = dist2f(fx, fy, x2, y2);
+        const float d3 = dist2f(fx, fy, x3, y3);
+        const float d1 = dist2f(fx, fy, x1, y1);
+        return Lin3f{ __builtin_amdgcn_rcpf(d2 - d1), __builtin_amdgcn_rcpf(-d2 - d3), d2 };
+    }
 }
 template <class TF>
-__device__ __forceinline__ double lin3r(double t1, double t2, double t3, const Lin3& g, TF i12, TF i23)
+__device__ __forceinline__ double lin3v(double t1, double t2, double t3, const Lin3T<TF>& g)
 {
     if constexpr (sizeof(TF) == 8) {
         const double d = minmod<double>((t1 - t2) / g.den12, (t2 - t3) / g.den23);
         return d * g.d2 + t2;
     } else {
-        const float d = minmod<float>((float(t1) - float(t2)) * i12, (float(t2) - float(t3)) * i23);
-        return double(d * float(g.d2)) + t2;
+        const float d = minmod<float>((float(t1) - float(t2)) * g.i12, (float(t2) - float(t3)) * g.i23);
+        return double(d * g.d2) + t2;
     }
-}
-template <class TF> __device__ __forceinline__ TF recip(double x)
-{
-    if constexpr (sizeof(TF) == 8) return 0.0; else return float(1.0 / x);
 }
 
 // Gran::Get_par_TVD, Usualy face (Gran.cpp:294-457): master M's value at face centre
@@ -182,25 +204,17 @@ __device__ __forceinline__ void tvd_usualy(const CellT<Ph, TS>& M, const CellT<P
 #pragma unroll
         for (int k = 0; k < Ph::NV; k++) r[k] = double(M.s[k]);
         if (m_axis) {
-            const double d2 = dist2d(fx, fy, M.cx, M.cy);
-            const double d3 = dist2d(fx, fy, N.cx, N.cy);
-            const double d1 = dist2d(fx, fy, M.cx, -M.cy);
-            const Lin3 g = lin3_geom(d1, d2, d3);
-            const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
+            const Lin3T<TF> g = lin3_at<TF>(fx, fy, M.cx, -M.cy, M.cx, M.cy, N.cx, N.cy);
 #pragma unroll
             for (int k = 0; k < Ph::NV; k++)
-                if (Ph::odd(k)) r[k] = lin3r<TF>(-double(M.s[k]), double(M.s[k]), double(N.s[k]), g, i12, i23);
+                if (Ph::odd(k)) r[k] = lin3v<TF>(-double(M.s[k]), double(M.s[k]), double(N.s[k]), g);
         }
         return;
     }
-    const double d2 = dist2d(fx, fy, M.cx, M.cy);
-    const double d3 = dist2d(fx, fy, N.cx, N.cy);
-    const double d1 = dist2d(fx, fy, D.cx, D.cy);
-    const Lin3 g = lin3_geom(d1, d2, d3);
-    const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
+    const Lin3T<TF> g = lin3_at<TF>(fx, fy, D.cx, D.cy, M.cx, M.cy, N.cx, N.cy);
 #pragma unroll
     for (int k = 0; k < Ph::NV; k++)
-        r[k] = lin3r<TF>(double(D.s[k]), double(M.s[k]), double(N.s[k]), g, i12, i23);
+        r[k] = lin3v<TF>(double(D.s[k]), double(M.s[k]), double(N.s[k]), g);
     if (r[0] <= 0.0 || r[1] <= 0.0) bad = 1;
 }
 
@@ -268,7 +282,16 @@ struct Face {          // geometry of a face as seen from its owning cell
     double n1, n2, S, fx, fy, W, dis;
 };
 
-template <bool MOVING>
+// CFL time of a cell through a face (Setka.cpp:10388-10393): cfl * |centre - face
+// centre| / max|S|.  FP32 bulk forms it in FP32.
+template <class TF>
+__device__ __forceinline__ double cfl_time(double cfl, double cx, double cy, const Face& f, TF uu)
+{
+    if constexpr (sizeof(TF) == 8) return cfl * sqrt(sq(cx - f.fx) + sq(cy - f.fy)) / uu;
+    else return double(float(cfl) * dist2f(cx, cy, f.fx, f.fy) * __builtin_amdgcn_rcpf(uu));
+}
+
+template <bool MOVING, class TF>
 __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, double by,
                                           double ax2, double ay2, double bx2, double by2, double dt)
 {
@@ -278,7 +301,12 @@ __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, doubl
     f.fy = (ay + by) / 2.0;
     double t1 = bx - ax, t2 = by - ay;                                    // Gran.cpp:171-181
     const double nn = sqrt(sq(t1) + sq(t2));
-    t1 = t1 / nn; t2 = t2 / nn;
+    if constexpr (sizeof(TF) == 8) {
+        t1 = t1 / nn; t2 = t2 / nn;
+    } else {                            // FP32 bulk: one FP64 reciprocal
+        const double inn = 1.0 / nn;
+        t1 = t1 * inn; t2 = t2 * inn;
+    }
     f.n1 = t2; f.n2 = -t1;
     if (MOVING) {
         const double fx2 = (ax2 + bx2) / 2.0, fy2 = (ay2 + by2) / 2.0;
@@ -309,10 +337,10 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
         } else {
             const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
             const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
-            const float cL = sqrtf(g * float(L[1]) / float(L[0])), cR = sqrtf(g * float(R[1]) / float(R[0]));
+            const float cL = sr(dv(g * float(L[1]), float(L[0]))), cR = sr(dv(g * float(R[1]), float(R[0])));
             const float SL = mn(u1, u2) - mx(cL, cR), SR = mx(u1, u2) + mx(cL, cR);
             if (SL >= W || SR <= W) {
-                gas_upwind<double>(SL >= W ? gl : gr, f.W, f.n1, f.n2, gam, P);
+                gas_upwind<double>(SL >= W ? gl : gr, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), P);
                 return mx(fabsf(SL), fabsf(SR));
             }
             const Gas<float> fl{ float(L[0]), float(L[1]), float(L[2]), float(L[3]), float(L[4]) };
@@ -329,23 +357,20 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             return hlld<double>(L, R, f.W, f.n1, f.n2, gam, P);
         } else {
             const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
-            const float spi4 = 3.5449077018110318f;
-            const float bnL = (float(L[5]) * n1 + float(L[6]) * n2) / spi4;
-            const float bnR = (float(R[5]) * n1 + float(R[6]) * n2) / spi4;
-            const float bbL = (sq(float(L[5])) + sq(float(L[6])) + sq(float(L[7]))) / (spi4 * spi4);
-            const float bbR = (sq(float(R[5])) + sq(float(R[6])) + sq(float(R[7]))) / (spi4 * spi4);
+            const float ispi4 = 0.28209479177387814f, i4pi = 0.07957747154594767f;   // 1/sqrt(4 pi), 1/(4 pi)
+            const float bnL = (float(L[5]) * n1 + float(L[6]) * n2) * ispi4;
+            const float bnR = (float(R[5]) * n1 + float(R[6]) * n2) * ispi4;
+            const float bbL = (sq(float(L[5])) + sq(float(L[6])) + sq(float(L[7]))) * i4pi;
+            const float bbR = (sq(float(R[5])) + sq(float(R[6])) + sq(float(R[7]))) * i4pi;
             const float gpL = g * float(L[1]), gpR = g * float(R[1]);
-            const float cfL = sqrtf((gpL + bbL + sqrtf(fmaxf(sq(gpL + bbL) - 4.0f * gpL * sq(bnL), 0.0f))) / (2.0f * float(L[0])));
-            const float cfR = sqrtf((gpR + bbR + sqrtf(fmaxf(sq(gpR + bbR) - 4.0f * gpR * sq(bnR), 0.0f))) / (2.0f * float(R[0])));
+            const float cfL = sr(dv(gpL + bbL + sr(fmaxf(sq(gpL + bbL) - 4.0f * gpL * sq(bnL), 0.0f)), 2.0f * float(L[0])));
+            const float cfR = sr(dv(gpR + bbR + sr(fmaxf(sq(gpR + bbR) - 4.0f * gpR * sq(bnR), 0.0f)), 2.0f * float(R[0])));
             const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
             const float SL = mn(u1, u2) - mx(cfL, cfR), SR = mx(u1, u2) + mx(cfL, cfR);
             const float SN = mx(fabsf(SL), fabsf(SR));
-            if (SL >= W) {
-                mhd_upwind<double>(L, R, f.W, f.n1, f.n2, gam, double(SN), true, P);
-                return SN;
-            }
-            if (SR <= W) {
-                mhd_upwind<double>(R, L, f.W, f.n1, f.n2, gam, double(SN), false, P);
+            if (SL >= W || SR <= W) {
+                const bool left = SL >= W;
+                mhd_upwind<double>(left ? L : R, left ? R : L, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), double(SN), left, P);
                 return SN;
             }
             float fl[9], fr[9], Pf[9];
@@ -401,14 +426,12 @@ __device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, c
             }
         }
         if (!done) {                                          // Gran.cpp:486-519
-            const double d2 = dist2d(f.fx, f.fy, M.cx, M.cy);
-            const double d1 = dist2d(f.fx, f.fy, D.cx, D.cy);
-            const Lin3 g = lin3_geom(d1, d2, d2);
-            const TF i12 = recip<TF>(g.den12), i23 = recip<TF>(g.den23);
+            // the mirror image of M lies as far across the face as M itself
+            const Lin3T<TF> g = lin3_at<TF>(f.fx, f.fy, D.cx, D.cy, M.cx, M.cy, M.cx, M.cy);
 #pragma unroll
             for (int k = 0; k < NV; k++) {
                 L[k] = double(M.s[k]);
-                if (Ph::odd(k)) L[k] = lin3r<TF>(double(D.s[k]), double(M.s[k]), -double(M.s[k]), g, i12, i23);
+                if (Ph::odd(k)) L[k] = lin3v<TF>(double(D.s[k]), double(M.s[k]), -double(M.s[k]), g);
             }
         }
 #pragma unroll
@@ -444,13 +467,53 @@ __device__ __forceinline__ void interior_states(const CellT<Ph, TS>& M, const Ce
 // r-weighted: Volume = int y dA before, Volume2 after the step; Ar = planar area.
 // MHD adds the axisymmetric geometric sources (hoop stress with field and swirl,
 // angular momentum, azimuthal induction) per unit planar area.
-template <class Ph>
+// FAST (FP32 bulk): the same update with the divisions by Volume2 and by the new
+// density taken as multiplications by reciprocals (FP64 throughout).
+template <class Ph, bool FAST>
 __device__ __forceinline__ void cell_update(const double* s, const double* Pot, double Volume, double Volume2,
                                             double vs1, double vs2, double T1, double gam, double qm, double q1,
                                             double q2, double q3, double qQ, bool table, double* o, int& n_rho,
                                             int& n_p)
 {
-    if constexpr (Ph::NV == 5) {
+    if constexpr (FAST) {
+        constexpr int NV = Ph::NV;
+        const double iV2 = 1.0 / Volume2, rV = Volume * iV2, ig1 = 1.0 / (gam - 1.0);
+        const double ro = s[0], p = s[1], u = s[2], v = s[3];
+        const double Ar = 0.5 * (vs1 + vs2);
+        double w = 0.0, bx = 0.0, by = 0.0, bz = 0.0, BB = 0.0, Sv = p, Sw = 0.0, Sb = 0.0;
+        if constexpr (NV == 9) {
+            constexpr double i4pi = 0.07957747154594767, i8pi = 0.039788735772973836;
+            w = s[4]; bx = s[5]; by = s[6]; bz = s[7];
+            BB = bx * bx + by * by + bz * bz;
+            Sv = ro * w * w + p + BB * i8pi - bz * bz * i4pi;
+            Sw = -(ro * v * w - by * bz * i4pi);
+            Sb = v * bz - w * by;
+        }
+        const double Q = s[NV - 1];
+        const double E = p * ig1 + ro * (u * u + v * v + w * w) * 0.5 + BB * 0.039788735772973836;
+        double ro3 = ro * rV - T1 * (Pot[0] * iV2 - qm);
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double iro3 = 1.0 / ro3;
+        const double pv = (NV == 9) ? Sv : p;   // gas: the hoop term is p (Setka.cpp:10440)
+        const double u3 = (ro * u * rV - T1 * (Pot[1] * iV2 - q1)) * iro3;
+        const double v3 = (ro * v * rV - T1 * ((Pot[2] - pv * Ar) * iV2 - q2)) * iro3;
+        double w3 = 0.0, bx3 = 0.0, by3 = 0.0, bz3 = 0.0;
+        const int ie = (NV == 9) ? 4 : 3;
+        if constexpr (NV == 9) {
+            w3 = (ro * w * rV - T1 * ((Pot[3] - Sw * Ar) * iV2)) * iro3;
+            bx3 = bx * rV - T1 * (Pot[5] * iV2);
+            by3 = by * rV - T1 * (Pot[6] * iV2);
+            bz3 = bz * rV - T1 * ((Pot[7] - Sb * Ar) * iV2);
+        }
+        const double E3 = E * rV - T1 * (Pot[ie] * iV2 - q3);
+        double Q3 = Q * rV - T1 * (Pot[NV - 1] * iV2);
+        if (table) Q3 = Q3 + T1 * qQ;
+        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) * 0.039788735772973836) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3;
+        if constexpr (NV == 9) { o[4] = w3; o[5] = bx3; o[6] = by3; o[7] = bz3; }
+        o[NV - 1] = Q3;
+    } else if constexpr (Ph::NV == 5) {
         const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
         double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
         double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
@@ -651,7 +714,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const bool inner = (i < 0);
                 NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
                 if (inner) { const NodeP t = na; na = nb; nb = t; }
-                const Face f = face_geom<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
                 double L[NV], R[NV], P[NV];
                 int b = 0;
                 if (inner) {
@@ -666,8 +729,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const int fs = fslot(i);
 #pragma unroll
                 for (int k = 0; k < NV; k++) S.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
-                S.ftm[fs][lane] = cfl * sqrt(sq(N.cx - f.fx) + sq(N.cy - f.fy)) / double(uu);
-                t_dn = cfl * sqrt(sq(M.cx - f.fx) + sq(M.cy - f.fy)) / double(uu);
+                S.ftm[fs][lane] = cfl_time<TF>(cfl, N.cx, N.cy, f, uu);
+                t_dn = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
                 if (out_lane) acc.bad += b;
             }
             if (upd) {
@@ -676,7 +739,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const bool axl = (j == -1);
                 NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
                 if (axl) { const NodeP t = na; na = nb; nb = t; }
-                const Face f = face_geom<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
                 const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
                 double L[NV], R[NV], P[NV];
                 int b = 0;
@@ -686,8 +749,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
 #pragma unroll
                 for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * f.S;
-                t_pol = cfl * sqrt(sq(M.cx - f.fx) + sq(M.cy - f.fy)) / double(uu);
-                const double t_right = cfl * sqrt(sq(Mr.cx - f.fx) + sq(Mr.cy - f.fy)) / double(uu);
+                t_pol = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
+                const double t_right = cfl_time<TF>(cfl, Mr.cx, Mr.cy, f, uu);
                 if (used_polar) acc.bad += b;
                 // the face left of each column is the lane below's face, seen from the other side
 #pragma unroll
@@ -758,7 +821,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                         qQs = a.qtab[4 * nc + kc];
                     }
                     double o[NV];
-                    cell_update<Ph>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
+                    cell_update<Ph, sizeof(TF) == 4>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
                                     SRC == GEL_SRC_TABLE, o, acc.n_rho, acc.n_p);
 #pragma unroll
                     for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);

@@ -19,6 +19,14 @@ template <class T> GEL_HD T mn(T a, T b) { return (b < a) ? b : a; }   // std::m
 template <class T> GEL_HD T mx(T a, T b) { return (a < b) ? b : a; }   // std::max
 template <class T> GEL_HD T sq(T a) { return a * a; }
 
+// Division and square root inside the Riemann solvers: IEEE in FP64 (the
+// reference's arithmetic); in FP32 (the FP32-bulk solvers) the hardware reciprocal
+// and square root, 1 ulp each, instead of the correctly rounded sequences.
+GEL_HD double dv(double a, double b) { return a / b; }
+GEL_HD double sr(double x) { return sqrt(x); }
+__device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ float sr(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Primitive gas state in the reference's order (Parametr, Cell.h:21-25).
 template <class T> struct Gas { T ro, p, u, v, Q; };
 
@@ -111,15 +119,15 @@ GEL_HD T hllc2d(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T gam, T P[4]
     const T u1 = L.u * n1 + L.v * n2, v1 = L.u * t1 + L.v * t2;
     const T u2 = R.u * n1 + R.v * n2, v2 = R.u * t1 + R.v * t2;
     const T g1 = gam - T(1);
-    const T cL = sqrt(gam * L.p / L.ro);
-    const T cR = sqrt(gam * R.p / R.ro);
+    const T cL = sr(dv(gam * L.p, L.ro));
+    const T cR = sr(dv(gam * R.p, R.ro));
     const T kL = (sq(L.u) + sq(L.v)) / T(2);
     const T kR = (sq(R.u) + sq(R.v)) / T(2);
     const T SL = mn(u1, u2) - mx(cL, cR);
     const T SR = mx(u1, u2) + mx(cL, cR);
     const T UU = mx(fabs(SL), fabs(SR));
-    const T eL = L.p / g1 + L.ro * kL;
-    const T eR = R.p / g1 + R.ro * kR;
+    const T eL = dv(L.p, g1) + L.ro * kL;
+    const T eR = dv(R.p, g1) + R.ro * kR;
     const T FL[5] = { L.ro * u1, L.ro * u1 * u1 + L.p, L.ro * u1 * v1, (eL + L.p) * u1, L.Q * u1 };
     const T FR[5] = { R.ro * u2, R.ro * u2 * u2 + R.p, R.ro * u2 * v2, (eR + R.p) * u2, R.Q * u2 };
     const T UL[5] = { L.ro, L.ro * u1, L.ro * v1, eL, L.Q };
@@ -137,15 +145,15 @@ GEL_HD T hllc2d(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T gam, T P[4]
         return UU;
     }
     const T suR = SR - u2, suL = SL - u1;
-    const T SM = (suR * R.ro * u2 - suL * L.ro * u1 - R.p + L.p) / (suR * R.ro - suL * L.ro);
+    const T SM = dv(suR * R.ro * u2 - suL * L.ro * u1 - R.p + L.p, suR * R.ro - suL * L.ro);
     T UZ[5];
 #pragma unroll
-    for (int k = 0; k < 5; k++) UZ[k] = (SR * UR[k] - SL * UL[k] + FL[k] - FR[k]) / (SR - SL);
+    for (int k = 0; k < 5; k++) UZ[k] = dv(SR * UR[k] - SL * UL[k] + FL[k] - FR[k], SR - SL);
     T PO[5];
     if (!HLLC) {                        // HLL, Setka.cpp:10606-10618
 #pragma unroll
         for (int k = 0; k < 5; k++)
-            PO[k] = (SR * FL[k] - SL * FR[k] + SR * SL * (UR[k] - UL[k])) / (SR - SL);
+            PO[k] = dv(SR * FL[k] - SL * FR[k] + SR * SL * (UR[k] - UL[k]), SR - SL);
         P[1] = n1 * (PO[1] - W * UZ[1]) + t1 * (PO[2] - W * UZ[2]);
         P[2] = n2 * (PO[1] - W * UZ[1]) + t2 * (PO[2] - W * UZ[2]);
         P[0] = PO[0] - W * UZ[0];
@@ -154,16 +162,16 @@ GEL_HD T hllc2d(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T gam, T P[4]
         return UU;
     }
     // HLLC, Setka.cpp:10621-10697 (nul_potok = false on the sweep's faces)
-    const T roLL = L.ro * (SL - u1) / (SL - SM);
-    const T roRR = R.ro * (SR - u2) / (SR - SM);
-    const T QLL = L.Q * (SL - u1) / (SL - SM);
-    const T QRR = R.Q * (SR - u2) / (SR - SM);
-    const T suRm = suR / (SR - SM), suLm = suL / (SL - SM);
+    const T roLL = dv(L.ro * (SL - u1), SL - SM);
+    const T roRR = dv(R.ro * (SR - u2), SR - SM);
+    const T QLL = dv(L.Q * (SL - u1), SL - SM);
+    const T QRR = dv(R.Q * (SR - u2), SR - SM);
+    const T suRm = dv(suR, SR - SM), suLm = dv(suL, SL - SM);
     const T ptzR = R.p + R.ro * suR * (SM - u2);
     const T ptzL = L.p + L.ro * suL * (SM - u1);
     const T ptz = (ptzR + ptzL) / T(2);
-    const T ee2 = eR * suRm + (ptz * SM - R.p * u2) / (SR - SM);
-    const T ee1 = eL * suLm + (ptz * SM - L.p * u1) / (SL - SM);
+    const T ee2 = eR * suRm + dv(ptz * SM - R.p * u2, SR - SM);
+    const T ee1 = eL * suLm + dv(ptz * SM - L.p * u1, SL - SM);
     if (SM >= W) {
         const T ULL[5] = { roLL, roLL * SM, roLL * v1, ee1, QLL };
 #pragma unroll
@@ -290,8 +298,8 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
     const T g1 = gam - T(1);
     const T t1 = -n2, t2 = n1;
     const T ro_L = L[0], p_L = L[1], Q_L = L[8], ro_R = R[0], p_R = R[1], Q_R = R[8];
-    const T bx_L = L[5] / spi4, by_L = L[6] / spi4, bz_L = L[7] / spi4;
-    const T bx_R = R[5] / spi4, by_R = R[6] / spi4, bz_R = R[7] / spi4;
+    const T bx_L = dv(L[5], spi4), by_L = dv(L[6], spi4), bz_L = dv(L[7], spi4);
+    const T bx_R = dv(R[5], spi4), by_R = dv(R[6], spi4), bz_R = dv(R[7], spi4);
     // three-term dot products with the zero components kept, as the reference forms them
     const T z = T(0), o = T(1);
     const T u1 = L[2] * n1 + L[3] * n2 + L[4] * z;
@@ -310,18 +318,18 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
     const T bb_R = sq(bx_R) + sq(by_R) + sq(bz_R);
     const T uu_L = (sq(L[2]) + sq(L[3]) + sq(L[4])) / T(2);
     const T uu_R = (sq(R[2]) + sq(R[3]) + sq(R[4])) / T(2);
-    const T cfL = sqrt((gam * p_L + bb_L + sqrt(sq(gam * p_L + bb_L) - T(4) * gam * p_L * sq(bn1))) / (T(2) * ro_L));
-    const T cfR = sqrt((gam * p_R + bb_R + sqrt(sq(gam * p_R + bb_R) - T(4) * gam * p_R * sq(bn2))) / (T(2) * ro_R));
+    const T cfL = sr(dv(gam * p_L + bb_L + sr(sq(gam * p_L + bb_L) - T(4) * gam * p_L * sq(bn1)), T(2) * ro_L));
+    const T cfR = sr(dv(gam * p_R + bb_R + sr(sq(gam * p_R + bb_R) - T(4) * gam * p_R * sq(bn2)), T(2) * ro_R));
     const T SL = mn(u1, u2) - mx(cfL, cfR);
     const T SR = mx(u1, u2) + mx(cfL, cfR);
     const T pTL = p_L + bb_L / T(2);
     const T pTR = p_R + bb_R / T(2);
     const T suR = SR - u2, suL = SL - u1;
-    const T SM = (suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL) / (suR * ro_R - suL * ro_L);
-    const T PTT = (suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1)) / (suR * ro_R - suL * ro_L);
+    const T SM = dv(suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL, suR * ro_R - suL * ro_L);
+    const T PTT = dv(suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1), suR * ro_R - suL * ro_L);
     const T UU = mx(fabs(SL), fabs(SR));
-    const T e1 = p_L / g1 + ro_L * uu_L + bb_L / T(2);
-    const T e2 = p_R / g1 + ro_R * uu_R + bb_R / T(2);
+    const T e1 = dv(p_L, g1) + ro_L * uu_L + bb_L / T(2);
+    const T e2 = dv(p_R, g1) + ro_R * uu_R + bb_R / T(2);
     const T FL[9] = { ro_L * u1, ro_L * u1 * u1 + pTL - sq(bn1), ro_L * u1 * v1 - bn1 * bt1, ro_L * u1 * w1 - bn1 * bm1,
                       (e1 + pTL) * u1 - bn1 * (u1 * bn1 + v1 * bt1 + w1 * bm1), T(0), u1 * bt1 - v1 * bn1,
                       u1 * bm1 - w1 * bn1, Q_L * u1 };
@@ -330,43 +338,43 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
                       u2 * bm2 - w2 * bn2, Q_R * u2 };
     const T UL[9] = { ro_L, ro_L * u1, ro_L * v1, ro_L * w1, e1, bn1, bt1, bm1, Q_L };
     const T UR[9] = { ro_R, ro_R * u2, ro_R * v2, ro_R * w2, e2, bn2, bt2, bm2, Q_R };
-    const T bn = (SR * UR[5] - SL * UL[5] + FL[5] - FR[5]) / (SR - SL);
+    const T bn = dv(SR * UR[5] - SL * UL[5] + FL[5] - FR[5], SR - SL);
     const T bbn = bn * bn;
-    const T ro_LL = ro_L * (SL - u1) / (SL - SM);
-    const T ro_RR = ro_R * (SR - u2) / (SR - SM);
-    const T Q_LL = Q_L * (SL - u1) / (SL - SM);
-    const T Q_RR = Q_R * (SR - u2) / (SR - SM);
+    const T ro_LL = dv(ro_L * (SL - u1), SL - SM);
+    const T ro_RR = dv(ro_R * (SR - u2), SR - SM);
+    const T Q_LL = dv(Q_L * (SL - u1), SL - SM);
+    const T Q_RR = dv(Q_R * (SR - u2), SR - SM);
     const T ttL = ro_L * suL * (SL - SM) - bbn;
     const T ttR = ro_R * suR * (SR - SM) - bbn;
     T vLL = v1, wLL = w1, btt1 = T(0), bmm1 = T(0);
     T vRR = v2, wRR = w2, btt2 = T(0), bmm2 = T(0);
     if (fabs(ttL) >= T(0.000001)) {
-        vLL = v1 - bn * bt1 * (SM - u1) / ttL;
-        wLL = w1 - bn * bm1 * (SM - u1) / ttL;
-        btt1 = bt1 * (ro_L * suL * suL - bbn) / ttL;
-        bmm1 = bm1 * (ro_L * suL * suL - bbn) / ttL;
+        vLL = v1 - dv(bn * bt1 * (SM - u1), ttL);
+        wLL = w1 - dv(bn * bm1 * (SM - u1), ttL);
+        btt1 = dv(bt1 * (ro_L * suL * suL - bbn), ttL);
+        bmm1 = dv(bm1 * (ro_L * suL * suL - bbn), ttL);
     }
     if (fabs(ttR) >= T(0.000001)) {
-        vRR = v2 - bn * bt2 * (SM - u2) / ttR;
-        wRR = w2 - bn * bm2 * (SM - u2) / ttR;
-        btt2 = bt2 * (ro_R * suR * suR - bbn) / ttR;
-        bmm2 = bm2 * (ro_R * suR * suR - bbn) / ttR;
+        vRR = v2 - dv(bn * bt2 * (SM - u2), ttR);
+        wRR = w2 - dv(bn * bm2 * (SM - u2), ttR);
+        btt2 = dv(bt2 * (ro_R * suR * suR - bbn), ttR);
+        bmm2 = dv(bm2 * (ro_R * suR * suR - bbn), ttR);
     }
-    const T eLL = (e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1))) / (SL - SM);
-    const T eRR = (e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2))) / (SR - SM);
-    const T sLL = sqrt(ro_LL), sRR = sqrt(ro_RR);
-    T SLL = SM - fabs(bn) / sLL;
-    T SRR = SM + fabs(bn) / sRR;
+    const T eLL = dv(e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1)), SL - SM);
+    const T eRR = dv(e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2)), SR - SM);
+    const T sLL = sr(ro_LL), sRR = sr(ro_RR);
+    T SLL = SM - dv(fabs(bn), sLL);
+    T SRR = SM + dv(fabs(bn), sRR);
     T idbn;
     if (fabs(bn) > T(0.001)) {
         idbn = T(1) * sgn(bn);
     } else {
         idbn = T(0); SLL = SM; SRR = SM;
     }
-    const T vLLL = (sLL * vLL + sRR * vRR + idbn * (btt2 - btt1)) / (sLL + sRR);
-    const T wLLL = (sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1)) / (sLL + sRR);
-    const T bttt = (sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL)) / (sLL + sRR);
-    const T bmmm = (sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL)) / (sLL + sRR);
+    const T vLLL = dv(sLL * vLL + sRR * vRR + idbn * (btt2 - btt1), sLL + sRR);
+    const T wLLL = dv(sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1), sLL + sRR);
+    const T bttt = dv(sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL), sLL + sRR);
+    const T bmmm = dv(sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL), sLL + sRR);
     const T eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
     const T eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
     T PO[9];
@@ -423,13 +431,15 @@ namespace gel {
 // Upwind (supersonic) gas flux of the state S through a face moving with W: the
 // branch SL >= W / SR <= W of HLLC_2d_Korolkov_b_s (Setka.cpp:10569-10589).
 // P: mass, x-momentum, y-momentum, energy, tracer.
+// ig1 = 1 / (gamma - 1).  Used by the FP32-bulk path only, so its divisions are
+// multiplications by reciprocals.
 template <class T>
-GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T gam, T P[5])
+GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T ig1, T P[5])
 {
     const T t1 = n2, t2 = -n1;
     const T u = S.u * n1 + S.v * n2, v = S.u * t1 + S.v * t2;
-    const T k = (sq(S.u) + sq(S.v)) / T(2);
-    const T e = S.p / (gam - T(1)) + S.ro * k;
+    const T k = (sq(S.u) + sq(S.v)) * T(0.5);
+    const T e = S.p * ig1 + S.ro * k;
     const T F[5] = { S.ro * u, S.ro * u * u + S.p, S.ro * u * v, (e + S.p) * u, S.Q * u };
     const T U[5] = { S.ro, S.ro * u, S.ro * v, e, S.Q };
     P[1] = n1 * (F[1] - W * U[1]) + t1 * (F[2] - W * U[2]);
@@ -442,25 +452,26 @@ GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T gam, T P[5])
 // Upwind MHD flux of state q (the side the whole fan leaves behind), regions
 // SL >= W / SR <= W of the HLLD above; o is the other side, needed for the
 // normal-field term -SN (bn_R - bn_L) (Solvers.cpp:2161-2163).  q_is_left tells
-// which side q is.
+// which side q is; ig1 = 1 / (gamma - 1).  FP32-bulk path only: divisions are
+// multiplications by reciprocals.
 template <class T>
-GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T gam, T SN, bool q_is_left, T P[9])
+GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T ig1, T SN, bool q_is_left, T P[9])
 {
-    const T spi4 = T(3.5449077018110318);
+    const T spi4 = T(3.5449077018110318), ispi4 = T(0.28209479177387814);   // sqrt(4 pi), 1 / sqrt(4 pi)
     const T t1 = -n2, t2 = n1;
     const T z = T(0), one = T(1);
-    const T bx = q[5] / spi4, by = q[6] / spi4, bz = q[7] / spi4;
+    const T bx = q[5] * ispi4, by = q[6] * ispi4, bz = q[7] * ispi4;
     const T u = q[2] * n1 + q[3] * n2 + q[4] * z;
     const T v = q[2] * t1 + q[3] * t2 + q[4] * z;
     const T w = q[2] * z + q[3] * z + q[4] * one;
     const T bn = bx * n1 + by * n2 + bz * z;
     const T bt = bx * t1 + by * t2 + bz * z;
     const T bm = bx * z + by * z + bz * one;
-    const T bno = (o[5] / spi4) * n1 + (o[6] / spi4) * n2 + (o[7] / spi4) * z;
+    const T bno = (o[5] * ispi4) * n1 + (o[6] * ispi4) * n2;
     const T bb = sq(bx) + sq(by) + sq(bz);
-    const T uu = (sq(q[2]) + sq(q[3]) + sq(q[4])) / T(2);
-    const T pT = q[1] + bb / T(2);
-    const T e = q[1] / (gam - T(1)) + q[0] * uu + bb / T(2);
+    const T uu = (sq(q[2]) + sq(q[3]) + sq(q[4])) * T(0.5);
+    const T pT = q[1] + bb * T(0.5);
+    const T e = q[1] * ig1 + q[0] * uu + bb * T(0.5);
     const T F[9] = { q[0] * u, q[0] * u * u + pT - sq(bn), q[0] * u * v - bn * bt, q[0] * u * w - bn * bm,
                      (e + pT) * u - bn * (u * bn + v * bt + w * bm), T(0), u * bt - v * bn, u * bm - w * bn,
                      q[8] * u };
