@@ -41,14 +41,16 @@ def log(*a):
 
 
 def bytes_per_cell_update(w):
-    """Algorithmic HBM bytes per cell-update: state read once and written once, node
-    coordinates read (16 B), plus 16 B of written node coordinates on a moving grid
-    (per cell-update: 2 x variables x bytes per value + node bytes)."""
-    from gelios_2d_amd.params import GEL_PREC_FP32_BULK
+    """Algorithmic HBM bytes per cell-update: the state read once and written once
+    (2 x variables x bytes per value), one node's coordinates (16 B; 32 B on a moving
+    grid, positions before and after the step), and the per-cell source inputs: the
+    table (5 x 8 B) or the hydrogen moments (19 x 8 B)."""
+    from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_MK1, GEL_SRC_TABLE
     nv = 9 if w.params.physics == 1 else 5
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
     geom = 32 if w.motion else 16
-    return 2 * nv * s + geom
+    src = {GEL_SRC_TABLE: 40, GEL_SRC_MK1: 152}.get(w.params.source_mode, 0)
+    return 2 * nv * s + geom + src
 
 
 def run_gpu(w, K, W, label):
@@ -86,15 +88,18 @@ def run_gpu(w, K, W, label):
 
 
 def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
-    """Build the 64 cases, refresh their sources from the network, autotune, then
-    time `steps` steps (graph replays, with a source refresh every refresh_every
-    steps inside the timed region).  Returns (result dict, final states)."""
-    import numpy as np
+    """Build the 64 cases, fill their source tables from the network (one fused launch
+    for all cases), autotune, then time `steps` steps: HIP-graph replays in which the
+    sources of every case are refreshed from the network, with the analytic judge,
+    after every `refresh_every` steps.  Returns (result dict, final states, specs,
+    steps run before the timed region)."""
     from gelios_2d_amd.batch import Batch
     from gelios_2d_amd.configs import ensemble_cases
     from gelios_2d_amd.setka import Setka
     from gelios_2d_amd.surrogate import SourceModel
     from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    import math
+    import numpy as np
     model = SourceModel.load(DEFAULT_OUT)
     specs = ensemble_cases(64, seed=5, steps=steps)
     cases = []
@@ -102,29 +107,38 @@ def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
         s = Setka(w.grid, w.params)
         s.upload(w.prim0)
         s.set_sources(f)
+        s.set_source_table(np.zeros((5, s.ncells)))
         cases.append(s)
-    nfall = sum(model.refresh(s, tol=1e-3, judge=True) for s in cases)
     ncells = sum(s.ncells for s in cases)
     b = Batch(cases)
+    nfall = b.refresh(model, tol=1e-3, judge=True)
     rows, wg, ms_trial = b.autotune(6)
+    b.set_source_model(model, refresh_every, tol=1e-3, judge=True)
+    cyc = 6 * refresh_every // math.gcd(6, refresh_every)
     log(f"bench[{log_prefix}]: {len(cases)} cases, {ncells} cells; autotuner rows {rows} workgroups {wg} "
-        f"({ms_trial * 1e3:.1f} us/step); judge fallback {nfall}/{ncells} cells")
-    b.step(6, True)                               # warm-up (graph capture)
+        f"({ms_trial * 1e3:.1f} us/step sweep only); initial judge fallback {nfall}/{ncells} cells; "
+        f"source refresh every {refresh_every} steps, graph cycle {cyc} steps")
+    b.step(cyc, True)                             # warm-up (graph capture)
+    b.source_stats()
     t0 = time.perf_counter()
-    dev_ms = 0.0
-    done = 0
-    while done < steps:
-        k = min(refresh_every, steps - done)
-        dev_ms += b.step(k, True)
-        done += k
-        if done < steps:
-            for s in cases:
-                model.refresh(s, tol=1e-3, judge=True)
+    dev_ms = b.step(steps, True)
     wall = time.perf_counter() - t0
+    nref, nfb = b.source_stats()
     finals = [s.download() for s in cases]
+    # device time of the sweep kernel alone (HIP events on the batch stream)
+    b.set_source_model(None, 0)
+    sweep_ms = b.step(10 * 6, True) / 60
+    # device time of the refresh alone: one fused launch over all cases
+    t1 = time.perf_counter()
+    reps = 10
+    for _ in range(reps):
+        b.refresh(model, tol=1e-3, judge=True)
+    refresh_ms = (time.perf_counter() - t1) * 1e3 / reps
     return ({"value": ncells * steps / wall, "wall_s": wall, "cells": ncells, "cases": len(cases),
-             "autotune": {"rows": rows, "workgroups": wg}, "fallback_cells": nfall,
-             "kernel_ms_per_step": dev_ms / steps}, finals, specs)
+             "autotune": {"rows": rows, "workgroups": wg, "sweep_us_per_step": ms_trial * 1e3},
+             "fallback_cells_initial": nfall, "refreshes": nref,
+             "fallback_fraction": nfb / max(1, nref * ncells), "refresh_ms_host_timed": refresh_ms,
+             "kernel_ms_per_step": dev_ms / steps, "sweep_ms_per_step": sweep_ms}, finals, specs, cyc)
 
 
 def run_ensemble_reference(specs, steps):
@@ -142,7 +156,7 @@ def run_ensemble_reference(specs, steps):
         s.set_sources(f)
         cases.append(s)
     b = Batch(cases)
-    b.step(steps + 6, True)
+    b.step(steps, True)
     return [s.download() for s in cases]
 
 
@@ -184,7 +198,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ensemble")
-    ap.add_argument("--refresh-every", type=int, default=100)
+    ap.add_argument("--refresh-every", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -205,9 +219,9 @@ def main():
         w64 = configs.hlld_mhd_800x400(GEL_PREC_FP64)
         log(f"bench: batched ensemble, {K} steps; FP64 plain reference workload {w64.name}")
         r64 = run_gpu(w64, 1000, W, "hlld-800x400-fp64-plain")
-        r5, finals, specs = run_ensemble(K, args.refresh_every)
-        # the timed run did 6 warm-up steps first: the reference runs K + 6 steps
-        ref = run_ensemble_reference(specs, K)
+        r5, finals, specs, warm = run_ensemble(K, args.refresh_every)
+        # the timed run did `warm` warm-up steps first: the reference runs K + warm steps
+        ref = run_ensemble_reference(specs, K + warm)
         num = {0: 0.0, 1: 0.0}
         den = {0: 0.0, 1: 0.0}
         for a, b in zip(finals, ref):
@@ -217,21 +231,24 @@ def main():
         l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
         log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
             f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
-        B = 2 * 9 * 4 + 32 + 32      # FP32 state, nodes, source / hydrogen moments per cell-update
-        achieved = B * r5["cells"] / (r5["kernel_ms_per_step"] * 1e-3) / 1e9
+        B = bytes_per_cell_update(specs[0][0])
+        achieved = B * r5["cells"] / (r5["sweep_ms_per_step"] * 1e-3) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
                   "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
-                               "kernel_avg_us": r5["kernel_ms_per_step"] * 1e3}}
+                               "kernel_avg_us": r5["sweep_ms_per_step"] * 1e3}}
         main_w = None
         extra = {
             "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / 1000,
                            "roofline": r64["roofline"]},
             "speedup_vs_fp64_plain": r5["value"] / r64["value"],
-            "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + 6),
+            "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + warm),
             "batch": {"cases": r5["cases"], "cells": r5["cells"], "autotune": r5["autotune"],
                       "source_refresh_every": args.refresh_every, "judge_tol": 1e-3,
-                      "judge_fallback_cells_initial": r5["fallback_cells"]},
+                      "judge_fallback_cells_initial": r5["fallback_cells_initial"],
+                      "refreshes_timed": r5["refreshes"], "judge_fallback_fraction": r5["fallback_fraction"],
+                      "refresh_ms": r5["refresh_ms_host_timed"],
+                      "kernel_ms_per_step_incl_refresh": r5["kernel_ms_per_step"]},
         }
         precision, dtype = "fp32-bulk + learned source (BF16 MFMA), batched, HIP graph", "f32"
         config = {"workload": "ensemble-64-cases-100x50-to-800x400-hlld-fp32-learned-source", "cells": r5["cells"], "cases": r5["cases"],

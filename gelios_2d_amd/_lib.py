@@ -53,6 +53,7 @@ SIGNATURES = {
                                              ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                              _dp, _dp]),
     "gel_set_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_get_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
     "gel_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]),
@@ -77,6 +78,12 @@ SIGNATURES = {
     "gel_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
     "gel_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), _dp]),
+    "gel_batch_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_longlong)]),
+    "gel_batch_set_source_model": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_int]),
+    "gel_batch_source_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
+                                              ctypes.POINTER(ctypes.c_longlong)]),
     "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gel_last_error": (ctypes.c_char_p, []),
 }

@@ -2,8 +2,10 @@
 
 Each case is a :class:`~gelios_2d_amd.setka.Setka` (its own grid, state, inflow,
 sources and CFL time step); the batch owns the launch (csrc/gelios_sweep.hip,
-sweep_batch_kernel), replays whole 6-step cycles from a captured HIP graph and
-chooses its launch shape with the project autotuner (gel_batch_autotune).
+sweep_batch_kernel), replays whole cycles of steps from a captured HIP graph,
+chooses its launch shape with the project autotuner (gel_batch_autotune) and, with
+a source model attached, refreshes every case's learned charge-exchange source in
+one fused launch every few steps (gel_batch_set_source_model).
 """
 from __future__ import annotations
 
@@ -48,3 +50,23 @@ class Batch:
         _lib.check(_lib.lib().gel_batch_autotune(self._h, int(trial_steps), ctypes.byref(r), ctypes.byref(w),
                                                  ctypes.byref(ms)))
         return r.value, w.value, ms.value
+
+    def refresh(self, model, tol: float = 1e-3, judge: bool = True) -> int:
+        """Refresh every case's source table from its current state now; returns the
+        number of cells that fell back to the analytic source."""
+        n = ctypes.c_longlong()
+        _lib.check(_lib.lib().gel_batch_refresh(self._h, model.mlp._h, float(tol), 1 if judge else 0,
+                                                ctypes.byref(n)))
+        return n.value
+
+    def set_source_model(self, model, every: int, tol: float = 1e-3, judge: bool = True) -> None:
+        """Refresh the source tables inside step() after every `every` steps (0: off)."""
+        self._model = model          # the batch holds the network's handle
+        _lib.check(_lib.lib().gel_batch_set_source_model(self._h, model.mlp._h if model else None, int(every),
+                                                         float(tol), 1 if judge else 0))
+
+    def source_stats(self):
+        """(refreshes, fallback cells) since the last call."""
+        r, n = ctypes.c_longlong(), ctypes.c_longlong()
+        _lib.check(_lib.lib().gel_batch_source_stats(self._h, ctypes.byref(r), ctypes.byref(n)))
+        return r.value, n.value

@@ -115,15 +115,19 @@ def hlld_mhd_800x400(precision: int = GEL_PREC_FP64) -> Workload:
     return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"hlld-mhd-{tag}-800x400-moving")
 
 
-def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None):
+def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None,
+                   source_mode: int = None):
     """Batched ensemble: grid sizes log-uniform from 100 x 50 to 800 x 400, inflow Mach
     uniform in [1.5, 3], field angle uniform in [0, 90] deg, |B| uniform in [0.02, 0.1],
-    hydrogen density uniform in [0.05, 0.2]; FP32-bulk HLLD with the learned source.
-    Returns a list of (workload, hydrogen fields)."""
-    from .params import GEL_PREC_FP32_BULK, GEL_SRC_TABLE
+    hydrogen density uniform in [0.05, 0.2]; FP32-bulk HLLD with the learned source and
+    its analytic fallback (GEL_SRC_HYBRID).  Returns a list of (workload, hydrogen
+    fields)."""
+    from .params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID
     from .surrogate import workload_with_hydrogen
     if precision is None:
         precision = GEL_PREC_FP32_BULK
+    if source_mode is None:
+        source_mode = GEL_SRC_HYBRID
     rng = np.random.default_rng(seed)
     out = []
     for k in range(n_cases):
@@ -141,7 +145,7 @@ def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision
         w.prim0[5] = (1 - s) * w.params.inf_bx
         w.prim0[6] = (1 - s) * w.params.inf_by
         w.params.precision = precision
-        w.params.source_mode = GEL_SRC_TABLE
+        w.params.source_mode = source_mode
         w.name = f"ensemble-case{k:02d}-{ni}x{nj}"
         out.append((w, f))
     return out

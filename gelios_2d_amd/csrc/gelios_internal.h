@@ -9,6 +9,45 @@
 
 #include "../../include/gelios.h"
 
+namespace gel {
+
+// Arguments of one sweep step of one case (also what the fused source refresh reads:
+// the grid, the state after the step in `next`, the hydrogen moments and the table).
+struct SweepArgs {
+    int ni, nj;
+    int rows;                     // rows per chunk (the tile of one work-group)
+    int dt_rd, dt_wr, dt_clr;     // slots of the rotating time-step buffer
+    double s_old, s_new;          // node scale factors before / after the step (moving)
+    const double* __restrict__ x; // node positions before the step are x*s_old, y*s_old
+    const double* __restrict__ y;
+    const double* __restrict__ x2;  // after the step: x2*s_new, y2*s_new
+    const double* __restrict__ y2;
+    const void* __restrict__ now; // [NV][ncells] of the storage type
+    void* __restrict__ next;
+    const int8_t* __restrict__ ctype;
+    const int8_t* __restrict__ otype;
+    const double* __restrict__ src;
+    const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
+    const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
+    unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
+    unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
+    gel_params prm;
+};
+
+}  // namespace gel
+
+struct gel_mlp;
+
+// Fused source refresh (surrogate.hip): the source table args[k].qtab of each case
+// from its state args[k].next, hydrogen moments args[k].src and the network m, with
+// the analytic judge; fallback cells are added to *d_nfall.  One work-group per
+// tile of 256 cells; d_tiles[g] = (case, tile of the case) for work-group g, or null
+// for a single case (args[0]).  Asynchronous on stream s; no allocation.
+hipError_t gel_refresh_launch(hipStream_t s, const gel::SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                              gel_mlp* m, bool f32, double tol, int judge, unsigned long long* d_nfall);
+// tiles the refresh of an ni x nj case takes
+inline long long gel_refresh_tiles(int ni, int nj) { return ((long long)ni * nj + 255) / 256; }
+
 struct gel_ctx {
     int ni = 0, nj = 0;
     size_t nc = 0, nn = 0;
@@ -25,7 +64,8 @@ struct gel_ctx {
     int8_t* d_otype = nullptr;
     bool has_types = false;
     double* d_src = nullptr;       // [GEL_MK1_NFIELDS][nc] hydrogen moments and k factors
-    double* d_qtab = nullptr;      // [5][nc] per-cell source table (GEL_SRC_TABLE)
+    double* d_qtab = nullptr;      // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID)
+    int8_t* d_qflag = nullptr;     // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
     unsigned long long* d_dt = nullptr;   // [3]
     unsigned long long* d_cnt = nullptr;  // [3]
     long long steps = 0;

@@ -51,26 +51,6 @@ struct MhdPhys {
     __host__ __device__ static constexpr bool odd(int k) { return k == 3 || k == 4 || k == 6 || k == 7; }
 };
 
-struct SweepArgs {
-    int ni, nj;
-    int rows;                     // rows per wavefront chunk
-    int dt_rd, dt_wr, dt_clr;     // slots of the rotating time-step buffer
-    double s_old, s_new;          // node scale factors before / after the step (moving)
-    const double* __restrict__ x; // node positions before the step are x*s_old, y*s_old
-    const double* __restrict__ y;
-    const double* __restrict__ x2;  // after the step: x2*s_new, y2*s_new
-    const double* __restrict__ y2;
-    const void* __restrict__ now; // [NV][ncells] of the storage type
-    void* __restrict__ next;
-    const int8_t* __restrict__ ctype;
-    const int8_t* __restrict__ otype;
-    const double* __restrict__ src;
-    const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
-    unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
-    unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
-    gel_params prm;
-};
-
 template <class Ph, class TS>
 struct CellT {          // one cell of the stencil as seen by a lane
     TS s[Ph::NV];
@@ -809,11 +789,12 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 #pragma unroll
                     for (int k = 0; k < NV; k++) sd[k] = double(M.s[k]);
                     double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
-                    if (SRC == GEL_SRC_MK1) {
+                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && a.qflag[kc]);
+                    if (analytic) {
                         const double* __restrict__ sp = a.src + kc;
                         source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
                                    q1s, q2s, q3s);
-                    } else if (SRC == GEL_SRC_TABLE) {
+                    } else if (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) {
                         qms = a.qtab[kc];
                         q1s = a.qtab[nc + kc];
                         q2s = a.qtab[2 * nc + kc];
@@ -822,7 +803,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     }
                     double o[NV];
                     cell_update<Ph, sizeof(TF) == 4>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
-                                    SRC == GEL_SRC_TABLE, o, acc.n_rho, acc.n_p);
+                                    (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) && !analytic, o, acc.n_rho,
+                                    acc.n_p);
 #pragma unroll
                     for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
                 }
@@ -994,6 +976,7 @@ static void free_ctx(gel_ctx* c)
     (void)hipFree(c->d_x); (void)hipFree(c->d_y); (void)hipFree(c->d_x2); (void)hipFree(c->d_y2);
     (void)hipFree(c->d_state[0]); (void)hipFree(c->d_state[1]);
     (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src); (void)hipFree(c->d_qtab);
+    (void)hipFree(c->d_qflag);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1030,7 +1013,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD) return fail(GEL_E_UNSUPPORTED, "physics");
     if (prm->physics == GEL_PHYS_GAS && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
         return fail(GEL_E_UNSUPPORTED, "flux");
-    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1 && prm->source_mode != GEL_SRC_TABLE)
+    if (prm->source_mode < GEL_SRC_NONE || prm->source_mode > GEL_SRC_HYBRID)
         return fail(GEL_E_UNSUPPORTED, "source_mode");
     if (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK) return fail(GEL_E_UNSUPPORTED, "precision");
     if (!(prm->dt0 > 0.0) || !(prm->cfl > 0.0) || !(prm->gamma > 1.0)) return fail(GEL_E_ARG, "dt0, cfl, gamma");
@@ -1175,11 +1158,22 @@ int gel_set_source_table(gel_ctx* c, const double* q)
 {
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     const size_t nb = (size_t)5 * c->nc * sizeof(double);
+    if (!c->d_qflag && hipMalloc(&c->d_qflag, c->nc) != hipSuccess) return fail(GEL_E_NOMEM, "source flags");
+    HIP_OK(hipMemset(c->d_qflag, 0, c->nc));
     if (!c->d_qtab) {
         hipError_t e = hipMalloc(&c->d_qtab, nb);
         if (e != hipSuccess) return fail(GEL_E_NOMEM, "source table");
     }
     HIP_OK(hipMemcpy(c->d_qtab, q, nb, hipMemcpyHostToDevice));
+    return GEL_OK;
+}
+
+int gel_get_source_table(gel_ctx* c, double* q)
+{
+    if (!c || !q) return fail(GEL_E_ARG, "null argument");
+    if (!c->d_qtab) return fail(GEL_E_STATE, "source table not set");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(q, c->d_qtab, (size_t)5 * c->nc * sizeof(double), hipMemcpyDeviceToHost));
     return GEL_OK;
 }
 
@@ -1232,11 +1226,19 @@ static hipError_t launch_ms(gel_ctx* c, SweepArgs& a, hipStream_t s, bool moving
 {
 #define GEL_L(MV, SR) launch_t<Ph, TS, TF, MV, TYPES, SR, FLUX>(c, a, s)
     if (moving) {
-        return src == GEL_SRC_MK1 ? GEL_L(true, GEL_SRC_MK1) : src == GEL_SRC_TABLE ? GEL_L(true, GEL_SRC_TABLE)
-                                                                                    : GEL_L(true, GEL_SRC_NONE);
+        switch (src) {
+        case GEL_SRC_MK1: return GEL_L(true, GEL_SRC_MK1);
+        case GEL_SRC_TABLE: return GEL_L(true, GEL_SRC_TABLE);
+        case GEL_SRC_HYBRID: return GEL_L(true, GEL_SRC_HYBRID);
+        default: return GEL_L(true, GEL_SRC_NONE);
+        }
     }
-    return src == GEL_SRC_MK1 ? GEL_L(false, GEL_SRC_MK1) : src == GEL_SRC_TABLE ? GEL_L(false, GEL_SRC_TABLE)
-                                                                                 : GEL_L(false, GEL_SRC_NONE);
+    switch (src) {
+    case GEL_SRC_MK1: return GEL_L(false, GEL_SRC_MK1);
+    case GEL_SRC_TABLE: return GEL_L(false, GEL_SRC_TABLE);
+    case GEL_SRC_HYBRID: return GEL_L(false, GEL_SRC_HYBRID);
+    default: return GEL_L(false, GEL_SRC_NONE);
+    }
 #undef GEL_L
 }
 
@@ -1284,11 +1286,14 @@ static int one_step(gel_ctx* c, int moving)
     a.ctype = c->d_ctype; a.otype = c->d_otype;
     a.src = c->d_src;
     a.qtab = c->d_qtab;
+    a.qflag = c->d_qflag;
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
     a.prm = c->prm;
-    if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
-    if (c->prm.source_mode == GEL_SRC_TABLE && !c->d_qtab) return fail(GEL_E_STATE, "source table not set");
+    if ((c->prm.source_mode == GEL_SRC_MK1 || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_src)
+        return fail(GEL_E_STATE, "sources not set");
+    if ((c->prm.source_mode == GEL_SRC_TABLE || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_qtab)
+        return fail(GEL_E_STATE, "source table not set");
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         HIP_OK(hipEventCreate(&e0));
@@ -1474,6 +1479,15 @@ struct gel_batch {
     hipGraphExec_t exec = nullptr;
     bool graph_valid = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // learned source refreshed inside gel_batch_step every `every` steps
+    gel_mlp* model = nullptr;
+    int every = 0;
+    double tol = 1e-3;
+    int judge = 1;
+    long long refresh_tiles = 0;
+    int2* d_tiles = nullptr;                 // refresh work-group -> (case, tile of the case)
+    unsigned long long* d_nfall = nullptr;   // [1] fallback cells since the last stats call
+    long long refreshes = 0;
 };
 
 static constexpr int kCycle = 6;  // two state buffers x three time-step slots
@@ -1507,12 +1521,13 @@ static int batch_build_items(gel_batch* b)
     return GEL_OK;
 }
 
-// argument sets of the next kCycle steps, from each case's own buffer/slot phase
+// argument sets of the next kCycle steps, from each case's own buffer/slot phase,
+// and (block kCycle) the cases as they are now, for a refresh before any step
 static int batch_upload_args(gel_batch* b)
 {
     const size_t n = b->cases.size();
-    std::vector<SweepArgs> args(kCycle * n);
-    for (int p = 0; p < kCycle; p++)
+    std::vector<SweepArgs> args((kCycle + 1) * n);
+    for (int p = 0; p <= kCycle; p++)
         for (size_t cs = 0; cs < n; cs++) {
             gel_ctx* c = b->cases[cs];
             SweepArgs& a = args[p * n + cs];
@@ -1525,9 +1540,10 @@ static int batch_upload_args(gel_batch* b)
             a.now = c->d_state[cur];
             a.next = c->d_state[cur ^ 1];
             a.ctype = nullptr; a.otype = c->d_otype;
-            a.src = c->d_src; a.qtab = c->d_qtab;
+            a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag;
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
+            if (p == kCycle) a.next = c->d_state[c->cur];
         }
     HIP_OK(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(SweepArgs), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
@@ -1549,6 +1565,7 @@ static void batch_launch_s(const gel_batch* b, int phase, int src)
 {
     if (src == GEL_SRC_MK1) batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(b, phase);
     else if (src == GEL_SRC_TABLE) batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(b, phase);
+    else if (src == GEL_SRC_HYBRID) batch_launch_t<Ph, TS, TF, GEL_SRC_HYBRID, FLUX>(b, phase);
     else batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(b, phase);
 }
 
@@ -1569,6 +1586,28 @@ static hipError_t batch_launch(const gel_batch* b, int phase)
     return hipGetLastError();
 }
 
+// steps per captured graph: the time-step / buffer cycle, times what makes the
+// refresh schedule repeat
+static int batch_cycle(const gel_batch* b)
+{
+    if (!b->model || b->every <= 0) return kCycle;
+    int a = kCycle, c = b->every;
+    while (c) { const int t = a % c; a = c; c = t; }
+    return kCycle / a * b->every;
+}
+
+// step i of a cycle: the sweep, then the source refresh when one is due
+static hipError_t batch_launch_step(const gel_batch* b, int i)
+{
+    const int phase = i % kCycle;
+    hipError_t e = batch_launch(b, phase);
+    if (e == hipSuccess && b->model && b->every > 0 && (i + 1) % b->every == 0)
+        e = gel_refresh_launch(b->stream, b->d_args + (size_t)phase * b->cases.size(), b->d_tiles,
+                               b->refresh_tiles, b->model, b->cases[0]->prm.precision == GEL_PREC_FP32_BULK, b->tol,
+                               b->judge, b->d_nfall);
+    return e;
+}
+
 static void batch_advance(gel_batch* b, int n)
 {
     for (gel_ctx* c : b->cases) {
@@ -1585,6 +1624,8 @@ void gel_batch_destroy(gel_batch* b)
     batch_drop_graph(b);
     (void)hipFree(b->d_args);
     (void)hipFree(b->d_items);
+    (void)hipFree(b->d_nfall);
+    (void)hipFree(b->d_tiles);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1603,17 +1644,30 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
         if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
             c->prm.source_mode != p0.source_mode)
             return fail(GEL_E_UNSUPPORTED, "batched cases share physics, precision, flux and source mode");
-        if (c->prm.source_mode == GEL_SRC_MK1 && !c->d_src) return fail(GEL_E_STATE, "sources not set");
-        if (c->prm.source_mode == GEL_SRC_TABLE && !c->d_qtab) return fail(GEL_E_STATE, "source table not set");
+        if ((c->prm.source_mode == GEL_SRC_MK1 || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_src)
+            return fail(GEL_E_STATE, "sources not set");
+        if ((c->prm.source_mode == GEL_SRC_TABLE || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_qtab)
+            return fail(GEL_E_STATE, "source table not set");
     }
     gel_batch* b = new (std::nothrow) gel_batch();
     if (!b) return fail(GEL_E_NOMEM, "host");
     b->cases.assign(cases, cases + ncases);
+    std::vector<int2> tiles;
+    for (int k = 0; k < ncases; k++)
+        for (long long t = 0; t < gel_refresh_tiles(cases[k]->ni, cases[k]->nj); t++) tiles.push_back(make_int2(k, (int)t));
+    b->refresh_tiles = (long long)tiles.size();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-        hipMalloc(&b->d_args, (size_t)kCycle * ncases * sizeof(SweepArgs)) != hipSuccess) {
+        hipMalloc(&b->d_args, (size_t)(kCycle + 1) * ncases * sizeof(SweepArgs)) != hipSuccess ||
+        hipMalloc(&b->d_nfall, sizeof(unsigned long long)) != hipSuccess ||
+        hipMemset(b->d_nfall, 0, sizeof(unsigned long long)) != hipSuccess) {
         gel_batch_destroy(b);
         return fail(GEL_E_NOMEM, "batch");
+    }
+    if (hipMalloc(&b->d_tiles, tiles.size() * sizeof(int2)) != hipSuccess ||
+        hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+        gel_batch_destroy(b);
+        return fail(GEL_E_NOMEM, "batch refresh tiles");
     }
     const int st = batch_build_items(b);
     if (st != GEL_OK) { gel_batch_destroy(b); return st; }
@@ -1640,12 +1694,13 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
     if (st != GEL_OK) return st;
     HIP_OK(hipEventRecord(b->ev0, b->stream));
     int done = 0;
-    if (use_graph && nsteps >= kCycle) {
+    const int cyc = batch_cycle(b);
+    if (use_graph && nsteps >= cyc) {
         if (!b->graph_valid) {
             hipGraph_t g = nullptr;
             HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
             hipError_t e = hipSuccess;
-            for (int p = 0; p < kCycle && e == hipSuccess; p++) e = batch_launch(b, p);
+            for (int p = 0; p < cyc && e == hipSuccess; p++) e = batch_launch_step(b, p);
             const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
             if (e != hipSuccess || e2 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1656,15 +1711,16 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
             if (e != hipSuccess) return fail(GEL_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
             b->graph_valid = true;
         }
-        while (nsteps - done >= kCycle) {
+        while (nsteps - done >= cyc) {
             HIP_OK(hipGraphLaunch(b->exec, b->stream));
-            done += kCycle;
+            done += cyc;
         }
     }
-    for (int p = 0; done < nsteps; p++, done++) {
-        const hipError_t e = batch_launch(b, p % kCycle);
+    for (; done < nsteps; done++) {
+        const hipError_t e = batch_launch_step(b, done % cyc);
         if (e != hipSuccess) return fail(GEL_E_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     }
+    if (b->model && b->every > 0) b->refreshes += nsteps / b->every;
     HIP_OK(hipEventRecord(b->ev1, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     if (ms) {
@@ -1673,6 +1729,60 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
         *ms = t;
     }
     batch_advance(b, nsteps);
+    return GEL_OK;
+}
+
+int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, int judge)
+{
+    if (!b || every < 0 || (every > 0 && !m) || !(tol >= 0.0)) return fail(GEL_E_ARG, "bad argument");
+    if (every > 0)
+        for (const gel_ctx* c : b->cases)
+            if ((c->prm.source_mode != GEL_SRC_TABLE && c->prm.source_mode != GEL_SRC_HYBRID) || !c->d_src)
+                return fail(GEL_E_STATE, "a learned source needs GEL_SRC_TABLE or GEL_SRC_HYBRID cases with "
+                                         "hydrogen moments set");
+    b->model = every > 0 ? m : nullptr;
+    b->every = every;
+    b->tol = tol;
+    b->judge = judge ? 1 : 0;
+    batch_drop_graph(b);
+    return GEL_OK;
+}
+
+int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback)
+{
+    if (!b || !m || !(tol >= 0.0)) return fail(GEL_E_ARG, "bad argument");
+    for (const gel_ctx* c : b->cases) {
+        if (!c->d_src) return fail(GEL_E_STATE, "hydrogen moments not set");
+        if (!c->d_qtab) return fail(GEL_E_STATE, "source table not allocated (gel_set_source_table)");
+    }
+    for (gel_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
+    int st = batch_upload_args(b);
+    if (st != GEL_OK) return st;
+    unsigned long long* cnt = nullptr;
+    if (hipMalloc(&cnt, sizeof *cnt) != hipSuccess) return fail(GEL_E_NOMEM, "counter");
+    hipError_t e = hipMemsetAsync(cnt, 0, sizeof *cnt, b->stream);
+    if (e == hipSuccess)
+        e = gel_refresh_launch(b->stream, b->d_args + (size_t)kCycle * b->cases.size(), b->d_tiles,
+                               b->refresh_tiles, m, b->cases[0]->prm.precision == GEL_PREC_FP32_BULK, tol, judge, cnt);
+    unsigned long long h = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(cnt);
+    if (e != hipSuccess) return fail(GEL_E_HIP, std::string("batch refresh: ") + hipGetErrorString(e));
+    if (n_fallback) *n_fallback = (long long)h;
+    return GEL_OK;
+}
+
+int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallback_cells)
+{
+    if (!b) return fail(GEL_E_ARG, "null batch");
+    unsigned long long h = 0;
+    HIP_OK(hipStreamSynchronize(b->stream));
+    HIP_OK(hipMemcpy(&h, b->d_nfall, sizeof h, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemset(b->d_nfall, 0, sizeof h));
+    if (refreshes) *refreshes = b->refreshes;
+    if (fallback_cells) *fallback_cells = (long long)h;
+    b->refreshes = 0;
     return GEL_OK;
 }
 
@@ -1726,6 +1836,9 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         resident = resident_groups(kern, kNW * kWave);
     }
     const int wg_c[] = { resident, 2 * resident };
+    gel_mlp* model = b->model;      // trials run the sweep alone (the refresh does not depend on the shape)
+    b->model = nullptr;
+    batch_drop_graph(b);
     double best = 1e300;
     int br = 0, bw = 0;
     int st = GEL_OK;
@@ -1745,6 +1858,8 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         if (st != GEL_OK) break;
     }
     for (auto& s : saved) { (void)hipFree(s.st[0]); (void)hipFree(s.st[1]); }
+    b->model = model;
+    batch_drop_graph(b);
     if (st != GEL_OK) return st;
     b->rows = br;
     b->workgroups = bw;

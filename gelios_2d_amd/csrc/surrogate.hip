@@ -37,13 +37,9 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 using bf16 = unsigned short;
 
-__host__ __device__ inline bf16 f2bf(float f)
-{   // round to nearest even
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (bf16)0x7fc0;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
+__device__ inline bf16 f2bf(float f)
+{   // round to nearest even (v_cvt_pk_bf16_f32)
+    return __builtin_bit_cast(bf16, (__bf16)f);
 }
 __host__ __device__ inline float bf2f(bf16 b)
 {
@@ -53,7 +49,7 @@ __host__ __device__ inline float bf2f(bf16 b)
     return f;
 }
 
-__device__ inline float sigmoidf(float z) { return 1.0f / (1.0f + __expf(-z)); }
+__device__ inline float sigmoidf(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
 // network shape (inputs and outputs padded to the 16-wide MFMA tile)
 constexpr int kIn = 14, kInP = 16, kHid = 128, kOut = 5, kOutP = 16, kLayers = 5;
@@ -281,45 +277,190 @@ __global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __res
     for (int k = 0; k < 5; k++) qtab[k * nc + c] = q[k];
 }
 
-// Network output -> source table, with the analytic judge: a cell falls back to the
-// analytic source when any component misses by more than tol relative to
-// max(|q_analytic|, sd_k * scale), scale = src_coef * ro * n_H (the per-cell unit
-// of the normalised targets).
+// ------------------------------------------------------------ fused refresh ---
+// One work-group of 4 wavefronts per tile of 256 cells of one case: features (a
+// cell per thread) -> 5 MFMA layers over four passes of 64 rows with the
+// activations in LDS -> de-normalisation, analytic judge and fallback (a cell per
+// thread) -> source table.  The products are the forward's (gemm_nt): the same
+// v_mfma_f32_16x16x16bf16 sequence over k, the same bias / SiLU / BF16 rounding, so
+// the table equals the one the unfused forward gives.
+constexpr int kTile = 256;         // cells per work-group (one per thread)
+constexpr int kSub = 64;           // rows per MFMA pass
+constexpr int kLd = kHid + 8;      // LDS row stride (BF16) of the activations
+
+struct NetDev {
+    const bf16* w[kLayers];        // BF16 W_l [out_l][in_l] (padded)
+    const float* b[kLayers];
+    const float* mu;
+    const float* sd;
+    const float* ymu;
+    const float* ysd;
+};
+
+// normalised features of cell c (features_kernel), BF16-rounded input row
 template <class TS>
-__global__ void finalize_kernel(size_t c0, int n, size_t nc, gel_params prm, const TS* __restrict__ st,
-                                const double* __restrict__ src, const float* __restrict__ y,
-                                const float* __restrict__ ymu, const float* __restrict__ ysd, double tol,
-                                int judge, double* __restrict__ qtab, unsigned long long* __restrict__ nfall)
+__device__ inline void refresh_features(const SweepArgs& a, size_t c, const NetDev& net, bf16* row)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= n) return;
-    const size_t c = c0 + b;
-    const double ro = st[c];
-    const double nH = src[c] + src[nc + c] + src[2 * nc + c] + src[3 * nc + c];
-    const double scale = prm.src_coef * ro * nH;
-    double qm[5];
+    const size_t nc = (size_t)a.ni * a.nj;
+    const int nj = a.nj;
+    const int i = (int)(c / nj), j = (int)(c % nj);
+    const size_t ka = (size_t)i * (nj + 1) + j, kb = ka + (nj + 1);
+    const double px[4] = { a.x[ka], a.x[kb], a.x[kb + 1], a.x[ka + 1] };
+    const double py[4] = { a.y[ka], a.y[kb], a.y[kb + 1], a.y[ka + 1] };
+    double cx, cy;
+    quad_center(px, py, cx, cy);
+    const TS* st = (const TS*)a.next;
+    const double* src = a.src;
+    const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
+    double f[kIn];
+    f[0] = log(ro); f[1] = u; f[2] = v; f[3] = log(p / ro);
+    for (int g = 0; g < 2; g++) {
+        const int k0 = 2 * g, k1 = 2 * g + 1;
+        const double n0 = src[k0 * nc + c], n1 = src[k1 * nc + c];
+        const double n = n0 + n1;
+        const double w0 = n > 0 ? n0 / n : 0.5, w1 = n > 0 ? n1 / n : 0.5;
+        f[4 + 4 * g] = log(fmax(n, 1e-30));
+        f[5 + 4 * g] = w0 * src[(4 + k0) * nc + c] + w1 * src[(4 + k1) * nc + c];
+        f[6 + 4 * g] = w0 * src[(8 + k0) * nc + c] + w1 * src[(8 + k1) * nc + c];
+        f[7 + 4 * g] = log(fmax(w0 * src[(12 + k0) * nc + c] + w1 * src[(12 + k1) * nc + c], 1e-30));
+    }
+    f[12] = cy; f[13] = cx;
 #pragma unroll
-    for (int k = 0; k < 5; k++) qm[k] = ((double)y[(size_t)b * kOutP + k] * ysd[k] + ymu[k]) * scale;
-    bool fall = false;
-    if (judge) {
-        double qa[5];
-        analytic_q(prm, nc, c, st, src, qa);
-        double err = 0.0;
+    for (int k = 0; k < kIn; k++) row[k] = f2bf((float)((f[k] - net.mu[k]) / net.sd[k]));
+    row[14] = 0; row[15] = 0;
+}
+
+// One hidden layer for the 64 rows of the tile: wavefront w computes output
+// columns [32 w, 32 w + 32) of all rows; Z = A W^T + b, H = silu(Z) in BF16.
+template <int K>
+__device__ inline void refresh_hidden(const bf16* __restrict__ A, int lda, const bf16* __restrict__ W,
+                                      const float* __restrict__ bias, bf16* __restrict__ H)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r = lane & 15, q = lane >> 4;
+    const int n0 = wave * 32;
+    f4 acc[4][2];
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const double den = fmax(fabs(qa[k]), (double)ysd[k] * scale);
-            if (den > 0.0) err = fmax(err, fabs(qm[k] - qa[k]) / den);
-        }
-        if (!(err <= tol)) {
-            fall = true;
+    for (int a = 0; a < 4; a++)
 #pragma unroll
-            for (int k = 0; k < 5; k++) qm[k] = qa[k];
-        }
+        for (int b = 0; b < 2; b++) acc[a][b] = f4{ 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+    for (int k = 0; k < K; k += 16) {
+        s4 af[4], bfr[2];
+#pragma unroll
+        for (int b = 0; b < 2; b++) bfr[b] = *(const s4*)(W + (size_t)(n0 + b * 16 + r) * K + k + 4 * q);
+#pragma unroll
+        for (int a = 0; a < 4; a++) af[a] = *(const s4*)(A + (a * 16 + r) * lda + k + 4 * q);
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
 #pragma unroll
-    for (int k = 0; k < 5; k++) qtab[k * nc + c] = qm[k];
-    if (fall) atomicAdd(nfall, 1ull);
+    for (int b = 0; b < 2; b++) {
+        const int n = n0 + b * 16 + r;
+        const float bn = bias[n];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float z = acc[a][b][j] + bn;
+                H[(a * 16 + 4 * q + j) * kLd + n] = f2bf(z * sigmoidf(z));
+            }
+    }
 }
+
+template <class TS>
+__global__ void __launch_bounds__(256)
+refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tiles, NetDev net, double tol,
+               int judge, unsigned long long* __restrict__ nfall)
+{
+    // xs: the BF16 input rows; a row's 32 bytes take its 5 FP32 outputs once the
+    // output layer has run (the row's input is consumed by then)
+    __shared__ __attribute__((aligned(16))) bf16 xs[kTile * kInP];
+    __shared__ __attribute__((aligned(16))) bf16 ha[kSub * kLd];
+    __shared__ __attribute__((aligned(16))) bf16 hb[kSub * kLd];
+    float* ys = (float*)xs;                 // [kTile][8] FP32
+
+    int cs = 0;
+    long long t = blockIdx.x;
+    if (tiles) {                        // (case, tile of that case)
+        const int2 e = tiles[blockIdx.x];
+        cs = e.x;
+        t = e.y;
+    }
+    const SweepArgs& a = args[cs];
+    const size_t nc = (size_t)a.ni * a.nj;
+    const size_t c0 = (size_t)t * kTile;
+    const int tid = threadIdx.x;
+    const size_t c = c0 + tid;
+    {
+        bf16* row = xs + tid * kInP;
+        if (c < nc) refresh_features<TS>(a, c, net, row);
+        else
+            for (int k = 0; k < kInP; k++) row[k] = 0;
+    }
+    __syncthreads();
+    for (int sub = 0; sub < kTile / kSub; sub++) {
+        if (c0 + (size_t)sub * kSub >= nc) break;       // uniform
+        refresh_hidden<kInP>(xs + sub * kSub * kInP, kInP, net.w[0], net.b[0], ha);
+        __syncthreads();
+        refresh_hidden<kHid>(ha, kLd, net.w[1], net.b[1], hb);
+        __syncthreads();
+        refresh_hidden<kHid>(hb, kLd, net.w[2], net.b[2], ha);
+        __syncthreads();
+        refresh_hidden<kHid>(ha, kLd, net.w[3], net.b[3], hb);
+        __syncthreads();
+        {   // output layer: wavefront w takes rows [16 w, 16 w + 16), all 16 (padded) outputs
+            const int lane = tid & 63, wave = tid >> 6;
+            const int r = lane & 15, q = lane >> 4;
+            f4 acc = f4{ 0.f, 0.f, 0.f, 0.f };
+            for (int k = 0; k < kHid; k += 16) {
+                const s4 af = *(const s4*)(hb + (wave * 16 + r) * kLd + k + 4 * q);
+                const s4 bfr = *(const s4*)(net.w[4] + (size_t)r * kHid + k + 4 * q);
+                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc, 0, 0, 0);
+            }
+            if (r < kOut) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) ys[(sub * kSub + wave * 16 + 4 * q + j) * 8 + r] = acc[j] + net.b[4][r];
+            }
+        }
+        __syncthreads();
+    }
+    bool fall = false;
+    if (c < nc) {
+        const TS* st = (const TS*)a.next;
+        const double* src = a.src;
+        double* qtab = const_cast<double*>(a.qtab);
+        const double ro = st[c];
+        const double nH = src[c] + src[nc + c] + src[2 * nc + c] + src[3 * nc + c];
+        const double scale = a.prm.src_coef * ro * nH;
+        double qm[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) qm[k] = ((double)ys[tid * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
+        if (judge) {
+            double qa[5];
+            analytic_q(a.prm, nc, c, st, src, qa);
+            double err = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; k++) {
+                const double den = fmax(fabs(qa[k]), (double)net.ysd[k] * scale);
+                if (den > 0.0) err = fmax(err, fabs(qm[k] - qa[k]) / den);
+            }
+            if (!(err <= tol)) {
+                fall = true;
+#pragma unroll
+                for (int k = 0; k < 5; k++) qm[k] = qa[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) qtab[k * nc + c] = qm[k];
+        if (a.qflag) const_cast<int8_t*>(a.qflag)[c] = fall ? 1 : 0;
+    }
+    const unsigned long long mask = __ballot(fall);
+    if ((tid & 63) == 0 && mask) atomicAdd(nfall, (unsigned long long)__popcll(mask));
+}
+
 }  // namespace gel
 
 using namespace gel;
@@ -366,6 +507,9 @@ struct gel_mlp {
     float* ymu = nullptr;
     float* ysd = nullptr;
     hipStream_t stream = nullptr;
+    // single-case refresh: its argument block and fallback counter
+    SweepArgs* d_rargs = nullptr;
+    unsigned long long* d_rcount = nullptr;
 };
 
 static size_t wsz(int l) { return (size_t)kOut_[l] * kIn_[l]; }
@@ -463,6 +607,22 @@ static void mlp_quantize(gel_mlp* m)
     }
 }
 
+hipError_t gel_refresh_launch(hipStream_t s, const SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                              gel_mlp* m, bool f32, double tol, int judge, unsigned long long* d_nfall)
+{
+    if (tiles <= 0) return hipSuccess;
+    NetDev net;
+    for (int l = 0; l < kLayers; l++) { net.w[l] = m->wb[l]; net.b[l] = m->b[l]; }
+    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd;
+    if (f32)
+        hipLaunchKernelGGL(refresh_kernel<float>, dim3((unsigned)tiles), dim3(256), 0, s, d_args, d_tiles, net, tol,
+                           judge, d_nfall);
+    else
+        hipLaunchKernelGGL(refresh_kernel<double>, dim3((unsigned)tiles), dim3(256), 0, s, d_args, d_tiles, net, tol,
+                           judge, d_nfall);
+    return hipGetLastError();
+}
+
 static uint64_t splitmix(uint64_t& s)
 {
     uint64_t z = (s += 0x9e3779b97f4a7c15ull);
@@ -493,6 +653,7 @@ void gel_mlp_destroy(gel_mlp* m)
     }
     (void)hipFree(m->set_x); (void)hipFree(m->set_t);
     (void)hipFree(m->mu); (void)hipFree(m->sd); (void)hipFree(m->ymu); (void)hipFree(m->ysd);
+    (void)hipFree(m->d_rargs); (void)hipFree(m->d_rcount);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }
@@ -769,51 +930,28 @@ int gel_surrogate_refresh(gel_ctx* c, gel_mlp* m, double tol, int judge, long lo
     if (!m || !m->mu) return mfail(GEL_E_STATE, "network scaling not set");
     if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * sizeof(double)) != hipSuccess)
         return mfail(GEL_E_NOMEM, "source table");
-    const int chunk = 65536;
-    st = mlp_reserve(m, chunk);
-    if (st != GEL_OK) return st;
-    float* feat = nullptr;
-    unsigned long long* dcount = nullptr;
-    if (hipMalloc((void**)&feat, c->nc * kIn * sizeof(float)) != hipSuccess ||
-        hipMalloc((void**)&dcount, sizeof(unsigned long long)) != hipSuccess) {
-        (void)hipFree(feat);
+    if (!c->d_qflag && (hipMalloc(&c->d_qflag, c->nc) != hipSuccess || hipMemset(c->d_qflag, 0, c->nc) != hipSuccess))
+        return mfail(GEL_E_NOMEM, "source flags");
+    if (!m->d_rargs && (hipMalloc((void**)&m->d_rargs, sizeof(SweepArgs)) != hipSuccess ||
+                        hipMalloc((void**)&m->d_rcount, sizeof(unsigned long long)) != hipSuccess))
         return mfail(GEL_E_NOMEM, "refresh buffers");
-    }
-    // the network runs on the context's stream so that the refresh orders with the sweep
-    hipStream_t saved = m->stream;
-    m->stream = c->stream;
-    hipError_t e = hipMemsetAsync(dcount, 0, sizeof(unsigned long long), c->stream);
-    const dim3 g((unsigned)((c->nc + 255) / 256));
-    const bool f64 = (c->ts == 8);
-    if (f64)
-        hipLaunchKernelGGL(features_kernel<double>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
-                           (const double*)c->d_state[c->cur], c->d_src, m->mu, m->sd, feat);
-    else
-        hipLaunchKernelGGL(features_kernel<float>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
-                           (const float*)c->d_state[c->cur], c->d_src, m->mu, m->sd, feat);
-    for (size_t c0 = 0; c0 < c->nc && e == hipSuccess; c0 += chunk) {
-        const int n = (int)std::min<size_t>(chunk, c->nc - c0);
-        const int B = (n + 15) / 16 * 16;
-        hipLaunchKernelGGL(pack_input_kernel, dim3((B * kInP + 255) / 256), dim3(256), 0, c->stream, n, B,
-                           feat + c0 * kIn, m->x, m->xt);
-        mlp_forward_dev(m, B);
-        const dim3 gf((unsigned)((n + 255) / 256));
-        if (f64)
-            hipLaunchKernelGGL(finalize_kernel<double>, gf, dim3(256), 0, c->stream, c0, n, c->nc, c->prm,
-                               (const double*)c->d_state[c->cur], c->d_src, m->y, m->ymu, m->ysd, tol, judge,
-                               c->d_qtab, dcount);
-        else
-            hipLaunchKernelGGL(finalize_kernel<float>, gf, dim3(256), 0, c->stream, c0, n, c->nc, c->prm,
-                               (const float*)c->d_state[c->cur], c->d_src, m->y, m->ymu, m->ysd, tol, judge,
-                               c->d_qtab, dcount);
-        e = hipGetLastError();
-    }
+    SweepArgs a{};
+    a.ni = c->ni; a.nj = c->nj;
+    a.x = c->d_x; a.y = c->d_y; a.x2 = c->d_x; a.y2 = c->d_y;
+    a.now = c->d_state[c->cur];
+    a.next = c->d_state[c->cur];
+    a.src = c->d_src;
+    a.qtab = c->d_qtab;
+    a.qflag = c->d_qflag;
+    a.prm = c->prm;
+    hipError_t e = hipMemcpyAsync(m->d_rargs, &a, sizeof a, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(m->d_rcount, 0, sizeof(unsigned long long), c->stream);
+    if (e == hipSuccess)
+        e = gel_refresh_launch(c->stream, m->d_rargs, nullptr, gel_refresh_tiles(c->ni, c->nj), m, c->ts == 4, tol,
+                               judge, m->d_rcount);
     unsigned long long cnt = 0;
-    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, dcount, sizeof cnt, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(&cnt, m->d_rcount, sizeof cnt, hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    m->stream = saved;
-    (void)hipFree(feat);
-    (void)hipFree(dcount);
     if (e != hipSuccess) return mfail(GEL_E_HIP, std::string("refresh: ") + hipGetErrorString(e));
     if (n_fallback) *n_fallback = (long long)cnt;
     return GEL_OK;

@@ -15,7 +15,7 @@ GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_C_NO = range(7)
 GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
 GEL_PHYS_GAS, GEL_PHYS_MHD = 0, 1
 NVAR_MHD = 9
-GEL_SRC_NONE, GEL_SRC_MK1, GEL_SRC_TABLE = 0, 1, 2
+GEL_SRC_NONE, GEL_SRC_MK1, GEL_SRC_TABLE, GEL_SRC_HYBRID = 0, 1, 2, 3
 GEL_PREC_FP64, GEL_PREC_FP32_BULK = 0, 1
 GEL_MK1_NFIELDS = 19
 NVAR_GAS = 5

@@ -69,6 +69,12 @@ class Setka:
         + phase)), k = step mod period; steps become moving steps."""
         _lib.check(_lib.lib().gel_set_motion(self._ctx, float(amp), int(period), float(phase)))
 
+    def get_source_table(self) -> np.ndarray:
+        """The source table [5][ncells] (mass, momentum x, momentum y, energy, tracer)."""
+        q = np.empty((5, self.ncells))
+        _lib.check(_lib.lib().gel_get_source_table(self._ctx, _lib.dptr(q)))
+        return q
+
     def set_source_table(self, q: np.ndarray) -> None:
         """Per-cell sources for GEL_SRC_TABLE: (mass, mom x, mom y, energy, tracer) x ncells."""
         a = np.ascontiguousarray(q, np.float64).reshape(5, self.ncells)

@@ -91,7 +91,11 @@ enum gel_source_mode {
     /* Per-cell source table (mass, momentum x, momentum y, energy, tracer per unit
        r-weighted volume), set with gel_set_source_table or filled on the device by
        gel_surrogate_refresh. */
-    GEL_SRC_TABLE = 2
+    GEL_SRC_TABLE = 2,
+    /* The learned source with its analytic fallback: cells the last refresh's judge
+       flagged take the GEL_SRC_MK1 closure at every step, the others the table.
+       Needs the hydrogen moments and the table; the flags start cleared. */
+    GEL_SRC_HYBRID = 3
 };
 
 /* Number of per-cell source inputs for GEL_SRC_MK1, variable-major like the state:
@@ -165,8 +169,10 @@ int gel_set_sources(gel_ctx* ctx, const double* fields);
 
 /* Per-cell source table for GEL_SRC_TABLE: 5 variable-major arrays of ncells doubles
  * (mass, momentum x, momentum y, energy, tracer), applied like the sources q2_1, q2_2,
- * q3 of the update (Setka.cpp:10444-10446). */
+ * q3 of the update (Setka.cpp:10444-10446).  Clears the fallback flags. */
 int gel_set_source_table(gel_ctx* ctx, const double* q);
+/* Copy the source table [5][ncells] back to the host (after a refresh, for tests). */
+int gel_get_source_table(gel_ctx* ctx, double* q);
 
 /* Advance nsteps steps: replaces the body of the step loop of
  * Setka::Go_5_komponent__MK2 (Setka.cpp:10131-10481) on a static grid.
@@ -260,8 +266,19 @@ const char* gel_mlp_last_error(void);
 int gel_surrogate_features(gel_ctx* ctx, float* x);
 int gel_source_analytic(gel_ctx* ctx, double* q);
 /* Fill the context's source table (GEL_SRC_TABLE) from the network; with judge != 0
- * cells whose error against the analytic source exceeds tol fall back to it. */
+ * cells whose error against the analytic source exceeds tol fall back to it.  One
+ * fused launch (features, five MFMA layers, judge) over tiles of 64 cells. */
 int gel_surrogate_refresh(gel_ctx* ctx, gel_mlp* m, double tol, int judge, long long* n_fallback);
+/* The same for every case of a batch at once (their current states). */
+int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback);
+/* Attach a network to a batch of GEL_SRC_TABLE cases: gel_batch_step then refreshes
+ * every case's source table after every `every`-th step of a call (also inside the
+ * captured graph, whose cycle becomes a common multiple of 6 and `every`).
+ * every = 0 detaches. */
+int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, int judge);
+/* Refreshes run by gel_batch_step and the cells that fell back in them, since the
+ * last call (both reset). */
+int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallback_cells);
 
 const char* gel_strerror(int status);
 const char* gel_last_error(void);

@@ -60,3 +60,45 @@ def test_autotune_restores_state():
     assert rows > 0 and wg > 0 and ms > 0
     for s, x in zip(cases, before):
         assert np.array_equal(s.download(), x)
+
+
+def _model():
+    from gelios_2d_amd.surrogate import SourceModel
+    from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    return SourceModel.load(DEFAULT_OUT)
+
+
+def test_batch_refresh_equals_per_case_refresh():
+    from gelios_2d_amd.batch import Batch
+    m = _model()
+    a, b_ = _cases(4), _cases(4)
+    na = sum(m.refresh(s, tol=1e-3, judge=True) for s in a)
+    B = Batch(b_)
+    nb = B.refresh(m, tol=1e-3, judge=True)
+    assert na == nb
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.get_source_table(), y.get_source_table())
+
+
+@pytest.mark.parametrize("every", [3, 4])
+def test_refresh_inside_graph_equals_interleaved_calls(every):
+    """Sources refreshed by the batch every `every` steps (inside the captured graph)
+    give the same states and tables as stepping and refreshing call by call."""
+    from gelios_2d_amd.batch import Batch
+    m = _model()
+    a, b_ = _cases(3), _cases(3)
+    A, B = Batch(a), Batch(b_)
+    A.refresh(m)
+    B.refresh(m)
+    B.set_source_model(m, every, tol=1e-3, judge=True)
+    total = 2 * 12 + every + 1
+    B.step(total, True)
+    for k in range(total // every):
+        A.step(every, False)
+        A.refresh(m)
+    A.step(total % every, False)
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.download(), y.download())
+        assert np.array_equal(x.get_source_table(), y.get_source_table())
+    r, n = B.source_stats()
+    assert r == total // every and 0 <= n

@@ -77,3 +77,71 @@ def test_trained_network_judge_and_gain():
     err_none = rel_err_conserved(Z, A)
     print(f"fallback cells {nfall}/{net.ncells}; error vs analytic: network {err_net:.3e}, no source {err_none:.3e}")
     assert err_net < 0.1 * err_none
+
+
+def _model():
+    from gelios_2d_amd.surrogate import SourceModel
+    from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    return SourceModel.load(DEFAULT_OUT)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_refresh_judge_at_zero_tolerance_is_the_analytic_table(precision):
+    """With tol = 0 every cell falls back, so the fused refresh must leave exactly the
+    analytic sources (Cell::Get_Sourse_MK1, Cell.cpp:590-664) of the current state."""
+    from gelios_2d_amd import surrogate as sg
+    w, f = sg.workload_with_hydrogen(70, 50, 0, 2.0, 0.1, 7, moving=False)
+    w.params.precision = precision
+    s = _setka(w, f, GEL_SRC_TABLE)
+    n = _model().refresh(s, tol=0.0, judge=True)
+    assert n == s.ncells
+    assert np.array_equal(s.get_source_table(), sg.analytic_sources(s))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_refresh_matches_unfused_forward(precision):
+    """Network-only table (no judge) against the unfused MFMA forward (gel_mlp_forward)
+    on the same normalised features: the layers, biases, SiLU and the output scaling
+    agree; inputs that round to a different BF16 on the two paths (features normalised
+    from FP64 on the device, from FP32 on the host) move single cells slightly."""
+    from gelios_2d_amd import surrogate as sg
+    w, f = sg.workload_with_hydrogen(90, 60, 0, 2.4, 0.12, 9, moving=False)
+    w.params.precision = precision
+    s = _setka(w, f, GEL_SRC_TABLE)
+    m = _model()
+    m.refresh(s, tol=1e-3, judge=False)
+    got = s.get_source_table()
+    x = ((sg.features(s).astype(np.float64) - m.mu) / m.sd).astype(np.float32)
+    y = m.mlp.forward(x).astype(np.float64)
+    prim = s.download()
+    scale = w.params.src_coef * prim[0] * f[0:4].sum(axis=0)
+    want = ((y * m.ysd.astype(np.float64) + m.ymu.astype(np.float64)) * scale[:, None]).T
+    unit = np.maximum(m.ysd.astype(np.float64)[:, None] * scale[None, :], 1e-300)
+    rel = np.abs(got - want) / unit
+    assert np.median(rel) < 1e-6
+    assert np.mean(rel < 1e-6) > 0.9
+    assert rel.max() < 0.1
+
+
+def test_hybrid_mode_limits():
+    """GEL_SRC_HYBRID after a refresh with tol = 0 (every cell flagged) is the MK1 sweep
+    bit for bit; with no judge (no cell flagged) it is the table sweep."""
+    from gelios_2d_amd import surrogate as sg
+    from gelios_2d_amd.params import GEL_SRC_HYBRID
+    w, f = sg.workload_with_hydrogen(80, 64, 0, 2.2, 0.1, 13, moving=False)
+    m = _model()
+    mk1 = _setka(w, f, GEL_SRC_MK1)
+    hyb = _setka(w, f, GEL_SRC_HYBRID)
+    hyb.set_source_table(np.zeros((5, hyb.ncells)))
+    assert m.refresh(hyb, tol=0.0, judge=True) == hyb.ncells
+    mk1.Go_5_komponent__MK2(12)
+    hyb.Go_5_komponent__MK2(12)
+    assert np.array_equal(mk1.download(), hyb.download())
+    tab = _setka(w, f, GEL_SRC_TABLE)
+    hyb2 = _setka(w, f, GEL_SRC_HYBRID)
+    hyb2.set_source_table(np.zeros((5, hyb2.ncells)))
+    m.refresh(tab, tol=1e-3, judge=False)
+    assert m.refresh(hyb2, tol=1e-3, judge=False) == 0
+    tab.Go_5_komponent__MK2(12)
+    hyb2.Go_5_komponent__MK2(12)
+    assert np.array_equal(tab.download(), hyb2.download())
