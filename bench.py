@@ -40,16 +40,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def bytes_per_cell_update(w):
+def bytes_per_cell_update(w, fallback_fraction=0.0):
     """Algorithmic HBM bytes per cell-update: the state read once and written once
     (2 x variables x bytes per value), one node's coordinates (16 B; 32 B on a moving
     grid, positions before and after the step), and the per-cell source inputs: the
-    table (5 x 8 B) or the hydrogen moments (19 x 8 B)."""
-    from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_MK1, GEL_SRC_TABLE
+    table (5 x 8 B), the hydrogen moments (19 x 8 B), or in the hybrid mode a flag
+    byte and the moments for the fallback fraction of cells, the table for the rest."""
+    from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID, GEL_SRC_MK1, GEL_SRC_TABLE
     nv = 9 if w.params.physics == 1 else 5
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
     geom = 32 if w.motion else 16
     src = {GEL_SRC_TABLE: 40, GEL_SRC_MK1: 152}.get(w.params.source_mode, 0)
+    if w.params.source_mode == GEL_SRC_HYBRID:
+        src = 1 + fallback_fraction * 152 + (1.0 - fallback_fraction) * 40
     return 2 * nv * s + geom + src
 
 
@@ -112,7 +115,7 @@ def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
     ncells = sum(s.ncells for s in cases)
     b = Batch(cases)
     nfall = b.refresh(model, tol=1e-3, judge=True)
-    rows, wg, ms_trial = b.autotune(6)
+    rows, wg, ms_trial = b.autotune(12)
     b.set_source_model(model, refresh_every, tol=1e-3, judge=True)
     cyc = 6 * refresh_every // math.gcd(6, refresh_every)
     log(f"bench[{log_prefix}]: {len(cases)} cases, {ncells} cells; autotuner rows {rows} workgroups {wg} "
@@ -198,7 +201,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="ensemble")
-    ap.add_argument("--refresh-every", type=int, default=4)
+    ap.add_argument("--refresh-every", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -231,7 +234,7 @@ def main():
         l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
         log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
             f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
-        B = bytes_per_cell_update(specs[0][0])
+        B = bytes_per_cell_update(specs[0][0], r5["fallback_fraction"])
         achieved = B * r5["cells"] / (r5["sweep_ms_per_step"] * 1e-3) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
                   "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
