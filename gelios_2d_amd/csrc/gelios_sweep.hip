@@ -792,8 +792,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && a.qflag[kc]);
                     if (analytic) {
                         const double* __restrict__ sp = a.src + kc;
-                        source_mk1(a.prm, [&](int k) { return sp[(size_t)k * nc]; }, sd[2], sd[3], sd[0], sd[1],
-                                   q1s, q2s, q3s);
+                        auto fk = [&](int k) { return sp[(size_t)k * nc]; };
+                        source_mk1<decltype(fk), sizeof(TF) == 4>(a.prm, fk, sd[2], sd[3], sd[0], sd[1], q1s, q2s, q3s);
                     } else if (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) {
                         qms = a.qtab[kc];
                         q1s = a.qtab[nc + kc];

@@ -279,11 +279,11 @@ __global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __res
 
 // ------------------------------------------------------------ fused refresh ---
 // One work-group of 4 wavefronts per tile of 256 cells of one case: features (a
-// cell per thread) -> 5 MFMA layers over four passes of 64 rows with the
+// cell per thread) -> 5 MFMA layers over four passes of 64 cells with the
 // activations in LDS -> de-normalisation, analytic judge and fallback (a cell per
-// thread) -> source table.  The products are the forward's (gemm_nt): the same
-// v_mfma_f32_16x16x16bf16 sequence over k, the same bias / SiLU / BF16 rounding, so
-// the table equals the one the unfused forward gives.
+// thread) -> source table.  Bias, SiLU and BF16 rounding are the forward's
+// (gemm_nt); the 128-wide products run on the K = 32 MFMA, whose FP32 sums group
+// the terms differently from the forward's K = 16 steps.
 constexpr int kTile = 256;         // cells per work-group (one per thread)
 constexpr int kSub = 64;           // rows per MFMA pass
 constexpr int kLd = kHid + 8;      // LDS row stride (BF16) of the activations
@@ -330,43 +330,65 @@ __device__ inline void refresh_features(const SweepArgs& a, size_t c, const NetD
     row[14] = 0; row[15] = 0;
 }
 
-// One hidden layer for the 64 rows of the tile: wavefront w computes output
-// columns [32 w, 32 w + 32) of all rows; Z = A W^T + b, H = silu(Z) in BF16.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+
+// One hidden layer for the 64 cells of a pass, as D[unit][cell] = sum_k W[unit][k] *
+// A[cell][k] (weights as the MFMA's first operand, cells as its columns), so that a
+// lane's four results are four consecutive units of one cell: one 8-byte LDS store.
+// Wavefront w computes units [32 w, 32 w + 32).  K = 16 (the padded input) takes
+// v_mfma_f32_16x16x16_bf16, K = 128 v_mfma_f32_16x16x32_bf16 (lane l: row l % 16,
+// k 8 (l / 16) .. + 7).
 template <int K>
 __device__ inline void refresh_hidden(const bf16* __restrict__ A, int lda, const bf16* __restrict__ W,
                                       const float* __restrict__ bias, bf16* __restrict__ H)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r = lane & 15, q = lane >> 4;
-    const int n0 = wave * 32;
-    f4 acc[4][2];
+    const int u0 = wave * 32;
+    f4 acc[2][4];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = f4{ 0.f, 0.f, 0.f, 0.f };
+        for (int c = 0; c < 4; c++) acc[b][c] = f4{ 0.f, 0.f, 0.f, 0.f };
+    if constexpr (K == 16) {
+        s4 wa[2], hb[4];
 #pragma unroll
-    for (int k = 0; k < K; k += 16) {
-        s4 af[4], bfr[2];
+        for (int b = 0; b < 2; b++) wa[b] = *(const s4*)(W + (size_t)(u0 + b * 16 + r) * K + 4 * q);
 #pragma unroll
-        for (int b = 0; b < 2; b++) bfr[b] = *(const s4*)(W + (size_t)(n0 + b * 16 + r) * K + k + 4 * q);
+        for (int c = 0; c < 4; c++) hb[c] = *(const s4*)(A + (c * 16 + r) * lda + 4 * q);
 #pragma unroll
-        for (int a = 0; a < 4; a++) af[a] = *(const s4*)(A + (a * 16 + r) * lda + k + 4 * q);
+        for (int b = 0; b < 2; b++)
 #pragma unroll
-        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 4; c++) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wa[b], hb[c], acc[b][c], 0, 0, 0);
+    } else {
 #pragma unroll
-            for (int b = 0; b < 2; b++) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[a], bfr[b], acc[a][b], 0, 0, 0);
+        for (int k = 0; k < K; k += 32) {
+            bf8 wa[2], hb[4];
+#pragma unroll
+            for (int b = 0; b < 2; b++) wa[b] = *(const bf8*)(W + (size_t)(u0 + b * 16 + r) * K + k + 8 * q);
+#pragma unroll
+            for (int c = 0; c < 4; c++) hb[c] = *(const bf8*)(A + (c * 16 + r) * lda + k + 8 * q);
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[b][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa[b], hb[c], acc[b][c], 0, 0, 0);
+        }
     }
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-        const int n = n0 + b * 16 + r;
-        const float bn = bias[n];
+        const int u = u0 + b * 16 + 4 * q;
+        const float b0 = bias[u], b1 = bias[u + 1], b2 = bias[u + 2], b3 = bias[u + 3];
 #pragma unroll
-        for (int a = 0; a < 4; a++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float z = acc[a][b][j] + bn;
-                H[(a * 16 + 4 * q + j) * kLd + n] = f2bf(z * sigmoidf(z));
-            }
+        for (int c = 0; c < 4; c++) {
+            const float z0 = acc[b][c][0] + b0, z1 = acc[b][c][1] + b1, z2 = acc[b][c][2] + b2, z3 = acc[b][c][3] + b3;
+            const bf2 lo = { (__bf16)(z0 * sigmoidf(z0)), (__bf16)(z1 * sigmoidf(z1)) };
+            const bf2 hi = { (__bf16)(z2 * sigmoidf(z2)), (__bf16)(z3 * sigmoidf(z3)) };
+            uint2 v;
+            v.x = __builtin_bit_cast(unsigned int, lo);
+            v.y = __builtin_bit_cast(unsigned int, hi);
+            *(uint2*)(H + (c * 16 + r) * kLd + u) = v;
+        }
     }
 }
 
@@ -411,19 +433,21 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         __syncthreads();
         refresh_hidden<kHid>(ha, kLd, net.w[3], net.b[3], hb);
         __syncthreads();
-        {   // output layer: wavefront w takes rows [16 w, 16 w + 16), all 16 (padded) outputs
+        {   // output layer, transposed: wavefront w takes cells [16 w, 16 w + 16); a lane
+            // holds outputs 4 q .. 4 q + 3 of cell 16 w + r (outputs 0 .. 4 are real)
             const int lane = tid & 63, wave = tid >> 6;
             const int r = lane & 15, q = lane >> 4;
             f4 acc = f4{ 0.f, 0.f, 0.f, 0.f };
-            for (int k = 0; k < kHid; k += 16) {
-                const s4 af = *(const s4*)(hb + (wave * 16 + r) * kLd + k + 4 * q);
-                const s4 bfr = *(const s4*)(net.w[4] + (size_t)r * kHid + k + 4 * q);
-                acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc, 0, 0, 0);
-            }
-            if (r < kOut) {
 #pragma unroll
-                for (int j = 0; j < 4; j++) ys[(sub * kSub + wave * 16 + 4 * q + j) * 8 + r] = acc[j] + net.b[4][r];
+            for (int k = 0; k < kHid; k += 32) {
+                const bf8 wa = *(const bf8*)(net.w[4] + (size_t)r * kHid + k + 8 * q);
+                const bf8 hv = *(const bf8*)(hb + (wave * 16 + r) * kLd + k + 8 * q);
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hv, acc, 0, 0, 0);
             }
+            float* yrow = ys + (sub * kSub + wave * 16 + r) * 8;
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (4 * q + j < kOut) yrow[4 * q + j] = acc[j] + net.b[4][4 * q + j];
         }
         __syncthreads();
     }

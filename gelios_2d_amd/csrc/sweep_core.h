@@ -246,7 +246,9 @@ GEL_HD T hllc2d_full(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T rad, T
 }
 
 // Cell::Get_Sourse_MK1, own-cell moments (Cell.cpp:590-664).  f(k) returns field k.
-template <class F>
+// LOG32 (the FP32-bulk sweep): the logarithm of the cross section in FP32 (its
+// relative error, a few 1e-8, enters sigma as an absolute error b * 1e-7).
+template <class F, bool LOG32 = false>
 GEL_HD void source_mk1(const gel_params& prm, F f, double u, double v, double ro, double p,
                        double& q1, double& q2, double& q3)
 {
@@ -263,7 +265,10 @@ GEL_HD void source_mk1(const gel_params& prm, F f, double u, double v, double ro
     for (int k = 0; k < 4; k++) {
         UM[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (64.0 / (9.0 * kPi)) * (p / ro + 2.0 * pH[k] / roH[k]));
         UU[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (4.0 / kPi) * (p / ro + 2.0 * pH[k] / roH[k]));
-        const double sig = sq(prm.sig_a - prm.sig_b * log(UM[k]));
+        double lg;
+        if constexpr (LOG32) lg = (double)__logf((float)UM[k]);
+        else lg = log(UM[k]);
+        const double sig = sq(prm.sig_a - prm.sig_b * lg);
         nu[k] = ro * roH[k] * UM[k] * sig;
     }
     double Mu = nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u);
