@@ -906,6 +906,25 @@ __global__ void hlld_batch_kernel(int n, const double* __restrict__ L, const dou
     uu[k] = hlld<double>(L + 9 * (size_t)k, R + 9 * (size_t)k, W[k], n1[k], n2[k], gam, P + 9 * (size_t)k);
 }
 
+// Batched exact Riemann solver for tests (gel_flux_godunov_batch): out [n][10] =
+// q0..q4, dsl, dsp, dsc, dt, status.
+__global__ void godunov_batch_kernel(int n, const double* __restrict__ L, const double* __restrict__ R,
+                                     const double* __restrict__ nrm, const double* __restrict__ W,
+                                     const double* __restrict__ dist, double cfl, double gam,
+                                     double* __restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double q[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, dsl = 0.0, dsp = 0.0, dsc = 0.0;
+    int st = 0;
+    const double* nk = nrm + 3 * (size_t)k;
+    const double t = godunov_exact(L + 5 * (size_t)k, R + 5 * (size_t)k, nk[0], nk[1], nk[2], W[k], dist[k], cfl, gam,
+                                   q, dsl, dsp, dsc, st);
+    double* o = out + 10 * (size_t)k;
+    for (int i = 0; i < 5; i++) o[i] = q[i];
+    o[5] = dsl; o[6] = dsp; o[7] = dsc; o[8] = t; o[9] = st;
+}
+
 }  // namespace gel
 
 // ------------------------------------------------------------------ C-ABI ------
@@ -1448,6 +1467,30 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
         const double* q = &o[(size_t)9 * k];
         P[4 * k] = q[0]; P[4 * k + 1] = q[1]; P[4 * k + 2] = q[2]; P[4 * k + 3] = q[3];
         PQ[k] = q[4]; Vl[k] = q[5]; Vc[k] = q[6]; Vp[k] = q[7]; dt[k] = q[8];
+    }
+    return GEL_OK;
+}
+
+int gel_flux_godunov_batch(int n, const double* L, const double* R, const double* nrm, const double* W,
+                           const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                           int* status)
+{
+    if (n < 0 || (n > 0 && (!L || !R || !nrm || !W || !dist || !q || !speeds || !dt || !status)))
+        return fail(GEL_E_ARG, "null argument");
+    if (n == 0) return GEL_OK;
+    std::vector<double> out((size_t)n * 10);
+    const int st = batch_run(n, { { L, 5 }, { R, 5 }, { nrm, 3 }, { W, 1 }, { dist, 1 } }, { { out.data(), 10 } },
+                             [&](std::vector<double*>& i, std::vector<double*>& d) {
+                                 hipLaunchKernelGGL(godunov_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n,
+                                                    i[0], i[1], i[2], i[3], i[4], cfl, gamma, d[0]);
+                             });
+    if (st != GEL_OK) return st;
+    for (int k = 0; k < n; k++) {
+        const double* o = &out[(size_t)k * 10];
+        for (int i = 0; i < 5; i++) q[(size_t)k * 5 + i] = o[i];
+        for (int i = 0; i < 3; i++) speeds[(size_t)k * 3 + i] = o[5 + i];
+        dt[k] = o[8];
+        status[k] = (int)o[9];
     }
     return GEL_OK;
 }

@@ -497,3 +497,247 @@ GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T ig1, T SN, boo
 }
 
 }  // namespace gel
+
+namespace gel {
+
+// ---- exact Riemann solver for a gamma-gas with a moving face ---------------------
+// Solvers::Godunov_Solver_Alexashov (Solvers.cpp:693-1172) with its pressure solvers
+// newton (:1174-1229), devtwo (:1231-1305) and lev (:1307-1321).  States are
+// (ro, p, u, v, w) in Cartesian components; n = (n1, n2, n3) with the tangents
+// t1 = (n2, -n1, 0), t2 = (0, 0, 1) of the reference.  The pattern (vacuum, two
+// rarefactions, shock + rarefaction, two shocks) follows from the velocity jump
+// against the thresholds Uvk < Urz < Uud after ordering the sides so that pI <= pII;
+// the star pressure is closed-form, bisection or Newton accordingly; the solution is
+// sampled at xi = w and the flux is taken relative to the face.  Returns
+// cfl * dist / max|wave speed| (the reference's local time step); status 1 when an
+// iteration does not converge or meets a non-positive pressure (the reference exits).
+struct GodunovK {
+    double ga, g1, g2, gp, gm;
+};
+GEL_HD GodunovK godunov_k(double ga)
+{
+    GodunovK k;
+    k.ga = ga;
+    k.g1 = ga - 1.0;
+    k.g2 = ga + 1.0;
+    k.gp = (k.g2 / ga) / 2.0;
+    k.gm = (k.g1 / ga) / 2.0;
+    return k;
+}
+
+// lev: f(P) = fI(P) + fII(P) - (enI - enII), shock branch on side I, rarefaction on II
+GEL_HD double godunov_lev(const GodunovK& K, double enI, double pI, double rI, double enII, double pII, double rII,
+                          double P)
+{
+    const double cI = sqrt(K.ga * pI / rI);
+    const double cII = sqrt(K.ga * pII / rII);
+    const double fI = (P - pI) / (rI * cI * sqrt(K.gp * P / pI + K.gm));
+    const double fII = 2.0 / K.g1 * cII * (pow((P / pII), K.gm) - 1.0);
+    return (fI + fII) - (enI - enII);
+}
+
+// devtwo: bisection of lev on [pI, pII]
+GEL_HD int godunov_bisect(const GodunovK& K, double enI, double pI, double rI, double enII, double pII, double rII,
+                          double& p)
+{
+    const double tol = 10e-10;
+    double lo = pI, hi = pII;
+    double flo = godunov_lev(K, enI, pI, rI, enII, pII, rII, lo);
+    const double fhi = godunov_lev(K, enI, pI, rI, enII, pII, rII, hi);
+    if (fabs(flo) <= tol) { p = lo; return 0; }
+    if (fabs(fhi) <= tol) { p = hi; return 0; }
+    for (int it = 1; it < 1100; it++) {
+        const double mid = (lo + hi) / 2.0;
+        const double fm = godunov_lev(K, enI, pI, rI, enII, pII, rII, mid);
+        if (fabs(fm) <= tol) { p = mid; return 0; }
+        if ((flo * fm) <= 0.0) {
+            hi = mid;
+        } else {
+            lo = mid;
+            flo = fm;
+        }
+    }
+    p = 0.0;
+    return 1;
+}
+
+// newton: two shocks, from the acoustic estimate
+GEL_HD int godunov_newton(const GodunovK& K, double enI, double pI, double rI, double enII, double pII, double rII,
+                          double& p)
+{
+    const double cI = sqrt(K.ga * pI / rI);
+    const double cII = sqrt(K.ga * pII / rII);
+    double pn = pI * rII * cII + pII * rI * cI + (enI - enII) * rI * cI * rII * cII;
+    pn = pn / (rI * cI + rII * cII);
+    const double p0 = pn;
+    for (int it = 1; it < 1100; it++) {
+        p = pn;
+        if (p <= 0.0) return 1;
+        double fI = (p - pI) / (rI * cI * sqrt(K.gp * p / pI + K.gm));
+        double dfI = (K.ga + 1.0) * p / pI + (3.0 * K.ga - 1.0);
+        dfI = dfI / (4.0 * K.ga * rI * cI * pow((K.gp * p / pI + K.gm), (3.0 / 2.0)));
+        double fII = (p - pII) / (rII * cII * sqrt(K.gp * p / pII + K.gm));
+        double dfII = (K.ga + 1.0) * p / pII + (3.0 * K.ga - 1.0);
+        dfII = dfII / (4.0 * K.ga * rII * cII * pow((K.gp * p / pII + K.gm), (3.0 / 2.0)));
+        pn = p - (fI + fII - (enI - enII)) / (dfI + dfII);
+        if (!(fabs(pn / p0 - p / p0) >= 10e-10)) {
+            p = pn;
+            return 0;
+        }
+    }
+    return 1;
+}
+
+// q1, q2: (ro, p, u, v, w); q out: mass, momentum x/y/z, energy fluxes through the face
+GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, double n2, double n3, double w,
+                            double dist, double cfl, double gam, double* q, double& dsl, double& dsp, double& dsc,
+                            int& status)
+{
+    const GodunovK K = godunov_k(gam);
+    const double ga = K.ga, g1 = K.g1, g2 = K.g2, gm = K.gm;
+    const double al = n1, be = n2, ge = n3;
+    const double al2 = be, be2 = -al, ge2 = 0.0;
+    const double al3 = 0.0, be3 = 0.0, ge3 = 1.0;
+    double enI = al * q1[2] + be * q1[3] + ge * q1[4];
+    double teI2 = al2 * q1[2] + be2 * q1[3] + ge2 * q1[4];
+    double teI3 = al3 * q1[2] + be3 * q1[3] + ge3 * q1[4];
+    double enII = al * q2[2] + be * q2[3] + ge * q2[4];
+    double teII2 = al2 * q2[2] + be2 * q2[3] + ge2 * q2[4];
+    double teII3 = al3 * q2[2] + be3 * q2[3] + ge3 * q2[4];
+    double pI = q1[1], pII = q2[1], rI = q1[0], rII = q2[0];
+    status = 0;
+    double time = 0.0;
+    const bool swapped = pI > pII;      // side I becomes the low-pressure side, mirrored
+    if (swapped) {
+        const double e1 = enI, t12 = teI2, t13 = teI3, p1 = pI, r1 = rI;
+        enI = -enII; teI2 = teII2; teI3 = teII3; pI = pII; rI = rII;
+        enII = -e1; teII2 = t12; teII3 = t13; pII = p1; rII = r1;
+        w = -w;
+    }
+    const double cI = (pI != 0.0) ? sqrt(ga * pI / rI) : 0.0;
+    const double cII = (pII != 0.0) ? sqrt(ga * pII / rII) : 0.0;
+    const double a = sqrt(rI * (g2 * pII + g1 * pI) / 2.0);
+    const double Uud = (pII - pI) / a;
+    const double Urz = -2.0 * cII / g1 * (1.0 - pow((pI / pII), gm));
+    const double Uvk = -2.0 * (cII + cI) / g1;
+    const double Udf = enI - enII;
+
+    double p = 0.0, r = 0.0, te2 = 0.0, te3 = 0.0, en = 0.0;
+    // star-region density behind a shock (side I, left; side II, right)
+    auto shock_rho_I = [&](double aI, double u) { return rI * aI / (aI - rI * (enI - u)); };
+    auto shock_rho_II = [&](double aII, double u) { return rII * aII / (aII + rII * (enII - u)); };
+    // mass flux across a rarefaction with star pressure p (|p - p_side| < eps: acoustic limit)
+    auto fan_a = [&](double rs, double cs, double ps, double P) {
+        return (fabs(P - ps) < 10e-10) ? rs * cs : gm * rs * cs * (1.0 - P / ps) / (1.0 - pow((P / ps), gm));
+    };
+    auto time_of = [&](double UU) { return (UU > 10e-8) ? cfl * dist / UU : cfl * dist / 10e-8; };
+    auto state_I = [&]() { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; };
+    auto state_II = [&]() { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; };
+
+    if (Udf < Uvk) {                                        // vacuum between two fans
+        const double gg1 = ga - 1.0, gg2 = ga + 1.0;
+        const double dI = enI - cI, ddI = enI + 2.0 / gg1 * cI;
+        const double dII = enII + cII, ddII = enII - 2.0 / gg1 * cII;
+        dsl = dI; dsp = dII; dsc = (dI + dII) / 2.0;
+        double UU = mx(fabs(dsl), fabs(dsp));
+        UU = mx(UU, fabs(ddII));
+        UU = mx(UU, fabs(ddI));
+        time = time_of(UU);
+        if (w <= dI) state_I();
+        if ((w > dI) && (w < ddI)) {
+            const double ce = gg1 / gg2 * (enI - w) + 2.0 / gg2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / gm)); r = ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w >= ddI) && (w <= ddII)) { en = w; p = 0.0; r = 0.0; te2 = 0.0; te3 = 0.0; }
+        if ((w > ddII) && (w < dII)) {
+            const double ce = -gg1 / gg2 * (enII - w) + 2.0 / gg2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / gm)); r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) state_II();
+    } else if ((Udf >= Uvk) && (Udf <= Urz)) {              // two rarefactions
+        p = pI * pow(((Udf - Uvk) / (Urz - Uvk)), (1.0 / gm));
+        const double aI = fan_a(rI, cI, pI, p), aII = fan_a(rII, cII, pII, p);
+        const double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        const double dI = enI - cI, ddI = u - cI - g1 * (enI - u) / 2.0;
+        const double dII = enII + cII, ddII = u + cII - g1 * (enII - u) / 2.0;
+        dsl = dI; dsp = dII; dsc = u;
+        double UU = mx(fabs(dsl), fabs(dsp));
+        UU = mx(UU, fabs(ddII));
+        UU = mx(UU, fabs(ddI));
+        time = time_of(UU);
+        if (w <= dI) state_I();
+        if ((w > dI) && (w < ddI)) {
+            const double ce = g1 / g2 * (enI - w) + 2.0 / g2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / gm)); r = ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w >= ddI) && (w <= u)) {
+            const double ce = cI + g1 / 2.0 * (enI - u);
+            en = u; r = ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w > u) && (w <= ddII)) {
+            const double ce = cII - g1 / 2.0 * (enII - u);
+            en = u; r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if ((w > ddII) && (w < dII)) {
+            const double ce = -g1 / g2 * (enII - w) + 2.0 / g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / gm)); r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) state_II();
+    } else if ((Udf > Urz) && (Udf <= Uud)) {               // shock into side I, fan into side II
+        status = godunov_bisect(K, enI, pI, rI, enII, pII, rII, p);
+        if (status) return 0.0;
+        const double aI = sqrt(rI * (g2 / 2.0 * p + g1 / 2.0 * pI));
+        const double aII = (fabs(p - pII) < 10e-10) ? rII * cII
+                                                     : gm * rII * cII * (1.0 - p / pII) / (1.0 - pow((p / pII), gm));
+        const double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        const double dI = enI - aI / rI, dII = enII + cII;
+        const double ddII = u + cII - g1 * (enII - u) / 2.0;
+        dsl = dI; dsp = dII; dsc = u;
+        double UU = mx(fabs(dsl), fabs(dsp));
+        UU = mx(UU, fabs(ddII));
+        time = time_of(UU);
+        if (w <= dI) state_I();
+        if ((w > dI) && (w <= u)) { en = u; r = shock_rho_I(aI, u); te2 = teI2; te3 = teI3; }
+        if ((w > u) && (w <= ddII)) {
+            const double ce = cII - g1 / 2.0 * (enII - u);
+            en = u; r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if ((w > ddII) && (w < dII)) {
+            const double ce = -g1 / g2 * (enII - w) + 2.0 / g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / gm)); r = ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) state_II();
+    } else if (Udf > Uud) {                                 // two shocks
+        status = godunov_newton(K, enI, pI, rI, enII, pII, rII, p);
+        if (status) return 0.0;
+        const double aI = sqrt(rI * (g2 / 2.0 * p + g1 / 2.0 * pI));
+        const double aII = sqrt(rII * (g2 / 2.0 * p + g1 / 2.0 * pII));
+        const double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        const double dI = enI - aI / rI, dII = enII + aII / rII;
+        dsl = dI; dsp = dII; dsc = u;
+        time = time_of(mx(fabs(dsl), fabs(dsp)));
+        if (w <= dI) state_I();
+        else if ((w > dI) && (w <= u)) { en = u; r = shock_rho_I(aI, u); te2 = teI2; te3 = teI3; }
+        else if ((w > u) && (w < dII)) { en = u; r = shock_rho_II(aII, u); te2 = teII2; te3 = teII3; }
+        else if (w >= dII) state_II();
+    }
+    if (swapped) {
+        en = -en;
+        const double l = dsl, rr = dsp;
+        dsl = -rr; dsp = -l; dsc = -dsc;
+        w = -w;
+    }
+    const double uo = al * en + al2 * te2 + al3 * te3;
+    const double vo = be * en + be2 * te2 + be3 * te3;
+    const double wo = ge * en + ge2 * te2 + ge3 * te3;
+    const double eo = p / g1 + 0.5 * r * (uo * uo + vo * vo + wo * wo);
+    en = al * uo + be * vo + ge * wo;
+    q[0] = (r * (en - w));
+    q[1] = (r * (en - w) * uo + al * p);
+    q[2] = (r * (en - w) * vo + be * p);
+    q[3] = (r * (en - w) * wo + ge * p);
+    q[4] = ((en - w) * eo + en * p);
+    return time;
+}
+
+}  // namespace gel

@@ -226,6 +226,19 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
                         const double* n2, double gamma, double* P, double* smax);
 
+/* Batched device evaluation of the exact Riemann solver for a gamma-gas with a moving
+ * face, Solvers::Godunov_Solver_Alexashov with newton / devtwo / lev (Solvers.h:23-25,
+ * Solvers.cpp:693-1321; the reference applies it to fitted-surface faces,
+ * Setka.cpp:10335-10370).  L, R: [n][5] = (ro, p, u, v, w); nrm: [n][3] unit normals
+ * (tangents (n2, -n1, 0) and (0, 0, 1), as the reference); W, dist: [n].  Out: q [n][5]
+ * = mass, momentum x/y/z and energy fluxes relative to the face; speeds [n][3] = (dsl,
+ * dsp, dsc); dt [n] = cfl * dist / max|wave speed|; status [n] = 0, or 1 where the
+ * reference's pressure iteration would exit (no convergence in 1100 steps, or a
+ * non-positive pressure), q then zero. */
+int gel_flux_godunov_batch(int n, const double* L, const double* R, const double* nrm, const double* W,
+                           const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                           int* status);
+
 /* ---- batched cases (no counterpart in the reference, which runs one grid) ----
  * Many independent contexts (static grids, same physics / precision / flux / source
  * mode) advanced together: one persistent launch per step walks the (case, strip,

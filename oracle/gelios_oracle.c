@@ -894,3 +894,227 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
     if (n_p_clamp) *n_p_clamp += npc;
     return bad_any ? GEL_E_NUMERIC : GEL_OK;
 }
+
+/* ---- exact Riemann solver (Solvers::Godunov_Solver_Alexashov, Solvers.cpp:693-1172,
+ * with newton :1174-1229, devtwo :1231-1305, lev :1307-1321) -------------------------
+ * Constants as the reference's macros (Help.h:39-41, Solvers.cpp:7-12) for gamma = gam;
+ * eps = 10e-10, eps8 = 10e-8 (Solvers.h:7-8).  Returns the local time step; *status
+ * = 1 where the reference would exit. */
+typedef struct { double ga, g1, g2, gp, gm; } ora_gk;
+
+static double ora_gd_lev(const ora_gk* k, double enI, double pI, double rI, double enII, double pII, double rII,
+                         double uuu)
+{   /* Solvers.cpp:1307-1321 */
+    double cI = sqrt(k->ga * pI / rI);
+    double cII = sqrt(k->ga * pII / rII);
+    double fI = (uuu - pI) / (rI * cI * sqrt(k->gp * uuu / pI + k->gm));
+    double fII = 2.0 / k->g1 * cII * (pow((uuu / pII), k->gm) - 1.0);
+    double f1 = fI + fII;
+    double f2 = enI - enII;
+    return f1 - f2;
+}
+
+static int ora_gd_devtwo(const ora_gk* k, double enI, double pI, double rI, double enII, double pII, double rII,
+                         double* p)
+{   /* Solvers.cpp:1231-1305: bisection between pI and pII */
+    const double epsil = 10e-10;
+    double kl = pI, kp = pII, ksi, ksir, ksit, kc;
+    int it;
+    ksi = ora_gd_lev(k, enI, pI, rI, enII, pII, rII, kl);
+    ksir = ora_gd_lev(k, enI, pI, rI, enII, pII, rII, kp);
+    if (fabs(ksi) <= epsil) { *p = kl; return 0; }
+    if (fabs(ksir) <= epsil) { *p = kp; return 0; }
+    for (it = 1; it < 1100; it++) {
+        kc = (kl + kp) / 2.0;
+        ksit = ora_gd_lev(k, enI, pI, rI, enII, pII, rII, kc);
+        if (fabs(ksit) <= epsil) { *p = kc; return 0; }
+        if ((ksi * ksit) <= 0.0) { kp = kc; ksir = ksit; }
+        else { kl = kc; ksi = ksit; }
+    }
+    (void)ksir;
+    return 1;
+}
+
+static int ora_gd_newton(const ora_gk* k, double enI, double pI, double rI, double enII, double pII, double rII,
+                         double* pout)
+{   /* Solvers.cpp:1174-1229 */
+    double cI = sqrt(k->ga * pI / rI);
+    double cII = sqrt(k->ga * pII / rII);
+    double pn = pI * rII * cII + pII * rI * cI + (enI - enII) * rI * cI * rII * cII;
+    double pee, p, fI, fIs, fII, fIIs;
+    int kiter = 0;
+    pn = pn / (rI * cI + rII * cII);
+    pee = pn;
+    for (;;) {
+        p = pn;
+        if (p <= 0.0) return 1;
+        kiter = kiter + 1;
+        fI = (p - pI) / (rI * cI * sqrt(k->gp * p / pI + k->gm));
+        fIs = (k->ga + 1.0) * p / pI + (3.0 * k->ga - 1.0);
+        fIs = fIs / (4.0 * k->ga * rI * cI * pow((k->gp * p / pI + k->gm), (3.0 / 2.0)));
+        fII = (p - pII) / (rII * cII * sqrt(k->gp * p / pII + k->gm));
+        fIIs = (k->ga + 1.0) * p / pII + (3.0 * k->ga - 1.0);
+        fIIs = fIIs / (4.0 * k->ga * rII * cII * pow((k->gp * p / pII + k->gm), (3.0 / 2.0)));
+        if (kiter == 1100) return 1;
+        pn = p - (fI + fII - (enI - enII)) / (fIs + fIIs);
+        if (!(fabs(pn / pee - p / pee) >= 10e-10)) break;
+    }
+    *pout = pn;
+    return 0;
+}
+
+static double ora_gd_time(double cfl, double dist, double UU)
+{
+    if (UU > 10e-8) return cfl * dist / UU;
+    return cfl * dist / 10e-8;
+}
+
+double ora_godunov(const double q1[5], const double q2[5], const double n[3], double w, double dist, double cfl,
+                   double gam, double qqq[5], double* dsl, double* dsp, double* dsc, int* status)
+{
+    ora_gk K;
+    double al = n[0], be = n[1], ge = n[2];
+    double al2 = be, be2 = -al, ge2 = 0.0, al3 = 0.0, be3 = 0.0, ge3 = 1.0;
+    double enI, teI2, teI3, enII, teII2, teII3, pI, pII, rI, rII, cI, cII, a, Uud, Urz, Uvk, Udf;
+    double time = 0.0, p = 0.0, r = 0.0, te2 = 0.0, te3 = 0.0, en = 0.0, uo, vo, wo, eo;
+    int ipiz = 0, il = -2, ip = -2;
+    K.ga = gam; K.g1 = gam - 1.0; K.g2 = gam + 1.0;
+    K.gp = (K.g2 / gam) / 2.0; K.gm = (K.g1 / gam) / 2.0;
+    *status = 0;
+    *dsl = *dsp = *dsc = 0.0;
+    enI = al * q1[2] + be * q1[3] + ge * q1[4];
+    teI2 = al2 * q1[2] + be2 * q1[3] + ge2 * q1[4];
+    teI3 = al3 * q1[2] + be3 * q1[3] + ge3 * q1[4];
+    enII = al * q2[2] + be * q2[3] + ge * q2[4];
+    teII2 = al2 * q2[2] + be2 * q2[3] + ge2 * q2[4];
+    teII3 = al3 * q2[2] + be3 * q2[3] + ge3 * q2[4];
+    pI = q1[1]; pII = q2[1]; rI = q1[0]; rII = q2[0];
+    if (pI > pII) {                                    /* Solvers.cpp:750-777 */
+        double eno2 = enII, teo22 = teII2, teo23 = teII3, p2 = pII, r2 = rII;
+        double eno1 = enI, teo12 = teI2, teo13 = teI3, p1 = pI, r1 = rI;
+        enI = -eno2; teI2 = teo22; teI3 = teo23; pI = p2; rI = r2;
+        enII = -eno1; teII2 = teo12; teII3 = teo13; pII = p1; rII = r1;
+        w = -w;
+        ipiz = 1;
+    }
+    cI = (pI != 0.0) ? sqrt(K.ga * pI / rI) : 0.0;
+    cII = (pII != 0.0) ? sqrt(K.ga * pII / rII) : 0.0;
+    a = sqrt(rI * (K.g2 * pII + K.g1 * pI) / 2.0);
+    Uud = (pII - pI) / a;
+    Urz = -2.0 * cII / K.g1 * (1.0 - pow((pI / pII), K.gm));
+    Uvk = -2.0 * (cII + cI) / K.g1;
+    Udf = enI - enII;
+    if (Udf < Uvk) { il = -1; ip = -1; }               /* Solvers.cpp:793-824 */
+    else if ((Udf >= Uvk) && (Udf <= Urz)) { p = pI * pow(((Udf - Uvk) / (Urz - Uvk)), (1.0 / K.gm)); il = 0; ip = 0; }
+    else if ((Udf > Urz) && (Udf <= Uud)) {
+        if (ora_gd_devtwo(&K, enI, pI, rI, enII, pII, rII, &p)) { *status = 1; memset(qqq, 0, 5 * sizeof(double)); return 0.0; }
+        il = 1; ip = 0;
+    } else if (Udf > Uud) {
+        if (ora_gd_newton(&K, enI, pI, rI, enII, pII, rII, &p)) { *status = 1; memset(qqq, 0, 5 * sizeof(double)); return 0.0; }
+        il = 1; ip = 1;
+    }
+    if (il == 1 && ip == 1) {                          /* two shocks, :828-883 */
+        double aI = sqrt(rI * (K.g2 / 2.0 * p + K.g1 / 2.0 * pI));
+        double aII = sqrt(rII * (K.g2 / 2.0 * p + K.g1 / 2.0 * pII));
+        double u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        double dI = enI - aI / rI, dII = enII + aII / rII;
+        *dsl = dI; *dsp = dII; *dsc = u;
+        time = ora_gd_time(cfl, dist, mx(fabs(*dsl), fabs(*dsp)));
+        if (w <= dI) { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; }
+        else if ((w > dI) && (w <= u)) { en = u; r = rI * aI / (aI - rI * (enI - u)); te2 = teI2; te3 = teI3; }
+        else if ((w > u) && (w < dII)) { en = u; r = rII * aII / (aII + rII * (enII - u)); te2 = teII2; te3 = teII3; }
+        else if (w >= dII) { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; }
+    }
+    if (il == 1 && ip == 0) {                          /* shock + fan, :886-966 */
+        double aI = sqrt(rI * (K.g2 / 2.0 * p + K.g1 / 2.0 * pI)), aII, u, dI, dII, ddII, UU;
+        if (fabs(p - pII) < 10e-10) aII = rII * cII;
+        else aII = K.gm * rII * cII * (1.0 - p / pII) / (1.0 - pow((p / pII), K.gm));
+        u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        dI = enI - aI / rI; dII = enII + cII; ddII = u + cII - K.g1 * (enII - u) / 2.0;
+        *dsl = dI; *dsp = dII; *dsc = u;
+        UU = mx(fabs(*dsl), fabs(*dsp));
+        UU = mx(UU, fabs(ddII));
+        time = ora_gd_time(cfl, dist, UU);
+        if (w <= dI) { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; }
+        if ((w > dI) && (w <= u)) { en = u; r = rI * aI / (aI - rI * (enI - u)); te2 = teI2; te3 = teI3; }
+        if ((w > u) && (w <= ddII)) {
+            double ce = cII - K.g1 / 2.0 * (enII - u);
+            en = u; r = K.ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -K.g1 / K.g2 * (enII - w) + 2.0 / K.g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / K.gm)); r = K.ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; }
+    }
+    if (il == 0 && ip == 0) {                          /* two fans, :968-1065 */
+        double aI, aII, u, dI, ddI, dII, ddII, UU;
+        if (fabs(p - pI) < 10e-10) aI = rI * cI;
+        else aI = K.gm * rI * cI * (1.0 - p / pI) / (1.0 - pow((p / pI), K.gm));
+        if (fabs(p - pII) < 10e-10) aII = rII * cII;
+        else aII = K.gm * rII * cII * (1.0 - p / pII) / (1.0 - pow((p / pII), K.gm));
+        u = (aI * enI + aII * enII + pI - pII) / (aI + aII);
+        dI = enI - cI; ddI = u - cI - K.g1 * (enI - u) / 2.0;
+        dII = enII + cII; ddII = u + cII - K.g1 * (enII - u) / 2.0;
+        *dsl = dI; *dsp = dII; *dsc = u;
+        UU = mx(fabs(*dsl), fabs(*dsp));
+        UU = mx(UU, fabs(ddII));
+        UU = mx(UU, fabs(ddI));
+        time = ora_gd_time(cfl, dist, UU);
+        if (w <= dI) { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; }
+        if ((w > dI) && (w < ddI)) {
+            double ce = K.g1 / K.g2 * (enI - w) + 2.0 / K.g2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / K.gm)); r = K.ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w >= ddI) && (w <= u)) {
+            double ce = cI + K.g1 / 2.0 * (enI - u);
+            en = u; r = K.ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w > u) && (w <= ddII)) {
+            double ce = cII - K.g1 / 2.0 * (enII - u);
+            en = u; r = K.ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -K.g1 / K.g2 * (enII - w) + 2.0 / K.g2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / K.gm)); r = K.ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; }
+    }
+    if (il == -1 && ip == -1) {                        /* vacuum, :1068-1140 */
+        double gg1 = K.ga - 1.0, gg2 = K.ga + 1.0;
+        double dI = enI - cI, ddI = enI + 2.0 / gg1 * cI, dII = enII + cII, ddII = enII - 2.0 / gg1 * cII, UU;
+        *dsl = dI; *dsp = dII; *dsc = (dI + dII) / 2.0;
+        UU = mx(fabs(*dsl), fabs(*dsp));
+        UU = mx(UU, fabs(ddII));
+        UU = mx(UU, fabs(ddI));
+        time = ora_gd_time(cfl, dist, UU);
+        if (w <= dI) { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; }
+        if ((w > dI) && (w < ddI)) {
+            double ce = gg1 / gg2 * (enI - w) + 2.0 / gg2 * cI;
+            en = w + ce; p = pI * pow((ce / cI), (1.0 / K.gm)); r = K.ga * p / ce / ce; te2 = teI2; te3 = teI3;
+        }
+        if ((w >= ddI) && (w <= ddII)) { en = w; p = 0.0; r = 0.0; te2 = 0.0; te3 = 0.0; }
+        if ((w > ddII) && (w < dII)) {
+            double ce = -gg1 / gg2 * (enII - w) + 2.0 / gg2 * cII;
+            en = w - ce; p = pII * pow((ce / cII), (1.0 / K.gm)); r = K.ga * p / ce / ce; te2 = teII2; te3 = teII3;
+        }
+        if (w >= dII) { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; }
+    }
+    if (ipiz == 1) {                                   /* :1144-1153 */
+        double dsl1 = *dsl, dsp1 = *dsp;
+        en = -en;
+        *dsl = -dsp1; *dsp = -dsl1; *dsc = -*dsc;
+        w = -w;
+    }
+    uo = al * en + al2 * te2 + al3 * te3;              /* :1155-1168 */
+    vo = be * en + be2 * te2 + be3 * te3;
+    wo = ge * en + ge2 * te2 + ge3 * te3;
+    eo = p / K.g1 + 0.5 * r * (uo * uo + vo * vo + wo * wo);
+    en = al * uo + be * vo + ge * wo;
+    qqq[0] = (r * (en - w));
+    qqq[1] = (r * (en - w) * uo + al * p);
+    qqq[2] = (r * (en - w) * vo + be * p);
+    qqq[3] = (r * (en - w) * wo + ge * p);
+    qqq[4] = ((en - w) * eo + en * p);
+    return time;
+}

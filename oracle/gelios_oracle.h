@@ -51,6 +51,13 @@ typedef struct ora_mhd { double ro, p, u, v, w, bx, by, bz, Q; } ora_mhd;
  * Returns max(|SL|,|SR|). */
 double ora_hlld(const ora_mhd* L, const ora_mhd* R, double W, double n1, double n2, double gam, double P[9]);
 
+/* Exact Riemann solver for a gamma-gas with face speed w (Solvers::Godunov_Solver_
+ * Alexashov, Solvers.cpp:693-1321).  q1, q2 = (ro, p, u, v, w); n = unit normal.
+ * qqq = mass, momentum x/y/z, energy fluxes relative to the face; returns the local
+ * time step cfl * dist / max|speed|; *status = 1 where the reference would exit. */
+double ora_godunov(const double q1[5], const double q2[5], const double n[3], double w, double dist, double cfl,
+                   double gam, double qqq[5], double* dsl, double* dsp, double* dsc, int* status);
+
 /* Setka::HLLC_2d_Korolkov_b_s (Setka.cpp:10484-10700). Returns cfl*rad/max|S|. */
 double ora_hllc2d(const ora_gas* L, const ora_gas* R, double W, double n1, double n2,
                   double rad, double cfl, double gam, int metod, int nul_potok,

@@ -77,6 +77,9 @@ def lib():
             f.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, _dp,
                           ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
+        L.ora_godunov.restype = ctypes.c_double
+        L.ora_godunov.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         L.ora_hlld.restype = ctypes.c_double
         L.ora_hlld.argtypes = [ctypes.POINTER(OraMhd), ctypes.POINTER(OraMhd), ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, _dp]
@@ -111,6 +114,20 @@ def hlld(L, R, W, n1, n2, gamma=5.0 / 3.0):
     a, b = OraMhd(*map(float, L)), OraMhd(*map(float, R))
     uu = lib().ora_hlld(ctypes.byref(a), ctypes.byref(b), W, n1, n2, gamma, _d(P))
     return P, uu
+
+
+def godunov(q1, q2, n, w=0.0, dist=1.0, cfl=0.1, gamma=5.0 / 3.0):
+    """Point-wise exact Riemann solver.  q1, q2 = (ro, p, u, v, w); n = (n1, n2, n3).
+    Returns (q[5] fluxes, (dsl, dsp, dsc), time, status)."""
+    a = np.ascontiguousarray(q1, np.float64)
+    b = np.ascontiguousarray(q2, np.float64)
+    nn = np.ascontiguousarray(n, np.float64)
+    q = np.zeros(5)
+    sl, sp, sc = np.zeros(1), np.zeros(1), np.zeros(1)
+    st = ctypes.c_int(0)
+    t = lib().ora_godunov(_d(a), _d(b), _d(nn), float(w), float(dist), float(cfl), float(gamma), _d(q), _d(sl),
+                          _d(sp), _d(sc), ctypes.byref(st))
+    return q, (sl[0], sp[0], sc[0]), t, st.value
 
 
 class OracleSweep:

@@ -143,6 +143,29 @@ static int mode_hlld(const char* in, const char* out)
     return 0;
 }
 
+// ---- point-wise: Solvers::Godunov_Solver_Alexashov (exact, moving face) ------------
+// in : int32 n; float64 [n][15] = roL pL uL vL wL roR pR uR vR wR n1 n2 n3 W DIST
+// out: float64 [n][9] = q0..q4 dsl dsp dsc time
+static int mode_godunov(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t n; rd(f, &n, 1);
+    std::vector<double> a((size_t)n * 15); rd(f, a.data(), a.size()); std::fclose(f);
+    Solvers S;
+    std::vector<double> o((size_t)n * 9);
+    for (int k = 0; k < n; k++) {
+        const double* r = &a[(size_t)k * 15];
+        std::vector<double> q1(r, r + 5), q2(r + 5, r + 10), nn(r + 10, r + 13), q(5);
+        double dsl = 0.0, dsp = 0.0, dsc = 0.0;
+        const double t = S.Godunov_Solver_Alexashov(q1, q2, nn, q, dsl, dsp, dsc, r[13], r[14]);
+        double* z = &o[(size_t)k * 9];
+        for (int i = 0; i < 5; i++) z[i] = q[i];
+        z[5] = dsl; z[6] = dsp; z[7] = dsc; z[8] = t;
+    }
+    f = xopen(out, "wb"); wr(f, o.data(), o.size()); std::fclose(f);
+    return 0;
+}
+
 // ---- whole sweep -----------------------------------------------------------------
 // in : int32 ni nj steps moving has_src
 //      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
@@ -337,12 +360,13 @@ static int mode_sweep(const char* in, const char* out)
 
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "sweep")) return mode_sweep(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "hlld")) return mode_hlld(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "godunov")) return mode_godunov(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

@@ -6,7 +6,8 @@ script feeds the harness seeded inputs and stores inputs and outputs as .npz fil
 The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py godunov    # one family
 """
 from __future__ import annotations
 
@@ -132,6 +133,34 @@ def gen_hlld(rng, tmp, n=1200):
     np.savez_compressed(os.path.join(OUT, "hlld_vectors.npz"), inp=inp, out=res)
 
 
+def gen_godunov(rng, tmp, n=1600):
+    """Solvers::Godunov_Solver_Alexashov (gamma 5/3, kurant 0.1): states drawn so that the
+    four wave patterns (two shocks, shock + fan, two fans, vacuum) all occur, with
+    pI > pII on about half of the faces (the swap of Solvers.cpp:750-777), face speeds
+    W != 0 on half, an out-of-plane velocity on a quarter, in-plane unit normals."""
+    ro = rng.uniform(0.05, 3.0, (2, n))
+    p = rng.uniform(0.01, 3.0, (2, n))
+    c = np.sqrt(5.0 / 3.0 * p / ro)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    nrm = np.stack([np.cos(ang), np.sin(ang), np.zeros(n)], axis=1)
+    # normal velocity jump enL - enR in units of the sound speeds: from strongly
+    # separating (vacuum, < -3 (cL + cR)) to strongly colliding (two shocks)
+    jump = rng.uniform(-5.0, 4.0, n) * (c[0] + c[1])
+    tang = rng.normal(0, 1.0, (2, n))
+    drift = rng.normal(0, 1.0, n)
+    enL, enR = drift + 0.5 * jump, drift - 0.5 * jump
+    vL = np.stack([enL * nrm[:, 0] + tang[0] * nrm[:, 1], enL * nrm[:, 1] - tang[0] * nrm[:, 0]])
+    vR = np.stack([enR * nrm[:, 0] + tang[1] * nrm[:, 1], enR * nrm[:, 1] - tang[1] * nrm[:, 0]])
+    wz = np.where(rng.random((2, n)) < 0.25, rng.normal(0, 1.0, (2, n)), 0.0)
+    W = np.where(rng.random(n) < 0.5, 0.0, rng.normal(0, 1.5, n) * (c[0] + c[1]) / 2)
+    dist = rng.uniform(0.01, 1.0, n)
+    inp = np.concatenate([ro[0][:, None], p[0][:, None], vL.T, wz[0][:, None],
+                          ro[1][:, None], p[1][:, None], vR.T, wz[1][:, None], nrm, W[:, None], dist[:, None]], axis=1)
+    out = harness("godunov", np.int32(n).tobytes() + np.ascontiguousarray(inp).tobytes(), tmp)
+    res = np.frombuffer(out, np.float64).reshape(n, 9)
+    np.savez_compressed(os.path.join(OUT, "godunov_vectors.npz"), inp=inp, out=res)
+
+
 def harness_grid(ni=16, nj=10, n_c1=3):
     g = polar_grid(ni, nj, 4.0, 12.0, spacing="log")
     ct = np.full(ni * nj, GEL_C_2, np.int8)
@@ -223,6 +252,12 @@ def gen_sweeps(rng, tmp):
 
 
 def main():
+    only = sys.argv[1:]
+    if only:
+        with tempfile.TemporaryDirectory() as tmp:
+            if "godunov" in only:
+                gen_godunov(np.random.default_rng(20261205), tmp)
+        return
     rng = np.random.default_rng(20261203)
     with tempfile.TemporaryDirectory() as tmp:
         gen_hllc(rng, tmp)
@@ -230,6 +265,7 @@ def main():
         gen_cell(rng, tmp)
         gen_hlld(np.random.default_rng(20261204), tmp)
         gen_sweeps(rng, tmp)
+        gen_godunov(np.random.default_rng(20261205), tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -143,3 +143,30 @@ def test_large_grid_free_stream():
     assert np.max(np.abs(far[0] - 1.0)) < 1e-12
     assert np.max(np.abs(far[1] - 1.0)) < 1e-12
     assert np.max(np.abs(far[2:4])) < 1e-12
+
+
+def _godunov_dev(L, R, nrm, W, dist, cfl, gam):
+    import ctypes
+    from gelios_2d_amd import _lib
+    n = len(L)
+    q, sp, dt = np.zeros((n, 5)), np.zeros((n, 3)), np.zeros(n)
+    st = np.zeros(n, np.int32)
+    arr = [np.ascontiguousarray(a, np.float64) for a in (L, R, nrm, W, dist)]
+    _lib.check(_lib.lib().gel_flux_godunov_batch(n, *[_lib.dptr(a) for a in arr], cfl, gam, _lib.dptr(q),
+                                                 _lib.dptr(sp), _lib.dptr(dt),
+                                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return q, sp, dt, st
+
+
+def test_godunov_batch_matches_reference_vectors():
+    """Device exact Riemann solver against the compiled reference's vectors (the
+    device's pow / sqrt may differ from glibc's in the last bit, and the Newton and
+    bisection iterations then stop one step apart at most)."""
+    d = golden("godunov_vectors.npz")
+    inp, out = d["inp"], d["out"]
+    q, sp, dt, st = _godunov_dev(inp[:, 0:5], inp[:, 5:10], inp[:, 10:13], inp[:, 13], inp[:, 14], 0.1, 5.0 / 3.0)
+    assert np.all(st == 0)
+    got = np.concatenate([q, sp, dt[:, None]], axis=1)
+    scale = np.maximum(np.abs(out[:, :5]).max(axis=1, keepdims=True), 1e-12)
+    assert np.max(np.abs(got[:, :5] - out[:, :5]) / scale) <= 1e-12
+    assert np.allclose(got[:, 5:], out[:, 5:], rtol=1e-12, atol=1e-12)
