@@ -1,0 +1,978 @@
+// sweep_kernel.h - the CDNA4 (gfx950) Godunov flux-and-update sweep kernels and their
+// launch templates, included by gelios_sweep.hip (FP64 kernels, FP contraction off:
+// the reference's arithmetic) and by sweep_fp32.hip (FP32-bulk kernels, compiled
+// with FP contraction on).
+//
+// Kernel layout.  A work-group of four wavefronts owns a strip of 60 neighbouring
+// rays (columns j) plus 2 halo columns on each side and marches up a chunk of rows
+// four rows at a time (march_tile below); every face is solved once, the four faces
+// of a cell are summed in the reference's face order (Setka.cpp:10372-10468), and
+// the global CFL step is a wave min-reduction plus one integer atomicMin on the bit
+// pattern of a positive double per wavefront (deterministic).
+//
+// Parity with the reference: the per-face arithmetic is that of sweep_core.h
+// (reference operation order); solving a face once instead of once from each side,
+// as the reference's cell gather does (Setka.cpp:10195), changes the flux on the
+// second side only by a sign, exactly.
+#pragma once
+
+#include "sweep_core.h"
+#include "gelios_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+namespace gel {
+
+constexpr int kWave = 64;
+constexpr int kHalo = 2;
+constexpr int kOut = kWave - 2 * kHalo;   // output columns per wavefront
+constexpr int kNW = 4;                    // wavefronts per work-group, marching together
+
+// State vectors.  GAS: (ro, p, u, v, Q), the fields of Parametr (Cell.h:21-25).
+// MHD: (ro, p, u, v, w, bx, by, bz, Q).  odd(k): the primitive changes sign across the
+// symmetry axis (the reference mirrors v, Gran.cpp:225-233; with swirl and field the
+// azimuthal velocity and the radial and azimuthal field change sign as well).
+struct GasPhys {
+    static constexpr int NV = 5;
+    __host__ __device__ static constexpr bool odd(int k) { return k == 3; }
+};
+struct MhdPhys {
+    static constexpr int NV = 9;
+    __host__ __device__ static constexpr bool odd(int k) { return k == 3 || k == 4 || k == 6 || k == 7; }
+};
+
+template <class Ph, class TS>
+struct CellT {          // one cell of the stencil as seen by a lane
+    TS s[Ph::NV];
+    double cx, cy;
+    int type;
+    int ok;             // exists (inside the grid)
+};
+
+struct Nodes {          // nodes (r, j) and (r, j+1) of a lane
+    double xa, ya, xb, yb;
+};
+
+template <class T> __device__ __forceinline__ T shfl_dn(T v, int d) { return __shfl_down(v, d, kWave); }
+template <class T> __device__ __forceinline__ T shfl_up(T v, int d) { return __shfl_up(v, d, kWave); }
+
+template <bool DOWN, class Ph, class TS>
+__device__ __forceinline__ CellT<Ph, TS> cell_shfl(const CellT<Ph, TS>& c, int d)
+{
+    CellT<Ph, TS> r;
+#pragma unroll
+    for (int k = 0; k < Ph::NV; k++) r.s[k] = DOWN ? shfl_dn(c.s[k], d) : shfl_up(c.s[k], d);
+    r.cx = DOWN ? shfl_dn(c.cx, d) : shfl_up(c.cx, d);
+    r.cy = DOWN ? shfl_dn(c.cy, d) : shfl_up(c.cy, d);
+    r.type = DOWN ? shfl_dn(c.type, d) : shfl_up(c.type, d);
+    r.ok = DOWN ? shfl_dn(c.ok, d) : shfl_up(c.ok, d);
+    return r;
+}
+
+template <bool MOVING>
+__device__ __forceinline__ Nodes load_nodes(const SweepArgs& a, const double* __restrict__ X,
+                                            const double* __restrict__ Y, double sc, int r, int j, bool col)
+{
+    Nodes n{1.0, 1.0, 1.0, 1.0};
+    if (col && r >= 0 && r <= a.ni) {
+        const size_t k = (size_t)r * (a.nj + 1) + j;
+        n.xa = X[k]; n.ya = Y[k]; n.xb = X[k + 1]; n.yb = Y[k + 1];
+        if (MOVING) { n.xa *= sc; n.ya *= sc; n.xb *= sc; n.yb *= sc; }
+    }
+    return n;
+}
+
+template <class Ph, class TS, bool TYPES>
+__device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, int j, bool col,
+                                                   const Nodes& lo, const Nodes& hi)
+{
+    CellT<Ph, TS> c;
+    c.ok = (col && r >= 0 && r < a.ni) ? 1 : 0;
+    c.type = GEL_C_2;
+    if (c.ok) {
+        const size_t nc = (size_t)a.ni * a.nj;
+        const size_t k = (size_t)r * a.nj + j;
+        const TS* __restrict__ now = (const TS*)a.now;
+#pragma unroll
+        for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)v * nc + k];
+        if (TYPES) c.type = a.ctype[k];
+        const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
+        const double py[4] = { lo.ya, hi.ya, hi.yb, lo.yb };
+        quad_center(px, py, c.cx, c.cy);
+    } else {
+#pragma unroll
+        for (int v = 0; v < Ph::NV; v++) c.s[v] = TS(0);
+        c.s[0] = TS(1); c.s[1] = TS(1);
+        c.cx = 1.0; c.cy = 1.0;
+    }
+    return c;
+}
+
+__device__ __forceinline__ double dist2d(double ax, double ay, double bx, double by)
+{
+    return sqrt(sq(ax - bx) + sq(ay - by));
+}
+
+// Limited linear value at the face (Help.cpp:96-113 with y = 0, x1 = -d1, x2 = -d2,
+// x3 = d3): t2 + minmod((t1-t2)/(x1-x2), (t2-t3)/(x2-x3)) * (0 - x2), where d1, d2,
+// d3 are the distances of the three cell centres from the face centre.  FP64 keeps
+// the reference's divisions.  FP32 bulk computes the distances in FP32, multiplies by
+// the reciprocals of the two denominators, which all primitives of a face side
+// share, and assembles the face value in FP64 from the exact cell value t2, so that
+// only the small slope correction carries FP32 rounding.
+struct Lin3 {
+    double den12, den23, d2;   // x1 - x2, x2 - x3, 0 - x2
+};
+struct Lin3f {
+    float i12, i23, d2;        // 1 / (x1 - x2), 1 / (x2 - x3), 0 - x2
+};
+template <class TF> using Lin3T = typename std::conditional<sizeof(TF) == 8, Lin3, Lin3f>::type;
+
+__device__ __forceinline__ float dist2f(double ax, double ay, double bx, double by)
+{
+    const float dx = float(ax - bx), dy = float(ay - by);
+    return sr(dx * dx + dy * dy);
+}
+
+// stencil weights at face centre (fx, fy) for centres 1 (far, behind), 2 (owner), 3 (across)
+template <class TF>
+__device__ __forceinline__ Lin3T<TF> lin3_at(double fx, double fy, double x1, double y1, double x2, double y2,
+                                             double x3, double y3)
+{
+    if constexpr (sizeof(TF) == 8) {
+        const double d2 = dist2d(fx, fy, x2, y2);
+        const double d3 = dist2d(fx, fy, x3, y3);
+        const double d1 = dist2d(fx, fy, x1, y1);
+        return Lin3{ -d1 - (-d2), -d2 - d3, 0.0 - (-d2) };
+    } else {
+        const float d2 = dist2f(fx, fy, x2, y2);
+        const float d3 = dist2f(fx, fy, x3, y3);
+        const float d1 = dist2f(fx, fy, x1, y1);
+        return Lin3f{ __builtin_amdgcn_rcpf(d2 - d1), __builtin_amdgcn_rcpf(-d2 - d3), d2 };
+    }
+}
+template <class TF>
+__device__ __forceinline__ double lin3v(double t1, double t2, double t3, const Lin3T<TF>& g)
+{
+    if constexpr (sizeof(TF) == 8) {
+        const double d = minmod<double>((t1 - t2) / g.den12, (t2 - t3) / g.den23);
+        return d * g.d2 + t2;
+    } else {
+        const float d = minmod<float>((float(t1) - float(t2)) * g.i12, (float(t2) - float(t3)) * g.i23);
+        return double(d * g.d2) + t2;
+    }
+}
+
+// Gran::Get_par_TVD, Usualy face (Gran.cpp:294-457): master M's value at face centre
+// (fx,fy) from the line (D, M, N).  D.ok == 0: mirrored ghost across the axis when M
+// touches it (:297-339), else M itself (:340-343).  Distances are FP64 geometry.
+template <class Ph, class TS, class TF>
+__device__ __forceinline__ void tvd_usualy(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
+                                           bool m_axis, double fx, double fy, double* r, int& bad)
+{
+    if (!D.ok) {
+#pragma unroll
+        for (int k = 0; k < Ph::NV; k++) r[k] = double(M.s[k]);
+        if (m_axis) {
+            const Lin3T<TF> g = lin3_at<TF>(fx, fy, M.cx, -M.cy, M.cx, M.cy, N.cx, N.cy);
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++)
+                if (Ph::odd(k)) r[k] = lin3v<TF>(-double(M.s[k]), double(M.s[k]), double(N.s[k]), g);
+        }
+        return;
+    }
+    const Lin3T<TF> g = lin3_at<TF>(fx, fy, D.cx, D.cy, M.cx, M.cy, N.cx, N.cy);
+#pragma unroll
+    for (int k = 0; k < Ph::NV; k++)
+        r[k] = lin3v<TF>(double(D.s[k]), double(M.s[k]), double(N.s[k]), g);
+    if (r[0] <= 0.0 || r[1] <= 0.0) bad = 1;
+}
+
+// Gran::Get_par_TVD_radial, density and pressure (Gran.cpp:562-675); FP64 gas only.
+template <class Ph>
+__device__ __forceinline__ void tvd_radial(const CellT<Ph, double>& M, const CellT<Ph, double>& N,
+                                           const CellT<Ph, double>& D, bool m_axis, double fx, double fy,
+                                           double gam, double& ro, double& pr)
+{
+    double xd, yd, dro, dp;
+    if (!D.ok) {
+        if (!m_axis) { ro = M.s[0]; pr = M.s[1]; return; }
+        xd = M.cx; yd = -M.cy; dro = M.s[0]; dp = M.s[1];
+    } else {
+        xd = D.cx; yd = D.cy; dro = D.s[0]; dp = D.s[1];
+    }
+    const double g2 = 2.0 * gam;
+    const double r = sqrt(sq(fx) + sq(fy));
+    const double r2 = sqrt(sq(M.cx) + sq(M.cy));
+    const double d2 = dist2d(fx, fy, M.cx, M.cy);
+    const double r3 = sqrt(sq(N.cx) + sq(N.cy));
+    const double d3 = dist2d(fx, fy, N.cx, N.cy);
+    const double r1 = sqrt(sq(xd) + sq(yd));
+    const double d1 = dist2d(fx, fy, xd, yd);
+    ro = linear3(-d1, dro * sq(r1), -d2, M.s[0] * sq(r2), d3, N.s[0] * sq(r3), 0.0) / sq(r);
+    pr = linear3(-d1, dp * pow(r1, g2), -d2, M.s[1] * pow(r2, g2), d3, N.s[1] * pow(r3, g2), 0.0) / pow(r, g2);
+}
+
+// Radial rescaling of a cell value to the face radius (Setka.cpp:10221-10226):
+// ro and the tracer by (rc/dis)^2, p by (rc/dis)^(2 gamma); C_1 cells also rotate u,v.
+template <int NV>
+__device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, double gam,
+                                           double cx, double cy, double fx, double fy, bool rotate)
+{
+    g[0] = g[0] * sq(rc) / sq(dis);
+    g[NV - 1] = g[NV - 1] * sq(rc) / sq(dis);
+    g[1] = g[1] * pow(rc / dis, 2 * gam);
+    if (rotate) polar_perenos(cx, cy, fx, fy, g[2], g[3]);
+}
+
+// Value of cell M on its side of an interior face (Setka.cpp:10213-10251).
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N, const CellT<Ph, TS>& D,
+                                           bool m_axis, double fx, double fy, double dis, double gam, double* r, int& bad)
+{
+    if constexpr (TYPES) {
+        if (M.type == GEL_C_1) {
+            double g[Ph::NV];
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++) g[k] = M.s[k];
+            const double rc = sqrt(sq(M.cx) + sq(M.cy));
+            rescale_c1<Ph::NV>(g, rc, dis, gam, M.cx, M.cy, fx, fy, true);
+            double ro, pr;
+            tvd_radial<Ph>(M, N, D, m_axis, fx, fy, gam, ro, pr);
+            g[0] = ro; g[1] = pr;
+#pragma unroll
+            for (int k = 0; k < Ph::NV; k++) r[k] = g[k];
+            return;
+        }
+    }
+    tvd_usualy<Ph, TS, TF>(M, N, D, m_axis, fx, fy, r, bad);
+}
+
+struct Face {          // geometry of a face as seen from its owning cell
+    double n1, n2, S, fx, fy, W, dis;
+};
+
+// CFL time of a cell through a face (Setka.cpp:10388-10393): cfl * |centre - face
+// centre| / max|S|.  FP32 bulk forms it in FP32.
+template <class TF>
+__device__ __forceinline__ double cfl_time(double cfl, double cx, double cy, const Face& f, TF uu)
+{
+    if constexpr (sizeof(TF) == 8) return cfl * sqrt(sq(cx - f.fx) + sq(cy - f.fy)) / uu;
+    else return double(float(cfl) * dist2f(cx, cy, f.fx, f.fy) * __builtin_amdgcn_rcpf(uu));
+}
+
+template <bool MOVING, class TF>
+__device__ __forceinline__ Face face_geom(double ax, double ay, double bx, double by,
+                                          double ax2, double ay2, double bx2, double by2, double dt)
+{
+    Face f;
+    f.S = sqrt(sq(bx - ax) + sq(by - ay)) * (ay + by) * 0.5;           // Setka.cpp:10198
+    f.fx = (ax + bx) / 2.0;
+    f.fy = (ay + by) / 2.0;
+    double t1 = bx - ax, t2 = by - ay;                                    // Gran.cpp:171-181
+    const double nn = sqrt(sq(t1) + sq(t2));
+    if constexpr (sizeof(TF) == 8) {
+        t1 = t1 / nn; t2 = t2 / nn;
+    } else {                            // FP32 bulk: one FP64 reciprocal
+        const double inn = 1.0 / nn;
+        t1 = t1 * inn; t2 = t2 * inn;
+    }
+    f.n1 = t2; f.n2 = -t1;
+    if (MOVING) {
+        const double fx2 = (ax2 + bx2) / 2.0, fy2 = (ay2 + by2) / 2.0;
+        f.W = ((fx2 - f.fx) * f.n1 + (fy2 - f.fy) * f.n2) / dt;          // Setka.cpp:10209
+    } else {
+        f.W = 0.0;
+    }
+    f.dis = sqrt(sq(f.fx) + sq(f.fy));
+    return f;
+}
+
+// Face flux from left/right face primitives (FP64); P in the order of the cell
+// accumulators.  TF = double: the reference arithmetic.  TF = float (FP32 bulk): the
+// wave speeds are estimated in FP32; a supersonic face (the whole solar wind) takes
+// the exact FP64 upwind flux, a subsonic one the FP32 Riemann solver.  Rounding the
+// flux of a steady supersonic stream to FP32 would bias every step the same way.
+template <class Ph, class TF, int FLUX>
+__device__ __forceinline__ TF face_flux(const double* L, const double* R, const Face& f, double gam, double* P)
+{
+    if constexpr (Ph::NV == 5) {
+        const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
+        const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
+        if constexpr (sizeof(TF) == 8) {
+            double PQ;
+            const double UU = hllc2d<double, FLUX == GEL_FLUX_HLLC>(gl, gr, f.W, f.n1, f.n2, gam, P, PQ);
+            P[4] = PQ;
+            return UU;
+        } else {
+            const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
+            const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
+            const float cL = sr(dv(g * float(L[1]), float(L[0]))), cR = sr(dv(g * float(R[1]), float(R[0])));
+            const float SL = mn(u1, u2) - mx(cL, cR), SR = mx(u1, u2) + mx(cL, cR);
+            if (SL >= W || SR <= W) {
+                gas_upwind<double>(SL >= W ? gl : gr, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), P);
+                return mx(fabsf(SL), fabsf(SR));
+            }
+            const Gas<float> fl{ float(L[0]), float(L[1]), float(L[2]), float(L[3]), float(L[4]) };
+            const Gas<float> fr{ float(R[0]), float(R[1]), float(R[2]), float(R[3]), float(R[4]) };
+            float Pf[4], PQ;
+            const float UU = hllc2d<float, FLUX == GEL_FLUX_HLLC>(fl, fr, W, n1, n2, g, Pf, PQ);
+#pragma unroll
+            for (int k = 0; k < 4; k++) P[k] = Pf[k];
+            P[4] = PQ;
+            return UU;
+        }
+    } else {
+        if constexpr (sizeof(TF) == 8) {
+            return hlld<double>(L, R, f.W, f.n1, f.n2, gam, P);
+        } else {
+            const float n1 = float(f.n1), n2 = float(f.n2), W = float(f.W), g = float(gam);
+            const float ispi4 = 0.28209479177387814f, i4pi = 0.07957747154594767f;   // 1/sqrt(4 pi), 1/(4 pi)
+            const float bnL = (float(L[5]) * n1 + float(L[6]) * n2) * ispi4;
+            const float bnR = (float(R[5]) * n1 + float(R[6]) * n2) * ispi4;
+            const float bbL = (sq(float(L[5])) + sq(float(L[6])) + sq(float(L[7]))) * i4pi;
+            const float bbR = (sq(float(R[5])) + sq(float(R[6])) + sq(float(R[7]))) * i4pi;
+            const float gpL = g * float(L[1]), gpR = g * float(R[1]);
+            const float cfL = sr(dv(gpL + bbL + sr(fmaxf(sq(gpL + bbL) - 4.0f * gpL * sq(bnL), 0.0f)), 2.0f * float(L[0])));
+            const float cfR = sr(dv(gpR + bbR + sr(fmaxf(sq(gpR + bbR) - 4.0f * gpR * sq(bnR), 0.0f)), 2.0f * float(R[0])));
+            const float u1 = float(L[2]) * n1 + float(L[3]) * n2, u2 = float(R[2]) * n1 + float(R[3]) * n2;
+            const float SL = mn(u1, u2) - mx(cfL, cfR), SR = mx(u1, u2) + mx(cfL, cfR);
+            const float SN = mx(fabsf(SL), fabsf(SR));
+            if (SL >= W || SR <= W) {
+                const bool left = SL >= W;
+                mhd_upwind<double>(left ? L : R, left ? R : L, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), double(SN), left, P);
+                return SN;
+            }
+            float fl[9], fr[9], Pf[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) { fl[k] = float(L[k]); fr[k] = float(R[k]); }
+            const float UU = hlld<float>(fl, fr, W, n1, n2, g, Pf);
+#pragma unroll
+            for (int k = 0; k < 9; k++) P[k] = Pf[k];
+            return UU;
+        }
+    }
+}
+
+// Boundary face of cell M: Gran::Get_par ghost (Gran.cpp:205-289) and the sweep's
+// treatment of the cell side (Setka.cpp:10252-10274).  D is the cell across M from
+// the face (Sosed_down), used on the axis.  L is M's side, R the ghost.
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, const CellT<Ph, TS>& M,
+                                                const CellT<Ph, TS>& D, const Face& f, double* L, double* R)
+{
+    constexpr int NV = Ph::NV;
+    const gel_params& p = a.prm;
+    const double gam = p.gamma;
+    const double radius = sqrt(sq(M.cx) + sq(M.cy));
+    if (ftype == GEL_F_INNER_SPHERE) {
+        double g[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) g[k] = M.s[k];
+        rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, false);
+        const double d = sqrt(f.fx * f.fx + f.fy * f.fy);
+        const double ro = 1.0 / (p.in_rho_div * d * d);
+        double w[NV];
+#pragma unroll
+        for (int k = 0; k < NV; k++) w[k] = 0.0;
+        w[0] = ro;
+        w[1] = ro * p.in_p_a * p.in_p_a / p.in_p_den;
+        w[2] = p.in_u * f.fx / d;
+        w[3] = p.in_u * f.fy / d;
+        w[NV - 1] = ro * p.in_q;
+#pragma unroll
+        for (int k = 0; k < NV; k++) { L[k] = g[k]; R[k] = w[k]; }
+    } else if (ftype == GEL_F_AXIS) {
+        bool done = false;
+        if constexpr (TYPES) {
+            if (M.type == GEL_C_1) {
+                double g[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++) g[k] = M.s[k];
+                rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
+#pragma unroll
+                for (int k = 0; k < NV; k++) L[k] = g[k];
+                done = true;
+            }
+        }
+        if (!done) {                                          // Gran.cpp:486-519
+            // the mirror image of M lies as far across the face as M itself
+            const Lin3T<TF> g = lin3_at<TF>(f.fx, f.fy, D.cx, D.cy, M.cx, M.cy, M.cx, M.cy);
+#pragma unroll
+            for (int k = 0; k < NV; k++) {
+                L[k] = double(M.s[k]);
+                if (Ph::odd(k)) L[k] = lin3v<TF>(double(D.s[k]), double(M.s[k]), -double(M.s[k]), g);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NV; k++) R[k] = Ph::odd(k) ? -L[k] : L[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < NV; k++) { L[k] = double(M.s[k]); R[k] = L[k]; }
+        if (ftype == GEL_F_INPUT) {
+            R[0] = p.inf_rho; R[1] = p.inf_p; R[2] = p.inf_u; R[3] = p.inf_v;
+            R[NV - 1] = p.inf_q;
+            if constexpr (NV == 9) {
+                R[4] = p.inf_w; R[5] = p.inf_bx; R[6] = p.inf_by; R[7] = p.inf_bz;
+            }
+        } else if (ftype == GEL_F_EXTERN) {
+            if (R[2] >= 0.0) R[2] = p.extern_u;
+        }
+    }
+}
+
+// Interior face between M (owner, L side) and N (R side); D lies beyond M and U beyond
+// N on the same line (Setka.cpp:10213-10251 from each side).
+template <class Ph, class TS, class TF, bool TYPES>
+__device__ __forceinline__ void interior_states(const CellT<Ph, TS>& M, const CellT<Ph, TS>& N,
+                                                const CellT<Ph, TS>& D, const CellT<Ph, TS>& U, bool m_axis,
+                                                bool n_axis, const Face& f, double gam, double* L, double* R,
+                                                int& bad)
+{
+    side_value<Ph, TS, TF, TYPES>(M, N, D, m_axis, f.fx, f.fy, f.dis, gam, L, bad);
+    side_value<Ph, TS, TF, TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, R, bad);
+}
+
+// Conservative ALE update of one cell and primitive recovery (Setka.cpp:10423-10460),
+// r-weighted: Volume = int y dA before, Volume2 after the step; Ar = planar area.
+// MHD adds the axisymmetric geometric sources (hoop stress with field and swirl,
+// angular momentum, azimuthal induction) per unit planar area.
+// FAST (FP32 bulk): the same update with the divisions by Volume2 and by the new
+// density taken as multiplications by reciprocals (FP64 throughout).
+template <class Ph, bool FAST>
+__device__ __forceinline__ void cell_update(const double* s, const double* Pot, double Volume, double Volume2,
+                                            double vs1, double vs2, double T1, double gam, double qm, double q1,
+                                            double q2, double q3, double qQ, bool table, double* o, int& n_rho,
+                                            int& n_p)
+{
+    if constexpr (FAST) {
+        constexpr int NV = Ph::NV;
+        const double iV2 = 1.0 / Volume2, rV = Volume * iV2, ig1 = 1.0 / (gam - 1.0);
+        const double ro = s[0], p = s[1], u = s[2], v = s[3];
+        const double Ar = 0.5 * (vs1 + vs2);
+        double w = 0.0, bx = 0.0, by = 0.0, bz = 0.0, BB = 0.0, Sv = p, Sw = 0.0, Sb = 0.0;
+        if constexpr (NV == 9) {
+            constexpr double i4pi = 0.07957747154594767, i8pi = 0.039788735772973836;
+            w = s[4]; bx = s[5]; by = s[6]; bz = s[7];
+            BB = bx * bx + by * by + bz * bz;
+            Sv = ro * w * w + p + BB * i8pi - bz * bz * i4pi;
+            Sw = -(ro * v * w - by * bz * i4pi);
+            Sb = v * bz - w * by;
+        }
+        const double Q = s[NV - 1];
+        const double E = p * ig1 + ro * (u * u + v * v + w * w) * 0.5 + BB * 0.039788735772973836;
+        double ro3 = ro * rV - T1 * (Pot[0] * iV2 - qm);
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double iro3 = 1.0 / ro3;
+        const double pv = (NV == 9) ? Sv : p;   // gas: the hoop term is p (Setka.cpp:10440)
+        const double u3 = (ro * u * rV - T1 * (Pot[1] * iV2 - q1)) * iro3;
+        const double v3 = (ro * v * rV - T1 * ((Pot[2] - pv * Ar) * iV2 - q2)) * iro3;
+        double w3 = 0.0, bx3 = 0.0, by3 = 0.0, bz3 = 0.0;
+        const int ie = (NV == 9) ? 4 : 3;
+        if constexpr (NV == 9) {
+            w3 = (ro * w * rV - T1 * ((Pot[3] - Sw * Ar) * iV2)) * iro3;
+            bx3 = bx * rV - T1 * (Pot[5] * iV2);
+            by3 = by * rV - T1 * (Pot[6] * iV2);
+            bz3 = bz * rV - T1 * ((Pot[7] - Sb * Ar) * iV2);
+        }
+        const double E3 = E * rV - T1 * (Pot[ie] * iV2 - q3);
+        double Q3 = Q * rV - T1 * (Pot[NV - 1] * iV2);
+        if (table) Q3 = Q3 + T1 * qQ;
+        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) * 0.039788735772973836) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3;
+        if constexpr (NV == 9) { o[4] = w3; o[5] = bx3; o[6] = by3; o[7] = bz3; }
+        o[NV - 1] = Q3;
+    } else if constexpr (Ph::NV == 5) {
+        const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
+        double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+        if (table) Q33 = Q33 + T1 * qQ;
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2)) / ro3;
+        double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3)) -
+                     0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = Q33;
+    } else {
+        const double pi4 = 4.0 * kPi, pi8 = 8.0 * kPi;
+        const double ro = s[0], p = s[1], u = s[2], v = s[3], w = s[4];
+        const double bx = s[5], by = s[6], bz = s[7], Q = s[8];
+        const double Ar = 0.5 * (vs1 + vs2);
+        const double BB = bx * bx + by * by + bz * bz;
+        const double Sv = ro * w * w + p + BB / pi8 - bz * bz / pi4;
+        const double Sw = -(ro * v * w - by * bz / pi4);
+        const double Sb = v * bz - w * by;
+        const double E = p / (gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + BB / pi8;
+        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
+        if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
+        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
+        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - Sv * Ar) / Volume2 - q2)) / ro3;
+        const double w3 = (ro * w * Volume / Volume2 - T1 * ((Pot[3] - Sw * Ar) / Volume2)) / ro3;
+        const double E3 = E * Volume / Volume2 - T1 * (Pot[4] / Volume2 - q3);
+        const double bx3 = bx * Volume / Volume2 - T1 * (Pot[5] / Volume2);
+        const double by3 = by * Volume / Volume2 - T1 * (Pot[6] / Volume2);
+        const double bz3 = bz * Volume / Volume2 - T1 * ((Pot[7] - Sb * Ar) / Volume2);
+        const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2 - qQ);
+        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / pi8) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; n_p++; }
+        o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = w3;
+        o[5] = bx3; o[6] = by3; o[7] = bz3; o[8] = Q3;
+    }
+}
+
+__device__ __forceinline__ double wave_min(double v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------- marching tile ---
+//
+// A workgroup of NW wavefronts owns a strip of 64 columns (60 output columns and
+// 2 halo columns each side) and a chunk of rows [i0, i1).  It marches up the
+// chunk NW rows at a time: in a step, wavefront w owns row i = base + w, with
+// base = i0 - 1, i0 - 1 + NW, ...  (row i0 - 1 only contributes the radial face
+// below the chunk).  A step has three phases separated by barriers:
+//   load    each wavefront brings cell row i + 2 (state, centroid, type) and node
+//           row i + 3 from HBM into LDS rings of NW + 3 rows;
+//   faces   each wavefront solves the radial face above its row and the polar
+//           face (j, j+1) of its row, from the LDS rings; the radial face goes to
+//           a ring of NW + 1 LDS slots as flux x area products;
+//   update  each wavefront sums the four faces of its cells (the polar face left
+//           of a column comes from the lane below by a cross-lane shuffle, the
+//           radial faces from the slots of its row and of the row below) and writes
+//           the updated primitives.
+// The update phase reads nothing from the rings, so the next step's loads may
+// overwrite them as soon as a wavefront gets there.  Every face is solved once, at
+// one call site per direction; boundary faces only choose their two states.
+template <class Ph, class TS, bool MOVING, bool TYPES, int NW>
+struct MarchLds {
+    static constexpr int NR = NW + 3;                 // rows held by the rings
+    TS st[NR][Ph::NV][kWave];                         // primitives of cell rows
+    double cx[NR][kWave], cy[NR][kWave];              // cell centroids
+    int8_t ty[TYPES ? NR : 1][kWave];                 // cell types
+    double nx[NR][kWave + 1], ny[NR][kWave + 1];      // node rows before the step
+    double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
+    double ny2[MOVING ? NR : 1][kWave + 1];
+    double fps[NW + 1][Ph::NV][kWave];                // radial flux x area above a row
+    double ftm[NW + 1][kWave];                        // CFL time of the cell above that face
+};
+
+struct MarchAcc {       // per-lane results folded into the case's dt buffer and counters
+    double tmin;
+    int n_rho, n_p, bad;
+};
+
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
+__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, int bx,
+                                           int i0, int i1, MarchAcc& acc)
+{
+    constexpr int NV = Ph::NV;
+    constexpr int NR = NW + 3;
+    using C = CellT<Ph, TS>;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int ni = a.ni, nj = a.nj;
+    const int j = bx * kOut + lane - kHalo;
+    const bool col = (j >= 0 && j < nj);
+    const bool out_lane = col && lane >= kHalo && lane < kWave - kHalo;
+    const bool used_polar = col && lane >= 1 && lane < kWave - kHalo;
+    const double gam = a.prm.gamma;
+    const double cfl = a.prm.cfl;
+    const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
+    const bool axis_lo = (j == 0), axis_hi = (j == nj - 1);
+    const bool m_axis = axis_lo || axis_hi;
+    const bool n_axis = (j + 1 == nj - 1);
+    const size_t nc = (size_t)ni * nj;
+
+    if (bx == 0 && i0 == 0 && threadIdx.x == 0)
+        a.dtbuf[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
+
+    auto slot = [&](int r) { return (r - i0 + 2) % NR; };     // rows and node rows >= i0 - 2
+    auto fslot = [&](int r) { return (r - i0 + 1) % (NW + 1); };  // radial face above row r >= i0 - 1
+
+    // cell row r and node row r + 1 from HBM into the rings
+    auto load_row = [&](int r) {
+        const Nodes lo = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r, j, col);
+        const Nodes hi = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r + 1, j, col);
+        const C c = load_cell<Ph, TS, TYPES>(a, r, j, col, lo, hi);
+        const int s = slot(r);
+#pragma unroll
+        for (int v = 0; v < NV; v++) S.st[s][v][lane] = c.s[v];
+        S.cx[s][lane] = c.cx;
+        S.cy[s][lane] = c.cy;
+        if constexpr (TYPES) S.ty[s][lane] = (int8_t)c.type;
+        const int q = r + 1, sq1 = slot(q);
+        auto put = [&](int idx, int jj) {
+            const bool v = (q >= 0 && q <= ni && jj >= 0 && jj <= nj);
+            const size_t k = (size_t)q * (nj + 1) + jj;
+            double x = 1.0, y = 1.0;
+            if (v) {
+                x = a.x[k]; y = a.y[k];
+                if (MOVING) { x *= a.s_old; y *= a.s_old; }
+            }
+            S.nx[sq1][idx] = x;
+            S.ny[sq1][idx] = y;
+            if constexpr (MOVING) {
+                double x2 = 1.0, y2 = 1.0;
+                if (v) { x2 = a.x2[k] * a.s_new; y2 = a.y2[k] * a.s_new; }
+                S.nx2[sq1][idx] = x2;
+                S.ny2[sq1][idx] = y2;
+            }
+        };
+        put(lane, j);
+        if (lane == kWave - 1) put(kWave, j + 1);
+    };
+
+    auto cell = [&](int r, int ln) -> C {
+        ln = min(max(ln, 0), kWave - 1);
+        const int s = slot(r);
+        C c;
+#pragma unroll
+        for (int v = 0; v < NV; v++) c.s[v] = S.st[s][v][ln];
+        c.cx = S.cx[s][ln];
+        c.cy = S.cy[s][ln];
+        c.type = TYPES ? (int)S.ty[s][ln] : (int)GEL_C_2;
+        const int jj = bx * kOut + ln - kHalo;
+        c.ok = (jj >= 0 && jj < nj && r >= 0 && r < ni) ? 1 : 0;
+        return c;
+    };
+
+    struct NodeP { double x, y, x2, y2; };
+    auto node = [&](int q, int idx) -> NodeP {
+        const int s = slot(q);
+        NodeP n;
+        n.x = S.nx[s][idx];
+        n.y = S.ny[s][idx];
+        if constexpr (MOVING) { n.x2 = S.nx2[s][idx]; n.y2 = S.ny2[s][idx]; }
+        else { n.x2 = n.x; n.y2 = n.y; }
+        return n;
+    };
+
+    // initial rows i0 - 2 .. i0 (and node rows i0 - 1 .. i0 + 1)
+    for (int r = i0 - 2 + w; r <= i0; r += NW) load_row(r);
+
+    for (int base = i0 - 1; base < i1; base += NW) {
+        const int i = base + w;
+        const bool act = (i < i1);
+        const bool upd = act && (i >= i0);
+
+        // ---- load
+        if (act) load_row(i + 2);
+        __syncthreads();
+
+        // ---- faces
+        C M;
+        double psp[NV], ps0[NV];
+        double t_dn = 0.0, t_pol = 0.0, t0 = 0.0;
+        double vs1 = 1.0, vol = 1.0, vs2 = 1.0, vol2 = 1.0;
+        if (act) {
+            M = cell(i, lane);
+            const C D = cell(i - 1, lane), N = cell(i + 1, lane), U = cell(i + 2, lane);
+            // radial face above row i, nodes (i+1, j) -> (i+1, j+1); below row 0 it is the
+            // inner sphere, oriented from row 0 inward: (0, j+1) -> (0, j)
+            {
+                const bool inner = (i < 0);
+                NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
+                if (inner) { const NodeP t = na; na = nb; nb = t; }
+                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                double L[NV], R[NV], P[NV];
+                int b = 0;
+                if (inner) {
+                    boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
+                } else if (i + 1 >= ni) {
+                    const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
+                    boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
+                } else {
+                    interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
+                }
+                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+                const int fs = fslot(i);
+#pragma unroll
+                for (int k = 0; k < NV; k++) S.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
+                S.ftm[fs][lane] = cfl_time<TF>(cfl, N.cx, N.cy, f, uu);
+                t_dn = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
+                if (out_lane) acc.bad += b;
+            }
+            if (upd) {
+                // polar face (j, j+1), nodes (i+1, j+1) -> (i, j+1); the lane left of
+                // column 0 solves the axis face of column 0, oriented (i, 0) -> (i+1, 0)
+                const bool axl = (j == -1);
+                NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
+                if (axl) { const NodeP t = na; na = nb; nb = t; }
+                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
+                double L[NV], R[NV], P[NV];
+                int b = 0;
+                if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
+                else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
+                else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
+                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+#pragma unroll
+                for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * f.S;
+                t_pol = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
+                const double t_right = cfl_time<TF>(cfl, Mr.cx, Mr.cy, f, uu);
+                if (used_polar) acc.bad += b;
+                // the face left of each column is the lane below's face, seen from the other side
+#pragma unroll
+                for (int k = 0; k < NV; k++) ps0[k] = -shfl_up(psp[k], 1);
+                t0 = shfl_up(t_right, 1);
+                // cell geometry before and after the step
+                const NodeP c00 = node(i, lane), c10 = node(i + 1, lane), c11 = node(i + 1, lane + 1),
+                            c01 = node(i, lane + 1);
+                const double px[4] = { c00.x, c10.x, c11.x, c01.x };
+                const double py[4] = { c00.y, c10.y, c11.y, c01.y };
+                vs1 = quad_area(px, py);
+                vol = quad_volume_r(px, py);
+                vs2 = vs1; vol2 = vol;
+                if (MOVING) {
+                    const double px2[4] = { c00.x2, c10.x2, c11.x2, c01.x2 };
+                    const double py2[4] = { c00.y2, c10.y2, c11.y2, c01.y2 };
+                    vs2 = quad_area(px2, py2);
+                    vol2 = quad_volume_r(px2, py2);
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- update (Setka.cpp:10372-10468)
+        if (upd) {
+            const int fa = fslot(i), fb = fslot(i - 1);
+            const double x = M.cx, y = M.cy;
+            const double radius = sqrt(sq(x) + sq(y));
+            if (out_lane && !(radius < a.prm.r_skip)) {
+                // faces in the reference's order: edge 0 (polar, left), 1 (radial, above),
+                // 2 (polar, right), 3 (radial, below)
+                double Pot[NV];
+#pragma unroll
+                for (int k = 0; k < NV; k++) {
+                    double s = 0.0;
+                    s = s + ps0[k];
+                    s = s + S.fps[fa][k][lane];
+                    s = s + psp[k];
+                    s = s + (-S.fps[fb][k][lane]);
+                    Pot[k] = s;
+                }
+                double time = 10000000;
+                time = mn(time, t0);
+                time = mn(time, t_dn);
+                time = mn(time, t_pol);
+                time = mn(time, S.ftm[fb][lane]);
+                acc.tmin = fmin(acc.tmin, time);
+
+                const size_t kc = (size_t)i * nj + j;
+                TS* __restrict__ nxt = (TS*)a.next;
+                if (TYPES && M.type == GEL_C_CENTR) {
+#pragma unroll
+                    for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = M.s[k];
+                } else {
+                    double sd[NV];
+#pragma unroll
+                    for (int k = 0; k < NV; k++) sd[k] = double(M.s[k]);
+                    double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
+                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && a.qflag[kc]);
+                    if (analytic) {
+                        const double* __restrict__ sp = a.src + kc;
+                        auto fk = [&](int k) { return sp[(size_t)k * nc]; };
+                        source_mk1<decltype(fk), sizeof(TF) == 4>(a.prm, fk, sd[2], sd[3], sd[0], sd[1], q1s, q2s, q3s);
+                    } else if (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) {
+                        qms = a.qtab[kc];
+                        q1s = a.qtab[nc + kc];
+                        q2s = a.qtab[2 * nc + kc];
+                        q3s = a.qtab[3 * nc + kc];
+                        qQs = a.qtab[4 * nc + kc];
+                    }
+                    double o[NV];
+                    cell_update<Ph, sizeof(TF) == 4>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
+                                    (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) && !analytic, o, acc.n_rho,
+                                    acc.n_p);
+#pragma unroll
+                    for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
+                }
+            }
+        }
+    }
+}
+
+// Fold a wavefront's results into the case's dt buffer and counters.
+__device__ __forceinline__ void march_flush(const SweepArgs& a, MarchAcc& acc)
+{
+    const double tmin = wave_min(acc.tmin);
+    const int s_rho = wave_sum(acc.n_rho), s_p = wave_sum(acc.n_p), s_bad = wave_sum(acc.bad);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (tmin < __builtin_huge_val())
+            atomicMin(&a.dtbuf[a.dt_wr], (unsigned long long)__double_as_longlong(tmin));
+        if (s_rho) atomicAdd(&a.counters[0], (unsigned long long)s_rho);
+        if (s_p) atomicAdd(&a.counters[1], (unsigned long long)s_p);
+        if (s_bad) atomicAdd(&a.counters[2], (unsigned long long)s_bad);
+    }
+    acc = MarchAcc{ __builtin_huge_val(), 0, 0, 0 };
+}
+
+// Work-group index -> work item, so that the work-groups one XCD runs at a time
+// (dispatch deals work-groups to the 8 XCDs round-robin) take neighbouring tiles.
+__device__ __forceinline__ int xcd_item(int g, int ng)
+{
+    constexpr int kXcd = 8;
+    if (ng % kXcd != 0) return g;
+    return (g % kXcd) * (ng / kXcd) + g / kXcd;
+}
+
+// One step of one case: persistent work-groups over the (strip, chunk) tiles.
+// Two work-groups of four wavefronts per CU (the LDS rings of the FP64 MHD kernel
+// take 78 KB of the 160 KB) need at most 256 registers per lane.
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
+__global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
+sweep_kernel(SweepArgs a, int strips, int ntiles)
+{
+    __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
+    MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
+    for (int t = xcd_item(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
+        const int bx = t % strips, by = t / strips;
+        const int i0 = by * a.rows, i1 = min(i0 + a.rows, a.ni);
+        // no barrier between tiles: the update phase of the previous tile reads no ring,
+        // and its faces phase ended at a barrier every wavefront has passed
+        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, bx, i0, i1, acc);
+    }
+    march_flush(a, acc);
+}
+
+// One step of many independent cases in one persistent launch: the work-groups walk
+// the (case, strip, chunk) list round-robin; args[case] carries each case's grid,
+// buffers, time-step slots and parameters.
+struct BatchItem {
+    int cs, bx, by;
+};
+
+template <class Ph, class TS, class TF, int SRC, int FLUX, int NW>
+__global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
+sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
+{
+    __shared__ MarchLds<Ph, TS, false, false, NW> S;
+    MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
+    for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
+        const BatchItem b = items[it];
+        const SweepArgs& a = args[b.cs];
+        const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
+        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, b.bx, i0, i1, acc);
+        march_flush(a, acc);
+    }
+}
+
+
+// ------------------------------------------------------------------ launches ---
+
+// Resident work-groups of a kernel on the current device (occupancy x CUs).
+inline int resident_groups(const void* kern, int block)
+{
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, block, 0) != hipSuccess || per < 1 || cus < 1)
+        return 256;
+    return per * cus;
+}
+
+// Rows per chunk for one case run by `groups` resident work-groups: a work-group
+// takes ceil((rows + 1) / kNW) marching steps per tile (the extra row is the face
+// below the chunk), and the tiles run in ceil(tiles / groups) rounds; pick the rows
+// with the fewest steps on the critical path, the longest chunks among equals.
+inline int choose_rows(int ni, int nj, int groups)
+{
+    const int strips = (nj + kOut - 1) / kOut;
+    long best = -1;
+    int br = kNW - 1;
+    for (int k = 1; k <= 64; k++) {
+        const int rows = k * kNW - 1;
+        const long tiles = (long)strips * ((ni + rows - 1) / rows);
+        const long rounds = (tiles + groups - 1) / groups;
+        const long cost = rounds * k;
+        if (best < 0 || cost <= best) { best = cost; br = rows; }
+        if (rows >= ni) break;
+    }
+    return br;
+}
+
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
+hipError_t launch_t(gel_ctx* c, SweepArgs& a, hipStream_t s)
+{
+    const auto kern = sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, kNW>;
+    const int groups = resident_groups((const void*)kern, kNW * kWave);
+    if (c->rows <= 0) c->rows = choose_rows(c->ni, c->nj, groups);
+    a.rows = c->rows;
+    const int strips = (a.nj + kOut - 1) / kOut;
+    const int ntiles = strips * ((a.ni + a.rows - 1) / a.rows);
+    const int grid = ntiles < groups ? ntiles : groups;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, a, strips, ntiles);
+    return hipGetLastError();
+}
+
+template <class Ph, class TS, class TF, bool TYPES, int FLUX>
+hipError_t launch_ms(gel_ctx* c, SweepArgs& a, hipStream_t s, bool moving, int src)
+{
+#define GEL_L(MV, SR) launch_t<Ph, TS, TF, MV, TYPES, SR, FLUX>(c, a, s)
+    if (moving) {
+        switch (src) {
+        case GEL_SRC_MK1: return GEL_L(true, GEL_SRC_MK1);
+        case GEL_SRC_TABLE: return GEL_L(true, GEL_SRC_TABLE);
+        case GEL_SRC_HYBRID: return GEL_L(true, GEL_SRC_HYBRID);
+        default: return GEL_L(true, GEL_SRC_NONE);
+        }
+    }
+    switch (src) {
+    case GEL_SRC_MK1: return GEL_L(false, GEL_SRC_MK1);
+    case GEL_SRC_TABLE: return GEL_L(false, GEL_SRC_TABLE);
+    case GEL_SRC_HYBRID: return GEL_L(false, GEL_SRC_HYBRID);
+    default: return GEL_L(false, GEL_SRC_NONE);
+    }
+#undef GEL_L
+}
+
+// One step of a batch: the persistent launch over the (case, strip, chunk) items;
+// workgroups = 0: the kernel's resident work-groups.
+template <class Ph, class TS, class TF, int SRC, int FLUX>
+hipError_t batch_launch_t(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
+                          hipStream_t s)
+{
+    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW>;
+    int grid = workgroups > 0 ? workgroups : resident_groups((const void*)kern, kNW * kWave);
+    if (grid > nitems) grid = nitems;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, d_args, d_items, nitems);
+    return hipGetLastError();
+}
+
+template <class Ph, class TS, class TF, int FLUX>
+hipError_t batch_launch_s(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
+                          hipStream_t s, int src)
+{
+    if (src == GEL_SRC_MK1) return batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(d_args, d_items, nitems, workgroups, s);
+    if (src == GEL_SRC_TABLE) return batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(d_args, d_items, nitems, workgroups, s);
+    if (src == GEL_SRC_HYBRID) return batch_launch_t<Ph, TS, TF, GEL_SRC_HYBRID, FLUX>(d_args, d_items, nitems, workgroups, s);
+    return batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(d_args, d_items, nitems, workgroups, s);
+}
+
+// The FP32-bulk launches live in sweep_fp32.hip (FP contraction on there).
+hipError_t launch_sweep_fp32(gel_ctx* c, SweepArgs& a, bool moving);
+hipError_t launch_batch_fp32(const gel_params& p, const SweepArgs* d_args, const BatchItem* d_items, int nitems,
+                             int workgroups, hipStream_t s);
+int resident_batch_fp32(const gel_params& p);
+
+}  // namespace gel
+
