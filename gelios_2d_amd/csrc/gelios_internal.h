@@ -29,6 +29,7 @@ struct SweepArgs {
     const double* __restrict__ src;
     const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
     const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
+    const double* __restrict__ geo;   // static grids: the geometry table (geometry_kernel)
     unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
@@ -66,6 +67,8 @@ struct gel_ctx {
     double* d_src = nullptr;       // [GEL_MK1_NFIELDS][nc] hydrogen moments and k factors
     double* d_qtab = nullptr;      // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID)
     int8_t* d_qflag = nullptr;     // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
+    double* d_geo = nullptr;       // geometry table of the static grid (built on first use)
+    bool geo_valid = false;        // false after the nodes move
     unsigned long long* d_dt = nullptr;   // [3]
     unsigned long long* d_cnt = nullptr;  // [3]
     long long steps = 0;

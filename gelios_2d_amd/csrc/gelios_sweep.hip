@@ -123,6 +123,7 @@ static void free_ctx(gel_ctx* c)
     (void)hipFree(c->d_state[0]); (void)hipFree(c->d_state[1]);
     (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src); (void)hipFree(c->d_qtab);
     (void)hipFree(c->d_qflag);
+    (void)hipFree(c->d_geo);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -337,9 +338,32 @@ static hipError_t launch(gel_ctx* c, SweepArgs& a, bool moving)
                 : launch_ms<GasPhys, double, double, false, GEL_FLUX_HLL>(c, a, s, moving, src);
 }
 
+// The geometry table of a static grid (GeoTable in sweep_kernel.h), built on the
+// context's stream at the first static step after the nodes last moved.
+static int ensure_geometry(gel_ctx* c)
+{
+    if (c->geo_valid) return GEL_OK;
+    if (!c->d_geo && hipMalloc(&c->d_geo, GeoTable::doubles(c->ni, c->nj) * sizeof(double)) != hipSuccess)
+        return fail(GEL_E_NOMEM, "geometry table");
+    const long long n = (long long)(c->ni + 1) * (c->nj + 1);
+    const dim3 g((unsigned)((n + 255) / 256));
+    if (c->prm.precision == GEL_PREC_FP32_BULK)
+        hipLaunchKernelGGL(geometry_kernel<float>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y, c->d_geo);
+    else
+        hipLaunchKernelGGL(geometry_kernel<double>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y, c->d_geo);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(GEL_E_HIP, std::string("geometry: ") + hipGetErrorString(e));
+    c->geo_valid = true;
+    return GEL_OK;
+}
+
 // moving: 0 static, 1 explicit next positions in d_x2/d_y2, 2 prescribed motion
 static int one_step(gel_ctx* c, int moving)
 {
+    if (moving == 0) {
+        const int st = ensure_geometry(c);
+        if (st != GEL_OK) return st;
+    }
     SweepArgs a;
     a.ni = c->ni; a.nj = c->nj; a.rows = c->rows;
     a.dt_rd = (int)(c->steps % 3);
@@ -360,6 +384,7 @@ static int one_step(gel_ctx* c, int moving)
     a.src = c->d_src;
     a.qtab = c->d_qtab;
     a.qflag = c->d_qflag;
+    a.geo = c->d_geo;
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
     a.prm = c->prm;
@@ -459,6 +484,7 @@ int gel_step_moving(gel_ctx* c, const double* x2, const double* y2, gel_stats* o
     if (st != GEL_OK) return st;
     std::swap(c->d_x, c->d_x2);       // Point::x = x2 after the step (Setka.cpp:10473-10480)
     std::swap(c->d_y, c->d_y2);
+    c->geo_valid = false;
     return fill_stats(c, out);
 }
 
@@ -637,7 +663,7 @@ static int batch_upload_args(gel_batch* b)
             a.now = c->d_state[cur];
             a.next = c->d_state[cur ^ 1];
             a.ctype = nullptr; a.otype = c->d_otype;
-            a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag;
+            a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag; a.geo = c->d_geo;
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
             if (p == kCycle) a.next = c->d_state[c->cur];
@@ -729,6 +755,10 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     gel_batch* b = new (std::nothrow) gel_batch();
     if (!b) return fail(GEL_E_NOMEM, "host");
     b->cases.assign(cases, cases + ncases);
+    for (int k = 0; k < ncases; k++) {
+        const int st = ensure_geometry(cases[k]);
+        if (st != GEL_OK) { gel_batch_destroy(b); return st; }
+    }
     std::vector<int2> tiles;
     for (int k = 0; k < ncases; k++)
         for (long long t = 0; t < gel_refresh_tiles(cases[k]->ni, cases[k]->nj); t++) tiles.push_back(make_int2(k, (int)t));

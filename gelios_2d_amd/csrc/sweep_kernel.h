@@ -90,7 +90,7 @@ __device__ __forceinline__ Nodes load_nodes(const SweepArgs& a, const double* __
     return n;
 }
 
-template <class Ph, class TS, bool TYPES>
+template <class Ph, class TS, bool TYPES, bool CENTROID = true>
 __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, int j, bool col,
                                                    const Nodes& lo, const Nodes& hi)
 {
@@ -104,9 +104,11 @@ __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, in
 #pragma unroll
         for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)v * nc + k];
         if (TYPES) c.type = a.ctype[k];
-        const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
-        const double py[4] = { lo.ya, hi.ya, hi.yb, lo.yb };
-        quad_center(px, py, c.cx, c.cy);
+        if constexpr (CENTROID) {
+            const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
+            const double py[4] = { lo.ya, hi.ya, hi.yb, lo.yb };
+            quad_center(px, py, c.cx, c.cy);
+        }
     } else {
 #pragma unroll
         for (int v = 0; v < Ph::NV; v++) c.s[v] = TS(0);
@@ -543,6 +545,81 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
+// ------------------------------------------------------------ geometry table ---
+// Static grids read their geometry from a table built once (geometry_kernel) with the
+// same functions and the same node order the kernels use on moving grids, so the
+// values are identical.  Layout, FP64, structure of arrays:
+//   cell  [4][ni*nj]        centroid x, y (Cell::Get_Center), planar area vs1
+//                           (Get_Volume), r-weighted volume (Get_Volume_rotate / 2 pi);
+//   rad   [6][(ni+1)*nj]    the radial face at node row r, column j, as edge 1 of
+//                           cell (r-1, j) (for r = 0 as edge 3 of cell (0, j), the
+//                           inner sphere): n1, n2, S, fx, fy, dis;
+//   pol   [6][ni*(nj+1)]    the polar face of row i at node column c, as edge 2 of
+//                           cell (i, c-1) (for c = 0 as edge 0 of cell (i, 0), the
+//                           axis): n1, n2, S, fx, fy, dis.
+struct GeoTable {
+    const double* base;
+    size_t nc, nr, np;
+    __host__ __device__ GeoTable(const double* b, int ni, int nj)
+        : base(b), nc((size_t)ni * nj), nr((size_t)(ni + 1) * nj), np((size_t)ni * (nj + 1)) {}
+    __host__ __device__ const double* cell(int k) const { return base + (size_t)k * nc; }
+    __host__ __device__ const double* rad(int k) const { return base + 4 * nc + (size_t)k * nr; }
+    __host__ __device__ const double* pol(int k) const { return base + 4 * nc + 6 * nr + (size_t)k * np; }
+    __host__ __device__ static size_t doubles(int ni, int nj)
+    {
+        return 4 * (size_t)ni * nj + 6 * (size_t)(ni + 1) * nj + 6 * (size_t)ni * (nj + 1);
+    }
+};
+
+// One thread per node position (r, c), r in [0, ni], c in [0, nj]: the cell (r, c),
+// the radial face at (r, c) and the polar face at (r, c) where they exist.
+template <class TF>
+__global__ void geometry_kernel(int ni, int nj, const double* __restrict__ X, const double* __restrict__ Y,
+                                double* __restrict__ geo)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)(ni + 1) * (nj + 1)) return;
+    const int r = (int)(t / (nj + 1)), c = (int)(t % (nj + 1));
+    const GeoTable G(geo, ni, nj);
+    double* cell[4];
+    double* rad[6];
+    double* pol[6];
+    for (int k = 0; k < 4; k++) cell[k] = const_cast<double*>(G.cell(k));
+    for (int k = 0; k < 6; k++) { rad[k] = const_cast<double*>(G.rad(k)); pol[k] = const_cast<double*>(G.pol(k)); }
+    auto nd = [&](int q, int cc, double& x, double& y) {
+        const size_t k = (size_t)q * (nj + 1) + cc;
+        x = X[k]; y = Y[k];
+    };
+    auto store = [&](double* const* F, size_t k, const Face& f) {
+        F[0][k] = f.n1; F[1][k] = f.n2; F[2][k] = f.S; F[3][k] = f.fx; F[4][k] = f.fy; F[5][k] = f.dis;
+    };
+    if (r < ni && c < nj) {
+        double x0, y0, x1, y1, x2, y2, x3, y3;    // (r, c), (r+1, c), (r+1, c+1), (r, c+1)
+        nd(r, c, x0, y0); nd(r + 1, c, x1, y1); nd(r + 1, c + 1, x2, y2); nd(r, c + 1, x3, y3);
+        const double px[4] = { x0, x1, x2, x3 };
+        const double py[4] = { y0, y1, y2, y3 };
+        const size_t k = (size_t)r * nj + c;
+        double cx, cy;
+        quad_center(px, py, cx, cy);
+        cell[0][k] = cx;
+        cell[1][k] = cy;
+        cell[2][k] = quad_area(px, py);
+        cell[3][k] = quad_volume_r(px, py);
+    }
+    if (c < nj) {                                  // radial face at node row r
+        double ax, ay, bx, by;
+        nd(r, c, ax, ay); nd(r, c + 1, bx, by);
+        if (r == 0) { double t0 = ax; ax = bx; bx = t0; t0 = ay; ay = by; by = t0; }
+        store(rad, (size_t)r * nj + c, face_geom<false, TF>(ax, ay, bx, by, ax, ay, bx, by, 1.0));
+    }
+    if (r < ni) {                                  // polar face of row r at node column c
+        double ax, ay, bx, by;
+        nd(r + 1, c, ax, ay); nd(r, c, bx, by);
+        if (c == 0) { double t0 = ax; ax = bx; bx = t0; t0 = ay; ay = by; by = t0; }
+        store(pol, (size_t)r * (nj + 1) + c, face_geom<false, TF>(ax, ay, bx, by, ax, ay, bx, by, 1.0));
+    }
+}
+
 // ---------------------------------------------------------------- marching tile ---
 //
 // A workgroup of NW wavefronts owns a strip of 64 columns (60 output columns and
@@ -568,7 +645,8 @@ struct MarchLds {
     TS st[NR][Ph::NV][kWave];                         // primitives of cell rows
     double cx[NR][kWave], cy[NR][kWave];              // cell centroids
     int8_t ty[TYPES ? NR : 1][kWave];                 // cell types
-    double nx[NR][kWave + 1], ny[NR][kWave + 1];      // node rows before the step
+    double nx[MOVING ? NR : 1][kWave + 1];            // node rows before the step (static
+    double ny[MOVING ? NR : 1][kWave + 1];            // grids read the geometry table)
     double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
     double ny2[MOVING ? NR : 1][kWave + 1];
     double fps[NW + 1][Ph::NV][kWave];                // radial flux x area above a row
@@ -608,17 +686,31 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
     auto slot = [&](int r) { return (r - i0 + 2) % NR; };     // rows and node rows >= i0 - 2
     auto fslot = [&](int r) { return (r - i0 + 1) % (NW + 1); };  // radial face above row r >= i0 - 1
 
+    const GeoTable G(a.geo, ni, nj);
+
     // cell row r and node row r + 1 from HBM into the rings
     auto load_row = [&](int r) {
-        const Nodes lo = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r, j, col);
-        const Nodes hi = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r + 1, j, col);
-        const C c = load_cell<Ph, TS, TYPES>(a, r, j, col, lo, hi);
+        C c;
+        if constexpr (MOVING) {
+            const Nodes lo = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r, j, col);
+            const Nodes hi = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r + 1, j, col);
+            c = load_cell<Ph, TS, TYPES>(a, r, j, col, lo, hi);
+        } else {
+            const Nodes none{ 1.0, 1.0, 1.0, 1.0 };
+            c = load_cell<Ph, TS, TYPES, false>(a, r, j, col, none, none);
+            if (c.ok) {
+                const size_t k = (size_t)r * nj + j;
+                c.cx = G.cell(0)[k];
+                c.cy = G.cell(1)[k];
+            }
+        }
         const int s = slot(r);
 #pragma unroll
         for (int v = 0; v < NV; v++) S.st[s][v][lane] = c.s[v];
         S.cx[s][lane] = c.cx;
         S.cy[s][lane] = c.cy;
         if constexpr (TYPES) S.ty[s][lane] = (int8_t)c.type;
+        if constexpr (!MOVING) return;
         const int q = r + 1, sq1 = slot(q);
         auto put = [&](int idx, int jj) {
             const bool v = (q >= 0 && q <= ni && jj >= 0 && jj <= nj);
@@ -665,6 +757,12 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
         else { n.x2 = n.x; n.y2 = n.y; }
         return n;
     };
+    // face geometry of a static grid from the table (defaults where the lane has none)
+    auto table_face = [&](const double* const* F, bool valid, size_t k) -> Face {
+        Face f{ 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0 };
+        if (valid) { f.n1 = F[0][k]; f.n2 = F[1][k]; f.S = F[2][k]; f.fx = F[3][k]; f.fy = F[4][k]; f.dis = F[5][k]; }
+        return f;
+    };
 
     // initial rows i0 - 2 .. i0 (and node rows i0 - 1 .. i0 + 1)
     for (int r = i0 - 2 + w; r <= i0; r += NW) load_row(r);
@@ -690,9 +788,15 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
             // inner sphere, oriented from row 0 inward: (0, j+1) -> (0, j)
             {
                 const bool inner = (i < 0);
-                NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
-                if (inner) { const NodeP t = na; na = nb; nb = t; }
-                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                Face f;
+                if constexpr (MOVING) {
+                    NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
+                    if (inner) { const NodeP t = na; na = nb; nb = t; }
+                    f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                } else {
+                    const double* F[6] = { G.rad(0), G.rad(1), G.rad(2), G.rad(3), G.rad(4), G.rad(5) };
+                    f = table_face(F, col && i + 1 <= ni, (size_t)(i + 1) * nj + j);
+                }
                 double L[NV], R[NV], P[NV];
                 int b = 0;
                 if (inner) {
@@ -715,9 +819,15 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 // polar face (j, j+1), nodes (i+1, j+1) -> (i, j+1); the lane left of
                 // column 0 solves the axis face of column 0, oriented (i, 0) -> (i+1, 0)
                 const bool axl = (j == -1);
-                NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
-                if (axl) { const NodeP t = na; na = nb; nb = t; }
-                const Face f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                Face f;
+                if constexpr (MOVING) {
+                    NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
+                    if (axl) { const NodeP t = na; na = nb; nb = t; }
+                    f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
+                } else {
+                    const double* F[6] = { G.pol(0), G.pol(1), G.pol(2), G.pol(3), G.pol(4), G.pol(5) };
+                    f = table_face(F, j >= -1 && j < nj, (size_t)i * (nj + 1) + (j + 1));
+                }
                 const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
                 double L[NV], R[NV], P[NV];
                 int b = 0;
@@ -735,6 +845,14 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 for (int k = 0; k < NV; k++) ps0[k] = -shfl_up(psp[k], 1);
                 t0 = shfl_up(t_right, 1);
                 // cell geometry before and after the step
+                if constexpr (!MOVING) {
+                    if (col) {
+                        const size_t k = (size_t)i * nj + j;
+                        vs1 = G.cell(2)[k];
+                        vol = G.cell(3)[k];
+                    }
+                    vs2 = vs1; vol2 = vol;
+                } else {
                 const NodeP c00 = node(i, lane), c10 = node(i + 1, lane), c11 = node(i + 1, lane + 1),
                             c01 = node(i, lane + 1);
                 const double px[4] = { c00.x, c10.x, c11.x, c01.x };
@@ -742,7 +860,6 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 vs1 = quad_area(px, py);
                 vol = quad_volume_r(px, py);
                 vs2 = vs1; vol2 = vol;
-                if (MOVING) {
                     const double px2[4] = { c00.x2, c10.x2, c11.x2, c01.x2 };
                     const double py2[4] = { c00.y2, c10.y2, c11.y2, c01.y2 };
                     vs2 = quad_area(px2, py2);
