@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .params import GelParams
+from .params import GelParams, GelTracking
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgelios.so")
@@ -55,6 +55,9 @@ SIGNATURES = {
     "gel_set_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_get_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
+    "gel_tracking_reference": (None, [ctypes.POINTER(GelTracking)]),
+    "gel_set_tracking": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelTracking), _dp]),
+    "gel_get_nodes": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
     "gel_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]),
     "gel_mlp_destroy": (None, [ctypes.c_void_p]),

@@ -149,3 +149,78 @@ def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision
         w.name = f"ensemble-case{k:02d}-{ni}x{nj}"
         out.append((w, f))
     return out
+
+
+@dataclass
+class TrackedCase:
+    name: str
+    grid: PolarGrid
+    params: GelParams
+    tracking: object              # params.GelTracking
+    rails: np.ndarray             # nj+1 rail angles
+    prim0: np.ndarray
+    steps: int
+
+
+def tracked_key_radii(s: float, theta1: float):
+    """Inner shock, contact and outer shock radii (units of 390.222 AU, the reference's
+    RR_) on the rail at angle s: about 98, 148 and 242 AU at the nose, widening to the
+    flank."""
+    q = (s / theta1) ** 2
+    return 0.25 * (1.0 + 0.25 * q), 0.38 * (1.0 + 0.3 * q), 0.62 * (1.0 + 0.4 * q)
+
+
+def tracked_heliosphere(nj: int, blocks, steps: int, seed: int, flux: int = 0,
+                        theta1: float = 2.0 * math.pi / 3.0, outer_faces: int = None,
+                        key_radii=None, name: str = "tracked-heliosphere") -> TrackedCase:
+    """A heliosphere with fitted inner shock, contact and outer shock on the reference's
+    A rails (Rail.cpp:19-75), the reference's own constants (Help.h) and boundary
+    states: supersonic wind (the Inner_sphere formula, Gran.cpp:234-265) inside the
+    inner shock (C_1 cells), the wind shocked by a strong shock between the inner shock
+    and the contact, the inflow (Gran.cpp:276-282) shocked at Mach 1.97 between the
+    contact and the outer shock, the undisturbed inflow outside; +-1% seeded density
+    noise.  blocks = (M1, M2, M3, M4) rail nodes per block; key_radii(s) -> (inner shock,
+    contact, outer shock) radii on the rail at angle s (default tracked_key_radii)."""
+    from .grid import tracked_polar_grid
+    from .params import GEL_C_1, GEL_C_2, reference_params, reference_tracking
+    prm = reference_params()
+    prm.flux = flux
+    t = reference_tracking(outer_faces=outer_faces if outer_faces is not None else 1 << 30)
+    if key_radii is None:
+        key_radii = lambda s: tracked_key_radii(s, theta1)   # noqa: E731
+    g, rails, rows = tracked_polar_grid(nj, blocks, key_radii, t, theta1)
+    t.row_inner_shock, t.row_contact, t.row_outer_shock = rows
+    gam = prm.gamma
+    cx, cy = g.cell_centers()
+    r = np.hypot(cx, cy)
+    i = np.repeat(np.arange(g.ni), g.nj)
+    ro_w = 1.0 / (prm.in_rho_div * r * r)
+    p_w = ro_w * prm.in_p_a * prm.in_p_a / prm.in_p_den
+    u_w, v_w = prm.in_u * cx / r, prm.in_u * cy / r
+    M = abs(prm.inf_u) / math.sqrt(gam * prm.inf_p / prm.inf_rho)
+    comp = (gam + 1) * M * M / ((gam - 1) * M * M + 2)
+    p_ratio = (2 * gam * M * M - (gam - 1)) / (gam + 1)
+    ro = np.where(i < rows[0], ro_w, 0.0)
+    ro = np.where((i >= rows[0]) & (i < rows[1]), 4.0 * ro_w, ro)
+    ro = np.where((i >= rows[1]) & (i < rows[2]), prm.inf_rho * comp, ro)
+    ro = np.where(i >= rows[2], prm.inf_rho, ro)
+    p = np.where(i < rows[0], p_w, 0.0)
+    p = np.where((i >= rows[0]) & (i < rows[1]), 0.75 * ro_w * prm.in_u ** 2, p)
+    p = np.where((i >= rows[1]) & (i < rows[2]), prm.inf_p * p_ratio, p)
+    p = np.where(i >= rows[2], prm.inf_p, p)
+    u = np.where(i < rows[0], u_w, np.where(i < rows[1], 0.25 * u_w,
+                 np.where(i < rows[2], prm.inf_u / comp, prm.inf_u)))
+    v = np.where(i < rows[0], v_w, np.where(i < rows[1], 0.25 * v_w, 0.0))
+    Q = np.where(i < rows[1], ro, prm.inf_q)
+    rng = np.random.default_rng(seed)
+    ro = ro * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, ro.shape))
+    g.cell_type = np.where(i < rows[0], GEL_C_1, GEL_C_2).astype(np.int8)
+    prim0 = np.ascontiguousarray(np.stack([ro, p, u, v, Q]))
+    return TrackedCase(name, g, prm, t, rails, prim0, steps)
+
+
+def tracked_spherical_ts(s: float, theta1: float = 2.0 * math.pi / 3.0):
+    """Key radii with a spherical inner shock at 0.25 (contact and outer shock as in
+    tracked_key_radii)."""
+    _, hp, bs = tracked_key_radii(s, theta1)
+    return 0.25, hp, bs

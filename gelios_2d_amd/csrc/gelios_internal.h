@@ -30,9 +30,25 @@ struct SweepArgs {
     const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
     const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
     const double* __restrict__ geo;   // static grids: the geometry table (geometry_kernel)
+    int srow[3];                  // node rows of the fitted surfaces, -1 where none
     unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
+};
+
+// Device view of the fitted-surface tracking (tracking.hip).
+struct TrackDev {
+    int rows[3];                  // node rows of the contact, inner shock, outer shock
+    int outer_faces, average_lines, n_inner, m4;
+    double koef, k_line[3];       // speed factors of the three lines
+    double tang_inner, tang_outer;
+    double r_inner, r_mid, r_outer;
+    const double* cs;             // [nj+1] cos, sin of the rail angles
+    const double* sn;
+    const double* f_in;           // [n_inner] (j/n_inner)^p_inner
+    const double* f_mid;          // [row_inner_shock - n_inner] ((j-n_inner)/(M1-1-n_inner))^p_mid
+    const double* f_out;          // [nj+1][M4] ((j+1)/M4)^(p_outer (1 - (s/dd)^2))
+    double* face;                 // [3][nj][6] node contributions of every surface face
 };
 
 }  // namespace gel
@@ -48,6 +64,13 @@ hipError_t gel_refresh_launch(hipStream_t s, const gel::SweepArgs* d_args, const
                               gel_mlp* m, bool f32, double tol, int judge, unsigned long long* d_nfall);
 // tiles the refresh of an ni x nj case takes
 inline long long gel_refresh_tiles(int ni, int nj) { return ((long long)ni * nj + 255) / 256; }
+
+struct gel_ctx;
+namespace gel {
+// One tracked step (tracking.hip): surface node speeds, rail redistribution into
+// ctx->d_x2/d_y2, and the sweep with surface faces; asynchronous on ctx->stream.
+hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a);
+}
 
 struct gel_ctx {
     int ni = 0, nj = 0;
@@ -69,6 +92,10 @@ struct gel_ctx {
     int8_t* d_qflag = nullptr;     // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
     double* d_geo = nullptr;       // geometry table of the static grid (built on first use)
     bool geo_valid = false;        // false after the nodes move
+    bool tracking = false;         // fitted-surface tracking on (gel_set_tracking)
+    gel_tracking trk{};
+    gel::TrackDev tdev{};
+    double* d_trk = nullptr;       // tables and face contributions of tdev
     unsigned long long* d_dt = nullptr;   // [3]
     unsigned long long* d_cnt = nullptr;  // [3]
     long long steps = 0;

@@ -124,6 +124,7 @@ static void free_ctx(gel_ctx* c)
     (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src); (void)hipFree(c->d_qtab);
     (void)hipFree(c->d_qflag);
     (void)hipFree(c->d_geo);
+    (void)hipFree(c->d_trk);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -314,10 +315,101 @@ int gel_set_rows_per_wave(gel_ctx* c, int rows)
 int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
 {
     if (!c || period < 1 || !(amp >= 0.0) || !(amp < 0.5)) return fail(GEL_E_ARG, "motion: amp in [0,0.5), period >= 1");
+    if (c->tracking) return fail(GEL_E_STATE, "fitted-surface tracking is on");
     c->motion.assign((size_t)period, 1.0);
     const double pi = 3.14159265358979323846;
     for (int k = 0; k < period; k++) c->motion[k] = 1.0 + amp * std::sin(2.0 * pi * k / period + phase);
     c->motion_step = 0;
+    return GEL_OK;
+}
+
+void gel_tracking_reference(gel_tracking* t)
+{
+    if (!t) return;
+    std::memset(t, 0, sizeof(*t));
+    const double RR = 390.222;                         // Help.h:75
+    t->row_inner_shock = t->row_contact = t->row_outer_shock = -1;
+    t->outer_faces = 1 << 30;
+    t->average_lines = 0;
+    t->n_inner = 20;                                   // Help.h:66
+    t->r_inner = 1.0 / RR;                             // R1_, R111_, R5_ (Help.h:76-83)
+    t->r_mid = 50.0 / RR;
+    t->r_outer = 2200.0 / RR;
+    t->p_inner = 1.3; t->p_mid = 1.05; t->p_outer = 1.76; t->s_outer = 1.5708;   // Setka.cpp:4349-4380
+    t->koef = 0.1;                                     // Setka.cpp:3380
+    t->k_contact = 0.3; t->k_inner = 0.1; t->k_outer = 1.0;
+    t->tang_inner = 0.1; t->tang_outer = 0.01;
+}
+
+int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
+{
+    if (!c) return fail(GEL_E_ARG, "null argument");
+    if (!t) { c->tracking = false; return GEL_OK; }
+    if (!rail) return fail(GEL_E_ARG, "rail angles needed");
+    if (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64)
+        return fail(GEL_E_UNSUPPORTED, "fitted surfaces: gas physics in FP64");
+    if (c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)
+        return fail(GEL_E_UNSUPPORTED, "fitted surfaces: no source or the MK1 source");
+    if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
+    const int ni = c->ni, nj = c->nj;
+    const int r0 = t->row_inner_shock, r1 = t->row_contact, r2 = t->row_outer_shock;
+    if (nj < 2 || t->n_inner < 2 || r0 < t->n_inner || r0 < 2 || !(r0 < r1 && r1 < r2) || r2 > ni - 2)
+        return fail(GEL_E_ARG, "surface rows: n_inner <= inner shock < contact < outer shock <= ni - 2");
+    if (t->outer_faces < 0) return fail(GEL_E_ARG, "outer_faces must be >= 0");
+    const int M1 = r0 + 1, M4 = ni - r2;
+    const int nmid = M1 - 1 - t->n_inner;
+    const size_t n_cs = (size_t)(nj + 1), n_in = (size_t)t->n_inner, n_mid = (size_t)(nmid > 0 ? nmid : 0);
+    const size_t n_out = (size_t)(nj + 1) * M4, n_face = (size_t)3 * nj * 6;
+    std::vector<double> h(2 * n_cs + n_in + n_mid + n_out);
+    double* cs = h.data();
+    double* sn = cs + n_cs;
+    double* fin = sn + n_cs;
+    double* fmid = fin + n_in;
+    double* fout = fmid + n_mid;
+    // host libm, the functions the reference calls at run time (Setka.cpp:4331-4380)
+    const double dd = t->s_outer / std::sqrt(1.0 - 1.0 / t->p_outer);
+    for (int k = 0; k <= nj; k++) {
+        cs[k] = std::cos(rail[k]);
+        sn[k] = std::sin(rail[k]);
+        const double ex = t->p_outer * (1.0 - (rail[k] / dd) * (rail[k] / dd));
+        for (int j = 0; j < M4; j++) fout[(size_t)k * M4 + j] = std::pow((j + 1) * 1.0 / (M4), ex);
+    }
+    for (int j = 0; j < t->n_inner; j++) fin[j] = std::pow(j * 1.0 / (t->n_inner), t->p_inner);
+    for (int j = t->n_inner; j < M1 - 1; j++)
+        fmid[j - t->n_inner] = std::pow((j - t->n_inner) * 1.0 / (M1 - 1 - t->n_inner), t->p_mid);
+    (void)hipFree(c->d_trk);
+    c->d_trk = nullptr;
+    c->tracking = false;
+    if (hipMalloc(&c->d_trk, (h.size() + n_face) * sizeof(double)) != hipSuccess)
+        return fail(GEL_E_NOMEM, "tracking tables");
+    HIP_OK(hipMemcpy(c->d_trk, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    TrackDev& d = c->tdev;
+    d.rows[0] = r1; d.rows[1] = r0; d.rows[2] = r2;
+    d.outer_faces = t->outer_faces;
+    d.average_lines = t->average_lines ? 1 : 0;
+    d.n_inner = t->n_inner;
+    d.m4 = M4;
+    d.koef = t->koef;
+    d.k_line[0] = t->k_contact; d.k_line[1] = t->k_inner; d.k_line[2] = t->k_outer;
+    d.tang_inner = t->tang_inner; d.tang_outer = t->tang_outer;
+    d.r_inner = t->r_inner; d.r_mid = t->r_mid; d.r_outer = t->r_outer;
+    d.cs = c->d_trk;
+    d.sn = d.cs + n_cs;
+    d.f_in = d.sn + n_cs;
+    d.f_mid = d.f_in + n_in;
+    d.f_out = d.f_mid + n_mid;
+    d.face = c->d_trk + h.size();
+    c->trk = *t;
+    c->tracking = true;
+    return GEL_OK;
+}
+
+int gel_get_nodes(gel_ctx* c, double* x, double* y)
+{
+    if (!c || !x || !y) return fail(GEL_E_ARG, "null argument");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(x, c->d_x, c->nn * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(y, c->d_y, c->nn * sizeof(double), hipMemcpyDeviceToHost));
     return GEL_OK;
 }
 
@@ -357,7 +449,8 @@ static int ensure_geometry(gel_ctx* c)
     return GEL_OK;
 }
 
-// moving: 0 static, 1 explicit next positions in d_x2/d_y2, 2 prescribed motion
+// moving: 0 static, 1 explicit next positions in d_x2/d_y2, 2 prescribed motion,
+// 3 fitted-surface tracking (next positions computed into d_x2/d_y2, then committed)
 static int one_step(gel_ctx* c, int moving)
 {
     if (moving == 0) {
@@ -385,6 +478,8 @@ static int one_step(gel_ctx* c, int moving)
     a.qtab = c->d_qtab;
     a.qflag = c->d_qflag;
     a.geo = c->d_geo;
+    a.srow[0] = a.srow[1] = a.srow[2] = -1;
+    if (moving == 3) { a.srow[0] = c->trk.row_inner_shock; a.srow[1] = c->trk.row_contact; a.srow[2] = c->trk.row_outer_shock; }
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
     a.prm = c->prm;
@@ -398,7 +493,7 @@ static int one_step(gel_ctx* c, int moving)
         HIP_OK(hipEventCreate(&e1));
         HIP_OK(hipEventRecord(e0, c->stream));
     }
-    hipError_t e = launch(c, a, moving != 0);
+    hipError_t e = (moving == 3) ? launch_tracked_step(c, a) : launch(c, a, moving != 0);
     if (e != hipSuccess) return fail(GEL_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
     if (c->timing) {
         HIP_OK(hipEventRecord(e1, c->stream));
@@ -407,6 +502,11 @@ static int one_step(gel_ctx* c, int moving)
     c->cur ^= 1;
     c->steps++;
     if (moving == 2) c->motion_step++;
+    if (moving == 3) {                 // Point::x = x2 (Setka.cpp:10473-10480)
+        std::swap(c->d_x, c->d_x2);
+        std::swap(c->d_y, c->d_y2);
+        c->geo_valid = false;
+    }
     return GEL_OK;
 }
 
@@ -464,7 +564,7 @@ int gel_step(gel_ctx* c, int nsteps, gel_stats* out)
 {
     if (!c || nsteps < 0) return fail(GEL_E_ARG, "bad argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
-    const int mode = c->motion.empty() ? 0 : 2;
+    const int mode = c->tracking ? 3 : (c->motion.empty() ? 0 : 2);
     for (int s = 0; s < nsteps; s++) {
         const int st = one_step(c, mode);
         if (st != GEL_OK) return st;
@@ -477,6 +577,7 @@ int gel_step_moving(gel_ctx* c, const double* x2, const double* y2, gel_stats* o
     if (!c || !x2 || !y2) return fail(GEL_E_ARG, "null argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
     if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
+    if (c->tracking) return fail(GEL_E_STATE, "fitted-surface tracking moves the nodes");
     const size_t nb = c->nn * sizeof(double);
     HIP_OK(hipMemcpyAsync(c->d_x2, x2, nb, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->d_y2, y2, nb, hipMemcpyHostToDevice, c->stream));
@@ -664,6 +765,7 @@ static int batch_upload_args(gel_batch* b)
             a.next = c->d_state[cur ^ 1];
             a.ctype = nullptr; a.otype = c->d_otype;
             a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag; a.geo = c->d_geo;
+            a.srow[0] = a.srow[1] = a.srow[2] = -1;
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
             if (p == kCycle) a.next = c->d_state[c->cur];
@@ -743,7 +845,8 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c || !c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->has_types || !c->motion.empty()) return fail(GEL_E_UNSUPPORTED, "batched cases are static grids without C_1/C_centr");
+        if (c->has_types || !c->motion.empty() || c->tracking)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are static grids without C_1/C_centr");
         if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
             c->prm.source_mode != p0.source_mode)
             return fail(GEL_E_UNSUPPORTED, "batched cases share physics, precision, flux and source mode");

@@ -189,6 +189,54 @@ GEL_HD T hllc2d(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T gam, T P[4]
     return UU;
 }
 
+// HLLC with the reference's nul_potok option (Setka.cpp:10644-10653): the star states
+// of both sides take the tangential velocity UZ2/UZ0 of the HLL average state.  The
+// sweep sets it on faces with exactly one node on a fitted surface
+// (Setka.cpp:10300-10327).  Supersonic faces take the upwind flux as without it.
+template <class T>
+GEL_HD T hllc2d_nul(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T gam, T P[4], T& PQ)
+{
+    const T t1 = n2, t2 = -n1;
+    const T u1 = L.u * n1 + L.v * n2, v1 = L.u * t1 + L.v * t2;
+    const T u2 = R.u * n1 + R.v * n2, v2 = R.u * t1 + R.v * t2;
+    const T cL = sqrt(gam * L.p / L.ro);
+    const T cR = sqrt(gam * R.p / R.ro);
+    const T SL = mn(u1, u2) - mx(cL, cR);
+    const T SR = mx(u1, u2) + mx(cL, cR);
+    if (SL >= W || SR <= W) return hllc2d<T, true>(L, R, W, n1, n2, gam, P, PQ);
+    const T suR = SR - u2, suL = SL - u1;
+    const T SM = (suR * R.ro * u2 - suL * L.ro * u1 - R.p + L.p) / (suR * R.ro - suL * L.ro);
+    const T g1 = gam - T(1);
+    const T kL = (sq(L.u) + sq(L.v)) / T(2), kR = (sq(R.u) + sq(R.v)) / T(2);
+    const T eL = L.p / g1 + L.ro * kL, eR = R.p / g1 + R.ro * kR;
+    const T FL[5] = { L.ro * u1, L.ro * u1 * u1 + L.p, L.ro * u1 * v1, (eL + L.p) * u1, L.Q * u1 };
+    const T FR[5] = { R.ro * u2, R.ro * u2 * u2 + R.p, R.ro * u2 * v2, (eR + R.p) * u2, R.Q * u2 };
+    const T UL[5] = { L.ro, L.ro * u1, L.ro * v1, eL, L.Q };
+    const T UR[5] = { R.ro, R.ro * u2, R.ro * v2, eR, R.Q };
+    const T UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) / (SR - SL);
+    const T UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) / (SR - SL);
+    const T vv = UZ2 / UZ0;
+    const T suRm = suR / (SR - SM), suLm = suL / (SL - SM);
+    const T ptz = ((R.p + R.ro * suR * (SM - u2)) + (L.p + L.ro * suL * (SM - u1))) / T(2);
+    T PO[5];
+    if (SM >= W) {
+        const T r = L.ro * (SL - u1) / (SL - SM);
+        const T ULL[5] = { r, r * SM, r * vv, eL * suLm + (ptz * SM - L.p * u1) / (SL - SM), L.Q * (SL - u1) / (SL - SM) };
+        for (int k = 0; k < 5; k++) PO[k] = FL[k] + SL * ULL[k] - SL * UL[k] - W * ULL[k];
+    } else {
+        const T r = R.ro * (SR - u2) / (SR - SM);
+        const T URR[5] = { r, r * SM, r * vv, eR * suRm + (ptz * SM - R.p * u2) / (SR - SM), R.Q * (SR - u2) / (SR - SM) };
+        for (int k = 0; k < 5; k++) PO[k] = FR[k] + SR * URR[k] - SR * UR[k] - W * URR[k];
+    }
+    P[1] = n1 * PO[1] + t1 * PO[2];
+    P[2] = n2 * PO[1] + t2 * PO[2];
+    P[0] = PO[0];
+    P[3] = PO[3];
+    PQ = PO[4];
+
+    return mx(fabs(SL), fabs(SR));
+}
+
 // Flux into the Gas-independent form used by the batch test entry point.
 template <class T>
 GEL_HD T hllc2d_full(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T rad, T cfl, T gam,
@@ -211,36 +259,7 @@ GEL_HD T hllc2d_full(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T rad, T
     } else if (!nul_potok || SL >= W || SR <= W) {
         UU = hllc2d<T, true>(L, R, W, n1, n2, gam, P, PQ);
     } else {
-        // nul_potok: tangential star velocity UZ2/UZ0 on both sides (Setka.cpp:10644-10653)
-        const T g1 = gam - T(1);
-        const T kL = (sq(L.u) + sq(L.v)) / T(2), kR = (sq(R.u) + sq(R.v)) / T(2);
-        const T eL = L.p / g1 + L.ro * kL, eR = R.p / g1 + R.ro * kR;
-        const T FL[5] = { L.ro * u1, L.ro * u1 * u1 + L.p, L.ro * u1 * v1, (eL + L.p) * u1, L.Q * u1 };
-        const T FR[5] = { R.ro * u2, R.ro * u2 * u2 + R.p, R.ro * u2 * v2, (eR + R.p) * u2, R.Q * u2 };
-        const T UL[5] = { L.ro, L.ro * u1, L.ro * v1, eL, L.Q };
-        const T UR[5] = { R.ro, R.ro * u2, R.ro * v2, eR, R.Q };
-        const T SM = Vc;
-        const T UZ0 = (SR * UR[0] - SL * UL[0] + FL[0] - FR[0]) / (SR - SL);
-        const T UZ2 = (SR * UR[2] - SL * UL[2] + FL[2] - FR[2]) / (SR - SL);
-        const T vv = UZ2 / UZ0;
-        const T suRm = suR / (SR - SM), suLm = suL / (SL - SM);
-        const T ptz = ((R.p + R.ro * suR * (SM - u2)) + (L.p + L.ro * suL * (SM - u1))) / T(2);
-        T PO[5];
-        if (SM >= W) {
-            const T r = L.ro * (SL - u1) / (SL - SM);
-            const T ULL[5] = { r, r * SM, r * vv, eL * suLm + (ptz * SM - L.p * u1) / (SL - SM), L.Q * (SL - u1) / (SL - SM) };
-            for (int k = 0; k < 5; k++) PO[k] = FL[k] + SL * ULL[k] - SL * UL[k] - W * ULL[k];
-        } else {
-            const T r = R.ro * (SR - u2) / (SR - SM);
-            const T URR[5] = { r, r * SM, r * vv, eR * suRm + (ptz * SM - R.p * u2) / (SR - SM), R.Q * (SR - u2) / (SR - SM) };
-            for (int k = 0; k < 5; k++) PO[k] = FR[k] + SR * URR[k] - SR * UR[k] - W * URR[k];
-        }
-        P[1] = n1 * PO[1] + t1 * PO[2];
-        P[2] = n2 * PO[1] + t2 * PO[2];
-        P[0] = PO[0];
-        P[3] = PO[3];
-        PQ = PO[4];
-        UU = mx(fabs(SL), fabs(SR));
+        UU = hllc2d_nul<T>(L, R, W, n1, n2, gam, P, PQ);
     }
     return cfl * rad / UU;
 }
@@ -589,9 +608,11 @@ GEL_HD int godunov_newton(const GodunovK& K, double enI, double pI, double rI, d
 }
 
 // q1, q2: (ro, p, u, v, w); q out: mass, momentum x/y/z, energy fluxes through the face
+// uu_out (optional): the wave speed the returned time step is formed from, floored at
+// 1e-7 as there, so that cfl * dist / *uu_out equals the returned time.
 GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, double n2, double n3, double w,
                             double dist, double cfl, double gam, double* q, double& dsl, double& dsp, double& dsc,
-                            int& status)
+                            int& status, double* uu_out = nullptr)
 {
     const GodunovK K = godunov_k(gam);
     const double ga = K.ga, g1 = K.g1, g2 = K.g2, gm = K.gm;
@@ -630,7 +651,11 @@ GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, doubl
     auto fan_a = [&](double rs, double cs, double ps, double P) {
         return (fabs(P - ps) < 10e-10) ? rs * cs : gm * rs * cs * (1.0 - P / ps) / (1.0 - pow((P / ps), gm));
     };
-    auto time_of = [&](double UU) { return (UU > 10e-8) ? cfl * dist / UU : cfl * dist / 10e-8; };
+    double uu_eff = 10e-8;
+    auto time_of = [&](double UU) {
+        uu_eff = (UU > 10e-8) ? UU : 10e-8;
+        return (UU > 10e-8) ? cfl * dist / UU : cfl * dist / 10e-8;
+    };
     auto state_I = [&]() { en = enI; p = pI; r = rI; te2 = teI2; te3 = teI3; };
     auto state_II = [&]() { en = enII; p = pII; r = rII; te2 = teII2; te3 = teII3; };
 
@@ -737,6 +762,7 @@ GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, doubl
     q[2] = (r * (en - w) * vo + be * p);
     q[3] = (r * (en - w) * wo + ge * p);
     q[4] = ((en - w) * eo + en * p);
+    if (uu_out) *uu_out = uu_eff;
     return time;
 }
 

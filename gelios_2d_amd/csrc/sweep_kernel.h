@@ -443,6 +443,64 @@ __device__ __forceinline__ void interior_states(const CellT<Ph, TS>& M, const Ce
     side_value<Ph, TS, TF, TYPES>(N, M, U, n_axis, f.fx, f.fy, f.dis, gam, R, bad);
 }
 
+// ------------------------------------------------------------ fitted surfaces ---
+// Faces between two nodes of one fitted surface (the radial faces of a surface row)
+// and faces with one node on a surface (Setka.cpp:10287-10370).  Gas, FP64.
+
+// Gran::Get_par_TVD on a surface face (Gran.cpp:345-426): the straight line through
+// D (behind the master) and M at the face centre; a non-positive density or pressure
+// takes M's value.
+template <class Ph>
+__device__ __forceinline__ void tvd_surface(const CellT<Ph, double>& M, const CellT<Ph, double>& D, double fx,
+                                            double fy, double* r)
+{
+    const double d2 = dist2d(fx, fy, M.cx, M.cy);
+    const double d1 = dist2d(fx, fy, D.cx, D.cy);
+    const double x1 = -d1, x2 = -d2;
+#pragma unroll
+    for (int k = 0; k < Ph::NV; k++) {
+        const double d = (D.s[k] - M.s[k]) / (x1 - x2);            // Help.cpp:115-120, y = 0
+        r[k] = d * (0.0 - x2) + M.s[k];
+    }
+    if (r[0] <= 0.0) r[0] = M.s[0];
+    if (r[1] <= 0.0) r[1] = M.s[1];
+}
+
+// Both sides of a surface face: C_1 sides as on any face (rescaled, radial
+// reconstruction), the others one-sided.
+template <class Ph, bool TYPES>
+__device__ __forceinline__ void surface_states(const CellT<Ph, double>& M, const CellT<Ph, double>& N,
+                                               const CellT<Ph, double>& D, const CellT<Ph, double>& U,
+                                               bool axis, const Face& f, double gam, double* L, double* R, int& bad)
+{
+    if (TYPES && M.type == GEL_C_1) side_value<Ph, double, double, TYPES>(M, N, D, axis, f.fx, f.fy, f.dis, gam, L, bad);
+    else tvd_surface<Ph>(M, D, f.fx, f.fy, L);
+    if (TYPES && N.type == GEL_C_1) side_value<Ph, double, double, TYPES>(N, M, U, axis, f.fx, f.fy, f.dis, gam, R, bad);
+    else tvd_surface<Ph>(N, U, f.fx, f.fy, R);
+}
+
+// Flux through a surface face (Setka.cpp:10335-10370): mass, momentum and energy from
+// the exact solver at the face speed, the tracer from the HLL/HLLC flux; returns the
+// larger of the two wave-speed maxima, from which cfl * dist / max equals the smaller
+// of the two time steps the reference takes.
+template <bool HLLC>
+__device__ __forceinline__ double surface_flux(const double* L, const double* R, const Face& f, double gam,
+                                               double* P, int& bad)
+{
+    const double q1[5] = { L[0], L[1], L[2], L[3], 0.0 };
+    const double q2[5] = { R[0], R[1], R[2], R[3], 0.0 };
+    double q[5], dsl, dsp, dsc, uug = 0.0;
+    int st = 0;
+    godunov_exact(q1, q2, f.n1, f.n2, 0.0, f.W, 1.0, 1.0, gam, q, dsl, dsp, dsc, st, &uug);
+    if (st) bad += 1;
+    const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
+    const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
+    double Ph_[4], PQ;
+    const double uuh = hllc2d<double, HLLC>(gl, gr, f.W, f.n1, f.n2, gam, Ph_, PQ);
+    P[0] = q[0]; P[1] = q[1]; P[2] = q[2]; P[3] = q[4]; P[4] = PQ;
+    return mx(uug, uuh);
+}
+
 // Conservative ALE update of one cell and primitive recovery (Setka.cpp:10423-10460),
 // r-weighted: Volume = int y dA before, Volume2 after the step; Ar = planar area.
 // MHD adds the axisymmetric geometric sources (hoop stress with field and swirl,
@@ -620,6 +678,15 @@ __global__ void geometry_kernel(int ni, int nj, const double* __restrict__ X, co
     }
 }
 
+// Flag or-ed into the FLUX template argument of the sweep on grids with fitted surfaces.
+constexpr int kTrackFlux = 16;
+
+// Node row r is one of the fitted surfaces (SweepArgs::srow, -1 where unused).
+__device__ __forceinline__ bool surface_row(const SweepArgs& a, int r)
+{
+    return r == a.srow[0] || r == a.srow[1] || r == a.srow[2];
+}
+
 // ---------------------------------------------------------------- marching tile ---
 //
 // A workgroup of NW wavefronts owns a strip of 64 columns (60 output columns and
@@ -662,6 +729,11 @@ template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FL
 __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, int bx,
                                            int i0, int i1, MarchAcc& acc)
 {
+    // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
+    constexpr bool TRACK = (FLUX & kTrackFlux) != 0;
+    constexpr int FX = FLUX & ~kTrackFlux;
+    static_assert(!TRACK || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && MOVING),
+                  "fitted surfaces: gas, FP64, moving grids");
     constexpr int NV = Ph::NV;
     constexpr int NR = NW + 3;
     using C = CellT<Ph, TS>;
@@ -799,15 +871,24 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 }
                 double L[NV], R[NV], P[NV];
                 int b = 0;
+                // a face of a fitted surface: both nodes on surface row i + 1
+                const bool god = TRACK && !inner && i + 1 < ni && surface_row(a, i + 1);
                 if (inner) {
                     boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                 } else if (i + 1 >= ni) {
                     const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
                     boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
+                } else if (god) {
+                    if constexpr (TRACK) surface_states<Ph, TYPES>(M, N, D, U, m_axis, f, gam, L, R, b);
                 } else {
                     interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
                 }
-                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+                TF uu;
+                if constexpr (TRACK) {
+                    uu = god ? surface_flux<FX == GEL_FLUX_HLLC>(L, R, f, gam, P, b) : face_flux<Ph, TF, FX>(L, R, f, gam, P);
+                } else {
+                    uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
+                }
                 const int fs = fslot(i);
 #pragma unroll
                 for (int k = 0; k < NV; k++) S.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
@@ -834,7 +915,21 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
                 else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
                 else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
-                const TF uu = face_flux<Ph, TF, FLUX>(L, R, f, gam, P);
+                TF uu;
+                if constexpr (TRACK && FX == GEL_FLUX_HLLC) {
+                    // one node of the face on a surface row: nul_potok (Setka.cpp:10299-10327)
+                    if (surface_row(a, i) || surface_row(a, i + 1)) {
+                        const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
+                        const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
+                        double PQ;
+                        uu = hllc2d_nul<double>(gl, gr, f.W, f.n1, f.n2, gam, P, PQ);
+                        P[4] = PQ;
+                    } else {
+                        uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
+                    }
+                } else {
+                    uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
+                }
 #pragma unroll
                 for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * f.S;
                 t_pol = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);

@@ -160,3 +160,54 @@ def pack_general(g: PolarGrid) -> GeneralMesh:
                     else:
                         ft[s] = int(g.outer_type[j])
     return GeneralMesh(q, g.cell_type.astype(np.int32), np.full(nc, 4, np.int32), fa, fb, ft, fn, fd, fu)
+
+
+def rail_radii(t, blocks, s, R2, R3, R4):
+    """Node radii along one rail at angle s with key radii R2 (inner shock), R3
+    (contact), R4 (outer shock), by the spacing of Setka::Move_Setka_Calculate_2
+    (Setka.cpp:4349-4380); blocks = (M1, M2, M3, M4) nodes per block as on the
+    reference's rails (Rail.cpp:19-75)."""
+    M1, M2, M3, M4 = blocks
+    r = np.empty(M1 + M2 + M3 + M4)
+    n = t.n_inner
+    for j in range(n):
+        r[j] = t.r_inner + (j * 1.0 / n) ** t.p_inner * (t.r_mid - t.r_inner)
+    for j in range(n, M1 - 1):
+        r[j] = t.r_mid + ((j - n) * 1.0 / (M1 - 1 - n)) ** t.p_mid * (R2 - t.r_mid)
+    r[M1 - 1] = R2
+    for j in range(M2 - 1):
+        r[M1 + j] = R2 + (R3 - R2) * (j + 1) / M2
+    r[M1 + M2 - 1] = R3
+    for j in range(M3 - 1):
+        r[M1 + M2 + j] = R3 + (R4 - R3) * (j + 1) / M3
+    r[M1 + M2 + M3 - 1] = R4
+    dd = t.s_outer / np.sqrt(1.0 - 1.0 / t.p_outer)
+    for j in range(M4):
+        r[M1 + M2 + M3 + j] = R4 + ((j + 1) * 1.0 / M4) ** (t.p_outer * (1.0 - (s / dd) ** 2)) * (t.r_outer - R4)
+    return r
+
+
+def tracked_polar_grid(nj: int, blocks, key_radii, t, theta1: float = 2.0 * np.pi / 3.0):
+    """A polar grid whose node columns are rails at angles theta1*c/nj and whose node
+    rows M1-1, M1+M2-1, M1+M2+M3-1 are the inner shock, the contact and the outer
+    shock (the reference's A rails, Rail.cpp:19-75).  key_radii(s) -> (R2, R3, R4).
+    The outer-region exponent p_outer*(1 - (s/dd)^2) stays positive for
+    s < dd = s_outer/sqrt(1 - 1/p_outer) (about 137 degrees with the reference's
+    constants), which bounds theta1.  Returns (grid, rail angles, rows)."""
+    M1, M2, M3, M4 = blocks
+    ni = M1 + M2 + M3 + M4 - 1
+    rails = theta1 * np.arange(nj + 1, dtype=np.float64) / nj
+    dd = t.s_outer / np.sqrt(1.0 - 1.0 / t.p_outer)
+    if not theta1 < dd:
+        raise ValueError(f"rails beyond {dd:.4f} rad have a non-positive outer exponent")
+    x = np.empty((ni + 1, nj + 1))
+    y = np.empty((ni + 1, nj + 1))
+    for c, s in enumerate(rails):
+        R2, R3, R4 = key_radii(s)
+        r = rail_radii(t, blocks, s, R2, R3, R4)
+        x[:, c] = r * np.cos(s)
+        y[:, c] = r * np.sin(s)
+    y[:, 0] = 0.0
+    g = PolarGrid(ni, nj, x.ravel(), y.ravel())
+    rows = (M1 - 1, M1 + M2 - 1, M1 + M2 + M3 - 1)
+    return g, rails, rows

@@ -115,3 +115,61 @@ def heliosphere_params(*, flux: int = GEL_FLUX_HLLC, cfl: float = 0.4, rho_in: f
     p.inf_rho, p.inf_p, p.inf_u, p.inf_v, p.inf_q = rho_inf, p_inf, -u_inf, 0.0, 0.0
     return p
 
+
+
+N_INNER = 20            # rail nodes of the inner block, Help.h:66
+N_OUTER_SHOCK = 13      # Help.h:67
+
+
+class GelTracking(ctypes.Structure):
+    """Field-for-field mirror of ``struct gel_tracking`` (fitted-surface tracking)."""
+
+    _fields_ = [
+        ("row_inner_shock", ctypes.c_int),
+        ("row_contact", ctypes.c_int),
+        ("row_outer_shock", ctypes.c_int),
+        ("outer_faces", ctypes.c_int),
+        ("average_lines", ctypes.c_int),
+        ("n_inner", ctypes.c_int),
+        ("r_inner", ctypes.c_double),
+        ("r_mid", ctypes.c_double),
+        ("r_outer", ctypes.c_double),
+        ("p_inner", ctypes.c_double),
+        ("p_mid", ctypes.c_double),
+        ("p_outer", ctypes.c_double),
+        ("s_outer", ctypes.c_double),
+        ("koef", ctypes.c_double),
+        ("k_contact", ctypes.c_double),
+        ("k_inner", ctypes.c_double),
+        ("k_outer", ctypes.c_double),
+        ("tang_inner", ctypes.c_double),
+        ("tang_outer", ctypes.c_double),
+    ]
+
+    def copy(self) -> "GelTracking":
+        t = GelTracking()
+        ctypes.pointer(t)[0] = self
+        return t
+
+    @property
+    def rows(self):
+        return self.row_inner_shock, self.row_contact, self.row_outer_shock
+
+
+def reference_tracking(rows=(-1, -1, -1), outer_faces: int = 1 << 30) -> GelTracking:
+    """The constants of Setka::Move_surface (Setka.cpp:3378-3844) and of the A rails of
+    Setka::Move_Setka_Calculate_2 (Setka.cpp:4324-4381), with the Help.h radii.
+    rows = node rows of the inner shock, the contact and the outer shock."""
+    t = GelTracking()
+    t.row_inner_shock, t.row_contact, t.row_outer_shock = rows
+    t.outer_faces = outer_faces
+    t.average_lines = 0
+    t.n_inner = N_INNER
+    t.r_inner = 1.0 / RR_            # R1_
+    t.r_mid = 50.0 / RR_             # R111_
+    t.r_outer = 2200.0 / RR_         # R5_
+    t.p_inner, t.p_mid, t.p_outer, t.s_outer = 1.3, 1.05, 1.76, 1.5708
+    t.koef = 0.1
+    t.k_contact, t.k_inner, t.k_outer = 0.3, 0.1, 1.0
+    t.tang_inner, t.tang_outer = 0.1, 0.01
+    return t

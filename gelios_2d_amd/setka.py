@@ -69,6 +69,27 @@ class Setka:
         + phase)), k = step mod period; steps become moving steps."""
         _lib.check(_lib.lib().gel_set_motion(self._ctx, float(amp), int(period), float(phase)))
 
+    def set_tracking(self, tracking, rails) -> None:
+        """Fitted-surface tracking (Setka::Move_surface, Setka.cpp:3378-3844, and the A
+        rails of Setka::Move_Setka_Calculate_2, Setka.cpp:4324-4381): every step moves
+        the three surface rows of ``tracking`` (a :class:`~.params.GelTracking`) and
+        redistributes the rails (node columns at angles ``rails``) before the sweep.
+        ``None`` turns it off."""
+        if tracking is None:
+            _lib.check(_lib.lib().gel_set_tracking(self._ctx, None, None))
+            return
+        r = np.ascontiguousarray(rails, np.float64)
+        if r.shape != (self.grid.nj + 1,):
+            raise ValueError("one rail angle per node column")
+        _lib.check(_lib.lib().gel_set_tracking(self._ctx, ctypes.byref(tracking), _lib.dptr(r)))
+
+    def get_nodes(self):
+        """Node positions now, (x, y) of (ni+1)*(nj+1) each."""
+        x = np.empty(self.grid.nnodes)
+        y = np.empty(self.grid.nnodes)
+        _lib.check(_lib.lib().gel_get_nodes(self._ctx, _lib.dptr(x), _lib.dptr(y)))
+        return x, y
+
     def get_source_table(self) -> np.ndarray:
         """The source table [5][ncells] (mass, momentum x, momentum y, energy, tracer)."""
         q = np.empty((5, self.ncells))

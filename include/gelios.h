@@ -193,6 +193,52 @@ int gel_step_moving(gel_ctx* ctx, const double* x2, const double* y2, gel_stats*
  * gel_step then runs moving steps. */
 int gel_set_motion(gel_ctx* ctx, double amp, int period, double phase);
 
+/* Fitted-surface tracking (Setka::Move_surface, Setka.cpp:3378-3844, and the A rails of
+ * Setka::Move_Setka_Calculate_2, Setka.cpp:4324-4381) on the polar grid.  Three node
+ * rows are fitted surfaces: the inner shock, the contact and the outer shock, in that
+ * radial order, as the rails' key points (Rail.cpp:19-75).  Every node column c is a
+ * rail, the ray at angle rail_angle[c] from the origin.  Each step then:
+ *   1. solves the exact Riemann problem on every surface face from the one-sided
+ *      reconstructions of both sides (Gran.cpp:345-426) and moves the surface nodes
+ *      with a fraction of a wave speed (contact: koef*k_contact*contact speed; inner
+ *      shock: koef*k_inner*left wave; outer shock: koef*k_outer*right wave), scattered
+ *      to the face's nodes by the reference's rules and end-node corrections;
+ *   2. moves every rail's key points along the rail by dt times that speed and
+ *      redistributes the rail's other nodes between them (power law near the inner
+ *      boundary and beyond the outer shock, uniform between the surfaces);
+ *   3. takes the moving step (Setka.cpp:10148-10470) with the exact solver on the
+ *      surface faces (mass, momentum, energy; the tracer from the HLL/HLLC flux,
+ *      Setka.cpp:10335-10370) and the nul_potok HLLC on faces with one surface node
+ *      (Setka.cpp:10299-10327), and commits the nodes (Setka.cpp:10473-10480).
+ * Gas physics in FP64 only.  The rows must satisfy n_inner <= row_inner_shock <
+ * row_contact < row_outer_shock <= ni - 2, and the grid's rows 0..row_inner_shock
+ * must lie on the rails (the step places them by the rail formulas). */
+typedef struct gel_tracking {
+    int row_inner_shock;    /* node rows of the fitted surfaces (P_Inner_shock, */
+    int row_contact;        /* P_Contact, P_Outer_shock; Line_Inner, Line_Contact, */
+    int row_outer_shock;    /* Line_Outer) */
+    int outer_faces;        /* outer-shock faces j < outer_faces move it (Setka.cpp:3737,
+                               n_outer_shock + N1 + N2 + 1 on the reference grid) */
+    int average_lines;      /* 0: node speeds averaged as the reference does (the contact
+                               line's nodes after each of the three loops,
+                               Setka.cpp:3527-3533, :3718-3724, :3829-3835, so the
+                               shocks' nodes keep sums); 1: each line averages its own */
+    int n_inner;            /* rail nodes in the inner block (n_inner = 20, Help.h:66) */
+    double r_inner, r_mid, r_outer;          /* R1_, R111_, R5_ (Help.h:76-83) */
+    double p_inner, p_mid, p_outer, s_outer; /* rail spacing: 1.3, 1.05, 1.76, 1.5708
+                                                (Setka.cpp:4349-4380) */
+    double koef;                             /* 0.1 (Setka.cpp:3380) */
+    double k_contact, k_inner, k_outer;      /* 0.3, 0.1, 1.0 */
+    double tang_inner, tang_outer;           /* tangential-speed thresholds 0.1, 0.01 */
+} gel_tracking;
+
+/* Fill *t with the reference's constants; rows -1 (to be set), all outer faces. */
+void gel_tracking_reference(gel_tracking* t);
+/* Turn tracking on for gel_step (t != NULL; rail_angle[nj+1]) or off (t == NULL). */
+int gel_set_tracking(gel_ctx* ctx, const gel_tracking* t, const double* rail_angle);
+/* Node positions now (the reference's Point::x, y) into x, y[(ni+1)*(nj+1)]. */
+int gel_get_nodes(gel_ctx* ctx, double* x, double* y);
+
 /* Override the time step the next step uses (T[now1], Setka.cpp:10143-10145). */
 int gel_set_dt(gel_ctx* ctx, double dt);
 

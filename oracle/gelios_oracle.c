@@ -11,11 +11,15 @@
  * (Help.cpp:3-120) and the charge-exchange source Cell::Get_Sourse_MK1
  * (Cell.cpp:498-665).  Expressions keep the reference's operation order.
  *
- * Not restated here (no call on the polar grids this build runs): the fitted-surface
- * branch of the sweep (exact Godunov solver on faces whose nodes are P_Contact /
- * P_Inner_shock / P_Outer_shock, Setka.cpp:10287-10370) and the radial-chain
- * re-interpolation of the MC moments in Get_Sourse_MK1 (Cell.cpp:528-589), which on a
- * static grid returns the cell's own moments.
+ * Fitted surfaces: the sweep's exact-Godunov faces between nodes of one surface and
+ * nul_potok faces beside them (Setka.cpp:10287-10370), the one-sided reconstruction on
+ * surface faces (Gran.cpp:345-426), and the surface tracking of Setka::Move_surface
+ * (Setka.cpp:3378-3844) with the A-rail redistribution of Move_Setka_Calculate_2
+ * (Setka.cpp:4324-4381) on the polar grid (ora_track_nodes).
+ *
+ * Not restated here: the radial-chain re-interpolation of the MC moments in
+ * Get_Sourse_MK1 (Cell.cpp:528-589), which on a static grid returns the cell's own
+ * moments, and the B/C/D rails of the reference's own block grid.
  */
 #include "gelios_oracle.h"
 
@@ -317,10 +321,31 @@ static ora_gas get_par(const sweep_ctx* S, int M, int slot, double fx, double fy
     }
 }
 
+/* Gran::Get_par_TVD on a face whose two nodes lie on one fitted surface
+ * (Gran.cpp:345-426): the straight line through down D and master M at the face centre,
+ * non-positive density or pressure replaced by the master's. */
+static ora_gas tvd_surface(const double* now, int nc, int M, int D, double xm, double ym, double xd, double yd,
+                           double fx, double fy)
+{
+    const ora_gas pm = { now[M], now[nc + M], now[2 * nc + M], now[3 * nc + M], now[4 * nc + M] };
+    const ora_gas pd = { now[D], now[nc + D], now[2 * nc + D], now[3 * nc + D], now[4 * nc + D] };
+    const double d2 = sqrt(sq(fx - xm) + sq(fy - ym));
+    const double d1 = sqrt(sq(fx - xd) + sq(fy - yd));
+    ora_gas r;
+    r.ro = ora_linear2(-d1, pd.ro, -d2, pm.ro, 0.0);
+    r.p = ora_linear2(-d1, pd.p, -d2, pm.p, 0.0);
+    r.u = ora_linear2(-d1, pd.u, -d2, pm.u, 0.0);
+    r.v = ora_linear2(-d1, pd.v, -d2, pm.v, 0.0);
+    r.Q = ora_linear2(-d1, pd.Q, -d2, pm.Q, 0.0);
+    if (r.ro <= 0.0) r.ro = pm.ro;
+    if (r.p <= 0.0) r.p = pm.p;
+    return r;
+}
+
 /* Gran::Get_par_TVD for an Usualy face (Gran.cpp:294-457): value of master M at the
- * face centre (fx,fy) from the line (down D, M, sosed N).  *bad set where the
- * reference exits. */
-static ora_gas tvd_usualy(const sweep_ctx* S, int M, int N, int D, double fx, double fy, int* bad)
+ * face centre (fx,fy) from the line (down D, M, sosed N); surf: the face's nodes lie on
+ * one fitted surface.  *bad set where the reference exits. */
+static ora_gas tvd_usualy(const sweep_ctx* S, int M, int N, int D, double fx, double fy, int surf, int* bad)
 {
     ora_gas pm = load(S, M);
     if (D < 0) {
@@ -335,6 +360,7 @@ static ora_gas tvd_usualy(const sweep_ctx* S, int M, int N, int D, double fx, do
         }
         return pm;                             /* :340-343 */
     }
+    if (surf) return tvd_surface(S->now, S->nc, M, D, S->cx[M], S->cy[M], S->cx[D], S->cy[D], fx, fy);
     ora_gas pd = load(S, D), pn = load(S, N);
     const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
     const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
@@ -451,6 +477,11 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
             const double dis = sqrt(sq(fx) + sq(fy));
             ora_gas p11 = par1;
             ora_gas p2 = get_par(&S, c, s, fx, fy);
+            /* fitted surfaces (Setka.cpp:10287-10327): both nodes on one surface -> exact
+               solver; exactly one node on a surface (or nodes on two) -> nul_potok */
+            const int pa = m->ptype ? m->ptype[A] : 0, pb = m->ptype ? m->ptype[B] : 0;
+            const int god = (pa != 0 && pa == pb);
+            const int np = (pa != pb);
 
             if (t == GEL_F_USUALY) {                    /* :10213-10251 */
                 const int N = m->fnbr[s];
@@ -461,7 +492,7 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
                     tvd_radial(&S, c, N, m->fdown[s], fx, fy, &ro, &pr);
                     p11.ro = ro; p11.p = pr;
                 } else {
-                    p11 = tvd_usualy(&S, c, N, m->fdown[s], fx, fy, &bad);
+                    p11 = tvd_usualy(&S, c, N, m->fdown[s], fx, fy, god, &bad);
                 }
                 if (N1) {
                     const double dd = sqrt(sq(cx[N]) + sq(cy[N]));
@@ -470,7 +501,7 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
                     tvd_radial(&S, N, c, m->fup[s], fx, fy, &ro, &pr);
                     p2.ro = ro; p2.p = pr;
                 } else {
-                    p2 = tvd_usualy(&S, N, c, m->fup[s], fx, fy, &bad);
+                    p2 = tvd_usualy(&S, N, c, m->fup[s], fx, fy, god, &bad);
                 }
             } else if (t == GEL_F_INNER_SPHERE) {      /* :10252-10257 */
                 rescale_c1(&p11, radius, dis, gam, x, y, fx, fy, 0);
@@ -485,7 +516,24 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
             }
 
             double P[4], PQ, Vl, Vc, Vp;
-            const double tf = ora_hllc2d(&p11, &p2, W, n1, n2, dist, prm->cfl, gam, prm->flux, 0,
+            if (god) {                                  /* :10335-10370 */
+                const double q1[5] = { p11.ro, p11.p, p11.u, p11.v, 0.0 };
+                const double q2[5] = { p2.ro, p2.p, p2.u, p2.v, 0.0 };
+                const double nv[3] = { n1, n2, 0.0 };
+                double qqq[5], dl, dp, dc;
+                int gst = 0;
+                time = mn(time, ora_godunov(q1, q2, nv, W, dist, prm->cfl, gam, qqq, &dl, &dp, &dc, &gst));
+                if (gst) bad = 1;
+                Pot[0] = Pot[0] + qqq[0] * S_;
+                Pot[1] = Pot[1] + qqq[1] * S_;
+                Pot[2] = Pot[2] + qqq[2] * S_;
+                Pot[3] = Pot[3] + qqq[4] * S_;
+                time = mn(time, ora_hllc2d(&p11, &p2, W, n1, n2, dist, prm->cfl, gam, prm->flux, np,
+                                           P, &PQ, &Vl, &Vc, &Vp));
+                Pot[4] = Pot[4] + PQ * S_;
+                continue;
+            }
+            const double tf = ora_hllc2d(&p11, &p2, W, n1, n2, dist, prm->cfl, gam, prm->flux, np,
                                          P, &PQ, &Vl, &Vc, &Vp);
             time = mn(time, tf);
             for (int v = 0; v < 4; v++) Pot[v] = Pot[v] + P[v] * S_;   /* :10372-10376 */
@@ -1117,4 +1165,192 @@ double ora_godunov(const double q1[5], const double q2[5], const double n[3], do
     qqq[3] = (r * (en - w) * wo + ge * p);
     qqq[4] = ((en - w) * eo + en * p);
     return time;
+}
+
+/* ------------------------------------------------------ fitted-surface tracking --- */
+
+/* Node speeds of the fitted surfaces (Setka::Move_surface, Setka.cpp:3378-3844) and the
+ * A-rail redistribution of Setka::Move_Setka_Calculate_2 (Setka.cpp:4324-4381) on the
+ * polar grid: each surface is the node row t->row_*, its line the radial faces of that
+ * row in the order of the polar angle (face j joins nodes (k, j) -> (k, j+1), master
+ * cell (k-1, j), Setka.cpp:1007-1045), each node column c is a rail at angle rail_s[c]. */
+
+typedef struct trk_acc { double* vx; double* vy; int* cnt; } trk_acc;
+
+/* Get_par_TVD of both sides of the surface face j of row k, with the inner shock's
+ * master side taken from the cell centre rescaled to the face (Setka.cpp:3556-3580). */
+static void trk_face(const gel_tracking* t, int line, int ni, int nj, int k, int j, const double* x, const double* y,
+                     const double* now, double gam, double* n1o, double* n2o, ora_gas* p1, ora_gas* p2)
+{
+    (void)t; (void)ni;
+    const int nc = ni * nj;
+    const int a = k * (nj + 1) + j, b = a + 1;
+    const double fx = (x[a] + x[b]) / 2.0, fy = (y[a] + y[b]) / 2.0;
+    double t1 = x[b] - x[a], t2 = y[b] - y[a];
+    const double nn = sqrt(sq(t1) + sq(t2));
+    t1 = t1 / nn; t2 = t2 / nn;
+    *n1o = t2; *n2o = -t1;
+    int q[4];
+    double cx[4], cy[4];               /* cells (k-2, j), (k-1, j), (k, j), (k+1, j) */
+    for (int r = 0; r < 4; r++) {
+        const int i = k - 2 + r;
+        q[0] = i * (nj + 1) + j; q[1] = (i + 1) * (nj + 1) + j; q[2] = q[1] + 1; q[3] = q[0] + 1;
+        ora_cell_center(x, y, q, &cx[r], &cy[r]);
+    }
+    const int D = (k - 2) * nj + j, M = (k - 1) * nj + j, N = k * nj + j, U = (k + 1) * nj + j;
+    if (line == 1) {                   /* inner shock: Setka.cpp:3556-3580 */
+        ora_gas g = { now[M], now[nc + M], now[2 * nc + M], now[3 * nc + M], now[4 * nc + M] };
+        const double radius = sqrt(sq(cx[1]) + sq(cy[1]));
+        const double dis = sqrt(sq(fx) + sq(fy));
+        g.ro = g.ro * sq(radius) / sq(dis);
+        g.p = g.p * pow(radius / dis, 2.0 * gam);
+        ora_polar_perenos(cx[1], cy[1], fx, fy, &g.u, &g.v);
+        *p1 = g;
+    } else {
+        *p1 = tvd_surface(now, nc, M, D, cx[1], cy[1], cx[0], cy[0], fx, fy);
+    }
+    *p2 = tvd_surface(now, nc, N, U, cx[2], cy[2], cx[3], cy[3], fx, fy);
+}
+
+static void trk_add(trk_acc* acc, int node, double v, double n1, double n2)
+{
+    acc->vx[node] += v * n1;
+    acc->vy[node] += v * n2;
+    acc->cnt[node]++;
+}
+
+static void trk_norm(trk_acc* acc, int node)
+{
+    if (acc->cnt[node] > 0) {
+        acc->vx[node] /= (1.0 * acc->cnt[node]);
+        acc->vy[node] /= (1.0 * acc->cnt[node]);
+    }
+}
+
+int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, const double* y, const double* now,
+                    const double* rail_s, double gam, double dt, double* x2, double* y2)
+{
+    const int nn = (ni + 1) * (nj + 1);
+    trk_acc acc;
+    acc.vx = (double*)calloc((size_t)nn, sizeof(double));
+    acc.vy = (double*)calloc((size_t)nn, sizeof(double));
+    acc.cnt = (int*)calloc((size_t)nn, sizeof(int));
+    if (!acc.vx || !acc.vy || !acc.cnt) { free(acc.vx); free(acc.vy); free(acc.cnt); return GEL_E_NOMEM; }
+    int bad = 0;
+    const int rows[3] = { t->row_contact, t->row_inner_shock, t->row_outer_shock };
+    for (int line = 0; line < 3; line++) {          /* contact, inner shock, outer shock */
+        const int k = rows[line];
+        const int base = k * (nj + 1);
+        for (int j = 0; j < nj; j++) {
+            if (line == 2 && j >= t->outer_faces) continue;     /* Setka.cpp:3737-3740 */
+            double n1, n2;
+            ora_gas p1, p2;
+            trk_face(t, line, ni, nj, k, j, x, y, now, gam, &n1, &n2, &p1, &p2);
+            const double q1[5] = { p1.ro, p1.p, p1.u, p1.v, 0.0 }, q2[5] = { p2.ro, p2.p, p2.u, p2.v, 0.0 };
+            const double nv[3] = { n1, n2, 0.0 };
+            double qqq[5], Vl, Vp, VV;
+            int st = 0;
+            ora_godunov(q1, q2, nv, 0.0, 0.0, 0.1, gam, qqq, &Vl, &Vp, &VV, &st);
+            if (st) bad = 1;
+            const int A = base + j, B = A + 1;
+            const double t1 = -n2, t2 = n1;
+            if (line == 0) {                            /* Setka.cpp:3450-3465 */
+                VV = VV * t->koef * t->k_contact;
+                trk_add(&acc, B, VV, n1, n2);
+            } else if (line == 1) {                     /* :3608-3624 */
+                Vl = Vl * t->koef * t->k_inner;
+                if (fabs(p2.u * t1 + p2.v * t2) < t->tang_inner) {
+                    trk_add(&acc, A, Vl, n1, n2);
+                    trk_add(&acc, B, Vl, n1, n2);
+                } else {
+                    trk_add(&acc, B, Vl, n1, n2);
+                }
+            } else {                                    /* :3783-3825 */
+                Vp = Vp * t->koef * t->k_outer;
+                const double ut = p1.u * t1 + p1.v * t2;
+                if ((sq(p1.u) + sq(p1.v) + sq(0.0)) / (gam * p1.p / p1.ro) > 1.0) {
+                    trk_add(&acc, B, Vp, n1, n2);
+                } else if (fabs(ut) < t->tang_outer) {
+                    trk_add(&acc, A, Vp, n1, n2);
+                    trk_add(&acc, B, Vp, n1, n2);
+                } else if (ut > 0) {
+                    trk_add(&acc, B, Vp, n1, n2);
+                } else {
+                    trk_add(&acc, A, Vp, n1, n2);
+                }
+            }
+        }
+        /* averaging: the reference divides the A nodes of the contact line after each
+           of the three loops (Setka.cpp:3527-3533, :3718-3724, :3829-3835) */
+        const int avg_row = t->average_lines ? k : t->row_contact;
+        if (line == 0 || !t->average_lines) {
+            for (int j = 0; j < nj; j++) trk_norm(&acc, avg_row * (nj + 1) + j);
+        } else {
+            for (int j = 0; j < nj; j++) trk_norm(&acc, base + j);
+        }
+        /* end nodes (Setka.cpp:3535-3537, :3727-3729, :3838-3840) */
+        const int n0 = base, n1_ = base + 1, nl = base + nj, nl1 = base + nj - 1;
+        acc.vx[n0] = acc.vx[n1_] + (x[n1_] - x[n0]) / dt;
+        if (line == 1) acc.vx[nl] = acc.vx[nl1] + (x[nl1] - x[nl]) / dt;
+        else acc.vy[nl] = acc.vy[nl1] + (y[nl1] - y[nl]) / dt;
+    }
+
+    /* A rails (Setka.cpp:4328-4381): keys = inner shock, contact, outer shock, outer end */
+    const int M1 = t->row_inner_shock + 1;
+    const int M2 = t->row_contact - t->row_inner_shock;
+    const int M3 = t->row_outer_shock - t->row_contact;
+    const int M4 = ni - t->row_outer_shock;
+    const double dd = t->s_outer / sqrt(1.0 - 1.0 / t->p_outer);
+    for (int c = 0; c <= nj; c++) {
+        const double s = rail_s[c];
+        const int key[3] = { t->row_inner_shock, t->row_contact, t->row_outer_shock };
+        for (int q = 0; q < 3; q++) {
+            const int id = key[q] * (nj + 1) + c;
+            const double V = acc.vx[id] * cos(s) + acc.vy[id] * sin(s);
+            x2[id] = x[id] + dt * V * cos(s);
+            y2[id] = y[id] + dt * V * sin(s);
+        }
+        {
+            const int id = ni * (nj + 1) + c;
+            x2[id] = x[id];
+            y2[id] = y[id];
+        }
+        const int k0 = key[0] * (nj + 1) + c, k1 = key[1] * (nj + 1) + c, k2 = key[2] * (nj + 1) + c;
+        const double R2 = sqrt(sq(x2[k0]) + sq(y2[k0]));
+        double xx, r;
+        for (int j = 0; j < t->n_inner; j++) {
+            xx = j * 1.0 / (t->n_inner);
+            r = t->r_inner + pow(xx, t->p_inner) * (t->r_mid - t->r_inner);
+            x2[j * (nj + 1) + c] = r * cos(s);
+            y2[j * (nj + 1) + c] = r * sin(s);
+        }
+        for (int j = t->n_inner; j < M1 - 1; j++) {
+            xx = (j - t->n_inner) * 1.0 / (M1 - 1 - t->n_inner);
+            r = t->r_mid + pow(xx, t->p_mid) * (R2 - t->r_mid);
+            x2[j * (nj + 1) + c] = r * cos(s);
+            y2[j * (nj + 1) + c] = r * sin(s);
+        }
+        const double R3 = sqrt(sq(x2[k1]) + sq(y2[k1]));
+        for (int j = 0; j < M2 - 1; j++) {
+            r = R2 + (R3 - R2) * (j + 1) / (M2);
+            x2[(M1 + j) * (nj + 1) + c] = r * cos(s);
+            y2[(M1 + j) * (nj + 1) + c] = r * sin(s);
+        }
+        const double R4 = sqrt(sq(x2[k2]) + sq(y2[k2]));
+        for (int j = 0; j < M3 - 1; j++) {
+            r = R3 + (R4 - R3) * (j + 1) / (M3);
+            x2[(M1 + M2 + j) * (nj + 1) + c] = r * cos(s);
+            y2[(M1 + M2 + j) * (nj + 1) + c] = r * sin(s);
+        }
+        for (int j = 0; j < M4; j++) {
+            xx = (j + 1) * 1.0 / (M4);
+            r = R4 + pow(xx, t->p_outer * (1.0 - sq(s / dd))) * (t->r_outer - R4);
+            x2[(M1 + M2 + M3 + j) * (nj + 1) + c] = r * cos(s);
+            y2[(M1 + M2 + M3 + j) * (nj + 1) + c] = r * sin(s);
+        }
+    }
+    free(acc.vx);
+    free(acc.vy);
+    free(acc.cnt);
+    return bad ? GEL_E_NUMERIC : GEL_OK;
 }

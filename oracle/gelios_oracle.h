@@ -40,6 +40,9 @@ typedef struct ora_mesh {
     const int *cell_type;     /* enum gel_cell_type */
     const int *nface;         /* faces per cell */
     const int *fa, *fb, *ftype, *fnbr, *fdown, *fup;
+    /* node types of the fitted surfaces (0 none, 1 P_Contact, 2 P_Inner_shock,
+       3 P_Outer_shock, Point.h:15-17), or NULL: no surfaces */
+    const int *ptype;
 } ora_mesh;
 
 typedef struct ora_gas { double ro, p, u, v, Q; } ora_gas;
@@ -96,6 +99,14 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
 int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, double* next,
                  const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
                  long long* n_p_clamp, int nthreads);
+
+/* Fitted-surface tracking on the polar grid (Setka::Move_surface, Setka.cpp:3378-3844,
+ * and the A rails of Setka::Move_Setka_Calculate_2, Setka.cpp:4324-4381): node
+ * positions x2, y2 after a step of length dt from the state now (5 x ni*nj) and the
+ * node positions x, y; rail_s[c] is the angle of node column c.  Returns GEL_OK, or
+ * GEL_E_NUMERIC where the exact solver would have exited. */
+int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, const double* y, const double* now,
+                    const double* rail_s, double gam, double dt, double* x2, double* y2);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 #include "Point.h"
 #include "Help.h"
 #include "Solvers.h"
+#include "Rail.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -167,39 +168,30 @@ static int mode_godunov(const char* in, const char* out)
 }
 
 // ---- whole sweep -----------------------------------------------------------------
-// in : int32 ni nj steps moving has_src
-//      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
-//      each step (x2 then y2)
-//      int32 cell_type[nc] outer_type[nj]
-//      float64 prim[5][nc] (ro p u v Q); if has_src: float64 src[19][nc]
-// Static: Go_5_komponent__MK2(steps).  Moving: for each step set Point::x2,y2 and
-// call Go_5_komponent__MK2(1), so every step uses the reference's first-step dt.
-// out: float64 prim[5][nc]; int32 contour[nc][4]; int32 nface[nc];
-//      int32 slots[nc][8][6] = A B type sosed sosed_down sosed_up (-1 when absent)
-static int mode_sweep(const char* in, const char* out)
-{
-    FILE* f = xopen(in, "rb");
-    int32_t h[5]; rd(f, h, 5);
-    const int ni = h[0], nj = h[1], steps = h[2], moving = h[3], has_src = h[4];
-    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
-    std::vector<double> x(nn), y(nn);
-    rd(f, x.data(), nn); rd(f, y.data(), nn);
-    std::vector<double> traj;
-    if (moving) { traj.resize((size_t)steps * 2 * nn); rd(f, traj.data(), traj.size()); }
-    std::vector<int32_t> ctype(nc), otype(nj);
-    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
-    std::vector<double> prim((size_t)5 * nc); rd(f, prim.data(), prim.size());
-    std::vector<double> src;
-    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
-    std::fclose(f);
+// The polar grid as the reference's own objects: nodes (i, j) at i*(nj+1)+j, cells
+// (i, j) with contour (i,j),(i+1,j),(i+1,j+1),(i,j+1), the state in both buffers.
+// own1[c]: the radial face above cell c (main face, master c), own2[c]: the polar face
+// towards j+1.
+struct PolarRef {
+    Setka* S;
+    std::vector<Gran*> own1, own2;
+    int gnum;
+};
 
+static PolarRef build_polar(int ni, int nj, const std::vector<double>& x, const std::vector<double>& y,
+                            const std::vector<int32_t>& ctype, const std::vector<int32_t>& otype,
+                            const std::vector<double>& prim, const std::vector<double>& src, bool has_src)
+{
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    PolarRef P;
     Setka* S = new Setka();
+    P.S = S;
     auto nid = [&](int i, int j) { return i * (nj + 1) + j; };
     auto cid = [&](int i, int j) { return i * nj + j; };
     for (int k = 0; k < nn; k++) {
-        Point* P = new Point(x[k], y[k]);
-        P->number = k;
-        S->All_Points.push_back(P);
+        Point* Pt = new Point(x[k], y[k]);
+        Pt->number = k;
+        S->All_Points.push_back(Pt);
     }
     for (int i = 0; i < ni; i++)
         for (int j = 0; j < nj; j++) {
@@ -214,7 +206,10 @@ static int mode_sweep(const char* in, const char* out)
     // Edges 1 (outer, i+1) and 2 (towards j+1) are owned by the cell; edges 0 and 3 are
     // the copies of the neighbours' faces.  Boundary copies have no master, as the
     // reference builds its axis faces (Setka.cpp:711-741).
-    std::vector<Gran*> own1(nc, nullptr), own2(nc, nullptr);
+    std::vector<Gran*>& own1 = P.own1;
+    std::vector<Gran*>& own2 = P.own2;
+    own1.assign(nc, nullptr);
+    own2.assign(nc, nullptr);
     auto mk_boundary = [&](Cell* K, Point* A, Point* B, Gran_type t) {
         Gran* G = new Gran(A, B, t);
         Gran* G2 = new Gran(B, A, t);
@@ -282,6 +277,39 @@ static int mode_sweep(const char* in, const char* out)
         K->Back = nullptr;
         K->Next = nullptr;
     }
+    P.gnum = gnum;
+    return P;
+}
+
+// in : int32 ni nj steps moving has_src
+//      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
+//      each step (x2 then y2)
+//      int32 cell_type[nc] outer_type[nj]
+//      float64 prim[5][nc] (ro p u v Q); if has_src: float64 src[19][nc]
+// Static: Go_5_komponent__MK2(steps).  Moving: for each step set Point::x2,y2 and
+// call Go_5_komponent__MK2(1), so every step uses the reference's first-step dt.
+// out: float64 prim[5][nc]; int32 contour[nc][4]; int32 nface[nc];
+//      int32 slots[nc][8][6] = A B type sosed sosed_down sosed_up (-1 when absent)
+static int mode_sweep(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[5]; rd(f, h, 5);
+    const int ni = h[0], nj = h[1], steps = h[2], moving = h[3], has_src = h[4];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn);
+    rd(f, x.data(), nn); rd(f, y.data(), nn);
+    std::vector<double> traj;
+    if (moving) { traj.resize((size_t)steps * 2 * nn); rd(f, traj.data(), traj.size()); }
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)5 * nc); rd(f, prim.data(), prim.size());
+    std::vector<double> src;
+    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
+    std::fclose(f);
+
+    PolarRef PR = build_polar(ni, nj, x, y, ctype, otype, prim, src, has_src != 0);
+    Setka* S = PR.S;
+    int gnum = PR.gnum;
     S->TVD_prepare();
 
     // Setka::Move_surface (called by every step of Go_5_komponent__MK2) indexes the
@@ -358,15 +386,105 @@ static int mode_sweep(const char* in, const char* out)
     return 0;
 }
 
+// ---- whole sweep with fitted-surface tracking ----------------------------------------
+// in : int32 ni nj steps has_src row_inner_shock row_contact row_outer_shock outer_faces
+//      float64 x[nn] y[nn] rail_s[nj+1]
+//      int32 cell_type[nc] outer_type[nj]
+//      float64 prim[5][nc]; if has_src: float64 src[19][nc]
+// The three node rows get the point types P_Inner_shock, P_Contact, P_Outer_shock; the
+// radial faces of those rows (masters below) form Line_Inner, Line_Contact, Line_Outer
+// in the order of the polar angle (Setka.cpp:1007-1045); every node column is an A rail
+// with the four key points (Rail.cpp:19-75).  Setka::N1 is set so that the outer shock's
+// face limit n_outer_shock + N1 + N2 + 1 (Setka.cpp:3737) equals outer_faces.  Then
+// Go_5_komponent__MK2(steps), which calls Move_surface and Move_Setka_Calculate_2 in
+// every step (Setka.cpp:10144-10145).
+// out: float64 prim[5][nc]; float64 x[nn] y[nn] (node positions after the last step)
+static int mode_track(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[8]; rd(f, h, 8);
+    const int ni = h[0], nj = h[1], steps = h[2], has_src = h[3];
+    const int rin = h[4], rc = h[5], rout = h[6], outer_faces = h[7];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn), rail(nj + 1);
+    rd(f, x.data(), nn); rd(f, y.data(), nn); rd(f, rail.data(), nj + 1);
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)5 * nc); rd(f, prim.data(), prim.size());
+    std::vector<double> src;
+    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
+    std::fclose(f);
+
+    PolarRef PR = build_polar(ni, nj, x, y, ctype, otype, prim, src, has_src != 0);
+    Setka* S = PR.S;
+    auto nid = [&](int i, int j) { return i * (nj + 1) + j; };
+    for (int c = 0; c <= nj; c++) {
+        S->All_Points[nid(rin, c)]->type = P_Inner_shock;
+        S->All_Points[nid(rc, c)]->type = P_Contact;
+        S->All_Points[nid(rout, c)]->type = P_Outer_shock;
+    }
+    S->TVD_prepare();
+    // The source interpolation of Cell::Get_Sourse_MK1 (Cell.cpp:528-589) walks the
+    // radial Back/Next chains for 10/RR_ < r < 1500/RR_, x >= 0.  TVD_prepare links only
+    // neighbours whose centres align with the ray within 1e-3 (Setka.cpp:1438-1489),
+    // which the tracked grid's skewed cells can miss; the chains are set to the rays
+    // here.  The walk steps up to two cells past a cell whose radius moved, so the end
+    // rows link to themselves.  No source is applied in this mode (k_u = k_v = k_T = 0
+    // make it zero whatever the interpolated moments).
+    for (int i = 0; i < ni; i++)
+        for (int j = 0; j < nj; j++) {
+            Cell* K = S->All_Cells[i * nj + j];
+            K->Back = S->All_Cells[(i > 0 ? i - 1 : i) * nj + j];
+            K->Next = S->All_Cells[(i + 1 < ni ? i + 1 : i) * nj + j];
+        }
+    for (int j = 0; j < nj; j++) {
+        S->Line_Inner.push_back(PR.own1[(rin - 1) * nj + j]);
+        S->Line_Contact.push_back(PR.own1[(rc - 1) * nj + j]);
+        S->Line_Outer.push_back(PR.own1[(rout - 1) * nj + j]);
+    }
+    S->N1 = outer_faces - n_outer_shock - 1;
+    S->N2 = 0;
+    for (int c = 0; c <= nj; c++) {
+        Rail* R = new Rail(rail[c]);
+        R->type = A;
+        for (int i = 0; i <= ni; i++) R->All_point.push_back(S->All_Points[nid(i, c)]);
+        R->Key_point.push_back(S->All_Points[nid(rin, c)]);
+        R->Key_point.push_back(S->All_Points[nid(rc, c)]);
+        R->Key_point.push_back(S->All_Points[nid(rout, c)]);
+        R->Key_point.push_back(S->All_Points[nid(ni, c)]);
+        R->M1 = rin + 1;
+        R->M2 = rc - rin;
+        R->M3 = rout - rc;
+        R->M4 = ni - rout;
+        S->A_Rails.push_back(R);
+    }
+    S->Go_5_komponent__MK2(steps);
+    const int final_buf = steps % 2;
+
+    std::vector<double> res((size_t)5 * nc), xo(nn), yo(nn);
+    for (int c = 0; c < nc; c++) {
+        const Parametr& p = S->All_Cells[c]->par[final_buf];
+        res[c] = p.ro; res[nc + c] = p.p; res[2 * nc + c] = p.u; res[3 * nc + c] = p.v; res[4 * nc + c] = p.Q;
+    }
+    for (int k = 0; k < nn; k++) { xo[k] = S->All_Points[k]->x; yo[k] = S->All_Points[k]->y; }
+    f = xopen(out, "wb");
+    wr(f, res.data(), res.size());
+    wr(f, xo.data(), xo.size());
+    wr(f, yo.data(), yo.size());
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "sweep")) return mode_sweep(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "hlld")) return mode_hlld(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "godunov")) return mode_godunov(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "track")) return mode_track(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family
+    python tests/golden/make_golden.py godunov    # one family (godunov, track)
 """
 from __future__ import annotations
 
@@ -251,12 +251,53 @@ def gen_sweeps(rng, tmp):
                         src=src, prim=prim, contour=contour, nface=nface, slots=slots)
 
 
+def track_blob(c, steps, src=None):
+    g, t = c.grid, c.tracking
+    h = np.array([g.ni, g.nj, steps, 1 if src is not None else 0, t.row_inner_shock, t.row_contact,
+                  t.row_outer_shock, min(t.outer_faces, g.nj)], np.int32)
+    parts = [h.tobytes(), g.x.tobytes(), g.y.tobytes(), np.ascontiguousarray(c.rails, np.float64).tobytes(),
+             g.cell_type.astype(np.int32).tobytes(), g.outer_type.astype(np.int32).tobytes(),
+             np.ascontiguousarray(c.prim0, np.float64).tobytes()]
+    if src is not None:
+        parts.append(np.ascontiguousarray(src, np.float64).tobytes())
+    return b"".join(parts)
+
+
+def gen_tracked(tmp):
+    """Fitted-surface tracking: Go_5_komponent__MK2 with Move_surface and the A rails
+    of Move_Setka_Calculate_2 on tracked heliospheres (the reference's sweep runs HLL,
+    met = 0 at Setka.cpp:10210): 39 x 10 cells for 30 steps with the outer shock moved
+    by its first 7 faces, 44 x 16 cells for 60 steps with all of them, and 39 x 12 cells
+    for 40 steps with a spherical inner shock, where the radial shocked wind has no
+    tangential speed and the inner shock's faces move both of their nodes."""
+    from gelios_2d_amd.configs import tracked_heliosphere, tracked_spherical_ts
+    cases = {"a": (10, (24, 5, 5, 6), 30, 31, 7, None), "b": (16, (26, 6, 6, 7), 60, 32, 16, None),
+             "c": (12, (24, 5, 5, 6), 40, 33, 12, tracked_spherical_ts)}
+    res = {}
+    for tag, (nj, blocks, steps, seed, outer, kr) in cases.items():
+        c = tracked_heliosphere(nj, blocks, steps, seed, flux=0, outer_faces=outer, key_radii=kr)
+        out = harness("track", track_blob(c, steps), tmp)
+        g = c.grid
+        nc, nn = g.ncells, g.nnodes
+        prim = np.frombuffer(out, np.float64, 5 * nc, 0).reshape(5, nc).copy()
+        x = np.frombuffer(out, np.float64, nn, 5 * nc * 8).copy()
+        y = np.frombuffer(out, np.float64, nn, (5 * nc + nn) * 8).copy()
+        res[tag] = dict(nj=nj, blocks=np.array(blocks), seed=seed, steps=steps, outer_faces=outer,
+                        spherical_ts=kr is not None, prim0=c.prim0,
+                        prim=prim, x=x, y=y, x0=g.x, y0=g.y, rails=c.rails, cell_type=g.cell_type,
+                        outer_type=g.outer_type)
+    np.savez_compressed(os.path.join(OUT, "sweep_tracked.npz"),
+                        **{f"{k}_{tag}": v for tag, d in res.items() for k, v in d.items()})
+
+
 def main():
     only = sys.argv[1:]
     if only:
         with tempfile.TemporaryDirectory() as tmp:
             if "godunov" in only:
                 gen_godunov(np.random.default_rng(20261205), tmp)
+            if "track" in only:
+                gen_tracked(tmp)
         return
     rng = np.random.default_rng(20261203)
     with tempfile.TemporaryDirectory() as tmp:
@@ -266,6 +307,7 @@ def main():
         gen_hlld(np.random.default_rng(20261204), tmp)
         gen_sweeps(rng, tmp)
         gen_godunov(np.random.default_rng(20261205), tmp)
+        gen_tracked(tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
