@@ -16,7 +16,7 @@ namespace gel {
 
 // Both one-sided face values of surface face j of node row k (master cell (k-1, j)
 // below, Gran_copy side (k, j) above); the inner shock's master side is the cell value
-// rescaled to the face (Setka.cpp:3556-3580).
+// rescaled to the face (Setka.cpp:3560-3575).
 __device__ void surface_sides(const SweepArgs& a, int line, int k, int j, double fx, double fy, double gam,
                               double* p1, double* p2)
 {
@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
     if (g >= 3 * nj) return;
     const int line = g / nj, j = g - line * nj;
     double r[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
-    if (!(line == 2 && j >= t.outer_faces)) {             // Setka.cpp:3737-3740
+    if (!(line == 2 && j >= t.outer_faces)) {             // Setka.cpp:3746
         const int k = t.rows[line];
         const size_t ia = (size_t)k * (nj + 1) + j, ib = ia + 1;
         const double fx = (a.x[ia] + a.x[ib]) / 2.0, fy = (a.y[ia] + a.y[ib]) / 2.0;
@@ -80,14 +80,14 @@ __global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
         const double tt1 = -n2, tt2 = n1;
         bool toA = false, toB = false;
         double V = 0.0;
-        if (line == 0) {                                   // Setka.cpp:3450-3465
+        if (line == 0) {                                   // Setka.cpp:3434-3470
             V = VV * t.koef * t.k_line[0];
             toB = true;
-        } else if (line == 1) {                            // :3608-3624
+        } else if (line == 1) {                            // :3595-3616
             V = Vl * t.koef * t.k_line[1];
             toB = true;
             toA = fabs(p2[2] * tt1 + p2[3] * tt2) < t.tang_inner;
-        } else {                                           // :3783-3825
+        } else {                                           // :3782-3825
             V = Vp * t.koef * t.k_line[2];
             const double ut = p1[2] * tt1 + p1[3] * tt2;
             if ((sq(p1[2]) + sq(p1[3]) + sq(0.0)) / (gam * p1[1] / p1[0]) > 1.0) toB = true;
@@ -119,8 +119,8 @@ __device__ __forceinline__ void raw_speed(const TrackDev& t, int nj, int line, i
     }
 }
 
-// After the line's own averaging (Setka.cpp:3527-3533 for the contact; the shocks'
-// loops average the contact line again instead of their own, :3718-3724, :3829-3835).
+// After the line's own averaging (Setka.cpp:3522-3530 for the contact; the shocks'
+// loops average the contact line again instead of their own, :3718-3731, :3829-3837).
 __device__ __forceinline__ void first_speed(const TrackDev& t, int nj, int line, int c, double& vx, double& vy,
                                             int& n)
 {
@@ -131,8 +131,8 @@ __device__ __forceinline__ void first_speed(const TrackDev& t, int nj, int line,
     }
 }
 
-// Final speed of node c of a line: end-node corrections (Setka.cpp:3535-3537,
-// :3727-3729, :3838-3840), then the contact's second and third averaging.
+// Final speed of node c of a line: end-node corrections (Setka.cpp:3532-3535,
+// :3733-3736, :3839-3842), then the contact's second and third averaging.
 __device__ void line_speed(const SweepArgs& a, const TrackDev& t, int line, int c, double dt, double& vx,
                            double& vy)
 {

@@ -217,11 +217,11 @@ typedef struct gel_tracking {
     int row_inner_shock;    /* node rows of the fitted surfaces (P_Inner_shock, */
     int row_contact;        /* P_Contact, P_Outer_shock; Line_Inner, Line_Contact, */
     int row_outer_shock;    /* Line_Outer) */
-    int outer_faces;        /* outer-shock faces j < outer_faces move it (Setka.cpp:3737,
+    int outer_faces;        /* outer-shock faces j < outer_faces move it (Setka.cpp:3746,
                                n_outer_shock + N1 + N2 + 1 on the reference grid) */
     int average_lines;      /* 0: node speeds averaged as the reference does (the contact
                                line's nodes after each of the three loops,
-                               Setka.cpp:3527-3533, :3718-3724, :3829-3835, so the
+                               Setka.cpp:3522-3530, :3718-3731, :3829-3837, so the
                                shocks' nodes keep sums); 1: each line averages its own */
     int n_inner;            /* rail nodes in the inner block (n_inner = 20, Help.h:66) */
     double r_inner, r_mid, r_outer;          /* R1_, R111_, R5_ (Help.h:76-83) */

@@ -1178,7 +1178,7 @@ double ora_godunov(const double q1[5], const double q2[5], const double n[3], do
 typedef struct trk_acc { double* vx; double* vy; int* cnt; } trk_acc;
 
 /* Get_par_TVD of both sides of the surface face j of row k, with the inner shock's
- * master side taken from the cell centre rescaled to the face (Setka.cpp:3556-3580). */
+ * master side taken from the cell centre rescaled to the face (Setka.cpp:3560-3575). */
 static void trk_face(const gel_tracking* t, int line, int ni, int nj, int k, int j, const double* x, const double* y,
                      const double* now, double gam, double* n1o, double* n2o, ora_gas* p1, ora_gas* p2)
 {
@@ -1198,7 +1198,7 @@ static void trk_face(const gel_tracking* t, int line, int ni, int nj, int k, int
         ora_cell_center(x, y, q, &cx[r], &cy[r]);
     }
     const int D = (k - 2) * nj + j, M = (k - 1) * nj + j, N = k * nj + j, U = (k + 1) * nj + j;
-    if (line == 1) {                   /* inner shock: Setka.cpp:3556-3580 */
+    if (line == 1) {                   /* inner shock: Setka.cpp:3560-3575 */
         ora_gas g = { now[M], now[nc + M], now[2 * nc + M], now[3 * nc + M], now[4 * nc + M] };
         const double radius = sqrt(sq(cx[1]) + sq(cy[1]));
         const double dis = sqrt(sq(fx) + sq(fy));
@@ -1242,7 +1242,7 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
         const int k = rows[line];
         const int base = k * (nj + 1);
         for (int j = 0; j < nj; j++) {
-            if (line == 2 && j >= t->outer_faces) continue;     /* Setka.cpp:3737-3740 */
+            if (line == 2 && j >= t->outer_faces) continue;     /* Setka.cpp:3746 */
             double n1, n2;
             ora_gas p1, p2;
             trk_face(t, line, ni, nj, k, j, x, y, now, gam, &n1, &n2, &p1, &p2);
@@ -1254,10 +1254,10 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
             if (st) bad = 1;
             const int A = base + j, B = A + 1;
             const double t1 = -n2, t2 = n1;
-            if (line == 0) {                            /* Setka.cpp:3450-3465 */
+            if (line == 0) {                            /* Setka.cpp:3434-3470 */
                 VV = VV * t->koef * t->k_contact;
                 trk_add(&acc, B, VV, n1, n2);
-            } else if (line == 1) {                     /* :3608-3624 */
+            } else if (line == 1) {                     /* :3595-3616 */
                 Vl = Vl * t->koef * t->k_inner;
                 if (fabs(p2.u * t1 + p2.v * t2) < t->tang_inner) {
                     trk_add(&acc, A, Vl, n1, n2);
@@ -1265,7 +1265,7 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
                 } else {
                     trk_add(&acc, B, Vl, n1, n2);
                 }
-            } else {                                    /* :3783-3825 */
+            } else {                                    /* :3782-3825 */
                 Vp = Vp * t->koef * t->k_outer;
                 const double ut = p1.u * t1 + p1.v * t2;
                 if ((sq(p1.u) + sq(p1.v) + sq(0.0)) / (gam * p1.p / p1.ro) > 1.0) {
@@ -1281,14 +1281,14 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
             }
         }
         /* averaging: the reference divides the A nodes of the contact line after each
-           of the three loops (Setka.cpp:3527-3533, :3718-3724, :3829-3835) */
+           of the three loops (Setka.cpp:3522-3530, :3718-3731, :3829-3837) */
         const int avg_row = t->average_lines ? k : t->row_contact;
         if (line == 0 || !t->average_lines) {
             for (int j = 0; j < nj; j++) trk_norm(&acc, avg_row * (nj + 1) + j);
         } else {
             for (int j = 0; j < nj; j++) trk_norm(&acc, base + j);
         }
-        /* end nodes (Setka.cpp:3535-3537, :3727-3729, :3838-3840) */
+        /* end nodes (Setka.cpp:3532-3535, :3733-3736, :3839-3842) */
         const int n0 = base, n1_ = base + 1, nl = base + nj, nl1 = base + nj - 1;
         acc.vx[n0] = acc.vx[n1_] + (x[n1_] - x[n0]) / dt;
         if (line == 1) acc.vx[nl] = acc.vx[nl1] + (x[nl1] - x[nl]) / dt;

@@ -395,7 +395,7 @@ static int mode_sweep(const char* in, const char* out)
 // radial faces of those rows (masters below) form Line_Inner, Line_Contact, Line_Outer
 // in the order of the polar angle (Setka.cpp:1007-1045); every node column is an A rail
 // with the four key points (Rail.cpp:19-75).  Setka::N1 is set so that the outer shock's
-// face limit n_outer_shock + N1 + N2 + 1 (Setka.cpp:3737) equals outer_faces.  Then
+// face limit n_outer_shock + N1 + N2 + 1 (Setka.cpp:3746) equals outer_faces.  Then
 // Go_5_komponent__MK2(steps), which calls Move_surface and Move_Setka_Calculate_2 in
 // every step (Setka.cpp:10144-10145).
 // out: float64 prim[5][nc]; float64 x[nn] y[nn] (node positions after the last step)

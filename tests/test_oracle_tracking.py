@@ -35,7 +35,7 @@ def test_tracked_sweep_matches_reference(oracle_lib, tag):
 def test_reference_averaging_differs_from_per_line_averaging(oracle_lib):
     """Case c (spherical inner shock) moves both nodes of the inner-shock faces, so the
     reference's averaging of the contact line in place of the shocks' own lines
-    (Setka.cpp:3718-3724, :3829-3835) shows in the result."""
+    (Setka.cpp:3718-3731, :3829-3837) shows in the result."""
     d = golden("sweep_tracked.npz")
     c = _case(d, "c")
     t = c.tracking.copy()
