@@ -20,6 +20,10 @@ Workloads
   large-mhd        hlld-800x400 physics on 4096 x 2048 cells (state far beyond the 256 MB
                    Infinity Cache): the HBM roofline of the FP64 sweep
   large-gas        hllc-200x100 physics on 4096 x 2048 cells, FP64
+  tracked-800x400  a heliosphere with the three fitted surfaces tracked every step
+                   (exact-solver surface speeds, A-rail redistribution, sweep with
+                   exact-solver surface faces), 799 x 400 cells, FP64 HLL, the
+                   reference's constants (CFL 0.1), 200 steps
 """
 from __future__ import annotations
 
@@ -49,7 +53,7 @@ def bytes_per_cell_update(w, fallback_fraction=0.0):
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID, GEL_SRC_MK1, GEL_SRC_TABLE
     nv = 9 if w.params.physics == 1 else 5
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
-    geom = 32 if w.motion else 16
+    geom = 32 if (getattr(w, "motion", None) or getattr(w, "tracking", None)) else 16
     src = {GEL_SRC_TABLE: 40, GEL_SRC_MK1: 152}.get(w.params.source_mode, 0)
     if w.params.source_mode == GEL_SRC_HYBRID:
         src = 1 + fallback_fraction * 152 + (1.0 - fallback_fraction) * 40
@@ -60,9 +64,11 @@ def run_gpu(w, K, W, label):
     from gelios_2d_amd.setka import Setka
     nc = w.grid.ncells
     s = Setka(w.grid, w.params)
-    if w.motion:
+    if getattr(w, "motion", None):
         s.set_motion(*w.motion)
     s.upload(w.prim0)
+    if getattr(w, "tracking", None) is not None:
+        s.set_tracking(w.tracking, w.rails)
     s.Go_5_komponent__MK2(W)
     s.sync()
     t0 = time.perf_counter()
@@ -169,12 +175,24 @@ def cpu_baseline(w, budget_s=12.0):
     Setka.cpp:10148) on the host cores, over a bounded number of steps."""
     from oracle import oracle as ora
     threads = min(16, os.cpu_count() or 1)
-    sw = ora.OracleSweep.from_polar(w.grid, w.params)
+    motion = getattr(w, "motion", None)
+    if getattr(w, "tracking", None) is not None:
+        tr = ora.TrackedOracle(w.grid, w.params, w.tracking, w.rails)
+        sw = tr.sw
+    else:
+        tr = None
+        sw = ora.OracleSweep.from_polar(w.grid, w.params)
     x0, y0 = w.grid.x, w.grid.y
-    amp, period, phase = w.motion if w.motion else (0.0, 1, 0.0)
+    amp, period, phase = motion if motion else (0.0, 1, 0.0)
 
     def one(cur, n):
-        if w.motion:
+        if tr is not None:
+            x2, y2, st = ora.track_nodes(tr.t, w.grid.ni, w.grid.nj, tr.x, tr.y, cur, tr.rails, w.params.gamma, sw.dt)
+            sw.set_nodes(tr.x, tr.y, x2, y2)
+            nxt, _ = sw.step(cur, nthreads=threads)
+            tr.x, tr.y = x2, y2
+            return nxt
+        if motion:
             from gelios_2d_amd.configs import motion_scales
             sc = motion_scales(amp, period, phase)
             so, sn = sc[n % period], sc[(n + 1) % period]
@@ -280,6 +298,8 @@ def main():
             w = configs.mhd_heliosphere(4096, 2048, 20, seed=2, name="hlld-fp64-4096x2048-moving")
         elif wl == "large-gas":
             w = configs.large()
+        elif wl == "tracked-800x400":
+            w = configs.tracked_heliosphere(400, (200, 150, 150, 300), 200, seed=8, name="tracked-hll-fp64-799x400")
         else:
             raise SystemExit(f"unknown workload {wl}")
         K = args.steps if args.steps is not None else w.steps
@@ -304,7 +324,9 @@ def main():
         "config": config if config is not None else {
             "workload": main_w.name, "cells": main_w.grid.ncells, "grid": [main_w.grid.ni, main_w.grid.nj],
             "physics": "mhd-hlld" if main_w.params.physics == 1 else ("gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
-            "moving_grid": bool(main_w.motion), "precision": precision, "cfl": main_w.params.cfl},
+            "moving_grid": bool(getattr(main_w, "motion", None) or getattr(main_w, "tracking", None)),
+            "fitted_surfaces": getattr(main_w, "tracking", None) is not None,
+            "precision": precision, "cfl": main_w.params.cfl},
         "roofline": main_r["roofline"],
     }
     line.update(extra)

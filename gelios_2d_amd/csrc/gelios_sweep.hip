@@ -114,6 +114,11 @@ void gel_params_reference(gel_params* p)
     p->src_coef = 3.0 / 6.0;
     p->sig_a = 1.0;
     p->sig_b = 0.1307345665;
+    p->src_coef_h = 1.0 / 6.0;                              // n_H_LISM_ / Kn_
+    p->h_pow = 0.5;
+    p->h1_in_rho = 0.0000001;
+    p->h1_in_p = (0.0000001 * chi * chi / (ga * 14.1344 * 14.1344));
+    p->h4_inf_rho = 1.0; p->h4_inf_p = 0.5; p->h4_inf_u = -2.54338; p->h4_inf_v = 0.0;
 }
 
 static void free_ctx(gel_ctx* c)

@@ -483,6 +483,17 @@ __device__ __forceinline__ void surface_states(const CellT<Ph, double>& M, const
 // the exact solver at the face speed, the tracer from the HLL/HLLC flux; returns the
 // larger of the two wave-speed maxima, from which cfl * dist / max equals the smaller
 // of the two time steps the reference takes.
+// The nul_potok HLLC of the faces beside a surface.
+__device__ __forceinline__ double nul_potok_flux(const double* L, const double* R, const Face& f, double gam, double* P)
+{
+    const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
+    const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
+    double PQ;
+    const double uu = hllc2d_nul<double>(gl, gr, f.W, f.n1, f.n2, gam, P, PQ);
+    P[4] = PQ;
+    return uu;
+}
+
 template <bool HLLC>
 __device__ __forceinline__ double surface_flux(const double* L, const double* R, const Face& f, double gam,
                                                double* P, int& bad)
@@ -919,11 +930,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 if constexpr (TRACK && FX == GEL_FLUX_HLLC) {
                     // one node of the face on a surface row: nul_potok (Setka.cpp:10299-10327)
                     if (surface_row(a, i) || surface_row(a, i + 1)) {
-                        const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
-                        const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
-                        double PQ;
-                        uu = hllc2d_nul<double>(gl, gr, f.W, f.n1, f.n2, gam, P, PQ);
-                        P[4] = PQ;
+                        uu = nul_potok_flux(L, R, f, gam, P);
                     } else {
                         uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
                     }
@@ -1122,10 +1129,9 @@ inline int choose_rows(int ni, int nj, int groups)
     return br;
 }
 
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
-hipError_t launch_t(gel_ctx* c, SweepArgs& a, hipStream_t s)
+template <class K>
+hipError_t launch_k(K kern, gel_ctx* c, SweepArgs& a, hipStream_t s)
 {
-    const auto kern = sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, kNW>;
     const int groups = resident_groups((const void*)kern, kNW * kWave);
     if (c->rows <= 0) c->rows = choose_rows(c->ni, c->nj, groups);
     a.rows = c->rows;
@@ -1134,6 +1140,12 @@ hipError_t launch_t(gel_ctx* c, SweepArgs& a, hipStream_t s)
     const int grid = ntiles < groups ? ntiles : groups;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, a, strips, ntiles);
     return hipGetLastError();
+}
+
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX>
+hipError_t launch_t(gel_ctx* c, SweepArgs& a, hipStream_t s)
+{
+    return launch_k(sweep_kernel<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, kNW>, c, a, s);
 }
 
 template <class Ph, class TS, class TF, bool TYPES, int FLUX>

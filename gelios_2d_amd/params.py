@@ -13,7 +13,8 @@ import math
 GEL_F_USUALY, GEL_F_INNER_SPHERE, GEL_F_EXTERN, GEL_F_INPUT, GEL_F_AXIS, GEL_F_UPPER_WALL = range(6)
 GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_C_NO = range(7)
 GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
-GEL_PHYS_GAS, GEL_PHYS_MHD = 0, 1
+GEL_PHYS_GAS, GEL_PHYS_MHD, GEL_PHYS_GAS4H = 0, 1, 2
+NVAR_GAS4H = 21
 NVAR_MHD = 9
 GEL_SRC_NONE, GEL_SRC_MK1, GEL_SRC_TABLE, GEL_SRC_HYBRID = 0, 1, 2, 3
 GEL_PREC_FP64, GEL_PREC_FP32_BULK = 0, 1
@@ -27,6 +28,7 @@ VELOSITY_INF = -2.54338
 A_2 = 0.1307345665
 KN_ = 6.0
 N_P_LISM = 3.0
+N_H_LISM = 1.0
 GA = 5.0 / 3.0
 
 
@@ -60,6 +62,14 @@ class GelParams(ctypes.Structure):
         ("inf_bx", ctypes.c_double),
         ("inf_by", ctypes.c_double),
         ("inf_bz", ctypes.c_double),
+        ("src_coef_h", ctypes.c_double),
+        ("h_pow", ctypes.c_double),
+        ("h1_in_rho", ctypes.c_double),
+        ("h1_in_p", ctypes.c_double),
+        ("h4_inf_rho", ctypes.c_double),
+        ("h4_inf_p", ctypes.c_double),
+        ("h4_inf_u", ctypes.c_double),
+        ("h4_inf_v", ctypes.c_double),
     ]
 
     def copy(self) -> "GelParams":
@@ -92,6 +102,11 @@ def reference_params() -> GelParams:
     p.src_coef = N_P_LISM / KN_
     p.sig_a = 1.0
     p.sig_b = A_2
+    p.src_coef_h = N_H_LISM / KN_                # Setka.cpp:9528-9547
+    p.h_pow = 0.5                               # H_pow, Help.h:90
+    p.h1_in_rho = 0.0000001                     # Gran.cpp:244-245
+    p.h1_in_p = (0.0000001 * CHI_REAL * CHI_REAL / (GA * 14.1344 * 14.1344))
+    p.h4_inf_rho, p.h4_inf_p, p.h4_inf_u, p.h4_inf_v = 1.0, 0.5, VELOSITY_INF, 0.0   # Gran.cpp:278-280
     return p
 
 

@@ -74,8 +74,13 @@ enum gel_flux { GEL_FLUX_HLL = 0, GEL_FLUX_HLLC = 1 };
 /* GAS: 5 primitives (ro, p, u, v, Q), face flux of HLLC_2d_Korolkov_b_s.
  * MHD: 9 primitives (ro, p, u, v, w, bx, by, bz, Q), HLLD face flux after
  * Solvers::HLLDQ_Korolkov metod 3 (Solvers.cpp:1980-2185) for any in-plane normal and
- * moving faces; B in Gaussian units as there (b = B/sqrt(4 pi)). */
-enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1 };
+ * moving faces; B in Gaussian units as there (b = B/sqrt(4 pi)).
+ * GAS4H: the plasma and four hydrogen fluids of Setka::Go_5_komponent_2
+ * (Setka.cpp:8951-9777): 21 primitives in the order of Parametr (Cell.h:19-24):
+ * ro, p, u, v, Q, then ro_Hk, p_Hk, u_Hk, v_Hk for k = 1..4; every fluid takes the
+ * HLL/HLLC face flux, the fluids exchange mass, momentum and energy by charge exchange
+ * (Setka.cpp:9400-9617); FP64, no separate source mode. */
+enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1, GEL_PHYS_GAS4H = 2 };
 
 /* FP64: every operation in FP64 (the reference's arithmetic).
  * FP32_BULK: state stored in FP32 and face reconstruction + Riemann solver in FP32;
@@ -126,6 +131,11 @@ typedef struct gel_params {
     /* MHD Input ghost: azimuthal velocity and field (bx, by, bphi); the inner-sphere
        wind carries no field. */
     double inf_w, inf_bx, inf_by, inf_bz;
+    /* GAS4H: coefficient n_H/Kn of the hydrogen sources (Setka.cpp:9528-9547); the
+       exponent H_pow of the radial rescaling of ro_H1 (Help.h:90, Setka.cpp:10216); the
+       H1 inner-sphere ghost ro, p (velocity as the wind's, Gran.cpp:234-265); the H4
+       Input ghost ro, p, u, v (Gran.cpp:276-282). */
+    double src_coef_h, h_pow, h1_in_rho, h1_in_p, h4_inf_rho, h4_inf_p, h4_inf_u, h4_inf_v;
 } gel_params;
 
 /* Fill *p with the reference's own constants (Help.h, Gran.cpp) and HLL flux. */

@@ -1354,3 +1354,333 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
     free(acc.cnt);
     return bad ? GEL_E_NUMERIC : GEL_OK;
 }
+
+/* ------------------------------------------------ plasma + four hydrogen fluids --- */
+
+/* Setka::Go_5_komponent_2 (Setka.cpp:8951-9777): the plasma and the hydrogen fluids
+ * H1..H4 on one grid, 21 primitives per cell in Parametr order (ro, p, u, v, Q, then
+ * ro_Hk, p_Hk, u_Hk, v_Hk).  Every fluid takes the HLL/HLLC face flux; the fluids
+ * exchange mass, momentum and energy by charge exchange. */
+
+#define MFV 21
+static const int mf_odd[MFV] = { 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1 };
+
+typedef struct mfs { const ora_mesh* m; const gel_params* prm; const double* now; int nc;
+                     const double* cx; const double* cy; } mfs;
+
+static void mf_load(const mfs* S, int c, double* g)
+{
+    for (int k = 0; k < MFV; k++) g[k] = S->now[(size_t)k * S->nc + c];
+}
+
+/* Gran::Get_par_TVD for all 21 fields (Gran.cpp:294-474): no down cell -> master (with
+ * the mirrored ghost across the axis for every v), else the limited line through
+ * (down, master, sosed).  *bad where the plasma density or pressure is non-positive
+ * (the reference exits, :444-455). */
+static void mf_tvd(const mfs* S, int M, int N, int D, double fx, double fy, double* r, int* bad)
+{
+    double pm[MFV];
+    mf_load(S, M, pm);
+    if (D < 0) {
+        for (int k = 0; k < MFV; k++) r[k] = pm[k];
+        if (cell_has_axis(S->m, M)) {
+            double pn[MFV];
+            mf_load(S, N, pn);
+            const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+            const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+            const double d1 = dist2d(fx, fy, S->cx[M], -S->cy[M]);
+            for (int k = 0; k < MFV; k++)
+                if (mf_odd[k]) r[k] = ora_linear3(-d1, -pm[k], -d2, pm[k], d3, pn[k], 0.0);
+        }
+        return;
+    }
+    double pd[MFV], pn[MFV];
+    mf_load(S, D, pd);
+    mf_load(S, N, pn);
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+    const double d1 = dist2d(fx, fy, S->cx[D], S->cy[D]);
+    for (int k = 0; k < MFV; k++) r[k] = ora_linear3(-d1, pd[k], -d2, pm[k], d3, pn[k], 0.0);
+    if (r[0] <= 0.0 || r[1] <= 0.0) *bad = 1;
+}
+
+/* Gran::Get_par_TVD_radial, the fields the sweep takes from it (Gran.cpp:562-675,
+ * Setka.cpp:9034-9040): plasma ro (r^2-weighted), p (r^(2 gamma)), p_H1 (plain),
+ * ro_H1 (r^H_pow-weighted). */
+static void mf_tvd_radial(const mfs* S, int M, int N, int D, double fx, double fy, double* ro, double* pr,
+                          double* ph1, double* roh1)
+{
+    const double g2 = 2.0 * S->prm->gamma, hp = S->prm->h_pow;
+    double pm[MFV], pn[MFV], pd[MFV];
+    mf_load(S, M, pm);
+    mf_load(S, N, pn);
+    double xd, yd;
+    if (D < 0) {
+        if (!cell_has_axis(S->m, M)) { *ro = pm[0]; *pr = pm[1]; *ph1 = pm[6]; *roh1 = pm[5]; return; }
+        xd = S->cx[M]; yd = -S->cy[M];
+        for (int k = 0; k < MFV; k++) pd[k] = pm[k];
+    } else {
+        xd = S->cx[D]; yd = S->cy[D];
+        mf_load(S, D, pd);
+    }
+    const double r = sqrt(sq(fx) + sq(fy));
+    const double r2 = sqrt(sq(S->cx[M]) + sq(S->cy[M]));
+    const double d2 = dist2d(fx, fy, S->cx[M], S->cy[M]);
+    const double r3 = sqrt(sq(S->cx[N]) + sq(S->cy[N]));
+    const double d3 = dist2d(fx, fy, S->cx[N], S->cy[N]);
+    const double r1 = sqrt(sq(xd) + sq(yd));
+    const double d1 = dist2d(fx, fy, xd, yd);
+    *ro = ora_linear3(-d1, pd[0] * sq(r1), -d2, pm[0] * sq(r2), d3, pn[0] * sq(r3), 0.0) / sq(r);
+    *pr = ora_linear3(-d1, pd[1] * pow(r1, g2), -d2, pm[1] * pow(r2, g2), d3, pn[1] * pow(r3, g2), 0.0) / pow(r, g2);
+    *ph1 = ora_linear3(-d1, pd[6], -d2, pm[6], d3, pn[6], 0.0);
+    *roh1 = ora_linear3(-d1, pd[5] * pow(r1, hp), -d2, pm[5] * pow(r2, hp), d3, pn[5] * pow(r3, hp), 0.0) / pow(r, hp);
+}
+
+/* Gran::Get_par of the 21 fields (Gran.cpp:205-289). */
+static void mf_ghost(const mfs* S, int M, int slot, double fx, double fy, double* g)
+{
+    const gel_params* p = S->prm;
+    const int t = S->m->ftype[slot];
+    mf_load(S, M, g);
+    switch (t) {
+    case GEL_F_USUALY:
+        mf_load(S, S->m->fnbr[slot], g);
+        return;
+    case GEL_F_EXTERN:
+        if (g[2] >= 0.0) g[2] = p->extern_u;
+        return;
+    case GEL_F_AXIS:
+        for (int k = 0; k < MFV; k++) if (mf_odd[k]) g[k] = -g[k];
+        return;
+    case GEL_F_INNER_SPHERE: {
+        const double d = sqrt(fx * fx + fy * fy);
+        const double ro = 1.0 / (p->in_rho_div * d * d);
+        g[0] = ro; g[1] = ro * p->in_p_a * p->in_p_a / p->in_p_den;
+        g[2] = p->in_u * fx / d; g[3] = p->in_u * fy / d; g[4] = ro * p->in_q;
+        g[5] = p->h1_in_rho; g[6] = p->h1_in_p; g[7] = p->in_u * fx / d; g[8] = p->in_u * fy / d;
+        return;
+    }
+    case GEL_F_INPUT:
+        g[0] = p->inf_rho; g[1] = p->inf_p; g[2] = p->inf_u; g[3] = p->inf_v; g[4] = p->inf_q;
+        g[17] = p->h4_inf_rho; g[18] = p->h4_inf_p; g[19] = p->h4_inf_u; g[20] = p->h4_inf_v;
+        return;
+    default:
+        return;
+    }
+}
+
+/* rescaling of a C_1 cell value to the face (Setka.cpp:9021-9028): plasma ro, Q by
+ * (rc/dis)^2, p by (rc/dis)^(2 gamma), ro_H1 by (rc/dis)^H_pow; rotate: (u, v) and
+ * (u_H1, v_H1) to the face's polar frame. */
+static void mf_rescale(const gel_params* p, double* g, double rc, double dis, double cx, double cy, double fx,
+                       double fy, int rotate)
+{
+    g[0] = g[0] * sq(rc) / sq(dis);
+    g[5] = g[5] * pow(rc / dis, p->h_pow);
+    g[4] = g[4] * sq(rc) / sq(dis);
+    g[1] = g[1] * pow(rc / dis, 2 * p->gamma);
+    if (rotate) {
+        ora_polar_perenos(cx, cy, fx, fy, &g[2], &g[3]);
+        ora_polar_perenos(cx, cy, fx, fy, &g[7], &g[8]);
+    }
+}
+
+static double mf_flux(const double* L, const double* R, int h, double W, double n1, double n2, double dist,
+                      const gel_params* prm, int metod, double P[4], double* PQ)
+{
+    /* fluid h: 0 plasma (its own tracer), 1..4 hydrogen with the plasma tracer
+       (Setka.cpp:9176-9199; the tracer flux of the hydrogen calls is not used) */
+    double Vl, Vc, Vp;
+    ora_gas a, b;
+    if (h == 0) {
+        a.ro = L[0]; a.p = L[1]; a.u = L[2]; a.v = L[3]; a.Q = L[4];
+        b.ro = R[0]; b.p = R[1]; b.u = R[2]; b.v = R[3]; b.Q = R[4];
+    } else {
+        const int o = 5 + 4 * (h - 1);
+        a.ro = L[o]; a.p = L[o + 1]; a.u = L[o + 2]; a.v = L[o + 3]; a.Q = L[4];
+        b.ro = R[o]; b.p = R[o + 1]; b.u = R[o + 2]; b.v = R[o + 3]; b.Q = R[4];
+    }
+    return ora_hllc2d(&a, &b, W, n1, n2, dist, prm->cfl, prm->gamma, metod, 0, P, PQ, &Vl, &Vc, &Vp);
+}
+
+int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, double* next, double dt_prev,
+                double* dt_new, long long* n_rho_clamp, long long* n_p_clamp, int nthreads)
+{
+    const int nc = m->n_cells;
+    const double gam = prm->gamma;
+    double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
+    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    mfs S = { m, prm, now, nc, cx, cy };
+    double Tmin = 100000000;
+    long long nrc = 0, npc = 0;
+    int bad_any = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
+#endif
+    for (int c = 0; c < nc; c++) {
+        const int* q = m->contour + 4 * c;
+        const double x = cx[c], y = cy[c];
+        const double vs1 = ora_cell_area(m->x, m->y, q);
+        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double radius = sqrt(sq(x) + sq(y));
+        if (radius < prm->r_skip) continue;                       /* Setka.cpp:8997-9000 */
+        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
+        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        double par1[MFV];
+        mf_load(&S, c, par1);
+        const int ct = m->cell_type[c];
+        const int K1 = (ct == GEL_C_1), KC = (ct == GEL_C_CENTR);
+        double Pot[5] = { 0, 0, 0, 0, 0 }, PH[4][4];
+        memset(PH, 0, sizeof PH);
+        double time = 10000000;
+        int bad = 0;
+        for (int k = 0; k < m->nface[c]; k++) {
+            const int s = c * ORA_MAXF + k;
+            const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
+            const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+            const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
+            const double fx2 = (m->x2[A] + m->x2[B]) / 2.0, fy2 = (m->y2[A] + m->y2[B]) / 2.0;
+            const double dist = sqrt(sq(x - fx) + sq(y - fy));
+            double n1, n2;
+            {
+                double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
+                const double nn = sqrt(sq(t1) + sq(t2));
+                t1 = t1 / nn; t2 = t2 / nn;
+                n1 = t2; n2 = -t1;
+            }
+            const double W = ((fx2 - fx) * n1 + (fy2 - fy) * n2) / dt_prev;
+            const double dis = sqrt(sq(fx) + sq(fy));
+            double p11[MFV], p2[MFV];
+            for (int v = 0; v < MFV; v++) p11[v] = par1[v];
+            mf_ghost(&S, c, s, fx, fy, p2);
+            if (t == GEL_F_USUALY) {                              /* :9014-9070 */
+                const int N = m->fnbr[s];
+                if (K1) {
+                    mf_rescale(prm, p11, radius, dis, x, y, fx, fy, 1);
+                    double ro, pr, ph1, roh1;
+                    mf_tvd_radial(&S, c, N, m->fdown[s], fx, fy, &ro, &pr, &ph1, &roh1);
+                    p11[0] = ro; p11[1] = pr; p11[6] = ph1; p11[5] = roh1;
+                } else {
+                    mf_tvd(&S, c, N, m->fdown[s], fx, fy, p11, &bad);
+                }
+                if (m->cell_type[N] == GEL_C_1) {
+                    const double dd = sqrt(sq(cx[N]) + sq(cy[N]));
+                    mf_rescale(prm, p2, dd, dis, cx[N], cy[N], fx, fy, 1);
+                    double ro, pr, ph1, roh1;
+                    mf_tvd_radial(&S, N, c, m->fup[s], fx, fy, &ro, &pr, &ph1, &roh1);
+                    p2[0] = ro; p2[1] = pr; p2[6] = ph1; p2[5] = roh1;
+                } else {
+                    mf_tvd(&S, N, c, m->fup[s], fx, fy, p2, &bad);
+                }
+            } else if (t == GEL_F_INNER_SPHERE) {                 /* :9071-9077 */
+                mf_rescale(prm, p11, radius, dis, x, y, fx, fy, 0);
+            } else if (K1 && t == GEL_F_AXIS) {                   /* :9078-9093 */
+                mf_rescale(prm, p11, radius, dis, x, y, fx, fy, 1);
+                for (int v = 0; v < MFV; v++) p2[v] = mf_odd[v] ? -p11[v] : p11[v];
+            } else if (t == GEL_F_AXIS) {                         /* :9094-9104 */
+                /* Get_par_TVD on an axis face (Gran.cpp:486-519): the master's fields with
+                   the plasma v on the line through down, master and the master's mirror */
+                const int D = m->fdown[s];
+                const double d2 = dist2d(fx, fy, cx[c], cy[c]);
+                const double d1 = dist2d(fx, fy, cx[D], cy[D]);
+                p11[3] = ora_linear3(-d1, now[(size_t)3 * nc + D], -d2, par1[3], d2, -par1[3], 0.0);
+                for (int v = 0; v < MFV; v++) p2[v] = mf_odd[v] ? -p11[v] : p11[v];
+            }
+            double P[4], PQ;
+            if (!KC) {                                            /* :9172-9181 */
+                time = mn(time, mf_flux(p11, p2, 1, W, n1, n2, dist, prm, prm->flux, P, &PQ));
+                for (int v = 0; v < 4; v++) PH[0][v] = PH[0][v] + P[v] * S_;
+            }
+            for (int h = 2; h <= 4; h++) {                        /* :9182-9199 */
+                time = mn(time, mf_flux(p11, p2, h, W, n1, n2, dist, prm, prm->flux, P, &PQ));
+                for (int v = 0; v < 4; v++) PH[h - 1][v] = PH[h - 1][v] + P[v] * S_;
+            }
+            time = mn(time, mf_flux(p11, p2, 0, W, n1, n2, dist, prm, prm->flux, P, &PQ));   /* :9226-9234 */
+            for (int v = 0; v < 4; v++) Pot[v] = Pot[v] + P[v] * S_;
+            Pot[4] = Pot[4] + PQ * S_;
+        }
+        if (bad) bad_any = 1;
+        if (time < Tmin) Tmin = time;
+
+        /* charge exchange (Setka.cpp:9419-9470) from the cell's values */
+        const double u = par1[0 + 2], v = par1[3], ro = par1[0], p = par1[1], Q = par1[4];
+        double uH[4], vH[4], roH[4], pH[4], UM[4], UH[4], nu[4];
+        for (int h = 0; h < 4; h++) {
+            roH[h] = par1[5 + 4 * h]; pH[h] = par1[6 + 4 * h]; uH[h] = par1[7 + 4 * h]; vH[h] = par1[8 + 4 * h];
+        }
+        const double pi = 3.14159265358979323846;    /* Solvers.h:9 */
+        for (int h = 0; h < 4; h++) {
+            UM[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (64.0 / (9.0 * pi)) * (p / ro + 2.0 * pH[h] / roH[h]));
+            UH[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (4.0 / pi) * (p / ro + 2.0 * pH[h] / roH[h]));
+        }
+        for (int h = 0; h < 4; h++) {
+            const double sg = sq(prm->sig_a - prm->sig_b * log(UM[h]));
+            nu[h] = ro * roH[h] * UM[h] * sg;
+        }
+        const double cp = prm->src_coef, ch = prm->src_coef_h;
+        const double q2_1 = cp * (nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u));
+        const double q2_2 = cp * (nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v));
+        double e3[4];
+        for (int h = 0; h < 4; h++)
+            e3[h] = nu[h] * ((sq(uH[h]) + sq(vH[h]) - sq(u) - sq(v)) / 2.0 + (UH[h] / UM[h]) * (2.0 * pH[h] / roH[h] - p / ro));
+        const double q3 = cp * (e3[0] + e3[1] + e3[2] + e3[3]);
+        const double T1 = dt_prev;
+        if (!KC) {                                                /* :9491-9535 */
+            double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2);
+            const double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+            if (ro3 <= 0.0) { ro3 = 0.00001; nrc++; }
+            const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q2_1)) / ro3;
+            const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2_2)) / ro3;
+            double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3)) -
+                         0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+            if (p3 <= 0) { p3 = 0.000001; npc++; }
+            next[c] = ro3; next[(size_t)nc + c] = p3; next[(size_t)2 * nc + c] = u3; next[(size_t)3 * nc + c] = v3;
+            next[(size_t)4 * nc + c] = Q33;
+        } else {
+            for (int k = 0; k < 5; k++) next[(size_t)k * nc + c] = now[(size_t)k * nc + c];
+        }
+        /* populations of the source regions (Setka.cpp:9554-9586) */
+        int kk[4] = { 0, 0, 0, 0 };
+        if (ct == GEL_C_CENTR || ct == GEL_C_1 || ct == GEL_C_2 || ct == GEL_C_3) {
+            if (ct == GEL_C_1 || ct == GEL_C_CENTR) kk[0] = 1; else kk[1] = 1;
+        } else {
+            if (ct == GEL_C_4) kk[2] = 1; else kk[3] = 1;
+        }
+        const double S1 = nu[0] + nu[1] + nu[2] + nu[3];
+        const double S2 = nu[0] * ((sq(u) + sq(v)) / 2.0 + (UH[0] / UM[0]) * (p / ro))
+                        + nu[1] * ((sq(u) + sq(v)) / 2.0 + (UH[1] / UM[1]) * (p / ro))
+                        + nu[2] * ((sq(u) + sq(v)) / 2.0 + (UH[2] / UM[2]) * (p / ro))
+                        + nu[3] * ((sq(u) + sq(v)) / 2.0 + (UH[3] / UM[3]) * (p / ro));
+        static const double ro_floor[4] = { 0.0000001, 0.000001, 0.000001, 0.000001 };   /* :9627, :9672, :9701, :9733 */
+        static const double p_floor[4] = { 0.000001, 0.000001, 0.000001, 0.0000001 };     /* :9642, :9687, :9716, :9748 */
+        for (int h = 0; h < 4; h++) {                             /* :9593-9753 */
+            const int o = 5 + 4 * h;
+            if (h == 0 && KC) {
+                for (int k = 0; k < 4; k++) next[(size_t)(o + k) * nc + c] = now[(size_t)(o + k) * nc + c];
+                continue;
+            }
+            const double q1h = ch * (kk[h] * S1 - nu[h]);
+            const double q21h = ch * (kk[h] * S1 * u - nu[h] * uH[h]);
+            const double q22h = ch * (kk[h] * S1 * v - nu[h] * vH[h]);
+            const double q3h = ch * (kk[h] * S2 - nu[h] * ((sq(uH[h]) + sq(vH[h])) / 2.0 + (UH[h] / UM[h]) * 2.0 * (pH[h] / roH[h])));
+            double ro3 = roH[h] * Volume / Volume2 - T1 * (PH[h][0] / Volume2 - q1h);
+            if (ro3 <= 0.0) ro3 = ro_floor[h];
+            const double u3 = (roH[h] * uH[h] * Volume / Volume2 - T1 * (PH[h][1] / Volume2 - q21h)) / ro3;
+            const double v3 = (roH[h] * vH[h] * Volume / Volume2 - T1 * ((PH[h][2] - pH[h] * 0.5 * (vs1 + vs2)) / Volume2 - q22h)) / ro3;
+            double p3 = (((pH[h] / (gam - 1) + roH[h] * (uH[h] * uH[h] + vH[h] * vH[h]) * 0.5) * Volume / Volume2 -
+                          T1 * (PH[h][3] / Volume2 - q3h)) - 0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+            if (p3 <= 0) p3 = p_floor[h];
+            next[(size_t)o * nc + c] = ro3; next[(size_t)(o + 1) * nc + c] = p3;
+            next[(size_t)(o + 2) * nc + c] = u3; next[(size_t)(o + 3) * nc + c] = v3;
+        }
+    }
+    free(cx);
+    free(cy);
+    *dt_new = Tmin;
+    if (n_rho_clamp) *n_rho_clamp += nrc;
+    if (n_p_clamp) *n_p_clamp += npc;
+    return bad_any ? GEL_E_NUMERIC : GEL_OK;
+}

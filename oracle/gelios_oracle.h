@@ -100,6 +100,13 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
                  const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
                  long long* n_p_clamp, int nthreads);
 
+/* The plasma and four hydrogen fluids of Setka::Go_5_komponent_2 (Setka.cpp:8951-9777):
+ * 21 x n_cells primitives in Parametr order (ro, p, u, v, Q, then ro_Hk, p_Hk, u_Hk,
+ * v_Hk for k = 1..4), every fluid with the HLL/HLLC flux (prm->flux), charge exchange
+ * between the plasma and each fluid, hydrogen sources by cell-type region. */
+int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, double* next, double dt_prev,
+                double* dt_new, long long* n_rho_clamp, long long* n_p_clamp, int nthreads);
+
 /* Fitted-surface tracking on the polar grid (Setka::Move_surface, Setka.cpp:3378-3844,
  * and the A rails of Setka::Move_Setka_Calculate_2, Setka.cpp:4324-4381): node
  * positions x2, y2 after a step of length dt from the state now (5 x ni*nj) and the

@@ -81,6 +81,9 @@ def lib():
         L.ora_godunov.restype = ctypes.c_double
         L.ora_godunov.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.ora_step_mf.restype = ctypes.c_int
+        L.ora_step_mf.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, ctypes.c_double, _dp,
+                                  ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
         L.ora_track_nodes.restype = ctypes.c_int
         L.ora_track_nodes.argtypes = [ctypes.POINTER(GelTracking), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
@@ -145,7 +148,7 @@ class OracleSweep:
         self.x2 = np.ascontiguousarray(self.x if x2 is None else x2, np.float64)
         self.y2 = np.ascontiguousarray(self.y if y2 is None else y2, np.float64)
         self.nc = len(mesh.cell_type)
-        self.nv = 9 if prm.physics == 1 else 5
+        self.nv = {1: 9, 2: 21}.get(prm.physics, 5)
         self._keep = [np.ascontiguousarray(a, np.int32) for a in
                       (mesh.contour.ravel(), mesh.cell_type, mesh.nface, mesh.fa, mesh.fb, mesh.ftype,
                        mesh.fnbr, mesh.fdown, mesh.fup)]
@@ -179,10 +182,14 @@ class OracleSweep:
         dt_new = ctypes.c_double()
         m = self._mesh()
         src = _d(self.src) if self.src is not None else None
-        fn = lib().ora_step_mhd if self.nv == 9 else lib().ora_step_mk2
-        st = fn(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src,
-                                self.dt if dt_prev is None else dt_prev, ctypes.byref(dt_new),
-                                ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
+        dtp = self.dt if dt_prev is None else dt_prev
+        if self.nv == 21:
+            st = lib().ora_step_mf(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), dtp, ctypes.byref(dt_new),
+                                   ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
+        else:
+            fn = lib().ora_step_mhd if self.nv == 9 else lib().ora_step_mk2
+            st = fn(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src, dtp, ctypes.byref(dt_new),
+                    ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
         if dt_prev is None:
             self.dt = dt_new.value
         self.last_dt_new = dt_new.value

@@ -281,6 +281,37 @@ static PolarRef build_polar(int ni, int nj, const std::vector<double>& x, const 
     return P;
 }
 
+// Setka::Move_surface (called by every step of Go_5_komponent__MK2 and
+// Go_5_komponent_2) indexes the fitted-surface face lists without checking that they
+// are non-empty (Setka.cpp:3532, :3734, :3840).  A polar grid without fitted surfaces
+// gets one stand-in face per list, between two detached cells far from the grid.  Its
+// nodes are not in All_Points and there are no rails, so Move_Setka_Calculate_2 moves
+// nothing and the stand-in never touches the grid.
+static void add_standin_lines(Setka* S, int& gnum)
+{
+    {
+        S->N1 = S->N2 = S->N3 = S->N4 = S->M1 = S->M2 = S->M3 = S->M4 = 0;
+        std::vector<Point*> dp;
+        for (int k = 0; k <= 4; k++) { dp.push_back(new Point(1000.0 + k, 1000.0)); dp.push_back(new Point(1000.0 + k, 1001.0)); }
+        std::vector<Cell*> dc;
+        for (int k = 0; k < 4; k++) {
+            Cell* C = new Cell(dp[2 * k], dp[2 * k + 2], dp[2 * k + 3], dp[2 * k + 1]);
+            C->type = C_2;
+            for (int b = 0; b < 2; b++) { C->par[b].ro = 1.0; C->par[b].p = 1.0; C->par[b].u = 0.0; C->par[b].v = 0.0; C->par[b].Q = 1.0; }
+            dc.push_back(C);
+        }
+        Gran* G = new Gran(dp[4], dp[5], Usualy);
+        Gran* G2 = new Gran(dp[5], dp[4], Usualy);
+        G->Master = dc[1]; G->Sosed = dc[2]; G->Sosed_down = dc[0]; G->Sosed_up = dc[3];
+        G2->Master = dc[2]; G2->Sosed = dc[1]; G2->Sosed_down = dc[3]; G2->Sosed_up = dc[0];
+        G->Gran_copy = G2; G2->Gran_copy = G;
+        G->number = gnum++; G2->number = gnum++;
+        S->Line_Contact.push_back(G);
+        S->Line_Inner.push_back(G);
+        S->Line_Outer.push_back(G);
+    }
+}
+
 // in : int32 ni nj steps moving has_src
 //      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
 //      each step (x2 then y2)
@@ -312,33 +343,7 @@ static int mode_sweep(const char* in, const char* out)
     int gnum = PR.gnum;
     S->TVD_prepare();
 
-    // Setka::Move_surface (called by every step of Go_5_komponent__MK2) indexes the
-    // fitted-surface face lists without checking that they are non-empty
-    // (Setka.cpp:3532, :3734, :3840).  The polar grid has no fitted surfaces, so each
-    // list gets one stand-in face between two detached cells far from the grid.  Its
-    // nodes are not in All_Points and there are no rails, so Move_Setka_Calculate_2
-    // moves nothing and the stand-in never touches the grid.
-    {
-        S->N1 = S->N2 = S->N3 = S->N4 = S->M1 = S->M2 = S->M3 = S->M4 = 0;
-        std::vector<Point*> dp;
-        for (int k = 0; k <= 4; k++) { dp.push_back(new Point(1000.0 + k, 1000.0)); dp.push_back(new Point(1000.0 + k, 1001.0)); }
-        std::vector<Cell*> dc;
-        for (int k = 0; k < 4; k++) {
-            Cell* C = new Cell(dp[2 * k], dp[2 * k + 2], dp[2 * k + 3], dp[2 * k + 1]);
-            C->type = C_2;
-            for (int b = 0; b < 2; b++) { C->par[b].ro = 1.0; C->par[b].p = 1.0; C->par[b].u = 0.0; C->par[b].v = 0.0; C->par[b].Q = 1.0; }
-            dc.push_back(C);
-        }
-        Gran* G = new Gran(dp[4], dp[5], Usualy);
-        Gran* G2 = new Gran(dp[5], dp[4], Usualy);
-        G->Master = dc[1]; G->Sosed = dc[2]; G->Sosed_down = dc[0]; G->Sosed_up = dc[3];
-        G2->Master = dc[2]; G2->Sosed = dc[1]; G2->Sosed_down = dc[3]; G2->Sosed_up = dc[0];
-        G->Gran_copy = G2; G2->Gran_copy = G;
-        G->number = gnum++; G2->number = gnum++;
-        S->Line_Contact.push_back(G);
-        S->Line_Inner.push_back(G);
-        S->Line_Outer.push_back(G);
-    }
+    add_standin_lines(S, gnum);
 
     int final_buf;
     if (!moving) {
@@ -475,9 +480,73 @@ static int mode_track(const char* in, const char* out)
     return 0;
 }
 
+// ---- plasma + four hydrogen fluids ---------------------------------------------------
+// in : int32 ni nj steps moving
+//      float64 x[nn] y[nn]; if moving: float64 [steps][2][nn] node positions after
+//      each step (x2 then y2)
+//      int32 cell_type[nc] outer_type[nj]
+//      float64 prim[21][nc] (ro p u v Q, then ro_Hk p_Hk u_Hk v_Hk for k = 1..4)
+// Static: Go_5_komponent_2(steps).  Moving: Point::x2, y2 then Go_5_komponent_2(1) per
+// step.  out: float64 prim[21][nc]
+static int mode_mf(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[4]; rd(f, h, 4);
+    const int ni = h[0], nj = h[1], steps = h[2], moving = h[3];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn);
+    rd(f, x.data(), nn); rd(f, y.data(), nn);
+    std::vector<double> traj;
+    if (moving) { traj.resize((size_t)steps * 2 * nn); rd(f, traj.data(), traj.size()); }
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)21 * nc); rd(f, prim.data(), prim.size());
+    std::fclose(f);
+
+    std::vector<double> none;
+    PolarRef PR = build_polar(ni, nj, x, y, ctype, otype, prim, none, false);
+    Setka* S = PR.S;
+    int gnum = PR.gnum;
+    auto set_h = [&](Parametr& p, int c) {
+        const double* g = &prim[0];
+        p.ro_H1 = g[(size_t)5 * nc + c]; p.p_H1 = g[(size_t)6 * nc + c]; p.u_H1 = g[(size_t)7 * nc + c]; p.v_H1 = g[(size_t)8 * nc + c];
+        p.ro_H2 = g[(size_t)9 * nc + c]; p.p_H2 = g[(size_t)10 * nc + c]; p.u_H2 = g[(size_t)11 * nc + c]; p.v_H2 = g[(size_t)12 * nc + c];
+        p.ro_H3 = g[(size_t)13 * nc + c]; p.p_H3 = g[(size_t)14 * nc + c]; p.u_H3 = g[(size_t)15 * nc + c]; p.v_H3 = g[(size_t)16 * nc + c];
+        p.ro_H4 = g[(size_t)17 * nc + c]; p.p_H4 = g[(size_t)18 * nc + c]; p.u_H4 = g[(size_t)19 * nc + c]; p.v_H4 = g[(size_t)20 * nc + c];
+    };
+    for (int c = 0; c < nc; c++)
+        for (int b = 0; b < 2; b++) set_h(S->All_Cells[c]->par[b], c);
+    S->TVD_prepare();
+    add_standin_lines(S, gnum);
+    int final_buf;
+    if (!moving) {
+        S->Go_5_komponent_2(steps);
+        final_buf = steps % 2;
+    } else {
+        for (int s = 0; s < steps; s++) {
+            const double* t = &traj[(size_t)s * 2 * nn];
+            for (int k = 0; k < nn; k++) { S->All_Points[k]->x2 = t[k]; S->All_Points[k]->y2 = t[nn + k]; }
+            S->Go_5_komponent_2(1);
+            for (auto* K : S->All_Cells) K->par[0] = K->par[1];
+        }
+        final_buf = 0;
+    }
+    std::vector<double> res((size_t)21 * nc);
+    for (int c = 0; c < nc; c++) {
+        const Parametr& p = S->All_Cells[c]->par[final_buf];
+        const double v[21] = { p.ro, p.p, p.u, p.v, p.Q, p.ro_H1, p.p_H1, p.u_H1, p.v_H1, p.ro_H2, p.p_H2, p.u_H2,
+                               p.v_H2, p.ro_H3, p.p_H3, p.u_H3, p.v_H3, p.ro_H4, p.p_H4, p.u_H4, p.v_H4 };
+        for (int k = 0; k < 21; k++) res[(size_t)k * nc + c] = v[k];
+    }
+    f = xopen(out, "wb");
+    wr(f, res.data(), res.size());
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
@@ -485,6 +554,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "hlld")) return mode_hlld(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "godunov")) return mode_godunov(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "track")) return mode_track(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "mf")) return mode_mf(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

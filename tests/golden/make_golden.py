@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf)
 """
 from __future__ import annotations
 
@@ -290,6 +290,66 @@ def gen_tracked(tmp):
                         **{f"{k}_{tag}": v for tag, d in res.items() for k, v in d.items()})
 
 
+def mf_state(g, rng, prm):
+    """21 fields: the plasma of initial_state and four hydrogen fluids (seeded, positive
+    densities and pressures, flows around the inflow speed)."""
+    plasma = initial_state(g, rng, prm)
+    nc = g.ncells
+    h = []
+    for k in range(4):
+        h += [rng.uniform(0.05, 1.0, nc), rng.uniform(0.02, 0.6, nc),
+              prm.inf_u * rng.uniform(0.3, 1.2, nc), rng.normal(0.0, 0.4, nc)]
+    return np.concatenate([plasma, np.array(h)])
+
+
+def mf_blob(g, steps, prim, traj=None):
+    h = np.array([g.ni, g.nj, steps, 1 if traj is not None else 0], np.int32)
+    parts = [h.tobytes(), g.x.tobytes(), g.y.tobytes()]
+    if traj is not None:
+        parts.append(np.ascontiguousarray(traj, np.float64).tobytes())
+    parts += [g.cell_type.astype(np.int32).tobytes(), g.outer_type.astype(np.int32).tobytes(),
+              np.ascontiguousarray(prim, np.float64).tobytes()]
+    return b"".join(parts)
+
+
+def gen_multifluid(rng, tmp):
+    """Setka::Go_5_komponent_2 (plasma + four hydrogen fluids, HLLC, charge exchange) on
+    the 16 x 10 harness grid with every source region: C_1 wind rows, C_2, C_3, C_4 and
+    C_5 rows, two frozen C_centr cells; 30 static steps, and 6 steps with the nodes
+    moved by seeded radial displacements."""
+    from gelios_2d_amd.params import GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_PHYS_GAS4H
+    prm = reference_params()
+    g = harness_grid()
+    ct = g.cell_type.copy()
+    nj = g.nj
+    ct[6 * nj: 8 * nj] = GEL_C_3
+    ct[8 * nj: 11 * nj] = GEL_C_4
+    ct[13 * nj:] = GEL_C_5
+    ct[4 * nj + 3] = GEL_C_CENTR
+    ct[9 * nj + 6] = GEL_C_CENTR
+    g.cell_type = ct
+    steps = 30
+    prim0 = mf_state(g, rng, prm)
+    out = harness("mf", mf_blob(g, steps, prim0), tmp)
+    prim = np.frombuffer(out, np.float64).reshape(21, g.ncells).copy()
+    res = dict(ni=g.ni, nj=g.nj, x=g.x, y=g.y, cell_type=g.cell_type, outer_type=g.outer_type,
+               static_steps=steps, static_prim0=prim0, static_prim=prim)
+    steps = 6
+    traj = []
+    x, y = g.x.copy(), g.y.copy()
+    for _ in range(steps):
+        f = 1.0 + 1e-9 * rng.uniform(-1, 1, x.shape)
+        traj.append(np.stack([x * f, y * f]))
+        x, y = x * f, y * f
+    traj = np.array(traj)
+    prim0 = mf_state(g, rng, prm)
+    out = harness("mf", mf_blob(g, steps, prim0, traj), tmp)
+    prim = np.frombuffer(out, np.float64).reshape(21, g.ncells).copy()
+    res.update(moving_steps=steps, moving_prim0=prim0, moving_prim=prim, traj=traj)
+    np.savez_compressed(os.path.join(OUT, "sweep_multifluid.npz"), **res)
+    assert GEL_PHYS_GAS4H == 2
+
+
 def main():
     only = sys.argv[1:]
     if only:
@@ -298,6 +358,8 @@ def main():
                 gen_godunov(np.random.default_rng(20261205), tmp)
             if "track" in only:
                 gen_tracked(tmp)
+            if "mf" in only:
+                gen_multifluid(np.random.default_rng(20261206), tmp)
         return
     rng = np.random.default_rng(20261203)
     with tempfile.TemporaryDirectory() as tmp:
@@ -308,6 +370,7 @@ def main():
         gen_sweeps(rng, tmp)
         gen_godunov(np.random.default_rng(20261205), tmp)
         gen_tracked(tmp)
+        gen_multifluid(np.random.default_rng(20261206), tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
