@@ -224,3 +224,41 @@ def tracked_spherical_ts(s: float, theta1: float = 2.0 * math.pi / 3.0):
     tracked_key_radii)."""
     _, hp, bs = tracked_key_radii(s, theta1)
     return 0.25, hp, bs
+
+
+def multifluid_heliosphere(ni: int, nj: int, steps: int, seed: int, flux: int = 1,
+                           name: str = "plasma-4h-heliosphere") -> Workload:
+    """The plasma and the four hydrogen fluids of Setka::Go_5_komponent_2 on a polar
+    grid r in [4, 12] (units of 390.222 AU), theta in [0, pi], with the reference's
+    constants: the supersonic wind (the Inner_sphere formula, Gran.cpp:234-265) blended
+    into the inflow (Gran.cpp:276-282) at r = 7; cell types by radius select the
+    hydrogen population each region feeds (C_1 below 5, C_2 to 7, C_3 to 8, C_4 to 10,
+    C_5 beyond; Setka.cpp:9554-9586); H4 starts as the interstellar hydrogen (1, 0.5,
+    -2.54338, 0), H1..H3 as thinner, slower copies; +-1% seeded density noise."""
+    from .grid import polar_grid
+    from .params import (GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_PHYS_GAS4H,
+                         reference_params)
+    prm = reference_params()
+    prm.physics = GEL_PHYS_GAS4H
+    prm.flux = flux
+    g = polar_grid(ni, nj, 4.0, 12.0, spacing="log")
+    cx, cy = g.cell_centers()
+    r = np.hypot(cx, cy)
+    t = np.full(g.ncells, GEL_C_5, np.int8)
+    for lim, ty in ((10.0, GEL_C_4), (8.0, GEL_C_3), (7.0, GEL_C_2), (5.0, GEL_C_1)):
+        t[r < lim] = ty
+    g.cell_type = t
+    ro_w = 1.0 / (prm.in_rho_div * r * r)
+    p_w = ro_w * prm.in_p_a * prm.in_p_a / prm.in_p_den
+    s = 0.5 * (1 - np.tanh((r - 7.0) / 1.5))
+    ro = s * ro_w + (1 - s) * prm.inf_rho
+    p = s * p_w + (1 - s) * prm.inf_p
+    u = s * prm.in_u * cx / r + (1 - s) * prm.inf_u
+    v = s * prm.in_u * cy / r + (1 - s) * prm.inf_v
+    Q = s * ro_w + (1 - s) * prm.inf_q
+    rng = np.random.default_rng(seed)
+    fields = [ro * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, ro.shape)), p, u, v, Q]
+    for k, scale in enumerate((0.05, 0.1, 0.2, 1.0)):
+        rh = scale * prm.h4_inf_rho * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, ro.shape))
+        fields += [rh, scale * prm.h4_inf_p + 0.0 * r, (0.5 + 0.5 * scale) * prm.h4_inf_u + 0.0 * r, 0.0 * r]
+    return Workload(name, g, prm, np.ascontiguousarray(np.stack(fields)), steps)

@@ -31,6 +31,8 @@ struct SweepArgs {
     const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
     const double* __restrict__ geo;   // static grids: the geometry table (geometry_kernel)
     int srow[3];                  // node rows of the fitted surfaces, -1 where none
+    int fluid;                    // plasma + hydrogen sweep: 0 plasma, 1..4 hydrogen H1..H4
+    int voff[5];                  // its five variables' arrays in the 21-field state
     unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
@@ -70,6 +72,8 @@ namespace gel {
 // One tracked step (tracking.hip): surface node speeds, rail redistribution into
 // ctx->d_x2/d_y2, and the sweep with surface faces; asynchronous on ctx->stream.
 hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a);
+// One step of the plasma + hydrogen sweep (multifluid.hip): five launches, one per fluid.
+hipError_t launch_mf_step(gel_ctx* c, SweepArgs& a, bool moving);
 }
 
 struct gel_ctx {

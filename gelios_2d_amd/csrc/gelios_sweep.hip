@@ -143,9 +143,12 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if (!out || !g || !prm || !g->x || !g->y) return fail(GEL_E_ARG, "null argument");
     *out = nullptr;
     if (g->ni < 3 || g->nj < 3) return fail(GEL_E_ARG, "grid needs at least 3x3 cells");
-    if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD) return fail(GEL_E_UNSUPPORTED, "physics");
-    if (prm->physics == GEL_PHYS_GAS && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
+    if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD && prm->physics != GEL_PHYS_GAS4H)
+        return fail(GEL_E_UNSUPPORTED, "physics");
+    if (prm->physics != GEL_PHYS_MHD && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
         return fail(GEL_E_UNSUPPORTED, "flux");
+    if (prm->physics == GEL_PHYS_GAS4H && (prm->precision != GEL_PREC_FP64 || prm->source_mode != GEL_SRC_NONE))
+        return fail(GEL_E_UNSUPPORTED, "plasma + hydrogen: FP64, charge exchange built in (source_mode none)");
     if (prm->source_mode < GEL_SRC_NONE || prm->source_mode > GEL_SRC_HYBRID)
         return fail(GEL_E_UNSUPPORTED, "source_mode");
     if (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK) return fail(GEL_E_UNSUPPORTED, "precision");
@@ -155,7 +158,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     c->ni = g->ni; c->nj = g->nj;
     c->nc = (size_t)g->ni * g->nj;
     c->nn = (size_t)(g->ni + 1) * (g->nj + 1);
-    c->nv = prm->physics == GEL_PHYS_MHD ? 9 : 5;
+    c->nv = prm->physics == GEL_PHYS_MHD ? 9 : (prm->physics == GEL_PHYS_GAS4H ? 21 : 5);
     c->ts = prm->precision == GEL_PREC_FP32_BULK ? 4 : 8;
     c->prm = *prm;
     c->rows = 0;          // chosen at the first launch from the kernel's occupancy
@@ -198,8 +201,9 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
             if (t < GEL_C_1 || t > GEL_C_NO) { free_ctx(c); return fail(GEL_E_ARG, "cell_type out of range"); }
             if (t == GEL_C_1 || t == GEL_C_CENTR) any = true;
         }
-        if (any) {
-            if (prm->physics != GEL_PHYS_GAS || prm->precision != GEL_PREC_FP64) {
+        // the plasma + hydrogen sweep assigns hydrogen populations by every cell type
+        if (any || prm->physics == GEL_PHYS_GAS4H) {
+            if (prm->physics == GEL_PHYS_MHD || prm->precision != GEL_PREC_FP64) {
                 free_ctx(c);
                 return fail(GEL_E_UNSUPPORTED, "C_1 / C_centr cells need FP64 gas physics");
             }
@@ -278,6 +282,7 @@ int gel_download_state(gel_ctx* c, double* prim)
 int gel_set_sources(gel_ctx* c, const double* f)
 {
     if (!c || !f) return fail(GEL_E_ARG, "null argument");
+    if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
     const size_t nb = (size_t)GEL_MK1_NFIELDS * c->nc * sizeof(double);
     if (!c->d_src) {
         hipError_t e = hipMalloc(&c->d_src, nb);
@@ -290,6 +295,7 @@ int gel_set_sources(gel_ctx* c, const double* f)
 int gel_set_source_table(gel_ctx* c, const double* q)
 {
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
+    if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
     const size_t nb = (size_t)5 * c->nc * sizeof(double);
     if (!c->d_qflag && hipMalloc(&c->d_qflag, c->nc) != hipSuccess) return fail(GEL_E_NOMEM, "source flags");
     HIP_OK(hipMemset(c->d_qflag, 0, c->nc));
@@ -426,6 +432,7 @@ static hipError_t launch(gel_ctx* c, SweepArgs& a, bool moving)
     if (c->prm.precision == GEL_PREC_FP32_BULK) return launch_sweep_fp32(c, a, moving);
     hipStream_t s = c->stream;
     if (c->prm.physics == GEL_PHYS_MHD) return launch_ms<MhdPhys, double, double, false, 3>(c, a, s, moving, src);
+    if (c->prm.physics == GEL_PHYS_GAS4H) return launch_mf_step(c, a, moving);
     const bool hllc = (c->prm.flux == GEL_FLUX_HLLC);
     if (c->has_types) {
         return hllc ? launch_ms<GasPhys, double, double, true, GEL_FLUX_HLLC>(c, a, s, moving, src)
@@ -484,6 +491,8 @@ static int one_step(gel_ctx* c, int moving)
     a.qflag = c->d_qflag;
     a.geo = c->d_geo;
     a.srow[0] = a.srow[1] = a.srow[2] = -1;
+    a.fluid = 0;
+    for (int k = 0; k < 5; k++) a.voff[k] = k;
     if (moving == 3) { a.srow[0] = c->trk.row_inner_shock; a.srow[1] = c->trk.row_contact; a.srow[2] = c->trk.row_outer_shock; }
     a.dtbuf = c->d_dt;
     a.counters = c->d_cnt;
@@ -771,6 +780,8 @@ static int batch_upload_args(gel_batch* b)
             a.ctype = nullptr; a.otype = c->d_otype;
             a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag; a.geo = c->d_geo;
             a.srow[0] = a.srow[1] = a.srow[2] = -1;
+            a.fluid = 0;
+            for (int v = 0; v < 5; v++) a.voff[v] = v;
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
             if (p == kCycle) a.next = c->d_state[c->cur];
@@ -850,8 +861,8 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c || !c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->has_types || !c->motion.empty() || c->tracking)
-            return fail(GEL_E_UNSUPPORTED, "batched cases are static grids without C_1/C_centr");
+        if (c->has_types || !c->motion.empty() || c->tracking || c->prm.physics == GEL_PHYS_GAS4H)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are static single-fluid grids without C_1/C_centr");
         if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
             c->prm.source_mode != p0.source_mode)
             return fail(GEL_E_UNSUPPORTED, "batched cases share physics, precision, flux and source mode");

@@ -90,7 +90,9 @@ __device__ __forceinline__ Nodes load_nodes(const SweepArgs& a, const double* __
     return n;
 }
 
-template <class Ph, class TS, bool TYPES, bool CENTROID = true>
+// MF: the plasma + hydrogen sweep, whose five variables are the arrays a.voff[0..4]
+// of the 21-field state (one fluid per launch).
+template <class Ph, class TS, bool TYPES, bool CENTROID = true, bool MF = false>
 __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, int j, bool col,
                                                    const Nodes& lo, const Nodes& hi)
 {
@@ -102,7 +104,7 @@ __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, in
         const size_t k = (size_t)r * a.nj + j;
         const TS* __restrict__ now = (const TS*)a.now;
 #pragma unroll
-        for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)v * nc + k];
+        for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)(MF ? a.voff[v] : v) * nc + k];
         if (TYPES) c.type = a.ctype[k];
         if constexpr (CENTROID) {
             const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
@@ -689,6 +691,187 @@ __global__ void geometry_kernel(int ni, int nj, const double* __restrict__ X, co
     }
 }
 
+// ------------------------------------------------- plasma + four hydrogen fluids ---
+// Setka::Go_5_komponent_2 (Setka.cpp:8951-9777) as five launches of the gas sweep, one
+// per fluid (SweepArgs::fluid): 0 the plasma (ro, p, u, v, Q); k = 1..4 hydrogen Hk
+// (ro_Hk, p_Hk, u_Hk, v_Hk, and the plasma tracer in the fifth slot: the reference
+// passes it to the hydrogen fluxes and drops their tracer flux).  FP64.
+
+// Gran::Get_par_TVD_radial's H1 fields (Gran.cpp:562-675): ro_H1 on the line through
+// r^H_pow-weighted values, p_H1 on the plain line.
+__device__ __forceinline__ void mf_tvd_radial_h1(const CellT<GasPhys, double>& M, const CellT<GasPhys, double>& N,
+                                                 const CellT<GasPhys, double>& D, bool m_axis, double fx, double fy,
+                                                 double hp, double& ro, double& ph)
+{
+    double xd, yd, dro, dp;
+    if (!D.ok) {
+        if (!m_axis) { ro = M.s[0]; ph = M.s[1]; return; }
+        xd = M.cx; yd = -M.cy; dro = M.s[0]; dp = M.s[1];
+    } else {
+        xd = D.cx; yd = D.cy; dro = D.s[0]; dp = D.s[1];
+    }
+    const double r = sqrt(sq(fx) + sq(fy));
+    const double r2 = sqrt(sq(M.cx) + sq(M.cy));
+    const double d2 = dist2d(fx, fy, M.cx, M.cy);
+    const double r3 = sqrt(sq(N.cx) + sq(N.cy));
+    const double d3 = dist2d(fx, fy, N.cx, N.cy);
+    const double r1 = sqrt(sq(xd) + sq(yd));
+    const double d1 = dist2d(fx, fy, xd, yd);
+    ph = linear3(-d1, dp, -d2, M.s[1], d3, N.s[1], 0.0);
+    ro = linear3(-d1, dro * pow(r1, hp), -d2, M.s[0] * pow(r2, hp), d3, N.s[0] * pow(r3, hp), 0.0) / pow(r, hp);
+}
+
+// Value of M on its side of an interior face for the fluid of the launch
+// (Setka.cpp:9014-9070): C_1 cells give the plasma its rescaled radial reconstruction,
+// H1 its rotated velocity with the radial ro_H1, p_H1, H2..H4 their cell values; other
+// cells the limited line of every field.  Non-positive reconstructions count only for
+// the plasma (the reference exits there, Gran.cpp:444-455).
+template <bool TYPES>
+__device__ __forceinline__ void mf_side_value(const SweepArgs& a, const CellT<GasPhys, double>& M,
+                                              const CellT<GasPhys, double>& N, const CellT<GasPhys, double>& D,
+                                              bool m_axis, double fx, double fy, double dis, double* r, int& bad)
+{
+    if (TYPES && M.type == GEL_C_1) {
+        if (a.fluid == 0) {
+            side_value<GasPhys, double, double, TYPES>(M, N, D, m_axis, fx, fy, dis, a.prm.gamma, r, bad);
+            return;
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) r[k] = M.s[k];
+        if (a.fluid == 1) {
+            polar_perenos(M.cx, M.cy, fx, fy, r[2], r[3]);
+            mf_tvd_radial_h1(M, N, D, m_axis, fx, fy, a.prm.h_pow, r[0], r[1]);
+        }
+        return;
+    }
+    int b = 0;
+    tvd_usualy<GasPhys, double, double>(M, N, D, m_axis, fx, fy, r, b);
+    if (a.fluid == 0) bad += b;
+}
+
+template <bool TYPES>
+__device__ __forceinline__ void mf_interior_states(const SweepArgs& a, const CellT<GasPhys, double>& M,
+                                                   const CellT<GasPhys, double>& N, const CellT<GasPhys, double>& D,
+                                                   const CellT<GasPhys, double>& U, bool m_axis, bool n_axis,
+                                                   const Face& f, double* L, double* R, int& bad)
+{
+    mf_side_value<TYPES>(a, M, N, D, m_axis, f.fx, f.fy, f.dis, L, bad);
+    mf_side_value<TYPES>(a, N, M, U, n_axis, f.fx, f.fy, f.dis, R, bad);
+}
+
+// Boundary faces for the fluid of the launch (Gran.cpp:205-289, Setka.cpp:9071-9104):
+// the plasma as the single-fluid sweep; H1 rescales ro_H1 on the inner sphere and on the
+// axis of C_1 cells and meets the hydrogen inflow of the inner sphere; H4 meets the
+// interstellar hydrogen on Input faces; the other ghosts copy the cell.
+template <bool TYPES>
+__device__ __forceinline__ void mf_boundary_states(const SweepArgs& a, int ftype, const CellT<GasPhys, double>& M,
+                                                   const CellT<GasPhys, double>& D, const Face& f, double* L,
+                                                   double* R)
+{
+    if (a.fluid == 0) {
+        boundary_states<GasPhys, double, double, TYPES>(a, ftype, M, D, f, L, R);
+        return;
+    }
+    const gel_params& p = a.prm;
+#pragma unroll
+    for (int k = 0; k < 5; k++) L[k] = M.s[k];
+    const double radius = sqrt(sq(M.cx) + sq(M.cy));
+    if (ftype == GEL_F_INNER_SPHERE) {
+        if (a.fluid == 1) L[0] = L[0] * pow(radius / f.dis, p.h_pow);
+#pragma unroll
+        for (int k = 0; k < 5; k++) R[k] = M.s[k];
+        if (a.fluid == 1) {
+            const double d = sqrt(f.fx * f.fx + f.fy * f.fy);
+            R[0] = p.h1_in_rho; R[1] = p.h1_in_p; R[2] = p.in_u * f.fx / d; R[3] = p.in_u * f.fy / d;
+        }
+    } else if (ftype == GEL_F_AXIS) {
+        if (TYPES && M.type == GEL_C_1 && a.fluid == 1) {
+            L[0] = L[0] * pow(radius / f.dis, p.h_pow);
+            polar_perenos(M.cx, M.cy, f.fx, f.fy, L[2], L[3]);
+        }
+#pragma unroll
+        for (int k = 0; k < 5; k++) R[k] = (k == 3) ? -L[k] : L[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 5; k++) R[k] = L[k];
+        if (ftype == GEL_F_INPUT && a.fluid == 4) {
+            R[0] = p.h4_inf_rho; R[1] = p.h4_inf_p; R[2] = p.h4_inf_u; R[3] = p.h4_inf_v;
+        }
+    }
+}
+
+// Charge exchange from the cell's 21 values (Setka.cpp:9419-9470, :9554-9617): for the
+// plasma q = (0, momentum x, momentum y, energy); for hydrogen fluid Hk its mass,
+// momentum and energy sources, the population of the cell's region (C_1/C_centr H1,
+// C_2/C_3 H2, C_4 H3, else H4) receiving what the plasma loses.
+__device__ __forceinline__ void mf_sources(const SweepArgs& a, size_t kc, int ct, double* q)
+{
+    const size_t nc = (size_t)a.ni * a.nj;
+    const double* __restrict__ st = (const double*)a.now + kc;
+    const double ro = st[0], p = st[nc], u = st[2 * nc], v = st[3 * nc];
+    double roH[4], pH[4], uH[4], vH[4], UM[4], UH[4], nu[4];
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        roH[h] = st[(size_t)(5 + 4 * h) * nc]; pH[h] = st[(size_t)(6 + 4 * h) * nc];
+        uH[h] = st[(size_t)(7 + 4 * h) * nc]; vH[h] = st[(size_t)(8 + 4 * h) * nc];
+    }
+    const double pi = 3.14159265358979323846;         // Solvers.h:9
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        UM[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (64.0 / (9.0 * pi)) * (p / ro + 2.0 * pH[h] / roH[h]));
+        UH[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (4.0 / pi) * (p / ro + 2.0 * pH[h] / roH[h]));
+        nu[h] = ro * roH[h] * UM[h] * sq(a.prm.sig_a - a.prm.sig_b * log(UM[h]));
+    }
+    if (a.fluid == 0) {
+        const double cp = a.prm.src_coef;
+        q[0] = 0.0;
+        q[1] = cp * (nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u));
+        q[2] = cp * (nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v));
+        double e[4];
+#pragma unroll
+        for (int h = 0; h < 4; h++)
+            e[h] = nu[h] * ((sq(uH[h]) + sq(vH[h]) - sq(u) - sq(v)) / 2.0 + (UH[h] / UM[h]) * (2.0 * pH[h] / roH[h] - p / ro));
+        q[3] = cp * (e[0] + e[1] + e[2] + e[3]);
+        return;
+    }
+    const int h = a.fluid - 1;
+    int kk = 0;
+    if (ct == GEL_C_CENTR || ct == GEL_C_1 || ct == GEL_C_2 || ct == GEL_C_3)
+        kk = (h == ((ct == GEL_C_1 || ct == GEL_C_CENTR) ? 0 : 1)) ? 1 : 0;
+    else
+        kk = (h == ((ct == GEL_C_4) ? 2 : 3)) ? 1 : 0;
+    const double S1 = nu[0] + nu[1] + nu[2] + nu[3];
+    const double S2 = nu[0] * ((sq(u) + sq(v)) / 2.0 + (UH[0] / UM[0]) * (p / ro))
+                    + nu[1] * ((sq(u) + sq(v)) / 2.0 + (UH[1] / UM[1]) * (p / ro))
+                    + nu[2] * ((sq(u) + sq(v)) / 2.0 + (UH[2] / UM[2]) * (p / ro))
+                    + nu[3] * ((sq(u) + sq(v)) / 2.0 + (UH[3] / UM[3]) * (p / ro));
+    const double ch = a.prm.src_coef_h;
+    q[0] = ch * (kk * S1 - nu[h]);
+    q[1] = ch * (kk * S1 * u - nu[h] * uH[h]);
+    q[2] = ch * (kk * S1 * v - nu[h] * vH[h]);
+    q[3] = ch * (kk * S2 - nu[h] * ((sq(uH[h]) + sq(vH[h])) / 2.0 + (UH[h] / UM[h]) * 2.0 * (pH[h] / roH[h])));
+}
+
+// Update of a hydrogen fluid (Setka.cpp:9619-9753): the gas update with a mass source
+// and the fluid's own floors.
+__device__ __forceinline__ void h_update(const double* s, const double* Pot, double Volume, double Volume2, double vs1,
+                                         double vs2, double T1, double gam, const double* q, double ro_floor,
+                                         double p_floor, double* o)
+{
+    const double ro = s[0], p = s[1], u = s[2], v = s[3];
+    double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - q[0]);
+    if (ro3 <= 0.0) ro3 = ro_floor;
+    const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q[1])) / ro3;
+    const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q[2])) / ro3;
+    double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q[3])) -
+                 0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+    if (p3 <= 0) p3 = p_floor;
+    o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3;
+}
+
+// Flag or-ed into the FLUX template argument: the plasma + hydrogen sweep.
+constexpr int kMfFlux = 32;
+
 // Flag or-ed into the FLUX template argument of the sweep on grids with fitted surfaces.
 constexpr int kTrackFlux = 16;
 
@@ -742,9 +925,12 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 {
     // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
     constexpr bool TRACK = (FLUX & kTrackFlux) != 0;
-    constexpr int FX = FLUX & ~kTrackFlux;
+    constexpr bool MF = (FLUX & kMfFlux) != 0;        // plasma + hydrogen, one fluid per launch
+    constexpr int FX = FLUX & 15;
     static_assert(!TRACK || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && MOVING),
                   "fitted surfaces: gas, FP64, moving grids");
+    static_assert(!MF || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && !TRACK),
+                  "plasma + hydrogen: gas, FP64");
     constexpr int NV = Ph::NV;
     constexpr int NR = NW + 3;
     using C = CellT<Ph, TS>;
@@ -777,10 +963,10 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
         if constexpr (MOVING) {
             const Nodes lo = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r, j, col);
             const Nodes hi = load_nodes<MOVING>(a, a.x, a.y, a.s_old, r + 1, j, col);
-            c = load_cell<Ph, TS, TYPES>(a, r, j, col, lo, hi);
+            c = load_cell<Ph, TS, TYPES, true, MF>(a, r, j, col, lo, hi);
         } else {
             const Nodes none{ 1.0, 1.0, 1.0, 1.0 };
-            c = load_cell<Ph, TS, TYPES, false>(a, r, j, col, none, none);
+            c = load_cell<Ph, TS, TYPES, false, MF>(a, r, j, col, none, none);
             if (c.ok) {
                 const size_t k = (size_t)r * nj + j;
                 c.cx = G.cell(0)[k];
@@ -885,14 +1071,17 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 // a face of a fitted surface: both nodes on surface row i + 1
                 const bool god = TRACK && !inner && i + 1 < ni && surface_row(a, i + 1);
                 if (inner) {
-                    boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
+                    if constexpr (MF) mf_boundary_states<TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
+                    else boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                 } else if (i + 1 >= ni) {
                     const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
-                    boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
+                    if constexpr (MF) mf_boundary_states<TYPES>(a, ot, M, D, f, L, R);
+                    else boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
                 } else if (god) {
                     if constexpr (TRACK) surface_states<Ph, TYPES>(M, N, D, U, m_axis, f, gam, L, R, b);
                 } else {
-                    interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
+                    if constexpr (MF) mf_interior_states<TYPES>(a, M, N, D, U, m_axis, m_axis, f, L, R, b);
+                    else interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
                 }
                 TF uu;
                 if constexpr (TRACK) {
@@ -923,9 +1112,15 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
                 double L[NV], R[NV], P[NV];
                 int b = 0;
-                if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
-                else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
-                else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
+                if constexpr (MF) {
+                    if (axl) mf_boundary_states<TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
+                    else if (axis_hi) mf_boundary_states<TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
+                    else mf_interior_states<TYPES>(a, M, Mr, Ml, Mr2, m_axis, n_axis, f, L, R, b);
+                } else {
+                    if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
+                    else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
+                    else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
+                }
                 TF uu;
                 if constexpr (TRACK && FX == GEL_FLUX_HLLC) {
                     // one node of the face on a surface row: nul_potok (Setka.cpp:10299-10327)
@@ -994,10 +1189,37 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 time = mn(time, t_dn);
                 time = mn(time, t_pol);
                 time = mn(time, S.ftm[fb][lane]);
-                acc.tmin = fmin(acc.tmin, time);
-
                 const size_t kc = (size_t)i * nj + j;
                 TS* __restrict__ nxt = (TS*)a.next;
+                if constexpr (MF) {
+                    // C_centr cells solve no H1 faces (Setka.cpp:9172) and keep the plasma
+                    // and H1 (:9525-9531, :9645-9651); H2..H4 update everywhere
+                    const int ct = TYPES ? M.type : (int)GEL_C_2;
+                    if (!(a.fluid == 1 && ct == GEL_C_CENTR)) acc.tmin = fmin(acc.tmin, time);
+                    const int nw = a.fluid == 0 ? 5 : 4;
+                    double o[5];
+                    if (ct == GEL_C_CENTR && a.fluid <= 1) {
+#pragma unroll
+                        for (int k = 0; k < 5; k++) o[k] = M.s[k];
+                    } else {
+                        double q[4];
+                        mf_sources(a, kc, ct, q);
+                        double sd[5];
+#pragma unroll
+                        for (int k = 0; k < 5; k++) sd[k] = M.s[k];
+                        if (a.fluid == 0) {
+                            cell_update<Ph, false>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, 0.0, q[1], q[2], q[3], 0.0, false,
+                                                   o, acc.n_rho, acc.n_p);
+                        } else {
+                            // floors of Setka.cpp:9627, :9642, :9672, :9687, :9701, :9716, :9733, :9748
+                            const double rf = a.fluid == 1 ? 0.0000001 : 0.000001;
+                            const double pf = a.fluid == 4 ? 0.0000001 : 0.000001;
+                            h_update(sd, Pot, vol, vol2, vs1, vs2, dt, gam, q, rf, pf, o);
+                        }
+                    }
+                    for (int k = 0; k < nw; k++) nxt[(size_t)a.voff[k] * nc + kc] = TS(o[k]);
+                } else {
+                acc.tmin = fmin(acc.tmin, time);
                 if (TYPES && M.type == GEL_C_CENTR) {
 #pragma unroll
                     for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = M.s[k];
@@ -1024,6 +1246,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                                     acc.n_p);
 #pragma unroll
                     for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
+                }
                 }
             }
         }

@@ -15,7 +15,7 @@ import numpy as np
 
 from . import _lib
 from .grid import PolarGrid
-from .params import GEL_MK1_NFIELDS, GEL_PHYS_MHD, NVAR_GAS, NVAR_MHD, GelParams
+from .params import GEL_MK1_NFIELDS, GEL_PHYS_GAS4H, GEL_PHYS_MHD, NVAR_GAS, NVAR_GAS4H, NVAR_MHD, GelParams
 
 
 class Setka:
@@ -33,7 +33,7 @@ class Setka:
                          _lib.i8ptr(grid.cell_type), _lib.i8ptr(grid.outer_type))
         _lib.check(_lib.lib().gel_ctx_create(ctypes.byref(self._ctx), ctypes.byref(g), ctypes.byref(self.params)))
         self.stats = _lib.GelStats()
-        self.nvar = NVAR_MHD if self.params.physics == GEL_PHYS_MHD else NVAR_GAS
+        self.nvar = {GEL_PHYS_MHD: NVAR_MHD, GEL_PHYS_GAS4H: NVAR_GAS4H}.get(self.params.physics, NVAR_GAS)
 
     def close(self):
         if self._ctx:
