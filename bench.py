@@ -24,6 +24,8 @@ Workloads
                    (exact-solver surface speeds, A-rail redistribution, sweep with
                    exact-solver surface faces), 799 x 400 cells, FP64 HLL, the
                    reference's constants (CFL 0.1), 200 steps
+  plasma-4h-800x400  the plasma and four hydrogen fluids (21 fields, HLLC, charge
+                   exchange), 800 x 400 cells, static, FP64, 200 steps
 """
 from __future__ import annotations
 
@@ -51,7 +53,7 @@ def bytes_per_cell_update(w, fallback_fraction=0.0):
     table (5 x 8 B), the hydrogen moments (19 x 8 B), or in the hybrid mode a flag
     byte and the moments for the fallback fraction of cells, the table for the rest."""
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID, GEL_SRC_MK1, GEL_SRC_TABLE
-    nv = 9 if w.params.physics == 1 else 5
+    nv = {1: 9, 2: 21}.get(w.params.physics, 5)
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
     geom = 32 if (getattr(w, "motion", None) or getattr(w, "tracking", None)) else 16
     src = {GEL_SRC_TABLE: 40, GEL_SRC_MK1: 152}.get(w.params.source_mode, 0)
@@ -298,6 +300,8 @@ def main():
             w = configs.mhd_heliosphere(4096, 2048, 20, seed=2, name="hlld-fp64-4096x2048-moving")
         elif wl == "large-gas":
             w = configs.large()
+        elif wl == "plasma-4h-800x400":
+            w = configs.multifluid_heliosphere(800, 400, 200, seed=9, name="plasma-4h-hllc-fp64-800x400")
         elif wl == "tracked-800x400":
             w = configs.tracked_heliosphere(400, (200, 150, 150, 300), 200, seed=8, name="tracked-hll-fp64-799x400")
         else:
@@ -323,7 +327,8 @@ def main():
         "data": "synthetic",
         "config": config if config is not None else {
             "workload": main_w.name, "cells": main_w.grid.ncells, "grid": [main_w.grid.ni, main_w.grid.nj],
-            "physics": "mhd-hlld" if main_w.params.physics == 1 else ("gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
+            "physics": {1: "mhd-hlld", 2: "plasma+4H-hllc"}.get(main_w.params.physics,
+                                                                 "gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
             "moving_grid": bool(getattr(main_w, "motion", None) or getattr(main_w, "tracking", None)),
             "fitted_surfaces": getattr(main_w, "tracking", None) is not None,
             "precision": precision, "cfl": main_w.params.cfl},

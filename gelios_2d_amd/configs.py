@@ -262,3 +262,42 @@ def multifluid_heliosphere(ni: int, nj: int, steps: int, seed: int, flux: int = 
         rh = scale * prm.h4_inf_rho * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, ro.shape))
         fields += [rh, scale * prm.h4_inf_p + 0.0 * r, (0.5 + 0.5 * scale) * prm.h4_inf_u + 0.0 * r, 0.0 * r]
     return Workload(name, g, prm, np.ascontiguousarray(np.stack(fields)), steps)
+
+
+def inner_region(ni: int, nj: int, steps: int, seed: int, flux: int = 1, sources: bool = False,
+                 name: str = "inner-region") -> Workload:
+    """The region the inner-region solver (Setka::Go_stationary_5_komponent_inner_MK2)
+    advances: r in [R1_, R111_] = [1, 50] AU / 390.222, theta in [0, pi], log-spaced
+    rows; the supersonic wind of the Inner_sphere formula (Gran.cpp:234-265) with +-1%
+    seeded density noise; C_1 cells up to 3/4 of the rows, C_2 beyond, C_centr cells at
+    both axis ends of the first row; the outer faces copy the cell (Upper_wall).  With
+    sources: seeded hydrogen moments for the MK1 closure."""
+    from .grid import polar_grid
+    from .params import (GEL_C_1, GEL_C_2, GEL_C_CENTR, GEL_F_UPPER_WALL, GEL_PHYS_GAS_INNER,
+                         GEL_SRC_MK1, RR_, reference_params)
+    prm = reference_params()
+    prm.physics = GEL_PHYS_GAS_INNER
+    prm.flux = flux
+    if sources:
+        prm.source_mode = GEL_SRC_MK1
+    g = polar_grid(ni, nj, 1.0 / RR_, 50.0 / RR_, spacing="log")
+    i = np.repeat(np.arange(ni), nj)
+    t = np.where(i < (3 * ni) // 4, GEL_C_1, GEL_C_2).astype(np.int8)
+    t[0] = GEL_C_CENTR
+    t[nj - 1] = GEL_C_CENTR
+    g.cell_type = t
+    g.outer_type = np.full(nj, GEL_F_UPPER_WALL, np.int8)
+    cx, cy = g.cell_centers()
+    r = np.hypot(cx, cy)
+    ro = 1.0 / (prm.in_rho_div * r * r)
+    p = ro * prm.in_p_a * prm.in_p_a / prm.in_p_den
+    u, v = prm.in_u * cx / r, prm.in_u * cy / r
+    rng = np.random.default_rng(seed)
+    prim = np.stack([ro * (1.0 + 0.01 * rng.uniform(-1.0, 1.0, ro.shape)), p, u, v, ro.copy()])
+    w = Workload(name, g, prm, np.ascontiguousarray(prim), steps)
+    if sources:
+        nc = g.ncells
+        w.src = np.ascontiguousarray(np.concatenate([rng.uniform(0.05, 0.5, (4, nc)), rng.normal(-2.0, 0.5, (4, nc)),
+                                                     rng.normal(0.0, 0.5, (4, nc)), rng.uniform(0.5, 2.0, (4, nc)),
+                                                     rng.uniform(0.5, 2.0, (3, nc))]))
+    return w

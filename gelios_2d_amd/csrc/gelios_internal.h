@@ -74,6 +74,8 @@ namespace gel {
 hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a);
 // One step of the plasma + hydrogen sweep (multifluid.hip): five launches, one per fluid.
 hipError_t launch_mf_step(gel_ctx* c, SweepArgs& a, bool moving);
+// One step of the inner-region solver (inner.hip), static grids.
+hipError_t launch_inner_step(gel_ctx* c, SweepArgs& a);
 }
 
 struct gel_ctx {

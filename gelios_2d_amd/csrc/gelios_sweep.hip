@@ -143,8 +143,10 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     if (!out || !g || !prm || !g->x || !g->y) return fail(GEL_E_ARG, "null argument");
     *out = nullptr;
     if (g->ni < 3 || g->nj < 3) return fail(GEL_E_ARG, "grid needs at least 3x3 cells");
-    if (prm->physics != GEL_PHYS_GAS && prm->physics != GEL_PHYS_MHD && prm->physics != GEL_PHYS_GAS4H)
-        return fail(GEL_E_UNSUPPORTED, "physics");
+    if (prm->physics < GEL_PHYS_GAS || prm->physics > GEL_PHYS_GAS_INNER) return fail(GEL_E_UNSUPPORTED, "physics");
+    if (prm->physics == GEL_PHYS_GAS_INNER && (prm->precision != GEL_PREC_FP64 ||
+                                               (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1)))
+        return fail(GEL_E_UNSUPPORTED, "inner-region solver: FP64, no source or the MK1 source");
     if (prm->physics != GEL_PHYS_MHD && prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC)
         return fail(GEL_E_UNSUPPORTED, "flux");
     if (prm->physics == GEL_PHYS_GAS4H && (prm->precision != GEL_PREC_FP64 || prm->source_mode != GEL_SRC_NONE))
@@ -202,7 +204,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
             if (t == GEL_C_1 || t == GEL_C_CENTR) any = true;
         }
         // the plasma + hydrogen sweep assigns hydrogen populations by every cell type
-        if (any || prm->physics == GEL_PHYS_GAS4H) {
+        if (any || prm->physics == GEL_PHYS_GAS4H || prm->physics == GEL_PHYS_GAS_INNER) {
             if (prm->physics == GEL_PHYS_MHD || prm->precision != GEL_PREC_FP64) {
                 free_ctx(c);
                 return fail(GEL_E_UNSUPPORTED, "C_1 / C_centr cells need FP64 gas physics");
@@ -326,6 +328,7 @@ int gel_set_rows_per_wave(gel_ctx* c, int rows)
 int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
 {
     if (!c || period < 1 || !(amp >= 0.0) || !(amp < 0.5)) return fail(GEL_E_ARG, "motion: amp in [0,0.5), period >= 1");
+    if (c->prm.physics == GEL_PHYS_GAS_INNER) return fail(GEL_E_UNSUPPORTED, "the inner-region solver is static");
     if (c->tracking) return fail(GEL_E_STATE, "fitted-surface tracking is on");
     c->motion.assign((size_t)period, 1.0);
     const double pi = 3.14159265358979323846;
@@ -358,7 +361,7 @@ int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
     if (!t) { c->tracking = false; return GEL_OK; }
     if (!rail) return fail(GEL_E_ARG, "rail angles needed");
     if (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64)
-        return fail(GEL_E_UNSUPPORTED, "fitted surfaces: gas physics in FP64");
+        return fail(GEL_E_UNSUPPORTED, "fitted surfaces: gas physics in FP64 (the single-fluid sweep)");
     if (c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)
         return fail(GEL_E_UNSUPPORTED, "fitted surfaces: no source or the MK1 source");
     if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
@@ -433,6 +436,10 @@ static hipError_t launch(gel_ctx* c, SweepArgs& a, bool moving)
     hipStream_t s = c->stream;
     if (c->prm.physics == GEL_PHYS_MHD) return launch_ms<MhdPhys, double, double, false, 3>(c, a, s, moving, src);
     if (c->prm.physics == GEL_PHYS_GAS4H) return launch_mf_step(c, a, moving);
+    if (c->prm.physics == GEL_PHYS_GAS_INNER) {
+        if (moving) return hipErrorInvalidValue;           // rejected by the step entry points first
+        return launch_inner_step(c, a);
+    }
     const bool hllc = (c->prm.flux == GEL_FLUX_HLLC);
     if (c->has_types) {
         return hllc ? launch_ms<GasPhys, double, double, true, GEL_FLUX_HLLC>(c, a, s, moving, src)
@@ -592,6 +599,7 @@ int gel_step_moving(gel_ctx* c, const double* x2, const double* y2, gel_stats* o
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
     if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
     if (c->tracking) return fail(GEL_E_STATE, "fitted-surface tracking moves the nodes");
+    if (c->prm.physics == GEL_PHYS_GAS_INNER) return fail(GEL_E_UNSUPPORTED, "the inner-region solver is static");
     const size_t nb = c->nn * sizeof(double);
     HIP_OK(hipMemcpyAsync(c->d_x2, x2, nb, hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipMemcpyAsync(c->d_y2, y2, nb, hipMemcpyHostToDevice, c->stream));
@@ -861,7 +869,8 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c || !c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->has_types || !c->motion.empty() || c->tracking || c->prm.physics == GEL_PHYS_GAS4H)
+        if (c->has_types || !c->motion.empty() || c->tracking || c->prm.physics == GEL_PHYS_GAS4H ||
+            c->prm.physics == GEL_PHYS_GAS_INNER)
             return fail(GEL_E_UNSUPPORTED, "batched cases are static single-fluid grids without C_1/C_centr");
         if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
             c->prm.source_mode != p0.source_mode)

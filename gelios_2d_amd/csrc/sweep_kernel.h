@@ -872,6 +872,72 @@ __device__ __forceinline__ void h_update(const double* s, const double* Pot, dou
 // Flag or-ed into the FLUX template argument: the plasma + hydrogen sweep.
 constexpr int kMfFlux = 32;
 
+// -------------------------------------------------------------- inner region ---
+// Setka::Go_stationary_5_komponent_inner_MK2 (Setka.cpp:6999-7212): the static solver of
+// the inner region, the gas sweep with its own face values and update.
+constexpr int kInnerFlux = 64;
+
+// Both sides of an interior face: the cells' values rescaled to the face radius and
+// rotated to its polar frame, density and pressure from the radial reconstruction
+// where both cells are C_1 (Setka.cpp:7064-7091).
+template <bool TYPES>
+__device__ __forceinline__ void inner_interior_states(const SweepArgs& a, const CellT<GasPhys, double>& M,
+                                                      const CellT<GasPhys, double>& N, const CellT<GasPhys, double>& D,
+                                                      const CellT<GasPhys, double>& U, bool m_axis, bool n_axis,
+                                                      const Face& f, double* L, double* R)
+{
+    const double gam = a.prm.gamma;
+#pragma unroll
+    for (int k = 0; k < 5; k++) { L[k] = M.s[k]; R[k] = N.s[k]; }
+    rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
+    rescale_c1<5>(R, sqrt(sq(N.cx) + sq(N.cy)), f.dis, gam, N.cx, N.cy, f.fx, f.fy, true);
+    if (TYPES && M.type == GEL_C_1 && N.type == GEL_C_1) {
+        tvd_radial<GasPhys>(M, N, D, m_axis, f.fx, f.fy, gam, L[0], L[1]);
+        tvd_radial<GasPhys>(N, M, U, n_axis, f.fx, f.fy, gam, R[0], R[1]);
+    }
+}
+
+// Boundary faces (Setka.cpp:7092-7114): the inner sphere's ghost on both sides; on the
+// axis the rescaled, rotated cell value against its mirror image; other faces the
+// cell's value against the ghost of Gran::Get_par.
+template <bool TYPES>
+__device__ __forceinline__ void inner_boundary_states(const SweepArgs& a, int ftype, const CellT<GasPhys, double>& M,
+                                                      const CellT<GasPhys, double>& D, const Face& f, double* L,
+                                                      double* R)
+{
+    if (ftype == GEL_F_AXIS) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) L[k] = M.s[k];
+        rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, a.prm.gamma, M.cx, M.cy, f.fx, f.fy, true);
+#pragma unroll
+        for (int k = 0; k < 5; k++) R[k] = (k == 3) ? -L[k] : L[k];
+        return;
+    }
+    boundary_states<GasPhys, double, double, TYPES>(a, ftype, M, D, f, L, R);
+    if (ftype == GEL_F_INNER_SPHERE) {
+#pragma unroll
+        for (int k = 0; k < 5; k++) L[k] = R[k];
+    }
+}
+
+// Update (Setka.cpp:7168-7200): static, per r-weighted volume, the tracer losing a bare
+// dt (Setka.cpp:7173).
+__device__ __forceinline__ void inner_update(const double* s, const double* Pot, double Volume, double vs1, double vs2,
+                                             double T1, double gam, double q1, double q2, double q3, double* o,
+                                             int& n_rho, int& n_p)
+{
+    const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
+    double ro3 = ro - T1 * (Pot[0] / Volume);
+    const double Q33 = Q - (T1 / Volume) * Pot[4] - T1;
+    if (ro3 <= 0) { ro3 = 0.00001; n_rho++; }
+    const double u3 = (ro * u - T1 * (Pot[1] / Volume - q1)) / ro3;
+    const double v3 = (ro * v - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume - q2)) / ro3;
+    double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) - T1 * (Pot[3] / Volume - q3)) -
+                 0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+    if (p3 <= 0) { p3 = 0.000001; n_p++; }
+    o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = Q33;
+}
+
 // Flag or-ed into the FLUX template argument of the sweep on grids with fitted surfaces.
 constexpr int kTrackFlux = 16;
 
@@ -926,7 +992,10 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
     // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
     constexpr bool TRACK = (FLUX & kTrackFlux) != 0;
     constexpr bool MF = (FLUX & kMfFlux) != 0;        // plasma + hydrogen, one fluid per launch
+    constexpr bool INNER = (FLUX & kInnerFlux) != 0;  // the inner-region solver
     constexpr int FX = FLUX & 15;
+    static_assert(!INNER || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && !MOVING && !TRACK && !MF),
+                  "inner region: gas, FP64, static");
     static_assert(!TRACK || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && MOVING),
                   "fitted surfaces: gas, FP64, moving grids");
     static_assert(!MF || (Ph::NV == 5 && sizeof(TS) == 8 && sizeof(TF) == 8 && !TRACK),
@@ -1072,15 +1141,18 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 const bool god = TRACK && !inner && i + 1 < ni && surface_row(a, i + 1);
                 if (inner) {
                     if constexpr (MF) mf_boundary_states<TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
+                    else if constexpr (INNER) inner_boundary_states<TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                     else boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                 } else if (i + 1 >= ni) {
                     const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
                     if constexpr (MF) mf_boundary_states<TYPES>(a, ot, M, D, f, L, R);
+                    else if constexpr (INNER) inner_boundary_states<TYPES>(a, ot, M, D, f, L, R);
                     else boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
                 } else if (god) {
                     if constexpr (TRACK) surface_states<Ph, TYPES>(M, N, D, U, m_axis, f, gam, L, R, b);
                 } else {
                     if constexpr (MF) mf_interior_states<TYPES>(a, M, N, D, U, m_axis, m_axis, f, L, R, b);
+                    else if constexpr (INNER) inner_interior_states<TYPES>(a, M, N, D, U, m_axis, m_axis, f, L, R);
                     else interior_states<Ph, TS, TF, TYPES>(M, N, D, U, m_axis, m_axis, f, gam, L, R, b);
                 }
                 TF uu;
@@ -1116,6 +1188,10 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     if (axl) mf_boundary_states<TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
                     else if (axis_hi) mf_boundary_states<TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
                     else mf_interior_states<TYPES>(a, M, Mr, Ml, Mr2, m_axis, n_axis, f, L, R, b);
+                } else if constexpr (INNER) {
+                    if (axl) inner_boundary_states<TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
+                    else if (axis_hi) inner_boundary_states<TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
+                    else inner_interior_states<TYPES>(a, M, Mr, Ml, Mr2, m_axis, n_axis, f, L, R);
                 } else {
                     if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
                     else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
@@ -1129,6 +1205,9 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     } else {
                         uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
                     }
+                } else if constexpr (INNER) {
+                    // the axis takes the HLL flux (met = 0, Setka.cpp:7108)
+                    uu = (axl || axis_hi) ? face_flux<Ph, TF, GEL_FLUX_HLL>(L, R, f, gam, P) : face_flux<Ph, TF, FX>(L, R, f, gam, P);
                 } else {
                     uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
                 }
@@ -1171,7 +1250,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
             const int fa = fslot(i), fb = fslot(i - 1);
             const double x = M.cx, y = M.cy;
             const double radius = sqrt(sq(x) + sq(y));
-            if (out_lane && !(radius < a.prm.r_skip)) {
+            if (out_lane && (INNER || !(radius < a.prm.r_skip))) {
                 // faces in the reference's order: edge 0 (polar, left), 1 (radial, above),
                 // 2 (polar, right), 3 (radial, below)
                 double Pot[NV];
@@ -1191,7 +1270,29 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 time = mn(time, S.ftm[fb][lane]);
                 const size_t kc = (size_t)i * nj + j;
                 TS* __restrict__ nxt = (TS*)a.next;
-                if constexpr (MF) {
+                if constexpr (INNER) {
+                    // C_centr cells are skipped: no time step, no update (Setka.cpp:7025-7028)
+                    const int ct = TYPES ? M.type : (int)GEL_C_2;
+                    double o[5];
+                    if (ct == GEL_C_CENTR) {
+#pragma unroll
+                        for (int k = 0; k < 5; k++) o[k] = M.s[k];
+                    } else {
+                        acc.tmin = fmin(acc.tmin, time);
+                        double q1s = 0.0, q2s = 0.0, q3s = 0.0;
+                        if (SRC == GEL_SRC_MK1) {
+                            const double* __restrict__ sp = a.src + kc;
+                            auto fk = [&](int k) { return sp[(size_t)k * nc]; };
+                            source_mk1<decltype(fk), false>(a.prm, fk, M.s[2], M.s[3], M.s[0], M.s[1], q1s, q2s, q3s);
+                        }
+                        double sd[5];
+#pragma unroll
+                        for (int k = 0; k < 5; k++) sd[k] = M.s[k];
+                        inner_update(sd, Pot, vol, vs1, vs2, dt, gam, q1s, q2s, q3s, o, acc.n_rho, acc.n_p);
+                    }
+#pragma unroll
+                    for (int k = 0; k < 5; k++) nxt[(size_t)k * nc + kc] = TS(o[k]);
+                } else if constexpr (MF) {
                     // C_centr cells solve no H1 faces (Setka.cpp:9172) and keep the plasma
                     // and H1 (:9525-9531, :9645-9651); H2..H4 update everywhere
                     const int ct = TYPES ? M.type : (int)GEL_C_2;

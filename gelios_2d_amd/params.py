@@ -13,7 +13,7 @@ import math
 GEL_F_USUALY, GEL_F_INNER_SPHERE, GEL_F_EXTERN, GEL_F_INPUT, GEL_F_AXIS, GEL_F_UPPER_WALL = range(6)
 GEL_C_1, GEL_C_2, GEL_C_3, GEL_C_4, GEL_C_5, GEL_C_CENTR, GEL_C_NO = range(7)
 GEL_FLUX_HLL, GEL_FLUX_HLLC = 0, 1
-GEL_PHYS_GAS, GEL_PHYS_MHD, GEL_PHYS_GAS4H = 0, 1, 2
+GEL_PHYS_GAS, GEL_PHYS_MHD, GEL_PHYS_GAS4H, GEL_PHYS_GAS_INNER = 0, 1, 2, 3
 NVAR_GAS4H = 21
 NVAR_MHD = 9
 GEL_SRC_NONE, GEL_SRC_MK1, GEL_SRC_TABLE, GEL_SRC_HYBRID = 0, 1, 2, 3

@@ -80,7 +80,13 @@ enum gel_flux { GEL_FLUX_HLL = 0, GEL_FLUX_HLLC = 1 };
  * ro, p, u, v, Q, then ro_Hk, p_Hk, u_Hk, v_Hk for k = 1..4; every fluid takes the
  * HLL/HLLC face flux, the fluids exchange mass, momentum and energy by charge exchange
  * (Setka.cpp:9400-9617); FP64, no separate source mode. */
-enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1, GEL_PHYS_GAS4H = 2 };
+enum gel_physics { GEL_PHYS_GAS = 0, GEL_PHYS_MHD = 1, GEL_PHYS_GAS4H = 2, GEL_PHYS_GAS_INNER = 3 };
+/* GAS_INNER: the 5 gas primitives advanced by the static inner-region solver
+ * Setka::Go_stationary_5_komponent_inner_MK2 (Setka.cpp:6999-7212): both sides of every
+ * interior face rescaled radially to the face (radial reconstruction where both cells
+ * are C_1), ghost values on both sides of the inner sphere, HLL on the axis, no
+ * r_skip, C_centr cells skipped, the tracer losing a bare dt per step
+ * (Setka.cpp:7173).  FP64, static grids. */
 
 /* FP64: every operation in FP64 (the reference's arithmetic).
  * FP32_BULK: state stored in FP32 and face reconstruction + Riemann solver in FP32;

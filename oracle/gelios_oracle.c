@@ -1684,3 +1684,120 @@ int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, dou
     if (n_p_clamp) *n_p_clamp += npc;
     return bad_any ? GEL_E_NUMERIC : GEL_OK;
 }
+
+/* ------------------------------------------------------------ inner region --- */
+
+/* Setka::Go_stationary_5_komponent_inner_MK2 (Setka.cpp:6999-7212): the static solver
+ * of the inner region.  Every interior face takes both cells' values rescaled to the
+ * face radius (density and tracer by r^2, pressure by r^(2 gamma), velocity rotated to
+ * the face's polar frame), replaced by the radial reconstruction of density and
+ * pressure where both cells are C_1; inner-sphere faces take the ghost on both sides;
+ * axis faces the rescaled, mirrored cell value with the HLL flux; C_centr cells are
+ * skipped; the update is the static one with the tracer losing a bare dt per step
+ * (Setka.cpp:7173). */
+int ora_step_inner(const ora_mesh* m, const gel_params* prm, const double* now, double* next, const double* src,
+                   double dt_prev, double* dt_new, long long* n_rho_clamp, long long* n_p_clamp, int nthreads)
+{
+    const int nc = m->n_cells;
+    const double gam = prm->gamma;
+    double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
+    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    sweep_ctx S = { m, prm, now, nc, cx, cy };
+    double Tmin = 100000000;
+    long long nrc = 0, npc = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc)
+#endif
+    for (int c = 0; c < nc; c++) {
+        if (m->cell_type[c] == GEL_C_CENTR) continue;             /* Setka.cpp:7025-7028 */
+        const int* q = m->contour + 4 * c;
+        const double x = cx[c], y = cy[c];
+        const double vs1 = ora_cell_area(m->x, m->y, q);
+        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double radius = sqrt(sq(x) + sq(y));
+        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
+        const ora_gas par1 = load(&S, c);
+        const int K1 = (m->cell_type[c] == GEL_C_1);
+        double Pot[5] = { 0, 0, 0, 0, 0 };
+        double time = 10000000;
+        for (int k = 0; k < m->nface[c]; k++) {
+            const int s = c * ORA_MAXF + k;
+            const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
+            const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+            const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
+            const double dist = sqrt(sq(x - fx) + sq(y - fy));
+            double n1, n2;
+            {
+                double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
+                const double nn = sqrt(sq(t1) + sq(t2));
+                t1 = t1 / nn; t2 = t2 / nn;
+                n1 = t2; n2 = -t1;
+            }
+            const double dis = sqrt(sq(fx) + sq(fy));
+            ora_gas p11 = par1;
+            ora_gas p2 = get_par(&S, c, s, fx, fy);
+            int met = prm->flux;
+            if (t == GEL_F_USUALY) {                               /* :7064-7091 */
+                const int N = m->fnbr[s];
+                const double dd = sqrt(sq(cx[N]) + sq(cy[N]));
+                p11.ro = p11.ro * sq(radius) / sq(dis);
+                p2.ro = p2.ro * sq(dd) / sq(dis);
+                p11.Q = p11.Q * sq(radius) / sq(dis);
+                p2.Q = p2.Q * sq(dd) / sq(dis);
+                p11.p = p11.p * pow(radius / dis, 2.0 * gam);
+                p2.p = p2.p * pow(dd / dis, 2.0 * gam);
+                ora_polar_perenos(x, y, fx, fy, &p11.u, &p11.v);
+                ora_polar_perenos(cx[N], cy[N], fx, fy, &p2.u, &p2.v);
+                if (K1 && m->cell_type[N] == GEL_C_1) {
+                    double ro1, pr1, ro2, pr2;
+                    tvd_radial(&S, c, N, m->fdown[s], fx, fy, &ro1, &pr1);
+                    tvd_radial(&S, N, c, m->fup[s], fx, fy, &ro2, &pr2);
+                    p11.ro = ro1; p2.ro = ro2;
+                    p11.p = pr1; p2.p = pr2;
+                }
+            } else if (t == GEL_F_INNER_SPHERE) {                 /* :7092-7099 */
+                p11 = p2;
+            } else if (t == GEL_F_AXIS) {                         /* :7100-7114 */
+                p11.ro = p11.ro * sq(radius) / sq(dis);
+                p11.Q = p11.Q * sq(radius) / sq(dis);
+                p11.p = p11.p * pow(radius / dis, 2.0 * gam);
+                ora_polar_perenos(x, y, fx, fy, &p11.u, &p11.v);
+                p2 = p11;
+                p2.v = -p2.v;
+                met = 0;
+            }
+            double P[4], PQ, Vl, Vc, Vp;
+            time = mn(time, ora_hllc2d(&p11, &p2, 0.0, n1, n2, dist, prm->cfl, gam, met, 0, P, &PQ, &Vl, &Vc, &Vp));
+            for (int v = 0; v < 4; v++) Pot[v] = Pot[v] + P[v] * S_;
+            Pot[4] = Pot[4] + PQ * S_;
+        }
+        if (time < Tmin) Tmin = time;
+        const double u = par1.u, v = par1.v, ro = par1.ro, p = par1.p, Q = par1.Q;
+        double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+        if (prm->source_mode == GEL_SRC_MK1 && src) {
+            double f[GEL_MK1_NFIELDS];
+            for (int k = 0; k < GEL_MK1_NFIELDS; k++) f[k] = src[(size_t)k * nc + c];
+            ora_source_mk1(prm, f, u, v, ro, p, &q1, &q2, &q3);
+        }
+        const double T1 = dt_prev;                                 /* :7168-7200 */
+        double ro3 = ro - T1 * (Pot[0] / Volume);
+        const double Q33 = Q - (T1 / Volume) * Pot[4] - T1;
+        if (ro3 <= 0) { ro3 = 0.00001; nrc++; }
+        const double u3 = (ro * u - T1 * (Pot[1] / Volume - q1)) / ro3;
+        const double v3 = (ro * v - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume - q2)) / ro3;
+        double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) - T1 * (Pot[3] / Volume - q3)) -
+                     0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
+        if (p3 <= 0) { p3 = 0.000001; npc++; }
+        next[c] = ro3; next[(size_t)nc + c] = p3; next[(size_t)2 * nc + c] = u3; next[(size_t)3 * nc + c] = v3;
+        next[(size_t)4 * nc + c] = Q33;
+    }
+    free(cx);
+    free(cy);
+    *dt_new = Tmin;
+    if (n_rho_clamp) *n_rho_clamp += nrc;
+    if (n_p_clamp) *n_p_clamp += npc;
+    return GEL_OK;
+}

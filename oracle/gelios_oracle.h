@@ -100,6 +100,11 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
                  const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
                  long long* n_p_clamp, int nthreads);
 
+/* The static inner-region solver Setka::Go_stationary_5_komponent_inner_MK2
+ * (Setka.cpp:6999-7212): 5 x n_cells gas primitives; src as for ora_step_mk2. */
+int ora_step_inner(const ora_mesh* m, const gel_params* prm, const double* now, double* next, const double* src,
+                   double dt_prev, double* dt_new, long long* n_rho_clamp, long long* n_p_clamp, int nthreads);
+
 /* The plasma and four hydrogen fluids of Setka::Go_5_komponent_2 (Setka.cpp:8951-9777):
  * 21 x n_cells primitives in Parametr order (ro, p, u, v, Q, then ro_Hk, p_Hk, u_Hk,
  * v_Hk for k = 1..4), every fluid with the HLL/HLLC flux (prm->flux), charge exchange

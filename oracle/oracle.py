@@ -81,6 +81,10 @@ def lib():
         L.ora_godunov.restype = ctypes.c_double
         L.ora_godunov.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        L.ora_step_inner.restype = ctypes.c_int
+        L.ora_step_inner.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, _dp,
+                                     ctypes.c_double, _dp, ctypes.POINTER(ctypes.c_longlong),
+                                     ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
         L.ora_step_mf.restype = ctypes.c_int
         L.ora_step_mf.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, _dp, ctypes.c_double, _dp,
                                   ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]
@@ -188,6 +192,8 @@ class OracleSweep:
                                    ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
         else:
             fn = lib().ora_step_mhd if self.nv == 9 else lib().ora_step_mk2
+            if self.prm.physics == 3:
+                fn = lib().ora_step_inner
             st = fn(ctypes.byref(m), ctypes.byref(self.prm), _d(now), _d(nxt), src, dtp, ctypes.byref(dt_new),
                     ctypes.byref(self.n_rho_clamp), ctypes.byref(self.n_p_clamp), nthreads)
         if dt_prev is None:

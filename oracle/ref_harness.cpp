@@ -544,9 +544,65 @@ static int mode_mf(const char* in, const char* out)
     return 0;
 }
 
+// ---- inner-region solver ------------------------------------------------------------
+// in : int32 ni nj steps has_src
+//      float64 x[nn] y[nn]; int32 cell_type[nc] outer_type[nj]
+//      float64 prim[5][nc]; if has_src: float64 src[19][nc]
+// Every cell of the polar grid is an inner-region cell (Setka::All_Cells_Inner); then
+// Go_stationary_5_komponent_inner_MK2(steps).  Get_Sourse_MK1 walks the radial Back/Next
+// chains (Cell.cpp:528-589), set to the rays; on a static grid the walk from a cell's
+// own radius returns the cell's own moments.
+// out: float64 prim[5][nc]
+static int mode_inner(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[4]; rd(f, h, 4);
+    const int ni = h[0], nj = h[1], steps = h[2], has_src = h[3];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn);
+    rd(f, x.data(), nn); rd(f, y.data(), nn);
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)5 * nc); rd(f, prim.data(), prim.size());
+    std::vector<double> src;
+    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
+    std::fclose(f);
+
+    PolarRef PR = build_polar(ni, nj, x, y, ctype, otype, prim, src, has_src != 0);
+    Setka* S = PR.S;
+    S->TVD_prepare();
+    // each ray ends in a detached stand-in cell on both sides, so that the walk from an
+    // end row also comes back to that row (Cell.cpp:528-561)
+    for (int j = 0; j < nj; j++) {
+        Cell* first = S->All_Cells[j];
+        Cell* last = S->All_Cells[(ni - 1) * nj + j];
+        Cell* lo = new Cell(first->contour[0], first->contour[1], first->contour[2], first->contour[3]);
+        Cell* hi = new Cell(last->contour[0], last->contour[1], last->contour[2], last->contour[3]);
+        lo->r_istoch = -1.0; lo->Back = nullptr; lo->Next = first;
+        hi->r_istoch = 1e300; hi->Back = last; hi->Next = nullptr;
+        for (int i = 0; i < ni; i++) {
+            Cell* K = S->All_Cells[i * nj + j];
+            K->Back = i > 0 ? S->All_Cells[(i - 1) * nj + j] : lo;
+            K->Next = i + 1 < ni ? S->All_Cells[(i + 1) * nj + j] : hi;
+        }
+    }
+    for (Cell* K : S->All_Cells) S->All_Cells_Inner.push_back(K);
+    S->Go_stationary_5_komponent_inner_MK2(steps);
+    const int final_buf = steps % 2;
+    std::vector<double> res((size_t)5 * nc);
+    for (int c = 0; c < nc; c++) {
+        const Parametr& p = S->All_Cells[c]->par[final_buf];
+        res[c] = p.ro; res[nc + c] = p.p; res[2 * nc + c] = p.u; res[3 * nc + c] = p.v; res[4 * nc + c] = p.Q;
+    }
+    f = xopen(out, "wb");
+    wr(f, res.data(), res.size());
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
@@ -555,6 +611,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "godunov")) return mode_godunov(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "track")) return mode_track(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "mf")) return mode_mf(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "inner")) return mode_inner(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner)
 """
 from __future__ import annotations
 
@@ -350,6 +350,31 @@ def gen_multifluid(rng, tmp):
     assert GEL_PHYS_GAS4H == 2
 
 
+def gen_inner(tmp):
+    """Setka::Go_stationary_5_komponent_inner_MK2 on the inner region, 24 x 12 cells
+    (C_1 rows, C_2 rows, two C_centr cells), 40 steps without sources and 25 with the
+    MK1 closure of seeded hydrogen moments."""
+    from gelios_2d_amd.configs import inner_region
+    res = {}
+    for tag, steps, seed, srcs in (("plain", 40, 41, False), ("sources", 25, 42, True)):
+        w = inner_region(24, 12, steps, seed, sources=srcs)
+        g = w.grid
+        h = np.array([g.ni, g.nj, steps, 1 if srcs else 0], np.int32)
+        parts = [h.tobytes(), g.x.tobytes(), g.y.tobytes(), g.cell_type.astype(np.int32).tobytes(),
+                 g.outer_type.astype(np.int32).tobytes(), np.ascontiguousarray(w.prim0).tobytes()]
+        if srcs:
+            parts.append(w.src.tobytes())
+        out = harness("inner", b"".join(parts), tmp)
+        res[f"{tag}_prim0"] = w.prim0
+        res[f"{tag}_prim"] = np.frombuffer(out, np.float64).reshape(5, g.ncells).copy()
+        res[f"{tag}_steps"] = steps
+        res[f"{tag}_seed"] = seed
+        if srcs:
+            res[f"{tag}_src"] = w.src
+    res.update(ni=24, nj=12)
+    np.savez_compressed(os.path.join(OUT, "sweep_inner.npz"), **res)
+
+
 def main():
     only = sys.argv[1:]
     if only:
@@ -358,6 +383,8 @@ def main():
                 gen_godunov(np.random.default_rng(20261205), tmp)
             if "track" in only:
                 gen_tracked(tmp)
+            if "inner" in only:
+                gen_inner(tmp)
             if "mf" in only:
                 gen_multifluid(np.random.default_rng(20261206), tmp)
         return
@@ -371,6 +398,7 @@ def main():
         gen_godunov(np.random.default_rng(20261205), tmp)
         gen_tracked(tmp)
         gen_multifluid(np.random.default_rng(20261206), tmp)
+        gen_inner(tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
