@@ -600,9 +600,56 @@ static int mode_inner(const char* in, const char* out)
     return 0;
 }
 
+// ---- checkpoint writer ---------------------------------------------------------------
+// in : int32 ni nj has_src; float64 x[nn] y[nn]; int32 cell_type[nc] outer_type[nj];
+//      float64 prim[21][nc]; if has_src: float64 src[19][nc]
+// Builds the polar grid, records every cell in the lists of its four contour points in
+// contour order (as the grid constructor does, Setka.cpp:199-265) and writes the
+// reference's checkpoint with Setka::Save_Setka_ALL_ALPHA (Setka.cpp:2355-2574) to out.
+static int mode_save(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[3]; rd(f, h, 3);
+    const int ni = h[0], nj = h[1], has_src = h[2];
+    const int nn = (ni + 1) * (nj + 1), nc = ni * nj;
+    std::vector<double> x(nn), y(nn);
+    rd(f, x.data(), nn); rd(f, y.data(), nn);
+    std::vector<int32_t> ctype(nc), otype(nj);
+    rd(f, ctype.data(), nc); rd(f, otype.data(), nj);
+    std::vector<double> prim((size_t)21 * nc); rd(f, prim.data(), prim.size());
+    std::vector<double> src;
+    if (has_src) { src.resize((size_t)19 * nc); rd(f, src.data(), src.size()); }
+    std::fclose(f);
+    PolarRef PR = build_polar(ni, nj, x, y, ctype, otype, prim, src, has_src != 0);
+    Setka* S = PR.S;
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        Parametr& p = K->par[0];
+        double* hv[16] = { &p.ro_H1, &p.p_H1, &p.u_H1, &p.v_H1, &p.ro_H2, &p.p_H2, &p.u_H2, &p.v_H2,
+                           &p.ro_H3, &p.p_H3, &p.u_H3, &p.v_H3, &p.ro_H4, &p.p_H4, &p.u_H4, &p.v_H4 };
+        for (int k = 0; k < 16; k++) *hv[k] = prim[(size_t)(5 + k) * nc + c];
+        for (int k = 0; k < 4; k++) K->contour[k]->my_cell.push_back(K);
+    }
+    S->N1 = S->N2 = S->N3 = S->N4 = S->M1 = S->M2 = S->M3 = S->M4 = 0;   // no rails
+    S->Save_Setka_ALL_ALPHA(std::string(out));
+    return 0;
+}
+
+// ---- checkpoint reader ---------------------------------------------------------------
+// Setka::Download_Setka_ALL_ALPHA_2_0 (Setka.cpp:2899-3258) of the checkpoint in, then
+// Save_Setka_ALL_ALPHA to out: a checkpoint the reference reads back unchanged comes
+// out byte for byte.
+static int mode_resave(const char* in, const char* out)
+{
+    Setka* S = new Setka();
+    S->Download_Setka_ALL_ALPHA_2_0(std::string(in));
+    S->Save_Setka_ALL_ALPHA(std::string(out));
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner in out\n"); return 2; }
+    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
@@ -612,6 +659,8 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "track")) return mode_track(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "mf")) return mode_mf(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "inner")) return mode_inner(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "save")) return mode_save(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "resave")) return mode_resave(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

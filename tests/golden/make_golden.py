@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint)
 """
 from __future__ import annotations
 
@@ -375,6 +375,26 @@ def gen_inner(tmp):
     np.savez_compressed(os.path.join(OUT, "sweep_inner.npz"), **res)
 
 
+def gen_checkpoint(rng, tmp):
+    """Setka::Save_Setka_ALL_ALPHA of a 7 x 5 polar grid (C_1 and C_2 rows) with all 21
+    fields and the 19 MK1 inputs, and Download_Setka_ALL_ALPHA_2_0 + Save of that file
+    (the reference reads its own checkpoint back unchanged).  The file is kept as bytes."""
+    prm = reference_params()
+    g = harness_grid(7, 5, 2)
+    nc = g.ncells
+    prim = mf_state(g, rng, prm)
+    src = np.concatenate([rng.uniform(0.05, 0.5, (4, nc)), rng.normal(-2.0, 0.5, (4, nc)),
+                          rng.normal(0.0, 0.5, (4, nc)), rng.uniform(0.5, 2.0, (4, nc)),
+                          rng.uniform(0.5, 2.0, (3, nc))])
+    h = np.array([g.ni, g.nj, 1], np.int32)
+    blob = b"".join([h.tobytes(), g.x.tobytes(), g.y.tobytes(), g.cell_type.astype(np.int32).tobytes(),
+                     g.outer_type.astype(np.int32).tobytes(), prim.tobytes(), src.tobytes()])
+    text = harness("save", blob, tmp)
+    np.savez_compressed(os.path.join(OUT, "checkpoint_polar.npz"), ni=g.ni, nj=g.nj, x=g.x, y=g.y,
+                        cell_type=g.cell_type, outer_type=g.outer_type, prim=prim, src=src,
+                        ref_text=np.frombuffer(text, np.uint8))
+
+
 def main():
     only = sys.argv[1:]
     if only:
@@ -383,6 +403,8 @@ def main():
                 gen_godunov(np.random.default_rng(20261205), tmp)
             if "track" in only:
                 gen_tracked(tmp)
+            if "checkpoint" in only:
+                gen_checkpoint(np.random.default_rng(20261207), tmp)
             if "inner" in only:
                 gen_inner(tmp)
             if "mf" in only:
@@ -399,6 +421,7 @@ def main():
         gen_tracked(tmp)
         gen_multifluid(np.random.default_rng(20261206), tmp)
         gen_inner(tmp)
+        gen_checkpoint(np.random.default_rng(20261207), tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))

@@ -81,3 +81,25 @@ def test_matches_oracle_ragged(oracle_lib, flux):
     st = s.Go_5_komponent__MK2(w.steps)
     assert st.n_numeric == 0
     assert rel_err(s.download(), ref) <= TOL
+
+
+def test_reference_checkpoint_drives_the_sweep(oracle_lib, tmp_path):
+    """A checkpoint written by the reference (Save_Setka_ALL_ALPHA) read into the device
+    sweep: the plasma + hydrogen state and the grid it holds, 10 steps, against the
+    oracle from the same parsed data."""
+    from gelios_2d_amd import checkpoint as ck
+    from gelios_2d_amd.params import GEL_PHYS_GAS4H, reference_params
+    from gelios_2d_amd.setka import Setka
+    d = golden("checkpoint_polar.npz")
+    p = tmp_path / "ref.txt"
+    p.write_bytes(d["ref_text"].tobytes())
+    g, prim, _ = ck.polar_from_alpha(ck.read_alpha(p))
+    prm = reference_params()
+    prm.physics = GEL_PHYS_GAS4H
+    prm.flux = GEL_FLUX_HLLC
+    print("checkpoint-driven multifluid", flush=True)
+    ref = oracle_lib.OracleSweep.from_polar(g, prm).run(prim, 10)
+    s = Setka(g, prm)
+    s.upload(prim)
+    s.Go_5_komponent__MK2(10)
+    assert rel_err(s.download(), ref) <= TOL
