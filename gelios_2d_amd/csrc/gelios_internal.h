@@ -26,7 +26,7 @@ struct SweepArgs {
     void* __restrict__ next;
     const int8_t* __restrict__ ctype;
     const int8_t* __restrict__ otype;
-    const double* __restrict__ src;
+    const void* __restrict__ src;     // [19][ncells] hydrogen moments and k factors, storage type
     const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
     const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
     const double* __restrict__ geo;   // static grids: the geometry table (geometry_kernel)
@@ -93,7 +93,7 @@ struct gel_ctx {
     int8_t* d_ctype = nullptr;
     int8_t* d_otype = nullptr;
     bool has_types = false;
-    double* d_src = nullptr;       // [GEL_MK1_NFIELDS][nc] hydrogen moments and k factors
+    void* d_src = nullptr;         // [GEL_MK1_NFIELDS][nc] hydrogen moments and k factors, ts bytes each
     double* d_qtab = nullptr;      // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID)
     int8_t* d_qflag = nullptr;     // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
     double* d_geo = nullptr;       // geometry table of the static grid (built on first use)

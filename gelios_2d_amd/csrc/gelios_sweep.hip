@@ -285,12 +285,19 @@ int gel_set_sources(gel_ctx* c, const double* f)
 {
     if (!c || !f) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
-    const size_t nb = (size_t)GEL_MK1_NFIELDS * c->nc * sizeof(double);
+    // stored in the state's type: FP32 for the FP32-bulk sweep, whose closure runs in FP32
+    const size_t n = (size_t)GEL_MK1_NFIELDS * c->nc;
     if (!c->d_src) {
-        hipError_t e = hipMalloc(&c->d_src, nb);
+        hipError_t e = hipMalloc(&c->d_src, n * c->ts);
         if (e != hipSuccess) return fail(GEL_E_NOMEM, "sources");
     }
-    HIP_OK(hipMemcpy(c->d_src, f, nb, hipMemcpyHostToDevice));
+    if (c->ts == 8) {
+        HIP_OK(hipMemcpy(c->d_src, f, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(n);
+        for (size_t k = 0; k < n; k++) h[k] = (float)f[k];
+        HIP_OK(hipMemcpy(c->d_src, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
     return GEL_OK;
 }
 

@@ -224,7 +224,7 @@ __global__ void pack_input_kernel(int n, int B, const float* __restrict__ x, bf1
 // The hydrogen moments are the GEL_MK1_NFIELDS fields of gel_set_sources.
 template <class TS>
 __global__ void features_kernel(int ni, int nj, const double* __restrict__ X, const double* __restrict__ Y,
-                                const TS* __restrict__ st, const double* __restrict__ src,
+                                const TS* __restrict__ st, const TS* __restrict__ src,
                                 const float* __restrict__ mu, const float* __restrict__ sd, float* __restrict__ out)
 {
     const size_t nc = (size_t)ni * nj;
@@ -255,19 +255,21 @@ __global__ void features_kernel(int ni, int nj, const double* __restrict__ X, co
 }
 
 // Analytic charge-exchange source per cell (the judge and fallback): the device
-// closure of Cell::Get_Sourse_MK1 (Cell.cpp:590-664) on the cell's own moments.
+// closure of Cell::Get_Sourse_MK1 (Cell.cpp:590-664) on the cell's own moments, in
+// the state's type, as the sweep evaluates it.
 template <class TS>
 __device__ inline void analytic_q(const gel_params& prm, size_t nc, size_t c, const TS* __restrict__ st,
-                                  const double* __restrict__ src, double q[5])
+                                  const TS* __restrict__ src, double q[5])
 {
-    const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
-    q[0] = 0.0; q[4] = 0.0;
-    source_mk1(prm, [&](int k) { return src[(size_t)k * nc + c]; }, u, v, ro, p, q[1], q[2], q[3]);
+    TS r1, r2, r3;
+    auto fk = [&](int k) { return src[(size_t)k * nc + c]; };
+    source_mk1<decltype(fk), TS>(prm, fk, st[2 * nc + c], st[3 * nc + c], st[c], st[nc + c], r1, r2, r3);
+    q[0] = 0.0; q[1] = r1; q[2] = r2; q[3] = r3; q[4] = 0.0;
 }
 
 template <class TS>
 __global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __restrict__ st,
-                                      const double* __restrict__ src, double* __restrict__ qtab)
+                                      const TS* __restrict__ src, double* __restrict__ qtab)
 {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
@@ -310,7 +312,7 @@ __device__ inline void refresh_features(const SweepArgs& a, size_t c, const NetD
     double cx, cy;
     quad_center(px, py, cx, cy);
     const TS* st = (const TS*)a.next;
-    const double* src = a.src;
+    const TS* src = (const TS*)a.src;
     const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
     double f[kIn];
     f[0] = log(ro); f[1] = u; f[2] = v; f[3] = log(p / ro);
@@ -454,10 +456,10 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     bool fall = false;
     if (c < nc) {
         const TS* st = (const TS*)a.next;
-        const double* src = a.src;
+        const TS* src = (const TS*)a.src;
         double* qtab = const_cast<double*>(a.qtab);
         const double ro = st[c];
-        const double nH = src[c] + src[nc + c] + src[2 * nc + c] + src[3 * nc + c];
+        const double nH = (double)src[c] + (double)src[nc + c] + (double)src[2 * nc + c] + (double)src[3 * nc + c];
         const double scale = a.prm.src_coef * ro * nH;
         double qm[5];
 #pragma unroll
@@ -913,10 +915,10 @@ int gel_surrogate_features(gel_ctx* c, float* x)
     const dim3 g((unsigned)((c->nc + 255) / 256));
     if (c->ts == 8)
         hipLaunchKernelGGL(features_kernel<double>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
-                           (const double*)c->d_state[c->cur], c->d_src, nullptr, nullptr, d);
+                           (const double*)c->d_state[c->cur], (const double*)c->d_src, nullptr, nullptr, d);
     else
         hipLaunchKernelGGL(features_kernel<float>, g, dim3(256), 0, c->stream, c->ni, c->nj, c->d_x, c->d_y,
-                           (const float*)c->d_state[c->cur], c->d_src, nullptr, nullptr, d);
+                           (const float*)c->d_state[c->cur], (const float*)c->d_src, nullptr, nullptr, d);
     hipError_t e = hipMemcpyAsync(x, d, c->nc * kIn * sizeof(float), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
@@ -934,10 +936,10 @@ int gel_source_analytic(gel_ctx* c, double* q)
     const dim3 g((unsigned)((c->nc + 255) / 256));
     if (c->ts == 8)
         hipLaunchKernelGGL(analytic_table_kernel<double>, g, dim3(256), 0, c->stream, c->nc, c->prm,
-                           (const double*)c->d_state[c->cur], c->d_src, d);
+                           (const double*)c->d_state[c->cur], (const double*)c->d_src, d);
     else
         hipLaunchKernelGGL(analytic_table_kernel<float>, g, dim3(256), 0, c->stream, c->nc, c->prm,
-                           (const float*)c->d_state[c->cur], c->d_src, d);
+                           (const float*)c->d_state[c->cur], (const float*)c->d_src, d);
     hipError_t e = hipMemcpyAsync(q, d, 5 * c->nc * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);

@@ -265,44 +265,49 @@ GEL_HD T hllc2d_full(const Gas<T>& L, const Gas<T>& R, T W, T n1, T n2, T rad, T
 }
 
 // Cell::Get_Sourse_MK1, own-cell moments (Cell.cpp:590-664).  f(k) returns field k.
-// LOG32 (the FP32-bulk sweep): the logarithm of the cross section in FP32 (its
-// relative error, a few 1e-8, enters sigma as an absolute error b * 1e-7).
-template <class F, bool LOG32 = false>
-GEL_HD void source_mk1(const gel_params& prm, F f, double u, double v, double ro, double p,
-                       double& q1, double& q2, double& q3)
+// R is the arithmetic type: double for the FP64 sweep (the reference's arithmetic);
+// float for the FP32-bulk sweep, whose moments are stored in FP32, with the
+// hardware's reciprocal, square root and logarithm (a source term's relative error
+// of a few 1e-7 enters the step as dt * q, below the rounding of the FP32 state).
+template <class F, class R = double>
+GEL_HD void source_mk1(const gel_params& prm, F f, R u, R v, R ro, R p, R& q1, R& q2, R& q3)
 {
-    double nu[4], uH[4], vH[4], roH[4], pH[4], UM[4], UU[4];
+    R nu[4], uH[4], vH[4], roH[4], pH[4], UM[4], UU[4];
+    const R half = R(0.5), two = R(2.0);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        uH[k] = f(4 + k);
-        vH[k] = f(8 + k);
-        roH[k] = f(k);
-        pH[k] = 0.5 * f(12 + k) * f(k);
-        if (roH[k] <= 0.0) { roH[k] = 0.0000001; pH[k] = 0.0; }
+        uH[k] = R(f(4 + k));
+        vH[k] = R(f(8 + k));
+        roH[k] = R(f(k));
+        pH[k] = half * R(f(12 + k)) * R(f(k));
+        if (roH[k] <= R(0)) { roH[k] = R(0.0000001); pH[k] = R(0); }
     }
+    const R c_m = R(64.0 / (9.0 * kPi)), c_u = R(4.0 / kPi);
+    const R sig_a = R(prm.sig_a), sig_b = R(prm.sig_b);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        UM[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (64.0 / (9.0 * kPi)) * (p / ro + 2.0 * pH[k] / roH[k]));
-        UU[k] = sqrt(sq(u - uH[k]) + sq(v - vH[k]) + (4.0 / kPi) * (p / ro + 2.0 * pH[k] / roH[k]));
-        double lg;
-        if constexpr (LOG32) lg = (double)__logf((float)UM[k]);
+        UM[k] = sr(sq(u - uH[k]) + sq(v - vH[k]) + c_m * (dv(p, ro) + dv(two * pH[k], roH[k])));
+        UU[k] = sr(sq(u - uH[k]) + sq(v - vH[k]) + c_u * (dv(p, ro) + dv(two * pH[k], roH[k])));
+        R lg;
+        if constexpr (sizeof(R) == 4) lg = __logf(UM[k]);
         else lg = log(UM[k]);
-        const double sig = sq(prm.sig_a - prm.sig_b * lg);
+        const R sig = sq(sig_a - sig_b * lg);
         nu[k] = ro * roH[k] * UM[k] * sig;
     }
-    double Mu = nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u);
-    double Mv = nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v);
-    double t[4];
+    R Mu = nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u);
+    R Mv = nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v);
+    R t[4];
 #pragma unroll
     for (int k = 0; k < 4; k++)
-        t[k] = nu[k] * ((sq(uH[k]) + sq(vH[k]) - sq(u) - sq(v)) / 2.0 + (UU[k] / UM[k]) * (2.0 * pH[k] / roH[k] - p / ro));
-    double MT = t[0] + t[1] + t[2] + t[3];
-    Mu = prm.src_coef * Mu;
-    Mv = prm.src_coef * Mv;
-    MT = prm.src_coef * MT;
-    q1 = Mu * f(16);
-    q2 = Mv * f(17);
-    q3 = MT * f(18);
+        t[k] = nu[k] * ((sq(uH[k]) + sq(vH[k]) - sq(u) - sq(v)) / two + dv(UU[k], UM[k]) * (dv(two * pH[k], roH[k]) - dv(p, ro)));
+    R MT = t[0] + t[1] + t[2] + t[3];
+    const R coef = R(prm.src_coef);
+    Mu = coef * Mu;
+    Mv = coef * Mv;
+    MT = coef * MT;
+    q1 = Mu * R(f(16));
+    q2 = Mv * R(f(17));
+    q3 = MT * R(f(18));
 }
 
 }  // namespace gel

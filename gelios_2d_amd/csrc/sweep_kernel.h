@@ -34,6 +34,14 @@ namespace gel {
 constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kOut = kWave - 2 * kHalo;   // output columns per wavefront
+
+// Every buffer the sweep touches is in global memory, but the batched kernel reads
+// the buffers' addresses from a table in memory, where the compiler cannot tell their
+// address space: it would emit flat instructions, which also count on the LDS
+// counter, so that every wait for an LDS read would wait for the HBM traffic in
+// flight as well.  The hot accesses go through gp(), the address typed as global.
+template <class T> using GPtr = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ GPtr<T> gp(T* p) { return (GPtr<T>)p; }
 constexpr int kNW = 4;                    // wavefronts per work-group, marching together
 
 // State vectors.  GAS: (ro, p, u, v, Q), the fields of Parametr (Cell.h:21-25).
@@ -102,10 +110,10 @@ __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, in
     if (c.ok) {
         const size_t nc = (size_t)a.ni * a.nj;
         const size_t k = (size_t)r * a.nj + j;
-        const TS* __restrict__ now = (const TS*)a.now;
+        const GPtr<const TS> now = gp((const TS*)a.now);
 #pragma unroll
         for (int v = 0; v < Ph::NV; v++) c.s[v] = now[(size_t)(MF ? a.voff[v] : v) * nc + k];
-        if (TYPES) c.type = a.ctype[k];
+        if (TYPES) c.type = gp(a.ctype)[k];
         if constexpr (CENTROID) {
             const double px[4] = { lo.xa, hi.xa, hi.xb, lo.xb };
             const double py[4] = { lo.ya, hi.ya, hi.yb, lo.yb };
@@ -629,13 +637,13 @@ __device__ __forceinline__ int wave_sum(int v)
 //                           cell (i, c-1) (for c = 0 as edge 0 of cell (i, 0), the
 //                           axis): n1, n2, S, fx, fy, dis.
 struct GeoTable {
-    const double* base;
+    GPtr<const double> base;
     size_t nc, nr, np;
-    __host__ __device__ GeoTable(const double* b, int ni, int nj)
-        : base(b), nc((size_t)ni * nj), nr((size_t)(ni + 1) * nj), np((size_t)ni * (nj + 1)) {}
-    __host__ __device__ const double* cell(int k) const { return base + (size_t)k * nc; }
-    __host__ __device__ const double* rad(int k) const { return base + 4 * nc + (size_t)k * nr; }
-    __host__ __device__ const double* pol(int k) const { return base + 4 * nc + 6 * nr + (size_t)k * np; }
+    __device__ GeoTable(const double* b, int ni, int nj)
+        : base(gp(b)), nc((size_t)ni * nj), nr((size_t)(ni + 1) * nj), np((size_t)ni * (nj + 1)) {}
+    __device__ GPtr<const double> cell(int k) const { return base + (size_t)k * nc; }
+    __device__ GPtr<const double> rad(int k) const { return base + 4 * nc + (size_t)k * nr; }
+    __device__ GPtr<const double> pol(int k) const { return base + 4 * nc + 6 * nr + (size_t)k * np; }
     __host__ __device__ static size_t doubles(int ni, int nj)
     {
         return 4 * (size_t)ni * nj + 6 * (size_t)(ni + 1) * nj + 6 * (size_t)ni * (nj + 1);
@@ -655,8 +663,8 @@ __global__ void geometry_kernel(int ni, int nj, const double* __restrict__ X, co
     double* cell[4];
     double* rad[6];
     double* pol[6];
-    for (int k = 0; k < 4; k++) cell[k] = const_cast<double*>(G.cell(k));
-    for (int k = 0; k < 6; k++) { rad[k] = const_cast<double*>(G.rad(k)); pol[k] = const_cast<double*>(G.pol(k)); }
+    for (int k = 0; k < 4; k++) cell[k] = (double*)G.cell(k);
+    for (int k = 0; k < 6; k++) { rad[k] = (double*)G.rad(k); pol[k] = (double*)G.pol(k); }
     auto nd = [&](int q, int cc, double& x, double& y) {
         const size_t k = (size_t)q * (nj + 1) + cc;
         x = X[k]; y = Y[k];
@@ -985,9 +993,18 @@ struct MarchAcc {       // per-lane results folded into the case's dt buffer and
     int n_rho, n_p, bad;
 };
 
+// Static FP32-bulk sweeps with the analytic source copy the hydrogen moments of the
+// rows they update from HBM into LDS while they solve the faces (gfx950 LDS DMA,
+// global_load_lds_dword: no registers held, the latency hidden behind the faces).
+template <class TS, bool MOVING, int SRC, int FLUX>
+constexpr bool kMomDma = sizeof(TS) == 4 && !MOVING && (SRC == GEL_SRC_MK1 || SRC == GEL_SRC_HYBRID) &&
+                         (FLUX & (kTrackFlux | kMfFlux | kInnerFlux)) == 0;
+template <class TS, bool MOVING, int SRC, int FLUX, int NW>
+constexpr int kMomLds = kMomDma<TS, MOVING, SRC, FLUX> ? NW * GEL_MK1_NFIELDS * kWave : 1;
+
 template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
-__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, int bx,
-                                           int i0, int i1, MarchAcc& acc)
+__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, TS* mom,
+                                           int bx, int i0, int i1, MarchAcc& acc)
 {
     // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
     constexpr bool TRACK = (FLUX & kTrackFlux) != 0;
@@ -1012,14 +1029,14 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
     const bool used_polar = col && lane >= 1 && lane < kWave - kHalo;
     const double gam = a.prm.gamma;
     const double cfl = a.prm.cfl;
-    const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
+    const double dt = __longlong_as_double((long long)gp(a.dtbuf)[a.dt_rd]);
     const bool axis_lo = (j == 0), axis_hi = (j == nj - 1);
     const bool m_axis = axis_lo || axis_hi;
     const bool n_axis = (j + 1 == nj - 1);
     const size_t nc = (size_t)ni * nj;
 
     if (bx == 0 && i0 == 0 && threadIdx.x == 0)
-        a.dtbuf[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
+        gp(a.dtbuf)[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
 
     auto slot = [&](int r) { return (r - i0 + 2) % NR; };     // rows and node rows >= i0 - 2
     auto fslot = [&](int r) { return (r - i0 + 1) % (NW + 1); };  // radial face above row r >= i0 - 1
@@ -1096,7 +1113,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
         return n;
     };
     // face geometry of a static grid from the table (defaults where the lane has none)
-    auto table_face = [&](const double* const* F, bool valid, size_t k) -> Face {
+    auto table_face = [&](const GPtr<const double>* F, bool valid, size_t k) -> Face {
         Face f{ 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0 };
         if (valid) { f.n1 = F[0][k]; f.n2 = F[1][k]; f.S = F[2][k]; f.fx = F[3][k]; f.fy = F[4][k]; f.dis = F[5][k]; }
         return f;
@@ -1132,7 +1149,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     if (inner) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
                 } else {
-                    const double* F[6] = { G.rad(0), G.rad(1), G.rad(2), G.rad(3), G.rad(4), G.rad(5) };
+                    const GPtr<const double> F[6] = { G.rad(0), G.rad(1), G.rad(2), G.rad(3), G.rad(4), G.rad(5) };
                     f = table_face(F, col && i + 1 <= ni, (size_t)(i + 1) * nj + j);
                 }
                 double L[NV], R[NV], P[NV];
@@ -1144,7 +1161,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     else if constexpr (INNER) inner_boundary_states<TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                     else boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_INNER_SPHERE, N, U, f, L, R);
                 } else if (i + 1 >= ni) {
-                    const int ot = col ? (int)a.otype[j] : (int)GEL_F_EXTERN;
+                    const int ot = col ? (int)gp(a.otype)[j] : (int)GEL_F_EXTERN;
                     if constexpr (MF) mf_boundary_states<TYPES>(a, ot, M, D, f, L, R);
                     else if constexpr (INNER) inner_boundary_states<TYPES>(a, ot, M, D, f, L, R);
                     else boundary_states<Ph, TS, TF, TYPES>(a, ot, M, D, f, L, R);
@@ -1178,8 +1195,15 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     if (axl) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
                 } else {
-                    const double* F[6] = { G.pol(0), G.pol(1), G.pol(2), G.pol(3), G.pol(4), G.pol(5) };
+                    const GPtr<const double> F[6] = { G.pol(0), G.pol(1), G.pol(2), G.pol(3), G.pol(4), G.pol(5) };
                     f = table_face(F, j >= -1 && j < nj, (size_t)i * (nj + 1) + (j + 1));
+                    // the cell's geometry, before and after the step
+                    if (col) {
+                        const size_t k = (size_t)i * nj + j;
+                        vs1 = G.cell(2)[k];
+                        vol = G.cell(3)[k];
+                    }
+                    vs2 = vs1; vol2 = vol;
                 }
                 const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
                 double L[NV], R[NV], P[NV];
@@ -1196,6 +1220,26 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     if (axl) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, Mr, Mr2, f, L, R);
                     else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
                     else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
+                }
+                if constexpr (kMomDma<TS, MOVING, SRC, FLUX>) {
+                    // the moments of row i into mom[w] (field k, lane l -> column l), after
+                    // the last global load this phase waits for (memory counters retire in
+                    // order), so that they arrive during the polar solve; the barrier ending
+                    // the phase waits for them
+                    // every lane copies (lanes off the grid a cell of the row, unused), so
+                    // that no branch leaves the memory counters path-dependent.  The
+                    // compiler takes LDS DMA and loads into registers as retiring out of
+                    // order and would wait for the copies at the next use of a loaded
+                    // register: the loads issued so far are waited for first
+                    // (s_waitcnt vmcnt(0)), so that none is pending behind the copies.
+                    __builtin_amdgcn_sched_barrier(0);
+                    __builtin_amdgcn_s_waitcnt(0x0F70);
+                    const GPtr<const TS> src = gp((const TS*)a.src) + (size_t)i * nj + min(max(j, 0), nj - 1);
+#pragma unroll
+                    for (int k = 0; k < GEL_MK1_NFIELDS; k++)
+                        __builtin_amdgcn_global_load_lds((const void*)(src + (size_t)k * nc),
+                                                         (void*)(mom + (w * GEL_MK1_NFIELDS + k) * kWave), 4, 0, 0);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
                 TF uu;
                 if constexpr (TRACK && FX == GEL_FLUX_HLLC) {
@@ -1220,15 +1264,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 #pragma unroll
                 for (int k = 0; k < NV; k++) ps0[k] = -shfl_up(psp[k], 1);
                 t0 = shfl_up(t_right, 1);
-                // cell geometry before and after the step
-                if constexpr (!MOVING) {
-                    if (col) {
-                        const size_t k = (size_t)i * nj + j;
-                        vs1 = G.cell(2)[k];
-                        vol = G.cell(3)[k];
-                    }
-                    vs2 = vs1; vol2 = vol;
-                } else {
+                // cell geometry before and after the step (static grids: read above)
+                if constexpr (MOVING) {
                 const NodeP c00 = node(i, lane), c10 = node(i + 1, lane), c11 = node(i + 1, lane + 1),
                             c01 = node(i, lane + 1);
                 const double px[4] = { c00.x, c10.x, c11.x, c01.x };
@@ -1269,7 +1306,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 time = mn(time, t_pol);
                 time = mn(time, S.ftm[fb][lane]);
                 const size_t kc = (size_t)i * nj + j;
-                TS* __restrict__ nxt = (TS*)a.next;
+                const GPtr<TS> nxt = gp((TS*)a.next);
                 if constexpr (INNER) {
                     // C_centr cells are skipped: no time step, no update (Setka.cpp:7025-7028)
                     const int ct = TYPES ? M.type : (int)GEL_C_2;
@@ -1281,9 +1318,9 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                         acc.tmin = fmin(acc.tmin, time);
                         double q1s = 0.0, q2s = 0.0, q3s = 0.0;
                         if (SRC == GEL_SRC_MK1) {
-                            const double* __restrict__ sp = a.src + kc;
+                            const double* __restrict__ sp = (const double*)a.src + kc;
                             auto fk = [&](int k) { return sp[(size_t)k * nc]; };
-                            source_mk1<decltype(fk), false>(a.prm, fk, M.s[2], M.s[3], M.s[0], M.s[1], q1s, q2s, q3s);
+                            source_mk1<decltype(fk), double>(a.prm, fk, M.s[2], M.s[3], M.s[0], M.s[1], q1s, q2s, q3s);
                         }
                         double sd[5];
 #pragma unroll
@@ -1329,17 +1366,25 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 #pragma unroll
                     for (int k = 0; k < NV; k++) sd[k] = double(M.s[k]);
                     double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
-                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && a.qflag[kc]);
+                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && gp(a.qflag)[kc]);
                     if (analytic) {
-                        const double* __restrict__ sp = a.src + kc;
-                        auto fk = [&](int k) { return sp[(size_t)k * nc]; };
-                        source_mk1<decltype(fk), sizeof(TF) == 4>(a.prm, fk, sd[2], sd[3], sd[0], sd[1], q1s, q2s, q3s);
+                        // moments stored in the state's type, the closure in the face type
+                        const GPtr<const TS> sp = gp((const TS*)a.src) + kc;
+                        const TS* __restrict__ sl = mom + w * GEL_MK1_NFIELDS * kWave + lane;
+                        auto fk = [&](int k) {
+                            if constexpr (kMomDma<TS, MOVING, SRC, FLUX>) return sl[k * kWave];
+                            else return sp[(size_t)k * nc];
+                        };
+                        TF r1, r2, r3;
+                        source_mk1<decltype(fk), TF>(a.prm, fk, TF(M.s[2]), TF(M.s[3]), TF(M.s[0]), TF(M.s[1]), r1, r2, r3);
+                        q1s = r1; q2s = r2; q3s = r3;
                     } else if (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) {
-                        qms = a.qtab[kc];
-                        q1s = a.qtab[nc + kc];
-                        q2s = a.qtab[2 * nc + kc];
-                        q3s = a.qtab[3 * nc + kc];
-                        qQs = a.qtab[4 * nc + kc];
+                        const GPtr<const double> qt = gp(a.qtab);
+                        qms = qt[kc];
+                        q1s = qt[nc + kc];
+                        q2s = qt[2 * nc + kc];
+                        q3s = qt[3 * nc + kc];
+                        qQs = qt[4 * nc + kc];
                     }
                     double o[NV];
                     cell_update<Ph, sizeof(TF) == 4>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
@@ -1361,10 +1406,12 @@ __device__ __forceinline__ void march_flush(const SweepArgs& a, MarchAcc& acc)
     const int s_rho = wave_sum(acc.n_rho), s_p = wave_sum(acc.n_p), s_bad = wave_sum(acc.bad);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (tmin < __builtin_huge_val())
-            atomicMin(&a.dtbuf[a.dt_wr], (unsigned long long)__double_as_longlong(tmin));
-        if (s_rho) atomicAdd(&a.counters[0], (unsigned long long)s_rho);
-        if (s_p) atomicAdd(&a.counters[1], (unsigned long long)s_p);
-        if (s_bad) atomicAdd(&a.counters[2], (unsigned long long)s_bad);
+            __hip_atomic_fetch_min(gp(a.dtbuf) + a.dt_wr, (unsigned long long)__double_as_longlong(tmin),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const GPtr<unsigned long long> cnt = gp(a.counters);
+        if (s_rho) __hip_atomic_fetch_add(cnt, (unsigned long long)s_rho, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_p) __hip_atomic_fetch_add(cnt + 1, (unsigned long long)s_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s_bad) __hip_atomic_fetch_add(cnt + 2, (unsigned long long)s_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     acc = MarchAcc{ __builtin_huge_val(), 0, 0, 0 };
 }
@@ -1386,13 +1433,14 @@ __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu
 sweep_kernel(SweepArgs a, int strips, int ntiles)
 {
     __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
+    __shared__ TS mom[kMomLds<TS, MOVING, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int t = xcd_item(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
         const int bx = t % strips, by = t / strips;
         const int i0 = by * a.rows, i1 = min(i0 + a.rows, a.ni);
         // no barrier between tiles: the update phase of the previous tile reads no ring,
         // and its faces phase ended at a barrier every wavefront has passed
-        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, bx, i0, i1, acc);
+        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, mom, bx, i0, i1, acc);
     }
     march_flush(a, acc);
 }
@@ -1409,12 +1457,13 @@ __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
     __shared__ MarchLds<Ph, TS, false, false, NW> S;
+    __shared__ TS mom[kMomLds<TS, false, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
         const BatchItem b = items[it];
         const SweepArgs& a = args[b.cs];
         const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
-        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, b.bx, i0, i1, acc);
+        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, mom, b.bx, i0, i1, acc);
         march_flush(a, acc);
     }
 }

@@ -180,7 +180,9 @@ int gel_upload_state(gel_ctx* ctx, const double* prim);
 int gel_download_state(gel_ctx* ctx, double* prim);
 
 /* Per-cell source inputs, GEL_MK1_NFIELDS variable-major arrays of ncells doubles
- * (Parametr::H_n/H_u/H_v/H_T/k_u/k_v/k_T, Cell.h:55-66). */
+ * (Parametr::H_n/H_u/H_v/H_T/k_u/k_v/k_T, Cell.h:55-66).  Kept on the device in the
+ * state's precision: FP32-bulk contexts round them to FP32 and evaluate the analytic
+ * closure in FP32. */
 int gel_set_sources(gel_ctx* ctx, const double* fields);
 
 /* Per-cell source table for GEL_SRC_TABLE: 5 variable-major arrays of ncells doubles

@@ -123,12 +123,14 @@ def test_fused_refresh_matches_unfused_forward(precision):
     assert rel.max() < 0.1
 
 
-def test_hybrid_mode_limits():
+@pytest.mark.parametrize("precision", [0, 1])
+def test_hybrid_mode_limits(precision):
     """GEL_SRC_HYBRID after a refresh with tol = 0 (every cell flagged) is the MK1 sweep
     bit for bit; with no judge (no cell flagged) it is the table sweep."""
     from gelios_2d_amd import surrogate as sg
     from gelios_2d_amd.params import GEL_SRC_HYBRID
     w, f = sg.workload_with_hydrogen(80, 64, 0, 2.2, 0.1, 13, moving=False)
+    w.params.precision = precision
     m = _model()
     mk1 = _setka(w, f, GEL_SRC_MK1)
     hyb = _setka(w, f, GEL_SRC_HYBRID)
@@ -145,3 +147,25 @@ def test_hybrid_mode_limits():
     tab.Go_5_komponent__MK2(12)
     hyb2.Go_5_komponent__MK2(12)
     assert np.array_equal(tab.download(), hyb2.download())
+
+
+def test_fp32_bulk_analytic_source_against_fp64():
+    """The FP32-bulk static sweep with the analytic closure (moments in FP32, copied into
+    LDS during the face solves, the closure in FP32) against the FP64 sweep with the
+    reference's arithmetic, 120 steps on 90 x 70 cells (two strips, the second ragged):
+    density and pressure L1 errors within 2e-5 (the FP32-bulk ensemble measures a few
+    1e-6), and the source visibly acting (the run without it is 10x further off)."""
+    from gelios_2d_amd import surrogate as sg
+    w, f = sg.workload_with_hydrogen(90, 70, 0, 2.4, 0.15, 21, moving=False)
+    runs = {}
+    for name, prec, mode in (("fp64", 0, GEL_SRC_MK1), ("fp32", 1, GEL_SRC_MK1), ("none", 0, GEL_SRC_NONE)):
+        w.params.precision = prec
+        s = _setka(w, f, mode)
+        s.Go_5_komponent__MK2(120)
+        runs[name] = s.download()
+    for k in (0, 1):
+        err = l1_rel(runs["fp32"], runs["fp64"], k)
+        gap = l1_rel(runs["none"], runs["fp64"], k)
+        print(f"variable {k}: fp32 vs fp64 L1 {err:.3e}, no source vs fp64 {gap:.3e}")
+        assert err <= 2e-5
+        assert gap > 10 * err
