@@ -27,8 +27,8 @@ struct SweepArgs {
     const int8_t* __restrict__ ctype;
     const int8_t* __restrict__ otype;
     const void* __restrict__ src;     // [19][ncells] hydrogen moments and k factors, storage type
-    const double* __restrict__ qtab;  // [5][ncells] source table (mass, mom x, mom y, energy, tracer)
-    const int8_t* __restrict__ qflag; // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
+    const void* __restrict__ qtab;    // [5][ncells] source table (mass, mom x, mom y, energy, tracer), storage type
+    const int* __restrict__ qflag;    // [ncells] 1: the cell takes the analytic source (GEL_SRC_HYBRID)
     const double* __restrict__ geo;   // static grids: the geometry table (geometry_kernel)
     int srow[3];                  // node rows of the fitted surfaces, -1 where none
     int fluid;                    // plasma + hydrogen sweep: 0 plasma, 1..4 hydrogen H1..H4
@@ -94,8 +94,8 @@ struct gel_ctx {
     int8_t* d_otype = nullptr;
     bool has_types = false;
     void* d_src = nullptr;         // [GEL_MK1_NFIELDS][nc] hydrogen moments and k factors, ts bytes each
-    double* d_qtab = nullptr;      // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID)
-    int8_t* d_qflag = nullptr;     // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
+    void* d_qtab = nullptr;        // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID), ts bytes each
+    int* d_qflag = nullptr;        // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
     double* d_geo = nullptr;       // geometry table of the static grid (built on first use)
     bool geo_valid = false;        // false after the nodes move
     bool tracking = false;         // fitted-surface tracking on (gel_set_tracking)

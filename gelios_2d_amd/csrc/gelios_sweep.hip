@@ -305,14 +305,22 @@ int gel_set_source_table(gel_ctx* c, const double* q)
 {
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
-    const size_t nb = (size_t)5 * c->nc * sizeof(double);
-    if (!c->d_qflag && hipMalloc(&c->d_qflag, c->nc) != hipSuccess) return fail(GEL_E_NOMEM, "source flags");
-    HIP_OK(hipMemset(c->d_qflag, 0, c->nc));
+    // stored in the state's type, like the moments
+    const size_t n = (size_t)5 * c->nc;
+    if (!c->d_qflag && hipMalloc((void**)&c->d_qflag, c->nc * sizeof(int)) != hipSuccess)
+        return fail(GEL_E_NOMEM, "source flags");
+    HIP_OK(hipMemset(c->d_qflag, 0, c->nc * sizeof(int)));
     if (!c->d_qtab) {
-        hipError_t e = hipMalloc(&c->d_qtab, nb);
+        hipError_t e = hipMalloc(&c->d_qtab, n * c->ts);
         if (e != hipSuccess) return fail(GEL_E_NOMEM, "source table");
     }
-    HIP_OK(hipMemcpy(c->d_qtab, q, nb, hipMemcpyHostToDevice));
+    if (c->ts == 8) {
+        HIP_OK(hipMemcpy(c->d_qtab, q, n * sizeof(double), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> h(n);
+        for (size_t k = 0; k < n; k++) h[k] = (float)q[k];
+        HIP_OK(hipMemcpy(c->d_qtab, h.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    }
     return GEL_OK;
 }
 
@@ -321,7 +329,14 @@ int gel_get_source_table(gel_ctx* c, double* q)
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     if (!c->d_qtab) return fail(GEL_E_STATE, "source table not set");
     HIP_OK(hipStreamSynchronize(c->stream));
-    HIP_OK(hipMemcpy(q, c->d_qtab, (size_t)5 * c->nc * sizeof(double), hipMemcpyDeviceToHost));
+    const size_t n = (size_t)5 * c->nc;
+    if (c->ts == 8) {
+        HIP_OK(hipMemcpy(q, c->d_qtab, n * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> h(n);
+        HIP_OK(hipMemcpy(h.data(), c->d_qtab, n * sizeof(float), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < n; k++) q[k] = h[k];
+    }
     return GEL_OK;
 }
 

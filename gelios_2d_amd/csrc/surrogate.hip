@@ -457,7 +457,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     if (c < nc) {
         const TS* st = (const TS*)a.next;
         const TS* src = (const TS*)a.src;
-        double* qtab = const_cast<double*>(a.qtab);
+        TS* qtab = (TS*)const_cast<void*>(a.qtab);
         const double ro = st[c];
         const double nH = (double)src[c] + (double)src[nc + c] + (double)src[2 * nc + c] + (double)src[3 * nc + c];
         const double scale = a.prm.src_coef * ro * nH;
@@ -480,8 +480,8 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
             }
         }
 #pragma unroll
-        for (int k = 0; k < 5; k++) qtab[k * nc + c] = qm[k];
-        if (a.qflag) const_cast<int8_t*>(a.qflag)[c] = fall ? 1 : 0;
+        for (int k = 0; k < 5; k++) qtab[k * nc + c] = TS(qm[k]);
+        if (a.qflag) const_cast<int*>(a.qflag)[c] = fall ? 1 : 0;
     }
     const unsigned long long mask = __ballot(fall);
     if ((tid & 63) == 0 && mask) atomicAdd(nfall, (unsigned long long)__popcll(mask));
@@ -954,9 +954,10 @@ int gel_surrogate_refresh(gel_ctx* c, gel_mlp* m, double tol, int judge, long lo
     int st = check_ctx(c);
     if (st != GEL_OK) return st;
     if (!m || !m->mu) return mfail(GEL_E_STATE, "network scaling not set");
-    if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * sizeof(double)) != hipSuccess)
+    if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * c->ts) != hipSuccess)
         return mfail(GEL_E_NOMEM, "source table");
-    if (!c->d_qflag && (hipMalloc(&c->d_qflag, c->nc) != hipSuccess || hipMemset(c->d_qflag, 0, c->nc) != hipSuccess))
+    if (!c->d_qflag && (hipMalloc((void**)&c->d_qflag, c->nc * sizeof(int)) != hipSuccess ||
+                        hipMemset(c->d_qflag, 0, c->nc * sizeof(int)) != hipSuccess))
         return mfail(GEL_E_NOMEM, "source flags");
     if (!m->d_rargs && (hipMalloc((void**)&m->d_rargs, sizeof(SweepArgs)) != hipSuccess ||
                         hipMalloc((void**)&m->d_rcount, sizeof(unsigned long long)) != hipSuccess))

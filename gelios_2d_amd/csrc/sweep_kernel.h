@@ -993,17 +993,22 @@ struct MarchAcc {       // per-lane results folded into the case's dt buffer and
     int n_rho, n_p, bad;
 };
 
-// Static FP32-bulk sweeps with the analytic source copy the hydrogen moments of the
-// rows they update from HBM into LDS while they solve the faces (gfx950 LDS DMA,
-// global_load_lds_dword: no registers held, the latency hidden behind the faces).
+// Static FP32-bulk sweeps with a source copy the per-cell source inputs of the rows
+// they update from HBM into LDS while they solve the faces (gfx950 LDS DMA,
+// global_load_lds_dword: no registers held, the latency hidden behind the polar
+// solve).  Fields of a row, 64 lanes each: the hydrogen moments (analytic closure),
+// the source table, the fallback flag (hybrid), as the source mode needs them.
 template <class TS, bool MOVING, int SRC, int FLUX>
-constexpr bool kMomDma = sizeof(TS) == 4 && !MOVING && (SRC == GEL_SRC_MK1 || SRC == GEL_SRC_HYBRID) &&
+constexpr bool kSrcDma = sizeof(TS) == 4 && !MOVING && SRC != GEL_SRC_NONE &&
                          (FLUX & (kTrackFlux | kMfFlux | kInnerFlux)) == 0;
+template <int SRC> constexpr int kSrcMom = (SRC == GEL_SRC_MK1 || SRC == GEL_SRC_HYBRID) ? GEL_MK1_NFIELDS : 0;
+template <int SRC> constexpr int kSrcTab = (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) ? 5 : 0;
+template <int SRC> constexpr int kSrcFields = kSrcMom<SRC> + kSrcTab<SRC> + (SRC == GEL_SRC_HYBRID ? 1 : 0);
 template <class TS, bool MOVING, int SRC, int FLUX, int NW>
-constexpr int kMomLds = kMomDma<TS, MOVING, SRC, FLUX> ? NW * GEL_MK1_NFIELDS * kWave : 1;
+constexpr int kSrcLds = kSrcDma<TS, MOVING, SRC, FLUX> ? NW * kSrcFields<SRC> * kWave : 1;
 
 template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
-__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, TS* mom,
+__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, TS* srcl,
                                            int bx, int i0, int i1, MarchAcc& acc)
 {
     // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
@@ -1221,24 +1226,35 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     else if (axis_hi) boundary_states<Ph, TS, TF, TYPES>(a, GEL_F_AXIS, M, Ml, f, L, R);
                     else interior_states<Ph, TS, TF, TYPES>(M, Mr, Ml, Mr2, m_axis, n_axis, f, gam, L, R, b);
                 }
-                if constexpr (kMomDma<TS, MOVING, SRC, FLUX>) {
-                    // the moments of row i into mom[w] (field k, lane l -> column l), after
-                    // the last global load this phase waits for (memory counters retire in
-                    // order), so that they arrive during the polar solve; the barrier ending
-                    // the phase waits for them
-                    // every lane copies (lanes off the grid a cell of the row, unused), so
-                    // that no branch leaves the memory counters path-dependent.  The
-                    // compiler takes LDS DMA and loads into registers as retiring out of
-                    // order and would wait for the copies at the next use of a loaded
-                    // register: the loads issued so far are waited for first
-                    // (s_waitcnt vmcnt(0)), so that none is pending behind the copies.
+                if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) {
+                    // the source inputs of row i into srcl[w] (field f, lane l -> column
+                    // l), after the last global load this phase waits for (memory
+                    // counters retire in order), so that they arrive during the polar
+                    // solve; the barrier ending the phase waits for them.  Every lane
+                    // copies (lanes off the grid a cell of the row, unused), so that no
+                    // branch leaves the memory counters path-dependent.  The compiler
+                    // takes LDS DMA and loads into registers as retiring out of order and
+                    // would wait for the copies at the next use of a loaded register: the
+                    // loads issued so far are waited for first (s_waitcnt vmcnt(0)).
                     __builtin_amdgcn_sched_barrier(0);
                     __builtin_amdgcn_s_waitcnt(0x0F70);
-                    const GPtr<const TS> src = gp((const TS*)a.src) + (size_t)i * nj + min(max(j, 0), nj - 1);
+                    const size_t kc0 = (size_t)i * nj + min(max(j, 0), nj - 1);
+                    TS* dst = srcl + w * kSrcFields<SRC> * kWave;
+                    auto copy = [&](const void* p) {
+                        __builtin_amdgcn_global_load_lds(p, (void*)dst, 4, 0, 0);
+                        dst += kWave;
+                    };
+                    if constexpr (kSrcMom<SRC> > 0) {
+                        const GPtr<const TS> m = gp((const TS*)a.src) + kc0;
 #pragma unroll
-                    for (int k = 0; k < GEL_MK1_NFIELDS; k++)
-                        __builtin_amdgcn_global_load_lds((const void*)(src + (size_t)k * nc),
-                                                         (void*)(mom + (w * GEL_MK1_NFIELDS + k) * kWave), 4, 0, 0);
+                        for (int k = 0; k < kSrcMom<SRC>; k++) copy((const void*)(m + (size_t)k * nc));
+                    }
+                    if constexpr (kSrcTab<SRC> > 0) {
+                        const GPtr<const TS> t = gp((const TS*)a.qtab) + kc0;
+#pragma unroll
+                        for (int k = 0; k < kSrcTab<SRC>; k++) copy((const void*)(t + (size_t)k * nc));
+                    }
+                    if constexpr (SRC == GEL_SRC_HYBRID) copy((const void*)(gp(a.qflag) + kc0));
                     __builtin_amdgcn_sched_barrier(0);
                 }
                 TF uu;
@@ -1366,25 +1382,35 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 #pragma unroll
                     for (int k = 0; k < NV; k++) sd[k] = double(M.s[k]);
                     double qms = 0.0, q1s = 0.0, q2s = 0.0, q3s = 0.0, qQs = 0.0;
-                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && gp(a.qflag)[kc]);
+                    constexpr bool DMA = kSrcDma<TS, MOVING, SRC, FLUX>;
+                    const TS* sl = srcl + w * kSrcFields<SRC> * kWave + lane;   // this cell's copied inputs
+                    bool flag = false;
+                    if constexpr (SRC == GEL_SRC_HYBRID) {
+                        if constexpr (DMA) flag = __builtin_bit_cast(int, sl[(kSrcFields<SRC> - 1) * kWave]) != 0;
+                        else flag = gp(a.qflag)[kc] != 0;
+                    }
+                    const bool analytic = (SRC == GEL_SRC_MK1) || (SRC == GEL_SRC_HYBRID && flag);
                     if (analytic) {
                         // moments stored in the state's type, the closure in the face type
                         const GPtr<const TS> sp = gp((const TS*)a.src) + kc;
-                        const TS* __restrict__ sl = mom + w * GEL_MK1_NFIELDS * kWave + lane;
                         auto fk = [&](int k) {
-                            if constexpr (kMomDma<TS, MOVING, SRC, FLUX>) return sl[k * kWave];
+                            if constexpr (DMA) return sl[k * kWave];
                             else return sp[(size_t)k * nc];
                         };
                         TF r1, r2, r3;
                         source_mk1<decltype(fk), TF>(a.prm, fk, TF(M.s[2]), TF(M.s[3]), TF(M.s[0]), TF(M.s[1]), r1, r2, r3);
                         q1s = r1; q2s = r2; q3s = r3;
                     } else if (SRC == GEL_SRC_TABLE || SRC == GEL_SRC_HYBRID) {
-                        const GPtr<const double> qt = gp(a.qtab);
-                        qms = qt[kc];
-                        q1s = qt[nc + kc];
-                        q2s = qt[2 * nc + kc];
-                        q3s = qt[3 * nc + kc];
-                        qQs = qt[4 * nc + kc];
+                        const GPtr<const TS> qt = gp((const TS*)a.qtab);
+                        auto tk = [&](int k) {
+                            if constexpr (DMA) return sl[(kSrcMom<SRC> + k) * kWave];
+                            else return qt[(size_t)k * nc + kc];
+                        };
+                        qms = tk(0);
+                        q1s = tk(1);
+                        q2s = tk(2);
+                        q3s = tk(3);
+                        qQs = tk(4);
                     }
                     double o[NV];
                     cell_update<Ph, sizeof(TF) == 4>(sd, Pot, vol, vol2, vs1, vs2, dt, gam, qms, q1s, q2s, q3s, qQs,
@@ -1433,14 +1459,14 @@ __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu
 sweep_kernel(SweepArgs a, int strips, int ntiles)
 {
     __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
-    __shared__ TS mom[kMomLds<TS, MOVING, SRC, FLUX, NW>];
+    __shared__ TS srcl[kSrcLds<TS, MOVING, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int t = xcd_item(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
         const int bx = t % strips, by = t / strips;
         const int i0 = by * a.rows, i1 = min(i0 + a.rows, a.ni);
         // no barrier between tiles: the update phase of the previous tile reads no ring,
         // and its faces phase ended at a barrier every wavefront has passed
-        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, mom, bx, i0, i1, acc);
+        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, srcl, bx, i0, i1, acc);
     }
     march_flush(a, acc);
 }
@@ -1457,13 +1483,13 @@ __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
     __shared__ MarchLds<Ph, TS, false, false, NW> S;
-    __shared__ TS mom[kMomLds<TS, false, SRC, FLUX, NW>];
+    __shared__ TS srcl[kSrcLds<TS, false, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
         const BatchItem b = items[it];
         const SweepArgs& a = args[b.cs];
         const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
-        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, mom, b.bx, i0, i1, acc);
+        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, srcl, b.bx, i0, i1, acc);
         march_flush(a, acc);
     }
 }

@@ -187,7 +187,8 @@ int gel_set_sources(gel_ctx* ctx, const double* fields);
 
 /* Per-cell source table for GEL_SRC_TABLE: 5 variable-major arrays of ncells doubles
  * (mass, momentum x, momentum y, energy, tracer), applied like the sources q2_1, q2_2,
- * q3 of the update (Setka.cpp:10444-10446).  Clears the fallback flags. */
+ * q3 of the update (Setka.cpp:10444-10446).  Clears the fallback flags.  Kept on the
+ * device in the state's precision (FP32-bulk contexts round it to FP32). */
 int gel_set_source_table(gel_ctx* ctx, const double* q);
 /* Copy the source table [5][ncells] back to the host (after a refresh, for tests). */
 int gel_get_source_table(gel_ctx* ctx, double* q);
