@@ -257,9 +257,8 @@ __global__ void features_kernel(int ni, int nj, const double* __restrict__ X, co
 // Analytic charge-exchange source per cell (the judge and fallback): the device
 // closure of Cell::Get_Sourse_MK1 (Cell.cpp:590-664) on the cell's own moments, in
 // the state's type, as the sweep evaluates it.
-template <class TS>
-__device__ inline void analytic_q(const gel_params& prm, size_t nc, size_t c, const TS* __restrict__ st,
-                                  const TS* __restrict__ src, double q[5])
+template <class TS, class P>
+__device__ inline void analytic_q(const gel_params& prm, size_t nc, size_t c, P st, P src, double q[5])
 {
     TS r1, r2, r3;
     auto fk = [&](int k) { return src[(size_t)k * nc + c]; };
@@ -274,7 +273,7 @@ __global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __res
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     double q[5];
-    analytic_q(prm, nc, c, st, src, q);
+    analytic_q<TS>(prm, nc, c, st, src, q);
 #pragma unroll
     for (int k = 0; k < 5; k++) qtab[k * nc + c] = q[k];
 }
@@ -307,12 +306,13 @@ __device__ inline void refresh_features(const SweepArgs& a, size_t c, const NetD
     const int nj = a.nj;
     const int i = (int)(c / nj), j = (int)(c % nj);
     const size_t ka = (size_t)i * (nj + 1) + j, kb = ka + (nj + 1);
-    const double px[4] = { a.x[ka], a.x[kb], a.x[kb + 1], a.x[ka + 1] };
-    const double py[4] = { a.y[ka], a.y[kb], a.y[kb + 1], a.y[ka + 1] };
+    const GPtr<const double> X = gp(a.x), Y = gp(a.y);
+    const double px[4] = { X[ka], X[kb], X[kb + 1], X[ka + 1] };
+    const double py[4] = { Y[ka], Y[kb], Y[kb + 1], Y[ka + 1] };
     double cx, cy;
     quad_center(px, py, cx, cy);
-    const TS* st = (const TS*)a.next;
-    const TS* src = (const TS*)a.src;
+    const GPtr<const TS> st = gp((const TS*)a.next);
+    const GPtr<const TS> src = gp((const TS*)a.src);
     const double ro = st[c], p = st[nc + c], u = st[2 * nc + c], v = st[3 * nc + c];
     double f[kIn];
     f[0] = log(ro); f[1] = u; f[2] = v; f[3] = log(p / ro);
@@ -455,9 +455,9 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     }
     bool fall = false;
     if (c < nc) {
-        const TS* st = (const TS*)a.next;
-        const TS* src = (const TS*)a.src;
-        TS* qtab = (TS*)const_cast<void*>(a.qtab);
+        const GPtr<const TS> st = gp((const TS*)a.next);
+        const GPtr<const TS> src = gp((const TS*)a.src);
+        const GPtr<TS> qtab = gp((TS*)const_cast<void*>(a.qtab));
         const double ro = st[c];
         const double nH = (double)src[c] + (double)src[nc + c] + (double)src[2 * nc + c] + (double)src[3 * nc + c];
         const double scale = a.prm.src_coef * ro * nH;
@@ -466,7 +466,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         for (int k = 0; k < 5; k++) qm[k] = ((double)ys[tid * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
         if (judge) {
             double qa[5];
-            analytic_q(a.prm, nc, c, st, src, qa);
+            analytic_q<TS>(a.prm, nc, c, st, src, qa);
             double err = 0.0;
 #pragma unroll
             for (int k = 0; k < 5; k++) {
@@ -481,7 +481,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         }
 #pragma unroll
         for (int k = 0; k < 5; k++) qtab[k * nc + c] = TS(qm[k]);
-        if (a.qflag) const_cast<int*>(a.qflag)[c] = fall ? 1 : 0;
+        if (a.qflag) gp(const_cast<int*>(a.qflag))[c] = fall ? 1 : 0;
     }
     const unsigned long long mask = __ballot(fall);
     if ((tid & 63) == 0 && mask) atomicAdd(nfall, (unsigned long long)__popcll(mask));

@@ -27,6 +27,15 @@ GEL_HD double sr(double x) { return sqrt(x); }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float sr(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// Every buffer the kernels touch is in global memory, but kernels that read the
+// buffers' addresses from a table in memory (the batched sweep, the fused source
+// refresh) leave the compiler unable to tell their address space: it would emit flat
+// instructions, which also count on the LDS counter, so that every wait for an LDS
+// read would wait for the HBM traffic in flight as well.  The hot accesses go through
+// gp(), the address typed as global.
+template <class T> using GPtr = __attribute__((address_space(1))) T*;
+template <class T> __device__ __forceinline__ GPtr<T> gp(T* p) { return (GPtr<T>)p; }
+
 // Primitive gas state in the reference's order (Parametr, Cell.h:21-25).
 template <class T> struct Gas { T ro, p, u, v, Q; };
 

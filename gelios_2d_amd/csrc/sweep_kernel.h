@@ -34,14 +34,6 @@ namespace gel {
 constexpr int kWave = 64;
 constexpr int kHalo = 2;
 constexpr int kOut = kWave - 2 * kHalo;   // output columns per wavefront
-
-// Every buffer the sweep touches is in global memory, but the batched kernel reads
-// the buffers' addresses from a table in memory, where the compiler cannot tell their
-// address space: it would emit flat instructions, which also count on the LDS
-// counter, so that every wait for an LDS read would wait for the HBM traffic in
-// flight as well.  The hot accesses go through gp(), the address typed as global.
-template <class T> using GPtr = __attribute__((address_space(1))) T*;
-template <class T> __device__ __forceinline__ GPtr<T> gp(T* p) { return (GPtr<T>)p; }
 constexpr int kNW = 4;                    // wavefronts per work-group, marching together
 
 // State vectors.  GAS: (ro, p, u, v, Q), the fields of Parametr (Cell.h:21-25).
