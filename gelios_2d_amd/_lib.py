@@ -54,6 +54,7 @@ SIGNATURES = {
                                              _dp, _dp]),
     "gel_set_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_get_source_table": (ctypes.c_int, [ctypes.c_void_p, _dp]),
+    "gel_get_source_flags": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "gel_tracking_reference": (None, [ctypes.POINTER(GelTracking)]),
     "gel_set_tracking": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelTracking), _dp]),

@@ -51,19 +51,21 @@ class Batch:
                                                  ctypes.byref(ms)))
         return r.value, w.value, ms.value
 
-    def refresh(self, model, tol: float = 1e-3, judge: bool = True) -> int:
+    def refresh(self, model, tol: float = 1e-3, judge=True) -> int:
         """Refresh every case's source table from its current state now; returns the
-        number of cells that fell back to the analytic source."""
+        number of cells that fell back to the analytic source.  judge: False / 0 none,
+        True / 1 every cell, 2 (GEL_SRC_HYBRID) only the cells not flagged already."""
         n = ctypes.c_longlong()
-        _lib.check(_lib.lib().gel_batch_refresh(self._h, model.mlp._h, float(tol), 1 if judge else 0,
+        _lib.check(_lib.lib().gel_batch_refresh(self._h, model.mlp._h, float(tol), int(judge),
                                                 ctypes.byref(n)))
         return n.value
 
-    def set_source_model(self, model, every: int, tol: float = 1e-3, judge: bool = True) -> None:
-        """Refresh the source tables inside step() after every `every` steps (0: off)."""
+    def set_source_model(self, model, every: int, tol: float = 1e-3, judge=True) -> None:
+        """Refresh the source tables inside step() after every `every` steps (0: off);
+        judge as in refresh()."""
         self._model = model          # the batch holds the network's handle
         _lib.check(_lib.lib().gel_batch_set_source_model(self._h, model.mlp._h if model else None, int(every),
-                                                         float(tol), 1 if judge else 0))
+                                                         float(tol), int(judge)))
 
     def source_stats(self):
         """(refreshes, fallback cells) since the last call."""

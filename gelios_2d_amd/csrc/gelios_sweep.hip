@@ -340,6 +340,15 @@ int gel_get_source_table(gel_ctx* c, double* q)
     return GEL_OK;
 }
 
+int gel_get_source_flags(gel_ctx* c, int* flags)
+{
+    if (!c || !flags) return fail(GEL_E_ARG, "null argument");
+    if (!c->d_qflag) return fail(GEL_E_STATE, "source table not set");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    HIP_OK(hipMemcpy(flags, c->d_qflag, c->nc * sizeof(int), hipMemcpyDeviceToHost));
+    return GEL_OK;
+}
+
 int gel_set_rows_per_wave(gel_ctx* c, int rows)
 {
     if (!c || rows < 1 || rows > 4096) return fail(GEL_E_ARG, "rows");
@@ -997,18 +1006,24 @@ int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, 
             if ((c->prm.source_mode != GEL_SRC_TABLE && c->prm.source_mode != GEL_SRC_HYBRID) || !c->d_src)
                 return fail(GEL_E_STATE, "a learned source needs GEL_SRC_TABLE or GEL_SRC_HYBRID cases with "
                                          "hydrogen moments set");
+    if (judge < 0 || judge > 2) return fail(GEL_E_ARG, "judge is 0, 1 or 2");
+    if (judge == 2)
+        for (const gel_ctx* c : b->cases)
+            if (c->prm.source_mode != GEL_SRC_HYBRID) return fail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID cases");
     b->model = every > 0 ? m : nullptr;
     b->every = every;
     b->tol = tol;
-    b->judge = judge ? 1 : 0;
+    b->judge = judge;
     batch_drop_graph(b);
     return GEL_OK;
 }
 
 int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback)
 {
-    if (!b || !m || !(tol >= 0.0)) return fail(GEL_E_ARG, "bad argument");
+    if (!b || !m || !(tol >= 0.0) || judge < 0 || judge > 2) return fail(GEL_E_ARG, "bad argument");
     for (const gel_ctx* c : b->cases) {
+        if (judge == 2 && c->prm.source_mode != GEL_SRC_HYBRID)
+            return fail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID cases");
         if (!c->d_src) return fail(GEL_E_STATE, "hydrogen moments not set");
         if (!c->d_qtab) return fail(GEL_E_STATE, "source table not allocated (gel_set_source_table)");
     }

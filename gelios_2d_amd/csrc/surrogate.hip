@@ -405,6 +405,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     __shared__ __attribute__((aligned(16))) bf16 ha[kSub * kLd];
     __shared__ __attribute__((aligned(16))) bf16 hb[kSub * kLd];
     float* ys = (float*)xs;                 // [kTile][8] FP32
+    __shared__ int wcount[kTile / 64];
 
     int cs = 0;
     long long t = blockIdx.x;
@@ -418,15 +419,32 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     const size_t c0 = (size_t)t * kTile;
     const int tid = threadIdx.x;
     const size_t c = c0 + tid;
-    {
-        bf16* row = xs + tid * kInP;
-        if (c < nc) refresh_features<TS>(a, c, net, row);
-        else
-            for (int k = 0; k < kInP; k++) row[k] = 0;
+    // judge 2 (hybrid cases): cells the last refresh sent to the analytic closure keep
+    // it and are not evaluated; the others are packed to the front of the tile, so
+    // that the MFMA passes run over ceil(live / 64) rows of 64 instead of four
+    const bool sticky = judge == 2 && a.qflag;
+    const bool kept = sticky && c < nc && gp(a.qflag)[c] != 0;
+    const bool live = c < nc && !kept;
+    int pos = tid, nlive = (int)min((size_t)kTile, nc > c0 ? nc - c0 : (size_t)0);
+    if (sticky) {
+        const unsigned long long m = __ballot(live);
+        const int lane = tid & 63, wave = tid >> 6;
+        if (lane == 0) wcount[wave] = __popcll(m);
+        __syncthreads();
+        int before = 0;
+        nlive = 0;
+        for (int k = 0; k < kTile / 64; k++) {
+            if (k < wave) before += wcount[k];
+            nlive += wcount[k];
+        }
+        pos = before + __popcll(m & ((1ull << lane) - 1ull));
     }
+    if (live) refresh_features<TS>(a, c, net, xs + pos * kInP);
+    if (tid >= nlive)
+        for (int k = 0; k < kInP; k++) xs[tid * kInP + k] = 0;
     __syncthreads();
     for (int sub = 0; sub < kTile / kSub; sub++) {
-        if (c0 + (size_t)sub * kSub >= nc) break;       // uniform
+        if (sub * kSub >= nlive) break;                 // uniform
         refresh_hidden<kInP>(xs + sub * kSub * kInP, kInP, net.w[0], net.b[0], ha);
         __syncthreads();
         refresh_hidden<kHid>(ha, kLd, net.w[1], net.b[1], hb);
@@ -453,8 +471,8 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         }
         __syncthreads();
     }
-    bool fall = false;
-    if (c < nc) {
+    bool fall = kept;
+    if (live) {
         const GPtr<const TS> st = gp((const TS*)a.next);
         const GPtr<const TS> src = gp((const TS*)a.src);
         const GPtr<TS> qtab = gp((TS*)const_cast<void*>(a.qtab));
@@ -463,7 +481,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         const double scale = a.prm.src_coef * ro * nH;
         double qm[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) qm[k] = ((double)ys[tid * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
+        for (int k = 0; k < 5; k++) qm[k] = ((double)ys[pos * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
         if (judge) {
             double qa[5];
             analytic_q<TS>(a.prm, nc, c, st, src, qa);
@@ -954,6 +972,8 @@ int gel_surrogate_refresh(gel_ctx* c, gel_mlp* m, double tol, int judge, long lo
     int st = check_ctx(c);
     if (st != GEL_OK) return st;
     if (!m || !m->mu) return mfail(GEL_E_STATE, "network scaling not set");
+    if (judge < 0 || judge > 2) return mfail(GEL_E_ARG, "judge is 0, 1 or 2");
+    if (judge == 2 && c->prm.source_mode != GEL_SRC_HYBRID) return mfail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID");
     if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * c->ts) != hipSuccess)
         return mfail(GEL_E_NOMEM, "source table");
     if (!c->d_qflag && (hipMalloc((void**)&c->d_qflag, c->nc * sizeof(int)) != hipSuccess ||

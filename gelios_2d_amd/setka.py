@@ -96,6 +96,13 @@ class Setka:
         _lib.check(_lib.lib().gel_get_source_table(self._ctx, _lib.dptr(q)))
         return q
 
+    def get_source_flags(self) -> np.ndarray:
+        """The analytic-fallback flags [ncells] the last refresh's judge left (1: the cell
+        takes the analytic closure in GEL_SRC_HYBRID)."""
+        f = np.empty(self.ncells, np.int32)
+        _lib.check(_lib.lib().gel_get_source_flags(self._ctx, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return f
+
     def set_source_table(self, q: np.ndarray) -> None:
         """Per-cell sources for GEL_SRC_TABLE: (mass, mom x, mom y, energy, tracer) x ncells."""
         a = np.ascontiguousarray(q, np.float64).reshape(5, self.ncells)

@@ -126,12 +126,13 @@ class SourceModel:
         L = _setup_src(_setup(_lib.lib()))
         _check(L.gel_mlp_set_scaling(mlp._h, _f(self.mu), _f(self.sd), _f(self.ymu), _f(self.ysd)))
 
-    def refresh(self, setka, tol: float = 1e-3, judge: bool = True) -> int:
+    def refresh(self, setka, tol: float = 1e-3, judge=True) -> int:
         """Fill the Setka's source table from the network (GEL_SRC_TABLE); returns the
-        number of cells that fell back to the analytic source."""
+        number of cells that fell back to the analytic source.  judge: False / 0 none,
+        True / 1 every cell, 2 (GEL_SRC_HYBRID) only the cells not flagged already."""
         L = _setup_src(_setup(_lib.lib()))
         n = ctypes.c_longlong(0)
-        _check(L.gel_surrogate_refresh(setka._ctx, self.mlp._h, float(tol), 1 if judge else 0, ctypes.byref(n)))
+        _check(L.gel_surrogate_refresh(setka._ctx, self.mlp._h, float(tol), int(judge), ctypes.byref(n)))
         return n.value
 
     def save(self, path: str) -> None:

@@ -192,6 +192,10 @@ int gel_set_sources(gel_ctx* ctx, const double* fields);
 int gel_set_source_table(gel_ctx* ctx, const double* q);
 /* Copy the source table [5][ncells] back to the host (after a refresh, for tests). */
 int gel_get_source_table(gel_ctx* ctx, double* q);
+/* Copy the analytic-fallback flags [ncells] (1: the cell takes the GEL_SRC_MK1 closure
+ * in GEL_SRC_HYBRID) back to the host.  No reference counterpart: the flags belong to
+ * the learned source, which replaces the reference's per-cell source fields. */
+int gel_get_source_flags(gel_ctx* ctx, int* flags);
 
 /* Advance nsteps steps: replaces the body of the step loop of
  * Setka::Go_5_komponent__MK2 (Setka.cpp:10131-10481) on a static grid.
@@ -345,7 +349,11 @@ int gel_surrogate_features(gel_ctx* ctx, float* x);
 int gel_source_analytic(gel_ctx* ctx, double* q);
 /* Fill the context's source table (GEL_SRC_TABLE) from the network; with judge != 0
  * cells whose error against the analytic source exceeds tol fall back to it.  One
- * fused launch (features, five MFMA layers, judge) over tiles of 64 cells. */
+ * fused launch (features, five MFMA layers, judge) over tiles of 256 cells.
+ * judge = 2 (GEL_SRC_HYBRID contexts only): cells already flagged keep the analytic
+ * closure and are not evaluated again; only the others run the network and the judge
+ * (a tile packs them, so that the network runs over ceil(live / 64) rows of 64).
+ * n_fallback counts every flagged cell, kept or new. */
 int gel_surrogate_refresh(gel_ctx* ctx, gel_mlp* m, double tol, int judge, long long* n_fallback);
 /* The same for every case of a batch at once (their current states). */
 int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback);

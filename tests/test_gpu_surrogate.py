@@ -169,3 +169,32 @@ def test_fp32_bulk_analytic_source_against_fp64():
         print(f"variable {k}: fp32 vs fp64 L1 {err:.3e}, no source vs fp64 {gap:.3e}")
         assert err <= 2e-5
         assert gap > 10 * err
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_sticky_judge_keeps_flags_and_matches_full_refresh(precision):
+    """judge 2: a cell the previous refresh flagged keeps the analytic closure and is
+    not evaluated; every other cell gets exactly what a full refresh (judge 1) of the
+    same state gives it (table value and flag), though the tile packs it elsewhere."""
+    from gelios_2d_amd import surrogate as sg
+    from gelios_2d_amd.params import GEL_SRC_HYBRID
+    w, f = sg.workload_with_hydrogen(90, 70, 0, 2.3, 0.12, 17, moving=False)
+    w.params.precision = precision
+    m = _model()
+    a = _setka(w, f, GEL_SRC_HYBRID)
+    a.set_source_table(np.zeros((5, a.ncells)))
+    m.refresh(a, tol=1e-3, judge=1)
+    flags0 = a.get_source_flags()
+    assert 0 < flags0.sum() < a.ncells
+    a.Go_5_komponent__MK2(25)
+    n2 = m.refresh(a, tol=1e-3, judge=2)
+    ta, fa = a.get_source_table(), a.get_source_flags()
+    w.prim0 = a.download()
+    b = _setka(w, f, GEL_SRC_HYBRID)
+    b.set_source_table(np.zeros((5, b.ncells)))
+    m.refresh(b, tol=1e-3, judge=1)
+    tb, fb = b.get_source_table(), b.get_source_flags()
+    live = flags0 == 0
+    assert np.array_equal(fa, np.where(live, fb, 1))
+    assert n2 == int(fa.sum())
+    assert np.array_equal(ta[:, live], tb[:, live])
