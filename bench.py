@@ -49,16 +49,22 @@ def log(*a):
 def bytes_per_cell_update(w, fallback_fraction=0.0):
     """Algorithmic HBM bytes per cell-update: the state read once and written once
     (2 x variables x bytes per value), one node's coordinates (16 B; 32 B on a moving
-    grid, positions before and after the step), and the per-cell source inputs: the
-    table (5 x 8 B), the hydrogen moments (19 x 8 B), or in the hybrid mode a flag
-    byte and the moments for the fallback fraction of cells, the table for the rest."""
+    grid, positions before and after the step), and the per-cell source inputs, held in
+    the state's precision: the table (5 values), the hydrogen moments (19 values), or
+    in the hybrid mode a 4-byte flag and all of them (the static FP32-bulk sweep copies
+    every input of a row into LDS), the FP64 sweep the moments for the fallback
+    fraction of cells and the table for the rest."""
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID, GEL_SRC_MK1, GEL_SRC_TABLE
     nv = {1: 9, 2: 21}.get(w.params.physics, 5)
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
-    geom = 32 if (getattr(w, "motion", None) or getattr(w, "tracking", None)) else 16
-    src = {GEL_SRC_TABLE: 40, GEL_SRC_MK1: 152}.get(w.params.source_mode, 0)
+    moving = bool(getattr(w, "motion", None) or getattr(w, "tracking", None))
+    geom = 32 if moving else 16
+    src = {GEL_SRC_TABLE: 5 * s, GEL_SRC_MK1: 19 * s}.get(w.params.source_mode, 0)
     if w.params.source_mode == GEL_SRC_HYBRID:
-        src = 1 + fallback_fraction * 152 + (1.0 - fallback_fraction) * 40
+        if s == 4 and not moving:
+            src = 4 + 24 * s
+        else:
+            src = 4 + fallback_fraction * 19 * s + (1.0 - fallback_fraction) * 5 * s
     return 2 * nv * s + geom + src
 
 
@@ -153,6 +159,30 @@ def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
              "fallback_cells_initial": nfall, "refreshes": nref,
              "fallback_fraction": nfb / max(1, nref * ncells), "refresh_ms_host_timed": refresh_ms,
              "kernel_ms_per_step": dev_ms / steps, "sweep_ms_per_step": sweep_ms}, finals, specs, cyc)
+
+
+def run_ensemble_analytic(specs, steps, cycle):
+    """The same cases in FP32 bulk with the analytic closure at every step and no
+    network (GEL_SRC_MK1), batched and graph-replayed like the timed run: what the
+    learned source is measured against."""
+    from gelios_2d_amd.batch import Batch
+    from gelios_2d_amd.params import GEL_SRC_MK1
+    from gelios_2d_amd.setka import Setka
+    cases = []
+    for w, f in specs:
+        prm = w.params.copy()
+        prm.source_mode = GEL_SRC_MK1
+        s = Setka(w.grid, prm)
+        s.upload(w.prim0)
+        s.set_sources(f)
+        cases.append(s)
+    b = Batch(cases)
+    b.autotune(12)
+    b.step(cycle, True)
+    t0 = time.perf_counter()
+    b.step(steps, True)
+    wall = time.perf_counter() - t0
+    return {"value": sum(s.ncells for s in cases) * steps / wall, "ms_per_step": wall * 1e3 / steps}
 
 
 def run_ensemble_reference(specs, steps):
@@ -256,7 +286,10 @@ def main():
                 den[k] += float(np.abs(b[k]).sum())
         l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
         log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
-            f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}")
+            f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}; "
+            f"judge fallback fraction {r5['fallback_fraction']:.3f}")
+        ra = run_ensemble_analytic(specs, K, warm)
+        log(f"bench[ensemble]: same cases, FP32 bulk with the analytic closure only: {ra['value']:.4e} cell-updates/s")
         B = bytes_per_cell_update(specs[0][0], r5["fallback_fraction"])
         achieved = B * r5["cells"] / (r5["sweep_ms_per_step"] * 1e-3) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
@@ -268,6 +301,7 @@ def main():
             "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / 1000,
                            "roofline": r64["roofline"]},
             "speedup_vs_fp64_plain": r5["value"] / r64["value"],
+            "fp32_analytic_source_only": dict(ra, note="same cases, FP32 bulk, GEL_SRC_MK1 every step, no network"),
             "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + warm),
             "batch": {"cases": r5["cases"], "cells": r5["cells"], "autotune": r5["autotune"],
                       "source_refresh_every": args.refresh_every, "judge_tol": 1e-3,

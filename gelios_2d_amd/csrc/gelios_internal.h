@@ -66,6 +66,16 @@ hipError_t gel_refresh_launch(hipStream_t s, const gel::SweepArgs* d_args, const
                               gel_mlp* m, bool f32, double tol, int judge, unsigned long long* d_nfall);
 // tiles the refresh of an ni x nj case takes
 inline long long gel_refresh_tiles(int ni, int nj) { return ((long long)ni * nj + 255) / 256; }
+// Sticky refresh (judge 2) of a batch over d_list[0 .. d_count[0]), the (case, cell)
+// pairs still on the network, then the list rebuilt from the flags (the flagged
+// cells added to *d_nfall).  Asynchronous on stream s; no allocation.
+hipError_t gel_refresh_list_launch(hipStream_t s, const gel::SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                                   long long ncells, int2* d_list, unsigned* d_count, gel_mlp* m, bool f32,
+                                   double tol, unsigned long long* d_nfall);
+// Build d_list / d_count[0] from the flags (d_count[1] must be 0; it is left 0);
+// ncells: the cells of all cases.
+hipError_t gel_live_list_build(hipStream_t s, const gel::SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                               long long ncells, int2* d_list, unsigned* d_count);
 
 struct gel_ctx;
 namespace gel {

@@ -214,7 +214,10 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
             c->has_types = true;
         }
     }
-    if ((e = hipMemset(c->d_cnt, 0, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "memset");
+    // the context's stream does not synchronise with the null stream: clear on it
+    if ((e = hipMemsetAsync(c->d_cnt, 0, 3 * sizeof(unsigned long long), c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return bad(e, "memset");
     *out = c;
     return GEL_OK;
 }
@@ -285,7 +288,10 @@ int gel_set_sources(gel_ctx* c, const double* f)
 {
     if (!c || !f) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
-    // stored in the state's type: FP32 for the FP32-bulk sweep, whose closure runs in FP32
+    // stored in the state's type: FP32 for the FP32-bulk sweep, whose closure runs in FP32;
+    // the work queued on the context's stream finishes first (it does not synchronise
+    // with the null stream the blocking copy runs on)
+    HIP_OK(hipStreamSynchronize(c->stream));
     const size_t n = (size_t)GEL_MK1_NFIELDS * c->nc;
     if (!c->d_src) {
         hipError_t e = hipMalloc(&c->d_src, n * c->ts);
@@ -305,11 +311,14 @@ int gel_set_source_table(gel_ctx* c, const double* q)
 {
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
-    // stored in the state's type, like the moments
+    // stored in the state's type, like the moments; ordered after the work queued on
+    // the context's stream
+    HIP_OK(hipStreamSynchronize(c->stream));
     const size_t n = (size_t)5 * c->nc;
     if (!c->d_qflag && hipMalloc((void**)&c->d_qflag, c->nc * sizeof(int)) != hipSuccess)
         return fail(GEL_E_NOMEM, "source flags");
-    HIP_OK(hipMemset(c->d_qflag, 0, c->nc * sizeof(int)));
+    HIP_OK(hipMemsetAsync(c->d_qflag, 0, c->nc * sizeof(int), c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
     if (!c->d_qtab) {
         hipError_t e = hipMalloc(&c->d_qtab, n * c->ts);
         if (e != hipSuccess) return fail(GEL_E_NOMEM, "source table");
@@ -764,6 +773,10 @@ struct gel_batch {
     long long refresh_tiles = 0;
     int2* d_tiles = nullptr;                 // refresh work-group -> (case, tile of the case)
     unsigned long long* d_nfall = nullptr;   // [1] fallback cells since the last stats call
+    // judge 2: the (case, cell) pairs still on the network, [0] their number, [1] the
+    // counter of the next build (0 between builds)
+    int2* d_list = nullptr;
+    unsigned* d_lcount = nullptr;
     long long refreshes = 0;
 };
 
@@ -856,16 +869,46 @@ static int batch_cycle(const gel_batch* b)
     return kCycle / a * b->every;
 }
 
+static long long batch_cells(const gel_batch* b)
+{
+    long long n = 0;
+    for (const gel_ctx* c : b->cases) n += (long long)c->nc;
+    return n;
+}
+
 // step i of a cycle: the sweep, then the source refresh when one is due
 static hipError_t batch_launch_step(const gel_batch* b, int i)
 {
     const int phase = i % kCycle;
     hipError_t e = batch_launch(b, phase);
-    if (e == hipSuccess && b->model && b->every > 0 && (i + 1) % b->every == 0)
-        e = gel_refresh_launch(b->stream, b->d_args + (size_t)phase * b->cases.size(), b->d_tiles,
-                               b->refresh_tiles, b->model, b->cases[0]->prm.precision == GEL_PREC_FP32_BULK, b->tol,
-                               b->judge, b->d_nfall);
+    if (e == hipSuccess && b->model && b->every > 0 && (i + 1) % b->every == 0) {
+        const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
+        const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
+        if (b->judge == 2)
+            e = gel_refresh_list_launch(b->stream, args, b->d_tiles, b->refresh_tiles, batch_cells(b), b->d_list,
+                                        b->d_lcount, b->model, f32, b->tol, b->d_nfall);
+        else
+            e = gel_refresh_launch(b->stream, args, b->d_tiles, b->refresh_tiles, b->model, f32, b->tol, b->judge,
+                                   b->d_nfall);
+    }
     return e;
+}
+
+// judge 2: (re)build the list of cells still on the network from the current flags,
+// on the batch stream, before a call steps or refreshes with it
+static int batch_build_list(gel_batch* b)
+{
+    if (!b->d_list) {
+        size_t n = 0;
+        for (const gel_ctx* c : b->cases) n += c->nc;
+        if (hipMalloc((void**)&b->d_list, n * sizeof(int2)) != hipSuccess ||
+            hipMalloc((void**)&b->d_lcount, 2 * sizeof(unsigned)) != hipSuccess ||
+            hipMemsetAsync(b->d_lcount, 0, 2 * sizeof(unsigned), b->stream) != hipSuccess)
+            return fail(GEL_E_NOMEM, "list of the cells on the network");
+    }
+    const hipError_t e = gel_live_list_build(b->stream, b->d_args, b->d_tiles, b->refresh_tiles, batch_cells(b),
+                                             b->d_list, b->d_lcount);
+    return e == hipSuccess ? GEL_OK : fail(GEL_E_HIP, std::string("list build: ") + hipGetErrorString(e));
 }
 
 static void batch_advance(gel_batch* b, int n)
@@ -886,6 +929,8 @@ void gel_batch_destroy(gel_batch* b)
     (void)hipFree(b->d_items);
     (void)hipFree(b->d_nfall);
     (void)hipFree(b->d_tiles);
+    (void)hipFree(b->d_list);
+    (void)hipFree(b->d_lcount);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -926,7 +971,8 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
         hipMalloc(&b->d_args, (size_t)(kCycle + 1) * ncases * sizeof(SweepArgs)) != hipSuccess ||
         hipMalloc(&b->d_nfall, sizeof(unsigned long long)) != hipSuccess ||
-        hipMemset(b->d_nfall, 0, sizeof(unsigned long long)) != hipSuccess) {
+        hipMemsetAsync(b->d_nfall, 0, sizeof(unsigned long long), b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
         gel_batch_destroy(b);
         return fail(GEL_E_NOMEM, "batch");
     }
@@ -958,6 +1004,7 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
     for (gel_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
     int st = batch_upload_args(b);
     if (st != GEL_OK) return st;
+    if (b->model && b->every > 0 && b->judge == 2 && (st = batch_build_list(b)) != GEL_OK) return st;
     HIP_OK(hipEventRecord(b->ev0, b->stream));
     int done = 0;
     const int cyc = batch_cycle(b);
@@ -1030,12 +1077,16 @@ int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long
     for (gel_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
     int st = batch_upload_args(b);
     if (st != GEL_OK) return st;
+    if (judge == 2 && (st = batch_build_list(b)) != GEL_OK) return st;
     unsigned long long* cnt = nullptr;
     if (hipMalloc(&cnt, sizeof *cnt) != hipSuccess) return fail(GEL_E_NOMEM, "counter");
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof *cnt, b->stream);
+    const SweepArgs* args = b->d_args + (size_t)kCycle * b->cases.size();
+    const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
     if (e == hipSuccess)
-        e = gel_refresh_launch(b->stream, b->d_args + (size_t)kCycle * b->cases.size(), b->d_tiles,
-                               b->refresh_tiles, m, b->cases[0]->prm.precision == GEL_PREC_FP32_BULK, tol, judge, cnt);
+        e = judge == 2 ? gel_refresh_list_launch(b->stream, args, b->d_tiles, b->refresh_tiles, batch_cells(b),
+                                                 b->d_list, b->d_lcount, m, f32, tol, cnt)
+                       : gel_refresh_launch(b->stream, args, b->d_tiles, b->refresh_tiles, m, f32, tol, judge, cnt);
     unsigned long long h = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
@@ -1051,7 +1102,8 @@ int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallba
     unsigned long long h = 0;
     HIP_OK(hipStreamSynchronize(b->stream));
     HIP_OK(hipMemcpy(&h, b->d_nfall, sizeof h, hipMemcpyDeviceToHost));
-    HIP_OK(hipMemset(b->d_nfall, 0, sizeof h));
+    HIP_OK(hipMemsetAsync(b->d_nfall, 0, sizeof h, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
     if (refreshes) *refreshes = b->refreshes;
     if (fallback_cells) *fallback_cells = (long long)h;
     b->refreshes = 0;

@@ -394,64 +394,42 @@ __device__ inline void refresh_hidden(const bf16* __restrict__ A, int lda, const
     }
 }
 
-template <class TS>
-__global__ void __launch_bounds__(256)
-refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tiles, NetDev net, double tol,
-               int judge, unsigned long long* __restrict__ nfall)
-{
+// The network and the judge for up to kTile cells packed into rows [0, nlive) of a
+// work-group: thread t brings cell c of case cs (live: evaluate it) into row pos.
+// Features (a cell per thread) -> 5 MFMA layers over ceil(nlive / 64) passes of 64
+// rows with the activations in LDS -> de-normalisation, analytic judge and fallback
+// (a cell per thread) -> source table and flag.  kept: a cell not evaluated that
+// counts as fallen back.  Uniform across the work-group: nlive.
+struct RefreshLds {
     // xs: the BF16 input rows; a row's 32 bytes take its 5 FP32 outputs once the
     // output layer has run (the row's input is consumed by then)
-    __shared__ __attribute__((aligned(16))) bf16 xs[kTile * kInP];
-    __shared__ __attribute__((aligned(16))) bf16 ha[kSub * kLd];
-    __shared__ __attribute__((aligned(16))) bf16 hb[kSub * kLd];
-    float* ys = (float*)xs;                 // [kTile][8] FP32
-    __shared__ int wcount[kTile / 64];
+    __attribute__((aligned(16))) bf16 xs[kTile * kInP];
+    __attribute__((aligned(16))) bf16 ha[kSub * kLd];
+    __attribute__((aligned(16))) bf16 hb[kSub * kLd];
+    int wcount[kTile / 64];
+};
 
-    int cs = 0;
-    long long t = blockIdx.x;
-    if (tiles) {                        // (case, tile of that case)
-        const int2 e = tiles[blockIdx.x];
-        cs = e.x;
-        t = e.y;
-    }
-    const SweepArgs& a = args[cs];
-    const size_t nc = (size_t)a.ni * a.nj;
-    const size_t c0 = (size_t)t * kTile;
+template <class TS>
+__device__ __forceinline__ void refresh_rows(const SweepArgs& a, size_t c, bool live, bool kept, int pos, int nlive,
+                                             const NetDev& net, double tol, int judge,
+                                             unsigned long long* __restrict__ nfall, RefreshLds& L)
+{
+    bf16* xs = L.xs;
+    float* ys = (float*)xs;                 // [kTile][8] FP32
     const int tid = threadIdx.x;
-    const size_t c = c0 + tid;
-    // judge 2 (hybrid cases): cells the last refresh sent to the analytic closure keep
-    // it and are not evaluated; the others are packed to the front of the tile, so
-    // that the MFMA passes run over ceil(live / 64) rows of 64 instead of four
-    const bool sticky = judge == 2 && a.qflag;
-    const bool kept = sticky && c < nc && gp(a.qflag)[c] != 0;
-    const bool live = c < nc && !kept;
-    int pos = tid, nlive = (int)min((size_t)kTile, nc > c0 ? nc - c0 : (size_t)0);
-    if (sticky) {
-        const unsigned long long m = __ballot(live);
-        const int lane = tid & 63, wave = tid >> 6;
-        if (lane == 0) wcount[wave] = __popcll(m);
-        __syncthreads();
-        int before = 0;
-        nlive = 0;
-        for (int k = 0; k < kTile / 64; k++) {
-            if (k < wave) before += wcount[k];
-            nlive += wcount[k];
-        }
-        pos = before + __popcll(m & ((1ull << lane) - 1ull));
-    }
     if (live) refresh_features<TS>(a, c, net, xs + pos * kInP);
     if (tid >= nlive)
         for (int k = 0; k < kInP; k++) xs[tid * kInP + k] = 0;
     __syncthreads();
     for (int sub = 0; sub < kTile / kSub; sub++) {
         if (sub * kSub >= nlive) break;                 // uniform
-        refresh_hidden<kInP>(xs + sub * kSub * kInP, kInP, net.w[0], net.b[0], ha);
+        refresh_hidden<kInP>(xs + sub * kSub * kInP, kInP, net.w[0], net.b[0], L.ha);
         __syncthreads();
-        refresh_hidden<kHid>(ha, kLd, net.w[1], net.b[1], hb);
+        refresh_hidden<kHid>(L.ha, kLd, net.w[1], net.b[1], L.hb);
         __syncthreads();
-        refresh_hidden<kHid>(hb, kLd, net.w[2], net.b[2], ha);
+        refresh_hidden<kHid>(L.hb, kLd, net.w[2], net.b[2], L.ha);
         __syncthreads();
-        refresh_hidden<kHid>(ha, kLd, net.w[3], net.b[3], hb);
+        refresh_hidden<kHid>(L.ha, kLd, net.w[3], net.b[3], L.hb);
         __syncthreads();
         {   // output layer, transposed: wavefront w takes cells [16 w, 16 w + 16); a lane
             // holds outputs 4 q .. 4 q + 3 of cell 16 w + r (outputs 0 .. 4 are real)
@@ -461,7 +439,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
 #pragma unroll
             for (int k = 0; k < kHid; k += 32) {
                 const bf8 wa = *(const bf8*)(net.w[4] + (size_t)r * kHid + k + 8 * q);
-                const bf8 hv = *(const bf8*)(hb + (wave * 16 + r) * kLd + k + 8 * q);
+                const bf8 hv = *(const bf8*)(L.hb + (wave * 16 + r) * kLd + k + 8 * q);
                 acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa, hv, acc, 0, 0, 0);
             }
             float* yrow = ys + (sub * kSub + wave * 16 + r) * 8;
@@ -473,6 +451,7 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
     }
     bool fall = kept;
     if (live) {
+        const size_t nc = (size_t)a.ni * a.nj;
         const GPtr<const TS> st = gp((const TS*)a.next);
         const GPtr<const TS> src = gp((const TS*)a.src);
         const GPtr<TS> qtab = gp((TS*)const_cast<void*>(a.qtab));
@@ -501,8 +480,131 @@ refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tile
         for (int k = 0; k < 5; k++) qtab[k * nc + c] = TS(qm[k]);
         if (a.qflag) gp(const_cast<int*>(a.qflag))[c] = fall ? 1 : 0;
     }
-    const unsigned long long mask = __ballot(fall);
-    if ((tid & 63) == 0 && mask) atomicAdd(nfall, (unsigned long long)__popcll(mask));
+    if (nfall) {
+        const unsigned long long mask = __ballot(fall);
+        if ((tid & 63) == 0 && mask) atomicAdd(nfall, (unsigned long long)__popcll(mask));
+    }
+}
+
+// One work-group per tile of 256 cells of one case.  judge 2 (hybrid cases): cells
+// the last refresh sent to the analytic closure keep it and are not evaluated; the
+// others are packed to the front of the tile, so that the MFMA passes run over
+// ceil(live / 64) rows of 64 instead of four.
+template <class TS>
+__global__ void __launch_bounds__(256)
+refresh_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tiles, NetDev net, double tol,
+               int judge, unsigned long long* __restrict__ nfall)
+{
+    __shared__ RefreshLds L;
+    int cs = 0;
+    long long t = blockIdx.x;
+    if (tiles) {                        // (case, tile of that case)
+        const int2 e = tiles[blockIdx.x];
+        cs = e.x;
+        t = e.y;
+    }
+    const SweepArgs& a = args[cs];
+    const size_t nc = (size_t)a.ni * a.nj;
+    const size_t c0 = (size_t)t * kTile;
+    const int tid = threadIdx.x;
+    const size_t c = c0 + tid;
+    const bool sticky = judge == 2 && a.qflag;
+    const bool kept = sticky && c < nc && gp(a.qflag)[c] != 0;
+    const bool live = c < nc && !kept;
+    int pos = tid, nlive = (int)min((size_t)kTile, nc > c0 ? nc - c0 : (size_t)0);
+    if (sticky) {
+        const unsigned long long m = __ballot(live);
+        const int lane = tid & 63, wave = tid >> 6;
+        if (lane == 0) L.wcount[wave] = __popcll(m);
+        __syncthreads();
+        int before = 0;
+        nlive = 0;
+        for (int k = 0; k < kTile / 64; k++) {
+            if (k < wave) before += L.wcount[k];
+            nlive += L.wcount[k];
+        }
+        pos = before + __popcll(m & ((1ull << lane) - 1ull));
+    }
+    refresh_rows<TS>(a, c, live, kept, pos, nlive, net, tol, judge, nfall, L);
+}
+
+// judge 2 over a batch: the (case, cell) list of the cells still on the network, in
+// chunks of 256 (a persistent grid walks the chunks), so that the work scales with
+// the cells evaluated and not with the grid.  The list is rebuilt after every refresh
+// (live_list_kernel); the fallback count comes from there.
+template <class TS>
+__global__ void __launch_bounds__(256)
+refresh_list_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ list,
+                    const unsigned* __restrict__ count, NetDev net, double tol)
+{
+    __shared__ RefreshLds L;
+    const int n = (int)count[0];
+    const int tid = threadIdx.x;
+    for (int g = blockIdx.x; g * kTile < n; g += gridDim.x) {
+        const int nlive = min(kTile, n - g * kTile);
+        const bool live = tid < nlive;
+        int2 e = make_int2(0, 0);
+        if (live) e = list[g * kTile + tid];
+        refresh_rows<TS>(args[e.x], (size_t)e.y, live, false, tid, nlive, net, tol, 2, nullptr, L);
+        __syncthreads();    // the rows are rewritten by the next chunk
+    }
+}
+
+// The list of cells still on the network (flag 0) after a refresh, appended to
+// list[count[1]...].  A work-group takes kListTiles (case, tile) pairs: it counts its
+// cells, reserves their places with one atomic (a single counter takes one atomic
+// per work-group, not per wavefront: the reservations serialise), then each
+// wavefront writes its cells from its own offset.  The order varies from run to run;
+// the cells' results do not depend on it.
+constexpr int kListTiles = 16;
+__global__ void __launch_bounds__(256)
+live_list_kernel(const SweepArgs* __restrict__ args, const int2* __restrict__ tiles, int ntiles,
+                 int2* __restrict__ list, unsigned* __restrict__ count)
+{
+    __shared__ unsigned wsum[kTile / 64];
+    __shared__ unsigned gbase;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t0 = blockIdx.x * kListTiles, t1 = min(t0 + kListTiles, ntiles);
+    auto is_live = [&](int t, int2& e) -> bool {
+        const int2 tl = tiles[t];
+        const SweepArgs& a = args[tl.x];
+        const size_t nc = (size_t)a.ni * a.nj;
+        const size_t c = (size_t)tl.y * kTile + threadIdx.x;
+        e = make_int2(tl.x, (int)c);
+        return c < nc && gp(a.qflag)[c] == 0;
+    };
+    unsigned n = 0;
+    for (int t = t0; t < t1; t++) {
+        int2 e;
+        n += (unsigned)__popcll(__ballot(is_live(t, e)));
+    }
+    if (lane == 0) wsum[wave] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned tot = 0;
+        for (int k = 0; k < kTile / 64; k++) tot += wsum[k];
+        gbase = tot ? atomicAdd(&count[1], tot) : 0u;
+    }
+    __syncthreads();
+    unsigned off = gbase;
+    for (int k = 0; k < wave; k++) off += wsum[k];
+    for (int t = t0; t < t1; t++) {
+        int2 e;
+        const bool live = is_live(t, e);
+        const unsigned long long m = __ballot(live);
+        if (live) list[off + __popcll(m & ((1ull << lane) - 1ull))] = e;
+        off += (unsigned)__popcll(m);
+    }
+}
+
+// count[0] = the list just built, count[1] cleared for the next build; the cells not
+// on it (flagged) added to *nfall (one thread).
+__global__ void live_list_commit(unsigned* __restrict__ count, unsigned long long ncells,
+                                 unsigned long long* __restrict__ nfall)
+{
+    if (nfall) *nfall += ncells - count[1];
+    count[0] = count[1];
+    count[1] = 0;
 }
 
 }  // namespace gel
@@ -667,6 +769,40 @@ hipError_t gel_refresh_launch(hipStream_t s, const SweepArgs* d_args, const int2
     return hipGetLastError();
 }
 
+static void live_list_launch(hipStream_t s, const SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                             long long ncells, int2* d_list, unsigned* d_count, unsigned long long* d_nfall)
+{
+    const unsigned groups = (unsigned)((tiles + kListTiles - 1) / kListTiles);
+    hipLaunchKernelGGL(live_list_kernel, dim3(groups), dim3(256), 0, s, d_args, d_tiles, (int)tiles, d_list, d_count);
+    hipLaunchKernelGGL(live_list_commit, dim3(1), dim3(1), 0, s, d_count, (unsigned long long)ncells, d_nfall);
+}
+
+hipError_t gel_refresh_list_launch(hipStream_t s, const SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                                   long long ncells, int2* d_list, unsigned* d_count, gel_mlp* m, bool f32,
+                                   double tol, unsigned long long* d_nfall)
+{
+    if (tiles <= 0) return hipSuccess;
+    NetDev net;
+    for (int l = 0; l < kLayers; l++) { net.w[l] = m->wb[l]; net.b[l] = m->b[l]; }
+    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd;
+    // a persistent grid over the list's chunks: about three work-groups per CU
+    const dim3 grid(768);
+    if (f32)
+        hipLaunchKernelGGL(refresh_list_kernel<float>, grid, dim3(256), 0, s, d_args, d_list, d_count, net, tol);
+    else
+        hipLaunchKernelGGL(refresh_list_kernel<double>, grid, dim3(256), 0, s, d_args, d_list, d_count, net, tol);
+    live_list_launch(s, d_args, d_tiles, tiles, ncells, d_list, d_count, d_nfall);
+    return hipGetLastError();
+}
+
+hipError_t gel_live_list_build(hipStream_t s, const SweepArgs* d_args, const int2* d_tiles, long long tiles,
+                               long long ncells, int2* d_list, unsigned* d_count)
+{
+    if (tiles <= 0) return hipSuccess;
+    live_list_launch(s, d_args, d_tiles, tiles, ncells, d_list, d_count, nullptr);
+    return hipGetLastError();
+}
+
 static uint64_t splitmix(uint64_t& s)
 {
     uint64_t z = (s += 0x9e3779b97f4a7c15ull);
@@ -751,6 +887,9 @@ int gel_mlp_create(gel_mlp** out, uint64_t seed)
         for (auto f : bs) ok = ok && hipMalloc((void**)f, nb * 4) == hipSuccess && hipMemset(*f, 0, nb * 4) == hipSuccess;
         ok = ok && hipMalloc((void**)&m->wb[l], n * 2) == hipSuccess && hipMalloc((void**)&m->wt[l], n * 2) == hipSuccess;
     }
+    // the clears ran on the null stream, with which the network's stream does not
+    // synchronise
+    ok = ok && hipDeviceSynchronize() == hipSuccess;
     if (!ok) { gel_mlp_destroy(m); return mfail(GEL_E_NOMEM, "mlp parameters"); }
     // uniform(-a, a) with a = sqrt(6 / fan_in) for SiLU layers, sqrt(3 / fan_in) for the output
     std::vector<float> p((size_t)gel_mlp_param_count());
@@ -977,7 +1116,7 @@ int gel_surrogate_refresh(gel_ctx* c, gel_mlp* m, double tol, int judge, long lo
     if (!c->d_qtab && hipMalloc(&c->d_qtab, 5 * c->nc * c->ts) != hipSuccess)
         return mfail(GEL_E_NOMEM, "source table");
     if (!c->d_qflag && (hipMalloc((void**)&c->d_qflag, c->nc * sizeof(int)) != hipSuccess ||
-                        hipMemset(c->d_qflag, 0, c->nc * sizeof(int)) != hipSuccess))
+                        hipMemsetAsync(c->d_qflag, 0, c->nc * sizeof(int), c->stream) != hipSuccess))
         return mfail(GEL_E_NOMEM, "source flags");
     if (!m->d_rargs && (hipMalloc((void**)&m->d_rargs, sizeof(SweepArgs)) != hipSuccess ||
                         hipMalloc((void**)&m->d_rcount, sizeof(unsigned long long)) != hipSuccess))

@@ -81,24 +81,51 @@ def test_batch_refresh_equals_per_case_refresh():
 
 
 @pytest.mark.parametrize("every", [3, 4])
-def test_refresh_inside_graph_equals_interleaved_calls(every):
+@pytest.mark.parametrize("judge", [1, 2])
+def test_refresh_inside_graph_equals_interleaved_calls(every, judge):
     """Sources refreshed by the batch every `every` steps (inside the captured graph)
-    give the same states and tables as stepping and refreshing call by call."""
+    give the same states, tables and flags as stepping and refreshing call by call;
+    judge 2 runs the refresh over the batch's list of cells still on the network."""
     from gelios_2d_amd.batch import Batch
     m = _model()
     a, b_ = _cases(3), _cases(3)
     A, B = Batch(a), Batch(b_)
     A.refresh(m)
     B.refresh(m)
-    B.set_source_model(m, every, tol=1e-3, judge=True)
+    B.set_source_model(m, every, tol=1e-3, judge=judge)
     total = 2 * 12 + every + 1
     B.step(total, True)
     for k in range(total // every):
         A.step(every, False)
-        A.refresh(m)
+        A.refresh(m, judge=judge)
     A.step(total % every, False)
     for x, y in zip(a, b_):
         assert np.array_equal(x.download(), y.download())
         assert np.array_equal(x.get_source_table(), y.get_source_table())
+        assert np.array_equal(x.get_source_flags(), y.get_source_flags())
     r, n = B.source_stats()
     assert r == total // every and 0 <= n
+
+
+
+def test_batch_sticky_refresh_equals_per_case_refresh():
+    """judge 2 over a batch (the list of (case, cell) still on the network, a persistent
+    grid over its chunks) leaves every case the tables, flags and fallback count that
+    the per-case refresh (tiles of 256 cells, each packed) gives."""
+    from gelios_2d_amd.batch import Batch
+    m = _model()
+    a, b_ = _cases(4), _cases(4)
+    A = Batch(a)
+    A.refresh(m)
+    for s in b_:
+        m.refresh(s, tol=1e-3, judge=1)
+    A.step(18, True)
+    for s in b_:
+        s.Go_5_komponent__MK2(18)
+    na = A.refresh(m, judge=2)
+    nb = sum(m.refresh(s, tol=1e-3, judge=2) for s in b_)
+    assert na == nb
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.download(), y.download())
+        assert np.array_equal(x.get_source_flags(), y.get_source_flags())
+        assert np.array_equal(x.get_source_table(), y.get_source_table())
