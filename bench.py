@@ -144,9 +144,10 @@ def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
     wall = time.perf_counter() - t0
     nref, nfb = b.source_stats()
     finals = [s.download() for s in cases]
-    # device time of the sweep kernel alone (HIP events on the batch stream)
+    # device time of the sweep kernel alone: HIP events on the batch stream around
+    # single-step launches (a graph replay would add its launch gaps)
     b.set_source_model(None, 0)
-    sweep_ms = b.step(10 * 6, True) / 60
+    sweep_ms = sum(b.step(1, False) for _ in range(60)) / 60
     # device time of the refresh alone, as the timed steps run it: one fused launch
     # over all cases
     t1 = time.perf_counter()
