@@ -966,7 +966,15 @@ __device__ __forceinline__ bool surface_row(const SweepArgs& a, int r)
 // The update phase reads nothing from the rings, so the next step's loads may
 // overwrite them as soon as a wavefront gets there.  Every face is solved once, at
 // one call site per direction; boundary faces only choose their two states.
-template <class Ph, class TS, bool MOVING, bool TYPES, int NW>
+// Static FP32-bulk single-fluid sweeps copy the next step's cell rows by LDS DMA
+// (march_tile); their radial-face ring then lives in an LDS object of its own
+// (MarchFlux), so that the compiler sees that reading it does not touch the rows being
+// filled.  The other sweeps keep it in MarchLds (a separate object costs the FP64
+// kernels registers).
+template <class TS, bool MOVING, bool TYPES, int FLUX>
+constexpr bool kStateDmaT = sizeof(TS) == 4 && !MOVING && !TYPES && (FLUX & (kTrackFlux | kMfFlux | kInnerFlux)) == 0;
+
+template <class Ph, class TS, bool MOVING, bool TYPES, int NW, bool FAPART = false>
 struct MarchLds {
     static constexpr int NR = NW + 3;                 // rows held by the rings
     TS st[NR][Ph::NV][kWave];                         // primitives of cell rows
@@ -976,6 +984,13 @@ struct MarchLds {
     double ny[MOVING ? NR : 1][kWave + 1];            // grids read the geometry table)
     double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
     double ny2[MOVING ? NR : 1][kWave + 1];
+    double fps[FAPART ? 0 : NW + 1][Ph::NV][kWave];   // radial flux x area above a row (none when apart)
+    double ftm[FAPART ? 0 : NW + 1][kWave];           // CFL time of the cell above that face
+};
+
+// The radial faces' flux x area products and CFL times, apart from the rings.
+template <class Ph, int NW>
+struct MarchFlux {
     double fps[NW + 1][Ph::NV][kWave];                // radial flux x area above a row
     double ftm[NW + 1][kWave];                        // CFL time of the cell above that face
 };
@@ -999,9 +1014,9 @@ template <int SRC> constexpr int kSrcFields = kSrcMom<SRC> + kSrcTab<SRC> + (SRC
 template <class TS, bool MOVING, int SRC, int FLUX, int NW>
 constexpr int kSrcLds = kSrcDma<TS, MOVING, SRC, FLUX> ? NW * kSrcFields<SRC> * kWave : 1;
 
-template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW>
-__device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, MOVING, TYPES, NW>& S, TS* srcl,
-                                           int bx, int i0, int i1, MarchAcc& acc)
+template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FLUX, int NW, class LS, class FL>
+__device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS* srcl, int bx, int i0, int i1,
+                                           MarchAcc& acc)
 {
     // FLUX carries kTrackFlux on grids with fitted surfaces (gas, FP64, moving)
     constexpr bool TRACK = (FLUX & kTrackFlux) != 0;
@@ -1085,6 +1100,35 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
         if (lane == kWave - 1) put(kWave, j + 1);
     };
 
+    // Static FP32-bulk sweeps: a wavefront copies its next step's cell row from HBM
+    // straight into the row's ring slot by LDS DMA at the start of the update phase
+    // (which reads no ring; each wavefront waits for its copies before the next load
+    // phase's barrier), so that the load phase waits on no HBM load.  Every lane copies (lanes off the grid
+    // a cell of the row, unused); rows past the grid take load_row's defaults.
+    constexpr bool kStateDma = kStateDmaT<TS, MOVING, TYPES, FLUX>;
+    auto dma_row = [&](int r) {
+        if (r >= ni) { load_row(r); return; }
+        const int s = slot(r);
+        const GPtr<const TS> st0 = gp((const TS*)a.now) + (size_t)r * nj + min(max(j, 0), nj - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            __builtin_amdgcn_global_load_lds((const void*)(st0 + (size_t)v * nc), (void*)&S.st[s][v][0], 4, 0, 0);
+        // centroids (FP64): 64 x 8 B in two copies of 256 B, lane l taking half l % 2 of
+        // cell 32 h + l / 2
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int jj = min(max(bx * kOut + 32 * h + (lane >> 1) - kHalo, 0), nj - 1);
+            const size_t k = (size_t)r * nj + jj;
+            const GPtr<const char> ox = (GPtr<const char>)(G.cell(0) + k) + 4 * (lane & 1);
+            const GPtr<const char> oy = (GPtr<const char>)(G.cell(1) + k) + 4 * (lane & 1);
+            __builtin_amdgcn_global_load_lds((const void*)ox, (void*)((char*)&S.cx[s][0] + 256 * h), 4, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)oy, (void*)((char*)&S.cy[s][0] + 256 * h), 4, 0, 0);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    };
+
     auto cell = [&](int r, int ln) -> C {
         ln = min(max(ln, 0), kWave - 1);
         const int s = slot(r);
@@ -1124,8 +1168,11 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
         const bool act = (i < i1);
         const bool upd = act && (i >= i0);
 
-        // ---- load
-        if (act) load_row(i + 2);
+        // ---- load (after the first step of a tile, a DMA path copied the rows already)
+        if (act && (!kStateDma || base == i0 - 1)) load_row(i + 2);
+        // a barrier does not wait for a wavefront's LDS DMA: each waits for its own
+        // copies first (s_waitcnt vmcnt(0)), so that the rows are whole for every reader
+        if constexpr (kStateDma) __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
 
         // ---- faces
@@ -1177,8 +1224,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 }
                 const int fs = fslot(i);
 #pragma unroll
-                for (int k = 0; k < NV; k++) S.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
-                S.ftm[fs][lane] = cfl_time<TF>(cfl, N.cx, N.cy, f, uu);
+                for (int k = 0; k < NV; k++) F.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
+                F.ftm[fs][lane] = cfl_time<TF>(cfl, N.cx, N.cy, f, uu);
                 t_dn = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
                 if (out_lane) acc.bad += b;
             }
@@ -1194,13 +1241,16 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 } else {
                     const GPtr<const double> F[6] = { G.pol(0), G.pol(1), G.pol(2), G.pol(3), G.pol(4), G.pol(5) };
                     f = table_face(F, j >= -1 && j < nj, (size_t)i * (nj + 1) + (j + 1));
-                    // the cell's geometry, before and after the step
-                    if (col) {
-                        const size_t k = (size_t)i * nj + j;
-                        vs1 = G.cell(2)[k];
-                        vol = G.cell(3)[k];
+                    if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) {
+                        // the cell's geometry, loaded before the source inputs' LDS DMA
+                        // below waits for every load issued so far
+                        if (col) {
+                            const size_t k = (size_t)i * nj + j;
+                            vs1 = G.cell(2)[k];
+                            vol = G.cell(3)[k];
+                        }
+                        vs2 = vs1; vol2 = vol;
                     }
-                    vs2 = vs1; vol2 = vol;
                 }
                 const C Mr = cell(i, lane + 1), Ml = cell(i, lane - 1), Mr2 = cell(i, lane + 2);
                 double L[NV], R[NV], P[NV];
@@ -1222,7 +1272,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                     // the source inputs of row i into srcl[w] (field f, lane l -> column
                     // l), after the last global load this phase waits for (memory
                     // counters retire in order), so that they arrive during the polar
-                    // solve; the barrier ending the phase waits for them.  Every lane
+                    // solve; an explicit wait before the barrier ending the phase
+                    // completes them.  Every lane
                     // copies (lanes off the grid a cell of the row, unused), so that no
                     // branch leaves the memory counters path-dependent.  The compiler
                     // takes LDS DMA and loads into registers as retiring out of order and
@@ -1272,8 +1323,16 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
 #pragma unroll
                 for (int k = 0; k < NV; k++) ps0[k] = -shfl_up(psp[k], 1);
                 t0 = shfl_up(t_right, 1);
-                // cell geometry before and after the step (static grids: read above)
-                if constexpr (MOVING) {
+                // cell geometry before and after the step (static grids with the source
+                // DMA: read above)
+                if constexpr (!MOVING && !kSrcDma<TS, MOVING, SRC, FLUX>) {
+                    if (col) {
+                        const size_t k = (size_t)i * nj + j;
+                        vs1 = G.cell(2)[k];
+                        vol = G.cell(3)[k];
+                    }
+                    vs2 = vs1; vol2 = vol;
+                } else if constexpr (MOVING) {
                 const NodeP c00 = node(i, lane), c10 = node(i + 1, lane), c11 = node(i + 1, lane + 1),
                             c01 = node(i, lane + 1);
                 const double px[4] = { c00.x, c10.x, c11.x, c01.x };
@@ -1288,9 +1347,13 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 }
             }
         }
+        // the source inputs' LDS DMA of this phase: waited for before the barrier
+        if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) __builtin_amdgcn_s_waitcnt(0x0F70);
         __syncthreads();
 
         // ---- update (Setka.cpp:10372-10468)
+        if constexpr (kStateDma)
+            if (i + NW < i1) dma_row(i + NW + 2);
         if (upd) {
             const int fa = fslot(i), fb = fslot(i - 1);
             const double x = M.cx, y = M.cy;
@@ -1303,16 +1366,16 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, MarchLds<Ph, TS, 
                 for (int k = 0; k < NV; k++) {
                     double s = 0.0;
                     s = s + ps0[k];
-                    s = s + S.fps[fa][k][lane];
+                    s = s + F.fps[fa][k][lane];
                     s = s + psp[k];
-                    s = s + (-S.fps[fb][k][lane]);
+                    s = s + (-F.fps[fb][k][lane]);
                     Pot[k] = s;
                 }
                 double time = 10000000;
                 time = mn(time, t0);
                 time = mn(time, t_dn);
                 time = mn(time, t_pol);
-                time = mn(time, S.ftm[fb][lane]);
+                time = mn(time, F.ftm[fb][lane]);
                 const size_t kc = (size_t)i * nj + j;
                 const GPtr<TS> nxt = gp((TS*)a.next);
                 if constexpr (INNER) {
@@ -1450,7 +1513,9 @@ template <class Ph, class TS, class TF, bool MOVING, bool TYPES, int SRC, int FL
 __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_kernel(SweepArgs a, int strips, int ntiles)
 {
-    __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW> S;
+    constexpr bool apart = kStateDmaT<TS, MOVING, TYPES, FLUX>;
+    __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW, apart> S;
+    __shared__ MarchFlux<Ph, apart ? NW : 0> F;
     __shared__ TS srcl[kSrcLds<TS, MOVING, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int t = xcd_item(blockIdx.x, gridDim.x); t < ntiles; t += gridDim.x) {
@@ -1458,7 +1523,8 @@ sweep_kernel(SweepArgs a, int strips, int ntiles)
         const int i0 = by * a.rows, i1 = min(i0 + a.rows, a.ni);
         // no barrier between tiles: the update phase of the previous tile reads no ring,
         // and its faces phase ended at a barrier every wavefront has passed
-        march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, srcl, bx, i0, i1, acc);
+        if constexpr (apart) march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, F, srcl, bx, i0, i1, acc);
+        else march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, S, srcl, bx, i0, i1, acc);
     }
     march_flush(a, acc);
 }
@@ -1474,14 +1540,17 @@ template <class Ph, class TS, class TF, int SRC, int FLUX, int NW>
 __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
-    __shared__ MarchLds<Ph, TS, false, false, NW> S;
+    constexpr bool apart = kStateDmaT<TS, false, false, FLUX>;
+    __shared__ MarchLds<Ph, TS, false, false, NW, apart> S;
+    __shared__ MarchFlux<Ph, apart ? NW : 0> F;
     __shared__ TS srcl[kSrcLds<TS, false, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
         const BatchItem b = items[it];
         const SweepArgs& a = args[b.cs];
         const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
-        march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, srcl, b.bx, i0, i1, acc);
+        if constexpr (apart) march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, F, srcl, b.bx, i0, i1, acc);
+        else march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, S, srcl, b.bx, i0, i1, acc);
         march_flush(a, acc);
     }
 }
