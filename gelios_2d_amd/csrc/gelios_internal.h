@@ -33,6 +33,7 @@ struct SweepArgs {
     int srow[3];                  // node rows of the fitted surfaces, -1 where none
     int fluid;                    // plasma + hydrogen sweep: 0 plasma, 1..4 hydrogen H1..H4
     int voff[5];                  // its five variables' arrays in the 21-field state
+    const double* __restrict__ mfq;   // plasma + hydrogen: [5 fluids][4][ncells] sources of the step
     unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
@@ -107,6 +108,7 @@ struct gel_ctx {
     void* d_qtab = nullptr;        // [5][nc] per-cell source table (GEL_SRC_TABLE, GEL_SRC_HYBRID), ts bytes each
     int* d_qflag = nullptr;        // [nc] analytic-fallback flags (GEL_SRC_HYBRID)
     double* d_geo = nullptr;       // geometry table of the static grid (built on first use)
+    double* d_mfq = nullptr;       // plasma + hydrogen: charge-exchange sources of the step
     bool geo_valid = false;        // false after the nodes move
     bool tracking = false;         // fitted-surface tracking on (gel_set_tracking)
     gel_tracking trk{};

@@ -128,6 +128,7 @@ static void free_ctx(gel_ctx* c)
     (void)hipFree(c->d_state[0]); (void)hipFree(c->d_state[1]);
     (void)hipFree(c->d_ctype); (void)hipFree(c->d_otype); (void)hipFree(c->d_src); (void)hipFree(c->d_qtab);
     (void)hipFree(c->d_qflag);
+    (void)hipFree(c->d_mfq);
     (void)hipFree(c->d_geo);
     (void)hipFree(c->d_trk);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);

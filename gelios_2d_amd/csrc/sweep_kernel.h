@@ -800,14 +800,16 @@ __device__ __forceinline__ void mf_boundary_states(const SweepArgs& a, int ftype
     }
 }
 
-// Charge exchange from the cell's 21 values (Setka.cpp:9419-9470, :9554-9617): for the
-// plasma q = (0, momentum x, momentum y, energy); for hydrogen fluid Hk its mass,
-// momentum and energy sources, the population of the cell's region (C_1/C_centr H1,
-// C_2/C_3 H2, C_4 H3, else H4) receiving what the plasma loses.
-__device__ __forceinline__ void mf_sources(const SweepArgs& a, size_t kc, int ct, double* q)
+// Charge exchange from the cell's 21 values (Setka.cpp:9419-9470, :9554-9617), for the
+// five fluids at once (the rates nu are the plasma's with each population): q[0] for
+// the plasma = (0, momentum x, momentum y, energy); q[1 + k] for hydrogen fluid H(k+1)
+// its mass, momentum and energy sources, the population of the cell's region (C_1 /
+// C_centr H1, C_2 / C_3 H2, C_4 H3, else H4) receiving what the plasma loses.  The
+// arithmetic of each value is the per-fluid formula's, so that computing them together
+// changes no bit.
+__device__ __forceinline__ void mf_sources_all(const gel_params& prm, const double* __restrict__ st, size_t nc,
+                                               int ct, double q[5][4])
 {
-    const size_t nc = (size_t)a.ni * a.nj;
-    const double* __restrict__ st = (const double*)a.now + kc;
     const double ro = st[0], p = st[nc], u = st[2 * nc], v = st[3 * nc];
     double roH[4], pH[4], uH[4], vH[4], UM[4], UH[4], nu[4];
 #pragma unroll
@@ -820,36 +822,56 @@ __device__ __forceinline__ void mf_sources(const SweepArgs& a, size_t kc, int ct
     for (int h = 0; h < 4; h++) {
         UM[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (64.0 / (9.0 * pi)) * (p / ro + 2.0 * pH[h] / roH[h]));
         UH[h] = sqrt(sq(u - uH[h]) + sq(v - vH[h]) + (4.0 / pi) * (p / ro + 2.0 * pH[h] / roH[h]));
-        nu[h] = ro * roH[h] * UM[h] * sq(a.prm.sig_a - a.prm.sig_b * log(UM[h]));
+        nu[h] = ro * roH[h] * UM[h] * sq(prm.sig_a - prm.sig_b * log(UM[h]));
     }
-    if (a.fluid == 0) {
-        const double cp = a.prm.src_coef;
-        q[0] = 0.0;
-        q[1] = cp * (nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u));
-        q[2] = cp * (nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v));
+    {
+        const double cp = prm.src_coef;
+        q[0][0] = 0.0;
+        q[0][1] = cp * (nu[0] * (uH[0] - u) + nu[1] * (uH[1] - u) + nu[2] * (uH[2] - u) + nu[3] * (uH[3] - u));
+        q[0][2] = cp * (nu[0] * (vH[0] - v) + nu[1] * (vH[1] - v) + nu[2] * (vH[2] - v) + nu[3] * (vH[3] - v));
         double e[4];
 #pragma unroll
         for (int h = 0; h < 4; h++)
             e[h] = nu[h] * ((sq(uH[h]) + sq(vH[h]) - sq(u) - sq(v)) / 2.0 + (UH[h] / UM[h]) * (2.0 * pH[h] / roH[h] - p / ro));
-        q[3] = cp * (e[0] + e[1] + e[2] + e[3]);
-        return;
+        q[0][3] = cp * (e[0] + e[1] + e[2] + e[3]);
     }
-    const int h = a.fluid - 1;
-    int kk = 0;
+    int own;                                          // the population of the cell's region
     if (ct == GEL_C_CENTR || ct == GEL_C_1 || ct == GEL_C_2 || ct == GEL_C_3)
-        kk = (h == ((ct == GEL_C_1 || ct == GEL_C_CENTR) ? 0 : 1)) ? 1 : 0;
+        own = (ct == GEL_C_1 || ct == GEL_C_CENTR) ? 0 : 1;
     else
-        kk = (h == ((ct == GEL_C_4) ? 2 : 3)) ? 1 : 0;
+        own = (ct == GEL_C_4) ? 2 : 3;
     const double S1 = nu[0] + nu[1] + nu[2] + nu[3];
     const double S2 = nu[0] * ((sq(u) + sq(v)) / 2.0 + (UH[0] / UM[0]) * (p / ro))
                     + nu[1] * ((sq(u) + sq(v)) / 2.0 + (UH[1] / UM[1]) * (p / ro))
                     + nu[2] * ((sq(u) + sq(v)) / 2.0 + (UH[2] / UM[2]) * (p / ro))
                     + nu[3] * ((sq(u) + sq(v)) / 2.0 + (UH[3] / UM[3]) * (p / ro));
-    const double ch = a.prm.src_coef_h;
-    q[0] = ch * (kk * S1 - nu[h]);
-    q[1] = ch * (kk * S1 * u - nu[h] * uH[h]);
-    q[2] = ch * (kk * S1 * v - nu[h] * vH[h]);
-    q[3] = ch * (kk * S2 - nu[h] * ((sq(uH[h]) + sq(vH[h])) / 2.0 + (UH[h] / UM[h]) * 2.0 * (pH[h] / roH[h])));
+    const double ch = prm.src_coef_h;
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int kk = h == own ? 1 : 0;
+        q[1 + h][0] = ch * (kk * S1 - nu[h]);
+        q[1 + h][1] = ch * (kk * S1 * u - nu[h] * uH[h]);
+        q[1 + h][2] = ch * (kk * S1 * v - nu[h] * vH[h]);
+        q[1 + h][3] = ch * (kk * S2 - nu[h] * ((sq(uH[h]) + sq(vH[h])) / 2.0 + (UH[h] / UM[h]) * 2.0 * (pH[h] / roH[h])));
+    }
+}
+
+// One thread per cell: the five fluids' sources of the step into q[fluid][4][nc], from
+// the state the five launches of the step read (the reference takes them from
+// par[now1] for every fluid).
+template <bool TYPES>
+__global__ void __launch_bounds__(256) mf_source_kernel(SweepArgs a, double* __restrict__ out)
+{
+    const size_t nc = (size_t)a.ni * a.nj;
+    const size_t kc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (kc >= nc) return;
+    const int ct = TYPES ? (int)a.ctype[kc] : (int)GEL_C_2;
+    double q[5][4];
+    mf_sources_all(a.prm, (const double*)a.now + kc, nc, ct, q);
+#pragma unroll
+    for (int f = 0; f < 5; f++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) out[((size_t)f * 4 + k) * nc + kc] = q[f][k];
 }
 
 // Update of a hydrogen fluid (Setka.cpp:9619-9753): the gas update with a mass source
@@ -1411,8 +1433,11 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
 #pragma unroll
                         for (int k = 0; k < 5; k++) o[k] = M.s[k];
                     } else {
+                        // this fluid's sources of the step (mf_source_kernel)
                         double q[4];
-                        mf_sources(a, kc, ct, q);
+                        const GPtr<const double> qs = gp(a.mfq) + (size_t)a.fluid * 4 * nc + kc;
+#pragma unroll
+                        for (int k = 0; k < 4; k++) q[k] = qs[(size_t)k * nc];
                         double sd[5];
 #pragma unroll
                         for (int k = 0; k < 5; k++) sd[k] = M.s[k];
