@@ -55,6 +55,7 @@ struct CellT {          // one cell of the stencil as seen by a lane
     double cx, cy;
     int type;
     int ok;             // exists (inside the grid)
+    double rp;          // grids with cell types: (centroid radius)^(2 gamma), once per row load
 };
 
 struct Nodes {          // nodes (r, j) and (r, j+1) of a lane
@@ -222,7 +223,11 @@ __device__ __forceinline__ void tvd_radial(const CellT<Ph, double>& M, const Cel
     const double r1 = sqrt(sq(xd) + sq(yd));
     const double d1 = dist2d(fx, fy, xd, yd);
     ro = linear3(-d1, dro * sq(r1), -d2, M.s[0] * sq(r2), d3, N.s[0] * sq(r3), 0.0) / sq(r);
-    pr = linear3(-d1, dp * pow(r1, g2), -d2, M.s[1] * pow(r2, g2), d3, N.s[1] * pow(r3, g2), 0.0) / pow(r, g2);
+    // the cells' r^(2 gamma) come with the cells (computed once per row load as
+    // pow(sqrt(cx^2 + cy^2), 2 gamma), the same call with the same argument; the mirror
+    // image across the axis has the master's radius)
+    const double p1 = D.ok ? D.rp : M.rp;
+    pr = linear3(-d1, dp * p1, -d2, M.s[1] * M.rp, d3, N.s[1] * N.rp, 0.0) / pow(r, g2);
 }
 
 // Radial rescaling of a cell value to the face radius (Setka.cpp:10221-10226):
@@ -1002,6 +1007,7 @@ struct MarchLds {
     TS st[NR][Ph::NV][kWave];                         // primitives of cell rows
     double cx[NR][kWave], cy[NR][kWave];              // cell centroids
     int8_t ty[TYPES ? NR : 1][kWave];                 // cell types
+    double rp[TYPES ? NR : 1][kWave];                 // (centroid radius)^(2 gamma)
     double nx[MOVING ? NR : 1][kWave + 1];            // node rows before the step (static
     double ny[MOVING ? NR : 1][kWave + 1];            // grids read the geometry table)
     double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
@@ -1098,7 +1104,10 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         for (int v = 0; v < NV; v++) S.st[s][v][lane] = c.s[v];
         S.cx[s][lane] = c.cx;
         S.cy[s][lane] = c.cy;
-        if constexpr (TYPES) S.ty[s][lane] = (int8_t)c.type;
+        if constexpr (TYPES) {
+            S.ty[s][lane] = (int8_t)c.type;
+            S.rp[s][lane] = c.ok ? pow(sqrt(sq(c.cx) + sq(c.cy)), 2.0 * gam) : 0.0;
+        }
         if constexpr (!MOVING) return;
         const int q = r + 1, sq1 = slot(q);
         auto put = [&](int idx, int jj) {
@@ -1159,6 +1168,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         for (int v = 0; v < NV; v++) c.s[v] = S.st[s][v][ln];
         c.cx = S.cx[s][ln];
         c.cy = S.cy[s][ln];
+        c.rp = TYPES ? S.rp[s][ln] : 0.0;
         c.type = TYPES ? (int)S.ty[s][ln] : (int)GEL_C_2;
         const int jj = bx * kOut + ln - kHalo;
         c.ok = (jj >= 0 && jj < nj && r >= 0 && r < ni) ? 1 : 0;
