@@ -56,7 +56,21 @@ struct CellT {          // one cell of the stencil as seen by a lane
     int type;
     int ok;             // exists (inside the grid)
     double rp;          // grids with cell types: (centroid radius)^(2 gamma), once per row load
+    double ca, sa;      // cells the sweep rotates (C_1; every cell of the inner region): cos,
+                        // sin of the centroid's polar angle, once per row load
 };
+
+// Help.cpp:3-14 (polar_perenos) for a stencil cell: its angle's cosine and sine come
+// with the cell (cos(polar_angle(cx, cy)), sin(...), the same calls as polar_perenos).
+template <class C>
+__device__ __forceinline__ void rotate_cell(const C& M, double x2, double y2, double& u, double& v)
+{
+    const double a2 = polar_angle(x2, y2);
+    const double fr = u * M.ca + v * M.sa;
+    const double ff = -u * M.sa + v * M.ca;
+    u = fr * cos(a2) - ff * sin(a2);
+    v = fr * sin(a2) + ff * cos(a2);
+}
 
 struct Nodes {          // nodes (r, j) and (r, j+1) of a lane
     double xa, ya, xb, yb;
@@ -232,14 +246,14 @@ __device__ __forceinline__ void tvd_radial(const CellT<Ph, double>& M, const Cel
 
 // Radial rescaling of a cell value to the face radius (Setka.cpp:10221-10226):
 // ro and the tracer by (rc/dis)^2, p by (rc/dis)^(2 gamma); C_1 cells also rotate u,v.
-template <int NV>
-__device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, double gam,
-                                           double cx, double cy, double fx, double fy, bool rotate)
+template <int NV, class C>
+__device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, double gam, const C& M, double fx,
+                                           double fy, bool rotate)
 {
     g[0] = g[0] * sq(rc) / sq(dis);
     g[NV - 1] = g[NV - 1] * sq(rc) / sq(dis);
     g[1] = g[1] * pow(rc / dis, 2 * gam);
-    if (rotate) polar_perenos(cx, cy, fx, fy, g[2], g[3]);
+    if (rotate) rotate_cell(M, fx, fy, g[2], g[3]);
 }
 
 // Value of cell M on its side of an interior face (Setka.cpp:10213-10251).
@@ -253,7 +267,7 @@ __device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<P
 #pragma unroll
             for (int k = 0; k < Ph::NV; k++) g[k] = M.s[k];
             const double rc = sqrt(sq(M.cx) + sq(M.cy));
-            rescale_c1<Ph::NV>(g, rc, dis, gam, M.cx, M.cy, fx, fy, true);
+            rescale_c1<Ph::NV>(g, rc, dis, gam, M, fx, fy, true);
             double ro, pr;
             tvd_radial<Ph>(M, N, D, m_axis, fx, fy, gam, ro, pr);
             g[0] = ro; g[1] = pr;
@@ -386,7 +400,7 @@ __device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, c
         double g[NV];
 #pragma unroll
         for (int k = 0; k < NV; k++) g[k] = M.s[k];
-        rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, false);
+        rescale_c1<NV>(g, radius, f.dis, gam, M, f.fx, f.fy, false);
         const double d = sqrt(f.fx * f.fx + f.fy * f.fy);
         const double ro = 1.0 / (p.in_rho_div * d * d);
         double w[NV];
@@ -406,7 +420,7 @@ __device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, c
                 double g[NV];
 #pragma unroll
                 for (int k = 0; k < NV; k++) g[k] = M.s[k];
-                rescale_c1<NV>(g, radius, f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
+                rescale_c1<NV>(g, radius, f.dis, gam, M, f.fx, f.fy, true);
 #pragma unroll
                 for (int k = 0; k < NV; k++) L[k] = g[k];
                 done = true;
@@ -744,7 +758,7 @@ __device__ __forceinline__ void mf_side_value(const SweepArgs& a, const CellT<Ga
 #pragma unroll
         for (int k = 0; k < 5; k++) r[k] = M.s[k];
         if (a.fluid == 1) {
-            polar_perenos(M.cx, M.cy, fx, fy, r[2], r[3]);
+            rotate_cell(M, fx, fy, r[2], r[3]);
             mf_tvd_radial_h1(M, N, D, m_axis, fx, fy, a.prm.h_pow, r[0], r[1]);
         }
         return;
@@ -792,7 +806,7 @@ __device__ __forceinline__ void mf_boundary_states(const SweepArgs& a, int ftype
     } else if (ftype == GEL_F_AXIS) {
         if (TYPES && M.type == GEL_C_1 && a.fluid == 1) {
             L[0] = L[0] * pow(radius / f.dis, p.h_pow);
-            polar_perenos(M.cx, M.cy, f.fx, f.fy, L[2], L[3]);
+            rotate_cell(M, f.fx, f.fy, L[2], L[3]);
         }
 #pragma unroll
         for (int k = 0; k < 5; k++) R[k] = (k == 3) ? -L[k] : L[k];
@@ -916,8 +930,8 @@ __device__ __forceinline__ void inner_interior_states(const SweepArgs& a, const 
     const double gam = a.prm.gamma;
 #pragma unroll
     for (int k = 0; k < 5; k++) { L[k] = M.s[k]; R[k] = N.s[k]; }
-    rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, gam, M.cx, M.cy, f.fx, f.fy, true);
-    rescale_c1<5>(R, sqrt(sq(N.cx) + sq(N.cy)), f.dis, gam, N.cx, N.cy, f.fx, f.fy, true);
+    rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, gam, M, f.fx, f.fy, true);
+    rescale_c1<5>(R, sqrt(sq(N.cx) + sq(N.cy)), f.dis, gam, N, f.fx, f.fy, true);
     if (TYPES && M.type == GEL_C_1 && N.type == GEL_C_1) {
         tvd_radial<GasPhys>(M, N, D, m_axis, f.fx, f.fy, gam, L[0], L[1]);
         tvd_radial<GasPhys>(N, M, U, n_axis, f.fx, f.fy, gam, R[0], R[1]);
@@ -935,7 +949,7 @@ __device__ __forceinline__ void inner_boundary_states(const SweepArgs& a, int ft
     if (ftype == GEL_F_AXIS) {
 #pragma unroll
         for (int k = 0; k < 5; k++) L[k] = M.s[k];
-        rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, a.prm.gamma, M.cx, M.cy, f.fx, f.fy, true);
+        rescale_c1<5>(L, sqrt(sq(M.cx) + sq(M.cy)), f.dis, a.prm.gamma, M, f.fx, f.fy, true);
 #pragma unroll
         for (int k = 0; k < 5; k++) R[k] = (k == 3) ? -L[k] : L[k];
         return;
@@ -1008,6 +1022,8 @@ struct MarchLds {
     double cx[NR][kWave], cy[NR][kWave];              // cell centroids
     int8_t ty[TYPES ? NR : 1][kWave];                 // cell types
     double rp[TYPES ? NR : 1][kWave];                 // (centroid radius)^(2 gamma)
+    double ca[TYPES ? NR : 1][kWave];                 // cos, sin of the centroid's polar angle
+    double sa[TYPES ? NR : 1][kWave];
     double nx[MOVING ? NR : 1][kWave + 1];            // node rows before the step (static
     double ny[MOVING ? NR : 1][kWave + 1];            // grids read the geometry table)
     double nx2[MOVING ? NR : 1][kWave + 1];           // node rows after the step
@@ -1107,6 +1123,14 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         if constexpr (TYPES) {
             S.ty[s][lane] = (int8_t)c.type;
             S.rp[s][lane] = c.ok ? pow(sqrt(sq(c.cx) + sq(c.cy)), 2.0 * gam) : 0.0;
+            double ca = 1.0, sa = 0.0;
+            if (c.ok && (INNER || c.type == GEL_C_1)) {
+                const double a1 = polar_angle(c.cx, c.cy);
+                ca = cos(a1);
+                sa = sin(a1);
+            }
+            S.ca[s][lane] = ca;
+            S.sa[s][lane] = sa;
         }
         if constexpr (!MOVING) return;
         const int q = r + 1, sq1 = slot(q);
@@ -1169,6 +1193,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         c.cx = S.cx[s][ln];
         c.cy = S.cy[s][ln];
         c.rp = TYPES ? S.rp[s][ln] : 0.0;
+        c.ca = TYPES ? S.ca[s][ln] : 1.0;
+        c.sa = TYPES ? S.sa[s][ln] : 0.0;
         c.type = TYPES ? (int)S.ty[s][ln] : (int)GEL_C_2;
         const int jj = bx * kOut + ln - kHalo;
         c.ok = (jj >= 0 && jj < nj && r >= 0 && r < ni) ? 1 : 0;
