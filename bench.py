@@ -24,6 +24,8 @@ Workloads
                    (exact-solver surface speeds, A-rail redistribution, sweep with
                    exact-solver surface faces), 799 x 400 cells, FP64 HLL, the
                    reference's constants (CFL 0.1), 200 steps
+  hydrogen-1600x800  HLLD MHD, 1600 x 800 cells, moving grid, charge exchange with four
+                   frozen hydrogen populations, FP64 plain beside FP32 bulk, 2000 steps
   plasma-4h-800x400  the plasma and four hydrogen fluids (21 fields, HLLC, charge
                    exchange), 800 x 400 cells, static, FP64, 200 steps
 """
@@ -39,6 +41,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the batched ensemble's sweep kernel (hybrid source), measured
+# with rocprofv3 --pmc: FETCH_SIZE (scaled x2: a 1 GiB device copy reads as 524 302 KiB)
+# plus WRITE_SIZE, in KiB per dispatch (profiles/sweep_batch_ensemble_64_cases_fp32_hybrid.md)
+ENSEMBLE_TRAFFIC_BYTES = (2 * 746319 + 196987) * 1024
 METRIC = "cell-updates/s on 1 MI355X: FP32+surrogate vs FP64 plain at L1 err ≤1e-4"
 
 
@@ -75,6 +81,8 @@ def run_gpu(w, K, W, label):
     if getattr(w, "motion", None):
         s.set_motion(*w.motion)
     s.upload(w.prim0)
+    if getattr(w, "src", None) is not None:
+        s.set_sources(w.src)
     if getattr(w, "tracking", None) is not None:
         s.set_tracking(w.tracking, w.rails)
     s.Go_5_komponent__MK2(W)
@@ -218,6 +226,8 @@ def cpu_baseline(w, budget_s=12.0):
     else:
         tr = None
         sw = ora.OracleSweep.from_polar(w.grid, w.params)
+        if getattr(w, "src", None) is not None:
+            sw.src = w.src
     x0, y0 = w.grid.x, w.grid.y
     amp, period, phase = motion if motion else (0.0, 1, 0.0)
 
@@ -295,8 +305,9 @@ def main():
         achieved = B * r5["cells"] / (r5["sweep_ms_per_step"] * 1e-3) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
                   "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
-                               "kernel_avg_us": r5["sweep_ms_per_step"] * 1e3}}
+                               "frac": achieved / HBM_PEAK_GBS, "traffic": ENSEMBLE_TRAFFIC_BYTES,
+                               "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
+                               "bytes_per_cell_update": B, "kernel_avg_us": r5["sweep_ms_per_step"] * 1e3}}
         main_w = None
         extra = {
             "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / 1000,
@@ -314,8 +325,12 @@ def main():
         precision, dtype = "fp32-bulk + learned source (BF16 MFMA), batched, HIP graph", "f32"
         config = {"workload": "ensemble-64-cases-100x50-to-800x400-hlld-fp32-learned-source", "cells": r5["cells"], "cases": r5["cases"],
                   "physics": "mhd-hlld", "moving_grid": False, "precision": precision, "cfl": 0.4}
-    elif wl == "hlld-800x400":
-        w64, w32 = configs.hlld_mhd_800x400(GEL_PREC_FP64), configs.hlld_mhd_800x400(GEL_PREC_FP32_BULK)
+    elif wl in ("hlld-800x400", "hydrogen-1600x800"):
+        if wl == "hlld-800x400":
+            w64, w32 = configs.hlld_mhd_800x400(GEL_PREC_FP64), configs.hlld_mhd_800x400(GEL_PREC_FP32_BULK)
+        else:
+            w64 = configs.mhd_hydrogen_1600x800(GEL_PREC_FP64)
+            w32 = configs.mhd_hydrogen_1600x800(GEL_PREC_FP32_BULK)
         K = args.steps if args.steps is not None else w64.steps
         log(f"bench: {w64.name} / {w32.name}, {w64.grid.ncells} cells, warmup {W}, steps {K}")
         r64 = run_gpu(w64, K, W, "fp64-plain")
@@ -374,7 +389,7 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        cw = main_w if wl not in ("hlld-800x400", "ensemble") else w64
+        cw = main_w if wl not in ("hlld-800x400", "hydrogen-1600x800", "ensemble") else w64
         line["cpu_baseline"] = cpu_baseline(cw)
         log(f"bench: cpu baseline {line['cpu_baseline']['value']:.3e} cell-updates/s")
     print(json.dumps(line), flush=True)

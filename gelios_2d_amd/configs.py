@@ -26,6 +26,7 @@ class Workload:
     prim0: np.ndarray
     steps: int
     motion: tuple = None          # (amp, period, phase) of the prescribed node motion
+    src: np.ndarray = None        # [GEL_MK1_NFIELDS][ncells] hydrogen moments of the charge-exchange source
 
 
 WIND_RHO, WIND_U, WIND_P = 1.0, 10.0, 0.01
@@ -113,6 +114,22 @@ def hlld_mhd_800x400(precision: int = GEL_PREC_FP64) -> Workload:
     """HLLD MHD, 800 x 400 cells, moving grid, 1000 steps, CFL 0.4, seed 2."""
     tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
     return mhd_heliosphere(800, 400, 1000, seed=2, precision=precision, name=f"hlld-mhd-{tag}-800x400-moving")
+
+
+def mhd_hydrogen_1600x800(precision: int = GEL_PREC_FP64, steps: int = 2000) -> Workload:
+    """The multi-fluid form: HLLD MHD at 1600 x 800 cells on the moving grid, coupled to
+    four frozen hydrogen populations (n_H = 0.1 (1 - exp(-r/50)), bulk velocity -2.5
+    along the axis plus a +-5% seeded field, lognormal temperature around 1) by the
+    charge-exchange source with sigma(U) = (1 - 0.07 ln U)^2 (Cell::Get_Sourse_MK1,
+    Cell.cpp:498-665); inflow Mach 2, CFL 0.4, seed 3, 2000 steps.  The hydrogen fields
+    are in .src."""
+    from .surrogate import workload_with_hydrogen
+    tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
+    w, f = workload_with_hydrogen(1600, 800, steps, INF_MACH, 0.1, 3, moving=True)
+    w.params.precision = precision
+    w.src = f
+    w.name = f"hlld-mhd-{tag}-hydrogen-1600x800-moving"
+    return w
 
 
 def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None,

@@ -95,3 +95,82 @@ def test_fp32_bulk_error_against_fp64(mhd):
         outs.append(s.download())
     assert l1_rel(outs[1], outs[0], 0) <= 1e-4
     assert l1_rel(outs[1], outs[0], 1) <= 1e-4
+
+
+def _hydrogen_setka(w):
+    from gelios_2d_amd.setka import Setka
+    s = Setka(w.grid, w.params)
+    s.set_motion(*w.motion)
+    s.upload(w.prim0)
+    s.set_sources(w.src)
+    return s
+
+
+def test_mhd_moving_charge_exchange_matches_oracle(oracle_lib):
+    """HLLD on the moving grid with the charge-exchange source of four frozen hydrogen
+    populations (Cell::Get_Sourse_MK1, Cell.cpp:498-665) against the oracle."""
+    from gelios_2d_amd.surrogate import workload_with_hydrogen
+    w, f = workload_with_hydrogen(48, 36, 30, 2.0, 0.1, 11, moving=True)
+    w.src = f
+    amp, period, phase = w.motion
+    sw = oracle_lib.OracleSweep.from_polar(w.grid, w.params)
+    sw.src = f
+    ref = sw.run_motion(w.prim0, w.steps, w.grid.x, w.grid.y, amp, period, phase)
+    s = _hydrogen_setka(w)
+    st = s.Go_5_komponent__MK2(w.steps)
+    assert st.n_numeric == 0
+    assert rel_err_conserved(s.download(), ref) <= TOL
+
+
+def test_hydrogen_1600x800_full_size(oracle_lib):
+    """The multi-fluid form at its full size (1600 x 800 cells, moving grid, charge
+    exchange with frozen hydrogen): the FP64 sweep matches the oracle over 3 steps, and
+    after 40 steps the FP32-bulk sweep stays within density / pressure L1 1e-4 of it."""
+    from gelios_2d_amd.configs import mhd_hydrogen_1600x800
+    w64 = mhd_hydrogen_1600x800(steps=40)
+    w32 = mhd_hydrogen_1600x800(GEL_PREC_FP32_BULK, steps=40)
+    assert (w64.grid.ni, w64.grid.nj) == (1600, 800)
+    amp, period, phase = w64.motion
+    sw = oracle_lib.OracleSweep.from_polar(w64.grid, w64.params)
+    sw.src = w64.src
+    ref = sw.run_motion(w64.prim0, 3, w64.grid.x, w64.grid.y, amp, period, phase)
+    s64 = _hydrogen_setka(w64)
+    s64.Go_5_komponent__MK2(3)
+    assert rel_err_conserved(s64.download(), ref) <= TOL
+    st = s64.Go_5_komponent__MK2(w64.steps - 3)
+    assert st.n_numeric == 0
+    s32 = _hydrogen_setka(w32)
+    st = s32.Go_5_komponent__MK2(w32.steps)
+    assert st.n_numeric == 0
+    a, b = s32.download(), s64.download()
+    assert np.isfinite(a).all()
+    assert l1_rel(a, b, 0) <= 1e-4
+    assert l1_rel(a, b, 1) <= 1e-4
+    s32.close()
+    s64.close()
+
+
+def test_hlld_800x400_full_size(oracle_lib):
+    """The MHD workload at its full size (800 x 400 cells, moving grid, |B| 0.05 at 60
+    deg): the FP64 sweep matches the oracle over 4 steps, and after 200 steps the
+    FP32-bulk sweep stays within density / pressure L1 1e-4 of it."""
+    from gelios_2d_amd.configs import hlld_mhd_800x400
+    from gelios_2d_amd.setka import Setka
+    w64, w32 = hlld_mhd_800x400(), hlld_mhd_800x400(GEL_PREC_FP32_BULK)
+    amp, period, phase = w64.motion
+    ref = oracle_lib.OracleSweep.from_polar(w64.grid, w64.params).run_motion(w64.prim0, 4, w64.grid.x, w64.grid.y,
+                                                                             amp, period, phase)
+    outs = []
+    for w in (w64, w32):
+        s = Setka(w.grid, w.params)
+        s.set_motion(*w.motion)
+        s.upload(w.prim0)
+        s.Go_5_komponent__MK2(4)
+        if w is w64:
+            assert rel_err_conserved(s.download(), ref) <= TOL
+        st = s.Go_5_komponent__MK2(196)
+        assert st.n_numeric == 0
+        outs.append(s.download())
+        s.close()
+    assert l1_rel(outs[1], outs[0], 0) <= 1e-4
+    assert l1_rel(outs[1], outs[0], 1) <= 1e-4
