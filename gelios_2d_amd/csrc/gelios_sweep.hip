@@ -942,10 +942,16 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
 {
     if (!out || ncases <= 0 || !cases) return fail(GEL_E_ARG, "null argument");
     *out = nullptr;
+    for (int k = 0; k < ncases; k++) {
+        if (!cases[k]) return fail(GEL_E_ARG, "null case");
+        // one context twice would have two sweeps of a step write its one next buffer
+        for (int l = 0; l < k; l++)
+            if (cases[l] == cases[k]) return fail(GEL_E_ARG, "a context appears twice in the batch");
+    }
     const gel_params& p0 = cases[0]->prm;
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
-        if (!c || !c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
+        if (!c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
         if (c->has_types || !c->motion.empty() || c->tracking || c->prm.physics == GEL_PHYS_GAS4H ||
             c->prm.physics == GEL_PHYS_GAS_INNER)
             return fail(GEL_E_UNSUPPORTED, "batched cases are static single-fluid grids without C_1/C_centr");

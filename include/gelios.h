@@ -312,7 +312,8 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
  * Many independent contexts (static grids, same physics / precision / flux / source
  * mode) advanced together: one persistent launch per step walks the (case, strip,
  * chunk) work list; each case keeps its own state, buffers and CFL time step.
- * Whole cycles of 6 steps replay a captured HIP graph. */
+ * Whole cycles of 6 steps replay a captured HIP graph.  Each context may appear once
+ * (GEL_E_ARG otherwise, and for a null entry). */
 typedef struct gel_batch gel_batch;
 int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases);
 void gel_batch_destroy(gel_batch* b);

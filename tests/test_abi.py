@@ -45,5 +45,10 @@ def test_errors_without_device():
     assert L.gel_ctx_create(ctypes.byref(ctx), ctypes.byref(g), ctypes.byref(p)) == 1
     assert L.gel_ctx_create(None, None, None) == 1
     assert L.gel_step(None, 1, None) == 1
+    # batches: a null entry is rejected before any case is read
+    out = ctypes.c_void_p()
+    cases = (ctypes.c_void_p * 2)(None, None)
+    assert L.gel_batch_create(ctypes.byref(out), 2, cases) == 1
+    assert out.value is None
     with pytest.raises(_lib.GelError):
         _lib.check(1)

@@ -51,6 +51,17 @@ def test_graph_replays_match_eager_each_time():
         x0 = a[0].download()      # eager host work between replays
 
 
+def test_batch_rejects_a_context_twice():
+    from gelios_2d_amd._lib import GelError
+    from gelios_2d_amd.batch import Batch
+    cases = _cases(2)
+    with pytest.raises(GelError):
+        Batch([cases[0], cases[1], cases[0]])
+    b = Batch(cases)
+    b.step(2, False)
+    b.close()
+
+
 def test_autotune_restores_state():
     from gelios_2d_amd.batch import Batch
     cases = _cases(4)
