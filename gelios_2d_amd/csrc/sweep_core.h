@@ -47,10 +47,25 @@ template <class T> GEL_HD T minmod(T x, T y)
     if (s == T(0)) return T(0);
     return (s / T(2)) * mn(fabs(x), fabs(y));
 }
+// minmod(a / p, b / q) with one division: each ratio's sign from the signs of its
+// operands, the smaller magnitude chosen by the cross products |a| |q| and |b| |p|,
+// and only that ratio divided out (|a| / |p| has the bits of |a / p|).  It returns what
+// minmod does except where the two magnitudes agree to about an ulp, where either may
+// come back.  p, q nonzero (the limiter's centre-distance differences).
+template <class T> GEL_HD T minmod_ratio(T a, T p, T b, T q)
+{
+    if (a == T(0) || b == T(0)) return T(0);
+    const bool nx = (a < T(0)) != (p < T(0)), ny = (b < T(0)) != (q < T(0));
+    if (nx != ny) return T(0);
+    const T fa = fabs(a), fp = fabs(p), fb = fabs(b), fq = fabs(q);
+    const bool first = fa * fq < fb * fp;
+    const T m = (first ? fa : fb) / (first ? fp : fq);     // the one division
+    return nx ? -m : m;
+}
 // Help.cpp:96-113: minmod slope through (x1,t1),(x2,t2),(x3,t3), evaluated at y.
 template <class T> GEL_HD T linear3(T x1, T t1, T x2, T t2, T x3, T t3, T y)
 {
-    const T d = minmod((t1 - t2) / (x1 - x2), (t2 - t3) / (x2 - x3));
+    const T d = minmod_ratio(t1 - t2, x1 - x2, t2 - t3, x2 - x3);
     return d * (y - x2) + t2;
 }
 // Help.cpp:37-65.

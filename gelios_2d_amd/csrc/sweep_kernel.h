@@ -182,7 +182,7 @@ template <class TF>
 __device__ __forceinline__ double lin3v(double t1, double t2, double t3, const Lin3T<TF>& g)
 {
     if constexpr (sizeof(TF) == 8) {
-        const double d = minmod<double>((t1 - t2) / g.den12, (t2 - t3) / g.den23);
+        const double d = minmod_ratio<double>(t1 - t2, g.den12, t2 - t3, g.den23);
         return d * g.d2 + t2;
     } else {
         const float d = minmod<float>((float(t1) - float(t2)) * g.i12, (float(t2) - float(t3)) * g.i23);
