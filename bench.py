@@ -255,8 +255,17 @@ def cpu_baseline(w, budget_s=12.0):
     for n in range(steps):
         cur = one(cur, 2 + n)
     dt = time.perf_counter() - t0
-    return {"value": w.grid.ncells * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of {w.name} ({w.grid.ncells} cells), FP64, OpenMP {threads} threads"}
+    out = {"value": w.grid.ncells * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+           "sample": f"{steps} steps of {w.name} ({w.grid.ncells} cells), FP64, OpenMP {threads} threads"}
+    # fairness: the compiled reference sweep against the port on a grid both run (the
+    # reference has no MHD sweep), same threads; where the harness was built
+    from oracle import time_reference as tr
+    if tr.available():
+        m = tr.measure(200, 100, 20, threads)
+        out["fairness"] = {"workload": "reference gas HLL sweep, 200x100 polar grid with C_1 wind rows, 20 steps",
+                           "reference_cell_updates_per_s": m["reference"], "port_cell_updates_per_s": m["port"],
+                           "port_over_reference": m["port"] / m["reference"]}
+    return out
 
 
 def main():

@@ -17,6 +17,7 @@
 #include "Solvers.h"
 #include "Rail.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -347,7 +348,11 @@ static int mode_sweep(const char* in, const char* out)
 
     int final_buf;
     if (!moving) {
+        // the step loop alone, timed (the CPU baseline's fairness check)
+        const auto t0 = std::chrono::steady_clock::now();
         S->Go_5_komponent__MK2(steps);
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::fprintf(stderr, "ref_sweep_seconds %.6f\n", sec);
         final_buf = steps % 2;
     } else {
         for (int s = 0; s < steps; s++) {

@@ -1,0 +1,90 @@
+"""Fairness check of the CPU baseline: the compiled reference sweep against the oracle.
+
+Test infrastructure, run only in the build container, where /root/reference exists
+(oracle/Makefile builds oracle/_ref/ref_harness from the reference sources).  Both
+sides advance the same polar grid (C_1 wind rows at the inner boundary, the
+reference's constants: HLL, CFL 0.1) from the same state for the same number of steps
+with the same OpenMP thread count; only the step loop is timed (the harness times its
+call of Setka::Go_5_komponent__MK2 with std::chrono::steady_clock).
+
+    python oracle/time_reference.py [--ni 200] [--nj 100] [--steps 20] [--threads 8]
+
+Prints the cell-updates/s of both and their ratio.  With more than one thread the
+reference's shared time step races (Setka.cpp:10141), so only one thread is compared
+bit for bit.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+from gelios_2d_amd.params import reference_params  # noqa: E402
+from oracle import oracle as ora  # noqa: E402
+
+
+def available() -> bool:
+    """The reference harness exists (built here from the reference sources)."""
+    return os.path.exists(ora.REF_HARNESS)
+
+
+def measure(ni: int = 200, nj: int = 100, steps: int = 20, threads: int = 1) -> dict:
+    """Time both sweeps on the same grid, state, steps and threads; cell-updates/s."""
+    import make_golden as mg
+    if not available():
+        raise RuntimeError("the reference harness is not built (needs the reference sources)")
+    prm = reference_params()
+    g = mg.harness_grid(ni, nj, n_c1=max(ni // 5, 1))
+    prim0 = mg.initial_state(g, np.random.default_rng(1), prm)
+    nc = g.ncells
+
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "sweep.in"), os.path.join(tmp, "sweep.out")
+        with open(fin, "wb") as f:
+            f.write(mg.sweep_blob(g, steps, prim0))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([ora.REF_HARNESS, "sweep", fin, fout], check=True, cwd=tmp, env=env,
+                           stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+        t_ref = float(re.search(r"ref_sweep_seconds ([0-9.eE+-]+)", r.stderr).group(1))
+        with open(fout, "rb") as f:
+            ref_prim = mg.parse_sweep(f.read(), g)[0]
+
+    sw = ora.OracleSweep.from_polar(g, prm)
+    sw.step(prim0, nthreads=threads)               # warm-up (threads, pages)
+    sw = ora.OracleSweep.from_polar(g, prm)
+    t0 = time.perf_counter()
+    out = sw.run(prim0, steps, nthreads=threads)
+    t_ora = time.perf_counter() - t0
+    diff = float(np.max(np.abs(out - ref_prim) / np.maximum(np.abs(ref_prim), 1e-300)))
+    return {"grid": [ni, nj], "steps": steps, "threads": threads, "reference": nc * steps / t_ref,
+            "port": nc * steps / t_ora, "max_rel_diff": diff}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ni", type=int, default=200)
+    ap.add_argument("--nj", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    args = ap.parse_args()
+    ora.build()
+    m = measure(args.ni, args.nj, args.steps, args.threads)
+    print(f"grid {args.ni}x{args.nj}, {args.steps} steps, {args.threads} threads")
+    print(f"reference  {m['reference']:.4e} cell-updates/s")
+    print(f"oracle     {m['port']:.4e} cell-updates/s")
+    print(f"oracle / reference speed: {m['port'] / m['reference']:.2f}; "
+          f"max relative difference of the states {m['max_rel_diff']:.2e}")
+
+
+if __name__ == "__main__":
+    main()

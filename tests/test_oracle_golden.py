@@ -100,3 +100,16 @@ def test_whole_sweep_bitexact(oracle_lib, name):
     else:
         res = sw.run(d["prim0"], steps)
     assert np.array_equal(res, d["prim"])
+
+
+def test_port_matches_compiled_reference_sweep():
+    """Where the reference harness is built (the build container): on a 40 x 20 polar
+    grid with C_1 wind rows the oracle's sweep reproduces the reference's own
+    Go_5_komponent__MK2 bit for bit over 5 steps (one thread: the reference's shared
+    time step races with more)."""
+    from oracle import time_reference as tr
+    if not tr.available():
+        pytest.skip("reference harness not built (the reference sources are absent)")
+    m = tr.measure(40, 20, 5, threads=1)
+    assert m["max_rel_diff"] == 0.0
+    assert m["reference"] > 0 and m["port"] > 0
