@@ -284,7 +284,7 @@ def main():
 
     from gelios_2d_amd import configs
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_PREC_FP64
-    from gelios_2d_amd.state import l1_rel
+    from gelios_2d_amd.state import l1_rel, linf_rel
 
     wl = args.workload
     W = args.warmup
@@ -300,13 +300,19 @@ def main():
         ref = run_ensemble_reference(specs, K + warm)
         num = {0: 0.0, 1: 0.0}
         den = {0: 0.0, 1: 0.0}
+        emax = {0: 0.0, 1: 0.0}
+        vmax = {0: 0.0, 1: 0.0}
         for a, b in zip(finals, ref):
             for k in (0, 1):
                 num[k] += float(np.abs(a[k] - b[k]).sum())
                 den[k] += float(np.abs(b[k]).sum())
+                emax[k] = max(emax[k], float(np.abs(a[k] - b[k]).max()))
+                vmax[k] = max(vmax[k], float(np.abs(b[k]).max()))
         l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
+        linf = {"density": emax[0] / vmax[0], "pressure": emax[1] / vmax[1]}
         log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
             f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}; "
+            f"Linf {linf}; "
             f"judge fallback fraction {r5['fallback_fraction']:.3f}")
         ra = run_ensemble_analytic(specs, K, warm)
         log(f"bench[ensemble]: same cases, FP32 bulk with the analytic closure only: {ra['value']:.4e} cell-updates/s")
@@ -324,6 +330,7 @@ def main():
             "speedup_vs_fp64_plain": r5["value"] / r64["value"],
             "fp32_analytic_source_only": dict(ra, note="same cases, FP32 bulk, GEL_SRC_MK1 every step, no network"),
             "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + warm),
+            "linf_vs_fp64": dict(linf, definition="max |err| / max |FP64 value| over all cells of all cases"),
             "batch": {"cases": r5["cases"], "cells": r5["cells"], "autotune": r5["autotune"],
                       "source_refresh_every": args.refresh_every, "judge_tol": 1e-3,
                       "judge_fallback_cells_initial": r5["fallback_cells_initial"],
@@ -349,6 +356,9 @@ def main():
             "fp64_plain": {"value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / K,
                            "roofline": r64["roofline"]},
             "speedup_fp32_vs_fp64": r32["value"] / r64["value"],
+            "linf_vs_fp64": {"density": linf_rel(r32["final"], r64["final"], 0),
+                             "pressure": linf_rel(r32["final"], r64["final"], 1),
+                             "definition": "max |err| / max |FP64 value| over all cells"},
             "l1_vs_fp64": {"density": l1_rel(r32["final"], r64["final"], 0),
                            "pressure": l1_rel(r32["final"], r64["final"], 1),
                            "after_steps": W + K},

@@ -33,3 +33,10 @@ def l1_rel(a, b, k: int) -> float:
     a = np.asarray(a)[k]
     b = np.asarray(b)[k]
     return float(np.sum(np.abs(a - b)) / max(np.sum(np.abs(b)), 1e-300))
+
+
+def linf_rel(a, b, k: int) -> float:
+    """Largest error of variable k relative to the largest magnitude of b's variable k."""
+    a = np.asarray(a)[k]
+    b = np.asarray(b)[k]
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
