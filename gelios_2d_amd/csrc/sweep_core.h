@@ -430,13 +430,25 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
     const T bmmm = dv(sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL), sLL + sRR);
     const T eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
     const T eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+    // Regions of the fan at xi = W.  Both upwind regions are tested before the star
+    // regions, then the side of the contact: the Alfven speeds come from the HLL normal
+    // field bn, which the two sides' fast speeds need not bound (SLL < SL or SRR > SR),
+    // and the reference's order (L, L*, L**, R**, R*, R) would then sample a star region
+    // for (L, R, n, W) and the upwind state for the mirrored (R, L, -n, -W).  Here every
+    // branch is mirror-symmetric, so a face solved once gives what a cell loop solving
+    // it from both sides gives, to rounding; on monotone fans the region is the
+    // reference's (Solvers.cpp:2106-2156).
     T PO[9];
     T bnW;
     if (SL >= W) {
 #pragma unroll
         for (int i = 0; i < 9; i++) PO[i] = FL[i] - W * UL[i];
         bnW = UL[5];
-    } else if (SLL >= W) {
+    } else if (SR <= W) {
+#pragma unroll
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] - W * UR[i];
+        bnW = UR[5];
+    } else if (SM >= W && SLL >= W) {
         const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
 #pragma unroll
         for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
@@ -447,21 +459,17 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
 #pragma unroll
         for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
         bnW = bn;
-    } else if (SRR > W) {
-        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
-        const T URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
-#pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
-        bnW = bn;
-    } else if (SR > W) {
+    } else if (SRR <= W) {
         const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
 #pragma unroll
         for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
         bnW = bn;
     } else {
+        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+        const T URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
 #pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FR[i] - W * UR[i];
-        bnW = UR[5];
+        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
+        bnW = bn;
     }
     const T SN = mx(fabs(SL), fabs(SR));
     PO[5] = -SN * (bn2 - bn1) - W * bnW;

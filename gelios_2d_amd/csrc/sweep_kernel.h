@@ -1436,9 +1436,13 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 time = mn(time, F.ftm[fb][lane]);
                 const size_t kc = (size_t)i * nj + j;
                 const GPtr<TS> nxt = gp((TS*)a.next);
+                // the cell's type for the update rules, read here from the grid's type
+                // array: the update depends on no value carried over from the faces phase
+                // but the flux sums and the cell's own state
+                const int ctu = TYPES ? (int)gp(a.ctype)[kc] : (int)GEL_C_2;
                 if constexpr (INNER) {
                     // C_centr cells are skipped: no time step, no update (Setka.cpp:7025-7028)
-                    const int ct = TYPES ? M.type : (int)GEL_C_2;
+                    const int ct = ctu;
                     double o[5];
                     if (ct == GEL_C_CENTR) {
 #pragma unroll
@@ -1461,7 +1465,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 } else if constexpr (MF) {
                     // C_centr cells solve no H1 faces (Setka.cpp:9172) and keep the plasma
                     // and H1 (:9525-9531, :9645-9651); H2..H4 update everywhere
-                    const int ct = TYPES ? M.type : (int)GEL_C_2;
+                    const int ct = ctu;
                     if (!(a.fluid == 1 && ct == GEL_C_CENTR)) acc.tmin = fmin(acc.tmin, time);
                     const int nw = a.fluid == 0 ? 5 : 4;
                     double o[5];
@@ -1490,7 +1494,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                     for (int k = 0; k < nw; k++) nxt[(size_t)a.voff[k] * nc + kc] = TS(o[k]);
                 } else {
                 acc.tmin = fmin(acc.tmin, time);
-                if (TYPES && M.type == GEL_C_CENTR) {
+                if (TYPES && ctu == GEL_C_CENTR) {
 #pragma unroll
                     for (int k = 0; k < NV; k++) nxt[(size_t)k * nc + kc] = M.s[k];
                 } else {

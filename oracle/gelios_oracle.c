@@ -707,20 +707,33 @@ double ora_hlld(const ora_mhd* L, const ora_mhd* R, double W, double n1, double 
 
     double PO[9];
     double bnW = 0.0;   /* normal field of the sampled region, for the -W*U term */
-    /* regions of the fan at xi = W, :2106-2156 */
-    if (SL >= W) {
+    /* regions of the fan at xi = W (:2106-2156).  The reference tests L, L*, L**, R**,
+     * R*, R in that order.  Where the fan is not monotone (the Alfven speeds use the
+     * HLL normal field bn, which the fast speeds of the two sides need not bound, so
+     * that SLL < SL or SRR > SR), that order takes a star region on one side of the
+     * face and the upwind state on the other: the mirrored problem F(R, L, -n, -W),
+     * which a cell loop solving every face from both sides evaluates, then differs from
+     * -F(L, R, n, W) at O(1).  Both upwind regions are tested first here, then the side
+     * of the contact, so that every branch is mirror-symmetric; where the fan is
+     * monotone the region is the reference's. */
+    int reg;
+    if (SL >= W) reg = 0;
+    else if (SR <= W) reg = 5;
+    else if (SM >= W) reg = (SLL >= W) ? 1 : 2;
+    else reg = (SRR <= W) ? 4 : 3;
+    if (reg == 0) {
         for (int i = 0; i < 9; i++) PO[i] = FL[i] - W * UL[i];
         bnW = UL[5];
-    } else if (SL < W && SLL >= W) {
+    } else if (reg == 1) {
         for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
         bnW = bn;
-    } else if (SLL <= W && SM >= W) {
+    } else if (reg == 2) {
         for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
         bnW = bn;
-    } else if (SM < W && SRR > W) {
+    } else if (reg == 3) {
         for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
         bnW = bn;
-    } else if (SR > W && SRR <= W) {
+    } else if (reg == 4) {
         for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
         bnW = bn;
     } else {
@@ -826,6 +839,76 @@ static ora_mhd as_mhd(const double* q)
     return s;
 }
 
+/* Face-once summation (ora_set_face_once): each interior face is solved once, by the
+ * cell of lower index, and the other cell takes the negated flux x area product, as
+ * the device sweep sums them.  The reference solves every face from both sides
+ * (Setka.cpp:10195); this mode exists to measure what that difference does. */
+static int g_face_once = 0;
+void ora_set_face_once(int on) { g_face_once = on != 0; }
+
+/* One face slot of cell c as the cell loop of the MHD sweep evaluates it: the flux x
+ * area products P[9]*S and the max wave speed. */
+static void mhd_face(const msw* S, int c, int s, double dt_prev, double* PS, double* UUo, int* bad, double* LRW)
+{
+    const ora_mesh* m = S->m;
+    const double gam = S->prm->gamma;
+    const double x = S->cx[c], y = S->cy[c];
+    const double radius = sqrt(sq(x) + sq(y));
+    const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
+    const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+    const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
+    const double fx2 = (m->x2[A] + m->x2[B]) / 2.0, fy2 = (m->y2[A] + m->y2[B]) / 2.0;
+    double n1, n2;
+    {
+        double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
+        const double nn = sqrt(sq(t1) + sq(t2));
+        t1 = t1 / nn; t2 = t2 / nn;
+        n1 = t2; n2 = -t1;
+    }
+    const double W = ((fx2 - fx) * n1 + (fy2 - fy) * n2) / dt_prev;
+    const double dis = sqrt(sq(fx) + sq(fy));
+    double p11[MV], p2[MV];
+    mload(S, c, p11);
+    mget_par(S, c, s, fx, fy, p2);
+    if (t == GEL_F_USUALY) {
+        const int N = m->fnbr[s];
+        mtvd_usualy(S, c, N, m->fdown[s], fx, fy, p11, bad);
+        mtvd_usualy(S, N, c, m->fup[s], fx, fy, p2, bad);
+    } else if (t == GEL_F_INNER_SPHERE) {
+        p11[0] = p11[0] * sq(radius) / sq(dis);
+        p11[8] = p11[8] * sq(radius) / sq(dis);
+        p11[1] = p11[1] * pow(radius / dis, 2 * gam);
+    } else if (t == GEL_F_AXIS) {
+        mtvd_axis(S, c, m->fdown[s], fx, fy, p11);
+        for (int v = 0; v < MV; v++) p2[v] = mirror9[v] ? -p11[v] : p11[v];
+    }
+    const ora_mhd Lm = as_mhd(p11), Rm = as_mhd(p2);
+    if (LRW) {
+        for (int v = 0; v < MV; v++) { LRW[v] = p11[v]; LRW[MV + v] = p2[v]; }
+        LRW[2 * MV] = W; LRW[2 * MV + 1] = n1; LRW[2 * MV + 2] = n2; LRW[2 * MV + 3] = S_;
+    }
+    double P[MV];
+    *UUo = ora_hlld(&Lm, &Rm, W, n1, n2, gam, P);
+    for (int v = 0; v < MV; v++) PS[v] = P[v] * S_;
+}
+
+int ora_mhd_face(const ora_mesh* m, const gel_params* prm, const double* now, int c, int k, double dt_prev,
+                 double PS[9], double* UU, double* LRW)
+{
+    const int nc = m->n_cells;
+    if (c < 0 || c >= nc || k < 0 || k >= m->nface[c]) return GEL_E_ARG;
+    double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
+    double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
+    if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
+    for (int i = 0; i < nc; i++) ora_cell_center(m->x, m->y, m->contour + 4 * i, &cx[i], &cy[i]);
+    msw S = { m, prm, now, nc, cx, cy };
+    int bad = 0;
+    mhd_face(&S, c, c * ORA_MAXF + k, dt_prev, PS, UU, &bad, LRW);
+    free(cx);
+    free(cy);
+    return bad ? GEL_E_NUMERIC : GEL_OK;
+}
+
 int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, double* next,
                  const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
                  long long* n_p_clamp, int nthreads)
@@ -833,6 +916,7 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
     const int nc = m->n_cells;
     const double gam = prm->gamma;
     const double pi4 = 4.0 * PI_, pi8 = 8.0 * PI_;
+    const int once = g_face_once;
     for (int c = 0; c < nc; c++)
         if (m->cell_type[c] == GEL_C_1) return GEL_E_UNSUPPORTED;
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
@@ -863,40 +947,24 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
         int bad = 0;
         for (int k = 0; k < m->nface[c]; k++) {
             const int s = c * ORA_MAXF + k;
-            const int A = m->fa[s], B = m->fb[s], t = m->ftype[s];
-            const double S_ = sqrt(sq(m->x[B] - m->x[A]) + sq(m->y[B] - m->y[A])) * (m->y[A] + m->y[B]) * 0.5;
+            const int A = m->fa[s], B = m->fb[s];
             const double fx = (m->x[A] + m->x[B]) / 2.0, fy = (m->y[A] + m->y[B]) / 2.0;
-            const double fx2 = (m->x2[A] + m->x2[B]) / 2.0, fy2 = (m->y2[A] + m->y2[B]) / 2.0;
             const double dist = sqrt(sq(x - fx) + sq(y - fy));
-            double n1, n2;
-            {
-                double t1 = m->x[B] - m->x[A], t2 = m->y[B] - m->y[A];
-                const double nn = sqrt(sq(t1) + sq(t2));
-                t1 = t1 / nn; t2 = t2 / nn;
-                n1 = t2; n2 = -t1;
+            double PS[MV], UU;
+            const int N = m->fnbr[s];
+            int s2 = -1;
+            if (once && m->ftype[s] == GEL_F_USUALY && N < c)
+                for (int k2 = 0; k2 < m->nface[N]; k2++)
+                    if (m->fnbr[N * ORA_MAXF + k2] == c) { s2 = N * ORA_MAXF + k2; break; }
+            if (s2 >= 0) {
+                int b2 = 0;
+                mhd_face(&S, N, s2, dt_prev, PS, &UU, &b2, NULL);
+                for (int v = 0; v < MV; v++) PS[v] = -PS[v];
+            } else {
+                mhd_face(&S, c, s, dt_prev, PS, &UU, &bad, NULL);
             }
-            const double W = ((fx2 - fx) * n1 + (fy2 - fy) * n2) / dt_prev;
-            const double dis = sqrt(sq(fx) + sq(fy));
-            double p11[MV], p2[MV];
-            for (int v = 0; v < MV; v++) p11[v] = par1[v];
-            mget_par(&S, c, s, fx, fy, p2);
-            if (t == GEL_F_USUALY) {
-                const int N = m->fnbr[s];
-                mtvd_usualy(&S, c, N, m->fdown[s], fx, fy, p11, &bad);
-                mtvd_usualy(&S, N, c, m->fup[s], fx, fy, p2, &bad);
-            } else if (t == GEL_F_INNER_SPHERE) {
-                p11[0] = p11[0] * sq(radius) / sq(dis);
-                p11[8] = p11[8] * sq(radius) / sq(dis);
-                p11[1] = p11[1] * pow(radius / dis, 2 * gam);
-            } else if (t == GEL_F_AXIS) {
-                mtvd_axis(&S, c, m->fdown[s], fx, fy, p11);
-                for (int v = 0; v < MV; v++) p2[v] = mirror9[v] ? -p11[v] : p11[v];
-            }
-            const ora_mhd Lm = as_mhd(p11), Rm = as_mhd(p2);
-            double P[MV];
-            const double UU = ora_hlld(&Lm, &Rm, W, n1, n2, gam, P);
             time = mn(time, prm->cfl * dist / UU);
-            for (int v = 0; v < MV; v++) Pot[v] = Pot[v] + P[v] * S_;
+            for (int v = 0; v < MV; v++) Pot[v] = Pot[v] + PS[v];
         }
         if (bad) bad_any = 1;
         if (time < Tmin) Tmin = time;

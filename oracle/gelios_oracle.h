@@ -100,6 +100,18 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
                  const double* src, double dt_prev, double* dt_new, long long* n_rho_clamp,
                  long long* n_p_clamp, int nthreads);
 
+/* Face-once summation for ora_step_mhd (on != 0): every interior face is solved once,
+ * from the cell of lower index, and its neighbour takes the negated flux x area
+ * product (the device sweep's summation); 0 (default): every face from both sides, as
+ * the reference's cell loop does (Setka.cpp:10195). */
+void ora_set_face_once(int on);
+
+/* Face slot k of cell c as ora_step_mhd evaluates it from that cell: the flux x area
+ * products PS[9] and the largest wave speed; LRW (22 values, or NULL) receives the two
+ * face states (9 + 9 primitives), the face speed W, the normal (n1, n2) and the area S. */
+int ora_mhd_face(const ora_mesh* m, const gel_params* prm, const double* now, int c, int k, double dt_prev,
+                 double PS[9], double* UU, double* LRW);
+
 /* The static inner-region solver Setka::Go_stationary_5_komponent_inner_MK2
  * (Setka.cpp:6999-7212): 5 x n_cells gas primitives; src as for ora_step_mk2. */
 int ora_step_inner(const ora_mesh* m, const gel_params* prm, const double* now, double* next, const double* src,

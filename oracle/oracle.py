@@ -91,6 +91,11 @@ def lib():
         L.ora_track_nodes.restype = ctypes.c_int
         L.ora_track_nodes.argtypes = [ctypes.POINTER(GelTracking), ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
+        L.ora_mhd_face.restype = ctypes.c_int
+        L.ora_mhd_face.argtypes = [ctypes.POINTER(OraMesh), ctypes.POINTER(GelParams), _dp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, _dp, _dp, _dp]
+        L.ora_set_face_once.restype = None
+        L.ora_set_face_once.argtypes = [ctypes.c_int]
         L.ora_hlld.restype = ctypes.c_double
         L.ora_hlld.argtypes = [ctypes.POINTER(OraMhd), ctypes.POINTER(OraMhd), ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, _dp]
@@ -200,6 +205,20 @@ class OracleSweep:
             self.dt = dt_new.value
         self.last_dt_new = dt_new.value
         return nxt, st
+
+    def mhd_face(self, prim, c, k, dt_prev=None):
+        """Face slot k of cell c as the MHD cell loop evaluates it from that cell:
+        (flux x area products[9], largest wave speed, (L[9], R[9], W, n1, n2, S))."""
+        now = np.ascontiguousarray(prim, np.float64).reshape(self.nv, self.nc)
+        PS = np.zeros(9)
+        lrw = np.zeros(22)
+        uu = ctypes.c_double()
+        m = self._mesh()
+        st = lib().ora_mhd_face(ctypes.byref(m), ctypes.byref(self.prm), _d(now), int(c), int(k),
+                                self.dt if dt_prev is None else dt_prev, _d(PS), ctypes.byref(uu), _d(lrw))
+        if st not in (0, 6):
+            raise RuntimeError(f"ora_mhd_face failed with status {st}")
+        return PS, uu.value, (lrw[:9], lrw[9:18], lrw[18], lrw[19], lrw[20], lrw[21])
 
     def run_motion(self, prim, steps, x0, y0, amp, period, phase, nthreads=0):
         """Steps on the prescribed breathing grid: before step n the nodes are at

@@ -33,9 +33,14 @@ from gelios_2d_amd.params import reference_params  # noqa: E402
 from oracle import oracle as ora  # noqa: E402
 
 
+REF_SOURCE = "/root/reference/Setka.cpp"
+
+
 def available() -> bool:
-    """The reference harness exists (built here from the reference sources)."""
-    return os.path.exists(ora.REF_HARNESS)
+    """The reference sources are present (the build container) and the harness is
+    built from them.  Elsewhere (a GPU host) the reference is never run, even where a
+    stale harness binary exists."""
+    return os.path.exists(REF_SOURCE) and os.path.exists(ora.REF_HARNESS)
 
 
 def measure(ni: int = 200, nj: int = 100, steps: int = 20, threads: int = 1) -> dict:

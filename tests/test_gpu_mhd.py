@@ -33,6 +33,29 @@ def test_hlld_batch_matches_oracle(oracle_lib):
     assert worst <= 1e-14
 
 
+def test_hlld_batch_nonmonotone_fans(oracle_lib):
+    """Faces whose Alfven speeds leave the fan (the cold wind of the moving 800 x 400
+    workload): the device flux equals the oracle's and is mirror-symmetric,
+    F(L, R, n, W) = -F(R, L, -n, -W)."""
+    from gelios_2d_amd.setka import HLLD_flux
+    from test_oracle_mhd import nonmonotone_fans
+    faces = nonmonotone_fans(600)
+    L = np.array([f[0] for f in faces])
+    R = np.array([f[1] for f in faces])
+    W, n1, n2 = (np.array([f[k] for f in faces]) for k in (2, 3, 4))
+    P, smax = HLLD_flux(L, R, W, n1, n2)
+    Pm, smaxm = HLLD_flux(R, L, -W, -n1, -n2)
+    worst = mirror = 0.0
+    for k in range(len(faces)):
+        Po, uo = oracle_lib.hlld(L[k], R[k], W[k], n1[k], n2[k])
+        scale = np.abs(Po).max()
+        worst = max(worst, np.abs(P[k] - Po).max() / scale)
+        mirror = max(mirror, np.abs(P[k] + Pm[k]).max() / scale)
+        assert smax[k] == uo == smaxm[k]
+    assert worst <= 1e-14
+    assert mirror <= 1e-13
+
+
 @pytest.mark.parametrize("shape", [(40, 30, 40), (9, 131, 25)])
 def test_mhd_moving_sweep_matches_oracle(oracle_lib, shape):
     from gelios_2d_amd.setka import Setka
@@ -175,3 +198,60 @@ def test_hlld_800x400_full_size(oracle_lib):
         s.close()
     assert l1_rel(outs[1], outs[0], 0) <= 1e-4
     assert l1_rel(outs[1], outs[0], 1) <= 1e-4
+
+
+# Gap between the oracle and itself with the initial density moved up by one ulp, on
+# the 800 x 400 moving HLLD workload (max over the conserved variables of max|diff| /
+# max|value|): the growth of a rounding difference that this workload's cold Mach-77
+# wind amplifies, measured with the oracle.  Step: gap.
+TWIN_GAP_800x400 = {100: 1.56e-11, 500: 8.97e-10, 1000: 3.93e-6}
+
+
+def test_hlld_800x400_1000_steps_against_oracle(oracle_lib):
+    """The MHD workload over its full 1000 steps.  The FP64 sweep solves every face once;
+    the oracle solves every face from both sides, as the reference's cell loop does
+    (Setka.cpp:10195), and in its face-once mode sums as the sweep does.  The sweep
+    matches the face-once oracle to 1e-12 at every checkpoint; both stay within three
+    times the gap a one-ulp perturbation opens in the two-sided oracle; all three clamp
+    the same number of pressures."""
+    from gelios_2d_amd.configs import hlld_mhd_800x400
+    from gelios_2d_amd.configs import motion_scales
+    from gelios_2d_amd.setka import Setka
+    w = hlld_mhd_800x400()
+    amp, period, phase = w.motion
+    L = oracle_lib.lib()
+    two = oracle_lib.OracleSweep.from_polar(w.grid, w.params)
+    once = oracle_lib.OracleSweep.from_polar(w.grid, w.params)
+    sc = motion_scales(amp, period, phase)
+    s = Setka(w.grid, w.params)
+    s.set_motion(amp, period, phase)
+    s.upload(w.prim0)
+    c2 = c1 = np.ascontiguousarray(w.prim0, np.float64).reshape(9, -1)
+    done = 0
+    try:
+        for n, gap in sorted(TWIN_GAP_800x400.items()):
+            for k in range(done, n):
+                so, sn = sc[k % period], sc[(k + 1) % period]
+                for sw in (two, once):
+                    sw.set_nodes(w.grid.x * so, w.grid.y * so, w.grid.x * sn, w.grid.y * sn)
+                L.ora_set_face_once(0)
+                c2, st2 = two.step(c2)
+                L.ora_set_face_once(1)
+                c1, st1 = once.step(c1)
+                assert st1 == st2 == 0
+            st = s.Go_5_komponent__MK2(n - done)
+            done = n
+            got = s.download()
+            e_once, e_two = rel_err_conserved(got, c1), rel_err_conserved(got, c2)
+            print(f"step {n}: GPU vs face-once oracle {e_once:.3e}, vs two-sided oracle {e_two:.3e} (twin {gap:.2e}); "
+                  f"p clamps GPU {st.n_p_clamp} oracle {two.n_p_clamp.value} / {once.n_p_clamp.value}; "
+                  f"dt {st.dt_next:.9e} / {two.dt:.9e}", flush=True)
+            assert e_once <= TOL
+            assert e_two <= 3.0 * gap
+            assert st.n_p_clamp == two.n_p_clamp.value == once.n_p_clamp.value
+            assert st.n_rho_clamp == two.n_rho_clamp.value == once.n_rho_clamp.value
+            assert st.n_numeric == 0
+    finally:
+        L.ora_set_face_once(0)
+    assert two.n_p_clamp.value > 0
+    s.close()
