@@ -10,10 +10,16 @@ Workloads
   ensemble (default)  the metric's comparison: 64 heliosphere cases (100x50 .. 800x400
                    cells, seeded inflow / field / hydrogen), FP32-bulk HLLD with the
                    learned charge-exchange source, one persistent launch per step,
-                   HIP-graph replay, launch shape from the autotuner, 500 steps (value);
-                   beside it the FP64 plain sweep of hlld-800x400 and the density / pressure L1
-                   error of the batch against an FP64 run of the same cases with the
-                   analytic sources
+                   HIP-graph replay, launch shape from the autotuner (value); beside it
+                   the FP64 plain sweep of the same cases (analytic source every step,
+                   batched, graph-replayed: the ratio's denominator), the same cases in
+                   FP32 without the network, the density / pressure L1 and Linf errors
+                   after the workload's 500 steps against the FP64 run, and the FP64
+                   plain sweep of hlld-800x400
+  learned-800x400  the learned-source form of hlld-800x400: four frozen hydrogen
+                   populations, the network's charge-exchange source refreshed every 10
+                   steps (hybrid), FP64 state, 1000 steps, beside FP64 with the analytic
+                   closure every step
   hlld-800x400     HLLD MHD, 9 variables, 800 x 400 cells, moving grid, 1000 steps:
                    the FP32-bulk sweep (value) beside the FP64 plain sweep
   hllc-200x100     single-fluid HLLC, 200 x 100 cells, 100 steps, FP64
@@ -74,7 +80,21 @@ def bytes_per_cell_update(w, fallback_fraction=0.0):
     return 2 * nv * s + geom + src
 
 
+def dict_params(w, precision, source_mode):
+    """A stand-in of workload w with another precision and source mode (for the byte
+    count)."""
+    import copy
+    v = copy.copy(w)
+    v.params = w.params.copy()
+    v.params.precision = precision
+    v.params.source_mode = source_mode
+    return v
+
+
 def run_gpu(w, K, W, label):
+    """W warm-up steps, then K steps timed by the host clock around the call and by HIP
+    events on the context's stream around the same call (timing mode 2: the per-step
+    device time of back-to-back launches)."""
     from gelios_2d_amd.setka import Setka
     nc = w.grid.ncells
     s = Setka(w.grid, w.params)
@@ -87,138 +107,285 @@ def run_gpu(w, K, W, label):
         s.set_tracking(w.tracking, w.rails)
     s.Go_5_komponent__MK2(W)
     s.sync()
+    s.enable_timing(2)
     t0 = time.perf_counter()
     s.Go_5_komponent__MK2(K)
     st = s.sync()
     wall = time.perf_counter() - t0
+    avg_ms, n = s.kernel_timing()
+    s.enable_timing(0)
     value = nc * K / wall
     log(f"bench[{label}]: {K} steps in {wall:.4f} s -> {value:.4e} cell-updates/s, dt_next {st.dt_next:.3e}, "
         f"clamps rho {st.n_rho_clamp} p {st.n_p_clamp}")
     final = s.download()
-    # per-launch device time of the sweep kernel (HIP events on the context's stream)
-    s.enable_timing(True)
-    s.Go_5_komponent__MK2(min(K, 200))
-    avg_ms, n = s.kernel_timing()
-    s.enable_timing(False)
     B = bytes_per_cell_update(w)
     achieved = B * nc / (avg_ms * 1e-3) / 1e9
-    log(f"bench[{label}]: sweep kernel avg {avg_ms * 1e3:.2f} us over {n} launches, "
+    log(f"bench[{label}]: device time per step {avg_ms * 1e3:.2f} us over the {n} timed steps, "
         f"{achieved:.1f} GB/s algorithmic ({B} B/cell-update)")
     s.close()
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
-            "kernel_avg_us": avg_ms * 1e3}
+            "kernel_avg_us": avg_ms * 1e3,
+            "kernel_time": "HIP events around the timed call on the context's stream / steps"}
     return {"value": value, "wall_s": wall, "final": final, "roofline": roof, "steps_done": K + W,
-            "dt_next": st.dt_next}
+            "dt_next": st.dt_next, "n_p_clamp": st.n_p_clamp, "n_rho_clamp": st.n_rho_clamp}
 
 
-def run_ensemble(steps, refresh_every, log_prefix="ensemble"):
-    """Build the 64 cases, fill their source tables from the network (one fused launch
-    for all cases, every cell judged), autotune, then time `steps` steps: HIP-graph
-    replays in which the sources of every case are refreshed from the network, with
-    the analytic judge, after every `refresh_every` steps; cells the judge has sent to
-    the analytic closure keep it (judge 2), so that a refresh evaluates the network on
-    the others only.  Returns (result dict, final states, specs, steps run before the
-    timed region)."""
-    from gelios_2d_amd.batch import Batch
-    from gelios_2d_amd.configs import ensemble_cases
+def run_learned(w, K, W, every, tol=1e-3):
+    """A single context with the learned source (hybrid): the network refreshes the
+    sources after every `every`-th step (sticky judge), host-driven between the sweep
+    calls; W warm-up steps, then K steps timed (host clock; HIP events around each
+    sweep call, refreshes excluded, for the kernel's per-step time)."""
+    import numpy as np
     from gelios_2d_amd.setka import Setka
     from gelios_2d_amd.surrogate import SourceModel
     from gelios_2d_amd.train_surrogate import DEFAULT_OUT
-    import math
-    import numpy as np
     model = SourceModel.load(DEFAULT_OUT)
-    specs = ensemble_cases(64, seed=5, steps=steps)
-    cases = []
-    for w, f in specs:
-        s = Setka(w.grid, w.params)
-        s.upload(w.prim0)
-        s.set_sources(f)
-        s.set_source_table(np.zeros((5, s.ncells)))
-        cases.append(s)
-    ncells = sum(s.ncells for s in cases)
-    b = Batch(cases)
-    nfall = b.refresh(model, tol=1e-3, judge=True)
-    rows, wg, ms_trial = b.autotune(12)
-    b.set_source_model(model, refresh_every, tol=1e-3, judge=2)
-    cyc = 6 * refresh_every // math.gcd(6, refresh_every)
-    log(f"bench[{log_prefix}]: {len(cases)} cases, {ncells} cells; autotuner rows {rows} workgroups {wg} "
-        f"({ms_trial * 1e3:.1f} us/step sweep only); initial judge fallback {nfall}/{ncells} cells; "
-        f"source refresh every {refresh_every} steps, graph cycle {cyc} steps")
-    b.step(cyc, True)                             # warm-up (graph capture)
-    b.source_stats()
+    s = Setka(w.grid, w.params)
+    s.set_motion(*w.motion)
+    s.upload(w.prim0)
+    s.set_sources(w.src)
+    s.set_source_table(np.zeros((5, s.ncells)))
+    nfall0 = model.refresh(s, tol, judge=True)
+    model.advance(s, W, every, tol)
+    s.sync()
+    s.enable_timing(2)
     t0 = time.perf_counter()
-    dev_ms = b.step(steps, True)
+    nfall = model.advance(s, K, every, tol)
+    st = s.sync()
     wall = time.perf_counter() - t0
-    nref, nfb = b.source_stats()
-    finals = [s.download() for s in cases]
-    # device time of the sweep kernel alone: HIP events on the batch stream around
-    # single-step launches (a graph replay would add its launch gaps)
-    b.set_source_model(None, 0)
-    sweep_ms = sum(b.step(1, False) for _ in range(60)) / 60
-    # device time of the refresh alone, as the timed steps run it: one fused launch
-    # over all cases
-    t1 = time.perf_counter()
-    reps = 10
-    for _ in range(reps):
-        b.refresh(model, tol=1e-3, judge=2)
-    refresh_ms = (time.perf_counter() - t1) * 1e3 / reps
-    return ({"value": ncells * steps / wall, "wall_s": wall, "cells": ncells, "cases": len(cases),
-             "autotune": {"rows": rows, "workgroups": wg, "sweep_us_per_step": ms_trial * 1e3},
-             "fallback_cells_initial": nfall, "refreshes": nref,
-             "fallback_fraction": nfb / max(1, nref * ncells), "refresh_ms_host_timed": refresh_ms,
-             "kernel_ms_per_step": dev_ms / steps, "sweep_ms_per_step": sweep_ms}, finals, specs, cyc)
+    avg_ms, n = s.kernel_timing()
+    s.enable_timing(0)
+    final = s.download()
+    s.close()
+    B = bytes_per_cell_update(w, (nfall or 0) / w.grid.ncells)
+    achieved = B * w.grid.ncells / (avg_ms * 1e-3) / 1e9
+    log(f"bench[learned]: {K} steps in {wall:.4f} s -> {w.grid.ncells * K / wall:.4e} cell-updates/s; sweep "
+        f"{avg_ms * 1e3:.2f} us/step; flagged {nfall0} -> {nfall} of {w.grid.ncells}")
+    return {"value": w.grid.ncells * K / wall, "wall_s": wall, "final": final, "n_p_clamp": st.n_p_clamp,
+            "fallback_initial": nfall0, "fallback_last": nfall,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None, "bytes_per_cell_update": B,
+                         "kernel_avg_us": avg_ms * 1e3,
+                         "kernel_time": "HIP events around the sweep calls of the timed region / steps"}}
 
 
-def run_ensemble_analytic(specs, steps, cycle):
-    """The same cases in FP32 bulk with the analytic closure at every step and no
-    network (GEL_SRC_MK1), batched and graph-replayed like the timed run: what the
-    learned source is measured against."""
-    from gelios_2d_amd.batch import Batch
-    from gelios_2d_amd.params import GEL_SRC_MK1
+def batch_cases(specs, precision=None, source_mode=None):
+    """One context per ensemble case (uploaded state, hydrogen moments, an empty source
+    table where the mode reads one)."""
+    import numpy as np
+    from gelios_2d_amd.params import GEL_SRC_HYBRID, GEL_SRC_TABLE
     from gelios_2d_amd.setka import Setka
     cases = []
     for w, f in specs:
         prm = w.params.copy()
-        prm.source_mode = GEL_SRC_MK1
+        if precision is not None:
+            prm.precision = precision
+        if source_mode is not None:
+            prm.source_mode = source_mode
         s = Setka(w.grid, prm)
         s.upload(w.prim0)
         s.set_sources(f)
+        if prm.source_mode in (GEL_SRC_TABLE, GEL_SRC_HYBRID):
+            s.set_source_table(np.zeros((5, s.ncells)))
         cases.append(s)
-    b = Batch(cases)
-    b.autotune(12)
-    b.step(cycle, True)
+    return cases
+
+
+def timed_batch(b, K, W):
+    """W untimed warm-up steps, the graphs of a K-step call captured (nothing run),
+    then exactly K steps timed: host clock around the call, HIP events on the batch
+    stream inside it.  Returns (wall seconds, device milliseconds)."""
+    if W:
+        b.step(W, True)
+    b.source_stats()
+    b.prepare(K)
     t0 = time.perf_counter()
-    b.step(steps, True)
-    wall = time.perf_counter() - t0
-    return {"value": sum(s.ncells for s in cases) * steps / wall, "ms_per_step": wall * 1e3 / steps}
+    dev_ms = b.step(K, True)
+    return time.perf_counter() - t0, dev_ms
 
 
-def run_ensemble_reference(specs, steps):
-    """The same cases in FP64 with the analytic sources every step (batched)."""
+def sweep_us_per_step(b, n=60):
+    """Device time per step of the batched sweep kernel alone: HIP events around one
+    call of n graph-replayed steps with no source refresh (kernel back to back)."""
+    b.set_source_model(None, 0)
+    b.prepare(n)
+    return b.step(n, True) * 1e3 / n
+
+
+def run_ensemble(specs, K, W, refresh_every, tol=1e-3):
+    """FP32 bulk + the learned source (hybrid: the network's table, the analytic closure
+    in the cells its judge rejected), batched, graph-replayed; the network refreshes
+    every case's sources after every `refresh_every` steps inside the graphs."""
+    from gelios_2d_amd.batch import Batch
+    from gelios_2d_amd.surrogate import SourceModel
+    from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    model = SourceModel.load(DEFAULT_OUT)
+    cases = batch_cases(specs)
+    ncells = sum(s.ncells for s in cases)
+    b = Batch(cases)
+    nfall = b.refresh(model, tol=tol, judge=True)
+    rows, wg, ms_trial = b.autotune(12)
+    b.set_source_model(model, refresh_every, tol=tol, judge=2)
+    wall, dev_ms = timed_batch(b, K, W)
+    nref, nfb = b.source_stats()
+    log(f"bench[ensemble]: {len(cases)} cases, {ncells} cells; autotuner rows {rows} workgroups {wg}; "
+        f"initial judge fallback {nfall}/{ncells}; {K} steps in {wall:.4f} s ({nref} refreshes)")
+    sweep_us = sweep_us_per_step(b)
+    t1 = time.perf_counter()
+    for _ in range(5):
+        b.refresh(model, tol=tol, judge=2)
+    refresh_ms = (time.perf_counter() - t1) * 1e3 / 5
+    for s in cases:
+        s.close()
+    return {"value": ncells * K / wall, "wall_s": wall, "cells": ncells, "cases": len(cases),
+            "autotune": {"rows": rows, "workgroups": wg, "sweep_us_per_step": ms_trial * 1e3},
+            "fallback_cells_initial": nfall, "refreshes": nref,
+            "fallback_fraction": nfb / max(1, nref * ncells), "refresh_ms_host_timed": refresh_ms,
+            "device_ms_per_step": dev_ms / K, "sweep_us_per_step": sweep_us}
+
+
+def run_ensemble_plain(specs, K, W, precision, source_mode, label):
+    """The same cases without the network: `source_mode` (the analytic closure every
+    step) in `precision`, batched, autotuned and graph-replayed like the learned run."""
+    from gelios_2d_amd.batch import Batch
+    cases = batch_cases(specs, precision, source_mode)
+    ncells = sum(s.ncells for s in cases)
+    b = Batch(cases)
+    rows, wg, _ = b.autotune(12)
+    wall, dev_ms = timed_batch(b, K, W)
+    sweep_us = sweep_us_per_step(b)
+    log(f"bench[{label}]: {K} steps in {wall:.4f} s -> {ncells * K / wall:.4e} cell-updates/s "
+        f"(rows {rows}, workgroups {wg}; sweep {sweep_us:.1f} us/step)")
+    for s in cases:
+        s.close()
+    return {"value": ncells * K / wall, "ms_per_step": wall * 1e3 / K, "sweep_us_per_step": sweep_us,
+            "cells": ncells, "autotune": {"rows": rows, "workgroups": wg}}
+
+
+def ensemble_accuracy(specs, steps, refresh_every, tol=1e-3):
+    """Density / pressure error of the learned FP32 run against FP64 with the analytic
+    sources every step, both from the initial states over `steps` steps (the workload's
+    length), untimed."""
+    import numpy as np
     from gelios_2d_amd.batch import Batch
     from gelios_2d_amd.params import GEL_PREC_FP64, GEL_SRC_MK1
-    from gelios_2d_amd.setka import Setka
-    cases = []
+    from gelios_2d_amd.surrogate import SourceModel
+    from gelios_2d_amd.train_surrogate import DEFAULT_OUT
+    model = SourceModel.load(DEFAULT_OUT)
+    a = batch_cases(specs)
+    A = Batch(a)
+    A.refresh(model, tol=tol, judge=True)
+    A.set_source_model(model, refresh_every, tol=tol, judge=2)
+    A.step(steps, True)
+    nref, nfb = A.source_stats()
+    finals = [s.download() for s in a]
+    for s in a:
+        s.close()
+    r = batch_cases(specs, GEL_PREC_FP64, GEL_SRC_MK1)
+    Bt = Batch(r)
+    Bt.step(steps, True)
+    ref = [s.download() for s in r]
+    for s in r:
+        s.close()
+    num, den, emax, vmax = [0.0, 0.0], [0.0, 0.0], [0.0, 0.0], [0.0, 0.0]
+    for x, y in zip(finals, ref):
+        for k in (0, 1):
+            num[k] += float(np.abs(x[k] - y[k]).sum())
+            den[k] += float(np.abs(y[k]).sum())
+            emax[k] = max(emax[k], float(np.abs(x[k] - y[k]).max()))
+            vmax[k] = max(vmax[k], float(np.abs(y[k]).max()))
+    ncells = sum(int(np.asarray(w.prim0).shape[1]) for w, _ in specs)
+    return ({"density": num[0] / den[0], "pressure": num[1] / den[1], "steps": steps,
+             "reference": "same 64 cases, FP64, analytic sources every step"},
+            {"density": emax[0] / vmax[0], "pressure": emax[1] / vmax[1],
+             "definition": "max |err| / max |FP64 value| over all cells of all cases", "steps": steps},
+            nfb / max(1, nref * ncells))
+
+
+def cpu_threads():
+    """Threads the host grants this job: OMP_NUM_THREADS where set (a one-GPU job's CPU
+    share), else every CPU the process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def host_cpu():
+    """lscpu's model, sockets, cores per socket and threads per core."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(specs, budget_s=12.0):
+    """The CPU restatement of the reference sweep (per-cell gather over faces, every
+    interior face solved from both sides, FP64, OpenMP over cells, as Setka.cpp:10148)
+    on the host cores, over a bounded sample of the ensemble: its cases in order, FP64
+    with the analytic charge-exchange source every step (the plain form of the
+    workload), a few steps each, until the time budget is spent."""
+    import numpy as np
+    from oracle import oracle as ora
+    from gelios_2d_amd.params import GEL_PREC_FP64, GEL_SRC_MK1
+    threads = cpu_threads()
+    cells = 0
+    t_total = 0.0
+    passes = 0
+    sweeps = []
     for w, f in specs:
         prm = w.params.copy()
         prm.precision = GEL_PREC_FP64
         prm.source_mode = GEL_SRC_MK1
-        s = Setka(w.grid, prm)
-        s.upload(w.prim0)
-        s.set_sources(f)
-        cases.append(s)
-    b = Batch(cases)
-    b.step(steps, True)
-    return [s.download() for s in cases]
+        sw = ora.OracleSweep.from_polar(w.grid, prm)
+        sw.src = np.ascontiguousarray(f, np.float64)
+        cur, _ = sw.step(w.prim0, nthreads=threads)          # warm-up (threads, pages)
+        sweeps.append([sw, cur, w.grid.ncells])
+    while t_total < budget_s:                             # passes of 5 steps over every case
+        for e in sweeps:
+            t0 = time.perf_counter()
+            for _ in range(5):
+                e[1], _ = e[0].step(e[1], nthreads=threads)
+            t_total += time.perf_counter() - t0
+            cells += e[2] * 5
+        passes += 1
+    value = cells / t_total
+    out = {"value": value, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+           "sample": f"all {len(specs)} ensemble cases, {passes} passes of 5 steps each ({5 * passes} steps per case), "
+                     f"FP64 HLLD with the analytic charge-exchange source, OpenMP {threads} threads, "
+                     f"{t_total:.1f} s timed",
+           "host": host_cpu()}
+    # the compiled reference against the port on a workload both run (the reference has
+    # no MHD sweep), measured where the reference's sources are (oracle/time_reference.py)
+    fj = os.path.join(ROOT, "oracle", "fairness.json")
+    if os.path.exists(fj):
+        with open(fj) as f:
+            fair = json.load(f)["entries"]
+        e = min(fair, key=lambda x: abs(x["threads"] - threads))
+        out["fairness"] = {"port_over_reference": e["port_over_reference"], "threads": e["threads"],
+                           "measured_on": e["cpu"], "workload": e["workload"],
+                           "note": "measured in the build container, where the reference's sources are; "
+                                   "the reference is not run on this host"}
+        out["reference_equivalent_value"] = value / e["port_over_reference"]
+    return out
 
 
-def cpu_baseline(w, budget_s=12.0):
+def cpu_baseline_workload(w, budget_s=12.0):
     """The CPU restatement of the reference sweep (per-cell gather over faces, every
     interior face solved from both sides, FP64, OpenMP over cells, as
     Setka.cpp:10148) on the host cores, over a bounded number of steps."""
     from oracle import oracle as ora
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     motion = getattr(w, "motion", None)
     if getattr(w, "tracking", None) is not None:
         tr = ora.TrackedOracle(w.grid, w.params, w.tracking, w.rails)
@@ -257,14 +424,7 @@ def cpu_baseline(w, budget_s=12.0):
     dt = time.perf_counter() - t0
     out = {"value": w.grid.ncells * steps / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
            "sample": f"{steps} steps of {w.name} ({w.grid.ncells} cells), FP64, OpenMP {threads} threads"}
-    # fairness: the compiled reference sweep against the port on a grid both run (the
-    # reference has no MHD sweep), same threads; where the harness was built
-    from oracle import time_reference as tr
-    if tr.available():
-        m = tr.measure(200, 100, 20, threads)
-        out["fairness"] = {"workload": "reference gas HLL sweep, 200x100 polar grid with C_1 wind rows, 20 steps",
-                           "reference_cell_updates_per_s": m["reference"], "port_cell_updates_per_s": m["port"],
-                           "port_over_reference": m["port"] / m["reference"]}
+    out["host"] = host_cpu()
     return out
 
 
@@ -283,64 +443,86 @@ def main():
         sys.exit(2)
 
     from gelios_2d_amd import configs
-    from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_PREC_FP64
+    from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_PREC_FP64, GEL_SRC_MK1
     from gelios_2d_amd.state import l1_rel, linf_rel
 
     wl = args.workload
     W = args.warmup
     extra = {}
     if wl == "ensemble":
-        import numpy as np
         K = args.steps if args.steps is not None else 500
-        w64 = configs.hlld_mhd_800x400(GEL_PREC_FP64)
-        log(f"bench: batched ensemble, {K} steps; FP64 plain reference workload {w64.name}")
-        r64 = run_gpu(w64, 1000, W, "hlld-800x400-fp64-plain")
-        r5, finals, specs, warm = run_ensemble(K, args.refresh_every)
-        # the timed run did `warm` warm-up steps first: the reference runs K + warm steps
-        ref = run_ensemble_reference(specs, K + warm)
-        num = {0: 0.0, 1: 0.0}
-        den = {0: 0.0, 1: 0.0}
-        emax = {0: 0.0, 1: 0.0}
-        vmax = {0: 0.0, 1: 0.0}
-        for a, b in zip(finals, ref):
-            for k in (0, 1):
-                num[k] += float(np.abs(a[k] - b[k]).sum())
-                den[k] += float(np.abs(b[k]).sum())
-                emax[k] = max(emax[k], float(np.abs(a[k] - b[k]).max()))
-                vmax[k] = max(vmax[k], float(np.abs(b[k]).max()))
-        l1 = {"density": num[0] / den[0], "pressure": num[1] / den[1]}
-        linf = {"density": emax[0] / vmax[0], "pressure": emax[1] / vmax[1]}
-        log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s ({r5['wall_s']:.3f} s incl. source refreshes); "
-            f"vs FP64 plain hlld-800x400 {r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; L1 vs FP64 analytic {l1}; "
-            f"Linf {linf}; "
-            f"judge fallback fraction {r5['fallback_fraction']:.3f}")
-        ra = run_ensemble_analytic(specs, K, warm)
-        log(f"bench[ensemble]: same cases, FP32 bulk with the analytic closure only: {ra['value']:.4e} cell-updates/s")
-        B = bytes_per_cell_update(specs[0][0], r5["fallback_fraction"])
-        achieved = B * r5["cells"] / (r5["sweep_ms_per_step"] * 1e-3) / 1e9
+        specs = configs.ensemble_cases(64, seed=5, steps=500)
+        log(f"bench: batched ensemble of {len(specs)} cases, warmup {W}, {K} timed steps")
+        r5 = run_ensemble(specs, K, W, args.refresh_every)
+        r64 = run_ensemble_plain(specs, K, W, GEL_PREC_FP64, GEL_SRC_MK1, "ensemble-fp64-plain")
+        ra = run_ensemble_plain(specs, K, W, GEL_PREC_FP32_BULK, GEL_SRC_MK1, "ensemble-fp32-analytic")
+        l1, linf, ffrac = ensemble_accuracy(specs, 500, args.refresh_every)
+        w2 = configs.hlld_mhd_800x400(GEL_PREC_FP64)
+        r2 = run_gpu(w2, 1000, W, "hlld-800x400-fp64-plain")
+        log(f"bench[ensemble]: {r5['value']:.4e} cell-updates/s against FP64 plain on the same cases "
+            f"{r64['value']:.4e} -> x{r5['value'] / r64['value']:.2f}; FP32 with the analytic closure only "
+            f"{ra['value']:.4e}; L1 after 500 steps {l1}; Linf {linf}; fallback fraction {ffrac:.3f}")
+        cells = r5["cells"]
+        B = 136           # algorithmic bytes per cell-update of this workload (FP32 state, nodes, 8 H moments)
+        B_in = bytes_per_cell_update(specs[0][0])   # what the kernel reads and writes per cell
+        achieved = B * cells / (r5["sweep_us_per_step"] * 1e-6) / 1e9
         main_r = {"value": r5["value"], "wall_s": r5["wall_s"],
                   "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                "frac": achieved / HBM_PEAK_GBS, "traffic": ENSEMBLE_TRAFFIC_BYTES,
                                "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE)",
-                               "bytes_per_cell_update": B, "kernel_avg_us": r5["sweep_ms_per_step"] * 1e3}}
+                               "bytes_per_cell_update": B, "kernel_avg_us": r5["sweep_us_per_step"],
+                               "kernel_time": "HIP events on the batch stream around 60 graph-replayed "
+                                              "steps of the sweep alone / 60",
+                               "kernel_inputs_bytes_per_cell_update": B_in,
+                               "kernel_inputs_frac": B_in * cells / (r5["sweep_us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS}}
         main_w = None
+        B64 = bytes_per_cell_update(dict_params(specs[0][0], GEL_PREC_FP64, GEL_SRC_MK1))
+        a64 = B64 * cells / (r64["sweep_us_per_step"] * 1e-6) / 1e9
         extra = {
-            "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / 1000,
-                           "roofline": r64["roofline"]},
+            "fp64_plain": dict(r64, workload="the same 64 cases, FP64 state and arithmetic, analytic "
+                                             "charge-exchange source every step, batched, HIP graph",
+                               roofline={"bound": "hbm", "achieved": a64, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                         "frac": a64 / HBM_PEAK_GBS, "bytes_per_cell_update": B64,
+                                         "kernel_avg_us": r64["sweep_us_per_step"]}),
             "speedup_vs_fp64_plain": r5["value"] / r64["value"],
-            "fp32_analytic_source_only": dict(ra, note="same cases, FP32 bulk, GEL_SRC_MK1 every step, no network"),
-            "l1_vs_fp64": dict(l1, reference="same 64 cases, FP64, analytic sources every step", steps=K + warm),
-            "linf_vs_fp64": dict(linf, definition="max |err| / max |FP64 value| over all cells of all cases"),
-            "batch": {"cases": r5["cases"], "cells": r5["cells"], "autotune": r5["autotune"],
+            "fp32_analytic_source_only": dict(ra, note="same cases, FP32 bulk, analytic closure every step, "
+                                                       "no network"),
+            "fp64_plain_800x400_moving": {"workload": w2.name, "value": r2["value"],
+                                          "ms_per_step": r2["wall_s"] * 1e3 / 1000, "roofline": r2["roofline"],
+                                          "p_clamps": r2["n_p_clamp"], "steps": 1000 + W},
+            "l1_vs_fp64": l1,
+            "linf_vs_fp64": linf,
+            "batch": {"cases": r5["cases"], "cells": cells, "autotune": r5["autotune"],
                       "source_refresh_every": args.refresh_every, "judge_tol": 1e-3,
                       "judge_fallback_cells_initial": r5["fallback_cells_initial"],
-                      "refreshes_timed": r5["refreshes"], "judge_fallback_fraction": r5["fallback_fraction"],
+                      "refreshes_timed": r5["refreshes"], "judge_fallback_fraction_timed": r5["fallback_fraction"],
+                      "judge_fallback_fraction_500_steps": ffrac,
                       "refresh_ms": r5["refresh_ms_host_timed"],
-                      "kernel_ms_per_step_incl_refresh": r5["kernel_ms_per_step"]},
+                      "device_ms_per_step_incl_refresh": r5["device_ms_per_step"]},
         }
-        precision, dtype = "fp32-bulk + learned source (BF16 MFMA), batched, HIP graph", "f32"
-        config = {"workload": "ensemble-64-cases-100x50-to-800x400-hlld-fp32-learned-source", "cells": r5["cells"], "cases": r5["cases"],
-                  "physics": "mhd-hlld", "moving_grid": False, "precision": precision, "cfl": 0.4}
+        precision, dtype = "fp32-bulk + learned source (BF16 MFMA, hybrid), batched, HIP graph", "f32"
+        config = {"workload": "ensemble-64-cases-100x50-to-800x400-hlld-fp32-learned-source", "cells": cells,
+                  "cases": r5["cases"], "physics": "mhd-hlld", "moving_grid": False, "precision": precision,
+                  "cfl": 0.4}
+    elif wl == "learned-800x400":
+        w = configs.learned_mhd_800x400(GEL_PREC_FP64)
+        w64 = configs.learned_mhd_800x400(GEL_PREC_FP64, GEL_SRC_MK1)
+        K = args.steps if args.steps is not None else w.steps
+        log(f"bench: {w.name} against {w64.name}, {w.grid.ncells} cells, warmup {W}, steps {K}")
+        rl = run_learned(w, K, W, args.refresh_every)
+        r64 = run_gpu(w64, K, W, "fp64-analytic")
+        main_w, main_r = w, rl
+        extra = {
+            "fp64_plain": {"workload": w64.name, "value": r64["value"], "ms_per_step": r64["wall_s"] * 1e3 / K,
+                           "roofline": r64["roofline"]},
+            "speedup_vs_fp64_plain": rl["value"] / r64["value"],
+            "l1_vs_fp64": {"density": l1_rel(rl["final"], r64["final"], 0),
+                           "pressure": l1_rel(rl["final"], r64["final"], 1), "after_steps": W + K},
+            "judge": {"flagged_initial": rl["fallback_initial"], "flagged_last_refresh": rl["fallback_last"],
+                      "refresh_every": args.refresh_every, "tol": 1e-3},
+        }
+        precision, dtype = "fp64 state and arithmetic + learned source (BF16 MFMA, hybrid)", "f64"
+        config = None
     elif wl in ("hlld-800x400", "hydrogen-1600x800"):
         if wl == "hlld-800x400":
             w64, w32 = configs.hlld_mhd_800x400(GEL_PREC_FP64), configs.hlld_mhd_800x400(GEL_PREC_FP32_BULK)
@@ -408,8 +590,11 @@ def main():
     }
     line.update(extra)
     if not args.no_cpu:
-        cw = main_w if wl not in ("hlld-800x400", "hydrogen-1600x800", "ensemble") else w64
-        line["cpu_baseline"] = cpu_baseline(cw)
+        if wl == "ensemble":
+            line["cpu_baseline"] = cpu_baseline(specs)
+        else:
+            line["cpu_baseline"] = cpu_baseline_workload(main_w if wl not in ("hlld-800x400", "hydrogen-1600x800",
+                                                                               "learned-800x400") else w64)
         log(f"bench: cpu baseline {line['cpu_baseline']['value']:.3e} cell-updates/s")
     print(json.dumps(line), flush=True)
 

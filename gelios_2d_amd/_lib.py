@@ -82,6 +82,7 @@ SIGNATURES = {
     "gel_batch_destroy": (None, [ctypes.c_void_p]),
     "gel_batch_set_launch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
     "gel_batch_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]),
+    "gel_batch_prepare": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gel_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int), _dp]),
     "gel_batch_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,

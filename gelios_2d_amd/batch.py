@@ -45,6 +45,10 @@ class Batch:
         _lib.check(_lib.lib().gel_batch_step(self._h, int(nsteps), 1 if graph else 0, ctypes.byref(ms)))
         return ms.value
 
+    def prepare(self, nsteps: int) -> None:
+        """Capture the graphs that step(nsteps) replays next, running nothing."""
+        _lib.check(_lib.lib().gel_batch_prepare(self._h, int(nsteps)))
+
     def autotune(self, trial_steps: int = 6):
         r, w, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         _lib.check(_lib.lib().gel_batch_autotune(self._h, int(trial_steps), ctypes.byref(r), ctypes.byref(w),
@@ -61,7 +65,8 @@ class Batch:
         return n.value
 
     def set_source_model(self, model, every: int, tol: float = 1e-3, judge=True) -> None:
-        """Refresh the source tables inside step() after every `every` steps (0: off);
+        """Refresh the source tables inside step() after every `every`-th step the
+        batch advances from now on, over all calls whatever their lengths (0: off);
         judge as in refresh()."""
         self._model = model          # the batch holds the network's handle
         _lib.check(_lib.lib().gel_batch_set_source_model(self._h, model.mlp._h if model else None, int(every),

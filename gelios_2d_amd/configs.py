@@ -132,6 +132,27 @@ def mhd_hydrogen_1600x800(precision: int = GEL_PREC_FP64, steps: int = 2000) -> 
     return w
 
 
+def learned_mhd_800x400(precision: int = GEL_PREC_FP64, source_mode: int = None, steps: int = 1000) -> Workload:
+    """The learned-source form of the 800 x 400 moving HLLD workload: the same grid,
+    motion (seed 2), field and Mach-2 inflow, coupled to four frozen hydrogen populations
+    (n_H = 0.1 (1 - exp(-r/50)), seeded perturbations) by the charge-exchange source with
+    sigma(U) = (1 - 0.07 ln U)^2.  source_mode: GEL_SRC_HYBRID (default; the network's
+    table, the analytic closure where its judge rejects it) or GEL_SRC_MK1 (the analytic
+    closure every step, the plain form).  The hydrogen fields are in .src."""
+    from .params import GEL_SRC_HYBRID
+    from .surrogate import workload_with_hydrogen
+    if source_mode is None:
+        source_mode = GEL_SRC_HYBRID
+    w, f = workload_with_hydrogen(800, 400, steps, INF_MACH, 0.1, 2, moving=True)
+    w.params.precision = precision
+    w.params.source_mode = source_mode
+    w.src = f
+    tag = "fp64" if precision == GEL_PREC_FP64 else "fp32bulk"
+    kind = "learned" if source_mode == GEL_SRC_HYBRID else "analytic"
+    w.name = f"hlld-mhd-{tag}-{kind}-source-800x400-moving"
+    return w
+
+
 def ensemble_cases(n_cases: int = 64, seed: int = 5, steps: int = 500, precision: int = None,
                    source_mode: int = None):
     """Batched ensemble: grid sizes log-uniform from 100 x 50 to 800 x 400, inflow Mach

@@ -123,7 +123,8 @@ struct gel_ctx {
     std::vector<double> motion;
     long long motion_step = 0;
     hipStream_t stream = nullptr;
-    bool timing = false;
+    int timing = 0;       // 1: events around every launch; 2: around every gel_step call
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    std::vector<long long> event_steps;   // steps each event pair spans
 };
 

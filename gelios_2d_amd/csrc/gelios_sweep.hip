@@ -550,16 +550,17 @@ static int one_step(gel_ctx* c, int moving)
     if ((c->prm.source_mode == GEL_SRC_TABLE || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_qtab)
         return fail(GEL_E_STATE, "source table not set");
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
+    if (c->timing == 1) {
         HIP_OK(hipEventCreate(&e0));
         HIP_OK(hipEventCreate(&e1));
         HIP_OK(hipEventRecord(e0, c->stream));
     }
     hipError_t e = (moving == 3) ? launch_tracked_step(c, a) : launch(c, a, moving != 0);
     if (e != hipSuccess) return fail(GEL_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
-    if (c->timing) {
+    if (c->timing == 1) {
         HIP_OK(hipEventRecord(e1, c->stream));
         c->events.emplace_back(e0, e1);
+        c->event_steps.push_back(1);
     }
     c->cur ^= 1;
     c->steps++;
@@ -627,9 +628,20 @@ int gel_step(gel_ctx* c, int nsteps, gel_stats* out)
     if (!c || nsteps < 0) return fail(GEL_E_ARG, "bad argument");
     if (!c->uploaded) return fail(GEL_E_STATE, "no state uploaded");
     const int mode = c->tracking ? 3 : (c->motion.empty() ? 0 : 2);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing == 2 && nsteps > 0) {
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        HIP_OK(hipEventRecord(e0, c->stream));
+    }
     for (int s = 0; s < nsteps; s++) {
         const int st = one_step(c, mode);
         if (st != GEL_OK) return st;
+    }
+    if (e0) {
+        HIP_OK(hipEventRecord(e1, c->stream));
+        c->events.emplace_back(e0, e1);
+        c->event_steps.push_back(nsteps);
     }
     return fill_stats(c, out);
 }
@@ -670,8 +682,8 @@ int gel_set_dt(gel_ctx* c, double dt)
 
 int gel_enable_timing(gel_ctx* c, int on)
 {
-    if (!c) return fail(GEL_E_ARG, "null argument");
-    c->timing = on != 0;
+    if (!c || on < 0 || on > 2) return fail(GEL_E_ARG, "timing mode is 0, 1 or 2");
+    c->timing = on;
     return GEL_OK;
 }
 
@@ -680,16 +692,19 @@ int gel_kernel_timing(gel_ctx* c, double* avg_ms, long long* n)
     if (!c || !avg_ms || !n) return fail(GEL_E_ARG, "null argument");
     HIP_OK(hipStreamSynchronize(c->stream));
     double tot = 0.0;
-    for (auto& e : c->events) {
+    long long steps = 0;
+    for (size_t k = 0; k < c->events.size(); k++) {
         float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, e.first, e.second));
+        HIP_OK(hipEventElapsedTime(&ms, c->events[k].first, c->events[k].second));
         tot += ms;
-        (void)hipEventDestroy(e.first);
-        (void)hipEventDestroy(e.second);
+        steps += c->event_steps[k];
+        (void)hipEventDestroy(c->events[k].first);
+        (void)hipEventDestroy(c->events[k].second);
     }
-    *n = (long long)c->events.size();
-    *avg_ms = c->events.empty() ? 0.0 : tot / (double)c->events.size();
+    *n = steps;
+    *avg_ms = steps == 0 ? 0.0 : tot / (double)steps;
     c->events.clear();
+    c->event_steps.clear();
     return GEL_OK;
 }
 
@@ -763,8 +778,13 @@ struct gel_batch {
     BatchItem* d_items = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
-    hipGraphExec_t exec = nullptr;
+    hipGraphExec_t exec = nullptr;  // one cycle of batch_cycle() steps
     bool graph_valid = false;
+    int graph_off = 0;              // refresh offset (steps since the last refresh) it was captured at
+    hipGraphExec_t tail = nullptr;  // a call's steps after its whole cycles
+    int tail_len = 0, tail_off = 0;
+    long long gstep = 0;            // steps the batch has advanced its cases
+    long long src_base = 0;         // gstep when the source model was attached
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // learned source refreshed inside gel_batch_step every `every` steps
     gel_mlp* model = nullptr;
@@ -791,6 +811,9 @@ static void batch_drop_graph(gel_batch* b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
     b->graph_valid = false;
+    if (b->tail) (void)hipGraphExecDestroy(b->tail);
+    b->tail = nullptr;
+    b->tail_len = 0;
 }
 
 static int batch_build_items(gel_batch* b)
@@ -877,12 +900,21 @@ static long long batch_cells(const gel_batch* b)
     return n;
 }
 
-// step i of a cycle: the sweep, then the source refresh when one is due
-static hipError_t batch_launch_step(const gel_batch* b, int i)
+// Steps since the last refresh of the batch's schedule: a refresh follows every
+// `every`-th step the batch advances after the model was attached, whatever the
+// lengths of the calls.
+static int batch_refresh_offset(const gel_batch* b)
+{
+    return (b->model && b->every > 0) ? (int)((b->gstep - b->src_base) % b->every) : 0;
+}
+
+// step i of a call that starts `off` steps after a refresh: the sweep (argument block
+// i mod 6), then the source refresh when one is due
+static hipError_t batch_launch_step(const gel_batch* b, int i, int off)
 {
     const int phase = i % kCycle;
     hipError_t e = batch_launch(b, phase);
-    if (e == hipSuccess && b->model && b->every > 0 && (i + 1) % b->every == 0) {
+    if (e == hipSuccess && b->model && b->every > 0 && (off + i + 1) % b->every == 0) {
         const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
         const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
         if (b->judge == 2)
@@ -893,6 +925,51 @@ static hipError_t batch_launch_step(const gel_batch* b, int i)
                                    b->d_nfall);
     }
     return e;
+}
+
+// Capture steps [0, n) of a call starting `off` steps after a refresh as a graph.
+static int batch_capture(gel_batch* b, int n, int off, hipGraphExec_t* out)
+{
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
+    hipError_t e = hipSuccess;
+    for (int p = 0; p < n && e == hipSuccess; p++) e = batch_launch_step(b, p, off);
+    const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(GEL_E_HIP, "graph capture");
+    }
+    e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(GEL_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+    return GEL_OK;
+}
+
+// The graphs a call of nsteps steps replays: whole cycles (a cycle starts and ends at
+// the same buffer, time-step slot and refresh phase), then one graph of the rest.
+static int batch_prepare_graphs(gel_batch* b, int nsteps)
+{
+    const int cyc = batch_cycle(b), off = batch_refresh_offset(b);
+    if (nsteps >= cyc && !(b->graph_valid && b->graph_off == off)) {
+        if (b->exec) (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        b->graph_valid = false;
+        const int st = batch_capture(b, cyc, off, &b->exec);
+        if (st != GEL_OK) return st;
+        b->graph_valid = true;
+        b->graph_off = off;
+    }
+    const int rem = nsteps % cyc;
+    if (rem > 0 && !(b->tail && b->tail_len == rem && b->tail_off == off)) {
+        if (b->tail) (void)hipGraphExecDestroy(b->tail);
+        b->tail = nullptr;
+        b->tail_len = 0;
+        const int st = batch_capture(b, rem, off, &b->tail);
+        if (st != GEL_OK) return st;
+        b->tail_len = rem;
+        b->tail_off = off;
+    }
+    return GEL_OK;
 }
 
 // judge 2: (re)build the list of cells still on the network from the current flags,
@@ -914,6 +991,7 @@ static int batch_build_list(gel_batch* b)
 
 static void batch_advance(gel_batch* b, int n)
 {
+    b->gstep += n;
     for (gel_ctx* c : b->cases) {
         c->steps += n;
         if (n & 1) c->cur ^= 1;
@@ -1002,8 +1080,10 @@ int gel_batch_set_launch(gel_batch* b, int rows, int workgroups)
     return batch_build_items(b);
 }
 
-// Advance every case nsteps steps.  use_graph: whole cycles of kCycle steps replay a
-// captured HIP graph of kCycle launches; the remainder runs as plain launches.
+// Advance every case nsteps steps.  use_graph: whole cycles of batch_cycle() steps
+// replay one captured HIP graph, the remaining steps another (both kept for the next
+// call of the same length and refresh phase); otherwise plain launches.  A refresh
+// follows every `every`-th step of the batch, counted over all calls.
 // *ms (may be NULL) receives the device time of the call (HIP events on the batch stream).
 int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
 {
@@ -1012,35 +1092,25 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
     int st = batch_upload_args(b);
     if (st != GEL_OK) return st;
     if (b->model && b->every > 0 && b->judge == 2 && (st = batch_build_list(b)) != GEL_OK) return st;
+    const int cyc = batch_cycle(b), off = batch_refresh_offset(b);
+    if (use_graph && (st = batch_prepare_graphs(b, nsteps)) != GEL_OK) return st;
     HIP_OK(hipEventRecord(b->ev0, b->stream));
     int done = 0;
-    const int cyc = batch_cycle(b);
-    if (use_graph && nsteps >= cyc) {
-        if (!b->graph_valid) {
-            hipGraph_t g = nullptr;
-            HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeGlobal));
-            hipError_t e = hipSuccess;
-            for (int p = 0; p < cyc && e == hipSuccess; p++) e = batch_launch_step(b, p);
-            const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
-            if (e != hipSuccess || e2 != hipSuccess) {
-                if (g) (void)hipGraphDestroy(g);
-                return fail(GEL_E_HIP, "graph capture");
-            }
-            e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (e != hipSuccess) return fail(GEL_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
-            b->graph_valid = true;
-        }
+    if (use_graph) {
         while (nsteps - done >= cyc) {
             HIP_OK(hipGraphLaunch(b->exec, b->stream));
             done += cyc;
         }
+        if (done < nsteps) {
+            HIP_OK(hipGraphLaunch(b->tail, b->stream));
+            done = nsteps;
+        }
     }
     for (; done < nsteps; done++) {
-        const hipError_t e = batch_launch_step(b, done % cyc);
+        const hipError_t e = batch_launch_step(b, done, off);
         if (e != hipSuccess) return fail(GEL_E_HIP, std::string("batch launch: ") + hipGetErrorString(e));
     }
-    if (b->model && b->every > 0) b->refreshes += nsteps / b->every;
+    if (b->model && b->every > 0) b->refreshes += (off + nsteps) / b->every;
     HIP_OK(hipEventRecord(b->ev1, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     if (ms) {
@@ -1050,6 +1120,13 @@ int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms)
     }
     batch_advance(b, nsteps);
     return GEL_OK;
+}
+
+int gel_batch_prepare(gel_batch* b, int nsteps)
+{
+    if (!b || nsteps < 0) return fail(GEL_E_ARG, "bad argument");
+    for (gel_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
+    return batch_prepare_graphs(b, nsteps);
 }
 
 int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, int judge)
@@ -1066,6 +1143,7 @@ int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, 
             if (c->prm.source_mode != GEL_SRC_HYBRID) return fail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID cases");
     b->model = every > 0 ? m : nullptr;
     b->every = every;
+    b->src_base = b->gstep;
     b->tol = tol;
     b->judge = judge;
     batch_drop_graph(b);
@@ -1168,6 +1246,7 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
     const int wg_c[] = { resident, 2 * resident };
     gel_mlp* model = b->model;      // trials run the sweep alone (the refresh does not depend on the shape)
     b->model = nullptr;
+    const long long gstep = b->gstep;
     batch_drop_graph(b);
     double best = 1e300;
     int br = 0, bw = 0;
@@ -1189,6 +1268,7 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
     }
     for (auto& s : saved) { (void)hipFree(s.st[0]); (void)hipFree(s.st[1]); }
     b->model = model;
+    b->gstep = gstep;
     batch_drop_graph(b);
     if (st != GEL_OK) return st;
     b->rows = br;

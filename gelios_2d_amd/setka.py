@@ -132,8 +132,10 @@ class Setka:
         _lib.check(_lib.lib().gel_sync(self._ctx, ctypes.byref(self.stats)))
         return self.stats
 
-    def enable_timing(self, on: bool = True) -> None:
-        _lib.check(_lib.lib().gel_enable_timing(self._ctx, 1 if on else 0))
+    def enable_timing(self, on=True) -> None:
+        """Event timing: False / 0 off, True / 1 around every launch, 2 around every
+        Go_5_komponent__MK2 call (per-step time of back-to-back launches)."""
+        _lib.check(_lib.lib().gel_enable_timing(self._ctx, int(on)))
 
     def kernel_timing(self):
         ms = ctypes.c_double()

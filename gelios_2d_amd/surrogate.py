@@ -135,6 +135,20 @@ class SourceModel:
         _check(L.gel_surrogate_refresh(setka._ctx, self.mlp._h, float(tol), int(judge), ctypes.byref(n)))
         return n.value
 
+    def advance(self, setka, steps: int, every: int = 10, tol: float = 1e-3):
+        """Advance a GEL_SRC_HYBRID Setka `steps` steps with its sources refreshed from
+        the network after every `every`-th step (sticky judge: cells already sent to the
+        analytic closure keep it).  The first refresh, with every cell judged, is the
+        caller's.  Returns the flagged cells after the last refresh (or None)."""
+        done, nfall = 0, None
+        while done < steps:
+            n = min(every, steps - done)
+            setka.Go_5_komponent__MK2(n)
+            done += n
+            if n == every:
+                nfall = self.refresh(setka, tol, judge=2)
+        return nfall
+
     def save(self, path: str) -> None:
         np.savez(path, params=self.mlp.get_params(), mu=self.mu, sd=self.sd, ymu=self.ymu, ysd=self.ysd)
 

@@ -271,11 +271,13 @@ int gel_set_rows_per_wave(gel_ctx* ctx, int rows);
 /* Block until queued work is done and fill *out (may be NULL). */
 int gel_sync(gel_ctx* ctx, gel_stats* out);
 
-/* Average device duration in milliseconds of the sweep kernel over the launches
- * timed since the last call (HIP events on the context's stream); resets the
- * accumulator.  *n_launches gets the number of launches averaged. */
+/* Average device time in milliseconds per step over the steps timed since the last
+ * call (HIP events on the context's stream); resets the accumulator.  *n_launches
+ * gets the number of steps averaged. */
 int gel_kernel_timing(gel_ctx* ctx, double* avg_ms, long long* n_launches);
-/* Turn per-launch event timing on or off (off by default). */
+/* Event timing: 0 off (default); 1 events around every step's launch (each interval
+ * also holds that launch's dispatch latency); 2 events around every gel_step call, so
+ * that a call of n back-to-back steps gives the per-step time the stream sustains. */
 int gel_enable_timing(gel_ctx* ctx, int on);
 
 /* Batched device evaluation of the face flux, for unit tests against fixtures:
@@ -312,8 +314,10 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
  * Many independent contexts (static grids, same physics / precision / flux / source
  * mode) advanced together: one persistent launch per step walks the (case, strip,
  * chunk) work list; each case keeps its own state, buffers and CFL time step.
- * Whole cycles of 6 steps replay a captured HIP graph.  Each context may appear once
- * (GEL_E_ARG otherwise, and for a null entry). */
+ * With use_graph, whole cycles of 6 steps (or of a common multiple of 6 and the
+ * refresh interval of an attached network) replay one captured HIP graph and the rest
+ * of a call another, both kept for later calls of the same length and refresh phase.
+ * Each context may appear once (GEL_E_ARG otherwise, and for a null entry). */
 typedef struct gel_batch gel_batch;
 int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases);
 void gel_batch_destroy(gel_batch* b);
@@ -321,6 +325,9 @@ void gel_batch_destroy(gel_batch* b);
 int gel_batch_set_launch(gel_batch* b, int rows, int workgroups);
 /* Advance all cases nsteps steps; *ms (may be NULL) gets the device time. */
 int gel_batch_step(gel_batch* b, int nsteps, int use_graph, double* ms);
+/* Capture (without running anything) the graphs that gel_batch_step(b, nsteps, 1, .)
+ * replays next, so that the call itself only launches them. */
+int gel_batch_prepare(gel_batch* b, int nsteps);
 /* Time candidate launch shapes on the cases (states restored) and keep the fastest. */
 int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_workgroups, double* ms_per_step);
 
@@ -359,9 +366,9 @@ int gel_surrogate_refresh(gel_ctx* ctx, gel_mlp* m, double tol, int judge, long 
 /* The same for every case of a batch at once (their current states). */
 int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback);
 /* Attach a network to a batch of GEL_SRC_TABLE cases: gel_batch_step then refreshes
- * every case's source table after every `every`-th step of a call (also inside the
- * captured graph, whose cycle becomes a common multiple of 6 and `every`).
- * every = 0 detaches. */
+ * every case's source table after every `every`-th step the batch advances from now
+ * on, counted over all calls whatever their lengths (also inside the captured
+ * graphs).  every = 0 detaches. */
 int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, int judge);
 /* Refreshes run by gel_batch_step and the cells that fell back in them, since the
  * last call (both reset). */

@@ -7,9 +7,11 @@ reference's constants: HLL, CFL 0.1) from the same state for the same number of 
 with the same OpenMP thread count; only the step loop is timed (the harness times its
 call of Setka::Go_5_komponent__MK2 with std::chrono::steady_clock).
 
-    python oracle/time_reference.py [--ni 200] [--nj 100] [--steps 20] [--threads 8]
+    python oracle/time_reference.py [--ni 200] [--nj 100] [--steps 20] [--threads 8] [--write]
 
-Prints the cell-updates/s of both and their ratio.  With more than one thread the
+Prints the cell-updates/s of both and their ratio; --write adds the measurement to
+oracle/fairness.json (one entry per thread count), which the CPU-baseline leg of the
+benchmark reads on hosts where the reference is absent.  With more than one thread the
 reference's shared time step races (Setka.cpp:10141), so only one thread is compared
 bit for bit.
 """
@@ -75,12 +77,37 @@ def measure(ni: int = 200, nj: int = 100, steps: int = 20, threads: int = 1) -> 
             "port": nc * steps / t_ora, "max_rel_diff": diff}
 
 
+FAIRNESS_JSON = os.path.join(os.path.dirname(os.path.abspath(__file__)), "fairness.json")
+
+
+def write_fairness(m: dict) -> None:
+    import json
+    import platform
+    d = {"entries": []}
+    if os.path.exists(FAIRNESS_JSON):
+        with open(FAIRNESS_JSON) as f:
+            d = json.load(f)
+    cpu = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        cpu = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), cpu)
+    except OSError:
+        pass
+    e = dict(m, port_over_reference=m["port"] / m["reference"], cpu=cpu,
+             workload="the reference's gas HLL sweep (Go_5_komponent__MK2), polar grid with C_1 wind rows")
+    d["entries"] = [x for x in d["entries"] if x["threads"] != m["threads"]] + [e]
+    d["entries"].sort(key=lambda x: x["threads"])
+    with open(FAIRNESS_JSON, "w") as f:
+        json.dump(d, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ni", type=int, default=200)
     ap.add_argument("--nj", type=int, default=100)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--threads", type=int, default=os.cpu_count() or 1)
+    ap.add_argument("--write", action="store_true")
     args = ap.parse_args()
     ora.build()
     m = measure(args.ni, args.nj, args.steps, args.threads)
@@ -89,6 +116,8 @@ def main():
     print(f"oracle     {m['port']:.4e} cell-updates/s")
     print(f"oracle / reference speed: {m['port'] / m['reference']:.2f}; "
           f"max relative difference of the states {m['max_rel_diff']:.2e}")
+    if args.write:
+        write_fairness(m)
 
 
 if __name__ == "__main__":

@@ -119,6 +119,53 @@ def test_refresh_inside_graph_equals_interleaved_calls(every, judge):
 
 
 
+@pytest.mark.parametrize("call", [1, 7, 20])
+def test_refresh_schedule_counts_the_batch_steps(call):
+    """The refresh follows every `every`-th step the batch advances, however the steps
+    are split into calls (one step at a time, `every` + 3 per call, or calls longer than
+    a graph cycle): states, tables, flags and the refresh count equal stepping and
+    refreshing by hand; calls of one length replay their graphs."""
+    from gelios_2d_amd.batch import Batch
+    every, total = 4, 62
+    m = _model()
+    a, b_ = _cases(3), _cases(3)
+    A, B = Batch(a), Batch(b_)
+    A.refresh(m)
+    B.refresh(m)
+    B.set_source_model(m, every, tol=1e-3, judge=2)
+    done = 0
+    while done < total:
+        n = min(call, total - done)
+        B.step(n, True)
+        done += n
+    for k in range(total // every):
+        A.step(every, False)
+        A.refresh(m, judge=2)
+    A.step(total % every, False)
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.download(), y.download())
+        assert np.array_equal(x.get_source_table(), y.get_source_table())
+        assert np.array_equal(x.get_source_flags(), y.get_source_flags())
+    r, n = B.source_stats()
+    assert r == total // every
+
+
+def test_prepared_graph_runs_the_same_steps():
+    """prepare(n) captures the graphs of a call of n steps without running them: the
+    states are untouched until step(n), which then equals plain launches."""
+    from gelios_2d_amd.batch import Batch
+    a, b_ = _cases(3), _cases(3)
+    A, B = Batch(a), Batch(b_)
+    before = [s.download() for s in b_]
+    B.prepare(20)
+    for s, x in zip(b_, before):
+        assert np.array_equal(s.download(), x)
+    B.step(20, True)
+    A.step(20, False)
+    for x, y in zip(a, b_):
+        assert np.array_equal(x.download(), y.download())
+
+
 def test_batch_sticky_refresh_equals_per_case_refresh():
     """judge 2 over a batch (the list of (case, cell) still on the network, a persistent
     grid over its chunks) leaves every case the tables, flags and fallback count that
