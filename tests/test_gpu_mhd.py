@@ -72,6 +72,34 @@ def test_mhd_moving_sweep_matches_oracle(oracle_lib, shape):
     assert rel_err_conserved(s.download(), ref) <= TOL
 
 
+@pytest.mark.parametrize("moving", [False, True])
+def test_mhd_swirl_and_azimuthal_field_match_oracle(oracle_lib, moving):
+    """Nonzero swirl w and azimuthal field B_phi in the state and the inflow, so that
+    the geometric sources of the update (hoop stress with rho w^2 and B_phi^2,
+    angular-momentum and B_phi sources) act: the sweep matches the oracle."""
+    from gelios_2d_amd.setka import Setka
+    w = mhd_heliosphere(48, 40, 40, seed=3, moving=moving)
+    cx, cy = w.grid.cell_centers()
+    q = w.prim0.copy()
+    q[4] = 0.3 * np.sin(0.2 * cx) * cy / (1 + cy)
+    q[7] = 0.02 * np.cos(0.1 * cx) * cy / (1 + cy)
+    w.params.inf_w, w.params.inf_bz = 0.05, 0.01
+    sw = oracle_lib.OracleSweep.from_polar(w.grid, w.params)
+    s = Setka(w.grid, w.params)
+    if moving:
+        amp, period, phase = w.motion
+        ref = sw.run_motion(q, w.steps, w.grid.x, w.grid.y, amp, period, phase)
+        s.set_motion(amp, period, phase)
+    else:
+        ref = sw.run(q, w.steps)
+    s.upload(q)
+    st = s.Go_5_komponent__MK2(w.steps)
+    got = s.download()
+    assert st.n_numeric == 0
+    assert np.abs(ref[4]).max() > 1e-3 and np.abs(ref[7]).max() > 1e-4
+    assert rel_err_conserved(got, ref) <= TOL
+
+
 def test_mhd_static_sweep_matches_oracle(oracle_lib):
     from gelios_2d_amd.setka import Setka
     w = mhd_heliosphere(33, 64, 30, seed=6, moving=False)
