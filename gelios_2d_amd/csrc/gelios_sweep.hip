@@ -3,6 +3,8 @@
 // the reference's arithmetic), batches, and the face-flux test kernels.
 #include "sweep_kernel.h"
 
+#include <mutex>
+
 namespace gel {
 
 // Batched face flux for tests (gel_flux_hllc2d_batch).
@@ -764,6 +766,67 @@ int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W
                          hipLaunchKernelGGL(hlld_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, i[0], i[1],
                                             i[2], i[3], i[4], gamma, d[0], d[1]);
                      });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------- the no-allocation flux entry ---
+// gel_flux_hlld_batch_noalloc stages its host arrays in a file-scope device array
+// instead of device memory it allocates: [F][9] L, R, P, then [F] W, n1, n2, smax,
+// F = GEL_NOALLOC_MAX_FACES (31 doubles a face, 8 MB).  Each device holds its own
+// copy from the library's load until the process ends; calls on one device share it.
+static constexpr size_t kNoAllocFaces = GEL_NOALLOC_MAX_FACES;
+static __device__ __align__(256) double scratch[kNoAllocFaces * 31];
+
+static constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device with
+// that device current (the array's name in host code is not a device address).
+static int scratch_on_device(double** out)
+{
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) return fail(GEL_E_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GEL_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GEL_OK;
+}
+
+extern "C" {
+
+int gel_flux_hlld_batch_noalloc(int n, const double* L, const double* R, const double* W, const double* n1,
+                                const double* n2, double gamma, double* P, double* smax)
+{
+    if (n < 0 || (n > 0 && (!L || !R || !W || !n1 || !n2 || !P || !smax))) return fail(GEL_E_ARG, "null argument");
+    if ((size_t)n > kNoAllocFaces)
+        return fail(GEL_E_ARG, "n exceeds GEL_NOALLOC_MAX_FACES (" + std::to_string(kNoAllocFaces) + ") faces");
+    if (n == 0) return GEL_OK;
+    double* s = nullptr;
+    const int st = scratch_on_device(&s);
+    if (st != GEL_OK) return st;
+    const size_t F = kNoAllocFaces, b = sizeof(double);
+    double *dL = s, *dR = s + 9 * F, *dP = s + 18 * F, *dW = s + 27 * F, *dn1 = s + 28 * F, *dn2 = s + 29 * F,
+           *du = s + 30 * F;
+    HIP_OK(hipMemcpy(dL, L, 9 * (size_t)n * b, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dR, R, 9 * (size_t)n * b, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dW, W, (size_t)n * b, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dn1, n1, (size_t)n * b, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dn2, n2, (size_t)n * b, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(hlld_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, dL, dR, dW, dn1, dn2, gamma, dP,
+                       du);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpy(P, dP, 9 * (size_t)n * b, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(smax, du, (size_t)n * b, hipMemcpyDeviceToHost));
+    return GEL_OK;
 }
 
 }  // extern "C"

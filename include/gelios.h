@@ -297,6 +297,16 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
                         const double* n2, double gamma, double* P, double* smax);
 
+/* The same flux with no device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory, such as serving loops and harnesses
+ * that forbid hipMalloc.  The inputs and outputs are staged through a fixed device
+ * array of GEL_NOALLOC_MAX_FACES faces (8 MB) that each device holds for the life of
+ * the process; n above that limit gives GEL_E_ARG.  Calls that overlap on one device
+ * share the array and race: one call at a time per device. */
+#define GEL_NOALLOC_MAX_FACES 32768
+int gel_flux_hlld_batch_noalloc(int n, const double* L, const double* R, const double* W, const double* n1,
+                                const double* n2, double gamma, double* P, double* smax);
+
 /* Batched device evaluation of the exact Riemann solver for a gamma-gas with a moving
  * face, Solvers::Godunov_Solver_Alexashov with newton / devtwo / lev (Solvers.h:23-25,
  * Solvers.cpp:693-1321; the reference applies it to fitted-surface faces,
