@@ -377,9 +377,6 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
     const T SR = mx(u1, u2) + mx(cfL, cfR);
     const T pTL = p_L + bb_L / T(2);
     const T pTR = p_R + bb_R / T(2);
-    const T suR = SR - u2, suL = SL - u1;
-    const T SM = dv(suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL, suR * ro_R - suL * ro_L);
-    const T PTT = dv(suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1), suR * ro_R - suL * ro_L);
     const T UU = mx(fabs(SL), fabs(SR));
     const T e1 = dv(p_L, g1) + ro_L * uu_L + bb_L / T(2);
     const T e2 = dv(p_R, g1) + ro_R * uu_R + bb_R / T(2);
@@ -391,45 +388,6 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
                       u2 * bm2 - w2 * bn2, Q_R * u2 };
     const T UL[9] = { ro_L, ro_L * u1, ro_L * v1, ro_L * w1, e1, bn1, bt1, bm1, Q_L };
     const T UR[9] = { ro_R, ro_R * u2, ro_R * v2, ro_R * w2, e2, bn2, bt2, bm2, Q_R };
-    const T bn = dv(SR * UR[5] - SL * UL[5] + FL[5] - FR[5], SR - SL);
-    const T bbn = bn * bn;
-    const T ro_LL = dv(ro_L * (SL - u1), SL - SM);
-    const T ro_RR = dv(ro_R * (SR - u2), SR - SM);
-    const T Q_LL = dv(Q_L * (SL - u1), SL - SM);
-    const T Q_RR = dv(Q_R * (SR - u2), SR - SM);
-    const T ttL = ro_L * suL * (SL - SM) - bbn;
-    const T ttR = ro_R * suR * (SR - SM) - bbn;
-    T vLL = v1, wLL = w1, btt1 = T(0), bmm1 = T(0);
-    T vRR = v2, wRR = w2, btt2 = T(0), bmm2 = T(0);
-    if (fabs(ttL) >= T(0.000001)) {
-        vLL = v1 - dv(bn * bt1 * (SM - u1), ttL);
-        wLL = w1 - dv(bn * bm1 * (SM - u1), ttL);
-        btt1 = dv(bt1 * (ro_L * suL * suL - bbn), ttL);
-        bmm1 = dv(bm1 * (ro_L * suL * suL - bbn), ttL);
-    }
-    if (fabs(ttR) >= T(0.000001)) {
-        vRR = v2 - dv(bn * bt2 * (SM - u2), ttR);
-        wRR = w2 - dv(bn * bm2 * (SM - u2), ttR);
-        btt2 = dv(bt2 * (ro_R * suR * suR - bbn), ttR);
-        bmm2 = dv(bm2 * (ro_R * suR * suR - bbn), ttR);
-    }
-    const T eLL = dv(e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1)), SL - SM);
-    const T eRR = dv(e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2)), SR - SM);
-    const T sLL = sr(ro_LL), sRR = sr(ro_RR);
-    T SLL = SM - dv(fabs(bn), sLL);
-    T SRR = SM + dv(fabs(bn), sRR);
-    T idbn;
-    if (fabs(bn) > T(0.001)) {
-        idbn = T(1) * sgn(bn);
-    } else {
-        idbn = T(0); SLL = SM; SRR = SM;
-    }
-    const T vLLL = dv(sLL * vLL + sRR * vRR + idbn * (btt2 - btt1), sLL + sRR);
-    const T wLLL = dv(sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1), sLL + sRR);
-    const T bttt = dv(sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL), sLL + sRR);
-    const T bmmm = dv(sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL), sLL + sRR);
-    const T eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
-    const T eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
     // Regions of the fan at xi = W.  Both upwind regions are tested before the star
     // regions, then the side of the contact: the Alfven speeds come from the HLL normal
     // field bn, which the two sides' fast speeds need not bound (SLL < SL or SRR > SR),
@@ -448,28 +406,74 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
 #pragma unroll
         for (int i = 0; i < 9; i++) PO[i] = FR[i] - W * UR[i];
         bnW = UR[5];
-    } else if (SM >= W && SLL >= W) {
-        const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
-#pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
-        bnW = bn;
-    } else if (SM >= W) {
-        const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
-        const T ULLL[9] = { ro_LL, ro_LL * SM, ro_LL * vLLL, ro_LL * wLLL, eLLL, bn, bttt, bmmm, Q_LL };
-#pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
-        bnW = bn;
-    } else if (SRR <= W) {
-        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
-#pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
-        bnW = bn;
     } else {
-        const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
-        const T URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
+        // subsonic faces only: the star states (a wavefront of supersonic faces, the
+        // radial faces of a row in the wind, skips all of it)
+        const T suR = SR - u2, suL = SL - u1;
+        const T SM = dv(suR * ro_R * u2 - suL * ro_L * u1 - pTR + pTL, suR * ro_R - suL * ro_L);
+        const T PTT = dv(suR * ro_R * pTL - suL * ro_L * pTR + ro_L * ro_R * suR * suL * (u2 - u1), suR * ro_R - suL * ro_L);
+        const T bn = dv(SR * UR[5] - SL * UL[5] + FL[5] - FR[5], SR - SL);
+        const T bbn = bn * bn;
+        const T ro_LL = dv(ro_L * (SL - u1), SL - SM);
+        const T ro_RR = dv(ro_R * (SR - u2), SR - SM);
+        const T Q_LL = dv(Q_L * (SL - u1), SL - SM);
+        const T Q_RR = dv(Q_R * (SR - u2), SR - SM);
+        const T ttL = ro_L * suL * (SL - SM) - bbn;
+        const T ttR = ro_R * suR * (SR - SM) - bbn;
+        T vLL = v1, wLL = w1, btt1 = T(0), bmm1 = T(0);
+        T vRR = v2, wRR = w2, btt2 = T(0), bmm2 = T(0);
+        if (fabs(ttL) >= T(0.000001)) {
+            vLL = v1 - dv(bn * bt1 * (SM - u1), ttL);
+            wLL = w1 - dv(bn * bm1 * (SM - u1), ttL);
+            btt1 = dv(bt1 * (ro_L * suL * suL - bbn), ttL);
+            bmm1 = dv(bm1 * (ro_L * suL * suL - bbn), ttL);
+        }
+        if (fabs(ttR) >= T(0.000001)) {
+            vRR = v2 - dv(bn * bt2 * (SM - u2), ttR);
+            wRR = w2 - dv(bn * bm2 * (SM - u2), ttR);
+            btt2 = dv(bt2 * (ro_R * suR * suR - bbn), ttR);
+            bmm2 = dv(bm2 * (ro_R * suR * suR - bbn), ttR);
+        }
+        const T eLL = dv(e1 * suL + PTT * SM - pTL * u1 + bn * ((u1 * bn1 + v1 * bt1 + w1 * bm1) - (SM * bn + vLL * btt1 + wLL * bmm1)), SL - SM);
+        const T eRR = dv(e2 * suR + PTT * SM - pTR * u2 + bn * ((u2 * bn2 + v2 * bt2 + w2 * bm2) - (SM * bn + vRR * btt2 + wRR * bmm2)), SR - SM);
+        const T sLL = sr(ro_LL), sRR = sr(ro_RR);
+        T SLL = SM - dv(fabs(bn), sLL);
+        T SRR = SM + dv(fabs(bn), sRR);
+        T idbn;
+        if (fabs(bn) > T(0.001)) {
+            idbn = T(1) * sgn(bn);
+        } else {
+            idbn = T(0); SLL = SM; SRR = SM;
+        }
+        const T vLLL = dv(sLL * vLL + sRR * vRR + idbn * (btt2 - btt1), sLL + sRR);
+        const T wLLL = dv(sLL * wLL + sRR * wRR + idbn * (bmm2 - bmm1), sLL + sRR);
+        const T bttt = dv(sLL * btt2 + sRR * btt1 + idbn * sLL * sRR * (vRR - vLL), sLL + sRR);
+        const T bmmm = dv(sLL * bmm2 + sRR * bmm1 + idbn * sLL * sRR * (wRR - wLL), sLL + sRR);
+        const T eLLL = eLL - idbn * sLL * ((SM * bn + vLL * btt1 + wLL * bmm1) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+        const T eRRR = eRR + idbn * sRR * ((SM * bn + vRR * btt2 + wRR * bmm2) - (SM * bn + vLLL * bttt + wLLL * bmmm));
+        if (SM >= W && SLL >= W) {
+            const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
 #pragma unroll
-        for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
-        bnW = bn;
+            for (int i = 0; i < 9; i++) PO[i] = FL[i] + SL * ULL[i] - SL * UL[i] - W * ULL[i];
+            bnW = bn;
+        } else if (SM >= W) {
+            const T ULL[9] = { ro_LL, ro_LL * SM, ro_LL * vLL, ro_LL * wLL, eLL, bn, btt1, bmm1, Q_LL };
+            const T ULLL[9] = { ro_LL, ro_LL * SM, ro_LL * vLLL, ro_LL * wLLL, eLLL, bn, bttt, bmmm, Q_LL };
+#pragma unroll
+            for (int i = 0; i < 9; i++) PO[i] = FL[i] + SLL * ULLL[i] - (SLL - SL) * ULL[i] - SL * UL[i] - W * ULLL[i];
+            bnW = bn;
+        } else if (SRR <= W) {
+            const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+#pragma unroll
+            for (int i = 0; i < 9; i++) PO[i] = FR[i] + SR * URR[i] - SR * UR[i] - W * URR[i];
+            bnW = bn;
+        } else {
+            const T URR[9] = { ro_RR, ro_RR * SM, ro_RR * vRR, ro_RR * wRR, eRR, bn, btt2, bmm2, Q_RR };
+            const T URRR[9] = { ro_RR, ro_RR * SM, ro_RR * vLLL, ro_RR * wLLL, eRRR, bn, bttt, bmmm, Q_RR };
+#pragma unroll
+            for (int i = 0; i < 9; i++) PO[i] = FR[i] + SRR * URRR[i] - (SRR - SR) * URR[i] - SR * UR[i] - W * URRR[i];
+            bnW = bn;
+        }
     }
     const T SN = mx(fabs(SL), fabs(SR));
     PO[5] = -SN * (bn2 - bn1) - W * bnW;
