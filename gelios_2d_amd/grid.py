@@ -111,7 +111,9 @@ def contour(ni: int, nj: int) -> np.ndarray:
 @dataclass
 class GeneralMesh:
     """Per-cell face slots in Cell::Grans order (slot = cell*MAXF + k): nodes A, B as
-    seen from the cell, Gran_type, Sosed, Sosed_down, Sosed_up (-1 when absent)."""
+    seen from the cell, Gran_type, Sosed, Sosed_down, Sosed_up (-1 when absent).
+    contour: [ncells, 4] corner nodes of quads, or with coff the corners of polygons,
+    cell c's at contour[coff[c]:coff[c+1]] (Cell::contour order)."""
     contour: np.ndarray
     cell_type: np.ndarray
     nface: np.ndarray
@@ -121,6 +123,7 @@ class GeneralMesh:
     fnbr: np.ndarray
     fdown: np.ndarray
     fup: np.ndarray
+    coff: np.ndarray = None
 
 
 def pack_general(g: PolarGrid) -> GeneralMesh:

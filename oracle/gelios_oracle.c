@@ -129,6 +129,71 @@ double ora_cell_volume_rotate(const double* px, const double* py, const int* q, 
     return fabs(A) * (1.0 / (6.0 * A)) * G * PI_ * angle / 180.0;
 }
 
+/* Cell geometry of cell c of a mesh (quads or polygons), on node positions px, py. */
+static inline int mesh_ncorner(const ora_mesh* m, int c) { return m->coff ? m->coff[c + 1] - m->coff[c] : 4; }
+static inline const int* mesh_corners(const ora_mesh* m, int c) { return m->contour + (m->coff ? m->coff[c] : 4 * c); }
+
+static void mesh_center(const ora_mesh* m, const double* px, const double* py, int c, double* cx, double* cy)
+{   /* Cell::Get_Center (Cell.cpp:66-88) over the cell's contour */
+    const int n = mesh_ncorner(m, c);
+    const int* q = mesh_corners(m, c);
+    double A = 0.0, Gx = 0.0, Gy = 0.0;
+    for (int k = 0; k < n; k++) {
+        const int a = q[k], b = q[k + 1 == n ? 0 : k + 1];
+        const double cr = px[a] * py[b] - py[a] * px[b];
+        A = A + cr;
+        Gx = Gx + (px[a] + px[b]) * cr;
+        Gy = Gy + (py[a] + py[b]) * cr;
+    }
+    A = A * 0.5;
+    *cx = Gx / (6.0 * A);
+    *cy = Gy / (6.0 * A);
+}
+
+/* Cell::Get_Volume (posle = 0, Cell.cpp:146-186) and Get_Volume_posle (posle = 1,
+ * :208-254): quads by their diagonals; other polygons as the sum of Heron areas of the
+ * triangles (face, centre), the centre being the barycentre before the step and the
+ * mean of the corners after it. */
+static double mesh_area(const ora_mesh* m, const double* px, const double* py, int c, int posle)
+{
+    const int n = mesh_ncorner(m, c);
+    const int* q = mesh_corners(m, c);
+    if (n == 4) return ora_cell_area(px, py, q);
+    double x = 0.0, y = 0.0;
+    if (posle) {
+        for (int k = 0; k < n; k++) { x += px[q[k]]; y += py[q[k]]; }
+        x = x / (1.0 * n);
+        y = y / (1.0 * n);
+    } else {
+        mesh_center(m, px, py, c, &x, &y);
+    }
+    double S = 0.0;
+    for (int k = 0; k < m->nface[c]; k++) {
+        const int s = c * ORA_MAXF + k, A = m->fa[s], B = m->fb[s];
+        const double a1 = sqrt(sq(px[A] - x) + sq(py[A] - y));
+        const double a2 = sqrt(sq(px[B] - x) + sq(py[B] - y));
+        const double a3 = sqrt(sq(px[A] - px[B]) + sq(py[A] - py[B]));
+        const double p = (a1 + a2 + a3) / 2.0;
+        S = S + sqrt(p * (p - a1) * (p - a2) * (p - a3));
+    }
+    return S;
+}
+
+static double mesh_volume_rotate(const ora_mesh* m, const double* px, const double* py, int c, double angle)
+{   /* Cell::Get_Volume_rotate (Cell.cpp:188-206) over the cell's contour */
+    const int n = mesh_ncorner(m, c);
+    const int* q = mesh_corners(m, c);
+    double A = 0.0, G = 0.0;
+    for (int k = 0; k < n; k++) {
+        const int a = q[k], b = q[k + 1 == n ? 0 : k + 1];
+        const double cr = px[a] * py[b] - py[a] * px[b];
+        A = A + cr;
+        G = G + (py[a] + py[b]) * cr;
+    }
+    A = A * 0.5;
+    return fabs(A) * (1.0 / (6.0 * A)) * G * PI_ * angle / 180.0;
+}
+
 /* ---------------------------------------------------------------- face flux --- */
 
 double ora_hllc2d(const ora_gas* L, const ora_gas* R, double W, double n1, double n2,
@@ -433,7 +498,7 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
     double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
     if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
-    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    for (int c = 0; c < nc; c++) mesh_center(m, m->x, m->y, c, &cx[c], &cy[c]);
     sweep_ctx S = { m, prm, now, nc, cx, cy };
 
     double Tmin = 100000000;          /* T[now2], Setka.cpp:10142 */
@@ -444,14 +509,13 @@ int ora_step_mk2(const ora_mesh* m, const gel_params* prm, const double* now,
 #pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
 #endif
     for (int c = 0; c < nc; c++) {
-        const int* q = m->contour + 4 * c;
         const double x = cx[c], y = cy[c];
-        const double vs1 = ora_cell_area(m->x, m->y, q);
-        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double vs1 = mesh_area(m, m->x, m->y, c, 0);
+        const double vs2 = mesh_area(m, m->x2, m->y2, c, 1);
         const double radius = sqrt(sq(x) + sq(y));
         if (radius < prm->r_skip) continue;
-        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
-        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        const double Volume = mesh_volume_rotate(m, m->x, m->y, c, 360.0) / (2.0 * PI_);
+        const double Volume2 = mesh_volume_rotate(m, m->x2, m->y2, c, 360.0) / (2.0 * PI_);
         const ora_gas par1 = load(&S, c);
         const int K1 = (m->cell_type[c] == GEL_C_1);
         double Pot[5] = { 0, 0, 0, 0, 0 };
@@ -900,7 +964,7 @@ int ora_mhd_face(const ora_mesh* m, const gel_params* prm, const double* now, in
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
     double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
     if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
-    for (int i = 0; i < nc; i++) ora_cell_center(m->x, m->y, m->contour + 4 * i, &cx[i], &cy[i]);
+    for (int i = 0; i < nc; i++) mesh_center(m, m->x, m->y, i, &cx[i], &cy[i]);
     msw S = { m, prm, now, nc, cx, cy };
     int bad = 0;
     mhd_face(&S, c, c * ORA_MAXF + k, dt_prev, PS, UU, &bad, LRW);
@@ -922,7 +986,7 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
     double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
     if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
-    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    for (int c = 0; c < nc; c++) mesh_center(m, m->x, m->y, c, &cx[c], &cy[c]);
     msw S = { m, prm, now, nc, cx, cy };
     double Tmin = 100000000;
     long long nrc = 0, npc = 0;
@@ -932,14 +996,13 @@ int ora_step_mhd(const ora_mesh* m, const gel_params* prm, const double* now, do
 #pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
 #endif
     for (int c = 0; c < nc; c++) {
-        const int* q = m->contour + 4 * c;
         const double x = cx[c], y = cy[c];
-        const double vs1 = ora_cell_area(m->x, m->y, q);
-        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double vs1 = mesh_area(m, m->x, m->y, c, 0);
+        const double vs2 = mesh_area(m, m->x2, m->y2, c, 1);
         const double radius = sqrt(sq(x) + sq(y));
         if (radius < prm->r_skip) continue;
-        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
-        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        const double Volume = mesh_volume_rotate(m, m->x, m->y, c, 360.0) / (2.0 * PI_);
+        const double Volume2 = mesh_volume_rotate(m, m->x2, m->y2, c, 360.0) / (2.0 * PI_);
         double par1[MV];
         mload(&S, c, par1);
         double Pot[MV] = { 0 };
@@ -1579,7 +1642,7 @@ int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, dou
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
     double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
     if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
-    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    for (int c = 0; c < nc; c++) mesh_center(m, m->x, m->y, c, &cx[c], &cy[c]);
     mfs S = { m, prm, now, nc, cx, cy };
     double Tmin = 100000000;
     long long nrc = 0, npc = 0;
@@ -1589,14 +1652,13 @@ int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, dou
 #pragma omp parallel for num_threads(nthreads) schedule(static) reduction(min:Tmin) reduction(+:nrc,npc) reduction(|:bad_any)
 #endif
     for (int c = 0; c < nc; c++) {
-        const int* q = m->contour + 4 * c;
         const double x = cx[c], y = cy[c];
-        const double vs1 = ora_cell_area(m->x, m->y, q);
-        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double vs1 = mesh_area(m, m->x, m->y, c, 0);
+        const double vs2 = mesh_area(m, m->x2, m->y2, c, 1);
         const double radius = sqrt(sq(x) + sq(y));
         if (radius < prm->r_skip) continue;                       /* Setka.cpp:8997-9000 */
-        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
-        const double Volume2 = ora_cell_volume_rotate(m->x2, m->y2, q, 360.0) / (2.0 * PI_);
+        const double Volume = mesh_volume_rotate(m, m->x, m->y, c, 360.0) / (2.0 * PI_);
+        const double Volume2 = mesh_volume_rotate(m, m->x2, m->y2, c, 360.0) / (2.0 * PI_);
         double par1[MFV];
         mf_load(&S, c, par1);
         const int ct = m->cell_type[c];
@@ -1771,7 +1833,7 @@ int ora_step_inner(const ora_mesh* m, const gel_params* prm, const double* now, 
     double* cx = (double*)malloc(sizeof(double) * (size_t)nc);
     double* cy = (double*)malloc(sizeof(double) * (size_t)nc);
     if (!cx || !cy) { free(cx); free(cy); return GEL_E_NOMEM; }
-    for (int c = 0; c < nc; c++) ora_cell_center(m->x, m->y, m->contour + 4 * c, &cx[c], &cy[c]);
+    for (int c = 0; c < nc; c++) mesh_center(m, m->x, m->y, c, &cx[c], &cy[c]);
     sweep_ctx S = { m, prm, now, nc, cx, cy };
     double Tmin = 100000000;
     long long nrc = 0, npc = 0;
@@ -1781,12 +1843,11 @@ int ora_step_inner(const ora_mesh* m, const gel_params* prm, const double* now, 
 #endif
     for (int c = 0; c < nc; c++) {
         if (m->cell_type[c] == GEL_C_CENTR) continue;             /* Setka.cpp:7025-7028 */
-        const int* q = m->contour + 4 * c;
         const double x = cx[c], y = cy[c];
-        const double vs1 = ora_cell_area(m->x, m->y, q);
-        const double vs2 = ora_cell_area(m->x2, m->y2, q);
+        const double vs1 = mesh_area(m, m->x, m->y, c, 0);
+        const double vs2 = mesh_area(m, m->x2, m->y2, c, 1);
         const double radius = sqrt(sq(x) + sq(y));
-        const double Volume = ora_cell_volume_rotate(m->x, m->y, q, 360.0) / (2.0 * PI_);
+        const double Volume = mesh_volume_rotate(m, m->x, m->y, c, 360.0) / (2.0 * PI_);
         const ora_gas par1 = load(&S, c);
         const int K1 = (m->cell_type[c] == GEL_C_1);
         double Pot[5] = { 0, 0, 0, 0, 0 };

@@ -43,6 +43,9 @@ typedef struct ora_mesh {
     /* node types of the fitted surfaces (0 none, 1 P_Contact, 2 P_Inner_shock,
        3 P_Outer_shock, Point.h:15-17), or NULL: no surfaces */
     const int *ptype;
+    /* polygons: corners of cell c are contour[coff[c] .. coff[c+1]) (Cell::contour);
+       NULL: every cell is a quad with contour[4c .. 4c+4) */
+    const int *coff;
 } ora_mesh;
 
 typedef struct ora_gas { double ro, p, u, v, Q; } ora_gas;

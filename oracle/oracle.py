@@ -27,7 +27,7 @@ class OraMesh(ctypes.Structure):
                 ("x", _dp), ("y", _dp), ("x2", _dp), ("y2", _dp),
                 ("contour", _ip), ("cell_type", _ip), ("nface", _ip),
                 ("fa", _ip), ("fb", _ip), ("ftype", _ip), ("fnbr", _ip), ("fdown", _ip), ("fup", _ip),
-                ("ptype", _ip)]
+                ("ptype", _ip), ("coff", _ip)]
 
 
 class OraMhd(ctypes.Structure):
@@ -161,6 +161,7 @@ class OracleSweep:
         self._keep = [np.ascontiguousarray(a, np.int32) for a in
                       (mesh.contour.ravel(), mesh.cell_type, mesh.nface, mesh.fa, mesh.fb, mesh.ftype,
                        mesh.fnbr, mesh.fdown, mesh.fup)]
+        self._coff = None if mesh.coff is None else np.ascontiguousarray(mesh.coff, np.int32)
         self.n_rho_clamp = ctypes.c_longlong(0)
         self.n_p_clamp = ctypes.c_longlong(0)
         self.dt = prm.dt0
@@ -174,8 +175,9 @@ class OracleSweep:
     def _mesh(self):
         k = self._keep
         pt = _i(self.ptype) if self.ptype is not None else None
+        co = _i(self._coff) if self._coff is not None else None
         return OraMesh(len(self.x), self.nc, _d(self.x), _d(self.y), _d(self.x2), _d(self.y2),
-                       *[_i(a) for a in k], pt)
+                       *[_i(a) for a in k], pt, co)
 
     def set_nodes(self, x, y, x2, y2):
         self.x = np.ascontiguousarray(x, np.float64)

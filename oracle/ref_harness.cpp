@@ -652,9 +652,122 @@ static int mode_resave(const char* in, const char* out)
     return 0;
 }
 
+// ---- the reference's own block grid -------------------------------------------------
+// in : int32 N1 N2 N3 N4 M1 M2 M3 M4 steps
+// argv: in, out, checkpoint path (written)
+// Setka(N1..M4) builds the reference's heliosphere grid (rails of four kinds, C_centr
+// wedges, Setka.cpp:19-1076); Init_conditions fills it (Setka.cpp:5149), the hydrogen
+// moments and k factors are zeroed, and Save_Setka_ALL_ALPHA writes the checkpoint.  A
+// second Setka reads it back with Download_Setka_ALL_ALPHA_2_0 (so both sides of the
+// comparison start from the file's six-digit values), TVD_prepare builds the stencils,
+// and Go_5_komponent__MK2(steps) runs.  The fitted-surface lines are replaced by one
+// stand-in face each (add_standin_lines), so Move_surface moves no node of the grid;
+// Move_Setka_Calculate_2 then places every rail node by its rail's spacing law in the
+// first step (moving the nodes from the file's rounded positions) and leaves them there.
+// out: int32 nn nc; float64 x[nn] y[nn]; int32 ptype[nn] (0, 1 contact, 2 inner shock,
+//      3 outer shock); int32 ncorner[nc]; int32 corners[sum]; int32 cell_type[nc];
+//      int32 nface[nc]; int32 slots[nc][8][6]; float64 prim0[5][nc];
+//      float64 prim[5][nc] after the steps; float64 x[nn] y[nn] after the steps
+static void add_standin_lines_keep(Setka* S)
+{
+    std::vector<Point*> dp;
+    for (int k = 0; k <= 4; k++) { dp.push_back(new Point(1000.0 + k, 1000.0)); dp.push_back(new Point(1000.0 + k, 1001.0)); }
+    std::vector<Cell*> dc;
+    for (int k = 0; k < 4; k++) {
+        Cell* C = new Cell(dp[2 * k], dp[2 * k + 2], dp[2 * k + 3], dp[2 * k + 1]);
+        C->type = C_2;
+        for (int b = 0; b < 2; b++) { C->par[b].ro = 1.0; C->par[b].p = 1.0; C->par[b].u = 0.0; C->par[b].v = 0.0; C->par[b].Q = 1.0; }
+        dc.push_back(C);
+    }
+    Gran* G = new Gran(dp[4], dp[5], Usualy);
+    Gran* G2 = new Gran(dp[5], dp[4], Usualy);
+    G->Master = dc[1]; G->Sosed = dc[2]; G->Sosed_down = dc[0]; G->Sosed_up = dc[3];
+    G2->Master = dc[2]; G2->Sosed = dc[1]; G2->Sosed_down = dc[3]; G2->Sosed_up = dc[0];
+    G->Gran_copy = G2; G2->Gran_copy = G;
+    S->Line_Contact.clear(); S->Line_Inner.clear(); S->Line_Outer.clear();
+    S->Line_Contact.push_back(G);
+    S->Line_Inner.push_back(G);
+    S->Line_Outer.push_back(G);
+}
+
+static void zero_moments(Parametr& p)
+{
+    for (int h = 0; h < 4; h++) { p.H_n[h] = 0.0; p.H_u[h] = 0.0; p.H_v[h] = 0.0; p.H_T[h] = 0.0; }
+    p.k_u = 0.0; p.k_v = 0.0; p.k_T = 0.0;
+    p.I_u = 0.0; p.I_v = 0.0; p.I_T = 0.0;
+}
+
+static int mode_block(const char* in, const char* out, const char* ckpt)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[9]; rd(f, h, 9);
+    std::fclose(f);
+    const int steps = h[8];
+    Setka* S0 = new Setka(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7]);
+    S0->Init_conditions();
+    for (auto* K : S0->All_Cells) { zero_moments(K->par[0]); zero_moments(K->par[1]); }
+    S0->Save_Setka_ALL_ALPHA(ckpt);
+    Setka* S = new Setka();
+    S->Download_Setka_ALL_ALPHA_2_0(ckpt);
+    S->TVD_prepare();
+    add_standin_lines_keep(S);
+
+    const int nn = (int)S->All_Points.size(), nc = (int)S->All_Cells.size();
+    std::vector<double> x(nn), y(nn);
+    std::vector<int32_t> ptype(nn);
+    for (int k = 0; k < nn; k++) {
+        Point* P = S->All_Points[k];
+        P->number = k;
+        x[k] = P->x; y[k] = P->y;
+        ptype[k] = P->type == P_Contact ? 1 : P->type == P_Inner_shock ? 2 : P->type == P_Outer_shock ? 3 : 0;
+    }
+    std::vector<int32_t> ncorner(nc), corners, ctype(nc), nface(nc), slots((size_t)nc * 8 * 6, -1);
+    std::vector<double> prim0((size_t)5 * nc);
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        K->number = c;
+    }
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        ncorner[c] = (int32_t)K->contour.size();
+        for (auto* P : K->contour) corners.push_back(P->number);
+        ctype[c] = (int32_t)K->type;
+        nface[c] = (int32_t)K->Grans.size();
+        if (K->Grans.size() > 8) { std::fprintf(stderr, "cell %d has %zu faces (> 8)\n", c, K->Grans.size()); return 3; }
+        for (size_t k = 0; k < K->Grans.size(); k++) {
+            Gran* G = K->Grans[k];
+            int32_t* sl = &slots[((size_t)c * 8 + k) * 6];
+            sl[0] = G->A->number; sl[1] = G->B->number; sl[2] = (int32_t)G->type;
+            sl[3] = G->Sosed ? G->Sosed->number : -1;
+            sl[4] = G->Sosed_down ? G->Sosed_down->number : -1;
+            sl[5] = G->Sosed_up ? G->Sosed_up->number : -1;
+        }
+        const Parametr& p = K->par[0];
+        prim0[c] = p.ro; prim0[nc + c] = p.p; prim0[2 * nc + c] = p.u; prim0[3 * nc + c] = p.v; prim0[4 * nc + c] = p.Q;
+    }
+    S->Go_5_komponent__MK2(steps);
+    const int fb = steps % 2;
+    std::vector<double> res((size_t)5 * nc), x2(nn), y2(nn);
+    for (int c = 0; c < nc; c++) {
+        const Parametr& p = S->All_Cells[c]->par[fb];
+        res[c] = p.ro; res[nc + c] = p.p; res[2 * nc + c] = p.u; res[3 * nc + c] = p.v; res[4 * nc + c] = p.Q;
+    }
+    for (int k = 0; k < nn; k++) { x2[k] = S->All_Points[k]->x; y2[k] = S->All_Points[k]->y; }
+    f = xopen(out, "wb");
+    const int32_t hh[2] = { nn, nc };
+    wr(f, hh, 2);
+    wr(f, x.data(), nn); wr(f, y.data(), nn); wr(f, ptype.data(), nn);
+    wr(f, ncorner.data(), nc); wr(f, corners.data(), corners.size()); wr(f, ctype.data(), nc);
+    wr(f, nface.data(), nc); wr(f, slots.data(), slots.size());
+    wr(f, prim0.data(), prim0.size()); wr(f, res.data(), res.size());
+    wr(f, x2.data(), nn); wr(f, y2.data(), nn);
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
-    if (argc != 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out\n"); return 2; }
+    if (argc < 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out | block in out checkpoint\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
@@ -666,6 +779,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "inner")) return mode_inner(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "save")) return mode_save(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "resave")) return mode_resave(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "block") && argc >= 5) return mode_block(argv[2], argv[3], argv[4]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

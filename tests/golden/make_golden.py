@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint, block)
 """
 from __future__ import annotations
 
@@ -395,6 +395,60 @@ def gen_checkpoint(rng, tmp):
                         ref_text=np.frombuffer(text, np.uint8))
 
 
+BLOCK_GRIDS = {"a": ((4, 3, 3, 10, 22, 6, 7, 7), 30), "b": ((6, 3, 4, 8, 24, 6, 8, 8), 40)}
+
+
+def parse_block(b: bytes) -> dict:
+    """The harness's block-mode output (oracle/ref_harness.cpp, mode_block)."""
+    nn, nc = np.frombuffer(b, np.int32, 2)
+    off = 8
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(b, dt, n, off).copy()
+        off += a.nbytes
+        return a
+    d = {"x": take(np.float64, nn), "y": take(np.float64, nn), "ptype": take(np.int32, nn)}
+    d["ncorner"] = take(np.int32, nc)
+    d["corners"] = take(np.int32, int(d["ncorner"].sum()))
+    d["cell_type"] = take(np.int32, nc)
+    d["nface"] = take(np.int32, nc)
+    d["slots"] = take(np.int32, nc * 8 * 6).reshape(nc, 8, 6)
+    d["prim0"] = take(np.float64, 5 * nc).reshape(5, nc)
+    d["prim"] = take(np.float64, 5 * nc).reshape(5, nc)
+    d["x_after"] = take(np.float64, nn)
+    d["y_after"] = take(np.float64, nn)
+    assert off == len(b)
+    return d
+
+
+def gen_block(tmp):
+    """The reference's own block grid: Setka(N1..M4) (Setka.cpp:19-1076) filled by
+    Init_conditions, written by Save_Setka_ALL_ALPHA and read back by
+    Download_Setka_ALL_ALPHA_2_0, then TVD_prepare and Go_5_komponent__MK2(steps) with
+    stand-in fitted-surface lines (oracle/ref_harness.cpp, mode_block).  Kept: the
+    checkpoint text, the face tables as the reference holds them, the initial state,
+    the state and node positions after the steps."""
+    ora.build()
+    res = {}
+    for tag, (N, steps) in BLOCK_GRIDS.items():
+        fin, fout, ck = (os.path.join(tmp, f"block_{tag}.{e}") for e in ("in", "out", "txt"))
+        with open(fin, "wb") as f:
+            f.write(np.array(list(N) + [steps], np.int32).tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([ora.REF_HARNESS, "block", fin, fout, ck], check=True, cwd=tmp, env=env,
+                       stdout=subprocess.DEVNULL)
+        with open(fout, "rb") as f:
+            d = parse_block(f.read())
+        with open(ck, "rb") as f:
+            d["ref_text"] = np.frombuffer(f.read(), np.uint8)
+        d["N"] = np.array(N, np.int32)
+        d["steps"] = steps
+        for k, v in d.items():
+            res[f"{tag}_{k}"] = v
+    np.savez_compressed(os.path.join(OUT, "sweep_block.npz"), **res)
+
+
 def main():
     only = sys.argv[1:]
     if only:
@@ -409,6 +463,8 @@ def main():
                 gen_inner(tmp)
             if "mf" in only:
                 gen_multifluid(np.random.default_rng(20261206), tmp)
+            if "block" in only:
+                gen_block(tmp)
         return
     rng = np.random.default_rng(20261203)
     with tempfile.TemporaryDirectory() as tmp:
@@ -422,6 +478,7 @@ def main():
         gen_multifluid(np.random.default_rng(20261206), tmp)
         gen_inner(tmp)
         gen_checkpoint(np.random.default_rng(20261207), tmp)
+        gen_block(tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
