@@ -23,6 +23,16 @@ class GelGrid(ctypes.Structure):
                 ("cell_type", _i8p), ("outer_type", _i8p)]
 
 
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class GelMesh(ctypes.Structure):
+    _fields_ = [("n_nodes", ctypes.c_int), ("n_cells", ctypes.c_int), ("x", _dp), ("y", _dp),
+                ("point_type", _i8p), ("corner_offset", _ip), ("corners", _ip), ("cell_type", _i8p),
+                ("face_offset", _ip), ("face_a", _ip), ("face_b", _ip), ("face_type", _ip), ("face_sosed", _ip),
+                ("face_down", _ip), ("face_up", _ip)]
+
+
 class GelStats(ctypes.Structure):
     _fields_ = [("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double), ("steps", ctypes.c_longlong),
                 ("n_rho_clamp", ctypes.c_longlong), ("n_p_clamp", ctypes.c_longlong),
@@ -38,6 +48,8 @@ SIGNATURES = {
     "gel_params_reference": (None, [ctypes.POINTER(GelParams)]),
     "gel_ctx_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GelGrid),
                                       ctypes.POINTER(GelParams)]),
+    "gel_ctx_create_mesh": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(GelMesh),
+                                           ctypes.POINTER(GelParams)]),
     "gel_ctx_destroy": (None, [ctypes.c_void_p]),
     "gel_upload_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_download_state": (ctypes.c_int, [ctypes.c_void_p, _dp]),

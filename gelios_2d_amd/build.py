@@ -9,7 +9,8 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgelios.so")
-SOURCES = ["gelios_sweep.hip", "sweep_fp32.hip", "surrogate.hip", "tracking.hip", "multifluid.hip", "inner.hip"]
+SOURCES = ["gelios_sweep.hip", "sweep_fp32.hip", "surrogate.hip", "tracking.hip", "multifluid.hip", "inner.hip",
+           "mesh.hip"]
 HIPCC = "/opt/rocm/bin/hipcc"
 
 HIPCC_FLAGS = [

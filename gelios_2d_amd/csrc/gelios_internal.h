@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
 #include <utility>
 #include <vector>
 
@@ -79,7 +80,13 @@ hipError_t gel_live_list_build(hipStream_t s, const gel::SweepArgs* d_args, cons
                                long long ncells, int2* d_list, unsigned* d_count);
 
 struct gel_ctx;
+struct MeshBuf;                   // device face tables of a mesh context (mesh.hip)
+// Build / free the face tables of a general mesh (mesh.hip); *err names a rejected input.
+int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err);
+void gel_mesh_free(gel_ctx* c);
 namespace gel {
+// One step of the general-mesh sweep (mesh.hip); moving: next node positions in a.x2, a.y2.
+hipError_t gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving);
 // One tracked step (tracking.hip): surface node speeds, rail redistribution into
 // ctx->d_x2/d_y2, and the sweep with surface faces; asynchronous on ctx->stream.
 hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a);
@@ -123,6 +130,8 @@ struct gel_ctx {
     std::vector<double> motion;
     long long motion_step = 0;
     hipStream_t stream = nullptr;
+    MeshBuf* mesh = nullptr;      // a general mesh (gel_ctx_create_mesh) instead of a polar grid
+    int mesh_faces = 0;
     int timing = 0;       // 1: events around every launch; 2: around every gel_step call
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     std::vector<long long> event_steps;   // steps each event pair spans

@@ -134,6 +134,7 @@ static void free_ctx(gel_ctx* c)
     (void)hipFree(c->d_geo);
     (void)hipFree(c->d_trk);
     (void)hipFree(c->d_dt); (void)hipFree(c->d_cnt);
+    gel_mesh_free(c);
     for (auto& e : c->events) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -218,6 +219,58 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
         }
     }
     // the context's stream does not synchronise with the null stream: clear on it
+    if ((e = hipMemsetAsync(c->d_cnt, 0, 3 * sizeof(unsigned long long), c->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+        return bad(e, "memset");
+    *out = c;
+    return GEL_OK;
+}
+
+int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* g, const gel_params* prm)
+{
+    if (!out || !g || !prm || !g->x || !g->y || !g->corner_offset || !g->corners || !g->cell_type ||
+        !g->face_offset || !g->face_a || !g->face_b || !g->face_type || !g->face_sosed || !g->face_down || !g->face_up)
+        return fail(GEL_E_ARG, "null argument");
+    *out = nullptr;
+    if (g->n_cells < 1 || g->n_nodes < 3) return fail(GEL_E_ARG, "empty mesh");
+    if (g->corner_offset[0] != 0 || g->face_offset[0] != 0) return fail(GEL_E_ARG, "offsets start at 0");
+    for (int k = 0; k < g->n_cells; k++)
+        if (g->corner_offset[k + 1] < g->corner_offset[k] || g->face_offset[k + 1] < g->face_offset[k])
+            return fail(GEL_E_ARG, "offsets decrease");
+    if (prm->physics != GEL_PHYS_GAS || prm->precision != GEL_PREC_FP64)
+        return fail(GEL_E_UNSUPPORTED, "general meshes: gas physics in FP64");
+    if (prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC) return fail(GEL_E_UNSUPPORTED, "flux");
+    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1)
+        return fail(GEL_E_UNSUPPORTED, "general meshes: no source or the MK1 source");
+    if (!(prm->dt0 > 0.0) || !(prm->cfl > 0.0) || !(prm->gamma > 1.0)) return fail(GEL_E_ARG, "dt0, cfl, gamma");
+    gel_ctx* c = new (std::nothrow) gel_ctx();
+    if (!c) return fail(GEL_E_NOMEM, "host allocation");
+    c->nc = (size_t)g->n_cells;
+    c->nn = (size_t)g->n_nodes;
+    c->nv = 5;
+    c->ts = 8;
+    c->prm = *prm;
+    c->has_types = true;
+    auto bad = [&](hipError_t e, const char* what) {
+        free_ctx(c);
+        return fail(e == hipErrorOutOfMemory ? GEL_E_NOMEM : GEL_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    };
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bad(e, "stream");
+    const size_t nb = c->nn * sizeof(double);
+    if ((e = hipMalloc(&c->d_x, nb)) != hipSuccess) return bad(e, "nodes");
+    if ((e = hipMalloc(&c->d_y, nb)) != hipSuccess) return bad(e, "nodes");
+    if ((e = hipMalloc(&c->d_x2, nb)) != hipSuccess) return bad(e, "nodes");
+    if ((e = hipMalloc(&c->d_y2, nb)) != hipSuccess) return bad(e, "nodes");
+    for (int b = 0; b < 2; b++)
+        if ((e = hipMalloc(&c->d_state[b], (size_t)c->nv * c->nc * c->ts)) != hipSuccess) return bad(e, "state");
+    if ((e = hipMalloc(&c->d_dt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
+    if ((e = hipMalloc(&c->d_cnt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "counters");
+    if ((e = hipMemcpy(c->d_x, g->x, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
+    if ((e = hipMemcpy(c->d_y, g->y, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
+    std::string err;
+    const int st = gel_mesh_build(c, g, err);
+    if (st != GEL_OK) { free_ctx(c); return fail(st, err); }
     if ((e = hipMemsetAsync(c->d_cnt, 0, 3 * sizeof(unsigned long long), c->stream)) != hipSuccess ||
         (e = hipStreamSynchronize(c->stream)) != hipSuccess)
         return bad(e, "memset");
@@ -312,6 +365,7 @@ int gel_set_sources(gel_ctx* c, const double* f)
 
 int gel_set_source_table(gel_ctx* c, const double* q)
 {
+    if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "source tables: polar grids");
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
     // stored in the state's type, like the moments; ordered after the work queued on
@@ -363,6 +417,7 @@ int gel_get_source_flags(gel_ctx* c, int* flags)
 
 int gel_set_rows_per_wave(gel_ctx* c, int rows)
 {
+    if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "rows per chunk: polar grids");
     if (!c || rows < 1 || rows > 4096) return fail(GEL_E_ARG, "rows");
     c->rows = rows;
     return GEL_OK;
@@ -370,6 +425,7 @@ int gel_set_rows_per_wave(gel_ctx* c, int rows)
 
 int gel_set_motion(gel_ctx* c, double amp, int period, double phase)
 {
+    if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "prescribed motion: polar grids (general meshes: gel_step_moving)");
     if (!c || period < 1 || !(amp >= 0.0) || !(amp < 0.5)) return fail(GEL_E_ARG, "motion: amp in [0,0.5), period >= 1");
     if (c->prm.physics == GEL_PHYS_GAS_INNER) return fail(GEL_E_UNSUPPORTED, "the inner-region solver is static");
     if (c->tracking) return fail(GEL_E_STATE, "fitted-surface tracking is on");
@@ -400,6 +456,7 @@ void gel_tracking_reference(gel_tracking* t)
 
 int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
 {
+    if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "fitted-surface tracking: polar grids");
     if (!c) return fail(GEL_E_ARG, "null argument");
     if (!t) { c->tracking = false; return GEL_OK; }
     if (!rail) return fail(GEL_E_ARG, "rail angles needed");
@@ -515,7 +572,7 @@ static int ensure_geometry(gel_ctx* c)
 // 3 fitted-surface tracking (next positions computed into d_x2/d_y2, then committed)
 static int one_step(gel_ctx* c, int moving)
 {
-    if (moving == 0) {
+    if (moving == 0 && !c->mesh) {
         const int st = ensure_geometry(c);
         if (st != GEL_OK) return st;
     }
@@ -557,7 +614,8 @@ static int one_step(gel_ctx* c, int moving)
         HIP_OK(hipEventCreate(&e1));
         HIP_OK(hipEventRecord(e0, c->stream));
     }
-    hipError_t e = (moving == 3) ? launch_tracked_step(c, a) : launch(c, a, moving != 0);
+    hipError_t e = c->mesh ? gel_mesh_step(c, a, moving != 0)
+                           : (moving == 3) ? launch_tracked_step(c, a) : launch(c, a, moving != 0);
     if (e != hipSuccess) return fail(GEL_E_HIP, std::string("sweep launch: ") + hipGetErrorString(e));
     if (c->timing == 1) {
         HIP_OK(hipEventRecord(e1, c->stream));
@@ -1085,6 +1143,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     *out = nullptr;
     for (int k = 0; k < ncases; k++) {
         if (!cases[k]) return fail(GEL_E_ARG, "null case");
+        if (cases[k]->mesh) return fail(GEL_E_UNSUPPORTED, "batched cases are polar grids");
         // one context twice would have two sweeps of a step write its one next buffer
         for (int l = 0; l < k; l++)
             if (cases[l] == cases[k]) return fail(GEL_E_ARG, "a context appears twice in the batch");

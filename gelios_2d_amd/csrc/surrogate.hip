@@ -1056,6 +1056,7 @@ int gel_mlp_set_scaling(gel_mlp* m, const float* mu, const float* sd, const floa
 static int check_ctx(gel_ctx* c)
 {
     if (!c) return mfail(GEL_E_ARG, "null context");
+    if (c->mesh) return mfail(GEL_E_UNSUPPORTED, "the learned source runs on polar grids");
     if (!c->uploaded) return mfail(GEL_E_STATE, "no state uploaded");
     if (!c->d_src) return mfail(GEL_E_STATE, "hydrogen moments not set (gel_set_sources)");
     return GEL_OK;

@@ -14,7 +14,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .grid import PolarGrid
+from .grid import MAXF, PolarGrid
 from .params import GEL_MK1_NFIELDS, GEL_PHYS_GAS4H, GEL_PHYS_MHD, NVAR_GAS, NVAR_GAS4H, NVAR_MHD, GelParams
 
 
@@ -45,6 +45,34 @@ class Setka:
             self.close()
         except Exception:
             pass
+
+    @classmethod
+    def from_mesh(cls, mesh, params: GelParams) -> "Setka":
+        """A context on a general mesh (:class:`~gelios_2d_amd.mesh.Mesh`: the
+        reference's own block grid, or one read from its checkpoint): the FP64 gas
+        sweep, static (Go_5_komponent__MK2) or with next node positions (step_moving)."""
+        self = cls.__new__(cls)
+        self.grid = mesh
+        self.params = params.copy()
+        self._ctx = ctypes.c_void_p()
+        nc = mesh.ncells
+        off = np.zeros(nc + 1, np.int32)
+        off[1:] = np.cumsum(mesh.nface)
+        keep = dict(x=np.ascontiguousarray(mesh.x, np.float64), y=np.ascontiguousarray(mesh.y, np.float64),
+                    pt=np.ascontiguousarray(mesh.ptype, np.int8), coff=np.ascontiguousarray(mesh.coff, np.int32),
+                    corners=np.ascontiguousarray(mesh.corners, np.int32),
+                    ct=np.ascontiguousarray(mesh.cell_type, np.int8), foff=off)
+        sel = np.concatenate([np.arange(c * MAXF, c * MAXF + mesh.nface[c]) for c in range(nc)])
+        for k in ("fa", "fb", "ftype", "fnbr", "fdown", "fup"):
+            keep[k] = np.ascontiguousarray(getattr(mesh, k)[sel], np.int32)
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))   # noqa: E731
+        g = _lib.GelMesh(mesh.nnodes, nc, _lib.dptr(keep["x"]), _lib.dptr(keep["y"]), _lib.i8ptr(keep["pt"]),
+                         ip(keep["coff"]), ip(keep["corners"]), _lib.i8ptr(keep["ct"]), ip(keep["foff"]),
+                         *[ip(keep[k]) for k in ("fa", "fb", "ftype", "fnbr", "fdown", "fup")])
+        _lib.check(_lib.lib().gel_ctx_create_mesh(ctypes.byref(self._ctx), ctypes.byref(g), ctypes.byref(self.params)))
+        self.stats = _lib.GelStats()
+        self.nvar = NVAR_GAS
+        return self
 
     @property
     def ncells(self) -> int:

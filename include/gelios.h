@@ -171,6 +171,39 @@ typedef struct gel_ctx gel_ctx;
  * Replaces the set-up the reference does before its sweep (Setka::TVD_prepare,
  * Setka.cpp:1294-1492, builds the face stencils that are implicit here). */
 int gel_ctx_create(gel_ctx** out, const gel_grid* grid, const gel_params* prm);
+
+/* The reference's own grid: any polygon mesh with the face tables Setka::TVD_prepare
+ * leaves (Setka.cpp:1294-1434) -- the block-structured heliosphere grid of Setka(N1..M4)
+ * (Setka.cpp:19-1076: rails of four kinds, fitted-surface rows, C_centr wedges), or one
+ * read from a checkpoint by Setka::Download_Setka_ALL_ALPHA_2_0 (Setka.cpp:2899-3258).
+ * Cell c's corners are corners[corner_offset[c] .. corner_offset[c+1]) (Cell::contour),
+ * its faces the slots face_offset[c] .. face_offset[c+1]) in Cell::Grans order, each
+ * with the face's nodes A, B as the cell sees them (Gran::A, B), Gran::type, Sosed,
+ * Sosed_down and Sosed_up (-1 where absent).  Interior faces must come as a face and its
+ * copy (nodes reversed, Sosed_down and Sosed_up exchanged), as the reference builds
+ * them.  point_type: 0, 1 P_Contact, 2 P_Inner_shock, 3 P_Outer_shock (NULL: none); a
+ * face between two nodes of one fitted surface takes the exact solver, a face with one
+ * such node the nul_potok flux (Setka.cpp:10287-10370).  The mesh context runs the
+ * FP64 gas sweep (HLL / HLLC, GEL_SRC_NONE or GEL_SRC_MK1) with gel_step (static) and
+ * gel_step_moving (next node positions from the caller, as Move_Setka_Calculate_2 leaves
+ * them); state and sources as for polar grids, n_cells cells in the mesh's order. */
+typedef struct gel_mesh {
+    int n_nodes, n_cells;
+    const double* x;
+    const double* y;
+    const int8_t* point_type;
+    const int* corner_offset;
+    const int* corners;
+    const int8_t* cell_type;
+    const int* face_offset;
+    const int* face_a;
+    const int* face_b;
+    const int* face_type;
+    const int* face_sosed;
+    const int* face_down;
+    const int* face_up;
+} gel_mesh;
+int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* mesh, const gel_params* params);
 void gel_ctx_destroy(gel_ctx* ctx);
 
 /* Copy the primitive state (variable-major SoA) to the device; resets the time step

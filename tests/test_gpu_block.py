@@ -1,0 +1,77 @@
+"""The general-mesh sweep on the reference's own block grid: the grid and state read
+from the reference's checkpoint (Setka::Save_Setka_ALL_ALPHA of Setka(N1..M4) filled by
+Init_conditions), stepped on the device, against the reference's Go_5_komponent__MK2 on
+the same file (tests/golden/sweep_block.npz)."""
+import numpy as np
+import pytest
+
+from gelios_2d_amd.mesh import mesh_from_alpha, state_from_alpha
+from gelios_2d_amd.params import GEL_FLUX_HLLC, reference_params
+from gelios_2d_amd.state import rel_err_conserved
+from test_oracle_block import GRIDS, alpha, block
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.mark.parametrize("tag", GRIDS)
+def test_block_grid_sweep_matches_reference(tag):
+    """The first step moves the nodes to their rails' spacing (next positions as
+    Move_Setka_Calculate_2 leaves them, gel_step_moving), the others are static
+    (gel_step); fitted-surface faces take the exact solver, C_1 cells the radial
+    reconstruction, the C_centr wedges stay frozen."""
+    from gelios_2d_amd.setka import Setka
+    d = block(tag)
+    a = alpha(d)
+    m = mesh_from_alpha(a)
+    s = Setka.from_mesh(m, reference_params())
+    s.upload(state_from_alpha(a))
+    s.step_moving(d["x_after"], d["y_after"])
+    st = s.Go_5_komponent__MK2(int(d["steps"]) - 1)
+    got = s.download()
+    err = rel_err_conserved(got, d["prim"])
+    x, y = s.get_nodes()
+    print(f"block grid {tag} ({m.ncells} cells, {int(d['steps'])} steps): max rel err vs reference {err:.3e}, "
+          f"dt {st.dt_next:.6e}, numeric {st.n_numeric}", flush=True)
+    assert np.array_equal(x, d["x_after"]) and np.array_equal(y, d["y_after"])
+    assert err <= TOL
+    s.close()
+
+
+@pytest.mark.parametrize("flux", [0, GEL_FLUX_HLLC])
+def test_block_grid_matches_oracle(oracle_lib, flux):
+    """HLL and HLLC (with the nul_potok faces beside the fitted surfaces) on the block
+    grid against the oracle, which solves every face from both sides."""
+    from gelios_2d_amd.setka import Setka
+    from test_oracle_block import run_oracle
+    d = block("b")
+    a = alpha(d)
+    m = mesh_from_alpha(a)
+    prm = reference_params()
+    prm.flux = flux
+    steps = 25
+    sw = oracle_lib.OracleSweep(m.general(), d["x"], d["y"], prm)
+    sw.ptype = np.ascontiguousarray(m.ptype, np.int32)
+    sw.set_nodes(d["x"], d["y"], d["x_after"], d["y_after"])
+    cur, _ = sw.step(d["prim0"])
+    sw.set_nodes(d["x_after"], d["y_after"], d["x_after"], d["y_after"])
+    cur = sw.run(cur, steps - 1)
+    s = Setka.from_mesh(m, prm)
+    s.upload(d["prim0"])
+    s.step_moving(d["x_after"], d["y_after"])
+    s.Go_5_komponent__MK2(steps - 1)
+    assert rel_err_conserved(s.download(), cur) <= TOL
+    s.close()
+
+
+def test_mesh_rejects_unpaired_faces():
+    """An interior face whose other cell does not list its copy is an error."""
+    from gelios_2d_amd._lib import GelError
+    from gelios_2d_amd.setka import Setka
+    d = block("a")
+    m = mesh_from_alpha(alpha(d))
+    k = int(np.flatnonzero(m.ftype == 0)[5])
+    m.fdown[k] = (m.fdown[k] + 1) % m.ncells
+    with pytest.raises(GelError):
+        Setka.from_mesh(m, reference_params())
