@@ -34,6 +34,13 @@ Workloads
                    frozen hydrogen populations, FP64 plain beside FP32 bulk, 2000 steps
   plasma-4h-800x400  the plasma and four hydrogen fluids (21 fields, HLLC, charge
                    exchange), 800 x 400 cells, static, FP64, 200 steps
+  mesh-800x400     the gas heliosphere (HLLC, twenty rows of C_1 wind cells, FP64, 200
+                   steps) given as a general mesh: the general-mesh kernels (face table,
+                   per-face and per-cell launches); structured-800x400 runs the same on
+                   the structured marching kernel
+  block-731        the reference's own block grid (Setka(6,3,4,8,24,6,8,8), 731 cells, read
+                   from its checkpoint in tests/golden/sweep_block.npz), HLL, the
+                   reference's constants, 200 steps
 """
 from __future__ import annotations
 
@@ -97,7 +104,7 @@ def run_gpu(w, K, W, label):
     device time of back-to-back launches)."""
     from gelios_2d_amd.setka import Setka
     nc = w.grid.ncells
-    s = Setka(w.grid, w.params)
+    s = Setka.from_mesh(w.mesh, w.params) if getattr(w, "mesh", None) is not None else Setka(w.grid, w.params)
     if getattr(w, "motion", None):
         s.set_motion(*w.motion)
     s.upload(w.prim0)
@@ -384,12 +391,17 @@ def cpu_baseline_workload(w, budget_s=12.0):
     """The CPU restatement of the reference sweep (per-cell gather over faces, every
     interior face solved from both sides, FP64, OpenMP over cells, as
     Setka.cpp:10148) on the host cores, over a bounded number of steps."""
+    import numpy as np
     from oracle import oracle as ora
     threads = cpu_threads()
     motion = getattr(w, "motion", None)
     if getattr(w, "tracking", None) is not None:
         tr = ora.TrackedOracle(w.grid, w.params, w.tracking, w.rails)
         sw = tr.sw
+    elif getattr(w, "mesh", None) is not None:
+        tr = None
+        sw = ora.OracleSweep(w.mesh.general(), w.mesh.x, w.mesh.y, w.params)
+        sw.ptype = np.ascontiguousarray(w.mesh.ptype, np.int32)
     else:
         tr = None
         sw = ora.OracleSweep.from_polar(w.grid, w.params)
@@ -556,6 +568,27 @@ def main():
             w = configs.large()
         elif wl == "plasma-4h-800x400":
             w = configs.multifluid_heliosphere(800, 400, 200, seed=9, name="plasma-4h-hllc-fp64-800x400")
+        elif wl in ("mesh-800x400", "structured-800x400"):
+            from gelios_2d_amd.mesh import mesh_from_polar
+            w = configs.heliosphere(800, 400, 200, seed=1, flux=1, name=f"hllc-fp64-c1-800x400-{wl.split('-')[0]}")
+            w.grid.cell_type[: 20 * w.grid.nj] = 0                  # twenty rows of C_1 wind cells
+            if wl == "mesh-800x400":
+                w.mesh = mesh_from_polar(w.grid)
+        elif wl == "block-731":
+            from gelios_2d_amd.checkpoint import read_alpha
+            from gelios_2d_amd.mesh import mesh_from_alpha, state_from_alpha
+            from gelios_2d_amd.params import reference_params
+            import tempfile
+            import numpy as np
+            d = np.load(os.path.join(ROOT, "tests", "golden", "sweep_block.npz"))
+            with tempfile.NamedTemporaryFile("wb", suffix=".txt") as f:
+                f.write(d["b_ref_text"].tobytes())
+                f.flush()
+                alpha = read_alpha(f.name)
+            m = mesh_from_alpha(alpha)
+            w = configs.Workload("reference-block-grid-731-cells-hll-fp64", m, reference_params(),
+                                 state_from_alpha(alpha), 200, None)
+            w.mesh = m
         elif wl == "tracked-800x400":
             w = configs.tracked_heliosphere(400, (200, 150, 150, 300), 200, seed=8, name="tracked-hll-fp64-799x400")
         else:
@@ -580,7 +613,8 @@ def main():
         "dtype": dtype,
         "data": "synthetic",
         "config": config if config is not None else {
-            "workload": main_w.name, "cells": main_w.grid.ncells, "grid": [main_w.grid.ni, main_w.grid.nj],
+            "workload": main_w.name, "cells": main_w.grid.ncells,
+            "grid": [main_w.grid.ni, main_w.grid.nj] if hasattr(main_w.grid, "ni") else "general mesh",
             "physics": {1: "mhd-hlld", 2: "plasma+4H-hllc"}.get(main_w.params.physics,
                                                                  "gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
             "moving_grid": bool(getattr(main_w, "motion", None) or getattr(main_w, "tracking", None)),

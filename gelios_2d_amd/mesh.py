@@ -150,6 +150,19 @@ def mesh_from_alpha(m) -> Mesh:
     return Mesh(x, y, pt, coff, corners, m.cell_type.astype(np.int32), nface, **slots)
 
 
+def mesh_from_polar(g) -> Mesh:
+    """A polar grid (:class:`~gelios_2d_amd.grid.PolarGrid`) in the general form, with the
+    face tables of grid.pack_general (TVD_prepare's on such grids)."""
+    from .grid import pack_general
+    gm = pack_general(g)
+    nc = g.ncells
+    return Mesh(np.ascontiguousarray(g.x, np.float64), np.ascontiguousarray(g.y, np.float64),
+                np.zeros(g.nnodes, np.int32), np.arange(0, 4 * nc + 1, 4, dtype=np.int32),
+                np.ascontiguousarray(gm.contour.ravel(), np.int32), g.cell_type.astype(np.int32),
+                np.ascontiguousarray(gm.nface, np.int32), *[np.ascontiguousarray(getattr(gm, k), np.int32)
+                                                           for k in ("fa", "fb", "ftype", "fnbr", "fdown", "fup")])
+
+
 def state_from_alpha(m) -> np.ndarray:
     """The plasma state (ro, p, u, v, Q) x ncells of a checkpoint."""
     return np.ascontiguousarray(m.state[:, 0:5].T, np.float64)
