@@ -444,7 +444,7 @@ void gel_tracking_reference(gel_tracking* t)
     t->row_inner_shock = t->row_contact = t->row_outer_shock = -1;
     t->outer_faces = 1 << 30;
     t->average_lines = 0;
-    t->n_inner = 20;                                   // Help.h:66
+    t->inner_rail_nodes = 20;                                   // Help.h:66
     t->r_inner = 1.0 / RR;                             // R1_, R111_, R5_ (Help.h:76-83)
     t->r_mid = 50.0 / RR;
     t->r_outer = 2200.0 / RR;
@@ -467,12 +467,12 @@ int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
     if (!c->motion.empty()) return fail(GEL_E_STATE, "prescribed motion is set");
     const int ni = c->ni, nj = c->nj;
     const int r0 = t->row_inner_shock, r1 = t->row_contact, r2 = t->row_outer_shock;
-    if (nj < 2 || t->n_inner < 2 || r0 < t->n_inner || r0 < 2 || !(r0 < r1 && r1 < r2) || r2 > ni - 2)
-        return fail(GEL_E_ARG, "surface rows: n_inner <= inner shock < contact < outer shock <= ni - 2");
+    if (nj < 2 || t->inner_rail_nodes < 2 || r0 < t->inner_rail_nodes || r0 < 2 || !(r0 < r1 && r1 < r2) || r2 > ni - 2)
+        return fail(GEL_E_ARG, "surface rows: inner_rail_nodes <= inner shock < contact < outer shock <= ni - 2");
     if (t->outer_faces < 0) return fail(GEL_E_ARG, "outer_faces must be >= 0");
     const int M1 = r0 + 1, M4 = ni - r2;
-    const int nmid = M1 - 1 - t->n_inner;
-    const size_t n_cs = (size_t)(nj + 1), n_in = (size_t)t->n_inner, n_mid = (size_t)(nmid > 0 ? nmid : 0);
+    const int nmid = M1 - 1 - t->inner_rail_nodes;
+    const size_t n_cs = (size_t)(nj + 1), n_in = (size_t)t->inner_rail_nodes, n_mid = (size_t)(nmid > 0 ? nmid : 0);
     const size_t n_out = (size_t)(nj + 1) * M4, n_face = (size_t)3 * nj * 6;
     std::vector<double> h(2 * n_cs + n_in + n_mid + n_out);
     double* cs = h.data();
@@ -488,9 +488,9 @@ int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
         const double ex = t->p_outer * (1.0 - (rail[k] / dd) * (rail[k] / dd));
         for (int j = 0; j < M4; j++) fout[(size_t)k * M4 + j] = std::pow((j + 1) * 1.0 / (M4), ex);
     }
-    for (int j = 0; j < t->n_inner; j++) fin[j] = std::pow(j * 1.0 / (t->n_inner), t->p_inner);
-    for (int j = t->n_inner; j < M1 - 1; j++)
-        fmid[j - t->n_inner] = std::pow((j - t->n_inner) * 1.0 / (M1 - 1 - t->n_inner), t->p_mid);
+    for (int j = 0; j < t->inner_rail_nodes; j++) fin[j] = std::pow(j * 1.0 / (t->inner_rail_nodes), t->p_inner);
+    for (int j = t->inner_rail_nodes; j < M1 - 1; j++)
+        fmid[j - t->inner_rail_nodes] = std::pow((j - t->inner_rail_nodes) * 1.0 / (M1 - 1 - t->inner_rail_nodes), t->p_mid);
     (void)hipFree(c->d_trk);
     c->d_trk = nullptr;
     c->tracking = false;
@@ -501,7 +501,7 @@ int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
     d.rows[0] = r1; d.rows[1] = r0; d.rows[2] = r2;
     d.outer_faces = t->outer_faces;
     d.average_lines = t->average_lines ? 1 : 0;
-    d.n_inner = t->n_inner;
+    d.n_inner = t->inner_rail_nodes;
     d.m4 = M4;
     d.koef = t->koef;
     d.k_line[0] = t->k_contact; d.k_line[1] = t->k_inner; d.k_line[2] = t->k_outer;

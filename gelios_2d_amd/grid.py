@@ -172,7 +172,7 @@ def rail_radii(t, blocks, s, R2, R3, R4):
     reference's rails (Rail.cpp:19-75)."""
     M1, M2, M3, M4 = blocks
     r = np.empty(M1 + M2 + M3 + M4)
-    n = t.n_inner
+    n = t.inner_rail_nodes
     for j in range(n):
         r[j] = t.r_inner + (j * 1.0 / n) ** t.p_inner * (t.r_mid - t.r_inner)
     for j in range(n, M1 - 1):

@@ -145,7 +145,7 @@ class GelTracking(ctypes.Structure):
         ("row_outer_shock", ctypes.c_int),
         ("outer_faces", ctypes.c_int),
         ("average_lines", ctypes.c_int),
-        ("n_inner", ctypes.c_int),
+        ("inner_rail_nodes", ctypes.c_int),
         ("r_inner", ctypes.c_double),
         ("r_mid", ctypes.c_double),
         ("r_outer", ctypes.c_double),
@@ -179,7 +179,7 @@ def reference_tracking(rows=(-1, -1, -1), outer_faces: int = 1 << 30) -> GelTrac
     t.row_inner_shock, t.row_contact, t.row_outer_shock = rows
     t.outer_faces = outer_faces
     t.average_lines = 0
-    t.n_inner = N_INNER
+    t.inner_rail_nodes = N_INNER
     t.r_inner = 1.0 / RR_            # R1_
     t.r_mid = 50.0 / RR_             # R111_
     t.r_outer = 2200.0 / RR_         # R5_

@@ -266,7 +266,7 @@ int gel_set_motion(gel_ctx* ctx, double amp, int period, double phase);
  *      surface faces (mass, momentum, energy; the tracer from the HLL/HLLC flux,
  *      Setka.cpp:10335-10370) and the nul_potok HLLC on faces with one surface node
  *      (Setka.cpp:10299-10327), and commits the nodes (Setka.cpp:10473-10480).
- * Gas physics in FP64 only.  The rows must satisfy n_inner <= row_inner_shock <
+ * Gas physics in FP64 only.  The rows must satisfy inner_rail_nodes <= row_inner_shock <
  * row_contact < row_outer_shock <= ni - 2, and the grid's rows 0..row_inner_shock
  * must lie on the rails (the step places them by the rail formulas). */
 typedef struct gel_tracking {
@@ -279,7 +279,7 @@ typedef struct gel_tracking {
                                line's nodes after each of the three loops,
                                Setka.cpp:3522-3530, :3718-3731, :3829-3837, so the
                                shocks' nodes keep sums); 1: each line averages its own */
-    int n_inner;            /* rail nodes in the inner block (n_inner = 20, Help.h:66) */
+    int inner_rail_nodes;   /* rail nodes in the inner block (the macro n_inner = 20, Help.h:66) */
     double r_inner, r_mid, r_outer;          /* R1_, R111_, R5_ (Help.h:76-83) */
     double p_inner, p_mid, p_outer, s_outer; /* rail spacing: 1.3, 1.05, 1.76, 1.5708
                                                 (Setka.cpp:4349-4380) */

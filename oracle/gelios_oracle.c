@@ -1449,14 +1449,14 @@ int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, cons
         const int k0 = key[0] * (nj + 1) + c, k1 = key[1] * (nj + 1) + c, k2 = key[2] * (nj + 1) + c;
         const double R2 = sqrt(sq(x2[k0]) + sq(y2[k0]));
         double xx, r;
-        for (int j = 0; j < t->n_inner; j++) {
-            xx = j * 1.0 / (t->n_inner);
+        for (int j = 0; j < t->inner_rail_nodes; j++) {
+            xx = j * 1.0 / (t->inner_rail_nodes);
             r = t->r_inner + pow(xx, t->p_inner) * (t->r_mid - t->r_inner);
             x2[j * (nj + 1) + c] = r * cos(s);
             y2[j * (nj + 1) + c] = r * sin(s);
         }
-        for (int j = t->n_inner; j < M1 - 1; j++) {
-            xx = (j - t->n_inner) * 1.0 / (M1 - 1 - t->n_inner);
+        for (int j = t->inner_rail_nodes; j < M1 - 1; j++) {
+            xx = (j - t->inner_rail_nodes) * 1.0 / (M1 - 1 - t->inner_rail_nodes);
             r = t->r_mid + pow(xx, t->p_mid) * (R2 - t->r_mid);
             x2[j * (nj + 1) + c] = r * cos(s);
             y2[j * (nj + 1) + c] = r * sin(s);

@@ -52,3 +52,21 @@ def test_errors_without_device():
     assert out.value is None
     with pytest.raises(_lib.GelError):
         _lib.check(1)
+
+
+@pytest.mark.skipif(not os.path.exists("/root/reference/Setka.h"), reason="needs the reference's headers")
+def test_integration_binding_compiles_with_reference_headers(tmp_path):
+    """The C++ binding of INTEGRATION.md (a GpuSweep over any Setka through
+    gel_ctx_create_mesh) compiles against the reference's own headers: include/gelios.h
+    uses no name that the reference's macros (Help.h) rewrite."""
+    import subprocess
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    i = txt.index("// GpuSweep.cpp -- replaces the OpenMP cell loop")
+    src = txt[i:txt.index("```", i)]
+    src += ("\nint probe(Setka* S) { GpuSweep g(S); gel_stats st; g.put(S, 0); g.steps(3, &st); "
+            "g.step_moving(S, &st); g.get(S, 1); return 0; }\n")
+    f = tmp_path / "GpuSweep.cpp"
+    f.write_text(src)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-w", "-I/root/reference",
+                        "-I" + os.path.join(ROOT, "include"), str(f)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]

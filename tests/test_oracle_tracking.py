@@ -59,9 +59,9 @@ def test_nodes_stay_on_rails_and_surfaces_in_order(oracle_lib):
     rows = t.rows
     assert np.all(r[rows[0]] < r[rows[1]]) and np.all(r[rows[1]] < r[rows[2]])
     # the inner block is fixed by the spacing law r_j = R1 + (j/n)^1.3 (R111 - R1)
-    j = np.arange(t.n_inner)
-    want = t.r_inner + (j / t.n_inner) ** t.p_inner * (t.r_mid - t.r_inner)
-    assert np.allclose(r[: t.n_inner], want[:, None], rtol=1e-13, atol=0)
+    j = np.arange(t.inner_rail_nodes)
+    want = t.r_inner + (j / t.inner_rail_nodes) ** t.p_inner * (t.r_mid - t.r_inner)
+    assert np.allclose(r[: t.inner_rail_nodes], want[:, None], rtol=1e-13, atol=0)
     # the outer end sits on the outer sphere
     assert np.allclose(r[-1], t.r_outer, rtol=1e-14)
 
