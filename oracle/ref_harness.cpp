@@ -765,6 +765,60 @@ static int mode_block(const char* in, const char* out, const char* ckpt)
     return 0;
 }
 
+// ---- the reference's Monte Carlo of the hydrogen atoms on its own block grid -----------
+// in : int32 N1 N2 N3 N4 M1 M2 M3 M4 seed
+// Writes the seed tables M_K_prepare reads (my_dat_822.txt: 822 seed triples of
+// Sensor, Sensors276405.txt: seed quadruples of sensor2) into the working directory from
+// the seed, builds Setka(N1..M4) with Init_conditions, and runs M_K_prepare and
+// MK_start_new (Setka.cpp:11228-12535).
+// out: int32 nc; float64 per cell [nc][24]: cx cy ro p u v, H_n[4] H_u[4] H_v[4] H_T[4],
+//      I_u I_v I_T (the Monte Carlo's momentum and energy sources), k_u k_v k_T
+static int mode_mc(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[9]; rd(f, h, 9);
+    std::fclose(f);
+    {
+        std::srand((unsigned)h[8]);
+        FILE* g = xopen("my_dat_822.txt", "w");
+        for (int i = 0; i < 822; i++)
+            std::fprintf(g, "%d %d %d\n", 2 * (std::rand() % 16384) + 1, std::rand() % 32768, 2 * (std::rand() % 512) + 1);
+        std::fclose(g);
+        g = xopen("Sensors276405.txt", "w");
+        for (int i = 0; i < 276405; i++)
+            std::fprintf(g, "%u %u %u %u\n", 128u + (unsigned)std::rand(), 128u + (unsigned)std::rand(),
+                         128u + (unsigned)std::rand(), 128u + (unsigned)std::rand());
+        std::fclose(g);
+    }
+    Setka* S = new Setka(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7]);
+    S->Init_conditions();
+    for (auto* K : S->All_Cells) { zero_moments(K->par[0]); zero_moments(K->par[1]); }
+    S->TVD_prepare();
+    const auto t0 = std::chrono::steady_clock::now();
+    S->M_K_prepare();
+    S->MK_start_new();
+    const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::fprintf(stderr, "ref_mc_seconds %.3f\n", sec);
+    const int nc = (int)S->All_Cells.size();
+    std::vector<double> o((size_t)nc * 24);
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        const Parametr& p = K->par[0];
+        double x, y;
+        K->Get_Center(x, y);
+        double* r = &o[(size_t)c * 24];
+        r[0] = x; r[1] = y; r[2] = p.ro; r[3] = p.p; r[4] = p.u; r[5] = p.v;
+        for (int k = 0; k < 4; k++) { r[6 + k] = p.H_n[k]; r[10 + k] = p.H_u[k]; r[14 + k] = p.H_v[k]; r[18 + k] = p.H_T[k]; }
+        r[22] = p.I_u; r[23] = p.I_v;
+    }
+    f = xopen(out, "wb");
+    const int32_t n32 = nc;
+    wr(f, &n32, 1);
+    wr(f, o.data(), o.size());
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out | block in out checkpoint\n"); return 2; }
@@ -780,6 +834,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "save")) return mode_save(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "resave")) return mode_resave(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "block") && argc >= 5) return mode_block(argv[2], argv[3], argv[4]);
+    if (!std::strcmp(argv[1], "mc")) return mode_mc(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }
