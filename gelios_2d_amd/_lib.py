@@ -8,7 +8,7 @@ from __future__ import annotations
 import ctypes
 import os
 
-from .params import GelParams, GelTracking
+from .params import GelMcParams, GelMcStats, GelParams, GelTracking
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgelios.so")
@@ -104,6 +104,11 @@ SIGNATURES = {
                                                   ctypes.c_int]),
     "gel_batch_source_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
                                               ctypes.POINTER(ctypes.c_longlong)]),
+    "gel_mc_params_reference": (None, [ctypes.POINTER(GelMcParams)]),
+    "gel_mc_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelMcParams), _dp, ctypes.POINTER(GelMcStats)]),
+    "gel_mc_rates": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
+    "gel_mc_exchange": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dp,
+                                       ctypes.POINTER(ctypes.c_int)]),
     "gel_strerror": (ctypes.c_char_p, [ctypes.c_int]),
     "gel_last_error": (ctypes.c_char_p, []),
 }

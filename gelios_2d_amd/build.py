@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgelios.so")
 SOURCES = ["gelios_sweep.hip", "sweep_fp32.hip", "surrogate.hip", "tracking.hip", "multifluid.hip", "inner.hip",
-           "mesh.hip"]
+           "mesh.hip", "mc.hip"]
 HIPCC = "/opt/rocm/bin/hipcc"
 
 HIPCC_FLAGS = [

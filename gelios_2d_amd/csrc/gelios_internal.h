@@ -79,6 +79,9 @@ hipError_t gel_refresh_list_launch(hipStream_t s, const gel::SweepArgs* d_args, 
 hipError_t gel_live_list_build(hipStream_t s, const gel::SweepArgs* d_args, const int2* d_tiles, long long tiles,
                                long long ncells, int2* d_list, unsigned* d_count);
 
+// Record the detail gel_last_error() reports and return st (gelios_sweep.hip).
+int gel_set_error(int st, const std::string& msg);
+
 struct gel_ctx;
 struct MeshBuf;                   // device face tables of a mesh context (mesh.hip)
 // Build / free the face tables of a general mesh (mesh.hip); *err names a rejected input.

@@ -68,6 +68,8 @@ static int fail(int st, const std::string& msg)
     return st;
 }
 
+int gel_set_error(int st, const std::string& msg) { return fail(st, msg); }
+
 #define HIP_OK(call)                                                                  \
     do {                                                                              \
         hipError_t e_ = (call);                                                       \

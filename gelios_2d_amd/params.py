@@ -188,3 +188,53 @@ def reference_tracking(rows=(-1, -1, -1), outer_faces: int = 1 << 30) -> GelTrac
     t.k_contact, t.k_inner, t.k_outer = 0.3, 0.1, 1.0
     t.tang_inner, t.tang_outer = 0.1, 0.01
     return t
+
+
+class GelMcParams(ctypes.Structure):
+    """Field-for-field mirror of ``struct gel_mc_params`` (hydrogen Monte Carlo)."""
+
+    _fields_ = [
+        ("v_inf", ctypes.c_double),
+        ("kn", ctypes.c_double),
+        ("sig_a", ctypes.c_double),
+        ("n_p_lism", ctypes.c_double),
+        ("w_min", ctypes.c_double),
+        ("n_primary", ctypes.c_longlong),
+        ("seed", ctypes.c_ulonglong),
+        ("max_segments", ctypes.c_int),
+    ]
+
+
+class GelMcStats(ctypes.Structure):
+    """Field-for-field mirror of ``struct gel_mc_stats``."""
+
+    _fields_ = [
+        ("primaries_in", ctypes.c_longlong),
+        ("atoms", ctypes.c_longlong),
+        ("segments", ctypes.c_longlong),
+        ("lost", ctypes.c_longlong),
+        ("dropped", ctypes.c_longlong),
+        ("generations", ctypes.c_int),
+        ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+GEL_MC_NFIELDS = 29
+
+
+def mc_reference_params(n_primary: int = 1_000_000, seed: int = 1, w_min: float = 0.25) -> GelMcParams:
+    """The constants of the reference's Monte Carlo: Velosity_inf, Kn_, a_2, n_p_LISM_
+    (Help.h:53-64) and weight_ (Help.h:93) as the secondaries' roulette weight."""
+    p = GelMcParams()
+    p.v_inf = -2.54338
+    p.kn = 6.0
+    p.sig_a = 0.1307345665
+    p.n_p_lism = 3.0
+    p.w_min = w_min
+    p.n_primary = n_primary
+    p.seed = seed
+    p.max_segments = 1_000_000
+    return p

@@ -17,7 +17,8 @@
  *   - One context is driven by one host thread at a time.
  *   - Results are deterministic: fixed summation order, no floating-point atomics
  *     (the global time step is reduced with an integer min on the bit pattern of a
- *     positive double).
+ *     positive double).  The one exception is the Monte Carlo (gel_mc_run), whose
+ *     per-cell tallies are FP64 atomic sums: runs agree to rounding, not bit for bit.
  *
  * Data layout (host side and device side alike)
  *   - Grid: logically structured polar grid of ni (along a ray, index i) by nj
@@ -416,6 +417,69 @@ int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, 
 /* Refreshes run by gel_batch_step and the cells that fell back in them, since the
  * last call (both reset). */
 int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallback_cells);
+
+/* ---- Monte Carlo of the interstellar hydrogen --------------------------------------
+ * The trajectory tracer that gives the sweep its charge-exchange inputs: replaces
+ * Setka::MK_start_new (Setka.cpp:11989-12535) with its tracer
+ * Setka::Fly_exchenge_Imit_Korol (:12814-13981) and the moments' post-processing
+ * (:12241-12449).  Atoms of a drifting Maxwellian (density 1, bulk velocity v_inf along
+ * x, thermal speed 1) enter through the sphere around the grid; each flies straight,
+ * cell by cell, and in every cell it crosses gives up the weight fraction
+ * 1 - exp(-kappa) of the charge-exchange optical depth kappa to a secondary atom born at
+ * a sampled point with a velocity drawn from the plasma's exchange kernel (the parent
+ * flies on with the rest).  Secondaries lighter than w_min x a primary's weight survive
+ * with probability weight / (w_min x primary) at that weight.  Population of a
+ * secondary: 0 (C_1, C_centr cells), 1 (other), 2 (C_4), 3 (C_5); primaries are 3. */
+typedef struct gel_mc_params {
+    double v_inf;           /* hydrogen bulk velocity along x at infinity (Velosity_inf, Help.h:53) */
+    double kn;              /* Knudsen number (Kn_, Help.h:60) */
+    double sig_a;           /* a_2 of the cross section (1 - a_2 ln u)^2 (Help.h:63-68) */
+    double n_p_lism;        /* scale of the source integrals (n_p_LISM_, Help.h:64) */
+    double w_min;           /* roulette weight of secondaries, in primary weights */
+    long long n_primary;    /* primary atoms */
+    unsigned long long seed;
+    int max_segments;       /* bound on the cell segments of one atom */
+} gel_mc_params;
+
+/* The reference's constants; w_min 0.25 (weight_, Help.h:93, the roulette of
+ * Setka::Fly_exchenge, Setka.cpp:12654-12665), n_primary 1e6, seed 1, max_segments 1e6. */
+void gel_mc_params_reference(gel_mc_params* p);
+
+typedef struct gel_mc_stats {
+    long long primaries_in;  /* primaries that reached the grid */
+    long long atoms;         /* trajectories traced on the grid (primaries + secondaries kept) */
+    long long segments;      /* cell segments flown */
+    long long lost;          /* atoms whose way out of a cell could not be resolved */
+    long long dropped;       /* secondaries dropped for want of queue room (must be 0) */
+    int generations;         /* secondary generations traced */
+    double ms;               /* device time */
+} gel_mc_stats;
+
+/* Per-cell outputs of gel_mc_run, GEL_MC_NFIELDS variable-major arrays of ncells:
+ *   0..18   H_n[4], H_u[4], H_v[4], H_T[4], k_u, k_v, k_T: the GEL_MK1_NFIELDS source
+ *           inputs of the sweep (Cell.h:55-66), as Setka.cpp:12247-12449 leaves them;
+ *   19..21  I_u, I_v, I_T: momentum and energy sources from the rate integrals;
+ *   22..24  II_u, II_v, II_T: the same from the sampled exchanges;
+ *   25..28  F_n, F_u, F_v, F_T: moments of all populations together. */
+#define GEL_MC_NFIELDS 29
+
+/* Run the Monte Carlo on a context's grid and plasma state (polar grids, static, gas or
+ * MHD: rho, p, u, v are read).  out (NULL: none) receives GEL_MC_NFIELDS x ncells; a
+ * GEL_SRC_MK1 context also gets fields 0..18 as its source inputs, as from
+ * gel_set_sources. */
+int gel_mc_run(gel_ctx* ctx, const gel_mc_params* p, double* out, gel_mc_stats* st);
+
+/* Point-wise pieces of the Monte Carlo on the device, for checks against the
+ * reference.  Rates at n (u, cp): out[n][6] = Setka::Velosity_1..3 (Setka.cpp:15830-15866)
+ * and MKmethod::int_1..3 (MKmethod.cpp:2741-2858; here from the integrals' definition,
+ * 0 where u / cp > 7). */
+int gel_mc_rates(int n, const double* u, const double* cp, double* out);
+/* The exchange-velocity sampler of MKmethod::Change_Velosity4 with no splitting zones
+ * (MKmethod.cpp:2270-2531, I = 0) driven by the reference's random sensor
+ * (sensor.cpp:11-31): in[n][7] = Ur Uthe Uphi Vr Vthe Vphi cp (velocities in units of
+ * cp), seeds[n][3]; reps draws per input into out[n][reps][4] = Wr Wthe Wphi mu, and the
+ * sensors' final states into state[n][3]. */
+int gel_mc_exchange(int n, const double* in, const int* seeds, int reps, double* out, int* state);
 
 const char* gel_strerror(int status);
 const char* gel_last_error(void);

@@ -135,6 +135,22 @@ int ora_step_mf(const ora_mesh* m, const gel_params* prm, const double* now, dou
 int ora_track_nodes(const gel_tracking* t, int ni, int nj, const double* x, const double* y, const double* now,
                     const double* rail_s, double gam, double dt, double* x2, double* y2);
 
+/* ---- Monte Carlo of the interstellar hydrogen (mc_oracle.c) ---- */
+double ora_sensor_draw(int s[3]);                                      /* sensor.cpp:11-31 */
+void ora_mc_seed_primary(unsigned long long seed, long long k, int s[3]);
+void ora_mc_velosity(double u, double cp, double o[3]);                /* Setka.cpp:15830-15866 */
+void ora_mc_int(double u, double cp, double sig_a, double o[3]);       /* MKmethod.cpp:2741-2858 */
+/* MKmethod::Change_Velosity4 with I = 0 (MKmethod.cpp:2270-2531): in = Ur Uthe Uphi Vr
+ * Vthe Vphi cp (velocities in units of cp), out = Wr Wthe Wphi mu; returns mu */
+int ora_mc_exchange(int s[3], const double in[7], double sig_a, double out[4]);
+double ora_mc_inflow(double v_inf);
+/* The tracer on a polar grid (Setka::MK_start_new with Fly_exchenge_Imit_Korol,
+ * Setka.cpp:11989-13981, in the form of gel_mc_run): prim = the state (rho, p, u, v the
+ * first four of the variable-major fields), pop = population of a secondary born in each
+ * cell; out = GEL_MC_NFIELDS x ni*nj as gel_mc_run writes it. */
+int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* prim, const int8_t* pop,
+               const gel_mc_params* prm, double* out, gel_mc_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

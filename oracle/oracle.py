@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 from gelios_2d_amd.grid import MAXF, GeneralMesh, PolarGrid, pack_general
-from gelios_2d_amd.params import GelParams, GelTracking
+from gelios_2d_amd.params import GEL_MC_NFIELDS, GelMcParams, GelMcStats, GelParams, GelTracking
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgelios_oracle.so")
@@ -99,6 +99,21 @@ def lib():
         L.ora_hlld.restype = ctypes.c_double
         L.ora_hlld.argtypes = [ctypes.POINTER(OraMhd), ctypes.POINTER(OraMhd), ctypes.c_double, ctypes.c_double,
                                ctypes.c_double, ctypes.c_double, _dp]
+        L.ora_sensor_draw.restype = ctypes.c_double
+        L.ora_sensor_draw.argtypes = [_ip]
+        L.ora_mc_seed_primary.restype = None
+        L.ora_mc_seed_primary.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, _ip]
+        L.ora_mc_velosity.restype = None
+        L.ora_mc_velosity.argtypes = [ctypes.c_double, ctypes.c_double, _dp]
+        L.ora_mc_int.restype = None
+        L.ora_mc_int.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.ora_mc_exchange.restype = ctypes.c_int
+        L.ora_mc_exchange.argtypes = [_ip, _dp, ctypes.c_double, _dp]
+        L.ora_mc_inflow.restype = ctypes.c_double
+        L.ora_mc_inflow.argtypes = [ctypes.c_double]
+        L.ora_mc_run.restype = ctypes.c_int
+        L.ora_mc_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int8),
+                                 ctypes.POINTER(GelMcParams), _dp, ctypes.POINTER(GelMcStats)]
         _lib = L
     return _lib
 
@@ -296,3 +311,70 @@ class TrackedOracle:
                 raise RuntimeError(f"oracle step failed with status {st}")
             self.x, self.y = x2, y2
         return cur
+
+
+# ---------------------------------------------------------------- Monte Carlo ----
+def sensor_draws(seed, n):
+    """n draws of the reference's random sensor (sensor.cpp:11-31) from seed (a1, a2, a3)."""
+    s = np.ascontiguousarray(seed, np.int32).copy()
+    return np.array([lib().ora_sensor_draw(_i(s)) for _ in range(n)])
+
+
+def mc_rates(u, cp, sig_a=0.1307345665):
+    """[n][6]: Velosity_1..3 and int_1..3 (int_k 0 where u / cp > 7)."""
+    u = np.atleast_1d(np.asarray(u, np.float64))
+    cp = np.broadcast_to(np.asarray(cp, np.float64), u.shape)
+    out = np.zeros((len(u), 6))
+    o = np.zeros(3)
+    for k in range(len(u)):
+        lib().ora_mc_velosity(float(u[k]), float(cp[k]), _d(o))
+        out[k, :3] = o
+        if u[k] / cp[k] <= 7.0:
+            lib().ora_mc_int(float(u[k]), float(cp[k]), float(sig_a), _d(o))
+            out[k, 3:] = o
+    return out
+
+
+def mc_exchange(inp, seeds, reps, sig_a=0.1307345665):
+    """Change_Velosity4 with I = 0 from seeded sensors: inp[n][7], seeds[n][3] ->
+    (out[n][reps][4] = Wr Wthe Wphi mu, state[n][3])."""
+    inp = np.ascontiguousarray(inp, np.float64)
+    n = len(inp)
+    out = np.zeros((n, reps, 4))
+    state = np.zeros((n, 3), np.int32)
+    for k in range(n):
+        s = np.ascontiguousarray(seeds[k], np.int32).copy()
+        row = np.ascontiguousarray(inp[k])
+        for r in range(reps):
+            o = np.zeros(4)
+            lib().ora_mc_exchange(_i(s), _d(row), float(sig_a), _d(o))
+            out[k, r] = o
+        state[k] = s
+    return out, state
+
+
+def mc_run(g: PolarGrid, prim, pop, prm: GelMcParams):
+    """The tracer on a polar grid; returns (out[GEL_MC_NFIELDS][ncells], stats dict)."""
+    nc = g.ni * g.nj
+    x = np.ascontiguousarray(g.x, np.float64)
+    y = np.ascontiguousarray(g.y, np.float64)
+    prim = np.ascontiguousarray(prim, np.float64)
+    pop = np.ascontiguousarray(pop, np.int8)
+    out = np.zeros((GEL_MC_NFIELDS, nc))
+    st = GelMcStats()
+    lib().ora_mc_run(g.ni, g.nj, _d(x), _d(y), _d(prim), pop.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                     ctypes.byref(prm), _d(out), ctypes.byref(st))
+    return out, st.as_dict()
+
+
+def mc_population(cell_type, nc):
+    """Population of a secondary born in a cell (Setka.cpp:13384-13398): C_5 -> 3, C_4 -> 2,
+    C_1 and C_centr -> 0, others 1."""
+    if cell_type is None:
+        return np.ones(nc, np.int8)
+    ct = np.asarray(cell_type)
+    pop = np.ones(nc, np.int8)
+    pop[ct == 4] = 3
+    pop[ct == 3] = 2
+    pop[(ct == 0) | (ct == 5)] = 0
+    return pop

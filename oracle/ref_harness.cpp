@@ -819,6 +819,58 @@ static int mode_mc(const char* in, const char* out)
     return 0;
 }
 
+// Point-wise pieces of the reference's Monte Carlo (MKmethod.cpp, sensor.cpp,
+// Setka.cpp:15830-15866): the random sensor, the charge-exchange rate integrals and the
+// exchange-velocity sampler of Change_Velosity4 with no splitting zones (I = 0, the call
+// of Fly_exchenge_Imit_Korol at Setka.cpp:13465).
+// in : int32 nr; float64 [nr][2] = u cp
+//      int32 ns, nd; int32 [ns][3] sensor seeds
+//      int32 nx, reps; float64 [nx][7] = Ur Uthe Uphi Vr Vthe Vphi cp; int32 [nx][3] seeds
+// out: float64 [nr][6] = Velosity_1 Velosity_2 Velosity_3 int_1 int_2 int_3 (int_k only
+//      for u / cp <= 7, else 0)
+//      float64 [ns][nd] draws
+//      float64 [nx][reps][4] = Wr Wthe Wphi mu; int32 [nx][3] sensor state afterwards
+static int mode_mcunit(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t nr; rd(f, &nr, 1);
+    std::vector<double> ra((size_t)nr * 2); rd(f, ra.data(), ra.size());
+    int32_t nsd[2]; rd(f, nsd, 2);
+    std::vector<int32_t> seeds((size_t)nsd[0] * 3); rd(f, seeds.data(), seeds.size());
+    int32_t nxr[2]; rd(f, nxr, 2);
+    std::vector<double> xa((size_t)nxr[0] * 7); rd(f, xa.data(), xa.size());
+    std::vector<int32_t> xs((size_t)nxr[0] * 3); rd(f, xs.data(), xs.size());
+    std::fclose(f);
+    Setka* S = new Setka();
+    MKmethod MK;
+    f = xopen(out, "wb");
+    for (int k = 0; k < nr; k++) {
+        const double u = ra[2 * k], cp = ra[2 * k + 1];
+        double o[6] = { S->Velosity_1(u, cp), S->Velosity_2(u, cp), S->Velosity_3(u, cp), 0.0, 0.0, 0.0 };
+        if (u / cp <= 7.0) { o[3] = MK.int_1(u, cp); o[4] = MK.int_2(u, cp); o[5] = MK.int_3(u, cp); }
+        wr(f, o, 6);
+    }
+    for (int s = 0; s < nsd[0]; s++) {
+        Sensor sen(seeds[3 * s], seeds[3 * s + 1], seeds[3 * s + 2]);
+        for (int d = 0; d < nsd[1]; d++) { const double v = sen.MakeRandom(); wr(f, &v, 1); }
+    }
+    std::vector<int32_t> st((size_t)nxr[0] * 3);
+    for (int x = 0; x < nxr[0]; x++) {
+        const double* a = &xa[(size_t)x * 7];
+        Sensor sen(xs[3 * x], xs[3 * x + 1], xs[3 * x + 2]);
+        for (int r = 0; r < nxr[1]; r++) {
+            std::vector<double> Wr(1), Wt(1), Wp(1), mu(1);
+            MK.Change_Velosity4(&sen, a[0], a[1], a[2], a[3], a[4], a[5], Wr, Wt, Wp, mu, a[6], 1.0, 0);
+            const double o[4] = { Wr[0], Wt[0], Wp[0], mu[0] };
+            wr(f, o, 4);
+        }
+        st[3 * x] = sen.a1_; st[3 * x + 1] = sen.a2_; st[3 * x + 2] = sen.a3_;
+    }
+    wr(f, st.data(), st.size());
+    std::fclose(f);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out | block in out checkpoint\n"); return 2; }
@@ -835,6 +887,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "resave")) return mode_resave(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "block") && argc >= 5) return mode_block(argv[2], argv[3], argv[4]);
     if (!std::strcmp(argv[1], "mc")) return mode_mc(argv[2], argv[3]);
+    if (!std::strcmp(argv[1], "mcunit")) return mode_mcunit(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

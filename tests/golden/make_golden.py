@@ -7,7 +7,7 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint, block)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint, block, mc)
 """
 from __future__ import annotations
 
@@ -449,10 +449,51 @@ def gen_block(tmp):
     np.savez_compressed(os.path.join(OUT, "sweep_block.npz"), **res)
 
 
+def gen_mc(rng, tmp):
+    """Point-wise pieces of the reference's Monte Carlo (oracle/ref_harness.cpp,
+    mode_mcunit): Setka::Velosity_1..3 and MKmethod::int_1..3 on (u, cp) pairs, the
+    random sensor (sensor.cpp) from seeds, and Change_Velosity4 with no splitting zones
+    (I = 0) from seeded sensors, 50 samples per input."""
+    x = np.arange(0.0, 7.0 + 1e-9, 0.05)
+    dense = np.concatenate([np.stack([x * cp, np.full_like(x, cp)], 1) for cp in (0.5, 1.0, 2.5)])
+    grid = np.array([(u, cp) for u in (0.0, 1e-6, 3e-6, 1e-3, 0.1, 0.5, 1.0, 2.0, 3.5, 5.0, 6.9, 8.0, 12.0, 20.0)
+                     for cp in (0.2, 0.7, 1.0, 1.9, 4.0)])
+    rates_in = np.concatenate([dense, grid]).astype(np.float64)
+    nr = len(rates_in)
+    seeds = np.array([[1, 0, 1], [12345, 678, 91], [32767, 32767, 1023], [2 * 4711 + 1, 29000, 555]], np.int32)
+    nd = 400
+    xin = []
+    for X, cp in ((0.01, 1.0), (0.4, 0.3), (1.0, 1.0), (2.2, 0.8), (3.0, 2.0), (5.5, 0.5), (8.0, 1.0),
+                  (12.0, 0.1), (0.8, 4.0), (1.7, 0.05), (6.9, 1.5), (0.05, 0.6)):
+        U = rng.normal(size=3)
+        d = rng.normal(size=3)
+        V = U + X * d / np.linalg.norm(d)
+        xin.append([*U, *V, cp])
+    xin = np.array(xin, np.float64)
+    nx, reps = len(xin), 50
+    xseed = np.stack([2 * rng.integers(0, 16384, nx) + 1, rng.integers(0, 32768, nx), rng.integers(0, 1024, nx)],
+                     1).astype(np.int32)
+    blob = (np.int32(nr).tobytes() + rates_in.tobytes() + np.array([len(seeds), nd], np.int32).tobytes() +
+            seeds.tobytes() + np.array([nx, reps], np.int32).tobytes() + xin.tobytes() + xseed.tobytes())
+    out = harness("mcunit", blob, tmp)
+    o = 0
+    rates = np.frombuffer(out, np.float64, nr * 6, o).reshape(nr, 6)
+    o += nr * 6 * 8
+    draws = np.frombuffer(out, np.float64, len(seeds) * nd, o).reshape(len(seeds), nd)
+    o += len(seeds) * nd * 8
+    samples = np.frombuffer(out, np.float64, nx * reps * 4, o).reshape(nx, reps, 4)
+    o += nx * reps * 4 * 8
+    state = np.frombuffer(out, np.int32, nx * 3, o).reshape(nx, 3)
+    np.savez_compressed(os.path.join(OUT, "mc_vectors.npz"), rates_in=rates_in, rates=rates, seeds=seeds,
+                        draws=draws, xin=xin, xseed=xseed, samples=samples, state=state)
+
+
 def main():
     only = sys.argv[1:]
     if only:
         with tempfile.TemporaryDirectory() as tmp:
+            if "mc" in only:
+                gen_mc(np.random.default_rng(20261208), tmp)
             if "godunov" in only:
                 gen_godunov(np.random.default_rng(20261205), tmp)
             if "track" in only:
@@ -479,6 +520,7 @@ def main():
         gen_inner(tmp)
         gen_checkpoint(np.random.default_rng(20261207), tmp)
         gen_block(tmp)
+        gen_mc(np.random.default_rng(20261208), tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(OUT, f)))
