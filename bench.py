@@ -41,6 +41,9 @@ Workloads
   block-731        the reference's own block grid (Setka(6,3,4,8,24,6,8,8), 731 cells, read
                    from its checkpoint in tests/golden/sweep_block.npz), HLL, the
                    reference's constants, 200 steps
+  mc-200x100       the hydrogen Monte Carlo (gel_mc_run) on the hllc-200x100 heliosphere's
+                   plasma: a step is one run of 4e6 primary atoms with their secondaries;
+                   the line's value is cell segments flown per second (its own metric)
 """
 from __future__ import annotations
 
@@ -440,6 +443,64 @@ def cpu_baseline_workload(w, budget_s=12.0):
     return out
 
 
+# Algorithmic bytes per cell segment of the Monte Carlo: the cell's plasma (rho, cp, u, v,
+# p: 40 B) and its four nodes (64 B) read, four moment tallies and three source tallies
+# (read and written, 16 B each: 112 B); secondaries' queue traffic is not counted.
+MC_BYTES_PER_SEGMENT = 40 + 64 + 112
+
+
+def run_mc_workload(w, K, W, no_cpu, n_primary=4_000_000):
+    """K timed runs of the Monte Carlo after W warm-up runs of a tenth of the atoms."""
+    from gelios_2d_amd.mc import run_mc
+    from gelios_2d_amd.params import mc_reference_params
+    from gelios_2d_amd.setka import Setka
+    s = Setka(w.grid, w.params)
+    s.upload(w.prim0)
+    for k in range(max(W, 1)):
+        run_mc(s, mc_reference_params(n_primary // 10, seed=100 + k))
+    segs = atoms = lost = 0
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for k in range(K):
+        _, st = run_mc(s, mc_reference_params(n_primary, seed=1 + k))
+        segs += st["segments"]
+        atoms += st["atoms"]
+        lost += st["lost"]
+        dev_ms += st["ms"]
+        assert st["dropped"] == 0, st
+    wall = time.perf_counter() - t0
+    achieved = MC_BYTES_PER_SEGMENT * segs / (dev_ms * 1e-3) / 1e9
+    line = {"metric": "cell segments/s of the hydrogen Monte Carlo on 1 MI355X", "value": segs / wall,
+            "unit": "segments/s", "n_gpus": 1, "steps": K, "warmup": W, "ms_per_step": wall * 1e3 / K,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"mc-{w.name}", "cells": w.grid.ncells, "primaries_per_step": n_primary,
+                       "atoms_per_step": atoms / K, "segments_per_step": segs / K,
+                       "atoms_lost": lost},
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                         "bytes_per_segment": MC_BYTES_PER_SEGMENT, "kernel_ms_per_step": dev_ms / K,
+                         "note": "FP64 issue and divergence bound, not HBM: the fraction is what the "
+                                 "algorithmic bytes would need"}}
+    if not no_cpu:
+        from oracle import oracle as ora
+        import numpy as np
+        n_cpu, elapsed, segs_cpu = 20000, 0.0, 0
+        while elapsed < 8.0:
+            t = time.perf_counter()
+            _, st = ora.mc_run(w.grid, w.prim0, ora.mc_population(w.grid.cell_type, w.grid.ncells),
+                               mc_reference_params(n_cpu, seed=7))
+            elapsed = time.perf_counter() - t
+            segs_cpu = st["segments"]
+            if elapsed < 8.0:
+                n_cpu *= 2
+        line["cpu_baseline"] = {"value": segs_cpu / elapsed, "unit": "segments/s", "cores": 1, "kind": "port",
+                                "sample": f"{n_cpu} primaries of the same run, oracle/mc_oracle.c, one thread",
+                                "host": host_cpu(),
+                                "reference": "not run: the reference's MK_start_new needs about 8 h on 8 threads "
+                                             "for its 411 x 50 primaries on its production grid"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -559,6 +620,10 @@ def main():
         }
         precision, dtype = "fp32-bulk (FP32 state and face arithmetic; FP64 geometry, accumulation, update)", "f32"
         config = None
+    elif wl == "mc-200x100":
+        r = run_mc_workload(configs.hllc_gas_200x100(), args.steps if args.steps is not None else 3, W, args.no_cpu)
+        print(json.dumps(r), flush=True)
+        return
     else:
         if wl == "hllc-200x100":
             w = configs.hllc_gas_200x100()
