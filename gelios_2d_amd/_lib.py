@@ -84,6 +84,7 @@ SIGNATURES = {
     "gel_mlp_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _fp]),
     "gel_mlp_set_scaling": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, _fp, _fp]),
     "gel_mlp_last_error": (ctypes.c_char_p, []),
+    "gel_mlp_set_output_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "gel_surrogate_features": (ctypes.c_int, [ctypes.c_void_p, _fp]),
     "gel_source_analytic": (ctypes.c_int, [ctypes.c_void_p, _dp]),
     "gel_surrogate_refresh": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,

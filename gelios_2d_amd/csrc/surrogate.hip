@@ -266,6 +266,16 @@ __device__ inline void analytic_q(const gel_params& prm, size_t nc, size_t c, P 
     q[0] = 0.0; q[1] = r1; q[2] = r2; q[3] = r3; q[4] = 0.0;
 }
 
+// The closure without the Monte Carlo's k factors (k_u = k_v = k_T = 1).
+template <class TS, class P>
+__device__ inline void analytic_q_k1(const gel_params& prm, size_t nc, size_t c, P st, P src, double q[5])
+{
+    TS r1, r2, r3;
+    auto fk = [&](int k) { return k >= 16 ? TS(1) : src[(size_t)k * nc + c]; };
+    source_mk1<decltype(fk), TS>(prm, fk, st[2 * nc + c], st[3 * nc + c], st[c], st[nc + c], r1, r2, r3);
+    q[0] = 0.0; q[1] = r1; q[2] = r2; q[3] = r3; q[4] = 0.0;
+}
+
 template <class TS>
 __global__ void analytic_table_kernel(size_t nc, gel_params prm, const TS* __restrict__ st,
                                       const TS* __restrict__ src, double* __restrict__ qtab)
@@ -296,6 +306,7 @@ struct NetDev {
     const float* sd;
     const float* ymu;
     const float* ysd;
+    int mode;                      // GEL_MLP_OUT_SOURCE or GEL_MLP_OUT_FACTOR
 };
 
 // normalised features of cell c (features_kernel), BF16-rounded input row
@@ -458,16 +469,25 @@ __device__ __forceinline__ void refresh_rows(const SweepArgs& a, size_t c, bool 
         const double ro = st[c];
         const double nH = (double)src[c] + (double)src[nc + c] + (double)src[2 * nc + c] + (double)src[3 * nc + c];
         const double scale = a.prm.src_coef * ro * nH;
-        double qm[5];
+        double qm[5], q1[5];
+        if (net.mode == GEL_MLP_OUT_FACTOR) {
+            // the closure without k factors times the network's factors 1 + y
+            analytic_q_k1<TS>(a.prm, nc, c, st, src, q1);
 #pragma unroll
-        for (int k = 0; k < 5; k++) qm[k] = ((double)ys[pos * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
+            for (int k = 0; k < 5; k++) qm[k] = q1[k] * (1.0 + ((double)ys[pos * 8 + k] * net.ysd[k] + net.ymu[k]));
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; k++) qm[k] = ((double)ys[pos * 8 + k] * net.ysd[k] + net.ymu[k]) * scale;
+        }
         if (judge) {
             double qa[5];
             analytic_q<TS>(a.prm, nc, c, st, src, qa);
             double err = 0.0;
 #pragma unroll
             for (int k = 0; k < 5; k++) {
-                const double den = fmax(fabs(qa[k]), (double)net.ysd[k] * scale);
+                // factors: relative to the closure with the Monte Carlo's k factors
+                const double den = net.mode == GEL_MLP_OUT_FACTOR ? fabs(qa[k])
+                                                                   : fmax(fabs(qa[k]), (double)net.ysd[k] * scale);
                 if (den > 0.0) err = fmax(err, fabs(qm[k] - qa[k]) / den);
             }
             if (!(err <= tol)) {
@@ -648,6 +668,7 @@ struct gel_mlp {
     float* set_x = nullptr;
     float* set_t = nullptr;
     // normalisation of features and targets (device), set by gel_mlp_set_scaling
+    int mode = GEL_MLP_OUT_SOURCE;   // what the outputs are (gel_mlp_set_output_mode)
     float* mu = nullptr;
     float* sd = nullptr;
     float* ymu = nullptr;
@@ -759,7 +780,7 @@ hipError_t gel_refresh_launch(hipStream_t s, const SweepArgs* d_args, const int2
     if (tiles <= 0) return hipSuccess;
     NetDev net;
     for (int l = 0; l < kLayers; l++) { net.w[l] = m->wb[l]; net.b[l] = m->b[l]; }
-    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd;
+    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd; net.mode = m->mode;
     if (f32)
         hipLaunchKernelGGL(refresh_kernel<float>, dim3((unsigned)tiles), dim3(256), 0, s, d_args, d_tiles, net, tol,
                            judge, d_nfall);
@@ -784,7 +805,7 @@ hipError_t gel_refresh_list_launch(hipStream_t s, const SweepArgs* d_args, const
     if (tiles <= 0) return hipSuccess;
     NetDev net;
     for (int l = 0; l < kLayers; l++) { net.w[l] = m->wb[l]; net.b[l] = m->b[l]; }
-    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd;
+    net.mu = m->mu; net.sd = m->sd; net.ymu = m->ymu; net.ysd = m->ysd; net.mode = m->mode;
     // a persistent grid over the list's chunks: about three work-groups per CU
     const dim3 grid(768);
     if (f32)
@@ -1050,6 +1071,14 @@ int gel_mlp_set_scaling(gel_mlp* m, const float* mu, const float* sd, const floa
         if (hipMemcpy(*bufs[k], src[k], n[k] * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
             return mfail(GEL_E_HIP, "scaling copy");
     }
+    return GEL_OK;
+}
+
+int gel_mlp_set_output_mode(gel_mlp* m, int mode)
+{
+    if (!m) return mfail(GEL_E_ARG, "null argument");
+    if (mode != GEL_MLP_OUT_SOURCE && mode != GEL_MLP_OUT_FACTOR) return mfail(GEL_E_ARG, "unknown output mode");
+    m->mode = mode;
     return GEL_OK;
 }
 

@@ -108,23 +108,49 @@ def analytic_sources(setka) -> np.ndarray:
     return q
 
 
+def closure_k1(setka, fields) -> np.ndarray:
+    """The closure [5][ncells] with the hydrogen moments of fields and k factors 1 (the
+    context's source inputs are restored afterwards)."""
+    f1 = np.array(fields, np.float64, copy=True)
+    f1[16:19] = 1.0
+    setka.set_sources(f1)
+    q = analytic_sources(setka)
+    setka.set_sources(fields)
+    return q
+
+
 def target_scale(setka, fields) -> np.ndarray:
     """Per-cell unit of the targets: src_coef * ro * n_H (n_H summed over populations)."""
     prim = setka.download()
     return setka.params.src_coef * prim[0] * fields[0:4].sum(axis=0)
 
 
-class SourceModel:
-    """The MLP plus its feature / target scaling."""
+OUT_SOURCE, OUT_FACTOR = 0, 1     # enum gel_mlp_output
 
-    def __init__(self, mlp: MLP, mu, sd, ymu, ysd):
+
+class SourceModel:
+    """The MLP plus its feature / target scaling.  mode OUT_SOURCE: outputs are the
+    sources in units of src_coef * ro * n_H; OUT_FACTOR: factors on the closure without
+    the Monte Carlo's k factors (source = closure x (1 + y))."""
+
+    def __init__(self, mlp: MLP, mu, sd, ymu, ysd, mode: int = OUT_SOURCE):
         self.mlp = mlp
         self.mu = np.asarray(mu, np.float32)
         self.sd = np.asarray(sd, np.float32)
         self.ymu = np.asarray(ymu, np.float32)
         self.ysd = np.asarray(ysd, np.float32)
+        self.mode = int(mode)
         L = _setup_src(_setup(_lib.lib()))
         _check(L.gel_mlp_set_scaling(mlp._h, _f(self.mu), _f(self.sd), _f(self.ymu), _f(self.ysd)))
+        _check(L.gel_mlp_set_output_mode(mlp._h, self.mode))
+
+    def predict(self, setka, fields) -> np.ndarray:
+        """The network's sources [5][ncells] on a Setka's state (host-side de-normalisation,
+        as the device refresh does it)."""
+        y = self.mlp.forward((features(setka) - self.mu) / self.sd) * self.ysd + self.ymu
+        if self.mode == OUT_FACTOR:
+            return closure_k1(setka, fields) * (1.0 + y.T)
+        return y.T * target_scale(setka, fields)
 
     def refresh(self, setka, tol: float = 1e-3, judge=True) -> int:
         """Fill the Setka's source table from the network (GEL_SRC_TABLE); returns the
@@ -150,14 +176,15 @@ class SourceModel:
         return nfall
 
     def save(self, path: str) -> None:
-        np.savez(path, params=self.mlp.get_params(), mu=self.mu, sd=self.sd, ymu=self.ymu, ysd=self.ysd)
+        np.savez(path, params=self.mlp.get_params(), mu=self.mu, sd=self.sd, ymu=self.ymu, ysd=self.ysd,
+                 mode=np.int32(self.mode))
 
     @classmethod
     def load(cls, path: str) -> "SourceModel":
         d = np.load(path, allow_pickle=False)
         m = MLP(0)
         m.set_params(d["params"])
-        return cls(m, d["mu"], d["sd"], d["ymu"], d["ysd"])
+        return cls(m, d["mu"], d["sd"], d["ymu"], d["ysd"], int(d["mode"]) if "mode" in d else OUT_SOURCE)
 
 
 def inflow_sets(n_sets: int = 16, seed: int = 4):
@@ -207,8 +234,68 @@ def training_data(ni: int = 500, nj: int = 250, n_sets: int = 16, seed: int = 4,
     return np.concatenate(X), np.concatenate(Y)
 
 
+def mc_params_for(w, n_primary: int, seed: int):
+    """Monte Carlo constants matching a workload's closure: the cross section
+    (sig_a - sig_b ln U)^2 as (1 - a ln U)^2 with a = sig_b / sig_a, n_p / Kn = src_coef,
+    the hydrogen drifting with the interstellar inflow."""
+    from .params import mc_reference_params
+    p = mc_reference_params(n_primary, seed=seed)
+    p.sig_a = w.params.sig_b / w.params.sig_a
+    p.kn = p.n_p_lism / w.params.src_coef
+    p.v_inf = w.params.inf_u
+    return p
+
+
+def mc_sources(setka, w, nh: float, n_primary: int, seed: int):
+    """The Monte Carlo (gel_mc_run) on a Setka's plasma, scaled to the hydrogen density
+    nh at infinity: (19 source inputs, sources [5][ncells] = 0, I_u, I_v, I_T, 0, stats).
+    The context (GEL_SRC_MK1) takes the 19 fields as its source inputs."""
+    from .mc import run_mc
+    out, st = run_mc(setka, mc_params_for(w, n_primary, seed))
+    out[0:4] *= nh                  # densities and sources scale with the density at infinity
+    out[19:25] *= nh
+    setka.set_sources(out[:19])
+    q = np.zeros((5, setka.ncells))
+    q[1:4] = out[19:22]
+    return out[:19].copy(), q, st
+
+
+def training_data_mc(ni: int = 500, nj: int = 250, n_sets: int = 16, seed: int = 4, max_evolve: int = 600,
+                     n_primary: int = 20_000_000, mode: int = OUT_FACTOR):
+    """Features and targets from the hydrogen Monte Carlo on the device: per inflow set,
+    the plasma after k * max_evolve / n_sets steps (analytic source from the seeded
+    populations), the Monte Carlo's moments as the hydrogen features; targets its k
+    factors less one (OUT_FACTOR: k = I / closure, Setka.cpp:12395-12449) or its momentum
+    and energy sources I_u, I_v, I_T in units of src_coef * ro * n_H (OUT_SOURCE)."""
+    from .setka import Setka
+    X, Y = [], []
+    for k, (mach, nh, s) in enumerate(inflow_sets(n_sets, seed)):
+        w, fields = workload_with_hydrogen(ni, nj, 0, mach, nh, s, moving=False)
+        st = Setka(w.grid, w.params)
+        st.upload(w.prim0)
+        st.set_sources(fields)
+        ev = k * max_evolve // max(n_sets, 1)
+        if ev:
+            st.Go_5_komponent__MK2(ev)
+        f, q, stats = mc_sources(st, w, nh, n_primary, s)
+        print(f"inflow set {k}: Mach {mach:.2f} n_H {nh:.3f}, {ev} steps, Monte Carlo {stats['atoms']} atoms "
+              f"in {stats['ms']:.0f} ms", flush=True)
+        x = features(st)
+        scale = target_scale(st, f)
+        ok = scale > 0
+        X.append(x[ok])
+        if mode == OUT_FACTOR:
+            y = np.zeros((5, st.ncells))
+            y[1:4] = f[16:19] - 1.0
+            Y.append(y[:, ok].T.astype(np.float32))
+        else:
+            Y.append((q[:, ok] / scale[ok]).T.astype(np.float32))
+        st.close()
+    return np.concatenate(X), np.concatenate(Y)
+
+
 def train_source_model(X, Y, steps: int = 6000, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
-                       shuffle_seed: int = 1) -> SourceModel:
+                       shuffle_seed: int = 1, mode: int = OUT_SOURCE) -> SourceModel:
     """Normalise, shuffle, and train the MLP on the device; returns the model."""
     mu = X.mean(axis=0)
     sd = X.std(axis=0)
@@ -231,6 +318,6 @@ def train_source_model(X, Y, steps: int = 6000, batch: int = 4096, lr: float = 1
         history.append(m.train(k, batch, lr * (0.3 ** stage)))
         done += k
     # constant targets are reproduced exactly by the device (ysd = 0)
-    model = SourceModel(m, mu, sd, ymu, np.where(const, 0.0, ysd))
+    model = SourceModel(m, mu, sd, ymu, np.where(const, 0.0, ysd), mode)
     model.loss_history = np.concatenate(history)
     return model

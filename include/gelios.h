@@ -394,6 +394,15 @@ int gel_mlp_set_dataset(gel_mlp* m, int n, const float* x, const float* t);
 /* Adam over the stored set, batch rows in order; loss_out[steps] may be NULL. */
 int gel_mlp_train(gel_mlp* m, int steps, int batch, float lr, float* loss_out);
 int gel_mlp_set_scaling(gel_mlp* m, const float* mu, const float* sd, const float* ymu, const float* ysd);
+/* What the source model's outputs y (after the scaling) are.  GEL_MLP_OUT_SOURCE: the
+ * five sources in units of src_coef * ro * n_H (n_H summed over the populations).
+ * GEL_MLP_OUT_FACTOR: factors on the closure of Cell::Get_Sourse_MK1 taken without the
+ * k factors, source = closure x (1 + y) -- the network stands in for the Monte Carlo's
+ * k_u, k_v, k_T (Setka.cpp:12395-12449) as functions of the local state; its judge
+ * compares with the closure times the k factors the context holds.  No reference
+ * counterpart (the reference's k factors come from the last Monte Carlo run). */
+enum gel_mlp_output { GEL_MLP_OUT_SOURCE = 0, GEL_MLP_OUT_FACTOR = 1 };
+int gel_mlp_set_output_mode(gel_mlp* m, int mode);
 const char* gel_mlp_last_error(void);
 /* Raw features [ncells][14] and analytic sources [5][ncells] of a context's current
  * state and hydrogen moments (gel_set_sources). */
