@@ -47,6 +47,15 @@ template <class T> GEL_HD T minmod(T x, T y)
     if (s == T(0)) return T(0);
     return (s / T(2)) * mn(fabs(x), fabs(y));
 }
+// FP32 (the FP32-bulk limiter): the same value, the smaller magnitude with x's sign
+// where the sign bits agree and 0 where they differ (a zero operand gives 0 either way),
+// in five instructions instead of the two sign functions.
+template <> GEL_HD float minmod<float>(float x, float y)
+{
+    const float m = fminf(fabsf(x), fabsf(y));
+    const bool same = (__builtin_bit_cast(int, x) ^ __builtin_bit_cast(int, y)) >= 0;
+    return same ? copysignf(m, x) : 0.0f;
+}
 // minmod(a / p, b / q) with one division: each ratio's sign from the signs of its
 // operands, the smaller magnitude chosen by the cross products |a| |q| and |b| |p|,
 // and only that ratio divided out (|a| / |p| has the bits of |a / p|).  It returns what
@@ -522,16 +531,17 @@ GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T ig1, T P[5])
 template <class T>
 GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T ig1, T SN, bool q_is_left, T P[9])
 {
+    // only the FP32-bulk faces take this flux, so the zero components of the basis
+    // (t = (-n2, n1, 0), m = (0, 0, 1)) are left out of the dot products
     const T spi4 = T(3.5449077018110318), ispi4 = T(0.28209479177387814);   // sqrt(4 pi), 1 / sqrt(4 pi)
     const T t1 = -n2, t2 = n1;
-    const T z = T(0), one = T(1);
     const T bx = q[5] * ispi4, by = q[6] * ispi4, bz = q[7] * ispi4;
-    const T u = q[2] * n1 + q[3] * n2 + q[4] * z;
-    const T v = q[2] * t1 + q[3] * t2 + q[4] * z;
-    const T w = q[2] * z + q[3] * z + q[4] * one;
-    const T bn = bx * n1 + by * n2 + bz * z;
-    const T bt = bx * t1 + by * t2 + bz * z;
-    const T bm = bx * z + by * z + bz * one;
+    const T u = q[2] * n1 + q[3] * n2;
+    const T v = q[2] * t1 + q[3] * t2;
+    const T w = q[4];
+    const T bn = bx * n1 + by * n2;
+    const T bt = bx * t1 + by * t2;
+    const T bm = bz;
     const T bno = (o[5] * ispi4) * n1 + (o[6] * ispi4) * n2;
     const T bb = sq(bx) + sq(by) + sq(bz);
     const T uu = (sq(q[2]) + sq(q[3]) + sq(q[4])) * T(0.5);
@@ -546,13 +556,13 @@ GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T ig1, T SN, boo
     for (int i = 0; i < 9; i++) PO[i] = F[i] - W * U[i];
     PO[5] = q_is_left ? (-SN * (bno - bn) - W * bn) : (-SN * (bn - bno) - W * bn);
     P[0] = PO[0];
-    P[1] = n1 * PO[1] + t1 * PO[2] + z * PO[3];
-    P[2] = n2 * PO[1] + t2 * PO[2] + z * PO[3];
-    P[3] = z * PO[1] + z * PO[2] + one * PO[3];
+    P[1] = n1 * PO[1] + t1 * PO[2];
+    P[2] = n2 * PO[1] + t2 * PO[2];
+    P[3] = PO[3];
     P[4] = PO[4];
-    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6] + z * PO[7]);
-    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6] + z * PO[7]);
-    P[7] = spi4 * (z * PO[5] + z * PO[6] + one * PO[7]);
+    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6]);
+    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6]);
+    P[7] = spi4 * PO[7];
     P[8] = PO[8];
 }
 
