@@ -502,68 +502,55 @@ GEL_HD T hlld(const T* L, const T* R, T W, T n1, T n2, T gam, T P[9])
 
 namespace gel {
 
-// Upwind (supersonic) gas flux of the state S through a face moving with W: the
-// branch SL >= W / SR <= W of HLLC_2d_Korolkov_b_s (Setka.cpp:10569-10589).
-// P: mass, x-momentum, y-momentum, energy, tracer.
-// ig1 = 1 / (gamma - 1).  Used by the FP32-bulk path only, so its divisions are
-// multiplications by reciprocals.
+// Upwind (supersonic) fluxes of the FP32-bulk faces, the branches SL >= W / SR <= W
+// of HLLC_2d_Korolkov_b_s (Setka.cpp:10569-10589) and of the HLLD below
+// (Solvers.cpp:2106-2113, :2150-2156), times the face area.  The flux of a state
+// through a face is linear in the face's area-weighted normal A = S n, so it is
+// formed in Cartesian components from A and the swept-volume rate WS = W S, both
+// FP64, with no unit normal: the products entering the cell sums carry no rounding
+// of a normalisation, and a uniform stream through a closed cell stays balanced to
+// FP64 rounding.  P: the cell accumulators' order; ig1 = 1 / (gamma - 1).
 template <class T>
-GEL_HD void gas_upwind(const Gas<T>& S, T W, T n1, T n2, T ig1, T P[5])
+GEL_HD void gas_upwind_area(const Gas<T>& S, T A1, T A2, T WS, T ig1, T P[5])
 {
-    const T t1 = n2, t2 = -n1;
-    const T u = S.u * n1 + S.v * n2, v = S.u * t1 + S.v * t2;
-    const T k = (sq(S.u) + sq(S.v)) * T(0.5);
-    const T e = S.p * ig1 + S.ro * k;
-    const T F[5] = { S.ro * u, S.ro * u * u + S.p, S.ro * u * v, (e + S.p) * u, S.Q * u };
-    const T U[5] = { S.ro, S.ro * u, S.ro * v, e, S.Q };
-    P[1] = n1 * (F[1] - W * U[1]) + t1 * (F[2] - W * U[2]);
-    P[2] = n2 * (F[1] - W * U[1]) + t2 * (F[2] - W * U[2]);
-    P[0] = F[0] - W * U[0];
-    P[3] = F[3] - W * U[3];
-    P[4] = F[4] - W * U[4];
+    const T uS = S.u * A1 + S.v * A2;                   // normal velocity x S
+    const T e = S.p * ig1 + S.ro * ((sq(S.u) + sq(S.v)) * T(0.5));
+    const T m = S.ro * uS;
+    P[0] = m - WS * S.ro;
+    P[1] = m * S.u + S.p * A1 - WS * (S.ro * S.u);
+    P[2] = m * S.v + S.p * A2 - WS * (S.ro * S.v);
+    P[3] = (e + S.p) * uS - WS * e;
+    P[4] = S.Q * uS - WS * S.Q;
 }
 
-// Upwind MHD flux of state q (the side the whole fan leaves behind), regions
-// SL >= W / SR <= W of the HLLD above; o is the other side, needed for the
-// normal-field term -SN (bn_R - bn_L) (Solvers.cpp:2161-2163).  q_is_left tells
-// which side q is; ig1 = 1 / (gamma - 1).  FP32-bulk path only: divisions are
-// multiplications by reciprocals.
+// The MHD upwind flux of state q (the side the whole fan leaves behind; o is the other
+// side) times the area, Gaussian B: mass, momentum x/y/phi, energy, B x/y/phi, tracer.
+// The normal-field term of the HLLD, -SN (Bn_R - Bn_L) along n (Solvers.cpp:2161-2163),
+// is the one part that needs the unit normal (n1, n2).
 template <class T>
-GEL_HD void mhd_upwind(const T* q, const T* o, T W, T n1, T n2, T ig1, T SN, bool q_is_left, T P[9])
+GEL_HD void mhd_upwind_area(const T* q, const T* o, T A1, T A2, T WS, T n1, T n2, T ig1, T SN, bool q_is_left,
+                            T P[9])
 {
-    // only the FP32-bulk faces take this flux, so the zero components of the basis
-    // (t = (-n2, n1, 0), m = (0, 0, 1)) are left out of the dot products
-    const T spi4 = T(3.5449077018110318), ispi4 = T(0.28209479177387814);   // sqrt(4 pi), 1 / sqrt(4 pi)
-    const T t1 = -n2, t2 = n1;
-    const T bx = q[5] * ispi4, by = q[6] * ispi4, bz = q[7] * ispi4;
-    const T u = q[2] * n1 + q[3] * n2;
-    const T v = q[2] * t1 + q[3] * t2;
-    const T w = q[4];
-    const T bn = bx * n1 + by * n2;
-    const T bt = bx * t1 + by * t2;
-    const T bm = bz;
-    const T bno = (o[5] * ispi4) * n1 + (o[6] * ispi4) * n2;
-    const T bb = sq(bx) + sq(by) + sq(bz);
-    const T uu = (sq(q[2]) + sq(q[3]) + sq(q[4])) * T(0.5);
-    const T pT = q[1] + bb * T(0.5);
-    const T e = q[1] * ig1 + q[0] * uu + bb * T(0.5);
-    const T F[9] = { q[0] * u, q[0] * u * u + pT - sq(bn), q[0] * u * v - bn * bt, q[0] * u * w - bn * bm,
-                     (e + pT) * u - bn * (u * bn + v * bt + w * bm), T(0), u * bt - v * bn, u * bm - w * bn,
-                     q[8] * u };
-    const T U[9] = { q[0], q[0] * u, q[0] * v, q[0] * w, e, bn, bt, bm, q[8] };
-    T PO[9];
-#pragma unroll
-    for (int i = 0; i < 9; i++) PO[i] = F[i] - W * U[i];
-    PO[5] = q_is_left ? (-SN * (bno - bn) - W * bn) : (-SN * (bn - bno) - W * bn);
-    P[0] = PO[0];
-    P[1] = n1 * PO[1] + t1 * PO[2];
-    P[2] = n2 * PO[1] + t2 * PO[2];
-    P[3] = PO[3];
-    P[4] = PO[4];
-    P[5] = spi4 * (n1 * PO[5] + t1 * PO[6]);
-    P[6] = spi4 * (n2 * PO[5] + t2 * PO[6]);
-    P[7] = spi4 * PO[7];
-    P[8] = PO[8];
+    const T i4pi = T(0.07957747154594767), i8pi = T(0.039788735772973836);   // 1 / (4 pi), 1 / (8 pi)
+    const T ro = q[0], p = q[1], u = q[2], v = q[3], w = q[4], bx = q[5], by = q[6], bz = q[7], Q = q[8];
+    const T uS = u * A1 + v * A2;                       // normal velocity x S
+    const T BA = bx * A1 + by * A2;                     // normal field x S
+    const T BB = sq(bx) + sq(by) + sq(bz);
+    const T pT = p + BB * i8pi;
+    const T e = p * ig1 + ro * ((sq(u) + sq(v) + sq(w)) * T(0.5)) + BB * i8pi;
+    const T VB = u * bx + v * by + w * bz;
+    const T BAi = BA * i4pi;
+    const T dB = q_is_left ? ((o[5] - bx) * n1 + (o[6] - by) * n2) : ((bx - o[5]) * n1 + (by - o[6]) * n2);
+    const T m = ro * uS;
+    P[0] = m - WS * ro;
+    P[1] = m * u + pT * A1 - bx * BAi - WS * (ro * u);
+    P[2] = m * v + pT * A2 - by * BAi - WS * (ro * v);
+    P[3] = m * w - bz * BAi - WS * (ro * w);
+    P[4] = (e + pT) * uS - VB * BAi - WS * e;
+    P[5] = uS * bx - BA * u - WS * bx - SN * dB * A1;
+    P[6] = uS * by - BA * v - WS * by - SN * dB * A2;
+    P[7] = uS * bz - BA * w - WS * bz;
+    P[8] = Q * uS - WS * Q;
 }
 
 }  // namespace gel

@@ -281,6 +281,7 @@ __device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<P
 
 struct Face {          // geometry of a face as seen from its owning cell
     double n1, n2, S, fx, fy, W, dis;
+    double A1, A2, WS;  // FP32 bulk: area-weighted normal S n and swept-volume rate W S
 };
 
 // CFL time of a cell through a face (Setka.cpp:10388-10393): cfl * |centre - face
@@ -319,11 +320,47 @@ __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, doubl
     return f;
 }
 
+// FP32 bulk: a face from its two nodes with no FP64 square root or division.  What
+// enters the cell sums is FP64: the area-weighted normal A = S n = (by - ay, -(bx - ax))
+// (ay + by) / 2 (the reference's S = |b - a| (ay + by) / 2 times its n, Setka.cpp:10198,
+// Gran.cpp:171-181) and the swept-volume rate W S (Setka.cpp:10209 times S).  The unit
+// normal, S and W themselves, which only the FP32 Riemann solver and the normal-field
+// term use, come from the FP32 reciprocal square root.  idt = 1 / dt.  dis is left for
+// the caller (only the inner sphere's faces use it).
+template <bool MOVING>
+__device__ __forceinline__ Face face_geom_a(double ax, double ay, double bx, double by,
+                                            double ax2, double ay2, double bx2, double by2, double idt)
+{
+    Face f;
+    const double dx = bx - ax, dy = by - ay, h = (ay + by) * 0.5;
+    f.A1 = dy * h;
+    f.A2 = -dx * h;
+    f.fx = (ax + bx) * 0.5;
+    f.fy = h;
+    const float l2 = float(dx * dx + dy * dy);
+    const float il = __builtin_amdgcn_rsqf(l2);
+    f.n1 = double(float(dy) * il);
+    f.n2 = double(-float(dx) * il);
+    f.S = double(l2 * il * float(h));
+    if constexpr (MOVING) {
+        const double ex = (ax2 + bx2) * 0.5 - f.fx, ey = (ay2 + by2) * 0.5 - f.fy;
+        f.WS = (ex * f.A1 + ey * f.A2) * idt;
+        f.W = (ex * f.n1 + ey * f.n2) * idt;
+    } else {
+        f.WS = 0.0;
+        f.W = 0.0;
+    }
+    f.dis = 1.0;
+    return f;
+}
+
 // Face flux from left/right face primitives (FP64); P in the order of the cell
 // accumulators.  TF = double: the reference arithmetic.  TF = float (FP32 bulk): the
 // wave speeds are estimated in FP32; a supersonic face (the whole solar wind) takes
 // the exact FP64 upwind flux, a subsonic one the FP32 Riemann solver.  Rounding the
 // flux of a steady supersonic stream to FP32 would bias every step the same way.
+// FP32 bulk returns P already times the face area (face_geom_a's A, WS and S); FP64
+// returns the flux per unit area, which the caller multiplies by S as the reference does.
 template <class Ph, class TF, int FLUX>
 __device__ __forceinline__ TF face_flux(const double* L, const double* R, const Face& f, double gam, double* P)
 {
@@ -341,7 +378,7 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             const float cL = sr(dv(g * float(L[1]), float(L[0]))), cR = sr(dv(g * float(R[1]), float(R[0])));
             const float SL = mn(u1, u2) - mx(cL, cR), SR = mx(u1, u2) + mx(cL, cR);
             if (SL >= W || SR <= W) {
-                gas_upwind<double>(SL >= W ? gl : gr, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), P);
+                gas_upwind_area<double>(SL >= W ? gl : gr, f.A1, f.A2, f.WS, 1.0 / (gam - 1.0), P);
                 return mx(fabsf(SL), fabsf(SR));
             }
             const Gas<float> fl{ float(L[0]), float(L[1]), float(L[2]), float(L[3]), float(L[4]) };
@@ -349,8 +386,8 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             float Pf[4], PQ;
             const float UU = hllc2d<float, FLUX == GEL_FLUX_HLLC>(fl, fr, W, n1, n2, g, Pf, PQ);
 #pragma unroll
-            for (int k = 0; k < 4; k++) P[k] = Pf[k];
-            P[4] = PQ;
+            for (int k = 0; k < 4; k++) P[k] = double(Pf[k]) * f.S;
+            P[4] = double(PQ) * f.S;
             return UU;
         }
     } else {
@@ -371,7 +408,8 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             const float SN = mx(fabsf(SL), fabsf(SR));
             if (SL >= W || SR <= W) {
                 const bool left = SL >= W;
-                mhd_upwind<double>(left ? L : R, left ? R : L, f.W, f.n1, f.n2, 1.0 / (gam - 1.0), double(SN), left, P);
+                mhd_upwind_area<double>(left ? L : R, left ? R : L, f.A1, f.A2, f.WS, f.n1, f.n2, 1.0 / (gam - 1.0),
+                                        double(SN), left, P);
                 return SN;
             }
             float fl[9], fr[9], Pf[9];
@@ -379,7 +417,7 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             for (int k = 0; k < 9; k++) { fl[k] = float(L[k]); fr[k] = float(R[k]); }
             const float UU = hlld<float>(fl, fr, W, n1, n2, g, Pf);
 #pragma unroll
-            for (int k = 0; k < 9; k++) P[k] = Pf[k];
+            for (int k = 0; k < 9; k++) P[k] = double(Pf[k]) * f.S;
             return UU;
         }
     }
@@ -1218,6 +1256,24 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         return f;
     };
 
+    // FP32 bulk: faces from their nodes (face_geom_a); static grids read the nodes from
+    // HBM (the rows of two neighbouring wavefronts share one node row, an L2 hit for
+    // the second), moving grids from the node rings
+    constexpr bool F32 = sizeof(TF) == 4;
+    static_assert(!F32 || !TYPES, "FP32 bulk: grids without cell types");
+    const Face kNoFace{ 1.0, 0.0, 0.0, 1.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0 };
+    const double idt = (F32 && MOVING) ? 1.0 / dt : 0.0;
+    auto gnode = [&](int q, int c) -> NodeP {
+        NodeP n{ 1.0, 1.0, 1.0, 1.0 };
+        if (q >= 0 && q <= ni && c >= 0 && c <= nj) {
+            const size_t k = (size_t)q * (nj + 1) + c;
+            n.x = gp(a.x)[k];
+            n.y = gp(a.y)[k];
+        }
+        n.x2 = n.x; n.y2 = n.y;
+        return n;
+    };
+
     // initial rows i0 - 2 .. i0 (and node rows i0 - 1 .. i0 + 1)
     for (int r = i0 - 2 + w; r <= i0; r += NW) load_row(r);
 
@@ -1239,6 +1295,8 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         double t_dn = 0.0, t_pol = 0.0, t0 = 0.0;
         double vs1 = 1.0, vol = 1.0, vs2 = 1.0, vol2 = 1.0;
         if (act) {
+            // static FP32 bulk: node (i + 1, j + 1), shared by the two faces
+            const NodeP nB = (F32 && !MOVING) ? gnode(i + 1, j + 1) : NodeP{ 1.0, 1.0, 1.0, 1.0 };
             M = cell(i, lane);
             const C D = cell(i - 1, lane), N = cell(i + 1, lane), U = cell(i + 2, lane);
             // radial face above row i, nodes (i+1, j) -> (i+1, j+1); below row 0 it is the
@@ -1246,7 +1304,14 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
             {
                 const bool inner = (i < 0);
                 Face f;
-                if constexpr (MOVING) {
+                if constexpr (F32) {
+                    NodeP na = MOVING ? node(i + 1, lane) : gnode(i + 1, j);
+                    NodeP nb = MOVING ? node(i + 1, lane + 1) : nB;
+                    if (inner) { const NodeP t = na; na = nb; nb = t; }
+                    f = face_geom_a<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, idt);
+                    if (!(col && i + 1 <= ni)) f = kNoFace;
+                    if (inner) f.dis = sqrt(sq(f.fx) + sq(f.fy));
+                } else if constexpr (MOVING) {
                     NodeP na = node(i + 1, lane), nb = node(i + 1, lane + 1);
                     if (inner) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
@@ -1282,7 +1347,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 }
                 const int fs = fslot(i);
 #pragma unroll
-                for (int k = 0; k < NV; k++) F.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * f.S;
+                for (int k = 0; k < NV; k++) F.fps[fs][k][lane] = (inner ? -P[k] : P[k]) * (F32 ? 1.0 : f.S);
                 F.ftm[fs][lane] = cfl_time<TF>(cfl, N.cx, N.cy, f, uu);
                 t_dn = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
                 if (out_lane) acc.bad += b;
@@ -1292,13 +1357,21 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 // column 0 solves the axis face of column 0, oriented (i, 0) -> (i+1, 0)
                 const bool axl = (j == -1);
                 Face f;
-                if constexpr (MOVING) {
+                if constexpr (F32) {
+                    NodeP na = MOVING ? node(i + 1, lane + 1) : nB;
+                    NodeP nb = MOVING ? node(i, lane + 1) : gnode(i, j + 1);
+                    if (axl) { const NodeP t = na; na = nb; nb = t; }
+                    f = face_geom_a<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, idt);
+                    if (!(j >= -1 && j < nj)) f = kNoFace;
+                } else if constexpr (MOVING) {
                     NodeP na = node(i + 1, lane + 1), nb = node(i, lane + 1);
                     if (axl) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom<MOVING, TF>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, dt);
                 } else {
                     const GPtr<const double> F[6] = { G.pol(0), G.pol(1), G.pol(2), G.pol(3), G.pol(4), G.pol(5) };
                     f = table_face(F, j >= -1 && j < nj, (size_t)i * (nj + 1) + (j + 1));
+                }
+                if constexpr (!MOVING) {
                     if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) {
                         // the cell's geometry, loaded before the source inputs' LDS DMA
                         // below waits for every load issued so far
@@ -1373,7 +1446,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                     uu = face_flux<Ph, TF, FX>(L, R, f, gam, P);
                 }
 #pragma unroll
-                for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * f.S;
+                for (int k = 0; k < NV; k++) psp[k] = (axl ? -P[k] : P[k]) * (F32 ? 1.0 : f.S);
                 t_pol = cfl_time<TF>(cfl, M.cx, M.cy, f, uu);
                 const double t_right = cfl_time<TF>(cfl, Mr.cx, Mr.cy, f, uu);
                 if (used_polar) acc.bad += b;
