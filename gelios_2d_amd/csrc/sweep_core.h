@@ -19,11 +19,61 @@ template <class T> GEL_HD T mn(T a, T b) { return (b < a) ? b : a; }   // std::m
 template <class T> GEL_HD T mx(T a, T b) { return (a < b) ? b : a; }   // std::max
 template <class T> GEL_HD T sq(T a) { return a * a; }
 
+// FP64 division and square root, bit for bit what the compiler's IEEE sequences give
+// for operands in the normal range, in fewer instructions.  The compiler divides by
+// v_div_scale (operand scaling for extreme exponents), v_rcp_f64 and two Newton steps
+// on the reciprocal, the quotient a * r corrected once by its remainder, v_div_fmas
+// and v_div_fixup (special operands): 11 instructions.  Where no scaling applies
+// (|a| above 2^-969, exponents less than 768 apart, no denormal quotient) the scaling
+// steps pass the operands through and the fix-up passes the quotient through, so the
+// same reciprocal and correction alone (den, qt) give the same bits in 8, and a
+// denominator shared by several quotients pays for its reciprocal once (3 more per
+// quotient).  A zero numerator gives a zero (its sign may differ).  The callers'
+// denominators are positive distances, areas, volumes, densities, wave speeds and
+// wave-speed differences and gamma - 1, their numerators physical quantities.
+struct Den {
+    double b, r;        // the denominator and its refined reciprocal
+};
+__device__ __forceinline__ Den den(double b)
+{
+    double r = __builtin_amdgcn_rcp(b);
+    const double e0 = fma(-b, r, 1.0);
+    r = fma(r, e0, r);
+    const double e1 = fma(-b, r, 1.0);
+    r = fma(r, e1, r);
+    return Den{ b, r };
+}
+__device__ __forceinline__ double qt(double a, const Den& d)
+{
+    const double q = a * d.r;
+    const double rem = fma(-d.b, q, a);
+    return fma(rem, d.r, q);
+}
+// The compiler's square root is v_rsq_f64 with two Goldschmidt steps and a final
+// correction, wrapped in a scaling of arguments below 2^-767 and a pass-through of
+// zero and infinity: here the same steps without the scaling (the callers' arguments
+// are zero or far above 2^-767: squared distances, pressures over densities).
+__device__ __forceinline__ double sqn(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    const double d0 = fma(-g, g, x);
+    h = fma(h, r, h);
+    g = fma(d0, h, g);
+    const double d1 = fma(-g, g, x);
+    g = fma(d1, h, g);
+    return x == 0.0 ? x : g;
+}
+
 // Division and square root inside the Riemann solvers: IEEE in FP64 (the
-// reference's arithmetic); in FP32 (the FP32-bulk solvers) the hardware reciprocal
-// and square root, 1 ulp each, instead of the correctly rounded sequences.
-GEL_HD double dv(double a, double b) { return a / b; }
-GEL_HD double sr(double x) { return sqrt(x); }
+// reference's arithmetic, by den / qt and sqn); in FP32 (the FP32-bulk solvers) the
+// hardware reciprocal and square root, 1 ulp each, instead of the correctly rounded
+// sequences.
+__device__ __forceinline__ double dv(double a, double b) { return qt(a, den(b)); }
+__device__ __forceinline__ double sr(double x) { return sqn(x); }
 __device__ __forceinline__ float dv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ float sr(float x) { return __builtin_amdgcn_sqrtf(x); }
 
@@ -68,7 +118,21 @@ template <class T> GEL_HD T minmod_ratio(T a, T p, T b, T q)
     if (nx != ny) return T(0);
     const T fa = fabs(a), fp = fabs(p), fb = fabs(b), fq = fabs(q);
     const bool first = fa * fq < fb * fp;
-    const T m = (first ? fa : fb) / (first ? fp : fq);     // the one division
+    const T m = dv(first ? fa : fb, first ? fp : fq);      // the one division
+    return nx ? -m : m;
+}
+// minmod_ratio(a, p, b, q) with the reciprocals of |p| and |q| given (FP64: den of
+// fabs(p), fabs(q)), for the limiter, whose nine variables share a face side's two
+// centre-distance differences.  The same value as minmod_ratio.
+__device__ __forceinline__ double minmod_ratio_d(double a, const Den& p, bool p_neg, double b, const Den& q,
+                                                 bool q_neg)
+{
+    if (a == 0.0 || b == 0.0) return 0.0;
+    const bool nx = (a < 0.0) != p_neg, ny = (b < 0.0) != q_neg;
+    if (nx != ny) return 0.0;
+    const double fa = fabs(a), fb = fabs(b);
+    const bool first = fa * q.b < fb * p.b;
+    const double m = first ? qt(fa, p) : qt(fb, q);
     return nx ? -m : m;
 }
 // Help.cpp:96-113: minmod slope through (x1,t1),(x2,t2),(x3,t3), evaluated at y.
@@ -111,20 +175,22 @@ GEL_HD void quad_center(const double* px, const double* py, double& cx, double& 
         Gy = Gy + (py[k] + py[b]) * cr;
     }
     A = A * 0.5;
-    cx = Gx / (6.0 * A);
-    cy = Gy / (6.0 * A);
+    const Den d6A = den(6.0 * A);
+    cx = qt(Gx, d6A);
+    cy = qt(Gy, d6A);
 }
 // Cell::Get_Volume for quads (Cell.cpp:146-166).
 GEL_HD double quad_area(const double* px, const double* py)
 {
     double a1 = px[0] - px[2], a2 = py[0] - py[2];
-    const double a = sqrt(sq(a1) + sq(a2));
+    const double a = sqn(sq(a1) + sq(a2));
     double b1 = px[1] - px[3], b2 = py[1] - py[3];
-    const double b = sqrt(sq(b1) + sq(b2));
-    a1 = a1 / a; a2 = a2 / a;
-    b1 = b1 / b; b2 = b2 / b;
+    const double b = sqn(sq(b1) + sq(b2));
+    const Den da = den(a), db = den(b);
+    a1 = qt(a1, da); a2 = qt(a2, da);
+    b1 = qt(b1, db); b2 = qt(b2, db);
     const double c = a1 * b1 + a2 * b2;
-    const double s = sqrt(1 - sq(c));
+    const double s = sqn(1 - sq(c));
     return 0.5 * a * b * s;
 }
 // Cell::Get_Volume_rotate(360) / (2 pi) (Cell.cpp:188-206, Setka.cpp:10187-10189).
@@ -139,7 +205,7 @@ GEL_HD double quad_volume_r(const double* px, const double* py)
         G = G + (py[k] + py[b]) * cr;
     }
     A = A * 0.5;
-    return fabs(A) * (1.0 / (6.0 * A)) * G * kPi * 360.0 / 180.0 / (2.0 * kPi);
+    return qt(qt(fabs(A) * qt(1.0, den(6.0 * A)) * G * kPi * 360.0, den(180.0)), den(2.0 * kPi));
 }
 
 // Face flux of Setka::HLLC_2d_Korolkov_b_s (Setka.cpp:10484-10700), W = face speed

@@ -137,7 +137,7 @@ __device__ __forceinline__ CellT<Ph, TS> load_cell(const SweepArgs& a, int r, in
 
 __device__ __forceinline__ double dist2d(double ax, double ay, double bx, double by)
 {
-    return sqrt(sq(ax - bx) + sq(ay - by));
+    return sqn(sq(ax - bx) + sq(ay - by));
 }
 
 // Limited linear value at the face (Help.cpp:96-113 with y = 0, x1 = -d1, x2 = -d2,
@@ -148,7 +148,9 @@ __device__ __forceinline__ double dist2d(double ax, double ay, double bx, double
 // share, and assembles the face value in FP64 from the exact cell value t2, so that
 // only the small slope correction carries FP32 rounding.
 struct Lin3 {
-    double den12, den23, d2;   // x1 - x2, x2 - x3, 0 - x2
+    Den r12, r23;              // |x1 - x2|, |x2 - x3| and their reciprocals
+    bool n12, n23;             // x1 - x2 < 0, x2 - x3 < 0
+    double d2;                 // 0 - x2
 };
 struct Lin3f {
     float i12, i23, d2;        // 1 / (x1 - x2), 1 / (x2 - x3), 0 - x2
@@ -170,7 +172,8 @@ __device__ __forceinline__ Lin3T<TF> lin3_at(double fx, double fy, double x1, do
         const double d2 = dist2d(fx, fy, x2, y2);
         const double d3 = dist2d(fx, fy, x3, y3);
         const double d1 = dist2d(fx, fy, x1, y1);
-        return Lin3{ -d1 - (-d2), -d2 - d3, 0.0 - (-d2) };
+        const double den12 = -d1 - (-d2), den23 = -d2 - d3;
+        return Lin3{ den(fabs(den12)), den(fabs(den23)), den12 < 0.0, den23 < 0.0, 0.0 - (-d2) };
     } else {
         const float d2 = dist2f(fx, fy, x2, y2);
         const float d3 = dist2f(fx, fy, x3, y3);
@@ -182,7 +185,7 @@ template <class TF>
 __device__ __forceinline__ double lin3v(double t1, double t2, double t3, const Lin3T<TF>& g)
 {
     if constexpr (sizeof(TF) == 8) {
-        const double d = minmod_ratio<double>(t1 - t2, g.den12, t2 - t3, g.den23);
+        const double d = minmod_ratio_d(t1 - t2, g.r12, g.n12, t2 - t3, g.r23, g.n23);
         return d * g.d2 + t2;
     } else {
         const float d = minmod<float>((float(t1) - float(t2)) * g.i12, (float(t2) - float(t3)) * g.i23);
@@ -229,19 +232,19 @@ __device__ __forceinline__ void tvd_radial(const CellT<Ph, double>& M, const Cel
         xd = D.cx; yd = D.cy; dro = D.s[0]; dp = D.s[1];
     }
     const double g2 = 2.0 * gam;
-    const double r = sqrt(sq(fx) + sq(fy));
-    const double r2 = sqrt(sq(M.cx) + sq(M.cy));
+    const double r = sqn(sq(fx) + sq(fy));
+    const double r2 = sqn(sq(M.cx) + sq(M.cy));
     const double d2 = dist2d(fx, fy, M.cx, M.cy);
-    const double r3 = sqrt(sq(N.cx) + sq(N.cy));
+    const double r3 = sqn(sq(N.cx) + sq(N.cy));
     const double d3 = dist2d(fx, fy, N.cx, N.cy);
-    const double r1 = sqrt(sq(xd) + sq(yd));
+    const double r1 = sqn(sq(xd) + sq(yd));
     const double d1 = dist2d(fx, fy, xd, yd);
-    ro = linear3(-d1, dro * sq(r1), -d2, M.s[0] * sq(r2), d3, N.s[0] * sq(r3), 0.0) / sq(r);
+    ro = dv(linear3(-d1, dro * sq(r1), -d2, M.s[0] * sq(r2), d3, N.s[0] * sq(r3), 0.0), sq(r));
     // the cells' r^(2 gamma) come with the cells (computed once per row load as
     // pow(sqrt(cx^2 + cy^2), 2 gamma), the same call with the same argument; the mirror
     // image across the axis has the master's radius)
     const double p1 = D.ok ? D.rp : M.rp;
-    pr = linear3(-d1, dp * p1, -d2, M.s[1] * M.rp, d3, N.s[1] * N.rp, 0.0) / pow(r, g2);
+    pr = dv(linear3(-d1, dp * p1, -d2, M.s[1] * M.rp, d3, N.s[1] * N.rp, 0.0), pow(r, g2));
 }
 
 // Radial rescaling of a cell value to the face radius (Setka.cpp:10221-10226):
@@ -250,9 +253,9 @@ template <int NV, class C>
 __device__ __forceinline__ void rescale_c1(double* g, double rc, double dis, double gam, const C& M, double fx,
                                            double fy, bool rotate)
 {
-    g[0] = g[0] * sq(rc) / sq(dis);
-    g[NV - 1] = g[NV - 1] * sq(rc) / sq(dis);
-    g[1] = g[1] * pow(rc / dis, 2 * gam);
+    g[0] = dv(g[0] * sq(rc), sq(dis));
+    g[NV - 1] = dv(g[NV - 1] * sq(rc), sq(dis));
+    g[1] = g[1] * pow(dv(rc, dis), 2 * gam);
     if (rotate) rotate_cell(M, fx, fy, g[2], g[3]);
 }
 
@@ -266,7 +269,7 @@ __device__ __forceinline__ void side_value(const CellT<Ph, TS>& M, const CellT<P
             double g[Ph::NV];
 #pragma unroll
             for (int k = 0; k < Ph::NV; k++) g[k] = M.s[k];
-            const double rc = sqrt(sq(M.cx) + sq(M.cy));
+            const double rc = sqn(sq(M.cx) + sq(M.cy));
             rescale_c1<Ph::NV>(g, rc, dis, gam, M, fx, fy, true);
             double ro, pr;
             tvd_radial<Ph>(M, N, D, m_axis, fx, fy, gam, ro, pr);
@@ -289,7 +292,7 @@ struct Face {          // geometry of a face as seen from its owning cell
 template <class TF>
 __device__ __forceinline__ double cfl_time(double cfl, double cx, double cy, const Face& f, TF uu)
 {
-    if constexpr (sizeof(TF) == 8) return cfl * sqrt(sq(cx - f.fx) + sq(cy - f.fy)) / uu;
+    if constexpr (sizeof(TF) == 8) return dv(cfl * sqn(sq(cx - f.fx) + sq(cy - f.fy)), uu);
     else return double(float(cfl) * dist2f(cx, cy, f.fx, f.fy) * __builtin_amdgcn_rcpf(uu));
 }
 
@@ -298,25 +301,25 @@ __device__ __forceinline__ Face face_geom(double ax, double ay, double bx, doubl
                                           double ax2, double ay2, double bx2, double by2, double dt)
 {
     Face f;
-    f.S = sqrt(sq(bx - ax) + sq(by - ay)) * (ay + by) * 0.5;           // Setka.cpp:10198
+    f.S = sqn(sq(bx - ax) + sq(by - ay)) * (ay + by) * 0.5;            // Setka.cpp:10198
     f.fx = (ax + bx) / 2.0;
     f.fy = (ay + by) / 2.0;
     double t1 = bx - ax, t2 = by - ay;                                    // Gran.cpp:171-181
-    const double nn = sqrt(sq(t1) + sq(t2));
+    const double nn = sqn(sq(t1) + sq(t2));
     if constexpr (sizeof(TF) == 8) {
-        t1 = t1 / nn; t2 = t2 / nn;
+        t1 = dv(t1, nn); t2 = dv(t2, nn);
     } else {                            // FP32 bulk: one FP64 reciprocal
-        const double inn = 1.0 / nn;
+        const double inn = den(nn).r;
         t1 = t1 * inn; t2 = t2 * inn;
     }
     f.n1 = t2; f.n2 = -t1;
     if (MOVING) {
         const double fx2 = (ax2 + bx2) / 2.0, fy2 = (ay2 + by2) / 2.0;
-        f.W = ((fx2 - f.fx) * f.n1 + (fy2 - f.fy) * f.n2) / dt;          // Setka.cpp:10209
+        f.W = dv((fx2 - f.fx) * f.n1 + (fy2 - f.fy) * f.n2, dt);          // Setka.cpp:10209
     } else {
         f.W = 0.0;
     }
-    f.dis = sqrt(sq(f.fx) + sq(f.fy));
+    f.dis = sqn(sq(f.fx) + sq(f.fy));
     return f;
 }
 
@@ -378,7 +381,7 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             const float cL = sr(dv(g * float(L[1]), float(L[0]))), cR = sr(dv(g * float(R[1]), float(R[0])));
             const float SL = mn(u1, u2) - mx(cL, cR), SR = mx(u1, u2) + mx(cL, cR);
             if (SL >= W || SR <= W) {
-                gas_upwind_area<double>(SL >= W ? gl : gr, f.A1, f.A2, f.WS, 1.0 / (gam - 1.0), P);
+                gas_upwind_area<double>(SL >= W ? gl : gr, f.A1, f.A2, f.WS, den(gam - 1.0).r, P);
                 return mx(fabsf(SL), fabsf(SR));
             }
             const Gas<float> fl{ float(L[0]), float(L[1]), float(L[2]), float(L[3]), float(L[4]) };
@@ -408,7 +411,7 @@ __device__ __forceinline__ TF face_flux(const double* L, const double* R, const 
             const float SN = mx(fabsf(SL), fabsf(SR));
             if (SL >= W || SR <= W) {
                 const bool left = SL >= W;
-                mhd_upwind_area<double>(left ? L : R, left ? R : L, f.A1, f.A2, f.WS, f.n1, f.n2, 1.0 / (gam - 1.0),
+                mhd_upwind_area<double>(left ? L : R, left ? R : L, f.A1, f.A2, f.WS, f.n1, f.n2, den(gam - 1.0).r,
                                         double(SN), left, P);
                 return SN;
             }
@@ -433,21 +436,21 @@ __device__ __forceinline__ void boundary_states(const SweepArgs& a, int ftype, c
     constexpr int NV = Ph::NV;
     const gel_params& p = a.prm;
     const double gam = p.gamma;
-    const double radius = sqrt(sq(M.cx) + sq(M.cy));
+    const double radius = sqn(sq(M.cx) + sq(M.cy));
     if (ftype == GEL_F_INNER_SPHERE) {
         double g[NV];
 #pragma unroll
         for (int k = 0; k < NV; k++) g[k] = M.s[k];
         rescale_c1<NV>(g, radius, f.dis, gam, M, f.fx, f.fy, false);
-        const double d = sqrt(f.fx * f.fx + f.fy * f.fy);
-        const double ro = 1.0 / (p.in_rho_div * d * d);
+        const double d = sqn(f.fx * f.fx + f.fy * f.fy);
+        const double ro = dv(1.0, p.in_rho_div * d * d);
         double w[NV];
 #pragma unroll
         for (int k = 0; k < NV; k++) w[k] = 0.0;
         w[0] = ro;
-        w[1] = ro * p.in_p_a * p.in_p_a / p.in_p_den;
-        w[2] = p.in_u * f.fx / d;
-        w[3] = p.in_u * f.fy / d;
+        w[1] = dv(ro * p.in_p_a * p.in_p_a, p.in_p_den);
+        w[2] = dv(p.in_u * f.fx, d);
+        w[3] = dv(p.in_u * f.fy, d);
         w[NV - 1] = ro * p.in_q;
 #pragma unroll
         for (int k = 0; k < NV; k++) { L[k] = g[k]; R[k] = w[k]; }
@@ -585,7 +588,7 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
 {
     if constexpr (FAST) {
         constexpr int NV = Ph::NV;
-        const double iV2 = 1.0 / Volume2, rV = Volume * iV2, ig1 = 1.0 / (gam - 1.0);
+        const double iV2 = den(Volume2).r, rV = Volume * iV2, ig1 = den(gam - 1.0).r;
         const double ro = s[0], p = s[1], u = s[2], v = s[3];
         const double Ar = 0.5 * (vs1 + vs2);
         double w = 0.0, bx = 0.0, by = 0.0, bz = 0.0, BB = 0.0, Sv = p, Sw = 0.0, Sb = 0.0;
@@ -601,7 +604,7 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
         const double E = p * ig1 + ro * (u * u + v * v + w * w) * 0.5 + BB * 0.039788735772973836;
         double ro3 = ro * rV - T1 * (Pot[0] * iV2 - qm);
         if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
-        const double iro3 = 1.0 / ro3;
+        const double iro3 = den(ro3).r;
         const double pv = (NV == 9) ? Sv : p;   // gas: the hoop term is p (Setka.cpp:10440)
         const double u3 = (ro * u * rV - T1 * (Pot[1] * iV2 - q1)) * iro3;
         const double v3 = (ro * v * rV - T1 * ((Pot[2] - pv * Ar) * iV2 - q2)) * iro3;
@@ -623,13 +626,13 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
         o[NV - 1] = Q3;
     } else if constexpr (Ph::NV == 5) {
         const double ro = s[0], p = s[1], u = s[2], v = s[3], Q = s[4];
-        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
-        double Q33 = Q * Volume / Volume2 - (T1 / Volume2) * Pot[4];
+        double ro3 = dv(ro * Volume, Volume2) - T1 * (dv(Pot[0], Volume2) - qm);
+        double Q33 = dv(Q * Volume, Volume2) - dv(T1, Volume2) * Pot[4];
         if (table) Q33 = Q33 + T1 * qQ;
         if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
-        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
-        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - p * 0.5 * (vs1 + vs2)) / Volume2 - q2)) / ro3;
-        double p3 = (((p / (gam - 1) + ro * (u * u + v * v) * 0.5) * Volume / Volume2 - T1 * (Pot[3] / Volume2 - q3)) -
+        const double u3 = dv(dv(ro * u * Volume, Volume2) - T1 * (dv(Pot[1], Volume2) - q1), ro3);
+        const double v3 = dv(dv(ro * v * Volume, Volume2) - T1 * (dv(Pot[2] - p * 0.5 * (vs1 + vs2), Volume2) - q2), ro3);
+        double p3 = ((dv((dv(p, gam - 1) + ro * (u * u + v * v) * 0.5) * Volume, Volume2) - T1 * (dv(Pot[3], Volume2) - q3)) -
                      0.5 * ro3 * (u3 * u3 + v3 * v3)) * (gam - 1);
         if (p3 <= 0) { p3 = 0.000001; n_p++; }
         o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = Q33;
@@ -639,21 +642,21 @@ __device__ __forceinline__ void cell_update(const double* s, const double* Pot, 
         const double bx = s[5], by = s[6], bz = s[7], Q = s[8];
         const double Ar = 0.5 * (vs1 + vs2);
         const double BB = bx * bx + by * by + bz * bz;
-        const double Sv = ro * w * w + p + BB / pi8 - bz * bz / pi4;
-        const double Sw = -(ro * v * w - by * bz / pi4);
+        const double Sv = ro * w * w + p + dv(BB, pi8) - dv(bz * bz, pi4);
+        const double Sw = -(ro * v * w - dv(by * bz, pi4));
         const double Sb = v * bz - w * by;
-        const double E = p / (gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + BB / pi8;
-        double ro3 = ro * Volume / Volume2 - T1 * (Pot[0] / Volume2 - qm);
+        const double E = dv(p, gam - 1) + ro * (u * u + v * v + w * w) * 0.5 + dv(BB, pi8);
+        double ro3 = dv(ro * Volume, Volume2) - T1 * (dv(Pot[0], Volume2) - qm);
         if (ro3 <= 0.0) { ro3 = 0.00001; n_rho++; }
-        const double u3 = (ro * u * Volume / Volume2 - T1 * (Pot[1] / Volume2 - q1)) / ro3;
-        const double v3 = (ro * v * Volume / Volume2 - T1 * ((Pot[2] - Sv * Ar) / Volume2 - q2)) / ro3;
-        const double w3 = (ro * w * Volume / Volume2 - T1 * ((Pot[3] - Sw * Ar) / Volume2)) / ro3;
-        const double E3 = E * Volume / Volume2 - T1 * (Pot[4] / Volume2 - q3);
-        const double bx3 = bx * Volume / Volume2 - T1 * (Pot[5] / Volume2);
-        const double by3 = by * Volume / Volume2 - T1 * (Pot[6] / Volume2);
-        const double bz3 = bz * Volume / Volume2 - T1 * ((Pot[7] - Sb * Ar) / Volume2);
-        const double Q3 = Q * Volume / Volume2 - T1 * (Pot[8] / Volume2 - qQ);
-        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - (bx3 * bx3 + by3 * by3 + bz3 * bz3) / pi8) * (gam - 1);
+        const double u3 = dv(dv(ro * u * Volume, Volume2) - T1 * (dv(Pot[1], Volume2) - q1), ro3);
+        const double v3 = dv(dv(ro * v * Volume, Volume2) - T1 * (dv(Pot[2] - Sv * Ar, Volume2) - q2), ro3);
+        const double w3 = dv(dv(ro * w * Volume, Volume2) - T1 * dv(Pot[3] - Sw * Ar, Volume2), ro3);
+        const double E3 = dv(E * Volume, Volume2) - T1 * (dv(Pot[4], Volume2) - q3);
+        const double bx3 = dv(bx * Volume, Volume2) - T1 * dv(Pot[5], Volume2);
+        const double by3 = dv(by * Volume, Volume2) - T1 * dv(Pot[6], Volume2);
+        const double bz3 = dv(bz * Volume, Volume2) - T1 * dv(Pot[7] - Sb * Ar, Volume2);
+        const double Q3 = dv(Q * Volume, Volume2) - T1 * (dv(Pot[8], Volume2) - qQ);
+        double p3 = (E3 - 0.5 * ro3 * (u3 * u3 + v3 * v3 + w3 * w3) - dv(bx3 * bx3 + by3 * by3 + bz3 * bz3, pi8)) * (gam - 1);
         if (p3 <= 0) { p3 = 0.000001; n_p++; }
         o[0] = ro3; o[1] = p3; o[2] = u3; o[3] = v3; o[4] = w3;
         o[5] = bx3; o[6] = by3; o[7] = bz3; o[8] = Q3;
@@ -832,7 +835,7 @@ __device__ __forceinline__ void mf_boundary_states(const SweepArgs& a, int ftype
     const gel_params& p = a.prm;
 #pragma unroll
     for (int k = 0; k < 5; k++) L[k] = M.s[k];
-    const double radius = sqrt(sq(M.cx) + sq(M.cy));
+    const double radius = sqn(sq(M.cx) + sq(M.cy));
     if (ftype == GEL_F_INNER_SPHERE) {
         if (a.fluid == 1) L[0] = L[0] * pow(radius / f.dis, p.h_pow);
 #pragma unroll
@@ -1488,7 +1491,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         if (upd) {
             const int fa = fslot(i), fb = fslot(i - 1);
             const double x = M.cx, y = M.cy;
-            const double radius = sqrt(sq(x) + sq(y));
+            const double radius = sqn(sq(x) + sq(y));
             if (out_lane && (INNER || !(radius < a.prm.r_skip))) {
                 // faces in the reference's order: edge 0 (polar, left), 1 (radial, above),
                 // 2 (polar, right), 3 (radial, below)
