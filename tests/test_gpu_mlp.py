@@ -30,10 +30,16 @@ def test_forward_matches_bf16_restatement():
     ref, _ = mo.forward(p, X, "bf16")
     exact, _ = mo.forward(p, X.astype(np.float64), "exact")
     scale = np.abs(exact).max()
-    # same BF16 rounding points: differences come from FP32 accumulation order only
-    assert np.max(np.abs(y - ref)) <= 2e-2 * scale
-    # and BF16 compute stays within a few per cent of the FP64 network
-    assert np.sqrt(np.mean((y - exact) ** 2)) <= 2e-2 * np.sqrt(np.mean(exact ** 2))
+    # same BF16 rounding points: the FP32 sums differ only in accumulation order (a few
+    # 1e-8), which rounds an activation to the other BF16 neighbour (2^-9 apart) in a
+    # small fraction of samples; measured on MI355X: 99th percentile 8.6e-8 of the
+    # scale, 0.6 % of outputs above 1e-6, maximum 7.4e-4
+    d = np.abs(y - ref) / scale
+    assert np.quantile(d, 0.99) <= 1e-6
+    assert np.mean(d > 1e-6) <= 0.02
+    assert d.max() <= 3e-3
+    # and BF16 compute stays within about a per cent of the FP64 network (measured 6.3e-3)
+    assert np.sqrt(np.mean((y - exact) ** 2)) <= 1.5e-2 * np.sqrt(np.mean(exact ** 2))
 
 
 def test_gradients_match_restatement():
@@ -45,13 +51,16 @@ def test_gradients_match_restatement():
     p = m.get_params().astype(np.float64)
     lref, gref = mo.loss_and_grad(p, X, T, "bf16")
     lex, gex = mo.loss_and_grad(p, X.astype(np.float64), T.astype(np.float64), "exact")
-    assert abs(loss - lref) <= 1e-3 * lref
+    # measured on MI355X: loss 2.3e-6 relative; per weight and bias block, gradients
+    # 1.6e-5 .. 3.2e-4 of the block norm from the BF16 restatement (its rounding points,
+    # other accumulation order) and 3.9e-3 .. 6.5e-3 from the FP64 network
+    assert abs(loss - lref) <= 2e-5 * lref
     o = 0
     for (i, n) in mo.SIZES:
         for size in (i * n, n):
             a, b, c = g[o:o + size], gref[o:o + size], gex[o:o + size]
-            assert np.linalg.norm(a - b) <= 3e-2 * np.linalg.norm(b) + 1e-7
-            assert np.linalg.norm(a - c) <= 6e-2 * np.linalg.norm(c) + 1e-7
+            assert np.linalg.norm(a - b) <= 2e-3 * np.linalg.norm(b) + 1e-7
+            assert np.linalg.norm(a - c) <= 2e-2 * np.linalg.norm(c) + 1e-7
             o += size
 
 
