@@ -1277,8 +1277,28 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         return n;
     };
 
+    // Static FP32-bulk sweeps with a source: a row's face nodes (r + 1, j), (r + 1, j + 1),
+    // (r, j + 1) and its cells' area and volume are loaded a step ahead, during the
+    // update phase before the step that needs them, so that their latency hides behind
+    // the update instead of opening the faces phase.
+    constexpr bool kGeoAhead = F32 && !MOVING && kSrcDma<TS, MOVING, SRC, FLUX>;
+    NodeP gA{ 1.0, 1.0, 1.0, 1.0 }, gB{ 1.0, 1.0, 1.0, 1.0 }, gC{ 1.0, 1.0, 1.0, 1.0 };
+    double gv1 = 1.0, gvol = 1.0;
+    auto geo_ahead = [&](int r) {
+        gA = gnode(r + 1, j);
+        gB = gnode(r + 1, j + 1);
+        gC = gnode(r, j + 1);
+        gv1 = 1.0; gvol = 1.0;
+        if (col && r >= 0 && r < ni) {
+            const size_t k = (size_t)r * nj + j;
+            gv1 = G.cell(2)[k];
+            gvol = G.cell(3)[k];
+        }
+    };
+
     // initial rows i0 - 2 .. i0 (and node rows i0 - 1 .. i0 + 1)
     for (int r = i0 - 2 + w; r <= i0; r += NW) load_row(r);
+    if constexpr (kGeoAhead) geo_ahead(i0 - 1 + w);
 
     for (int base = i0 - 1; base < i1; base += NW) {
         const int i = base + w;
@@ -1299,7 +1319,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         double vs1 = 1.0, vol = 1.0, vs2 = 1.0, vol2 = 1.0;
         if (act) {
             // static FP32 bulk: node (i + 1, j + 1), shared by the two faces
-            const NodeP nB = (F32 && !MOVING) ? gnode(i + 1, j + 1) : NodeP{ 1.0, 1.0, 1.0, 1.0 };
+            const NodeP nB = kGeoAhead ? gB : (F32 && !MOVING) ? gnode(i + 1, j + 1) : NodeP{ 1.0, 1.0, 1.0, 1.0 };
             M = cell(i, lane);
             const C D = cell(i - 1, lane), N = cell(i + 1, lane), U = cell(i + 2, lane);
             // radial face above row i, nodes (i+1, j) -> (i+1, j+1); below row 0 it is the
@@ -1308,7 +1328,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 const bool inner = (i < 0);
                 Face f;
                 if constexpr (F32) {
-                    NodeP na = MOVING ? node(i + 1, lane) : gnode(i + 1, j);
+                    NodeP na = MOVING ? node(i + 1, lane) : kGeoAhead ? gA : gnode(i + 1, j);
                     NodeP nb = MOVING ? node(i + 1, lane + 1) : nB;
                     if (inner) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom_a<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, idt);
@@ -1362,7 +1382,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                 Face f;
                 if constexpr (F32) {
                     NodeP na = MOVING ? node(i + 1, lane + 1) : nB;
-                    NodeP nb = MOVING ? node(i, lane + 1) : gnode(i, j + 1);
+                    NodeP nb = MOVING ? node(i, lane + 1) : kGeoAhead ? gC : gnode(i, j + 1);
                     if (axl) { const NodeP t = na; na = nb; nb = t; }
                     f = face_geom_a<MOVING>(na.x, na.y, nb.x, nb.y, na.x2, na.y2, nb.x2, nb.y2, idt);
                     if (!(j >= -1 && j < nj)) f = kNoFace;
@@ -1375,7 +1395,10 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
                     f = table_face(F, j >= -1 && j < nj, (size_t)i * (nj + 1) + (j + 1));
                 }
                 if constexpr (!MOVING) {
-                    if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) {
+                    if constexpr (kGeoAhead) {
+                        vs1 = gv1; vol = gvol;
+                        vs2 = vs1; vol2 = vol;
+                    } else if constexpr (kSrcDma<TS, MOVING, SRC, FLUX>) {
                         // the cell's geometry, loaded before the source inputs' LDS DMA
                         // below waits for every load issued so far
                         if (col) {
@@ -1488,6 +1511,7 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
         // ---- update (Setka.cpp:10372-10468)
         if constexpr (kStateDma)
             if (i + NW < i1) dma_row(i + NW + 2);
+        if constexpr (kGeoAhead) geo_ahead(i + NW);
         if (upd) {
             const int fa = fslot(i), fb = fslot(i - 1);
             const double x = M.cx, y = M.cy;
