@@ -59,8 +59,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the batched ensemble's sweep kernel (hybrid source), measured
 # with rocprofv3 --pmc: FETCH_SIZE (scaled x2: a 1 GiB device copy reads as 524 302 KiB)
-# plus WRITE_SIZE, in KiB per dispatch (profiles/sweep_batch_ensemble_64_cases_fp32_hybrid.md)
-ENSEMBLE_TRAFFIC_BYTES = (2 * 543194 + 196268) * 1024
+# plus WRITE_SIZE, in KiB per dispatch, mean over the timed graph-replayed steps
+# (profiles/sweep_batch_ensemble_64_cases_fp32_hybrid.md)
+ENSEMBLE_TRAFFIC_BYTES = (2 * 549784 + 199636) * 1024
 METRIC = "cell-updates/s on 1 MI355X: FP32+surrogate vs FP64 plain at L1 err ≤1e-4"
 
 
