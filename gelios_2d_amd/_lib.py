@@ -72,6 +72,7 @@ SIGNATURES = {
     "gel_set_tracking": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelTracking), _dp]),
     "gel_get_nodes": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
+    "gel_fp64_div_sqrt": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),
     "gel_flux_hlld_batch_noalloc": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
     "gel_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]),
     "gel_mlp_destroy": (None, [ctypes.c_void_p]),

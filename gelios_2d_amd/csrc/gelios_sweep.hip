@@ -35,6 +35,20 @@ __global__ void hlld_batch_kernel(int n, const double* __restrict__ L, const dou
     uu[k] = hlld<double>(L + 9 * (size_t)k, R + 9 * (size_t)k, W[k], n1[k], n2[k], gam, P + 9 * (size_t)k);
 }
 
+// The sweep's FP64 division and square root (gel_fp64_div_sqrt): q = a / b as dv, three
+// quotients sharing b's reciprocal as the limiter and the update take them, and the
+// square root of a as sqn.
+__global__ void fp64_ops_kernel(int n, const double* __restrict__ a, const double* __restrict__ b,
+                                double* __restrict__ q, double* __restrict__ q2, double* __restrict__ s)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Den d = den(b[k]);
+    q[k] = dv(a[k], b[k]);
+    q2[k] = qt(a[(k + 1) % n], d);
+    s[k] = sqn(a[k]);
+}
+
 // Batched exact Riemann solver for tests (gel_flux_godunov_batch): out [n][10] =
 // q0..q4, dsl, dsp, dsc, dt, status.
 __global__ void godunov_batch_kernel(int n, const double* __restrict__ L, const double* __restrict__ R,
@@ -814,6 +828,17 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
         status[k] = (int)o[9];
     }
     return GEL_OK;
+}
+
+int gel_fp64_div_sqrt(int n, const double* a, const double* b, double* q, double* q2, double* s)
+{
+    if (n < 0 || (n > 0 && (!a || !b || !q || !q2 || !s))) return fail(GEL_E_ARG, "null argument");
+    if (n == 0) return GEL_OK;
+    return batch_run(n, { { a, 1 }, { b, 1 } }, { { q, 1 }, { q2, 1 }, { s, 1 } },
+                     [&](std::vector<double*>& i, std::vector<double*>& d) {
+                         hipLaunchKernelGGL(fp64_ops_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, i[0], i[1],
+                                            d[0], d[1], d[2]);
+                     });
 }
 
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,

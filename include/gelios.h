@@ -331,6 +331,12 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
 int gel_flux_hlld_batch(int n, const double* L, const double* R, const double* W, const double* n1,
                         const double* n2, double gamma, double* P, double* smax);
 
+/* The FP64 division and square root of the sweep's kernels, on the device, for checking
+ * them against IEEE arithmetic (no counterpart in the reference, which divides and takes
+ * square roots with the host's IEEE operations): q = a / b, q2[k] = a[(k + 1) % n] / b[k]
+ * from b[k]'s reciprocal shared as the sweep shares it, s = sqrt(a).  All arrays [n]. */
+int gel_fp64_div_sqrt(int n, const double* a, const double* b, double* q, double* q2, double* s);
+
 /* The same flux with no device allocation (no counterpart in the reference): for
  * callers that cannot allocate device memory, such as serving loops and harnesses
  * that forbid hipMalloc.  The inputs and outputs are staged through a fixed device
