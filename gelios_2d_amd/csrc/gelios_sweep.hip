@@ -922,7 +922,10 @@ struct gel_batch {
     std::vector<gel_ctx*> cases;
     int rows = 0;                 // rows per chunk for every case; 0 = per-case default
     int workgroups = 0;           // persistent grid; 0 = the resident work-groups of the kernel
-    SweepArgs* d_args = nullptr;  // [6][ncases]: one argument set per phase of the cycle
+    SweepArgs* d_args = nullptr;  // [cycle + 1][ncases]: one argument set per phase of the cycle
+    int cycle = 6;                // phases: two state buffers x three time-step slots (x the
+                                  // motion period of a moving batch)
+    bool moving = false;          // every case moves its nodes by its prescribed breathing
     BatchItem* d_items = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
@@ -950,6 +953,12 @@ struct gel_batch {
 };
 
 static constexpr int kCycle = 6;  // two state buffers x three time-step slots
+static int lcm_int(int a, int b)
+{
+    int x = a, y = b;
+    while (y) { const int t = x % y; x = y; y = t; }
+    return a / x * b;
+}
 
 static constexpr int kBatchRows = 4 * kNW - 1;   // rows per chunk until the autotuner picks
 static int batch_rows(const gel_batch* b, const gel_ctx*) { return b->rows > 0 ? b->rows : kBatchRows; }
@@ -983,13 +992,15 @@ static int batch_build_items(gel_batch* b)
     return GEL_OK;
 }
 
-// argument sets of the next kCycle steps, from each case's own buffer/slot phase,
-// and (block kCycle) the cases as they are now, for a refresh before any step
+// argument sets of the next b->cycle steps, from each case's own buffer/slot phase
+// (and motion phase), and (block b->cycle) the cases as they are now, for a refresh
+// before any step
 static int batch_upload_args(gel_batch* b)
 {
     const size_t n = b->cases.size();
-    std::vector<SweepArgs> args((kCycle + 1) * n);
-    for (int p = 0; p <= kCycle; p++)
+    const int cyc = b->cycle;
+    std::vector<SweepArgs> args((size_t)(cyc + 1) * n);
+    for (int p = 0; p <= cyc; p++)
         for (size_t cs = 0; cs < n; cs++) {
             gel_ctx* c = b->cases[cs];
             SweepArgs& a = args[p * n + cs];
@@ -998,6 +1009,11 @@ static int batch_upload_args(gel_batch* b)
             a.ni = c->ni; a.nj = c->nj; a.rows = batch_rows(b, c);
             a.dt_rd = (int)(st % 3); a.dt_wr = (int)((st + 1) % 3); a.dt_clr = (int)((st + 2) % 3);
             a.s_old = 1.0; a.s_new = 1.0;
+            if (b->moving) {
+                const long long P = (long long)c->motion.size();
+                a.s_old = c->motion[(size_t)((c->motion_step + p) % P)];
+                a.s_new = c->motion[(size_t)((c->motion_step + p + 1) % P)];
+            }
             a.x = c->d_x; a.y = c->d_y; a.x2 = c->d_x; a.y2 = c->d_y;
             a.now = c->d_state[cur];
             a.next = c->d_state[cur ^ 1];
@@ -1008,7 +1024,7 @@ static int batch_upload_args(gel_batch* b)
             for (int v = 0; v < 5; v++) a.voff[v] = v;
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
-            if (p == kCycle) a.next = c->d_state[c->cur];
+            if (p == cyc) a.next = c->d_state[c->cur];
         }
     HIP_OK(hipMemcpyAsync(b->d_args, args.data(), args.size() * sizeof(SweepArgs), hipMemcpyHostToDevice, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
@@ -1020,25 +1036,23 @@ static hipError_t batch_launch(const gel_batch* b, int phase)
     const gel_params& p = b->cases[0]->prm;
     const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
     if (p.precision == GEL_PREC_FP32_BULK)
-        return launch_batch_fp32(p, args, b->d_items, b->nitems, b->workgroups, b->stream);
+        return launch_batch_fp32(p, args, b->d_items, b->nitems, b->workgroups, b->stream, b->moving);
     if (p.physics == GEL_PHYS_MHD)
         return batch_launch_s<MhdPhys, double, double, 3>(args, b->d_items, b->nitems, b->workgroups, b->stream,
-                                                          p.source_mode);
+                                                          p.source_mode, b->moving);
     if (p.flux == GEL_FLUX_HLLC)
         return batch_launch_s<GasPhys, double, double, GEL_FLUX_HLLC>(args, b->d_items, b->nitems, b->workgroups,
-                                                                      b->stream, p.source_mode);
+                                                                      b->stream, p.source_mode, b->moving);
     return batch_launch_s<GasPhys, double, double, GEL_FLUX_HLL>(args, b->d_items, b->nitems, b->workgroups,
-                                                                 b->stream, p.source_mode);
+                                                                 b->stream, p.source_mode, b->moving);
 }
 
 // steps per captured graph: the time-step / buffer cycle, times what makes the
 // refresh schedule repeat
 static int batch_cycle(const gel_batch* b)
 {
-    if (!b->model || b->every <= 0) return kCycle;
-    int a = kCycle, c = b->every;
-    while (c) { const int t = a % c; a = c; c = t; }
-    return kCycle / a * b->every;
+    if (!b->model || b->every <= 0) return b->cycle;
+    return lcm_int(b->cycle, b->every);
 }
 
 static long long batch_cells(const gel_batch* b)
@@ -1060,7 +1074,7 @@ static int batch_refresh_offset(const gel_batch* b)
 // i mod 6), then the source refresh when one is due
 static hipError_t batch_launch_step(const gel_batch* b, int i, int off)
 {
-    const int phase = i % kCycle;
+    const int phase = i % b->cycle;
     hipError_t e = batch_launch(b, phase);
     if (e == hipSuccess && b->model && b->every > 0 && (off + i + 1) % b->every == 0) {
         const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
@@ -1143,6 +1157,7 @@ static void batch_advance(gel_batch* b, int n)
     for (gel_ctx* c : b->cases) {
         c->steps += n;
         if (n & 1) c->cur ^= 1;
+        if (b->moving) c->motion_step += n;
     }
 }
 
@@ -1176,12 +1191,17 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
             if (cases[l] == cases[k]) return fail(GEL_E_ARG, "a context appears twice in the batch");
     }
     const gel_params& p0 = cases[0]->prm;
+    const bool moving = !cases[0]->motion.empty();
+    const size_t period = cases[0]->motion.size();
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->has_types || !c->motion.empty() || c->tracking || c->prm.physics == GEL_PHYS_GAS4H ||
-            c->prm.physics == GEL_PHYS_GAS_INNER)
-            return fail(GEL_E_UNSUPPORTED, "batched cases are static single-fluid grids without C_1/C_centr");
+        if (c->has_types || c->tracking || c->prm.physics == GEL_PHYS_GAS4H || c->prm.physics == GEL_PHYS_GAS_INNER)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are single-fluid grids without C_1/C_centr or tracking");
+        // a moving batch: every case breathes with the same period (each with its own
+        // amplitude and phase), so that one cycle of argument sets repeats
+        if (c->motion.empty() == moving || c->motion.size() != period)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are all static or all moving with one motion period");
         if (c->prm.physics != p0.physics || c->prm.precision != p0.precision || c->prm.flux != p0.flux ||
             c->prm.source_mode != p0.source_mode)
             return fail(GEL_E_UNSUPPORTED, "batched cases share physics, precision, flux and source mode");
@@ -1189,11 +1209,15 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
             return fail(GEL_E_STATE, "sources not set");
         if ((c->prm.source_mode == GEL_SRC_TABLE || c->prm.source_mode == GEL_SRC_HYBRID) && !c->d_qtab)
             return fail(GEL_E_STATE, "source table not set");
+        if (moving && (c->prm.source_mode == GEL_SRC_TABLE || c->prm.source_mode == GEL_SRC_HYBRID))
+            return fail(GEL_E_UNSUPPORTED, "moving batches take no source or the analytic closure");
     }
     gel_batch* b = new (std::nothrow) gel_batch();
     if (!b) return fail(GEL_E_NOMEM, "host");
     b->cases.assign(cases, cases + ncases);
-    for (int k = 0; k < ncases; k++) {
+    b->moving = moving;
+    b->cycle = moving ? lcm_int(kCycle, (int)period) : kCycle;
+    for (int k = 0; k < ncases && !moving; k++) {
         const int st = ensure_geometry(cases[k]);
         if (st != GEL_OK) { gel_batch_destroy(b); return st; }
     }
@@ -1203,7 +1227,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     b->refresh_tiles = (long long)tiles.size();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-        hipMalloc(&b->d_args, (size_t)(kCycle + 1) * ncases * sizeof(SweepArgs)) != hipSuccess ||
+        hipMalloc(&b->d_args, (size_t)(b->cycle + 1) * ncases * sizeof(SweepArgs)) != hipSuccess ||
         hipMalloc(&b->d_nfall, sizeof(unsigned long long)) != hipSuccess ||
         hipMemsetAsync(b->d_nfall, 0, sizeof(unsigned long long), b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess) {
@@ -1302,6 +1326,7 @@ int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, 
 int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback)
 {
     if (!b || !m || !(tol >= 0.0) || judge < 0 || judge > 2) return fail(GEL_E_ARG, "bad argument");
+    if (b->moving) return fail(GEL_E_UNSUPPORTED, "moving batches take no learned source");
     for (const gel_ctx* c : b->cases) {
         if (judge == 2 && c->prm.source_mode != GEL_SRC_HYBRID)
             return fail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID cases");
@@ -1315,7 +1340,7 @@ int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long
     unsigned long long* cnt = nullptr;
     if (hipMalloc(&cnt, sizeof *cnt) != hipSuccess) return fail(GEL_E_NOMEM, "counter");
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof *cnt, b->stream);
-    const SweepArgs* args = b->d_args + (size_t)kCycle * b->cases.size();
+    const SweepArgs* args = b->d_args + (size_t)b->cycle * b->cases.size();
     const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
     if (e == hipSuccess)
         e = judge == 2 ? gel_refresh_list_launch(b->stream, args, b->d_tiles, b->refresh_tiles, batch_cells(b),
@@ -1350,7 +1375,7 @@ int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallba
 int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_wg, double* best_ms)
 {
     if (!b || trial_steps < 1) return fail(GEL_E_ARG, "bad argument");
-    struct Saved { void* st[2]; unsigned long long dt[3]; int cur; long long steps; };
+    struct Saved { void* st[2]; unsigned long long dt[3]; int cur; long long steps, motion_step; };
     std::vector<Saved> saved(b->cases.size());
     for (size_t k = 0; k < b->cases.size(); k++) {
         gel_ctx* c = b->cases[k];
@@ -1363,6 +1388,7 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         HIP_OK(hipMemcpy(saved[k].dt, c->d_dt, sizeof saved[k].dt, hipMemcpyDeviceToHost));
         saved[k].cur = c->cur;
         saved[k].steps = c->steps;
+        saved[k].motion_step = c->motion_step;
     }
     auto restore = [&]() -> int {
         for (size_t k = 0; k < b->cases.size(); k++) {
@@ -1372,6 +1398,7 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
             HIP_OK(hipMemcpy(c->d_dt, saved[k].dt, sizeof saved[k].dt, hipMemcpyHostToDevice));
             c->cur = saved[k].cur;
             c->steps = saved[k].steps;
+            c->motion_step = saved[k].motion_step;
         }
         return GEL_OK;
     };
@@ -1384,7 +1411,13 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
     {
         const gel_params& p = b->cases[0]->prm;
         if (p.precision == GEL_PREC_FP32_BULK) {
-            resident = resident_batch_fp32(p);
+            resident = resident_batch_fp32(p, b->moving);
+        } else if (b->moving) {
+            const void* kern =
+                p.physics == GEL_PHYS_MHD
+                    ? (const void*)sweep_batch_kernel<MhdPhys, double, double, GEL_SRC_NONE, 3, kNW, true>
+                    : (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC, kNW, true>;
+            resident = resident_groups(kern, kNW * kWave);
         } else {
             const void* kern = p.physics == GEL_PHYS_MHD
                                    ? (const void*)sweep_batch_kernel<MhdPhys, double, double, GEL_SRC_TABLE, 3, kNW>

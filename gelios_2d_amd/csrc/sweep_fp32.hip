@@ -19,21 +19,27 @@ hipError_t launch_sweep_fp32(gel_ctx* c, SweepArgs& a, bool moving)
 }
 
 hipError_t launch_batch_fp32(const gel_params& p, const SweepArgs* d_args, const BatchItem* d_items, int nitems,
-                             int workgroups, hipStream_t s)
+                             int workgroups, hipStream_t s, bool moving)
 {
     if (p.physics == GEL_PHYS_MHD)
-        return batch_launch_s<MhdPhys, float, float, 3>(d_args, d_items, nitems, workgroups, s, p.source_mode);
+        return batch_launch_s<MhdPhys, float, float, 3>(d_args, d_items, nitems, workgroups, s, p.source_mode, moving);
     if (p.flux == GEL_FLUX_HLLC)
         return batch_launch_s<GasPhys, float, float, GEL_FLUX_HLLC>(d_args, d_items, nitems, workgroups, s,
-                                                                    p.source_mode);
-    return batch_launch_s<GasPhys, float, float, GEL_FLUX_HLL>(d_args, d_items, nitems, workgroups, s, p.source_mode);
+                                                                    p.source_mode, moving);
+    return batch_launch_s<GasPhys, float, float, GEL_FLUX_HLL>(d_args, d_items, nitems, workgroups, s, p.source_mode,
+                                                               moving);
 }
 
-int resident_batch_fp32(const gel_params& p)
+int resident_batch_fp32(const gel_params& p, bool moving)
 {
-    const void* kern = p.physics == GEL_PHYS_MHD
-                           ? (const void*)sweep_batch_kernel<MhdPhys, float, float, GEL_SRC_TABLE, 3, kNW>
-                           : (const void*)sweep_batch_kernel<GasPhys, float, float, GEL_SRC_TABLE, GEL_FLUX_HLLC, kNW>;
+    const void* kern = moving ? (p.physics == GEL_PHYS_MHD
+                                     ? (const void*)sweep_batch_kernel<MhdPhys, float, float, GEL_SRC_NONE, 3, kNW, true>
+                                     : (const void*)sweep_batch_kernel<GasPhys, float, float, GEL_SRC_NONE,
+                                                                       GEL_FLUX_HLLC, kNW, true>)
+                              : (p.physics == GEL_PHYS_MHD
+                                     ? (const void*)sweep_batch_kernel<MhdPhys, float, float, GEL_SRC_TABLE, 3, kNW>
+                                     : (const void*)sweep_batch_kernel<GasPhys, float, float, GEL_SRC_TABLE,
+                                                                       GEL_FLUX_HLLC, kNW>);
     return resident_groups(kern, kNW * kWave);
 }
 

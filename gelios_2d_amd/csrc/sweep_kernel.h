@@ -1701,21 +1701,23 @@ struct BatchItem {
     int cs, bx, by;
 };
 
-template <class Ph, class TS, class TF, int SRC, int FLUX, int NW>
+// MOVING: every case of the batch moves its nodes by its prescribed breathing (the
+// step's scale factors in its argument set).
+template <class Ph, class TS, class TF, int SRC, int FLUX, int NW, bool MOVING = false>
 __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
-    constexpr bool apart = kStateDmaT<TS, false, false, FLUX>;
-    __shared__ MarchLds<Ph, TS, false, false, NW, apart> S;
+    constexpr bool apart = kStateDmaT<TS, MOVING, false, FLUX>;
+    __shared__ MarchLds<Ph, TS, MOVING, false, NW, apart> S;
     __shared__ MarchFlux<Ph, apart ? NW : 0> F;
-    __shared__ TS srcl[kSrcLds<TS, false, SRC, FLUX, NW>];
+    __shared__ TS srcl[kSrcLds<TS, MOVING, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     for (int it = xcd_item(blockIdx.x, gridDim.x); it < nitems; it += gridDim.x) {
         const BatchItem b = items[it];
         const SweepArgs& a = args[b.cs];
         const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
-        if constexpr (apart) march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, F, srcl, b.bx, i0, i1, acc);
-        else march_tile<Ph, TS, TF, false, false, SRC, FLUX, NW>(a, S, S, srcl, b.bx, i0, i1, acc);
+        if constexpr (apart) march_tile<Ph, TS, TF, MOVING, false, SRC, FLUX, NW>(a, S, F, srcl, b.bx, i0, i1, acc);
+        else march_tile<Ph, TS, TF, MOVING, false, SRC, FLUX, NW>(a, S, S, srcl, b.bx, i0, i1, acc);
         march_flush(a, acc);
     }
 }
@@ -1796,21 +1798,29 @@ hipError_t launch_ms(gel_ctx* c, SweepArgs& a, hipStream_t s, bool moving, int s
 
 // One step of a batch: the persistent launch over the (case, strip, chunk) items;
 // workgroups = 0: the kernel's resident work-groups.
-template <class Ph, class TS, class TF, int SRC, int FLUX>
+template <class Ph, class TS, class TF, int SRC, int FLUX, bool MOVING = false>
 hipError_t batch_launch_t(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
                           hipStream_t s)
 {
-    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW>;
+    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW, MOVING>;
     int grid = workgroups > 0 ? workgroups : resident_groups((const void*)kern, kNW * kWave);
     if (grid > nitems) grid = nitems;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, d_args, d_items, nitems);
     return hipGetLastError();
 }
 
+// moving batches (prescribed breathing) take no source or the analytic closure
 template <class Ph, class TS, class TF, int FLUX>
 hipError_t batch_launch_s(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
-                          hipStream_t s, int src)
+                          hipStream_t s, int src, bool moving)
 {
+    if (moving) {
+        if (src == GEL_SRC_MK1)
+            return batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX, true>(d_args, d_items, nitems, workgroups, s);
+        if (src == GEL_SRC_NONE)
+            return batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX, true>(d_args, d_items, nitems, workgroups, s);
+        return hipErrorInvalidValue;
+    }
     if (src == GEL_SRC_MK1) return batch_launch_t<Ph, TS, TF, GEL_SRC_MK1, FLUX>(d_args, d_items, nitems, workgroups, s);
     if (src == GEL_SRC_TABLE) return batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(d_args, d_items, nitems, workgroups, s);
     if (src == GEL_SRC_HYBRID) return batch_launch_t<Ph, TS, TF, GEL_SRC_HYBRID, FLUX>(d_args, d_items, nitems, workgroups, s);
@@ -1820,8 +1830,8 @@ hipError_t batch_launch_s(const SweepArgs* d_args, const BatchItem* d_items, int
 // The FP32-bulk launches live in sweep_fp32.hip (FP contraction on there).
 hipError_t launch_sweep_fp32(gel_ctx* c, SweepArgs& a, bool moving);
 hipError_t launch_batch_fp32(const gel_params& p, const SweepArgs* d_args, const BatchItem* d_items, int nitems,
-                             int workgroups, hipStream_t s);
-int resident_batch_fp32(const gel_params& p);
+                             int workgroups, hipStream_t s, bool moving);
+int resident_batch_fp32(const gel_params& p, bool moving);
 
 }  // namespace gel
 

@@ -187,3 +187,67 @@ def test_batch_sticky_refresh_equals_per_case_refresh():
         assert np.array_equal(x.download(), y.download())
         assert np.array_equal(x.get_source_flags(), y.get_source_flags())
         assert np.array_equal(x.get_source_table(), y.get_source_table())
+
+
+def _moving_setkas(precision, source, period=None):
+    """Three moving MHD heliospheres of different sizes, each breathing with its own
+    amplitude and phase and one common period; with source, the analytic closure of
+    four frozen hydrogen populations."""
+    from gelios_2d_amd.configs import mhd_heliosphere
+    from gelios_2d_amd.setka import Setka
+    from gelios_2d_amd.surrogate import workload_with_hydrogen
+    out = []
+    for k, (ni, nj, seed) in enumerate(((60, 30, 3), (97, 45, 4), (130, 70, 5))):
+        f = None
+        if source:
+            w, f = workload_with_hydrogen(ni, nj, 0, 2.0 + 0.3 * k, 0.1, seed, moving=True)
+        else:
+            w = mhd_heliosphere(ni, nj, 0, seed=seed)
+        w.params.precision = precision
+        amp, per, phase = w.motion
+        s = Setka(w.grid, w.params)
+        s.set_motion(amp * (1.0 + 0.5 * k), period or per, phase + 0.7 * k)
+        s.upload(w.prim0)
+        if f is not None:
+            s.set_sources(f)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("precision,source", [(GEL_PREC_FP64, False), (GEL_PREC_FP64, True), (1, False)])
+def test_moving_batch_equals_single_cases(precision, source):
+    """A batch of moving cases (prescribed breathing, one period, own amplitudes and
+    phases) is bit-identical to each case stepped alone, eagerly and by graph replay
+    over more than one 150-step cycle (6 buffer / time-step phases x the 50-step
+    motion period), and after the autotuner's trials."""
+    from gelios_2d_amd.batch import Batch
+    single = _moving_setkas(precision, source)
+    for s in single:
+        s.Go_5_komponent__MK2(40)
+    ref40 = [s.download() for s in single]
+    for s in single:
+        s.Go_5_komponent__MK2(157)
+    ref = [s.download() for s in single]
+    cases = _moving_setkas(precision, source)
+    b = Batch(cases)
+    b.autotune(3)
+    b.step(17, False)
+    b.step(23, True)
+    for s, r in zip(cases, ref40):
+        assert np.array_equal(s.download(), r)
+    b.step(157, True)
+    for s, r in zip(cases, ref):
+        assert np.array_equal(s.download(), r)
+    b.close()
+
+
+def test_batch_rejects_mixed_motion():
+    from gelios_2d_amd._lib import GelError
+    from gelios_2d_amd.batch import Batch
+    moving = _moving_setkas(GEL_PREC_FP64, False)
+    static = _cases(1, GEL_PREC_FP64)
+    with pytest.raises(GelError):
+        Batch([moving[0], static[0]])
+    other = _moving_setkas(GEL_PREC_FP64, False, period=30)
+    with pytest.raises(GelError):
+        Batch([moving[0], other[1]])
