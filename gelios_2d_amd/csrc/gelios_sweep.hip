@@ -926,6 +926,7 @@ struct gel_batch {
     int cycle = 6;                // phases: two state buffers x three time-step slots (x the
                                   // motion period of a moving batch)
     bool moving = false;          // every case moves its nodes by its prescribed breathing
+    bool types = false;           // every case has C_1 / C_centr cells (gas, FP64)
     BatchItem* d_items = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
@@ -1017,7 +1018,7 @@ static int batch_upload_args(gel_batch* b)
             a.x = c->d_x; a.y = c->d_y; a.x2 = c->d_x; a.y2 = c->d_y;
             a.now = c->d_state[cur];
             a.next = c->d_state[cur ^ 1];
-            a.ctype = nullptr; a.otype = c->d_otype;
+            a.ctype = b->types ? c->d_ctype : nullptr; a.otype = c->d_otype;
             a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag; a.geo = c->d_geo;
             a.srow[0] = a.srow[1] = a.srow[2] = -1;
             a.fluid = 0;
@@ -1037,6 +1038,12 @@ static hipError_t batch_launch(const gel_batch* b, int phase)
     const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
     if (p.precision == GEL_PREC_FP32_BULK)
         return launch_batch_fp32(p, args, b->d_items, b->nitems, b->workgroups, b->stream, b->moving);
+    if (b->types)
+        return p.flux == GEL_FLUX_HLLC
+                   ? batch_launch_typed<GEL_FLUX_HLLC>(args, b->d_items, b->nitems, b->workgroups, b->stream,
+                                                       p.source_mode, b->moving)
+                   : batch_launch_typed<GEL_FLUX_HLL>(args, b->d_items, b->nitems, b->workgroups, b->stream,
+                                                      p.source_mode, b->moving);
     if (p.physics == GEL_PHYS_MHD)
         return batch_launch_s<MhdPhys, double, double, 3>(args, b->d_items, b->nitems, b->workgroups, b->stream,
                                                           p.source_mode, b->moving);
@@ -1193,11 +1200,18 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     const gel_params& p0 = cases[0]->prm;
     const bool moving = !cases[0]->motion.empty();
     const size_t period = cases[0]->motion.size();
+    const bool types = cases[0]->has_types;
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->has_types || c->tracking || c->prm.physics == GEL_PHYS_GAS4H || c->prm.physics == GEL_PHYS_GAS_INNER)
-            return fail(GEL_E_UNSUPPORTED, "batched cases are single-fluid grids without C_1/C_centr or tracking");
+        if (c->tracking || c->prm.physics == GEL_PHYS_GAS4H || c->prm.physics == GEL_PHYS_GAS_INNER)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are single-fluid grids without tracking");
+        // grids with C_1 / C_centr cells: all of them, gas in FP64
+        if (c->has_types != types)
+            return fail(GEL_E_UNSUPPORTED, "batched cases all have C_1 / C_centr cells or none has");
+        if (types && (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64 ||
+                      (c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)))
+            return fail(GEL_E_UNSUPPORTED, "batched cases with C_1 / C_centr cells: gas, FP64, no source or MK1");
         // a moving batch: every case breathes with the same period (each with its own
         // amplitude and phase), so that one cycle of argument sets repeats
         if (c->motion.empty() == moving || c->motion.size() != period)
@@ -1216,6 +1230,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     if (!b) return fail(GEL_E_NOMEM, "host");
     b->cases.assign(cases, cases + ncases);
     b->moving = moving;
+    b->types = types;
     b->cycle = moving ? lcm_int(kCycle, (int)period) : kCycle;
     for (int k = 0; k < ncases && !moving; k++) {
         const int st = ensure_geometry(cases[k]);
@@ -1412,6 +1427,11 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         const gel_params& p = b->cases[0]->prm;
         if (p.precision == GEL_PREC_FP32_BULK) {
             resident = resident_batch_fp32(p, b->moving);
+        } else if (b->types) {
+            const void* kern =
+                b->moving ? (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC, kNW, true, true>
+                          : (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC, kNW, false, true>;
+            resident = resident_groups(kern, kNW * kWave);
         } else if (b->moving) {
             const void* kern =
                 p.physics == GEL_PHYS_MHD

@@ -1702,13 +1702,13 @@ struct BatchItem {
 };
 
 // MOVING: every case of the batch moves its nodes by its prescribed breathing (the
-// step's scale factors in its argument set).
-template <class Ph, class TS, class TF, int SRC, int FLUX, int NW, bool MOVING = false>
+// step's scale factors in its argument set).  TYPES: every case has C_1 / C_centr cells.
+template <class Ph, class TS, class TF, int SRC, int FLUX, int NW, bool MOVING = false, bool TYPES = false>
 __global__ void __launch_bounds__(NW * kWave) __attribute__((amdgpu_waves_per_eu(2)))
 sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restrict__ items, int nitems)
 {
-    constexpr bool apart = kStateDmaT<TS, MOVING, false, FLUX>;
-    __shared__ MarchLds<Ph, TS, MOVING, false, NW, apart> S;
+    constexpr bool apart = kStateDmaT<TS, MOVING, TYPES, FLUX>;
+    __shared__ MarchLds<Ph, TS, MOVING, TYPES, NW, apart> S;
     __shared__ MarchFlux<Ph, apart ? NW : 0> F;
     __shared__ TS srcl[kSrcLds<TS, MOVING, SRC, FLUX, NW>];
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
@@ -1716,8 +1716,8 @@ sweep_batch_kernel(const SweepArgs* __restrict__ args, const BatchItem* __restri
         const BatchItem b = items[it];
         const SweepArgs& a = args[b.cs];
         const int i0 = b.by * a.rows, i1 = min(i0 + a.rows, a.ni);
-        if constexpr (apart) march_tile<Ph, TS, TF, MOVING, false, SRC, FLUX, NW>(a, S, F, srcl, b.bx, i0, i1, acc);
-        else march_tile<Ph, TS, TF, MOVING, false, SRC, FLUX, NW>(a, S, S, srcl, b.bx, i0, i1, acc);
+        if constexpr (apart) march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, F, srcl, b.bx, i0, i1, acc);
+        else march_tile<Ph, TS, TF, MOVING, TYPES, SRC, FLUX, NW>(a, S, S, srcl, b.bx, i0, i1, acc);
         march_flush(a, acc);
     }
 }
@@ -1798,11 +1798,11 @@ hipError_t launch_ms(gel_ctx* c, SweepArgs& a, hipStream_t s, bool moving, int s
 
 // One step of a batch: the persistent launch over the (case, strip, chunk) items;
 // workgroups = 0: the kernel's resident work-groups.
-template <class Ph, class TS, class TF, int SRC, int FLUX, bool MOVING = false>
+template <class Ph, class TS, class TF, int SRC, int FLUX, bool MOVING = false, bool TYPES = false>
 hipError_t batch_launch_t(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
                           hipStream_t s)
 {
-    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW, MOVING>;
+    const auto kern = sweep_batch_kernel<Ph, TS, TF, SRC, FLUX, kNW, MOVING, TYPES>;
     int grid = workgroups > 0 ? workgroups : resident_groups((const void*)kern, kNW * kWave);
     if (grid > nitems) grid = nitems;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(kNW * kWave), 0, s, d_args, d_items, nitems);
@@ -1825,6 +1825,23 @@ hipError_t batch_launch_s(const SweepArgs* d_args, const BatchItem* d_items, int
     if (src == GEL_SRC_TABLE) return batch_launch_t<Ph, TS, TF, GEL_SRC_TABLE, FLUX>(d_args, d_items, nitems, workgroups, s);
     if (src == GEL_SRC_HYBRID) return batch_launch_t<Ph, TS, TF, GEL_SRC_HYBRID, FLUX>(d_args, d_items, nitems, workgroups, s);
     return batch_launch_t<Ph, TS, TF, GEL_SRC_NONE, FLUX>(d_args, d_items, nitems, workgroups, s);
+}
+
+// Batches of grids with C_1 / C_centr cells (gas, FP64): no source or the analytic
+// closure, static or moving.
+template <int FLUX>
+hipError_t batch_launch_typed(const SweepArgs* d_args, const BatchItem* d_items, int nitems, int workgroups,
+                              hipStream_t s, int src, bool moving)
+{
+    using G = GasPhys;
+    if (src != GEL_SRC_NONE && src != GEL_SRC_MK1) return hipErrorInvalidValue;
+    if (moving)
+        return src == GEL_SRC_MK1
+                   ? batch_launch_t<G, double, double, GEL_SRC_MK1, FLUX, true, true>(d_args, d_items, nitems, workgroups, s)
+                   : batch_launch_t<G, double, double, GEL_SRC_NONE, FLUX, true, true>(d_args, d_items, nitems, workgroups, s);
+    return src == GEL_SRC_MK1
+               ? batch_launch_t<G, double, double, GEL_SRC_MK1, FLUX, false, true>(d_args, d_items, nitems, workgroups, s)
+               : batch_launch_t<G, double, double, GEL_SRC_NONE, FLUX, false, true>(d_args, d_items, nitems, workgroups, s);
 }
 
 // The FP32-bulk launches live in sweep_fp32.hip (FP contraction on there).

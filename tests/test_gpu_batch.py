@@ -251,3 +251,45 @@ def test_batch_rejects_mixed_motion():
     other = _moving_setkas(GEL_PREC_FP64, False, period=30)
     with pytest.raises(GelError):
         Batch([moving[0], other[1]])
+
+
+def _typed_setkas(moving, source):
+    """Gas heliospheres (HLLC, FP64) with C_1 wind rows and two C_centr cells, static or
+    breathing; with source, the analytic closure of frozen hydrogen."""
+    from gelios_2d_amd.configs import MOTION_AMP, MOTION_PERIOD, heliosphere, motion_phase
+    from gelios_2d_amd.hydrogen import populations
+    from gelios_2d_amd.params import GEL_C_1, GEL_C_CENTR, GEL_SRC_MK1
+    from gelios_2d_amd.setka import Setka
+    out = []
+    for k, (ni, nj, seed) in enumerate(((50, 40, 21), (83, 67, 22), (120, 61, 23))):
+        w = heliosphere(ni, nj, 0, seed=seed, flux=1)
+        w.grid.cell_type[: (6 + 2 * k) * nj] = GEL_C_1
+        w.grid.cell_type[[nj // 2, 3 * nj // 2]] = GEL_C_CENTR
+        if source:
+            w.params.source_mode = GEL_SRC_MK1
+        s = Setka(w.grid, w.params)
+        if moving:
+            s.set_motion(MOTION_AMP * (1 + k), MOTION_PERIOD, motion_phase(seed))
+        s.upload(w.prim0)
+        if source:
+            s.set_sources(populations(w.grid, n0=0.1, seed=seed))
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("moving,source", [(False, False), (True, False), (False, True), (True, True)])
+def test_typed_batch_equals_single_cases(moving, source):
+    """Grids with C_1 (radial reconstruction, rotated wind) and C_centr (frozen) cells in
+    one batch: bit-identical with each case stepped alone, eagerly and by graph."""
+    from gelios_2d_amd.batch import Batch
+    single = _typed_setkas(moving, source)
+    for s in single:
+        s.Go_5_komponent__MK2(61)
+    ref = [s.download() for s in single]
+    cases = _typed_setkas(moving, source)
+    b = Batch(cases)
+    b.step(19, False)
+    b.step(42, True)
+    for s, r in zip(cases, ref):
+        assert np.array_equal(s.download(), r)
+    b.close()
