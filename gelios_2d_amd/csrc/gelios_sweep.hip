@@ -927,6 +927,8 @@ struct gel_batch {
                                   // motion period of a moving batch)
     bool moving = false;          // every case moves its nodes by its prescribed breathing
     bool types = false;           // every case has C_1 / C_centr cells (gas, FP64)
+    int nfl = 1;                  // sweep launches per step: 5 for the plasma + four hydrogen
+                                  // fluids (one argument block per fluid and phase)
     BatchItem* d_items = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
@@ -999,12 +1001,13 @@ static int batch_build_items(gel_batch* b)
 static int batch_upload_args(gel_batch* b)
 {
     const size_t n = b->cases.size();
-    const int cyc = b->cycle;
-    std::vector<SweepArgs> args((size_t)(cyc + 1) * n);
-    for (int p = 0; p <= cyc; p++)
+    const int cyc = b->cycle, nfl = b->nfl;
+    std::vector<SweepArgs> args((size_t)(cyc + 1) * nfl * n);
+    for (int pf = 0; pf < (cyc + 1) * nfl; pf++)
         for (size_t cs = 0; cs < n; cs++) {
+            const int p = pf / nfl, fl = pf % nfl;
             gel_ctx* c = b->cases[cs];
-            SweepArgs& a = args[p * n + cs];
+            SweepArgs& a = args[pf * n + cs];
             const long long st = c->steps + p;
             const int cur = c->cur ^ (p & 1);
             a.ni = c->ni; a.nj = c->nj; a.rows = batch_rows(b, c);
@@ -1023,6 +1026,14 @@ static int batch_upload_args(gel_batch* b)
             a.srow[0] = a.srow[1] = a.srow[2] = -1;
             a.fluid = 0;
             for (int v = 0; v < 5; v++) a.voff[v] = v;
+            a.mfq = c->d_mfq;
+            if (nfl == 5) {
+                // the plasma: ro, p, u, v, Q; fluid Hf: ro_Hf, p_Hf, u_Hf, v_Hf and the plasma Q
+                a.fluid = fl;
+                const int o = fl == 0 ? 0 : 5 + 4 * (fl - 1);
+                for (int k = 0; k < 4; k++) a.voff[k] = o + k;
+                a.voff[4] = 4;
+            }
             a.dtbuf = c->d_dt; a.counters = c->d_cnt;
             a.prm = c->prm;
             if (p == cyc) a.next = c->d_state[c->cur];
@@ -1032,9 +1043,37 @@ static int batch_upload_args(gel_batch* b)
     return GEL_OK;
 }
 
+template <int FX>
+static hipError_t batch_launch_mf_sweep(const gel_batch* b, const SweepArgs* args)
+{
+    if (b->moving)
+        return batch_launch_t<GasPhys, double, double, GEL_SRC_NONE, FX | kMfFlux, true, true>(args, b->d_items, b->nitems,
+                                                                                            b->workgroups, b->stream);
+    return batch_launch_t<GasPhys, double, double, GEL_SRC_NONE, FX | kMfFlux, false, true>(args, b->d_items, b->nitems,
+                                                                                         b->workgroups, b->stream);
+}
+
+// One step of a batch of plasma + hydrogen cases: the charge-exchange sources of every
+// case, then the sweep of each of the five fluids over every case.
+static hipError_t batch_launch_mf(const gel_batch* b, int phase)
+{
+    const size_t n = b->cases.size();
+    const SweepArgs* args0 = b->d_args + (size_t)phase * 5 * n;
+    hipLaunchKernelGGL(mf_source_batch_kernel<true>, dim3((unsigned)b->refresh_tiles), dim3(256), 0, b->stream, args0,
+                       b->d_tiles);
+    hipError_t e = hipGetLastError();
+    for (int f = 0; f < 5 && e == hipSuccess; f++) {
+        const SweepArgs* args = args0 + (size_t)f * n;
+        e = b->cases[0]->prm.flux == GEL_FLUX_HLLC ? batch_launch_mf_sweep<GEL_FLUX_HLLC>(b, args)
+                                                   : batch_launch_mf_sweep<GEL_FLUX_HLL>(b, args);
+    }
+    return e;
+}
+
 static hipError_t batch_launch(const gel_batch* b, int phase)
 {
     const gel_params& p = b->cases[0]->prm;
+    if (b->nfl == 5) return batch_launch_mf(b, phase);
     const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
     if (p.precision == GEL_PREC_FP32_BULK)
         return launch_batch_fp32(p, args, b->d_items, b->nitems, b->workgroups, b->stream, b->moving);
@@ -1084,7 +1123,7 @@ static hipError_t batch_launch_step(const gel_batch* b, int i, int off)
     const int phase = i % b->cycle;
     hipError_t e = batch_launch(b, phase);
     if (e == hipSuccess && b->model && b->every > 0 && (off + i + 1) % b->every == 0) {
-        const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
+        const SweepArgs* args = b->d_args + (size_t)phase * b->nfl * b->cases.size();
         const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
         if (b->judge == 2)
             e = gel_refresh_list_launch(b->stream, args, b->d_tiles, b->refresh_tiles, batch_cells(b), b->d_list,
@@ -1204,13 +1243,14 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->tracking || c->prm.physics == GEL_PHYS_GAS4H || c->prm.physics == GEL_PHYS_GAS_INNER)
-            return fail(GEL_E_UNSUPPORTED, "batched cases are single-fluid grids without tracking");
+        if (c->tracking || c->prm.physics == GEL_PHYS_GAS_INNER)
+            return fail(GEL_E_UNSUPPORTED, "batched cases are grids without tracking, not the inner region");
         // grids with C_1 / C_centr cells: all of them, gas in FP64
         if (c->has_types != types)
             return fail(GEL_E_UNSUPPORTED, "batched cases all have C_1 / C_centr cells or none has");
-        if (types && (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64 ||
-                      (c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)))
+        if (types && c->prm.physics != GEL_PHYS_GAS4H &&
+            (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64 ||
+             (c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)))
             return fail(GEL_E_UNSUPPORTED, "batched cases with C_1 / C_centr cells: gas, FP64, no source or MK1");
         // a moving batch: every case breathes with the same period (each with its own
         // amplitude and phase), so that one cycle of argument sets repeats
@@ -1231,6 +1271,14 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     b->cases.assign(cases, cases + ncases);
     b->moving = moving;
     b->types = types;
+    b->nfl = p0.physics == GEL_PHYS_GAS4H ? 5 : 1;
+    for (int k = 0; k < ncases && b->nfl == 5; k++) {
+        gel_ctx* c = cases[k];
+        if (!c->d_mfq && hipMalloc((void**)&c->d_mfq, 20 * c->nc * sizeof(double)) != hipSuccess) {
+            gel_batch_destroy(b);
+            return fail(GEL_E_NOMEM, "plasma + hydrogen sources");
+        }
+    }
     b->cycle = moving ? lcm_int(kCycle, (int)period) : kCycle;
     for (int k = 0; k < ncases && !moving; k++) {
         const int st = ensure_geometry(cases[k]);
@@ -1242,7 +1290,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     b->refresh_tiles = (long long)tiles.size();
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
-        hipMalloc(&b->d_args, (size_t)(b->cycle + 1) * ncases * sizeof(SweepArgs)) != hipSuccess ||
+        hipMalloc(&b->d_args, (size_t)(b->cycle + 1) * b->nfl * ncases * sizeof(SweepArgs)) != hipSuccess ||
         hipMalloc(&b->d_nfall, sizeof(unsigned long long)) != hipSuccess ||
         hipMemsetAsync(b->d_nfall, 0, sizeof(unsigned long long), b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess) {
@@ -1355,7 +1403,7 @@ int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long
     unsigned long long* cnt = nullptr;
     if (hipMalloc(&cnt, sizeof *cnt) != hipSuccess) return fail(GEL_E_NOMEM, "counter");
     hipError_t e = hipMemsetAsync(cnt, 0, sizeof *cnt, b->stream);
-    const SweepArgs* args = b->d_args + (size_t)b->cycle * b->cases.size();
+    const SweepArgs* args = b->d_args + (size_t)b->cycle * b->nfl * b->cases.size();
     const bool f32 = b->cases[0]->prm.precision == GEL_PREC_FP32_BULK;
     if (e == hipSuccess)
         e = judge == 2 ? gel_refresh_list_launch(b->stream, args, b->d_tiles, b->refresh_tiles, batch_cells(b),
@@ -1427,6 +1475,11 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         const gel_params& p = b->cases[0]->prm;
         if (p.precision == GEL_PREC_FP32_BULK) {
             resident = resident_batch_fp32(p, b->moving);
+        } else if (b->nfl == 5) {
+            const void* kern =
+                b->moving ? (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC | kMfFlux, kNW, true, true>
+                          : (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC | kMfFlux, kNW, false, true>;
+            resident = resident_groups(kern, kNW * kWave);
         } else if (b->types) {
             const void* kern =
                 b->moving ? (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC, kNW, true, true>

@@ -920,10 +920,9 @@ __device__ __forceinline__ void mf_sources_all(const gel_params& prm, const doub
 // the state the five launches of the step read (the reference takes them from
 // par[now1] for every fluid).
 template <bool TYPES>
-__global__ void __launch_bounds__(256) mf_source_kernel(SweepArgs a, double* __restrict__ out)
+__device__ __forceinline__ void mf_source_cell(const SweepArgs& a, size_t kc, double* __restrict__ out)
 {
     const size_t nc = (size_t)a.ni * a.nj;
-    const size_t kc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (kc >= nc) return;
     const int ct = TYPES ? (int)a.ctype[kc] : (int)GEL_C_2;
     double q[5][4];
@@ -932,6 +931,21 @@ __global__ void __launch_bounds__(256) mf_source_kernel(SweepArgs a, double* __r
     for (int f = 0; f < 5; f++)
 #pragma unroll
         for (int k = 0; k < 4; k++) out[((size_t)f * 4 + k) * nc + kc] = q[f][k];
+}
+template <bool TYPES>
+__global__ void __launch_bounds__(256) mf_source_kernel(SweepArgs a, double* __restrict__ out)
+{
+    mf_source_cell<TYPES>(a, (size_t)blockIdx.x * blockDim.x + threadIdx.x, out);
+}
+// The same for every case of a batch: work-group g takes the 256 cells of tile
+// tiles[g].y of case tiles[g].x; each case's sources go to its own args.mfq.
+template <bool TYPES>
+__global__ void __launch_bounds__(256) mf_source_batch_kernel(const SweepArgs* __restrict__ args,
+                                                              const int2* __restrict__ tiles)
+{
+    const int2 t = tiles[blockIdx.x];
+    const SweepArgs& a = args[t.x];
+    mf_source_cell<TYPES>(a, (size_t)t.y * 256 + threadIdx.x, (double*)a.mfq);
 }
 
 // Update of a hydrogen fluid (Setka.cpp:9619-9753): the gas update with a mass source

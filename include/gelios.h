@@ -361,10 +361,11 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
                            int* status);
 
 /* ---- batched cases (no counterpart in the reference, which runs one grid) ----
- * Many independent contexts (polar grids without tracking, same physics / precision /
- * flux / source mode) advanced together: one persistent launch per step walks the
- * (case, strip, chunk) work list; each case keeps its own state, buffers and CFL time
- * step.  The cases are all static, or all move by prescribed breathing (gel_set_motion)
+ * Many independent contexts (polar grids without tracking, not the inner region, same
+ * physics / precision / flux / source mode) advanced together: one persistent launch
+ * per step walks the (case, strip, chunk) work list (the plasma + hydrogen physics: one
+ * source launch and one launch per fluid); each case keeps its own state, buffers and
+ * CFL time step.  The cases are all static, or all move by prescribed breathing (gel_set_motion)
  * with one common period, each with its own amplitude and phase; all have C_1 /
  * C_centr cells (the grid's cell_type: gas, FP64) or none has.  Moving batches and
  * batches with cell types take GEL_SRC_NONE or GEL_SRC_MK1 (GEL_E_UNSUPPORTED

@@ -293,3 +293,38 @@ def test_typed_batch_equals_single_cases(moving, source):
     for s, r in zip(cases, ref):
         assert np.array_equal(s.download(), r)
     b.close()
+
+
+def _mf_setkas(moving):
+    """Plasma + four hydrogen fluids (HLLC, FP64, C_1 .. C_5 source regions) on three
+    grids of different sizes, static or breathing."""
+    from gelios_2d_amd.configs import MOTION_AMP, MOTION_PERIOD, motion_phase, multifluid_heliosphere
+    from gelios_2d_amd.setka import Setka
+    out = []
+    for k, (ni, nj, seed) in enumerate(((40, 30, 31), (71, 50, 32), (96, 63, 33))):
+        w = multifluid_heliosphere(ni, nj, 0, seed=seed)
+        s = Setka(w.grid, w.params)
+        if moving:
+            s.set_motion(MOTION_AMP * (1 + k), MOTION_PERIOD, motion_phase(seed))
+        s.upload(w.prim0)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("moving", [False, True])
+def test_multifluid_batch_equals_single_cases(moving):
+    """Plasma + hydrogen cases in one batch (one source launch and five fluid sweeps per
+    step over every case): bit-identical with each case stepped alone, eagerly and by
+    graph replay."""
+    from gelios_2d_amd.batch import Batch
+    single = _mf_setkas(moving)
+    for s in single:
+        s.Go_5_komponent__MK2(37)
+    ref = [s.download() for s in single]
+    cases = _mf_setkas(moving)
+    b = Batch(cases)
+    b.step(11, False)
+    b.step(26, True)
+    for s, r in zip(cases, ref):
+        assert np.array_equal(s.download(), r)
+    b.close()
