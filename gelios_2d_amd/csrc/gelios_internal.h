@@ -93,6 +93,13 @@ hipError_t gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving);
 // One tracked step (tracking.hip): surface node speeds, rail redistribution into
 // ctx->d_x2/d_y2, and the sweep with surface faces; asynchronous on ctx->stream.
 hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a);
+struct BatchItem;
+// The same for every case of a batch (args[k].x2 / y2 receive case k's next nodes);
+// stiles / rtiles: (case, block of 256) of the surface faces / of the nodes.
+hipError_t launch_tracked_batch(const SweepArgs* args, const TrackDev* tdev, const int2* stiles, int nstiles,
+                                const int2* rtiles, int nrtiles, const BatchItem* items, int nitems, int workgroups,
+                                hipStream_t s, bool hllc, bool types, int src);
+int resident_tracked_batch(bool types);
 // One step of the plasma + hydrogen sweep (multifluid.hip): five launches, one per fluid.
 hipError_t launch_mf_step(gel_ctx* c, SweepArgs& a, bool moving);
 // One step of the inner-region solver (inner.hip), static grids.

@@ -929,6 +929,13 @@ struct gel_batch {
     bool types = false;           // every case has C_1 / C_centr cells (gas, FP64)
     int nfl = 1;                  // sweep launches per step: 5 for the plasma + four hydrogen
                                   // fluids (one argument block per fluid and phase)
+    // fitted-surface tracking: every case's tracking tables, and the (case, block) work
+    // lists of the batched surface and rail launches
+    bool tracked = false;
+    TrackDev* d_tdev = nullptr;
+    int2* d_stiles = nullptr;
+    int2* d_rtiles = nullptr;
+    int nstiles = 0, nrtiles = 0;
     BatchItem* d_items = nullptr;
     int nitems = 0;
     hipStream_t stream = nullptr;
@@ -1019,11 +1026,19 @@ static int batch_upload_args(gel_batch* b)
                 a.s_new = c->motion[(size_t)((c->motion_step + p + 1) % P)];
             }
             a.x = c->d_x; a.y = c->d_y; a.x2 = c->d_x; a.y2 = c->d_y;
+            if (b->tracked) {
+                // the rail launch writes the next nodes into the other buffer; the two
+                // buffers change roles every step (Point::x = x2, Setka.cpp:10473-10480)
+                const bool odd = (p & 1) != 0;
+                a.x = odd ? c->d_x2 : c->d_x;  a.y = odd ? c->d_y2 : c->d_y;
+                a.x2 = odd ? c->d_x : c->d_x2; a.y2 = odd ? c->d_y : c->d_y2;
+                a.srow[0] = c->trk.row_inner_shock; a.srow[1] = c->trk.row_contact; a.srow[2] = c->trk.row_outer_shock;
+            }
             a.now = c->d_state[cur];
             a.next = c->d_state[cur ^ 1];
             a.ctype = b->types ? c->d_ctype : nullptr; a.otype = c->d_otype;
             a.src = c->d_src; a.qtab = c->d_qtab; a.qflag = c->d_qflag; a.geo = c->d_geo;
-            a.srow[0] = a.srow[1] = a.srow[2] = -1;
+            if (!b->tracked) a.srow[0] = a.srow[1] = a.srow[2] = -1;
             a.fluid = 0;
             for (int v = 0; v < 5; v++) a.voff[v] = v;
             a.mfq = c->d_mfq;
@@ -1075,6 +1090,10 @@ static hipError_t batch_launch(const gel_batch* b, int phase)
     const gel_params& p = b->cases[0]->prm;
     if (b->nfl == 5) return batch_launch_mf(b, phase);
     const SweepArgs* args = b->d_args + (size_t)phase * b->cases.size();
+    if (b->tracked)
+        return launch_tracked_batch(args, b->d_tdev, b->d_stiles, b->nstiles, b->d_rtiles, b->nrtiles, b->d_items,
+                                    b->nitems, b->workgroups, b->stream, p.flux == GEL_FLUX_HLLC, b->types,
+                                    p.source_mode);
     if (p.precision == GEL_PREC_FP32_BULK)
         return launch_batch_fp32(p, args, b->d_items, b->nitems, b->workgroups, b->stream, b->moving);
     if (b->types)
@@ -1204,6 +1223,11 @@ static void batch_advance(gel_batch* b, int n)
         c->steps += n;
         if (n & 1) c->cur ^= 1;
         if (b->moving) c->motion_step += n;
+        if (b->tracked && (n & 1)) {
+            std::swap(c->d_x, c->d_x2);
+            std::swap(c->d_y, c->d_y2);
+            c->geo_valid = false;
+        }
     }
 }
 
@@ -1219,6 +1243,9 @@ void gel_batch_destroy(gel_batch* b)
     (void)hipFree(b->d_tiles);
     (void)hipFree(b->d_list);
     (void)hipFree(b->d_lcount);
+    (void)hipFree(b->d_tdev);
+    (void)hipFree(b->d_stiles);
+    (void)hipFree(b->d_rtiles);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -1243,8 +1270,12 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     for (int k = 0; k < ncases; k++) {
         const gel_ctx* c = cases[k];
         if (!c->uploaded) return fail(GEL_E_STATE, "every case needs an uploaded state");
-        if (c->tracking || c->prm.physics == GEL_PHYS_GAS_INNER)
-            return fail(GEL_E_UNSUPPORTED, "batched cases are grids without tracking, not the inner region");
+        if (c->prm.physics == GEL_PHYS_GAS_INNER)
+            return fail(GEL_E_UNSUPPORTED, "the inner-region solver is not batched");
+        if (c->tracking != cases[0]->tracking)
+            return fail(GEL_E_UNSUPPORTED, "batched cases all track their fitted surfaces or none does");
+        if (c->tracking && c->prm.source_mode != GEL_SRC_NONE && c->prm.source_mode != GEL_SRC_MK1)
+            return fail(GEL_E_UNSUPPORTED, "tracked batches take no source or the analytic closure");
         // grids with C_1 / C_centr cells: all of them, gas in FP64
         if (c->has_types != types)
             return fail(GEL_E_UNSUPPORTED, "batched cases all have C_1 / C_centr cells or none has");
@@ -1272,6 +1303,29 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
     b->moving = moving;
     b->types = types;
     b->nfl = p0.physics == GEL_PHYS_GAS4H ? 5 : 1;
+    b->tracked = cases[0]->tracking;
+    if (b->tracked) {
+        std::vector<TrackDev> td(ncases);
+        std::vector<int2> st, rt;
+        for (int k = 0; k < ncases; k++) {
+            const gel_ctx* c = cases[k];
+            td[k] = c->tdev;
+            for (int q = 0; q < (3 * c->nj + 255) / 256; q++) st.push_back(make_int2(k, q));
+            const long long nn = (long long)(c->ni + 1) * (c->nj + 1);
+            for (long long q = 0; q < (nn + 255) / 256; q++) rt.push_back(make_int2(k, (int)q));
+        }
+        b->nstiles = (int)st.size();
+        b->nrtiles = (int)rt.size();
+        if (hipMalloc(&b->d_tdev, td.size() * sizeof(TrackDev)) != hipSuccess ||
+            hipMalloc(&b->d_stiles, st.size() * sizeof(int2)) != hipSuccess ||
+            hipMalloc(&b->d_rtiles, rt.size() * sizeof(int2)) != hipSuccess ||
+            hipMemcpy(b->d_tdev, td.data(), td.size() * sizeof(TrackDev), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->d_stiles, st.data(), st.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->d_rtiles, rt.data(), rt.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+            gel_batch_destroy(b);
+            return fail(GEL_E_NOMEM, "batch tracking tables");
+        }
+    }
     for (int k = 0; k < ncases && b->nfl == 5; k++) {
         gel_ctx* c = cases[k];
         if (!c->d_mfq && hipMalloc((void**)&c->d_mfq, 20 * c->nc * sizeof(double)) != hipSuccess) {
@@ -1280,7 +1334,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
         }
     }
     b->cycle = moving ? lcm_int(kCycle, (int)period) : kCycle;
-    for (int k = 0; k < ncases && !moving; k++) {
+    for (int k = 0; k < ncases && !moving && !b->tracked; k++) {
         const int st = ensure_geometry(cases[k]);
         if (st != GEL_OK) { gel_batch_destroy(b); return st; }
     }
@@ -1389,7 +1443,7 @@ int gel_batch_set_source_model(gel_batch* b, gel_mlp* m, int every, double tol, 
 int gel_batch_refresh(gel_batch* b, gel_mlp* m, double tol, int judge, long long* n_fallback)
 {
     if (!b || !m || !(tol >= 0.0) || judge < 0 || judge > 2) return fail(GEL_E_ARG, "bad argument");
-    if (b->moving) return fail(GEL_E_UNSUPPORTED, "moving batches take no learned source");
+    if (b->moving || b->tracked) return fail(GEL_E_UNSUPPORTED, "moving batches take no learned source");
     for (const gel_ctx* c : b->cases) {
         if (judge == 2 && c->prm.source_mode != GEL_SRC_HYBRID)
             return fail(GEL_E_UNSUPPORTED, "judge 2 needs GEL_SRC_HYBRID cases");
@@ -1438,7 +1492,11 @@ int gel_batch_source_stats(gel_batch* b, long long* refreshes, long long* fallba
 int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_wg, double* best_ms)
 {
     if (!b || trial_steps < 1) return fail(GEL_E_ARG, "bad argument");
-    struct Saved { void* st[2]; unsigned long long dt[3]; int cur; long long steps, motion_step; };
+    struct Saved {
+        void* st[2]; unsigned long long dt[3]; int cur; long long steps, motion_step;
+        double *x, *y, *x2, *y2;      // tracked cases: the node buffers and (nx, ny) their positions
+        void *nx, *ny;
+    };
     std::vector<Saved> saved(b->cases.size());
     for (size_t k = 0; k < b->cases.size(); k++) {
         gel_ctx* c = b->cases[k];
@@ -1452,6 +1510,15 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         saved[k].cur = c->cur;
         saved[k].steps = c->steps;
         saved[k].motion_step = c->motion_step;
+        saved[k].x = c->d_x; saved[k].y = c->d_y; saved[k].x2 = c->d_x2; saved[k].y2 = c->d_y2;
+        saved[k].nx = saved[k].ny = nullptr;
+        if (b->tracked) {
+            const size_t nb = c->nn * sizeof(double);
+            if (hipMalloc(&saved[k].nx, nb) != hipSuccess || hipMalloc(&saved[k].ny, nb) != hipSuccess)
+                return fail(GEL_E_NOMEM, "autotune snapshot");
+            HIP_OK(hipMemcpy(saved[k].nx, c->d_x, nb, hipMemcpyDeviceToDevice));
+            HIP_OK(hipMemcpy(saved[k].ny, c->d_y, nb, hipMemcpyDeviceToDevice));
+        }
     }
     auto restore = [&]() -> int {
         for (size_t k = 0; k < b->cases.size(); k++) {
@@ -1462,6 +1529,12 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
             c->cur = saved[k].cur;
             c->steps = saved[k].steps;
             c->motion_step = saved[k].motion_step;
+            if (b->tracked) {
+                c->d_x = saved[k].x; c->d_y = saved[k].y; c->d_x2 = saved[k].x2; c->d_y2 = saved[k].y2;
+                const size_t nb = c->nn * sizeof(double);
+                HIP_OK(hipMemcpy(c->d_x, saved[k].nx, nb, hipMemcpyDeviceToDevice));
+                HIP_OK(hipMemcpy(c->d_y, saved[k].ny, nb, hipMemcpyDeviceToDevice));
+            }
         }
         return GEL_OK;
     };
@@ -1475,6 +1548,8 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         const gel_params& p = b->cases[0]->prm;
         if (p.precision == GEL_PREC_FP32_BULK) {
             resident = resident_batch_fp32(p, b->moving);
+        } else if (b->tracked) {
+            resident = resident_tracked_batch(b->types);
         } else if (b->nfl == 5) {
             const void* kern =
                 b->moving ? (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC | kMfFlux, kNW, true, true>
@@ -1521,7 +1596,10 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
         }
         if (st != GEL_OK) break;
     }
-    for (auto& s : saved) { (void)hipFree(s.st[0]); (void)hipFree(s.st[1]); }
+    for (auto& s : saved) {
+        (void)hipFree(s.st[0]); (void)hipFree(s.st[1]);
+        (void)hipFree(s.nx); (void)hipFree(s.ny);
+    }
     b->model = model;
     b->gstep = gstep;
     batch_drop_graph(b);

@@ -53,10 +53,9 @@ __device__ void surface_sides(const SweepArgs& a, int line, int k, int j, double
 
 // face[line][j][6] = contribution to node A (x, y, count), to node B (x, y, count);
 // line 0 contact, 1 inner shock, 2 outer shock (the order of Move_surface's loops).
-__global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
+__device__ __forceinline__ void surface_face(const SweepArgs& a, const TrackDev& t, int g)
 {
     const int nj = a.nj;
-    const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= 3 * nj) return;
     const int line = g / nj, j = g - line * nj;
     double r[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
@@ -101,6 +100,10 @@ __global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
     double* o = t.face + (size_t)g * 6;
 #pragma unroll
     for (int q = 0; q < 6; q++) o[q] = r[q];
+}
+__global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
+{
+    surface_face(a, t, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Sum of the contributions to node c of a line, in the reference's loop order (face
@@ -162,11 +165,10 @@ __device__ void line_speed(const SweepArgs& a, const TrackDev& t, int line, int 
 // Node (r, c) after the step: the rail's key points move along the rail by dt times
 // their speed's rail component, the other nodes follow the rail spacing
 // (Setka.cpp:4328-4381).  Power-law factors come precomputed from the host.
-__global__ void __launch_bounds__(256) rail_kernel(SweepArgs a, TrackDev t, double* __restrict__ X2,
-                                                   double* __restrict__ Y2)
+__device__ __forceinline__ void rail_node(const SweepArgs& a, const TrackDev& t, double* __restrict__ X2,
+                                          double* __restrict__ Y2, long long g)
 {
     const int ni = a.ni, nj = a.nj;
-    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long long)(ni + 1) * (nj + 1)) return;
     const int r = (int)(g / (nj + 1)), c = (int)(g % (nj + 1));
     const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
@@ -204,6 +206,30 @@ __global__ void __launch_bounds__(256) rail_kernel(SweepArgs a, TrackDev t, doub
     X2[g] = x2;
     Y2[g] = y2;
 }
+__global__ void __launch_bounds__(256) rail_kernel(SweepArgs a, TrackDev t, double* __restrict__ X2,
+                                                   double* __restrict__ Y2)
+{
+    rail_node(a, t, X2, Y2, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The same two launches for every case of a batch: work-group g takes block tiles[g].y
+// of case tiles[g].x (surface faces, or nodes); the rail kernel writes each case's
+// next node positions into its argument set's x2, y2.
+__global__ void __launch_bounds__(256) surface_batch_kernel(const SweepArgs* __restrict__ args,
+                                                            const TrackDev* __restrict__ tdev,
+                                                            const int2* __restrict__ tiles)
+{
+    const int2 b = tiles[blockIdx.x];
+    surface_face(args[b.x], tdev[b.x], b.y * 256 + (int)threadIdx.x);
+}
+__global__ void __launch_bounds__(256) rail_batch_kernel(const SweepArgs* __restrict__ args,
+                                                         const TrackDev* __restrict__ tdev,
+                                                         const int2* __restrict__ tiles)
+{
+    const int2 b = tiles[blockIdx.x];
+    const SweepArgs& a = args[b.x];
+    rail_node(a, tdev[b.x], (double*)a.x2, (double*)a.y2, (long long)b.y * 256 + threadIdx.x);
+}
 
 template <bool TYPES, int FX>
 static hipError_t tracked_sweep(gel_ctx* c, SweepArgs& a, hipStream_t s)
@@ -211,6 +237,42 @@ static hipError_t tracked_sweep(gel_ctx* c, SweepArgs& a, hipStream_t s)
     if (c->prm.source_mode == GEL_SRC_MK1)
         return launch_t<GasPhys, double, double, true, TYPES, GEL_SRC_MK1, FX | kTrackFlux>(c, a, s);
     return launch_t<GasPhys, double, double, true, TYPES, GEL_SRC_NONE, FX | kTrackFlux>(c, a, s);
+}
+
+template <bool TYPES, int FX>
+static hipError_t tracked_batch_sweep(const SweepArgs* args, const BatchItem* items, int nitems, int workgroups,
+                                      hipStream_t s, int src)
+{
+    if (src == GEL_SRC_MK1)
+        return batch_launch_t<GasPhys, double, double, GEL_SRC_MK1, FX | kTrackFlux, true, TYPES>(args, items, nitems,
+                                                                                               workgroups, s);
+    return batch_launch_t<GasPhys, double, double, GEL_SRC_NONE, FX | kTrackFlux, true, TYPES>(args, items, nitems,
+                                                                                            workgroups, s);
+}
+
+hipError_t launch_tracked_batch(const SweepArgs* args, const TrackDev* tdev, const int2* stiles, int nstiles,
+                                const int2* rtiles, int nrtiles, const BatchItem* items, int nitems, int workgroups,
+                                hipStream_t s, bool hllc, bool types, int src)
+{
+    hipLaunchKernelGGL(surface_batch_kernel, dim3((unsigned)nstiles), dim3(256), 0, s, args, tdev, stiles);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(rail_batch_kernel, dim3((unsigned)nrtiles), dim3(256), 0, s, args, tdev, rtiles);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (types)
+        return hllc ? tracked_batch_sweep<true, GEL_FLUX_HLLC>(args, items, nitems, workgroups, s, src)
+                    : tracked_batch_sweep<true, GEL_FLUX_HLL>(args, items, nitems, workgroups, s, src);
+    return hllc ? tracked_batch_sweep<false, GEL_FLUX_HLLC>(args, items, nitems, workgroups, s, src)
+                : tracked_batch_sweep<false, GEL_FLUX_HLL>(args, items, nitems, workgroups, s, src);
+}
+
+int resident_tracked_batch(bool types)
+{
+    const void* kern =
+        types ? (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC | kTrackFlux, kNW, true, true>
+              : (const void*)sweep_batch_kernel<GasPhys, double, double, GEL_SRC_NONE, GEL_FLUX_HLLC | kTrackFlux, kNW, true, false>;
+    return resident_groups(kern, kNW * kWave);
 }
 
 hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a)

@@ -361,15 +361,15 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
                            int* status);
 
 /* ---- batched cases (no counterpart in the reference, which runs one grid) ----
- * Many independent contexts (polar grids without tracking, not the inner region, same
- * physics / precision / flux / source mode) advanced together: one persistent launch
- * per step walks the (case, strip, chunk) work list (the plasma + hydrogen physics: one
- * source launch and one launch per fluid); each case keeps its own state, buffers and
- * CFL time step.  The cases are all static, or all move by prescribed breathing (gel_set_motion)
+ * Many independent contexts (polar grids, not the inner region, same physics /
+ * precision / flux / source mode) advanced together: one persistent launch per step
+ * walks the (case, strip, chunk) work list (the plasma + hydrogen physics: one source
+ * launch and one launch per fluid; fitted-surface tracking, all cases or none: the
+ * batched surface-speed and rail launches first); each case keeps its own state,
+ * buffers and CFL time step.  The cases are all static, or all move by prescribed breathing (gel_set_motion)
  * with one common period, each with its own amplitude and phase; all have C_1 /
- * C_centr cells (the grid's cell_type: gas, FP64) or none has.  Moving batches and
- * batches with cell types take GEL_SRC_NONE or GEL_SRC_MK1 (GEL_E_UNSUPPORTED
- * otherwise).
+ * C_centr cells (the grid's cell_type: gas, FP64) or none has.  Moving, tracked and
+ * typed batches take GEL_SRC_NONE or GEL_SRC_MK1 (GEL_E_UNSUPPORTED otherwise).
  * With use_graph, whole cycles of 6 steps (of a common multiple of 6 and the motion
  * period, and of the refresh interval of an attached network) replay one captured HIP
  * graph and the rest of a call another, both kept for later calls of the same length

@@ -328,3 +328,42 @@ def test_multifluid_batch_equals_single_cases(moving):
     for s, r in zip(cases, ref):
         assert np.array_equal(s.download(), r)
     b.close()
+
+
+def _tracked_setkas(flux):
+    """Three tracked heliospheres (fitted inner shock, contact and outer shock on A rails,
+    gas, FP64) of different sizes."""
+    from gelios_2d_amd.configs import tracked_heliosphere
+    from gelios_2d_amd.setka import Setka
+    out = []
+    for k, (nj, blocks, seed) in enumerate(((16, (24, 5, 5, 6), 51), (97, (28, 12, 12, 20), 52),
+                                             (61, (28, 12, 10, 14), 53))):
+        c = tracked_heliosphere(nj, blocks, 0, seed, flux=flux)
+        s = Setka(c.grid, c.params)
+        s.upload(c.prim0)
+        s.set_tracking(c.tracking, c.rails)
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("flux", [0, 1])
+def test_tracked_batch_equals_single_cases(flux):
+    """Cases that track their fitted surfaces (batched surface-speed and rail launches,
+    then the sweep with surface faces; the node buffers change roles every step): state
+    and node positions bit-identical with each case stepped alone, eagerly, by graph
+    replay and after the autotuner's trials."""
+    from gelios_2d_amd.batch import Batch
+    single = _tracked_setkas(flux)
+    for s in single:
+        s.Go_5_komponent__MK2(23)
+    ref = [(s.download(), *s.get_nodes()) for s in single]
+    cases = _tracked_setkas(flux)
+    b = Batch(cases)
+    b.autotune(2)
+    b.step(5, False)
+    b.step(18, True)
+    for s, (r, x, y) in zip(cases, ref):
+        gx, gy = s.get_nodes()
+        assert np.array_equal(s.download(), r)
+        assert np.array_equal(gx, x) and np.array_equal(gy, y)
+    b.close()
