@@ -12,6 +12,18 @@
 
 namespace gel {
 
+// The rotating time-step buffer: three slots (the step's dt, the minimum the step
+// builds for the next one, the slot cleared for the step after), each of kDtLanes
+// entries kDtStride apart, bit patterns of positive doubles.  A wavefront takes its
+// atomic minimum into the entry of its index mod kDtLanes: atomics on one address
+// serialise (about 11 ns each on MI355X, 23 us for 2048 wavefronts), on addresses
+// 256 B apart they proceed side by side.  A slot's value is the minimum of its
+// entries; the minimum is exact in any order, so the value does not depend on it.
+constexpr int kDtLanes = 16;
+constexpr int kDtStride = 32;
+constexpr int kDtSlot = kDtLanes * kDtStride;   // unsigned long long per slot
+constexpr int kDtWords = 3 * kDtSlot;
+
 // Arguments of one sweep step of one case (also what the fused source refresh reads:
 // the grid, the state after the step in `next`, the hydrogen moments and the table).
 struct SweepArgs {
@@ -35,7 +47,7 @@ struct SweepArgs {
     int fluid;                    // plasma + hydrogen sweep: 0 plasma, 1..4 hydrogen H1..H4
     int voff[5];                  // its five variables' arrays in the 21-field state
     const double* __restrict__ mfq;   // plasma + hydrogen: [5 fluids][4][ncells] sources of the step
-    unsigned long long* dtbuf;    // [3] bit patterns of positive doubles
+    unsigned long long* dtbuf;    // [3][kDtSlot] time-step slots
     unsigned long long* counters; // [3] rho clamps, p clamps, numeric failures
     gel_params prm;
 };
@@ -131,7 +143,7 @@ struct gel_ctx {
     gel_tracking trk{};
     gel::TrackDev tdev{};
     double* d_trk = nullptr;       // tables and face contributions of tdev
-    unsigned long long* d_dt = nullptr;   // [3]
+    unsigned long long* d_dt = nullptr;   // [3][kDtSlot] time-step slots
     unsigned long long* d_cnt = nullptr;  // [3]
     long long steps = 0;
     bool uploaded = false;

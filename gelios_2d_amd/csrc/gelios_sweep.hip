@@ -198,7 +198,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* g, const gel_params* prm)
     for (int b = 0; b < 2; b++)
         if ((e = hipMalloc(&c->d_state[b], (size_t)c->nv * c->nc * c->ts)) != hipSuccess) return bad(e, "state");
     if ((e = hipMalloc(&c->d_otype, (size_t)g->nj)) != hipSuccess) return bad(e, "outer types");
-    if ((e = hipMalloc(&c->d_dt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
+    if ((e = hipMalloc(&c->d_dt, kDtWords * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
     if ((e = hipMalloc(&c->d_cnt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "counters");
     if ((e = hipMemcpy(c->d_x, g->x, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
     if ((e = hipMemcpy(c->d_y, g->y, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
@@ -280,7 +280,7 @@ int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* g, const gel_params* prm)
     if ((e = hipMalloc(&c->d_y2, nb)) != hipSuccess) return bad(e, "nodes");
     for (int b = 0; b < 2; b++)
         if ((e = hipMalloc(&c->d_state[b], (size_t)c->nv * c->nc * c->ts)) != hipSuccess) return bad(e, "state");
-    if ((e = hipMalloc(&c->d_dt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
+    if ((e = hipMalloc(&c->d_dt, kDtWords * sizeof(unsigned long long))) != hipSuccess) return bad(e, "dt");
     if ((e = hipMalloc(&c->d_cnt, 3 * sizeof(unsigned long long))) != hipSuccess) return bad(e, "counters");
     if ((e = hipMemcpy(c->d_x, g->x, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
     if ((e = hipMemcpy(c->d_y, g->y, nb, hipMemcpyHostToDevice)) != hipSuccess) return bad(e, "copy");
@@ -329,8 +329,9 @@ int gel_upload_state(gel_ctx* c, const double* prim)
     if (st != GEL_OK) return st;
     st = copy_state_in(c, c->d_state[1], prim);
     if (st != GEL_OK) return st;
-    const unsigned long long dt[3] = { dbits(c->prm.dt0), dbits(__builtin_huge_val()), dbits(__builtin_huge_val()) };
-    HIP_OK(hipMemcpyAsync(c->d_dt, dt, sizeof dt, hipMemcpyHostToDevice, c->stream));
+    std::vector<unsigned long long> dt(kDtWords, dbits(__builtin_huge_val()));
+    std::fill(dt.begin(), dt.begin() + kDtSlot, dbits(c->prm.dt0));
+    HIP_OK(hipMemcpyAsync(c->d_dt, dt.data(), kDtWords * sizeof(unsigned long long), hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->steps = 0;
@@ -652,13 +653,18 @@ static int one_step(gel_ctx* c, int moving)
 static int fill_stats(gel_ctx* c, gel_stats* out)
 {
     if (!out) return GEL_OK;
-    unsigned long long dt[3], cnt[3];
+    unsigned long long dt[kDtWords], cnt[3];
     HIP_OK(hipMemcpyAsync(dt, c->d_dt, sizeof dt, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipMemcpyAsync(cnt, c->d_cnt, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
+    auto slot = [&](long long k) {                          // the minimum of a slot's entries
+        unsigned long long m = dt[k * kDtSlot];
+        for (int e = 1; e < kDtLanes; e++) m = std::min(m, dt[k * kDtSlot + e * kDtStride]);
+        return bitsd(m);
+    };
     out->steps = c->steps;
-    out->dt_next = bitsd(dt[c->steps % 3]);
-    out->dt_last = c->steps > 0 ? bitsd(dt[(c->steps - 1) % 3]) : c->prm.dt0;
+    out->dt_next = slot(c->steps % 3);
+    out->dt_last = c->steps > 0 ? slot((c->steps - 1) % 3) : c->prm.dt0;
     out->n_rho_clamp = (long long)cnt[0];
     out->n_p_clamp = (long long)cnt[1];
     out->n_numeric = (long long)cnt[2];
@@ -750,8 +756,9 @@ int gel_sync(gel_ctx* c, gel_stats* out)
 int gel_set_dt(gel_ctx* c, double dt)
 {
     if (!c || !(dt > 0)) return fail(GEL_E_ARG, "dt must be positive");
-    const unsigned long long b = dbits(dt);
-    HIP_OK(hipMemcpyAsync(c->d_dt + (c->steps % 3), &b, sizeof b, hipMemcpyHostToDevice, c->stream));
+    const std::vector<unsigned long long> b(kDtSlot, dbits(dt));   // every entry of the step's slot
+    HIP_OK(hipMemcpyAsync(c->d_dt + (c->steps % 3) * kDtSlot, b.data(), kDtSlot * sizeof(unsigned long long),
+                          hipMemcpyHostToDevice, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     return GEL_OK;
 }
@@ -1493,7 +1500,7 @@ int gel_batch_autotune(gel_batch* b, int trial_steps, int* best_rows, int* best_
 {
     if (!b || trial_steps < 1) return fail(GEL_E_ARG, "bad argument");
     struct Saved {
-        void* st[2]; unsigned long long dt[3]; int cur; long long steps, motion_step;
+        void* st[2]; unsigned long long dt[kDtWords]; int cur; long long steps, motion_step;
         double *x, *y, *x2, *y2;      // tracked cases: the node buffers and (nx, ny) their positions
         void *nx, *ny;
     };

@@ -18,7 +18,8 @@
 //   mesh_cell_kernel  one thread per cell: the faces' products summed in Cell::Grans
 //                     order (the other cell of a face takes the exact negation), the
 //                     CFL time, the ALE update with the MK1 source, the clamps, and the
-//                     step's time-step minimum (wave min + one integer atomicMin).
+//                     step's time-step minimum (wave min + one integer atomicMin into one of the
+//                     slot's entries).
 // The per-face and per-cell arithmetic is that of the structured sweep (sweep_kernel.h,
 // sweep_core.h), so the two agree where a polar grid is given in both forms.
 #include "sweep_kernel.h"
@@ -184,7 +185,7 @@ __global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a)
     int b = 0;
     if (f < m.nf) {
         const double gam = a.prm.gamma;
-        const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
+        const double dt = dt_read(a);
         const int M = m.fown[f], N = m.fnbr[f], t = m.ftype[f];
         const int A = m.fA[f], B = m.fB[f];
         const Face g = face_geom<true, double>(a.x[A], a.y[A], a.x[B], a.y[B], a.x2[A], a.y2[A], a.x2[B], a.y2[B], dt);
@@ -219,8 +220,7 @@ template <int SRC>
 __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x == 0)
-        gp(a.dtbuf)[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
+    if (blockIdx.x == 0) dt_clear(a);
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     const size_t nc = m.nc, nf = m.nf;
     if (c < m.nc) {
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
                     auto fk = [&](int k) { return sp[(size_t)k * nc]; };
                     source_mk1<decltype(fk), double>(a.prm, fk, s[2], s[3], s[0], s[1], q1, q2, q3);
                 }
-                const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
+                const double dt = dt_read(a);
                 double o[5];
                 cell_update<GasPhys, false>(s, Pot, m.cgeo[4 * nc + c], m.cgeo[5 * nc + c], m.cgeo[2 * nc + c],
                                             m.cgeo[3 * nc + c], dt, a.prm.gamma, 0.0, q1, q2, q3, 0.0, false, o,

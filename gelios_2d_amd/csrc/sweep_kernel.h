@@ -8,7 +8,8 @@
 // four rows at a time (march_tile below); every face is solved once, the four faces
 // of a cell are summed in the reference's face order (Setka.cpp:10372-10468), and
 // the global CFL step is a wave min-reduction plus one integer atomicMin on the bit
-// pattern of a positive double per wavefront (deterministic).
+// pattern of a positive double per wavefront, into one of the time-step slot's
+// sixteen entries (deterministic: the slot's value is the minimum of its entries).
 //
 // Parity with the reference: the per-face arithmetic is that of sweep_core.h
 // (reference operation order); solving a face once instead of once from each side,
@@ -1099,6 +1100,34 @@ struct MarchAcc {       // per-lane results folded into the case's dt buffer and
     int n_rho, n_p, bad;
 };
 
+// The step's dt: the minimum of the read slot's entries (uniform loads).
+__device__ __forceinline__ double dt_read(const SweepArgs& a)
+{
+    const GPtr<const unsigned long long> p = gp((const unsigned long long*)a.dtbuf) + (size_t)a.dt_rd * kDtSlot;
+    unsigned long long m = p[0];
+#pragma unroll
+    for (int k = 1; k < kDtLanes; k++) {
+        const unsigned long long v = p[k * kDtStride];
+        m = v < m ? v : m;
+    }
+    return __longlong_as_double((long long)m);
+}
+
+// The slot the next step fills, reset to +inf by the first kDtLanes lanes of one
+// work-group of the step.
+__device__ __forceinline__ void dt_clear(const SweepArgs& a)
+{
+    if (threadIdx.x < kDtLanes)
+        gp(a.dtbuf)[(size_t)a.dt_clr * kDtSlot + threadIdx.x * kDtStride] =
+            (unsigned long long)__double_as_longlong(__builtin_huge_val());
+}
+
+// The slot entry a wavefront's minimum goes to: its index in the launch mod kDtLanes.
+__device__ __forceinline__ int dt_entry()
+{
+    return (int)((blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) % kDtLanes);
+}
+
 // Static FP32-bulk sweeps with a source copy the per-cell source inputs of the rows
 // they update from HBM into LDS while they solve the faces (gfx950 LDS DMA,
 // global_load_lds_dword: no registers held, the latency hidden behind the polar
@@ -1140,14 +1169,13 @@ __device__ __forceinline__ void march_tile(const SweepArgs& a, LS& S, FL& F, TS*
     const bool used_polar = col && lane >= 1 && lane < kWave - kHalo;
     const double gam = a.prm.gamma;
     const double cfl = a.prm.cfl;
-    const double dt = __longlong_as_double((long long)gp(a.dtbuf)[a.dt_rd]);
+    const double dt = dt_read(a);
     const bool axis_lo = (j == 0), axis_hi = (j == nj - 1);
     const bool m_axis = axis_lo || axis_hi;
     const bool n_axis = (j + 1 == nj - 1);
     const size_t nc = (size_t)ni * nj;
 
-    if (bx == 0 && i0 == 0 && threadIdx.x == 0)
-        gp(a.dtbuf)[a.dt_clr] = (unsigned long long)__double_as_longlong(__builtin_huge_val());
+    if (bx == 0 && i0 == 0) dt_clear(a);
 
     auto slot = [&](int r) { return (r - i0 + 2) % NR; };     // rows and node rows >= i0 - 2
     auto fslot = [&](int r) { return (r - i0 + 1) % (NW + 1); };  // radial face above row r >= i0 - 1
@@ -1666,8 +1694,9 @@ __device__ __forceinline__ void march_flush(const SweepArgs& a, MarchAcc& acc)
     const int s_rho = wave_sum(acc.n_rho), s_p = wave_sum(acc.n_p), s_bad = wave_sum(acc.bad);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (tmin < __builtin_huge_val())
-            __hip_atomic_fetch_min(gp(a.dtbuf) + a.dt_wr, (unsigned long long)__double_as_longlong(tmin),
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_min(gp(a.dtbuf) + (size_t)a.dt_wr * kDtSlot + dt_entry() * kDtStride,
+                                   (unsigned long long)__double_as_longlong(tmin), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
         const GPtr<unsigned long long> cnt = gp(a.counters);
         if (s_rho) __hip_atomic_fetch_add(cnt, (unsigned long long)s_rho, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (s_p) __hip_atomic_fetch_add(cnt + 1, (unsigned long long)s_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -171,7 +171,7 @@ __device__ __forceinline__ void rail_node(const SweepArgs& a, const TrackDev& t,
     const int ni = a.ni, nj = a.nj;
     if (g >= (long long)(ni + 1) * (nj + 1)) return;
     const int r = (int)(g / (nj + 1)), c = (int)(g % (nj + 1));
-    const double dt = __longlong_as_double((long long)a.dtbuf[a.dt_rd]);
+    const double dt = dt_read(a);
     const double cs = t.cs[c], sn = t.sn[c];
     const int key_row[3] = { t.rows[1], t.rows[0], t.rows[2] };   // inner shock, contact, outer shock
     const int key_line[3] = { 1, 0, 2 };
