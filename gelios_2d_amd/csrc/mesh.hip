@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) mesh_geo_kernel(MeshDev m, const double* 
                                                        const double* __restrict__ Y, const double* __restrict__ X2,
                                                        const double* __restrict__ Y2)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= m.nc) return;
     const int* q = m.corners + m.coff[c];
     const int n = m.coff[c + 1] - m.coff[c];
@@ -181,7 +181,7 @@ __device__ __forceinline__ void mesh_side(const CellT<GasPhys, double>& M, const
 template <int FX>
 __global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a)
 {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int b = 0;
     if (f < m.nf) {
         const double gam = a.prm.gamma;
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a)
 template <int SRC>
 __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (blockIdx.x == 0) dt_clear(a);
     MarchAcc acc{ __builtin_huge_val(), 0, 0, 0 };
     const size_t nc = m.nc, nf = m.nf;
@@ -412,11 +412,14 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
 {
     MeshDev& d = c->mesh->dev;
     hipStream_t s = c->stream;
+    // grids of a multiple of eight work-groups, so that each XCD takes a contiguous
+    // run of cells or faces (xcd_item) and neighbours share its L2
+    auto groups = [](int n) { return (unsigned)((((n + 255) / 256) + 7) / 8 * 8); };
     if (moving || !c->geo_valid) {
-        hipLaunchKernelGGL(mesh_geo_kernel, dim3((d.nc + 255) / 256), dim3(256), 0, s, d, a.x, a.y, a.x2, a.y2);
+        hipLaunchKernelGGL(mesh_geo_kernel, dim3(groups(d.nc)), dim3(256), 0, s, d, a.x, a.y, a.x2, a.y2);
         c->geo_valid = !moving;
     }
-    const dim3 gf((d.nf + 255) / 256), gc((d.nc + 255) / 256), blk(256);
+    const dim3 gf(groups(d.nf)), gc(groups(d.nc)), blk(256);
     if (c->prm.flux == GEL_FLUX_HLLC) hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLLC>, gf, blk, 0, s, d, a);
     else hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLL>, gf, blk, 0, s, d, a);
     if (c->prm.source_mode == GEL_SRC_MK1) hipLaunchKernelGGL(mesh_cell_kernel<GEL_SRC_MK1>, gc, blk, 0, s, d, a);
