@@ -92,6 +92,8 @@ SIGNATURES = {
                                              ctypes.POINTER(ctypes.c_longlong)]),
     "gel_flux_godunov_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
                                               _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
+    "gel_flux_godunov_wave": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_double,
+                                              _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int)]),
     "gel_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),
     "gel_batch_destroy": (None, [ctypes.c_void_p]),

@@ -68,6 +68,28 @@ __global__ void godunov_batch_kernel(int n, const double* __restrict__ L, const 
     o[5] = dsl; o[6] = dsp; o[7] = dsc; o[8] = t; o[9] = st;
 }
 
+// The same problems one per wavefront (four per work-group), as the fitted-surface
+// kernel solves them (godunov_exact<true>): out as godunov_batch_kernel.
+__global__ void __launch_bounds__(256) godunov_wave_kernel(int n, const double* __restrict__ L,
+                                                           const double* __restrict__ R,
+                                                           const double* __restrict__ nrm,
+                                                           const double* __restrict__ W,
+                                                           const double* __restrict__ dist, double cfl, double gam,
+                                                           double* __restrict__ out)
+{
+    const int k = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (k >= n) return;
+    double q[5] = { 0.0, 0.0, 0.0, 0.0, 0.0 }, dsl = 0.0, dsp = 0.0, dsc = 0.0;
+    int st = 0;
+    const double* nk = nrm + 3 * (size_t)k;
+    const double t = godunov_exact<true>(L + 5 * (size_t)k, R + 5 * (size_t)k, nk[0], nk[1], nk[2], W[k], dist[k],
+                                         cfl, gam, q, dsl, dsp, dsc, st);
+    if ((threadIdx.x & (kWave - 1)) != 0) return;
+    double* o = out + 10 * (size_t)k;
+    for (int i = 0; i < 5; i++) o[i] = q[i];
+    o[5] = dsl; o[6] = dsp; o[7] = dsc; o[8] = t; o[9] = st;
+}
+
 }  // namespace gel
 
 // ------------------------------------------------------------------ C-ABI ------
@@ -813,9 +835,9 @@ int gel_flux_hllc2d_batch(int n, const double* L, const double* R, const double*
     return GEL_OK;
 }
 
-int gel_flux_godunov_batch(int n, const double* L, const double* R, const double* nrm, const double* W,
-                           const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
-                           int* status)
+static int godunov_batch(bool wave, int n, const double* L, const double* R, const double* nrm, const double* W,
+                         const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                         int* status)
 {
     if (n < 0 || (n > 0 && (!L || !R || !nrm || !W || !dist || !q || !speeds || !dt || !status)))
         return fail(GEL_E_ARG, "null argument");
@@ -823,8 +845,12 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
     std::vector<double> out((size_t)n * 10);
     const int st = batch_run(n, { { L, 5 }, { R, 5 }, { nrm, 3 }, { W, 1 }, { dist, 1 } }, { { out.data(), 10 } },
                              [&](std::vector<double*>& i, std::vector<double*>& d) {
-                                 hipLaunchKernelGGL(godunov_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n,
-                                                    i[0], i[1], i[2], i[3], i[4], cfl, gamma, d[0]);
+                                 if (wave)
+                                     hipLaunchKernelGGL(godunov_wave_kernel, dim3((n + 3) / 4), dim3(256), 0, 0, n,
+                                                        i[0], i[1], i[2], i[3], i[4], cfl, gamma, d[0]);
+                                 else
+                                     hipLaunchKernelGGL(godunov_batch_kernel, dim3((n + 255) / 256), dim3(256), 0, 0,
+                                                        n, i[0], i[1], i[2], i[3], i[4], cfl, gamma, d[0]);
                              });
     if (st != GEL_OK) return st;
     for (int k = 0; k < n; k++) {
@@ -835,6 +861,20 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
         status[k] = (int)o[9];
     }
     return GEL_OK;
+}
+
+int gel_flux_godunov_batch(int n, const double* L, const double* R, const double* nrm, const double* W,
+                           const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                           int* status)
+{
+    return godunov_batch(false, n, L, R, nrm, W, dist, cfl, gamma, q, speeds, dt, status);
+}
+
+int gel_flux_godunov_wave(int n, const double* L, const double* R, const double* nrm, const double* W,
+                          const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                          int* status)
+{
+    return godunov_batch(true, n, L, R, nrm, W, dist, cfl, gamma, q, speeds, dt, status);
 }
 
 int gel_fp64_div_sqrt(int n, const double* a, const double* b, double* q, double* q2, double* s)
@@ -1317,7 +1357,7 @@ int gel_batch_create(gel_batch** out, int ncases, gel_ctx* const* cases)
         for (int k = 0; k < ncases; k++) {
             const gel_ctx* c = cases[k];
             td[k] = c->tdev;
-            for (int q = 0; q < (3 * c->nj + 255) / 256; q++) st.push_back(make_int2(k, q));
+            for (int q = 0; q < (3 * c->nj + 3) / 4; q++) st.push_back(make_int2(k, q));   // four faces each
             const long long nn = (long long)(c->ni + 1) * (c->nj + 1);
             for (long long q = 0; q < (nn + 255) / 256; q++) rt.push_back(make_int2(k, (int)q));
         }

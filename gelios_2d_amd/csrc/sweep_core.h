@@ -659,7 +659,68 @@ GEL_HD double godunov_lev(const GodunovK& K, double enI, double pI, double rI, d
     return (fI + fII) - (enI - enII);
 }
 
-// devtwo: bisection of lev on [pI, pII]
+// Where devtwo's bisection of lev on [pI, pII] may stop.  lev is increasing and
+// concave on [pI, pII] (its shock term has a negative second derivative, its
+// rarefaction term is a power below one); as a function of ln P it is convex (its
+// terms grow as powers of P), so Newton's iteration in ln P from the chord's zero
+// (right of the root) descends to the root without overshooting, in a handful of
+// steps for pressure ratios up to 1e6.  That iteration (kept inside the bracket it
+// narrows, its step's exponential in FP32: it only locates the root) ends within
+// tol / 8 of the root in lev; around the point it
+// reaches, the interval (a, b) is widened until lev, evaluated as the bisection
+// evaluates it, is below -(tol + 2E) at a and above tol + 2E at b, where E bounds the
+// rounding error of one evaluation anywhere on [pI, pII] (fifty times the worst case
+// of its terms: the shock term below |lev(pII)| + |Udf|, the rarefaction term's factor
+// 2 cII / g1 times the power's one-ulp error, the velocity jump).  Then at any P <= a
+// the evaluated lev is below -tol and at any P >= b above tol, so the bisection's
+// outcome at such a P is known without evaluating it.  false (a = -inf, b = +inf):
+// nothing is known and every midpoint is evaluated.
+GEL_HD bool godunov_bracket(const GodunovK& K, double enI, double pI, double rI, double enII, double pII, double rII,
+                            double flo, double fhi, double tol, double& a, double& b)
+{
+    a = -__builtin_huge_val();
+    b = __builtin_huge_val();
+    const double cI = sqrt(K.ga * pI / rI), cII = sqrt(K.ga * pII / rII);
+    const double rcI = rI * cI, kII = 2.0 / K.g1 * cII, Udf = enI - enII;
+    const double E = 1e-13 * (fabs(flo) + fabs(fhi) + 4.0 * (fabs(enI) + fabs(enII)) + kII);
+    const double D = tol + 2.0 * E;
+    double L = pI, R = pII;
+    double x = pI - flo * (pII - pI) / (fhi - flo);
+    if (!(x > L && x < R)) x = 0.5 * (L + R);
+    double d = 0.0;
+    for (int it = 0; it < 16; it++) {
+        const double A = K.gp * x / pI + K.gm, sA = sqrt(A);
+        const double xg = pow(x / pII, K.gm);
+        const double f = (x - pI) / (rcI * sA) + kII * (xg - 1.0) - Udf;
+        d = (K.gp * x / (2.0 * pI) + K.gm + K.gp / 2.0) / (rcI * A * sA) + kII * K.gm * xg / x;
+        if (!(d > 0.0) || !(f == f)) return false;
+        if (f < 0.0) L = x;
+        else R = x;
+        if (fabs(f) <= 0.125 * tol) break;
+        const double del = f / (x * d);                 // the Newton step in ln P
+        double xn = fabs(del) < 1e-3 ? x * (1.0 - del + 0.5 * del * del) : x * (double)expf((float)-del);
+        if (!(xn > L && xn < R)) xn = 0.5 * (L + R);
+        if (xn == x) break;
+        x = xn;
+    }
+    double h = 2.0 * D / d;
+    for (int t = 0; t < 3; t++, h *= 4.0) {
+        const double lo = x - h, hi = x + h;
+        const double fl = lo > pI ? godunov_lev(K, enI, pI, rI, enII, pII, rII, lo) : flo;
+        const double fh = hi < pII ? godunov_lev(K, enI, pI, rI, enII, pII, rII, hi) : fhi;
+        if (fl < -D && fh > D) {
+            a = lo > pI ? lo : pI;
+            b = hi < pII ? hi : pII;
+            return true;
+        }
+    }
+    return false;
+}
+
+// devtwo: bisection of lev on [pI, pII].  The midpoints, the stopping test and the
+// returned point are the reference's; a midpoint outside the bracket of
+// godunov_bracket is not evaluated, its lev taking the sign it certainly has (only
+// the sign enters the update: |flo|, |fm| > tol, so flo * fm cannot underflow to 0).
 GEL_HD int godunov_bisect(const GodunovK& K, double enI, double pI, double rI, double enII, double pII, double rII,
                           double& p)
 {
@@ -669,15 +730,75 @@ GEL_HD int godunov_bisect(const GodunovK& K, double enI, double pI, double rI, d
     const double fhi = godunov_lev(K, enI, pI, rI, enII, pII, rII, hi);
     if (fabs(flo) <= tol) { p = lo; return 0; }
     if (fabs(fhi) <= tol) { p = hi; return 0; }
+    double a, b;
+    godunov_bracket(K, enI, pI, rI, enII, pII, rII, flo, fhi, tol, a, b);
     for (int it = 1; it < 1100; it++) {
         const double mid = (lo + hi) / 2.0;
-        const double fm = godunov_lev(K, enI, pI, rI, enII, pII, rII, mid);
+        const double fm = mid <= a ? -1.0 : mid >= b ? 1.0 : godunov_lev(K, enI, pI, rI, enII, pII, rII, mid);
         if (fabs(fm) <= tol) { p = mid; return 0; }
         if ((flo * fm) <= 0.0) {
             hi = mid;
         } else {
             lo = mid;
             flo = fm;
+        }
+    }
+    p = 0.0;
+    return 1;
+}
+
+// Lane l's value of v, l the same in every lane (two v_readlane_b32).
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    l = __builtin_amdgcn_readfirstlane(l);
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// devtwo by one wavefront whose 64 lanes hold the same problem: the bisection's next
+// six midpoints along whichever path it takes are the 63 nodes of a binary tree
+// (node n's children 2n, the lower half, and 2n + 1); lane n - 1 reaches node n by the
+// halvings its path takes ((lo + hi) / 2 at every level, as the bisection computes
+// them) and evaluates lev there, all lanes at once; then the wavefront walks the path
+// with those values, six of the bisection's iterations for one evaluation's latency.
+// The midpoints, values, stopping test and iteration count are the bisection's.
+// Every lane of the wavefront must call it with identical arguments.
+__device__ __forceinline__ int godunov_bisect_wave(const GodunovK& K, double enI, double pI, double rI, double enII,
+                                                   double pII, double rII, double& p)
+{
+    const double tol = 10e-10;
+    double lo = pI, hi = pII;
+    double flo = godunov_lev(K, enI, pI, rI, enII, pII, rII, lo);
+    const double fhi = godunov_lev(K, enI, pI, rI, enII, pII, rII, hi);
+    if (fabs(flo) <= tol) { p = lo; return 0; }
+    if (fabs(fhi) <= tol) { p = hi; return 0; }
+    const int n = min((int)(threadIdx.x & 63) + 1, 63);      // this lane's tree node
+    const int depth = 31 - __builtin_clz(n);
+    int it = 1;
+    while (it < 1100) {
+        double l = lo, h = hi;
+        for (int d = depth - 1; d >= 0; d--) {
+            const double m = (l + h) / 2.0;
+            if ((n >> d) & 1) l = m;
+            else h = m;
+        }
+        const double mine = (l + h) / 2.0;
+        const double fmine = godunov_lev(K, enI, pI, rI, enII, pII, rII, mine);
+        int node = 1;
+        for (int s = 0; s < 6 && it < 1100; s++, it++) {
+            const double mid = lane_value(mine, node - 1);
+            const double fm = lane_value(fmine, node - 1);
+            if (fabs(fm) <= tol) { p = mid; return 0; }
+            if ((flo * fm) <= 0.0) {
+                hi = mid;
+                node = 2 * node;
+            } else {
+                lo = mid;
+                flo = fm;
+                node = 2 * node + 1;
+            }
         }
     }
     p = 0.0;
@@ -714,6 +835,9 @@ GEL_HD int godunov_newton(const GodunovK& K, double enI, double pI, double rI, d
 // q1, q2: (ro, p, u, v, w); q out: mass, momentum x/y/z, energy fluxes through the face
 // uu_out (optional): the wave speed the returned time step is formed from, floored at
 // 1e-7 as there, so that cfl * dist / *uu_out equals the returned time.
+// WAVE: one problem per wavefront, every lane with the same arguments (the bisection
+// by godunov_bisect_wave); otherwise one problem per lane.
+template <bool WAVE = false>
 GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, double n2, double n3, double w,
                             double dist, double cfl, double gam, double* q, double& dsl, double& dsp, double& dsc,
                             int& status, double* uu_out = nullptr)
@@ -813,7 +937,8 @@ GEL_HD double godunov_exact(const double* q1, const double* q2, double n1, doubl
         }
         if (w >= dII) state_II();
     } else if ((Udf > Urz) && (Udf <= Uud)) {               // shock into side I, fan into side II
-        status = godunov_bisect(K, enI, pI, rI, enII, pII, rII, p);
+        if constexpr (WAVE) status = godunov_bisect_wave(K, enI, pI, rI, enII, pII, rII, p);
+        else status = godunov_bisect(K, enI, pI, rI, enII, pII, rII, p);
         if (status) return 0.0;
         const double aI = sqrt(rI * (g2 / 2.0 * p + g1 / 2.0 * pI));
         const double aII = (fabs(p - pII) < 10e-10) ? rII * cII

@@ -3,11 +3,13 @@
 // Setka::Move_Setka_Calculate_2 (Setka.cpp:4324-4381), and the sweep instantiations
 // with surface faces (gas, FP64, FP contraction off as every FP64 kernel).
 //
-// One step is three launches on the context's stream: surface_kernel (one thread per
-// face of the three surface lines: Riemann problem, scaled wave speed, contributions
-// to the face's two nodes), rail_kernel (one thread per node: the speeds of its rail's
-// key points, their new positions, its own new position by the rail spacing), and
-// the moving sweep from the node positions before to those after.  Node speeds are
+// One step is three launches on the context's stream: surface_kernel (one wavefront
+// per face of the three surface lines: Riemann problem, scaled wave speed,
+// contributions to the face's two nodes; the exact solver's bisection runs six of its
+// iterations per evaluation over the wavefront's lanes, godunov_bisect_wave),
+// rail_kernel (one thread per node: the speeds of its rail's key points, their new
+// positions, its own new position by the rail spacing), and the moving sweep from the
+// node positions before to those after.  Node speeds are
 // combined per node in the reference's loop order, so the result does not depend on
 // scheduling; no atomics, no memset.
 #include "sweep_kernel.h"
@@ -53,10 +55,13 @@ __device__ void surface_sides(const SweepArgs& a, int line, int k, int j, double
 
 // face[line][j][6] = contribution to node A (x, y, count), to node B (x, y, count);
 // line 0 contact, 1 inner shock, 2 outer shock (the order of Move_surface's loops).
+// One face per wavefront: every lane computes the same values (the exact solver's
+// bisection spreads its midpoints over the lanes, godunov_exact<true>), lane 0 stores.
 __device__ __forceinline__ void surface_face(const SweepArgs& a, const TrackDev& t, int g)
 {
     const int nj = a.nj;
     if (g >= 3 * nj) return;
+    const bool lead = (threadIdx.x & (kWave - 1)) == 0;
     const int line = g / nj, j = g - line * nj;
     double r[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
     if (!(line == 2 && j >= t.outer_faces)) {             // Setka.cpp:3746
@@ -74,8 +79,8 @@ __device__ __forceinline__ void surface_face(const SweepArgs& a, const TrackDev&
         const double q2[5] = { p2[0], p2[1], p2[2], p2[3], 0.0 };
         double q[5], Vl = 0.0, Vp = 0.0, VV = 0.0;
         int st = 0;
-        godunov_exact(q1, q2, n1, n2, 0.0, 0.0, 0.0, 0.1, gam, q, Vl, Vp, VV, st);
-        if (st) atomicAdd(&a.counters[2], 1ull);
+        godunov_exact<true>(q1, q2, n1, n2, 0.0, 0.0, 0.0, 0.1, gam, q, Vl, Vp, VV, st);
+        if (st && lead) atomicAdd(&a.counters[2], 1ull);
         const double tt1 = -n2, tt2 = n1;
         bool toA = false, toB = false;
         double V = 0.0;
@@ -97,13 +102,14 @@ __device__ __forceinline__ void surface_face(const SweepArgs& a, const TrackDev&
         if (toA) { r[0] = V * n1; r[1] = V * n2; r[2] = 1.0; }
         if (toB) { r[3] = V * n1; r[4] = V * n2; r[5] = 1.0; }
     }
+    if (!lead) return;
     double* o = t.face + (size_t)g * 6;
 #pragma unroll
     for (int q = 0; q < 6; q++) o[q] = r[q];
 }
 __global__ void __launch_bounds__(256) surface_kernel(SweepArgs a, TrackDev t)
 {
-    surface_face(a, t, blockIdx.x * blockDim.x + threadIdx.x);
+    surface_face(a, t, blockIdx.x * 4 + (int)(threadIdx.x / kWave));
 }
 
 // Sum of the contributions to node c of a line, in the reference's loop order (face
@@ -212,15 +218,15 @@ __global__ void __launch_bounds__(256) rail_kernel(SweepArgs a, TrackDev t, doub
     rail_node(a, t, X2, Y2, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// The same two launches for every case of a batch: work-group g takes block tiles[g].y
-// of case tiles[g].x (surface faces, or nodes); the rail kernel writes each case's
-// next node positions into its argument set's x2, y2.
+// The same launches for every case of a batch: work-group g takes block tiles[g].y of
+// case tiles[g].x (four surface faces, or 256 nodes); the rail kernel writes each
+// case's next node positions into its argument set's x2, y2.
 __global__ void __launch_bounds__(256) surface_batch_kernel(const SweepArgs* __restrict__ args,
                                                             const TrackDev* __restrict__ tdev,
                                                             const int2* __restrict__ tiles)
 {
     const int2 b = tiles[blockIdx.x];
-    surface_face(args[b.x], tdev[b.x], b.y * 256 + (int)threadIdx.x);
+    surface_face(args[b.x], tdev[b.x], b.y * 4 + (int)(threadIdx.x / kWave));
 }
 __global__ void __launch_bounds__(256) rail_batch_kernel(const SweepArgs* __restrict__ args,
                                                          const TrackDev* __restrict__ tdev,
@@ -280,7 +286,8 @@ hipError_t launch_tracked_step(gel_ctx* c, SweepArgs& a)
     hipStream_t s = c->stream;
     const int nj = a.nj;
     const long long nn = (long long)(a.ni + 1) * (nj + 1);
-    hipLaunchKernelGGL(surface_kernel, dim3((3 * nj + 255) / 256), dim3(256), 0, s, a, c->tdev);
+    const dim3 gs((unsigned)((3 * nj + 3) / 4));          // four surface faces per work-group
+    hipLaunchKernelGGL(surface_kernel, gs, dim3(256), 0, s, a, c->tdev);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(rail_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, s, a, c->tdev, c->d_x2, c->d_y2);

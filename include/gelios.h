@@ -360,6 +360,13 @@ int gel_flux_godunov_batch(int n, const double* L, const double* R, const double
                            const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
                            int* status);
 
+/* The same evaluation with one problem per wavefront, the form in which the
+ * fitted-surface speeds are solved (the bisection's midpoints spread over the lanes);
+ * the same outputs, bit for bit.  No counterpart of its own in the reference. */
+int gel_flux_godunov_wave(int n, const double* L, const double* R, const double* nrm, const double* W,
+                          const double* dist, double cfl, double gamma, double* q, double* speeds, double* dt,
+                          int* status);
+
 /* ---- batched cases (no counterpart in the reference, which runs one grid) ----
  * Many independent contexts (polar grids, not the inner region, same physics /
  * precision / flux / source mode) advanced together: one persistent launch per step

@@ -145,17 +145,58 @@ def test_large_grid_free_stream():
     assert np.max(np.abs(far[2:4])) < 1e-12
 
 
-def _godunov_dev(L, R, nrm, W, dist, cfl, gam):
+def _godunov_dev(L, R, nrm, W, dist, cfl, gam, wave=False):
     import ctypes
     from gelios_2d_amd import _lib
     n = len(L)
     q, sp, dt = np.zeros((n, 5)), np.zeros((n, 3)), np.zeros(n)
     st = np.zeros(n, np.int32)
     arr = [np.ascontiguousarray(a, np.float64) for a in (L, R, nrm, W, dist)]
-    _lib.check(_lib.lib().gel_flux_godunov_batch(n, *[_lib.dptr(a) for a in arr], cfl, gam, _lib.dptr(q),
-                                                 _lib.dptr(sp), _lib.dptr(dt),
-                                                 st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    fn = _lib.lib().gel_flux_godunov_wave if wave else _lib.lib().gel_flux_godunov_batch
+    _lib.check(fn(n, *[_lib.dptr(a) for a in arr], cfl, gam, _lib.dptr(q), _lib.dptr(sp), _lib.dptr(dt),
+                  st.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
     return q, sp, dt, st
+
+
+def _riemann_problems(n, seed, gam=5.0 / 3.0):
+    """Seeded exact-solver problems over the solver's four wave patterns, placed by
+    the velocity jump between its thresholds (Solvers.cpp:693-1172): pressures and
+    densities over eight decades, pressure ratios up to 1e6, a tenth with ratios within
+    1e-4 of one; the velocity jump at a uniform fraction of the way from
+    the two-rarefaction threshold Urz to the single-shock one Uud (the bisection's
+    branch), beyond them on either side for the other patterns; random normals in the
+    plane, tangential velocities, face speeds; half the problems mirrored (sides
+    exchanged, normal reversed) so that the solver's side swap runs too."""
+    rng = np.random.default_rng(seed)
+    g1, g2 = gam - 1.0, gam + 1.0
+    gm = g1 / (2.0 * gam)
+    pI = 10.0 ** rng.uniform(-6, 2, n)
+    ratio = np.where(rng.uniform(size=n) < 0.1, 1.0 + 1e-4 * rng.uniform(size=n), 10.0 ** rng.uniform(0, 6, n))
+    pII = pI * ratio
+    rI, rII = 10.0 ** rng.uniform(-6, 2, n), 10.0 ** rng.uniform(-6, 2, n)
+    cI, cII = np.sqrt(gam * pI / rI), np.sqrt(gam * pII / rII)
+    Uud = (pII - pI) / np.sqrt(rI * (g2 * pII + g1 * pI) / 2.0)
+    Urz = -2.0 * cII / g1 * (1.0 - (pI / pII) ** gm)
+    Uvk = -2.0 * (cII + cI) / g1
+    f = rng.uniform(size=n)
+    pick = rng.integers(0, 6, n)                   # three in six in the bisection's branch
+    Udf = np.where(pick < 3, Urz + f * (Uud - Urz),
+                   np.where(pick == 3, Uud + f * (Uud - Urz + 1.0),
+                            np.where(pick == 4, Uvk + f * (Urz - Uvk), Uvk - f * abs(Uvk))))
+    th = rng.uniform(0, 2 * np.pi, n)
+    n1, n2 = np.cos(th), np.sin(th)
+    V = rng.uniform(-5, 5, n)
+    tI, tII = rng.uniform(-3, 3, n), rng.uniform(-3, 3, n)
+    # velocity = normal component * n + tangential component * (n2, -n1)
+    L = np.stack([rI, pI, (V + Udf) * n1 + tI * n2, (V + Udf) * n2 - tI * n1, np.zeros(n)], 1)
+    R = np.stack([rII, pII, V * n1 + tII * n2, V * n2 - tII * n1, np.zeros(n)], 1)
+    nrm = np.stack([n1, n2, np.zeros(n)], 1)
+    mir = rng.uniform(size=n) < 0.5
+    L[mir], R[mir] = R[mir].copy(), L[mir].copy()
+    nrm[mir] = -nrm[mir]
+    W = rng.uniform(-2, 2, n) * np.maximum(cI, cII) * (rng.uniform(size=n) < 0.5)
+    dist = 10.0 ** rng.uniform(-2, 1, n)
+    return L, R, nrm, W, dist
 
 
 def test_godunov_batch_matches_reference_vectors():
@@ -170,3 +211,42 @@ def test_godunov_batch_matches_reference_vectors():
     scale = np.maximum(np.abs(out[:, :5]).max(axis=1, keepdims=True), 1e-12)
     assert np.max(np.abs(got[:, :5] - out[:, :5]) / scale) <= 1e-12
     assert np.allclose(got[:, 5:], out[:, 5:], rtol=1e-12, atol=1e-12)
+
+
+def test_godunov_wave_form_equals_lane_form():
+    """The exact solver one problem per wavefront (the fitted-surface kernel's form,
+    the bisection's next six midpoints spread over the lanes) and one problem per lane
+    (the bisection skipping the midpoints whose outcome a Newton-located bracket
+    fixes) return the same bits: the reference's vectors and 20000 seeded problems over
+    the four wave patterns, half of them in the bisection's branch."""
+    d = golden("godunov_vectors.npz")
+    inp = d["inp"]
+    cases = [(inp[:, 0:5], inp[:, 5:10], inp[:, 10:13], inp[:, 13], inp[:, 14])]
+    cases.append(_riemann_problems(20000, 11))
+    for L, R, nrm, W, dist in cases:
+        print(f"exact solver, {len(L)} problems, lane and wavefront forms", flush=True)
+        a = _godunov_dev(L, R, nrm, W, dist, 0.1, 5.0 / 3.0)
+        b = _godunov_dev(L, R, nrm, W, dist, 0.1, 5.0 / 3.0, wave=True)
+        for x, y in zip(a, b):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+
+
+def test_godunov_random_problems_against_oracle(oracle_lib):
+    """The device solver against the oracle's restatement (plain bisection, every
+    midpoint evaluated; glibc pow) on 3000 seeded problems over the four wave patterns:
+    the same status, fluxes within 1e-12 of the largest, wave speeds and time within
+    1e-12 relative (device and host pow may differ in the last bit, which can move
+    an iteration's stopping point by one step)."""
+    L, R, nrm, W, dist = _riemann_problems(3000, 12)
+    q, sp, dt, st = _godunov_dev(L, R, nrm, W, dist, 0.1, 5.0 / 3.0)
+    worst = 0.0
+    for k in range(len(L)):
+        qo, so, to, sto = oracle_lib.godunov(L[k], R[k], nrm[k], W[k], dist[k], 0.1, 5.0 / 3.0)
+        assert st[k] == sto
+        if sto:
+            continue
+        worst = max(worst, np.max(np.abs(q[k] - qo)) / max(np.max(np.abs(qo)), 1e-300))
+        assert np.allclose(sp[k], so, rtol=1e-12, atol=0.0)
+        assert abs(dt[k] - to) <= 1e-12 * abs(to)
+    print(f"worst flux difference {worst:.2e}", flush=True)
+    assert worst <= 1e-12
