@@ -10,7 +10,9 @@
 //                     the r-weighted volumes (Get_Volume_rotate / 2 pi); on a static
 //                     grid once, on a moving step every step;
 //   mesh_face_kernel  one thread per face (a face with two cells once, seen from its
-//                     cell of lower index): both sides' values (Gran::Get_par,
+//                     cell of lower index; one wavefront per fitted-surface face, whose
+//                     exact solver's bisection spreads over the lanes, in the same
+//                     launch): both sides' values (Gran::Get_par,
 //                     Get_par_TVD, Get_par_TVD_radial, the C_1 rescaling, fitted-surface
 //                     one-sided lines), the face flux (HLL / HLLC, the exact solver on
 //                     faces between two nodes of one fitted surface, nul_potok beside
@@ -48,6 +50,8 @@ struct MeshDev {
     const int* fdown;     // Sosed_down of the owner's slot (the owner's stencil cell)
     const int* fup;       // Sosed_up of the owner's slot (the other cell's stencil cell)
     const int8_t* fsurf;  // bit 0: both nodes on one fitted surface; bit 1: exactly one
+    int ng;               // faces with fsurf bit 0 (the exact solver's faces)
+    const int* gface;     // [ng] their indices
     double* cgeo;         // [6][nc] cx, cy, vs1, vs2, Volume, Volume2
     double* fout;         // [8][nf] flux x area (5), max wave speed, face centre x, y
 };
@@ -178,38 +182,57 @@ __device__ __forceinline__ void mesh_side(const CellT<GasPhys, double>& M, const
     else tvd_usualy<GasPhys, double, double>(M, N, D, m_axis, f.fx, f.fy, r, bad);
 }
 
-template <int FX>
-__global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a)
+// Face f: both sides' values, the flux, flux x area, the wave speed and the face
+// centre into fout; returns the numeric-failure count.  WAVE: the whole wavefront
+// evaluates this one face (a fitted-surface face, whose exact solver's bisection then
+// spreads its midpoints over the lanes), lane 0 stores.
+template <int FX, bool WAVE>
+__device__ __forceinline__ int mesh_face(const MeshDev& m, const SweepArgs& a, int f)
 {
-    const int f = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int b = 0;
-    if (f < m.nf) {
-        const double gam = a.prm.gamma;
-        const double dt = dt_read(a);
-        const int M = m.fown[f], N = m.fnbr[f], t = m.ftype[f];
-        const int A = m.fA[f], B = m.fB[f];
-        const Face g = face_geom<true, double>(a.x[A], a.y[A], a.x[B], a.y[B], a.x2[A], a.y2[A], a.x2[B], a.y2[B], dt);
-        const bool c1 = m.ctype[M] == GEL_C_1 || (N >= 0 && m.ctype[N] == GEL_C_1);
-        const CellT<GasPhys, double> Mc = mesh_cell(m, a, M, c1), Dc = mesh_cell(m, a, m.fdown[f], c1);
-        const bool god = (m.fsurf[f] & 1) != 0, np = (m.fsurf[f] & 2) != 0;
-        double L[5], R[5], P[5];
-        if (t == GEL_F_USUALY) {
-            const CellT<GasPhys, double> Nc = mesh_cell(m, a, N, c1), Uc = mesh_cell(m, a, m.fup[f], c1);
-            mesh_side(Mc, Nc, Dc, m.caxis[M] != 0, god, g, gam, L, b);
-            mesh_side(Nc, Mc, Uc, m.caxis[N] != 0, god, g, gam, R, b);
-        } else {
-            boundary_states<GasPhys, double, double, true>(a, t, Mc, Dc, g, L, R);
-        }
-        double uu;
-        if (god) uu = surface_flux<FX == GEL_FLUX_HLLC>(L, R, g, gam, P, b);
-        else if (FX == GEL_FLUX_HLLC && np) uu = nul_potok_flux(L, R, g, gam, P);
-        else uu = face_flux<GasPhys, double, FX>(L, R, g, gam, P);
-        const size_t nf = m.nf;
+    const double gam = a.prm.gamma;
+    const double dt = dt_read(a);
+    const int M = m.fown[f], N = m.fnbr[f], t = m.ftype[f];
+    const int A = m.fA[f], B = m.fB[f];
+    const Face g = face_geom<true, double>(a.x[A], a.y[A], a.x[B], a.y[B], a.x2[A], a.y2[A], a.x2[B], a.y2[B], dt);
+    const bool c1 = m.ctype[M] == GEL_C_1 || (N >= 0 && m.ctype[N] == GEL_C_1);
+    const CellT<GasPhys, double> Mc = mesh_cell(m, a, M, c1), Dc = mesh_cell(m, a, m.fdown[f], c1);
+    const bool god = (m.fsurf[f] & 1) != 0, np = (m.fsurf[f] & 2) != 0;
+    double L[5], R[5], P[5];
+    if (t == GEL_F_USUALY) {
+        const CellT<GasPhys, double> Nc = mesh_cell(m, a, N, c1), Uc = mesh_cell(m, a, m.fup[f], c1);
+        mesh_side(Mc, Nc, Dc, m.caxis[M] != 0, god, g, gam, L, b);
+        mesh_side(Nc, Mc, Uc, m.caxis[N] != 0, god, g, gam, R, b);
+    } else {
+        boundary_states<GasPhys, double, double, true>(a, t, Mc, Dc, g, L, R);
+    }
+    double uu;
+    if (god) uu = surface_flux<FX == GEL_FLUX_HLLC, WAVE>(L, R, g, gam, P, b);
+    else if (FX == GEL_FLUX_HLLC && np) uu = nul_potok_flux(L, R, g, gam, P);
+    else uu = face_flux<GasPhys, double, FX>(L, R, g, gam, P);
+    if (WAVE && (threadIdx.x & (kWave - 1)) != 0) return 0;
+    const size_t nf = m.nf;
 #pragma unroll
-        for (int k = 0; k < 5; k++) m.fout[k * nf + f] = P[k] * g.S;
-        m.fout[5 * nf + f] = uu;
-        m.fout[6 * nf + f] = g.fx;
-        m.fout[7 * nf + f] = g.fy;
+    for (int k = 0; k < 5; k++) m.fout[k * nf + f] = P[k] * g.S;
+    m.fout[5 * nf + f] = uu;
+    m.fout[6 * nf + f] = g.fx;
+    m.fout[7 * nf + f] = g.fy;
+    return b;
+}
+
+// Work-groups [0, nfb) take 256 faces each, one per lane, skipping the fitted-surface
+// faces; the work-groups after them take four fitted-surface faces each, one per
+// wavefront.  nfb is a multiple of eight (xcd_item).
+template <int FX>
+__global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a, int nfb)
+{
+    int b = 0;
+    if ((int)blockIdx.x < nfb) {
+        const int f = xcd_item(blockIdx.x, nfb) * blockDim.x + threadIdx.x;
+        if (f < m.nf && !(m.fsurf[f] & 1)) b = mesh_face<FX, false>(m, a, f);
+    } else {
+        const int k = ((int)blockIdx.x - nfb) * 4 + (int)(threadIdx.x / kWave);
+        if (k < m.ng) b = mesh_face<FX, true>(m, a, m.gface[k]);
     }
     const int s_bad = wave_sum(b);
     if ((threadIdx.x & (kWave - 1)) == 0 && s_bad)
@@ -352,8 +375,13 @@ int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err)
     }
     if (!pair.empty()) { err = "an Usualy face is missing from its other cell"; return GEL_E_ARG; }
     const int nf = (int)fA.size();
+    std::vector<int> gface;
+    for (int f = 0; f < nf; f++)
+        if (fsurf[f] & 1) gface.push_back(f);
+    const int ng = (int)gface.size();
     // layout: ints, then int8, then doubles (aligned)
-    const size_t ints = (size_t)(nc + 1) + g->corner_offset[nc] + (size_t)(nc + 1) + 3 * (size_t)nslot + 7 * (size_t)nf;
+    const size_t ints = (size_t)(nc + 1) + g->corner_offset[nc] + (size_t)(nc + 1) + 3 * (size_t)nslot + 7 * (size_t)nf +
+                        (size_t)ng;
     const size_t bytes8 = 2 * (size_t)nc + nf;
     const size_t off8 = ints * 4, offd = ((off8 + bytes8 + 255) / 256) * 256;
     const size_t total = offd + (6 * (size_t)nc + 8 * (size_t)nf) * 8;
@@ -363,7 +391,7 @@ int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err)
     std::vector<char> h(offd);
     int* ip = (int*)h.data();
     size_t at = 0;
-    auto put = [&](const int* src, size_t n) { const int* p = (const int*)mb->base + at; std::memcpy(ip + at, src, n * 4); at += n; return p; };
+    auto put = [&](const int* src, size_t n) { const int* p = (const int*)mb->base + at; if (n) std::memcpy(ip + at, src, n * 4); at += n; return p; };
     MeshDev& d = mb->dev;
     d.nc = nc; d.nf = nf;
     d.coff = put(g->corner_offset, nc + 1);
@@ -379,6 +407,8 @@ int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err)
     d.fnbr = put(fnbr.data(), nf);
     d.fdown = put(fdown.data(), nf);
     d.fup = put(fup.data(), nf);
+    d.ng = ng;
+    d.gface = put(gface.data(), ng);
     char* b8 = h.data() + off8;
     std::memcpy(b8, ctype.data(), nc);
     std::memcpy(b8 + nc, caxis.data(), nc);
@@ -419,9 +449,10 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
         hipLaunchKernelGGL(mesh_geo_kernel, dim3(groups(d.nc)), dim3(256), 0, s, d, a.x, a.y, a.x2, a.y2);
         c->geo_valid = !moving;
     }
-    const dim3 gf(groups(d.nf)), gc(groups(d.nc)), blk(256);
-    if (c->prm.flux == GEL_FLUX_HLLC) hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLLC>, gf, blk, 0, s, d, a);
-    else hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLL>, gf, blk, 0, s, d, a);
+    const int nfb = (int)groups(d.nf);
+    const dim3 gf((unsigned)(nfb + (d.ng + 3) / 4)), gc(groups(d.nc)), blk(256);
+    if (c->prm.flux == GEL_FLUX_HLLC) hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLLC>, gf, blk, 0, s, d, a, nfb);
+    else hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLL>, gf, blk, 0, s, d, a, nfb);
     if (c->prm.source_mode == GEL_SRC_MK1) hipLaunchKernelGGL(mesh_cell_kernel<GEL_SRC_MK1>, gc, blk, 0, s, d, a);
     else hipLaunchKernelGGL(mesh_cell_kernel<GEL_SRC_NONE>, gc, blk, 0, s, d, a);
     return hipGetLastError();

@@ -557,7 +557,8 @@ __device__ __forceinline__ double nul_potok_flux(const double* L, const double* 
     return uu;
 }
 
-template <bool HLLC>
+// WAVE: the whole wavefront solves this one face (godunov_exact<true>).
+template <bool HLLC, bool WAVE = false>
 __device__ __forceinline__ double surface_flux(const double* L, const double* R, const Face& f, double gam,
                                                double* P, int& bad)
 {
@@ -565,7 +566,7 @@ __device__ __forceinline__ double surface_flux(const double* L, const double* R,
     const double q2[5] = { R[0], R[1], R[2], R[3], 0.0 };
     double q[5], dsl, dsp, dsc, uug = 0.0;
     int st = 0;
-    godunov_exact(q1, q2, f.n1, f.n2, 0.0, f.W, 1.0, 1.0, gam, q, dsl, dsp, dsc, st, &uug);
+    godunov_exact<WAVE>(q1, q2, f.n1, f.n2, 0.0, f.W, 1.0, 1.0, gam, q, dsl, dsp, dsc, st, &uug);
     if (st) bad += 1;
     const Gas<double> gl{ L[0], L[1], L[2], L[3], L[4] };
     const Gas<double> gr{ R[0], R[1], R[2], R[3], R[4] };
