@@ -61,7 +61,7 @@ HBM_PEAK_GBS = 8000.0
 # with rocprofv3 --pmc: FETCH_SIZE (scaled x2: a 1 GiB device copy reads as 524 302 KiB)
 # plus WRITE_SIZE, in KiB per dispatch, mean over the timed graph-replayed steps
 # (profiles/sweep_batch_ensemble_64_cases_fp32_hybrid.md)
-ENSEMBLE_TRAFFIC_BYTES = (2 * 549784 + 199636) * 1024
+ENSEMBLE_TRAFFIC_BYTES = (2 * 556549 + 200796) * 1024
 METRIC = "cell-updates/s on 1 MI355X: FP32+surrogate vs FP64 plain at L1 err ≤1e-4"
 
 
