@@ -33,6 +33,12 @@ class GelMesh(ctypes.Structure):
                 ("face_down", _ip), ("face_up", _ip)]
 
 
+class GelMeshRails(ctypes.Structure):
+    _fields_ = [("n_rails", ctypes.c_int), ("kind", _ip), ("m", _ip), ("s", _dp), ("all_offset", _ip),
+                ("all", _ip), ("key", _ip), ("n_line", ctypes.c_int * 3), ("line", _ip), ("left", ctypes.c_double),
+                ("d_chain", ctypes.c_int)]
+
+
 class GelStats(ctypes.Structure):
     _fields_ = [("dt_last", ctypes.c_double), ("dt_next", ctypes.c_double), ("steps", ctypes.c_longlong),
                 ("n_rho_clamp", ctypes.c_longlong), ("n_p_clamp", ctypes.c_longlong),
@@ -70,6 +76,8 @@ SIGNATURES = {
     "gel_set_motion": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_double]),
     "gel_tracking_reference": (None, [ctypes.POINTER(GelTracking)]),
     "gel_set_tracking": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelTracking), _dp]),
+    "gel_set_mesh_tracking": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GelTracking),
+                                             ctypes.POINTER(GelMeshRails)]),
     "gel_get_nodes": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp]),
     "gel_flux_hlld_batch": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, ctypes.c_double, _dp, _dp]),
     "gel_fp64_div_sqrt": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]),

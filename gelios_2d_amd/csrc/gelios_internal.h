@@ -98,6 +98,7 @@ struct gel_ctx;
 struct MeshBuf;                   // device face tables of a mesh context (mesh.hip)
 // Build / free the face tables of a general mesh (mesh.hip); *err names a rejected input.
 int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err);
+int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* t, const gel_mesh_rails* r, std::string& err);
 void gel_mesh_free(gel_ctx* c);
 namespace gel {
 // One step of the general-mesh sweep (mesh.hip); moving: next node positions in a.x2, a.y2.

@@ -493,6 +493,21 @@ void gel_tracking_reference(gel_tracking* t)
     t->tang_inner = 0.1; t->tang_outer = 0.01;
 }
 
+int gel_set_mesh_tracking(gel_ctx* c, const gel_tracking* t, const gel_mesh_rails* r)
+{
+    if (!c) return fail(GEL_E_ARG, "null argument");
+    if (!c->mesh) return fail(GEL_E_UNSUPPORTED, "mesh tracking: general-mesh contexts (polar grids: gel_set_tracking)");
+    if (t && (c->prm.physics != GEL_PHYS_GAS || c->prm.precision != GEL_PREC_FP64))
+        return fail(GEL_E_UNSUPPORTED, "fitted surfaces: gas physics in FP64");
+    HIP_OK(hipStreamSynchronize(c->stream));
+    std::string err;
+    const int st = gel_mesh_set_tracking(c, t, r, err);
+    if (st != GEL_OK) { c->tracking = false; return fail(st, err); }
+    c->tracking = t != nullptr;
+    if (t) c->trk = *t;
+    return GEL_OK;
+}
+
 int gel_set_tracking(gel_ctx* c, const gel_tracking* t, const double* rail)
 {
     if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "fitted-surface tracking: polar grids");

@@ -296,6 +296,283 @@ __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
     march_flush(a, acc);
 }
 
+// ---- fitted-surface tracking on the general mesh --------------------------------
+// Setka::Move_surface (Setka.cpp:3378-3844) over the three lines given as face lists
+// and Setka::Move_Setka_Calculate_2 (Setka.cpp:4324-4759) over the four rail
+// families.  mtrk_face_kernel solves the lines' faces, one wavefront per face (the
+// Riemann problem and the face's contributions to its two nodes); mtrk_move_kernel, one
+// wavefront, replays the reference's node-speed sums, averages and end corrections in
+// its statement order (the line nodes' speeds in LDS), then the rails in the
+// reference's order A, B, C, D, a rail's key points uniformly and each of its spacing
+// loops across the lanes, so that every node's last assignment is the reference's.
+struct MeshTrk {
+    int nlf;                  // line faces: Line_Contact, Line_Inner, Line_Outer concatenated
+    int nline[3];
+    int nln;                  // nodes of the lines (compact slots)
+    int nrail, nfree;
+    int n_inner, d_chain, outer_faces, average_lines;
+    double koef, k_line[3], tang_inner, tang_outer, r_inner, r_mid, r_outer, left;
+    const int* lf;            // [nlf][6] A, B (nodes), M, N (Master, Sosed), D, U (their Sosed_down)
+    const int* lslot;         // [nlf][2] line slots of A, B
+    const int* node_slot;     // [nn] line slot of a node, -1
+    const int* rail;          // [nrail][kRailInts]
+    const double* rcs;        // [nrail][2] cos s, sin s (host libm, as the reference calls them)
+    const double* fin;        // [n_inner] (j / n_inner)^p_inner
+    const double* fpow;       // per rail: the mid block's and (A rails) the outer block's factors
+    const int* allp;          // Rail::All_point, concatenated
+    const int* freep;         // [nfree] nodes on no rail (their next position is their position)
+    double* contrib;          // [nlf][6] to A (vx, vy, count), to B (vx, vy, count)
+};
+// rail row: kind, M1, M2, M3, M4, offset and count in allp, key points k0..k3, offsets
+// of its mid and outer factors in fpow
+constexpr int kRailInts = 13;
+constexpr int kMaxLineNodes = 1024;
+
+// Cell::Get_Center of cell c at node positions X, Y (as mesh_geo_kernel computes it).
+__device__ __forceinline__ void mesh_center(const MeshDev& m, const double* X, const double* Y, int c, double& cx,
+                                            double& cy)
+{
+    const int* q = m.corners + m.coff[c];
+    const int n = m.coff[c + 1] - m.coff[c];
+    if (n == 4) {
+        const double px[4] = { X[q[0]], X[q[1]], X[q[2]], X[q[3]] };
+        const double py[4] = { Y[q[0]], Y[q[1]], Y[q[2]], Y[q[3]] };
+        quad_center(px, py, cx, cy);
+    } else {
+        poly_center(X, Y, q, n, cx, cy);
+    }
+}
+
+// Line face g: Gran::Get_par_TVD of both sides (Gran.cpp:345-426: the one-sided line
+// through Sosed_down and Master; the inner shock's Master side is the cell value
+// rescaled to the face, Setka.cpp:3553-3575), the exact solver (Solvers.cpp:693-1172),
+// and the line's rules for its node speeds (Setka.cpp:3394-3470, :3576-3616,
+// :3746-3825).  One wavefront per face, every lane with the same values; lane 0 stores.
+__global__ void __launch_bounds__(256) mtrk_face_kernel(MeshDev m, MeshTrk t, SweepArgs a)
+{
+    const int g = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+    if (g >= t.nlf) return;
+    const int line = g < t.nline[0] ? 0 : (g < t.nline[0] + t.nline[1] ? 1 : 2);
+    const int j = g - (line == 0 ? 0 : (line == 1 ? t.nline[0] : t.nline[0] + t.nline[1]));
+    double r[6] = { 0.0, 0.0, 0.0, 0.0, 0.0, 0.0 };
+    if (!(line == 2 && j >= t.outer_faces)) {              // Setka.cpp:3746
+        const int* f = t.lf + (size_t)g * 6;
+        const int A = f[0], B = f[1];
+        const double* X = a.x;
+        const double* Y = a.y;
+        const double gam = a.prm.gamma;
+        const size_t nc = m.nc;
+        const double* st = (const double*)a.now;
+        CellT<GasPhys, double> C[4];                       // Master, Sosed_down; Sosed, its Sosed_down
+        const int cid[4] = { f[2], f[4], f[3], f[5] };
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            mesh_center(m, X, Y, cid[k], C[k].cx, C[k].cy);
+#pragma unroll
+            for (int v = 0; v < 5; v++) C[k].s[v] = st[v * nc + cid[k]];
+            C[k].type = GEL_C_2;
+            C[k].ok = 1;
+        }
+        const double fx = (X[A] + X[B]) / 2.0, fy = (Y[A] + Y[B]) / 2.0;
+        double t1 = X[B] - X[A], t2 = Y[B] - Y[A];          // Gran::Get_normal
+        const double nn = sqrt(sq(t1) + sq(t2));
+        t1 = t1 / nn; t2 = t2 / nn;
+        const double n1 = t2, n2 = -t1;
+        double p1[5], p2[5];
+        if (line == 1) {
+#pragma unroll
+            for (int v = 0; v < 5; v++) p1[v] = C[0].s[v];
+            const double radius = sqrt(sq(C[0].cx) + sq(C[0].cy));
+            const double dis = sqrt(sq(fx) + sq(fy));
+            p1[0] = p1[0] * sq(radius) / sq(dis);
+            p1[1] = p1[1] * pow(radius / dis, 2.0 * gam);
+            polar_perenos(C[0].cx, C[0].cy, fx, fy, p1[2], p1[3]);
+        } else {
+            tvd_surface<GasPhys>(C[0], C[1], fx, fy, p1);
+        }
+        tvd_surface<GasPhys>(C[2], C[3], fx, fy, p2);
+        const double q1[5] = { p1[0], p1[1], p1[2], p1[3], 0.0 };
+        const double q2[5] = { p2[0], p2[1], p2[2], p2[3], 0.0 };
+        double q[5], Vl = 0.0, Vp = 0.0, VV = 0.0;
+        int stt = 0;
+        godunov_exact<true>(q1, q2, n1, n2, 0.0, 0.0, 0.0, 0.1, gam, q, Vl, Vp, VV, stt);
+        if (stt && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(&a.counters[2], 1ull);
+        const double tt1 = -n2, tt2 = n1;
+        bool toA = false, toB = false;
+        double V = 0.0;
+        if (line == 0) {
+            V = VV * t.koef * t.k_line[0];
+            toB = true;
+        } else if (line == 1) {
+            V = Vl * t.koef * t.k_line[1];
+            toB = true;
+            toA = fabs(p2[2] * tt1 + p2[3] * tt2) < t.tang_inner;
+        } else {
+            V = Vp * t.koef * t.k_line[2];
+            const double ut = p1[2] * tt1 + p1[3] * tt2;
+            if ((sq(p1[2]) + sq(p1[3]) + sq(0.0)) / (gam * p1[1] / p1[0]) > 1.0) toB = true;
+            else if (fabs(ut) < t.tang_outer) { toA = true; toB = true; }
+            else if (ut > 0) toB = true;
+            else toA = true;
+        }
+        if (toA) { r[0] = V * n1; r[1] = V * n2; r[2] = 1.0; }
+        if (toB) { r[3] = V * n1; r[4] = V * n2; r[5] = 1.0; }
+    }
+    if ((threadIdx.x & (kWave - 1)) != 0) return;
+    double* o = t.contrib + (size_t)g * 6;
+#pragma unroll
+    for (int k = 0; k < 6; k++) o[k] = r[k];
+}
+
+// One wavefront: the node speeds (Move_surface's sums, averages and end corrections in
+// its statement order), then every rail (Move_Setka_Calculate_2), writing the next
+// node positions X2, Y2.
+__global__ void __launch_bounds__(64) mtrk_move_kernel(MeshTrk t, SweepArgs a, double* __restrict__ X2,
+                                                       double* __restrict__ Y2)
+{
+    __shared__ double vx[kMaxLineNodes], vy[kMaxLineNodes];
+    __shared__ int cnt[kMaxLineNodes];
+    const int lane = threadIdx.x;
+    const double* X = a.x;
+    const double* Y = a.y;
+    const double dt = dt_read(a);
+    for (int k = lane; k < t.nln; k += kWave) { vx[k] = 0.0; vy[k] = 0.0; cnt[k] = 0; }
+    __syncthreads();
+    if (lane == 0) {
+        const int* lnode = t.node_slot;
+        (void)lnode;
+        int base = 0;
+        for (int line = 0; line < 3; line++) {
+            const int n = t.nline[line];
+            for (int j = 0; j < n; j++) {                   // the line's sums, face by face
+                const double* r = t.contrib + (size_t)(base + j) * 6;
+                const int sa = t.lslot[(size_t)(base + j) * 2], sb = t.lslot[(size_t)(base + j) * 2 + 1];
+                if (r[2] > 0.0) { vx[sa] += r[0]; vy[sa] += r[1]; cnt[sa]++; }
+                if (r[5] > 0.0) { vx[sb] += r[3]; vy[sb] += r[4]; cnt[sb]++; }
+            }
+            // averaging: the contact line's A nodes (its own loop, then again in each
+            // shock's loop over the shock's face count: Setka.cpp:3522-3530,
+            // :3718-3731, :3829-3837), or each line its own (average_lines)
+            const int avg = (line == 0 || t.average_lines) ? base : 0;
+            for (int j = 0; j < n; j++) {
+                const int sa = t.lslot[(size_t)(avg + j) * 2];
+                if (cnt[sa] > 0) {
+                    vx[sa] /= (1.0 * cnt[sa]);
+                    vy[sa] /= (1.0 * cnt[sa]);
+                }
+            }
+            // end nodes (Setka.cpp:3532-3535, :3733-3736, :3839-3842)
+            const int* f0 = t.lf + (size_t)base * 6;
+            const int* fl = t.lf + (size_t)(base + n - 1) * 6;
+            const int a0 = t.lslot[(size_t)base * 2], b0 = t.lslot[(size_t)base * 2 + 1];
+            const int al = t.lslot[(size_t)(base + n - 1) * 2], bl = t.lslot[(size_t)(base + n - 1) * 2 + 1];
+            vx[a0] = vx[b0] + (X[f0[1]] - X[f0[0]]) / dt;
+            if (line == 1) vx[bl] = vx[al] + (X[fl[0]] - X[fl[1]]) / dt;
+            else vy[bl] = vy[al] + (Y[fl[0]] - Y[fl[1]]) / dt;
+            base += n;
+        }
+    }
+    __syncthreads();
+    auto speed = [&](int p, double& ux, double& uy) {
+        const int sl = t.node_slot[p];
+        ux = sl >= 0 ? vx[sl] : 0.0;
+        uy = sl >= 0 ? vy[sl] : 0.0;
+    };
+    for (int k = lane; k < t.nfree; k += kWave) {
+        const int p = t.freep[k];
+        X2[p] = X[p];
+        Y2[p] = Y[p];
+    }
+    const double R1 = t.r_inner, R111 = t.r_mid, R5 = t.r_outer;
+    double dprev = 0.0;                                      // the previous D rail's outer-shock y2
+    int dk = 0;
+    for (int q = 0; q < t.nrail; q++) {
+        const int* rr = t.rail + (size_t)q * kRailInts;
+        const int kind = rr[0], M1 = rr[1], M2 = rr[2], M3 = rr[3], M4 = rr[4];
+        const int* P = t.allp + rr[5];
+        const int nall = rr[6];
+        const int K0 = rr[7], K1 = rr[8], K2 = rr[9], K3 = rr[10];
+        const double* fmid = t.fpow + rr[11];
+        const double* fout = t.fpow + rr[12];
+        const double cs = t.rcs[2 * q], sn = t.rcs[2 * q + 1];
+        auto put = [&](int p, double x2, double y2) { X2[p] = x2; Y2[p] = y2; };
+        double ux, uy, V;
+        // inner and mid blocks of A, B, C rails (Setka.cpp:4391-4405 and the same loops of
+        // the B and C rails) from R2, the radius of key point 0 after its move
+        auto inner_blocks = [&](double R2) {
+            for (int j = lane; j < M1 - 1; j += kWave) {
+                double r;
+                if (j < t.n_inner) r = R1 + t.fin[j] * (R111 - R1);
+                else r = R111 + fmid[j - t.n_inner] * (R2 - R111);
+                put(P[j], r * cs, r * sn);
+            }
+        };
+        if (kind == 0) {                                     // A (Setka.cpp:4331-4457)
+            double k0x, k0y, k1x, k1y, k2x, k2y;
+            speed(K0, ux, uy); V = ux * cs + uy * sn; k0x = X[K0] + dt * V * cs; k0y = Y[K0] + dt * V * sn;
+            speed(K1, ux, uy); V = ux * cs + uy * sn; k1x = X[K1] + dt * V * cs; k1y = Y[K1] + dt * V * sn;
+            speed(K2, ux, uy); V = ux * cs + uy * sn; k2x = X[K2] + dt * V * cs; k2y = Y[K2] + dt * V * sn;
+            if (lane == 0) { put(K0, k0x, k0y); put(K1, k1x, k1y); put(K2, k2x, k2y); put(K3, X[K3], Y[K3]); }
+            __builtin_amdgcn_s_waitcnt(0);                   // the key points' stores before the loops'
+            const double R2 = sqrt(sq(k0x) + sq(k0y));
+            inner_blocks(R2);
+            const double R3 = sqrt(sq(k1x) + sq(k1y));
+            for (int j = lane; j < M2 - 1; j += kWave) {
+                const double r = R2 + (R3 - R2) * (j + 1) / (M2);
+                put(P[M1 + j], r * cs, r * sn);
+            }
+            const double R4 = sqrt(sq(k2x) + sq(k2y));
+            for (int j = lane; j < M3 - 1; j += kWave) {
+                const double r = R3 + (R4 - R3) * (j + 1) / (M3);
+                put(P[M1 + M2 + j], r * cs, r * sn);
+            }
+            for (int j = lane; j < M4; j += kWave) {
+                const double r = R4 + fout[j] * (R5 - R4);
+                put(P[M1 + M2 + M3 + j], r * cs, r * sn);
+            }
+        } else if (kind == 1) {                              // B (Setka.cpp:4468-4593)
+            speed(K0, ux, uy); V = ux * cs + uy * sn;
+            const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
+            speed(K1, ux, uy); V = ux * 0.0 + uy * 1.0;
+            const double k1y = Y[K1] + dt * V;
+            speed(K2, ux, uy); V = ux * 0.0 + uy * 1.0;
+            const double k2y = Y[K2] + dt * V;
+            if (lane == 0) { put(K0, k0x, k0y); put(K1, k0x, k1y); put(K2, k0x, k2y); put(K3, k0x, Y[K3]); }
+            __builtin_amdgcn_s_waitcnt(0);
+            inner_blocks(sqrt(sq(k0x) + sq(k0y)));
+            for (int j = lane; j < M2 - 1; j += kWave) put(P[M1 + j], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
+            for (int j = lane; j < M3 - 1; j += kWave) put(P[M1 + M2 + j], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
+            for (int j = lane; j < M4; j += kWave) put(P[M1 + M2 + M3 + j], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
+        } else if (kind == 2) {                              // C (Setka.cpp:4596-4670)
+            speed(K0, ux, uy); V = ux * cs + uy * sn;
+            const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
+            if (lane == 0) { put(K0, k0x, k0y); put(K1, t.left, Y[K1]); }
+            __builtin_amdgcn_s_waitcnt(0);
+            inner_blocks(sqrt(sq(k0x) + sq(k0y)));
+            const double R2 = k0x, R3 = t.left;
+            const double xg = pow(R3 / (R2), 1.0 / (M2));
+            for (int j = M1 + lane; j < nall; j += kWave) put(P[j], R2 * pow(xg, (double)(j + 1 - M1)), k0y);
+        } else {                                             // D (Setka.cpp:4672-4759)
+            __builtin_amdgcn_s_waitcnt(0);                   // key point 0 was placed by an earlier rail
+            __threadfence();
+            const double k0x = X2[K0], k0y = Y2[K0];
+            speed(K1, ux, uy); V = uy;
+            const double k1y = Y[K1] + dt * V;
+            speed(K2, ux, uy); V = uy;
+            double k2y = Y[K2] + dt * V;
+            if (dk >= t.d_chain) k2y = dprev;
+            dprev = k2y;
+            dk++;
+            if (lane == 0) { put(K1, k0x, k1y); put(K2, k0x, k2y); put(K3, k0x, Y[K3]); }
+            __builtin_amdgcn_s_waitcnt(0);
+            for (int j = lane; j < M2 - 1; j += kWave) put(P[j + 1], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
+            for (int j = lane; j < M3 - 1; j += kWave) put(P[M2 + j + 1], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
+            for (int j = lane; j < M4; j += kWave) put(P[M2 + M3 + j + 1], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
+        }
+        __builtin_amdgcn_s_waitcnt(0);                       // this rail's stores land before the next rail's
+    }
+}
+
 }  // namespace gel
 
 using namespace gel;
@@ -304,11 +581,17 @@ using namespace gel;
 struct MeshBuf {
     MeshDev dev{};
     void* base = nullptr;
+    // host copies of the face tables (fitted-surface lines are matched against them)
+    std::vector<int> hA, hB, hown, hnbr, hdown, hup;
+    // fitted-surface tracking (gel_set_mesh_tracking)
+    MeshTrk trk{};
+    void* trk_base = nullptr;
 };
 
 void gel_mesh_free(gel_ctx* c)
 {
     if (!c || !c->mesh) return;
+    (void)hipFree(c->mesh->trk_base);
     (void)hipFree(c->mesh->base);
     delete c->mesh;
     c->mesh = nullptr;
@@ -431,8 +714,192 @@ int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err)
         err = "copy of the mesh tables";
         return GEL_E_HIP;
     }
+    mb->hA = fA; mb->hB = fB; mb->hown = fown; mb->hnbr = fnbr; mb->hdown = fdown; mb->hup = fup;
     c->mesh = mb;
     c->mesh_faces = nf;
+    return GEL_OK;
+}
+
+// Tables of fitted-surface tracking on a mesh context (gel_set_mesh_tracking).  The
+// checks make the device replay the reference's assignments exactly: each line face
+// matched to a face of the mesh (its stencil cells from the face tables), the shocks'
+// averaging loops inside the contact line's face count, every rail's key points left
+// alone by its inner spacing blocks (n_inner <= M1 - 1), a D rail's outer-shock key
+// point outside its own loops (the next D rail reads it), and a D rail's key point 0
+// placed by an earlier rail or by none.
+int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rails* r, std::string& err)
+{
+    MeshBuf* mb = c->mesh;
+    (void)hipFree(mb->trk_base);
+    mb->trk_base = nullptr;
+    mb->trk = MeshTrk{};
+    if (!tr) return GEL_OK;
+    const int nn = (int)c->nn, nc = (int)c->nc;
+    if (!r || r->n_rails < 1 || !r->kind || !r->m || !r->s || !r->all_offset || !r->all || !r->key || !r->line) {
+        err = "rails and lines needed";
+        return GEL_E_ARG;
+    }
+    const int nl0 = r->n_line[0], nl1 = r->n_line[1], nl2 = r->n_line[2];
+    if (nl0 < 1 || nl1 < 1 || nl2 < 1 || nl1 > nl0 || nl2 > nl0) {
+        err = "lines: at least one face each, the shocks no longer than the contact (its averaging loops)";
+        return GEL_E_ARG;
+    }
+    const int n_inner = tr->inner_rail_nodes;
+    if (n_inner < 1 || tr->outer_faces < 0 || r->d_chain < 1) { err = "tracking constants"; return GEL_E_ARG; }
+    // line faces against the mesh's faces
+    std::map<std::pair<int, int>, std::vector<int>> by_nodes;
+    for (size_t f = 0; f < mb->hA.size(); f++)
+        by_nodes[{ std::min(mb->hA[f], mb->hB[f]), std::max(mb->hA[f], mb->hB[f]) }].push_back((int)f);
+    const int nlf = nl0 + nl1 + nl2;
+    std::vector<int> lf((size_t)nlf * 6), lslot((size_t)nlf * 2), node_slot(nn, -1), lnode;
+    for (int g = 0; g < nlf; g++) {
+        const int* L = r->line + (size_t)g * 4;
+        const int A = L[0], B = L[1], M = L[2], N = L[3];
+        if (A < 0 || A >= nn || B < 0 || B >= nn || M < 0 || M >= nc || N < 0 || N >= nc) {
+            err = "line face out of range";
+            return GEL_E_ARG;
+        }
+        int D = -1, U = -1;
+        auto it = by_nodes.find({ std::min(A, B), std::max(A, B) });
+        if (it != by_nodes.end())
+            for (int f : it->second) {
+                if (mb->hown[f] == M && mb->hnbr[f] == N) { D = mb->hdown[f]; U = mb->hup[f]; }
+                else if (mb->hown[f] == N && mb->hnbr[f] == M) { D = mb->hup[f]; U = mb->hdown[f]; }
+            }
+        if (D < 0 || U < 0) { err = "a line face is not an interior face of the mesh with stencils on both sides"; return GEL_E_ARG; }
+        int* o = &lf[(size_t)g * 6];
+        o[0] = A; o[1] = B; o[2] = M; o[3] = N; o[4] = D; o[5] = U;
+        for (int e = 0; e < 2; e++) {
+            const int p = e ? B : A;
+            if (node_slot[p] < 0) { node_slot[p] = (int)lnode.size(); lnode.push_back(p); }
+            lslot[(size_t)g * 2 + e] = node_slot[p];
+        }
+    }
+    if ((int)lnode.size() > kMaxLineNodes) { err = "more line nodes than the tracking kernel holds"; return GEL_E_ARG; }
+    // rails
+    const int nr = r->n_rails;
+    const double dd = tr->s_outer / std::sqrt(1.0 - 1.0 / tr->p_outer);
+    std::vector<int> rail((size_t)nr * kRailInts), allp;
+    std::vector<double> rcs((size_t)nr * 2), fin(n_inner), fpow;
+    for (int j = 0; j < n_inner; j++) fin[j] = std::pow(j * 1.0 / (n_inner), tr->p_inner);
+    std::vector<std::vector<int>> wrotes;               // every rail's assignments
+    int last_kind = 0;
+    for (int q = 0; q < nr; q++) {
+        const int kind = r->kind[q];
+        const int M1 = r->m[4 * q], M2 = r->m[4 * q + 1], M3 = r->m[4 * q + 2], M4 = r->m[4 * q + 3];
+        const int o0 = r->all_offset[q], o1 = r->all_offset[q + 1];
+        const int nall = o1 - o0;
+        const int* K = r->key + 4 * q;
+        if (kind < 0 || kind > 3 || kind < last_kind) { err = "rails: kinds 0..3 in the order A, B, C, D"; return GEL_E_ARG; }
+        last_kind = kind;
+        if (o0 < 0 || nall < 1) { err = "rail offsets"; return GEL_E_ARG; }
+        const int nkey = kind == 2 ? 2 : 4;
+        for (int k = 0; k < nkey; k++)
+            if (K[k] < 0 || K[k] >= nn) { err = "rail key point out of range"; return GEL_E_ARG; }
+        for (int k = o0; k < o1; k++)
+            if (r->all[k] < 0 || r->all[k] >= nn) { err = "rail point out of range"; return GEL_E_ARG; }
+        const int* P = r->all + o0;
+        std::vector<int> wrote;                          // the rail's assignments, for the checks
+        if (kind <= 2) {
+            if (M1 - 1 < n_inner || M1 < 1) { err = "rails: M1 - 1 must be at least n_inner"; return GEL_E_ARG; }
+            for (int j = 0; j < M1 - 1; j++) wrote.push_back(P[j]);
+        }
+        if (kind <= 1) {
+            if (M2 < 1 || M3 < 1 || M4 < 1 || M1 + M2 + M3 + M4 - 1 > nall) { err = "rails: All_point shorter than M1..M4"; return GEL_E_ARG; }
+            for (int j = 0; j < M2 - 1; j++) wrote.push_back(P[M1 + j]);
+            for (int j = 0; j < M3 - 1; j++) wrote.push_back(P[M1 + M2 + j]);
+            for (int j = 0; j < M4; j++) wrote.push_back(P[M1 + M2 + M3 + j]);
+            for (int k = 0; k < 4; k++) wrote.push_back(K[k]);
+        } else if (kind == 2) {
+            if (M2 < 1 || nall < M1) { err = "C rails: M2 >= 1, All_point at least M1"; return GEL_E_ARG; }
+            for (int j = M1; j < nall; j++) wrote.push_back(P[j]);
+            wrote.push_back(K[0]); wrote.push_back(K[1]);
+        } else {
+            if (M2 < 1 || M3 < 1 || M4 < 1 || M2 + M3 + M4 >= nall || K[2] != P[M2 + M3]) {
+                err = "D rails: All_point past M2 + M3 + M4, key point 2 at M2 + M3";
+                return GEL_E_ARG;
+            }
+            for (int j = 0; j < M2 - 1; j++) wrote.push_back(P[j + 1]);
+            for (int j = 0; j < M3 - 1; j++) wrote.push_back(P[M2 + j + 1]);
+            for (int j = 0; j < M4; j++) wrote.push_back(P[M2 + M3 + j + 1]);
+            wrote.push_back(K[1]); wrote.push_back(K[2]); wrote.push_back(K[3]);
+        }
+        wrotes.push_back(wrote);
+        int* rr = &rail[(size_t)q * kRailInts];
+        rr[0] = kind; rr[1] = M1; rr[2] = M2; rr[3] = M3; rr[4] = M4;
+        rr[5] = (int)allp.size(); rr[6] = nall;
+        for (int k = 0; k < 4; k++) rr[7 + k] = k < nkey ? K[k] : K[0];
+        allp.insert(allp.end(), P, P + nall);
+        // host libm, the calls the reference makes at run time (Setka.cpp:4335-4450)
+        rcs[2 * q] = std::cos(r->s[q]);
+        rcs[2 * q + 1] = std::sin(r->s[q]);
+        rr[11] = (int)fpow.size();
+        if (kind <= 2)
+            for (int j = n_inner; j < M1 - 1; j++)
+                fpow.push_back(std::pow((j - n_inner) * 1.0 / (M1 - 1 - n_inner), tr->p_mid));
+        rr[12] = (int)fpow.size();
+        if (kind == 0)
+            for (int j = 0; j < M4; j++) {
+                const double x = (j + 1) * 1.0 / (M4);
+                fpow.push_back(std::pow(x, tr->p_outer * (1.0 - (r->s[q] / dd) * (r->s[q] / dd))));
+            }
+    }
+    // a D rail reads its key point 0 where an earlier rail placed it, or where it is
+    std::vector<char> any(nn, 0), placed(nn, 0);
+    for (const auto& w : wrotes)
+        for (int p : w) any[p] = 1;
+    for (int q = 0; q < nr; q++) {
+        if (r->kind[q] == 3 && !placed[r->key[4 * q]] && any[r->key[4 * q]]) {
+            err = "D rails: key point 0 placed by a later rail";
+            return GEL_E_ARG;
+        }
+        for (int p : wrotes[q]) placed[p] = 1;
+    }
+    std::vector<int> freep;
+    for (int p = 0; p < nn; p++)
+        if (!any[p]) freep.push_back(p);
+    if (fpow.empty()) fpow.push_back(0.0);
+    // one allocation: ints, then doubles, then the face contributions
+    const size_t ni_ = lf.size() + lslot.size() + node_slot.size() + rail.size() + allp.size() + freep.size();
+    const size_t offd = ((ni_ * 4 + 255) / 256) * 256;
+    const size_t nd_ = rcs.size() + fin.size() + fpow.size();
+    const size_t total = offd + (nd_ + (size_t)nlf * 6) * 8;
+    void* base = nullptr;
+    if (hipMalloc(&base, total) != hipSuccess) { err = "tracking tables"; return GEL_E_NOMEM; }
+    std::vector<char> h(offd + nd_ * 8);
+    size_t at = 0;
+    auto puti = [&](const std::vector<int>& v) {
+        const int* p = (const int*)((char*)base + at);
+        if (!v.empty()) std::memcpy(h.data() + at, v.data(), v.size() * 4);
+        at += v.size() * 4;
+        return p;
+    };
+    MeshTrk t{};
+    t.lf = puti(lf); t.lslot = puti(lslot); t.node_slot = puti(node_slot); t.rail = puti(rail);
+    t.allp = puti(allp); t.freep = puti(freep);
+    at = offd;
+    auto putd = [&](const std::vector<double>& v) {
+        const double* p = (const double*)((char*)base + at);
+        std::memcpy(h.data() + at, v.data(), v.size() * 8);
+        at += v.size() * 8;
+        return p;
+    };
+    t.rcs = putd(rcs); t.fin = putd(fin); t.fpow = putd(fpow);
+    t.contrib = (double*)((char*)base + at);
+    if (hipMemcpy(base, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(t.contrib, 0, (size_t)nlf * 6 * 8) != hipSuccess) {
+        (void)hipFree(base);
+        err = "copy of the tracking tables";
+        return GEL_E_HIP;
+    }
+    t.nlf = nlf; t.nline[0] = nl0; t.nline[1] = nl1; t.nline[2] = nl2;
+    t.nln = (int)lnode.size(); t.nrail = nr; t.nfree = (int)freep.size();
+    t.n_inner = n_inner; t.d_chain = r->d_chain; t.outer_faces = tr->outer_faces; t.average_lines = tr->average_lines;
+    t.koef = tr->koef; t.k_line[0] = tr->k_contact; t.k_line[1] = tr->k_inner; t.k_line[2] = tr->k_outer;
+    t.tang_inner = tr->tang_inner; t.tang_outer = tr->tang_outer;
+    t.r_inner = tr->r_inner; t.r_mid = tr->r_mid; t.r_outer = tr->r_outer; t.left = r->left;
+    mb->trk = t;
+    mb->trk_base = base;
     return GEL_OK;
 }
 
@@ -442,6 +909,14 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
 {
     MeshDev& d = c->mesh->dev;
     hipStream_t s = c->stream;
+    if (c->tracking && c->mesh->trk_base) {            // Move_surface, Move_Setka_Calculate_2
+        const MeshTrk& t = c->mesh->trk;
+        a.x2 = c->d_x2;
+        a.y2 = c->d_y2;
+        hipLaunchKernelGGL(mtrk_face_kernel, dim3((unsigned)((t.nlf + 3) / 4)), dim3(256), 0, s, d, t, a);
+        hipLaunchKernelGGL(mtrk_move_kernel, dim3(1), dim3(kWave), 0, s, t, a, c->d_x2, c->d_y2);
+        moving = true;
+    }
     // grids of a multiple of eight work-groups, so that each XCD takes a contiguous
     // run of cells or faces (xcd_item) and neighbours share its L2
     auto groups = [](int n) { return (unsigned)((((n + 255) / 256) + 7) / 8 * 8); };

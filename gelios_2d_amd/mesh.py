@@ -115,6 +115,23 @@ def tvd_stencils(x, y, fA, fB, ftype, fsosed, fcopy, cell_faces, ccx, ccy):
     return up, down
 
 
+@dataclass
+class MeshRails:
+    """The reference's rails and fitted-surface lines of a block grid, by node and cell
+    index: Setka::A_Rails, B_Rails, C_Rails, D_Rails (Rail.h: type, s, M1..M4,
+    All_point, Key_point) and Line_Contact, Line_Inner, Line_Outer (per face Gran::A, B,
+    Master, Sosed), for gel_set_mesh_tracking."""
+    kind: np.ndarray          # [nr] 0 A, 1 B, 2 C, 3 D
+    m: np.ndarray             # [nr, 4] M1..M4
+    s: np.ndarray             # [nr]
+    all_offset: np.ndarray    # [nr + 1]
+    all: np.ndarray           # concatenated All_point
+    key: np.ndarray           # [nr, 4] (-1 where the rail has fewer)
+    lines: tuple              # three [n, 4] arrays: A, B, Master, Sosed
+    left: float = -1500.0 / 390.222          # Left_ (Help.h:85)
+    d_chain: int = 13                        # n_outer_shock (Help.h:68)
+
+
 def mesh_from_alpha(m) -> Mesh:
     """The grid of a checkpoint (:class:`checkpoint.AlphaMesh`) as the reference holds it
     after Download_Setka_ALL_ALPHA_2_0 and TVD_prepare."""
@@ -148,6 +165,22 @@ def mesh_from_alpha(m) -> Mesh:
     pt[m.ptype == _P_INNER_SHOCK] = 2
     pt[m.ptype == _P_OUTER_SHOCK] = 3
     return Mesh(x, y, pt, coff, corners, m.cell_type.astype(np.int32), nface, **slots)
+
+
+def mesh_from_slots(x, y, ptype, ncorner, corners, cell_type, nface, slots) -> Mesh:
+    """A mesh from the reference's objects flattened: node positions and point types
+    (0, 1 contact, 2 inner shock, 3 outer shock), each cell's contour and type, and for
+    each cell its faces in Cell::Grans order as [A, B, type, Sosed, Sosed_down,
+    Sosed_up] rows (slots [nc][MAXF][6], -1 past nface)."""
+    nc = len(cell_type)
+    coff = np.zeros(nc + 1, np.int32)
+    coff[1:] = np.cumsum(ncorner)
+    sl = np.asarray(slots, np.int32).reshape(nc, MAXF, 6)
+    cols = {k: np.ascontiguousarray(sl[:, :, i].ravel(), np.int32)
+            for i, k in enumerate(("fa", "fb", "ftype", "fnbr", "fdown", "fup"))}
+    return Mesh(np.ascontiguousarray(x, np.float64), np.ascontiguousarray(y, np.float64),
+                np.ascontiguousarray(ptype, np.int32), coff, np.ascontiguousarray(corners, np.int32),
+                np.ascontiguousarray(cell_type, np.int32), np.ascontiguousarray(nface, np.int32), **cols)
 
 
 def mesh_from_polar(g) -> Mesh:

@@ -111,6 +111,28 @@ class Setka:
             raise ValueError("one rail angle per node column")
         _lib.check(_lib.lib().gel_set_tracking(self._ctx, ctypes.byref(tracking), _lib.dptr(r)))
 
+    def set_mesh_tracking(self, tracking, rails) -> None:
+        """Fitted-surface tracking on a general-mesh context (the reference's own block
+        grid): every step runs Setka::Move_surface over the three lines and
+        Setka::Move_Setka_Calculate_2 over the four rail families (Setka.cpp:3378-3844,
+        :4324-4759) on the device before the sweep.  ``rails`` is a
+        :class:`~.mesh.MeshRails`; ``None`` tracking turns it off."""
+        if tracking is None:
+            _lib.check(_lib.lib().gel_set_mesh_tracking(self._ctx, None, None))
+            return
+        ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))   # noqa: E731
+        keep = dict(kind=np.ascontiguousarray(rails.kind, np.int32), m=np.ascontiguousarray(rails.m, np.int32),
+                    s=np.ascontiguousarray(rails.s, np.float64),
+                    off=np.ascontiguousarray(rails.all_offset, np.int32), all=np.ascontiguousarray(rails.all, np.int32),
+                    key=np.ascontiguousarray(rails.key, np.int32),
+                    line=np.ascontiguousarray(np.concatenate([np.asarray(l, np.int32).reshape(-1, 4)
+                                                              for l in rails.lines]), np.int32))
+        r = _lib.GelMeshRails(len(keep["kind"]), ip(keep["kind"]), ip(keep["m"]), _lib.dptr(keep["s"]), ip(keep["off"]),
+                              ip(keep["all"]), ip(keep["key"]),
+                              (ctypes.c_int * 3)(*[len(np.asarray(l).reshape(-1, 4)) for l in rails.lines]),
+                              ip(keep["line"]), float(rails.left), int(rails.d_chain))
+        _lib.check(_lib.lib().gel_set_mesh_tracking(self._ctx, ctypes.byref(tracking), ctypes.byref(r)))
+
     def get_nodes(self):
         """Node positions now, (x, y) of (ni+1)*(nj+1) each."""
         x = np.empty(self.grid.nnodes)

@@ -293,6 +293,33 @@ typedef struct gel_tracking {
 void gel_tracking_reference(gel_tracking* t);
 /* Turn tracking on for gel_step (t != NULL; rail_angle[nj+1]) or off (t == NULL). */
 int gel_set_tracking(gel_ctx* ctx, const gel_tracking* t, const double* rail_angle);
+/* Fitted-surface tracking on a general-mesh context (the reference's own block grid,
+ * gel_ctx_create_mesh): every gel_step first runs Setka::Move_surface over the three
+ * lines (Setka.cpp:3378-3844) and Setka::Move_Setka_Calculate_2 over the four rail
+ * families (Setka.cpp:4324-4759) on the device, then the sweep from the node positions
+ * before to those after, then commits the nodes (Setka.cpp:10473-10480).  The rails
+ * and lines are the reference's Setka::A_Rails, B_Rails, C_Rails, D_Rails (Rail.h:
+ * type, s, M1..M4, All_point, Key_point) and Line_Contact, Line_Inner, Line_Outer
+ * (Setka.h), given by node and cell index. */
+typedef struct gel_mesh_rails {
+    int n_rails;            /* A_Rails, then B_Rails, C_Rails, D_Rails, each in its order */
+    const int* kind;        /* [n_rails] Rail::type: 0 A, 1 B, 2 C, 3 D */
+    const int* m;           /* [n_rails][4] Rail::M1, M2, M3, M4 */
+    const double* s;        /* [n_rails] Rail::s (the ray's angle, or the rail's coordinate) */
+    const int* all_offset;  /* [n_rails + 1] offsets into all */
+    const int* all;         /* Rail::All_point of every rail, node indices */
+    const int* key;         /* [n_rails][4] Rail::Key_point (C rails: two, the rest -1) */
+    int n_line[3];          /* faces of Line_Contact, Line_Inner, Line_Outer */
+    const int* line;        /* [n_line[0] + n_line[1] + n_line[2]][4] per face, in the
+                               lists' order: Gran::A, B (nodes), Master, Sosed (cells) */
+    double left;            /* Left_ (Help.h:85): the C rails' outer end */
+    int d_chain;            /* n_outer_shock (Help.h:68): D rails from this index on take
+                               the previous D rail's outer-shock position */
+} gel_mesh_rails;
+/* t: the constants of gel_tracking_reference (rows unused; outer_faces = n_outer_shock
+ * + N1 + N2 + 1 on the reference's grid).  NULL t turns tracking off. */
+int gel_set_mesh_tracking(gel_ctx* ctx, const gel_tracking* t, const gel_mesh_rails* r);
+
 /* Node positions now (the reference's Point::x, y) into x, y[(ni+1)*(nj+1)]. */
 int gel_get_nodes(gel_ctx* ctx, double* x, double* y);
 

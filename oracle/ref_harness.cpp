@@ -765,6 +765,118 @@ static int mode_block(const char* in, const char* out, const char* ckpt)
     return 0;
 }
 
+// ---- the reference's own block grid with its fitted surfaces moving --------------------
+// in : int32 N1 N2 N3 N4 M1 M2 M3 M4 steps
+// Setka(N1..M4) as the constructor builds it (rails of four kinds, the fitted-surface
+// lines Line_Contact / Line_Inner / Line_Outer, Setka.cpp:19-1076), Init_conditions,
+// hydrogen moments and k factors zeroed, TVD_prepare; then Go_5_komponent__MK2, whose
+// every step calls Move_surface and Move_Setka_Calculate_2 (all four rail kinds) before
+// the sweep.  Two runs from the same start: 1 step and `steps` steps (each call starts
+// its own lagged time step at 1e-8).
+// out: int32 nn nc; float64 x[nn] y[nn]; int32 ptype[nn]; int32 ncorner[nc];
+//      int32 corners[sum]; int32 cell_type[nc]; int32 nface[nc]; int32 slots[nc][8][6];
+//      float64 prim0[5][nc];
+//      rails: int32 count[4] (A, B, C, D); per rail int32 M1 M2 M3 M4 nall nkey,
+//             float64 s, int32 all[nall], int32 key[nkey];
+//      lines: int32 count[3] (Contact, Inner, Outer); per face int32 A B Master Sosed
+//             (-1 where none), in the list's order;
+//      then for the 1-step and the `steps`-step runs: float64 prim[5][nc], x[nn], y[nn]
+static void block_numbering(Setka* S)
+{
+    for (size_t k = 0; k < S->All_Points.size(); k++) S->All_Points[k]->number = (int)k;
+    for (size_t c = 0; c < S->All_Cells.size(); c++) S->All_Cells[c]->number = (int)c;
+}
+
+static Setka* block_tracked_setka(const int32_t* h)
+{
+    Setka* S = new Setka(h[0], h[1], h[2], h[3], h[4], h[5], h[6], h[7]);
+    S->Init_conditions();
+    for (auto* K : S->All_Cells) { zero_moments(K->par[0]); zero_moments(K->par[1]); }
+    S->TVD_prepare();
+    block_numbering(S);
+    return S;
+}
+
+static int mode_blocktrack(const char* in, const char* out)
+{
+    FILE* f = xopen(in, "rb");
+    int32_t h[9]; rd(f, h, 9);
+    std::fclose(f);
+    const int steps = h[8];
+    Setka* S = block_tracked_setka(h);
+    const int nn = (int)S->All_Points.size(), nc = (int)S->All_Cells.size();
+    f = xopen(out, "wb");
+    const int32_t hh[2] = { nn, nc };
+    wr(f, hh, 2);
+    std::vector<double> x(nn), y(nn);
+    std::vector<int32_t> ptype(nn);
+    for (int k = 0; k < nn; k++) {
+        Point* P = S->All_Points[k];
+        x[k] = P->x; y[k] = P->y;
+        ptype[k] = P->type == P_Contact ? 1 : P->type == P_Inner_shock ? 2 : P->type == P_Outer_shock ? 3 : 0;
+    }
+    wr(f, x.data(), nn); wr(f, y.data(), nn); wr(f, ptype.data(), nn);
+    std::vector<int32_t> ncorner(nc), corners, ctype(nc), nface(nc), slots((size_t)nc * 8 * 6, -1);
+    std::vector<double> prim0((size_t)5 * nc);
+    for (int c = 0; c < nc; c++) {
+        Cell* K = S->All_Cells[c];
+        ncorner[c] = (int32_t)K->contour.size();
+        for (auto* P : K->contour) corners.push_back(P->number);
+        ctype[c] = (int32_t)K->type;
+        nface[c] = (int32_t)K->Grans.size();
+        if (K->Grans.size() > 8) { std::fprintf(stderr, "cell %d has %zu faces (> 8)\n", c, K->Grans.size()); return 3; }
+        for (size_t k = 0; k < K->Grans.size(); k++) {
+            Gran* G = K->Grans[k];
+            int32_t* sl = &slots[((size_t)c * 8 + k) * 6];
+            sl[0] = G->A->number; sl[1] = G->B->number; sl[2] = (int32_t)G->type;
+            sl[3] = G->Sosed ? G->Sosed->number : -1;
+            sl[4] = G->Sosed_down ? G->Sosed_down->number : -1;
+            sl[5] = G->Sosed_up ? G->Sosed_up->number : -1;
+        }
+        const Parametr& p = K->par[0];
+        prim0[c] = p.ro; prim0[nc + c] = p.p; prim0[2 * nc + c] = p.u; prim0[3 * nc + c] = p.v; prim0[4 * nc + c] = p.Q;
+    }
+    wr(f, ncorner.data(), nc); wr(f, corners.data(), corners.size()); wr(f, ctype.data(), nc);
+    wr(f, nface.data(), nc); wr(f, slots.data(), slots.size()); wr(f, prim0.data(), prim0.size());
+    const std::vector<Rail*>* kinds[4] = { &S->A_Rails, &S->B_Rails, &S->C_Rails, &S->D_Rails };
+    int32_t rc[4];
+    for (int q = 0; q < 4; q++) rc[q] = (int32_t)kinds[q]->size();
+    wr(f, rc, 4);
+    for (int q = 0; q < 4; q++)
+        for (Rail* R : *kinds[q]) {
+            const int32_t rh[6] = { R->M1, R->M2, R->M3, R->M4, (int32_t)R->All_point.size(), (int32_t)R->Key_point.size() };
+            wr(f, rh, 6);
+            wr(f, &R->s, 1);
+            for (Point* P : R->All_point) { const int32_t k = P->number; wr(f, &k, 1); }
+            for (Point* P : R->Key_point) { const int32_t k = P->number; wr(f, &k, 1); }
+        }
+    const std::vector<Gran*>* lines[3] = { &S->Line_Contact, &S->Line_Inner, &S->Line_Outer };
+    int32_t lc[3];
+    for (int q = 0; q < 3; q++) lc[q] = (int32_t)lines[q]->size();
+    wr(f, lc, 3);
+    for (int q = 0; q < 3; q++)
+        for (Gran* G : *lines[q]) {
+            const int32_t g[4] = { G->A->number, G->B->number, G->Master ? G->Master->number : -1,
+                                   G->Sosed ? G->Sosed->number : -1 };
+            wr(f, g, 4);
+        }
+    for (int run = 0; run < 2; run++) {
+        Setka* T = run == 0 ? S : block_tracked_setka(h);
+        const int n = run == 0 ? 1 : steps;
+        T->Go_5_komponent__MK2(n);
+        const int fb = n % 2;
+        std::vector<double> res((size_t)5 * nc), x2(nn), y2(nn);
+        for (int c = 0; c < nc; c++) {
+            const Parametr& p = T->All_Cells[c]->par[fb];
+            res[c] = p.ro; res[nc + c] = p.p; res[2 * nc + c] = p.u; res[3 * nc + c] = p.v; res[4 * nc + c] = p.Q;
+        }
+        for (int k = 0; k < nn; k++) { x2[k] = T->All_Points[k]->x; y2[k] = T->All_Points[k]->y; }
+        wr(f, res.data(), res.size()); wr(f, x2.data(), nn); wr(f, y2.data(), nn);
+    }
+    std::fclose(f);
+    return 0;
+}
+
 // ---- the reference's Monte Carlo of the hydrogen atoms on its own block grid -----------
 // in : int32 N1 N2 N3 N4 M1 M2 M3 M4 seed
 // Writes the seed tables M_K_prepare reads (my_dat_822.txt: 822 seed triples of
@@ -873,7 +985,7 @@ static int mode_mcunit(const char* in, const char* out)
 
 int main(int argc, char** argv)
 {
-    if (argc < 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave in out | block in out checkpoint\n"); return 2; }
+    if (argc < 4) { std::fprintf(stderr, "usage: ref_harness hllc|helpers|cell|sweep|hlld|godunov|track|mf|inner|save|resave|blocktrack in out | block in out checkpoint\n"); return 2; }
     if (!std::strcmp(argv[1], "hllc")) return mode_hllc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "helpers")) return mode_helpers(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "cell")) return mode_cell(argv[2], argv[3]);
@@ -886,6 +998,7 @@ int main(int argc, char** argv)
     if (!std::strcmp(argv[1], "save")) return mode_save(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "resave")) return mode_resave(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "block") && argc >= 5) return mode_block(argv[2], argv[3], argv[4]);
+    if (!std::strcmp(argv[1], "blocktrack")) return mode_blocktrack(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "mc")) return mode_mc(argv[2], argv[3]);
     if (!std::strcmp(argv[1], "mcunit")) return mode_mcunit(argv[2], argv[3]);
     std::fprintf(stderr, "unknown mode %s\n", argv[1]);

@@ -7,7 +7,8 @@ The reference's sweep has races on its shared time step (Setka.cpp:10141,
 :10388-10393), so the harness runs with OMP_NUM_THREADS=1.
 
     python tests/golden/make_golden.py            # every fixture
-    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint, block, mc)
+    python tests/golden/make_golden.py godunov    # one family (godunov, track, mf, inner, checkpoint, block,
+                                                  # blocktrack, mc)
 """
 from __future__ import annotations
 
@@ -449,6 +450,79 @@ def gen_block(tmp):
     np.savez_compressed(os.path.join(OUT, "sweep_block.npz"), **res)
 
 
+def parse_block_tracked(b: bytes) -> dict:
+    """The harness's blocktrack-mode output (oracle/ref_harness.cpp, mode_blocktrack)."""
+    nn, nc = np.frombuffer(b, np.int32, 2)
+    off = 8
+
+    def take(dt, n):
+        nonlocal off
+        a = np.frombuffer(b, dt, n, off).copy()
+        off += a.nbytes
+        return a
+    d = {"x": take(np.float64, nn), "y": take(np.float64, nn), "ptype": take(np.int32, nn)}
+    d["ncorner"] = take(np.int32, nc)
+    d["corners"] = take(np.int32, int(d["ncorner"].sum()))
+    d["cell_type"] = take(np.int32, nc)
+    d["nface"] = take(np.int32, nc)
+    d["slots"] = take(np.int32, nc * 8 * 6).reshape(nc, 8, 6)
+    d["prim0"] = take(np.float64, 5 * nc).reshape(5, nc)
+    rc = take(np.int32, 4)
+    kind, m, s, alls, key = [], [], [], [], []
+    for q in range(4):
+        for _ in range(rc[q]):
+            h = take(np.int32, 6)
+            kind.append(q)
+            m.append(h[:4])
+            s.append(take(np.float64, 1)[0])
+            alls.append(take(np.int32, h[4]))
+            k = take(np.int32, h[5])
+            key.append(np.concatenate([k, np.full(4 - len(k), -1, np.int32)]))
+    d["rail_kind"] = np.array(kind, np.int32)
+    d["rail_m"] = np.array(m, np.int32)
+    d["rail_s"] = np.array(s, np.float64)
+    d["rail_offset"] = np.concatenate([[0], np.cumsum([len(a) for a in alls])]).astype(np.int32)
+    d["rail_all"] = np.concatenate(alls).astype(np.int32)
+    d["rail_key"] = np.array(key, np.int32)
+    lc = take(np.int32, 3)
+    for q, name in enumerate(("contact", "inner", "outer")):
+        d["line_" + name] = take(np.int32, 4 * lc[q]).reshape(lc[q], 4)
+    for tag in ("1", "n"):
+        d["prim_" + tag] = take(np.float64, 5 * nc).reshape(5, nc)
+        d["x_" + tag] = take(np.float64, nn)
+        d["y_" + tag] = take(np.float64, nn)
+    assert off == len(b)
+    return d
+
+
+BLOCK_TRACKED = {"a": ((4, 3, 3, 10, 22, 6, 7, 7), 20), "b": ((6, 3, 4, 8, 24, 6, 8, 8), 20)}
+
+
+def gen_block_tracked(tmp):
+    """The reference's own block grid with its fitted surfaces moving: Setka(N1..M4) as
+    the constructor builds it (rails of four kinds, Line_Contact / Line_Inner /
+    Line_Outer), Init_conditions, TVD_prepare, then Go_5_komponent__MK2 for 1 step and
+    for `steps` steps, each step running Move_surface and Move_Setka_Calculate_2
+    (oracle/ref_harness.cpp, mode_blocktrack).  Kept: the grid, rails and lines, the
+    initial state, and the state and node positions after both runs."""
+    ora.build()
+    res = {}
+    for tag, (N, steps) in BLOCK_TRACKED.items():
+        fin, fout = (os.path.join(tmp, f"blocktrack_{tag}.{e}") for e in ("in", "out"))
+        with open(fin, "wb") as f:
+            f.write(np.array(list(N) + [steps], np.int32).tobytes())
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([ora.REF_HARNESS, "blocktrack", fin, fout], check=True, cwd=tmp, env=env,
+                       stdout=subprocess.DEVNULL)
+        with open(fout, "rb") as f:
+            d = parse_block_tracked(f.read())
+        d["N"] = np.array(N, np.int32)
+        d["steps"] = steps
+        for k, v in d.items():
+            res[f"{tag}_{k}"] = v
+    np.savez_compressed(os.path.join(OUT, "sweep_block_tracked.npz"), **res)
+
+
 def gen_mc(rng, tmp):
     """Point-wise pieces of the reference's Monte Carlo (oracle/ref_harness.cpp,
     mode_mcunit): Setka::Velosity_1..3 and MKmethod::int_1..3 on (u, cp) pairs, the
@@ -506,6 +580,8 @@ def main():
                 gen_multifluid(np.random.default_rng(20261206), tmp)
             if "block" in only:
                 gen_block(tmp)
+            if "blocktrack" in only:
+                gen_block_tracked(tmp)
         return
     rng = np.random.default_rng(20261203)
     with tempfile.TemporaryDirectory() as tmp:
@@ -520,6 +596,7 @@ def main():
         gen_inner(tmp)
         gen_checkpoint(np.random.default_rng(20261207), tmp)
         gen_block(tmp)
+        gen_block_tracked(tmp)
         gen_mc(np.random.default_rng(20261208), tmp)
     for f in sorted(os.listdir(OUT)):
         if f.endswith(".npz"):

@@ -106,3 +106,69 @@ def test_mesh_path_equals_structured_sweep(moving):
     assert err <= TOL
     a.close()
     b.close()
+
+
+def _tracked_block(tag):
+    from conftest import golden
+    from gelios_2d_amd.mesh import MeshRails, mesh_from_slots
+    d = golden("sweep_block_tracked.npz")
+    g = lambda k: d[f"{tag}_{k}"]   # noqa: E731
+    m = mesh_from_slots(g("x"), g("y"), g("ptype"), g("ncorner"), g("corners"), g("cell_type"), g("nface"),
+                        g("slots"))
+    rails = MeshRails(g("rail_kind"), g("rail_m"), g("rail_s"), g("rail_offset"), g("rail_all"), g("rail_key"),
+                      (g("line_contact"), g("line_inner"), g("line_outer")))
+    return d, g, m, rails
+
+
+@pytest.mark.parametrize("tag", GRIDS)
+def test_block_grid_tracked_matches_reference(tag):
+    """The reference's block grid with its fitted surfaces moving: every step runs
+    Move_surface over the three lines and Move_Setka_Calculate_2 over the A, B, C and D
+    rails on the device, then the sweep from the nodes before to those after, against
+    the reference's Go_5_komponent__MK2 for 1 step and for 20 (state 1e-12 relative per
+    conserved variable; nodes 1e-12 of the largest coordinate: the device's pow, cos
+    and atan can differ from glibc's in the last bit)."""
+    from gelios_2d_amd.params import N_OUTER_SHOCK, reference_tracking
+    from gelios_2d_amd.setka import Setka
+    d, g, m, rails = _tracked_block(tag)
+    N = g("N")
+    t = reference_tracking(outer_faces=N_OUTER_SHOCK + int(N[0]) + int(N[1]) + 1)
+    for steps, sfx in ((1, "1"), (int(g("steps")), "n")):
+        s = Setka.from_mesh(m, reference_params())
+        s.upload(g("prim0"))
+        s.set_mesh_tracking(t, rails)
+        st = s.Go_5_komponent__MK2(steps)
+        got = s.download()
+        x, y = s.get_nodes()
+        xr, yr = g("x_" + sfx), g("y_" + sfx)
+        nerr = max(np.abs(x - xr).max(), np.abs(y - yr).max()) / max(np.abs(xr).max(), np.abs(yr).max())
+        err = rel_err_conserved(got, g("prim_" + sfx))
+        print(f"tracked block grid {tag} ({m.ncells} cells), {steps} steps: state {err:.3e}, nodes {nerr:.3e}, "
+              f"numeric {st.n_numeric}", flush=True)
+        assert nerr <= TOL
+        assert err <= TOL
+        s.close()
+
+
+def test_mesh_tracking_rejects_bad_tables():
+    """Lines whose faces are not faces of the mesh, and shock lines longer than the
+    contact line (the reference's averaging loops index the contact line by the
+    shocks' face counts), are refused."""
+    from gelios_2d_amd._lib import GelError
+    from gelios_2d_amd.mesh import MeshRails
+    from gelios_2d_amd.params import reference_tracking
+    from gelios_2d_amd.setka import Setka
+    d, g, m, rails = _tracked_block("b")
+    s = Setka.from_mesh(m, reference_params())
+    s.upload(g("prim0"))
+    bad = g("line_contact").copy()
+    bad[0, 2], bad[0, 3] = bad[0, 3] + 1, bad[0, 2] + 1
+    r2 = MeshRails(rails.kind, rails.m, rails.s, rails.all_offset, rails.all, rails.key,
+                   (bad, rails.lines[1], rails.lines[2]))
+    with pytest.raises(GelError):
+        s.set_mesh_tracking(reference_tracking(), r2)
+    r3 = MeshRails(rails.kind, rails.m, rails.s, rails.all_offset, rails.all, rails.key,
+                   (rails.lines[1], rails.lines[0], rails.lines[2]))
+    with pytest.raises(GelError):
+        s.set_mesh_tracking(reference_tracking(), r3)
+    s.close()

@@ -70,3 +70,45 @@ def test_oracle_matches_reference_on_block_grid(oracle_lib, tag):
     m = mesh_from_alpha(alpha(d))
     got = run_oracle(oracle_lib, d, m, int(d["steps"]))
     assert np.array_equal(got, d["prim"])
+
+
+def tracked_block(tag):
+    """The tracked block-grid fixture (tests/golden/sweep_block_tracked.npz, harness mode
+    blocktrack) as (getter, mesh, rails, tracking constants)."""
+    from gelios_2d_amd.mesh import MeshRails, mesh_from_slots
+    from gelios_2d_amd.params import N_OUTER_SHOCK, reference_tracking
+    d = golden("sweep_block_tracked.npz")
+    g = lambda k: d[f"{tag}_{k}"]   # noqa: E731
+    m = mesh_from_slots(g("x"), g("y"), g("ptype"), g("ncorner"), g("corners"), g("cell_type"), g("nface"),
+                        g("slots"))
+    rails = MeshRails(g("rail_kind"), g("rail_m"), g("rail_s"), g("rail_offset"), g("rail_all"), g("rail_key"),
+                      (g("line_contact"), g("line_inner"), g("line_outer")))
+    N = g("N")
+    t = reference_tracking(outer_faces=N_OUTER_SHOCK + int(N[0]) + int(N[1]) + 1)
+    return g, m, rails, t
+
+
+@pytest.mark.parametrize("tag", GRIDS)
+def test_tracked_block_node_motion_matches_reference(tag):
+    """Move_surface over the three lines and Move_Setka_Calculate_2 over the A, B, C, D
+    rails (oracle/mesh_tracking.py) give the reference's next node positions bit for
+    bit from the fixture's first step (dt = 1e-8, the state before it)."""
+    from oracle import mesh_tracking as mt
+    g, m, rails, t = tracked_block(tag)
+    x2, y2 = mt.move_nodes(m, rails, t, g("x"), g("y"), g("prim0"), 1e-8, 5.0 / 3.0)
+    assert np.array_equal(x2, g("x_1")) and np.array_equal(y2, g("y_1"))
+
+
+@pytest.mark.parametrize("tag", GRIDS)
+def test_tracked_block_oracle_matches_reference(oracle_lib, tag):
+    """The oracle's tracked steps (node motion, cell loop, node commit) against the
+    reference's Go_5_komponent__MK2 over 1 and 20 steps: nodes bit for bit, state
+    within 1e-13 relative (one ulp of drift in one cell over 20 steps)."""
+    from gelios_2d_amd.state import rel_err_conserved
+    from oracle import mesh_tracking as mt
+    g, m, rails, t = tracked_block(tag)
+    for steps, sfx in ((1, "1"), (int(g("steps")), "n")):
+        o = mt.MeshTrackedOracle(m, rails, t, reference_params())
+        p = o.run(g("prim0"), steps)
+        assert np.array_equal(o.x, g("x_" + sfx)) and np.array_equal(o.y, g("y_" + sfx))
+        assert rel_err_conserved(p, g("prim_" + sfx)) <= 1e-13
