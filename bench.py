@@ -41,6 +41,10 @@ Workloads
   block-731        the reference's own block grid (Setka(6,3,4,8,24,6,8,8), 731 cells, read
                    from its checkpoint in tests/golden/sweep_block.npz), HLL, the
                    reference's constants, 200 steps
+  block-731-tracked  the same grid as the reference's constructor builds it
+                   (tests/golden/sweep_block_tracked.npz) with its fitted surfaces moving:
+                   every step Move_surface over the three lines, Move_Setka_Calculate_2
+                   over the A, B, C, D rails, the sweep; 200 steps
   mc-200x100       the hydrogen Monte Carlo (gel_mc_run) on the hllc-200x100 heliosphere's
                    plasma: a step is one run of 4e6 primary atoms with their secondaries;
                    the line's value is cell segments flown per second (its own metric)
@@ -80,7 +84,7 @@ def bytes_per_cell_update(w, fallback_fraction=0.0):
     from gelios_2d_amd.params import GEL_PREC_FP32_BULK, GEL_SRC_HYBRID, GEL_SRC_MK1, GEL_SRC_TABLE
     nv = {1: 9, 2: 21}.get(w.params.physics, 5)
     s = 4 if w.params.precision == GEL_PREC_FP32_BULK else 8
-    moving = bool(getattr(w, "motion", None) or getattr(w, "tracking", None))
+    moving = bool(getattr(w, "motion", None) or getattr(w, "tracking", None) or getattr(w, "mesh_rails", None))
     geom = 32 if moving else 16
     src = {GEL_SRC_TABLE: 5 * s, GEL_SRC_MK1: 19 * s}.get(w.params.source_mode, 0)
     if w.params.source_mode == GEL_SRC_HYBRID:
@@ -114,7 +118,9 @@ def run_gpu(w, K, W, label):
     s.upload(w.prim0)
     if getattr(w, "src", None) is not None:
         s.set_sources(w.src)
-    if getattr(w, "tracking", None) is not None:
+    if getattr(w, "mesh_rails", None) is not None:
+        s.set_mesh_tracking(w.mesh_tracking, w.mesh_rails)
+    elif getattr(w, "tracking", None) is not None:
         s.set_tracking(w.tracking, w.rails)
     s.Go_5_komponent__MK2(W)
     s.sync()
@@ -399,7 +405,13 @@ def cpu_baseline_workload(w, budget_s=12.0):
     from oracle import oracle as ora
     threads = cpu_threads()
     motion = getattr(w, "motion", None)
-    if getattr(w, "tracking", None) is not None:
+    mtr = None
+    if getattr(w, "mesh_rails", None) is not None:
+        from oracle import mesh_tracking as omt
+        tr = None
+        mtr = omt.MeshTrackedOracle(w.mesh, w.mesh_rails, w.mesh_tracking, w.params)
+        sw = mtr.sw
+    elif getattr(w, "tracking", None) is not None:
         tr = ora.TrackedOracle(w.grid, w.params, w.tracking, w.rails)
         sw = tr.sw
     elif getattr(w, "mesh", None) is not None:
@@ -415,6 +427,12 @@ def cpu_baseline_workload(w, budget_s=12.0):
     amp, period, phase = motion if motion else (0.0, 1, 0.0)
 
     def one(cur, n):
+        if mtr is not None:
+            x2, y2 = omt.move_nodes(mtr.mesh, mtr.rails, mtr.t, mtr.x, mtr.y, cur, sw.dt, w.params.gamma)
+            sw.set_nodes(mtr.x, mtr.y, x2, y2)
+            nxt, _ = sw.step(cur, nthreads=threads)
+            mtr.x, mtr.y = x2, y2
+            return nxt
         if tr is not None:
             x2, y2, st = ora.track_nodes(tr.t, w.grid.ni, w.grid.nj, tr.x, tr.y, cur, tr.rails, w.params.gamma, sw.dt)
             sw.set_nodes(tr.x, tr.y, x2, y2)
@@ -655,6 +673,21 @@ def main():
             w = configs.Workload("reference-block-grid-731-cells-hll-fp64", m, reference_params(),
                                  state_from_alpha(alpha), 200, None)
             w.mesh = m
+        elif wl == "block-731-tracked":
+            import numpy as np
+            from gelios_2d_amd.mesh import MeshRails, mesh_from_slots
+            from gelios_2d_amd.params import N_OUTER_SHOCK, reference_params, reference_tracking
+            d = np.load(os.path.join(ROOT, "tests", "golden", "sweep_block_tracked.npz"))
+            g = lambda k: d["b_" + k]   # noqa: E731
+            m = mesh_from_slots(g("x"), g("y"), g("ptype"), g("ncorner"), g("corners"), g("cell_type"), g("nface"),
+                                g("slots"))
+            w = configs.Workload("reference-block-grid-731-cells-tracked-hll-fp64", m, reference_params(),
+                                 g("prim0"), 200, None)
+            w.mesh = m
+            w.mesh_rails = MeshRails(g("rail_kind"), g("rail_m"), g("rail_s"), g("rail_offset"), g("rail_all"),
+                                     g("rail_key"), (g("line_contact"), g("line_inner"), g("line_outer")))
+            N = g("N")
+            w.mesh_tracking = reference_tracking(outer_faces=N_OUTER_SHOCK + int(N[0]) + int(N[1]) + 1)
         elif wl == "tracked-800x400":
             w = configs.tracked_heliosphere(400, (200, 150, 150, 300), 200, seed=8, name="tracked-hll-fp64-799x400")
         else:
@@ -683,8 +716,10 @@ def main():
             "grid": [main_w.grid.ni, main_w.grid.nj] if hasattr(main_w.grid, "ni") else "general mesh",
             "physics": {1: "mhd-hlld", 2: "plasma+4H-hllc"}.get(main_w.params.physics,
                                                                  "gas-hllc" if main_w.params.flux == 1 else "gas-hll"),
-            "moving_grid": bool(getattr(main_w, "motion", None) or getattr(main_w, "tracking", None)),
-            "fitted_surfaces": getattr(main_w, "tracking", None) is not None,
+            "moving_grid": bool(getattr(main_w, "motion", None) or getattr(main_w, "tracking", None)
+                                or getattr(main_w, "mesh_rails", None)),
+            "fitted_surfaces": (getattr(main_w, "tracking", None) is not None
+                                or getattr(main_w, "mesh_rails", None) is not None),
             "precision": precision, "cfl": main_w.params.cfl},
         "roofline": main_r["roofline"],
     }

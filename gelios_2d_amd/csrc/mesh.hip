@@ -26,6 +26,7 @@
 // sweep_core.h), so the two agree where a polar grid is given in both forms.
 #include "sweep_kernel.h"
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <tuple>
@@ -311,6 +312,13 @@ struct MeshTrk {
     int nln;                  // nodes of the lines (compact slots)
     int nrail, nfree;
     int n_inner, d_chain, outer_faces, average_lines;
+    const int* sum_off;       // [4] per line, into sums
+    const int* sums;          // [][3] line slot, its contributions (2 face + role, -1)
+    const int* avg_off;       // [4] per line, into avgs
+    const int* avgs;          // the line slots each averaging loop divides, in order
+    int nphase;               // runs of rails that run side by side
+    const int* phase;         // [nphase + 1] first rail of each run
+    int d_chain_rail;         // the D rail whose outer-shock position the chain copies, -1
     double koef, k_line[3], tang_inner, tang_outer, r_inner, r_mid, r_outer, left;
     const int* lf;            // [nlf][6] A, B (nodes), M, N (Master, Sosed), D, U (their Sosed_down)
     const int* lslot;         // [nlf][2] line slots of A, B
@@ -324,9 +332,13 @@ struct MeshTrk {
     double* contrib;          // [nlf][6] to A (vx, vy, count), to B (vx, vy, count)
 };
 // rail row: kind, M1, M2, M3, M4, offset and count in allp, key points k0..k3, offsets
-// of its mid and outer factors in fpow
-constexpr int kRailInts = 13;
+// of its mid and outer factors in fpow, 1 for a D rail that copies the outer-shock
+// position of the rail before, the key points the rail stores (bit k: key point k is not
+// overwritten by the rail's own spacing loops, which then place it)
+constexpr int kRailInts = 15;
 constexpr int kMaxLineNodes = 1024;
+constexpr int kMaxLineFaces = 1024;
+constexpr int kMoveThreads = 512;
 
 // Cell::Get_Center of cell c at node positions X, Y (as mesh_geo_kernel computes it).
 __device__ __forceinline__ void mesh_center(const MeshDev& m, const double* X, const double* Y, int c, double& cx,
@@ -424,152 +436,168 @@ __global__ void __launch_bounds__(256) mtrk_face_kernel(MeshDev m, MeshTrk t, Sw
     for (int k = 0; k < 6; k++) o[k] = r[k];
 }
 
-// One wavefront: the node speeds (Move_surface's sums, averages and end corrections in
-// its statement order), then every rail (Move_Setka_Calculate_2), writing the next
-// node positions X2, Y2.
-__global__ void __launch_bounds__(64) mtrk_move_kernel(MeshTrk t, SweepArgs a, double* __restrict__ X2,
-                                                       double* __restrict__ Y2)
+// One work-group: the node speeds (Move_surface's sums, averages and end corrections
+// in its statement order, one thread, from LDS), then the rails (Move_Setka_Calculate_2)
+// phase by phase: a phase is a run of consecutive rails of which none writes a point
+// another writes or reads, so its rails run side by side, one wavefront each, a rail's
+// key points uniformly and its spacing loops across the lanes; the phases follow the
+// reference's order, so every node's last assignment is the reference's.
+__global__ void __launch_bounds__(kMoveThreads) mtrk_move_kernel(MeshTrk t, SweepArgs a, double* __restrict__ X2,
+                                                                 double* __restrict__ Y2)
 {
     __shared__ double vx[kMaxLineNodes], vy[kMaxLineNodes];
     __shared__ int cnt[kMaxLineNodes];
-    const int lane = threadIdx.x;
+    __shared__ double sc[kMaxLineFaces * 6];
+    __shared__ int sl[kMaxLineFaces * 2];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1), wave = tid / kWave;
     const double* X = a.x;
     const double* Y = a.y;
     const double dt = dt_read(a);
-    for (int k = lane; k < t.nln; k += kWave) { vx[k] = 0.0; vy[k] = 0.0; cnt[k] = 0; }
+    for (int k = tid; k < t.nln; k += kMoveThreads) { vx[k] = 0.0; vy[k] = 0.0; cnt[k] = 0; }
+    for (int k = tid; k < t.nlf * 6; k += kMoveThreads) sc[k] = t.contrib[k];
+    for (int k = tid; k < t.nlf * 2; k += kMoveThreads) sl[k] = t.lslot[k];
     __syncthreads();
-    if (lane == 0) {
-        const int* lnode = t.node_slot;
-        (void)lnode;
-        int base = 0;
-        for (int line = 0; line < 3; line++) {
-            const int n = t.nline[line];
-            for (int j = 0; j < n; j++) {                   // the line's sums, face by face
-                const double* r = t.contrib + (size_t)(base + j) * 6;
-                const int sa = t.lslot[(size_t)(base + j) * 2], sb = t.lslot[(size_t)(base + j) * 2 + 1];
-                if (r[2] > 0.0) { vx[sa] += r[0]; vy[sa] += r[1]; cnt[sa]++; }
-                if (r[5] > 0.0) { vx[sb] += r[3]; vy[sb] += r[4]; cnt[sb]++; }
-            }
-            // averaging: the contact line's A nodes (its own loop, then again in each
-            // shock's loop over the shock's face count: Setka.cpp:3522-3530,
-            // :3718-3731, :3829-3837), or each line its own (average_lines)
-            const int avg = (line == 0 || t.average_lines) ? base : 0;
-            for (int j = 0; j < n; j++) {
-                const int sa = t.lslot[(size_t)(avg + j) * 2];
-                if (cnt[sa] > 0) {
-                    vx[sa] /= (1.0 * cnt[sa]);
-                    vy[sa] /= (1.0 * cnt[sa]);
+    // Move_surface section by section (contact, inner shock, outer shock): each line
+    // node's contributions in face order (at most two, those of the faces it ends),
+    // the averaging loop's nodes (distinct), the two end corrections
+    for (int line = 0; line < 3; line++) {
+        for (int k = t.sum_off[line] + tid; k < t.sum_off[line + 1]; k += kMoveThreads) {
+            const int* e = t.sums + 3 * k;
+            for (int c = 1; c <= 2; c++)
+                if (e[c] >= 0) {
+                    const double* r = sc + (e[c] >> 1) * 6 + (e[c] & 1) * 3;
+                    if (r[2] > 0.0) { vx[e[0]] += r[0]; vy[e[0]] += r[1]; cnt[e[0]]++; }
                 }
+        }
+        __syncthreads();
+        for (int k = t.avg_off[line] + tid; k < t.avg_off[line + 1]; k += kMoveThreads) {
+            const int sa = t.avgs[k];
+            if (cnt[sa] > 0) {
+                vx[sa] /= (1.0 * cnt[sa]);
+                vy[sa] /= (1.0 * cnt[sa]);
             }
-            // end nodes (Setka.cpp:3532-3535, :3733-3736, :3839-3842)
+        }
+        __syncthreads();
+        if (tid == 0) {                                     // Setka.cpp:3532-3535, :3733-3736, :3839-3842
+            const int base = line == 0 ? 0 : (line == 1 ? t.nline[0] : t.nline[0] + t.nline[1]);
+            const int n = t.nline[line];
             const int* f0 = t.lf + (size_t)base * 6;
             const int* fl = t.lf + (size_t)(base + n - 1) * 6;
-            const int a0 = t.lslot[(size_t)base * 2], b0 = t.lslot[(size_t)base * 2 + 1];
-            const int al = t.lslot[(size_t)(base + n - 1) * 2], bl = t.lslot[(size_t)(base + n - 1) * 2 + 1];
+            const int a0 = sl[base * 2], b0 = sl[base * 2 + 1];
+            const int al = sl[(base + n - 1) * 2], bl = sl[(base + n - 1) * 2 + 1];
             vx[a0] = vx[b0] + (X[f0[1]] - X[f0[0]]) / dt;
             if (line == 1) vx[bl] = vx[al] + (X[fl[0]] - X[fl[1]]) / dt;
             else vy[bl] = vy[al] + (Y[fl[0]] - Y[fl[1]]) / dt;
-            base += n;
         }
+        __syncthreads();
     }
-    __syncthreads();
     auto speed = [&](int p, double& ux, double& uy) {
-        const int sl = t.node_slot[p];
-        ux = sl >= 0 ? vx[sl] : 0.0;
-        uy = sl >= 0 ? vy[sl] : 0.0;
+        const int q = t.node_slot[p];
+        ux = q >= 0 ? vx[q] : 0.0;
+        uy = q >= 0 ? vy[q] : 0.0;
     };
-    for (int k = lane; k < t.nfree; k += kWave) {
+    for (int k = tid; k < t.nfree; k += kMoveThreads) {
         const int p = t.freep[k];
         X2[p] = X[p];
         Y2[p] = Y[p];
     }
+    __builtin_amdgcn_s_waitcnt(0);                           // a D rail may read a node no rail places
+    __threadfence();
+    __syncthreads();
     const double R1 = t.r_inner, R111 = t.r_mid, R5 = t.r_outer;
-    double dprev = 0.0;                                      // the previous D rail's outer-shock y2
-    int dk = 0;
-    for (int q = 0; q < t.nrail; q++) {
-        const int* rr = t.rail + (size_t)q * kRailInts;
-        const int kind = rr[0], M1 = rr[1], M2 = rr[2], M3 = rr[3], M4 = rr[4];
-        const int* P = t.allp + rr[5];
-        const int nall = rr[6];
-        const int K0 = rr[7], K1 = rr[8], K2 = rr[9], K3 = rr[10];
-        const double* fmid = t.fpow + rr[11];
-        const double* fout = t.fpow + rr[12];
-        const double cs = t.rcs[2 * q], sn = t.rcs[2 * q + 1];
-        auto put = [&](int p, double x2, double y2) { X2[p] = x2; Y2[p] = y2; };
-        double ux, uy, V;
-        // inner and mid blocks of A, B, C rails (Setka.cpp:4391-4405 and the same loops of
-        // the B and C rails) from R2, the radius of key point 0 after its move
-        auto inner_blocks = [&](double R2) {
-            for (int j = lane; j < M1 - 1; j += kWave) {
-                double r;
-                if (j < t.n_inner) r = R1 + t.fin[j] * (R111 - R1);
-                else r = R111 + fmid[j - t.n_inner] * (R2 - R111);
-                put(P[j], r * cs, r * sn);
+    // the outer-shock position D rails from d_chain on copy from the rail before, which
+    // is the one D rail d_chain - 1 computes (Setka.cpp:4700-4706)
+    double dchain = 0.0;
+    if (t.d_chain_rail >= 0) {
+        const int K2 = t.rail[(size_t)t.d_chain_rail * kRailInts + 9];
+        double ux, uy;
+        speed(K2, ux, uy);
+        const double V = uy;
+        dchain = Y[K2] + dt * V;
+    }
+    for (int ph = 0; ph < t.nphase; ph++) {
+        for (int q = t.phase[ph] + wave; q < t.phase[ph + 1]; q += kMoveThreads / kWave) {
+            const int* rr = t.rail + (size_t)q * kRailInts;
+            const int kind = rr[0], M1 = rr[1], M2 = rr[2], M3 = rr[3], M4 = rr[4];
+            const int* P = t.allp + rr[5];
+            const int nall = rr[6];
+            const int K0 = rr[7], K1 = rr[8], K2 = rr[9], K3 = rr[10];
+            const double* fmid = t.fpow + rr[11];
+            const double* fout = t.fpow + rr[12];
+            const double cs = t.rcs[2 * q], sn = t.rcs[2 * q + 1];
+            auto put = [&](int p, double x2, double y2) { X2[p] = x2; Y2[p] = y2; };
+            // a key point the rail's loops place later is not stored (its value would be
+            // overwritten), so that no two stores of one rail meet at one address
+            auto key = [&](int k, int p, double x2, double y2) { if ((rr[14] >> k) & 1) put(p, x2, y2); };
+            double ux, uy, V;
+            // inner and mid blocks of A, B, C rails (Setka.cpp:4391-4405 and the same
+            // loops of the B and C rails) from R2, the radius of key point 0 after its move
+            auto inner_blocks = [&](double R2) {
+                for (int j = lane; j < M1 - 1; j += kWave) {
+                    double r;
+                    if (j < t.n_inner) r = R1 + t.fin[j] * (R111 - R1);
+                    else r = R111 + fmid[j - t.n_inner] * (R2 - R111);
+                    put(P[j], r * cs, r * sn);
+                }
+            };
+            if (kind == 0) {                                 // A (Setka.cpp:4331-4457)
+                double k0x, k0y, k1x, k1y, k2x, k2y;
+                speed(K0, ux, uy); V = ux * cs + uy * sn; k0x = X[K0] + dt * V * cs; k0y = Y[K0] + dt * V * sn;
+                speed(K1, ux, uy); V = ux * cs + uy * sn; k1x = X[K1] + dt * V * cs; k1y = Y[K1] + dt * V * sn;
+                speed(K2, ux, uy); V = ux * cs + uy * sn; k2x = X[K2] + dt * V * cs; k2y = Y[K2] + dt * V * sn;
+                if (lane == 0) { key(0, K0, k0x, k0y); key(1, K1, k1x, k1y); key(2, K2, k2x, k2y); key(3, K3, X[K3], Y[K3]); }
+                const double R2 = sqrt(sq(k0x) + sq(k0y));
+                inner_blocks(R2);
+                const double R3 = sqrt(sq(k1x) + sq(k1y));
+                for (int j = lane; j < M2 - 1; j += kWave) {
+                    const double r = R2 + (R3 - R2) * (j + 1) / (M2);
+                    put(P[M1 + j], r * cs, r * sn);
+                }
+                const double R4 = sqrt(sq(k2x) + sq(k2y));
+                for (int j = lane; j < M3 - 1; j += kWave) {
+                    const double r = R3 + (R4 - R3) * (j + 1) / (M3);
+                    put(P[M1 + M2 + j], r * cs, r * sn);
+                }
+                for (int j = lane; j < M4; j += kWave) {
+                    const double r = R4 + fout[j] * (R5 - R4);
+                    put(P[M1 + M2 + M3 + j], r * cs, r * sn);
+                }
+            } else if (kind == 1) {                          // B (Setka.cpp:4468-4593)
+                speed(K0, ux, uy); V = ux * cs + uy * sn;
+                const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
+                speed(K1, ux, uy); V = ux * 0.0 + uy * 1.0;
+                const double k1y = Y[K1] + dt * V;
+                speed(K2, ux, uy); V = ux * 0.0 + uy * 1.0;
+                const double k2y = Y[K2] + dt * V;
+                if (lane == 0) { key(0, K0, k0x, k0y); key(1, K1, k0x, k1y); key(2, K2, k0x, k2y); key(3, K3, k0x, Y[K3]); }
+                inner_blocks(sqrt(sq(k0x) + sq(k0y)));
+                for (int j = lane; j < M2 - 1; j += kWave) put(P[M1 + j], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
+                for (int j = lane; j < M3 - 1; j += kWave) put(P[M1 + M2 + j], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
+                for (int j = lane; j < M4; j += kWave) put(P[M1 + M2 + M3 + j], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
+            } else if (kind == 2) {                          // C (Setka.cpp:4596-4670)
+                speed(K0, ux, uy); V = ux * cs + uy * sn;
+                const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
+                if (lane == 0) { key(0, K0, k0x, k0y); key(1, K1, t.left, Y[K1]); }
+                inner_blocks(sqrt(sq(k0x) + sq(k0y)));
+                const double R2 = k0x, R3 = t.left;
+                const double xg = pow(R3 / (R2), 1.0 / (M2));
+                for (int j = M1 + lane; j < nall; j += kWave) put(P[j], R2 * pow(xg, (double)(j + 1 - M1)), k0y);
+            } else {                                         // D (Setka.cpp:4672-4759)
+                const double k0x = X2[K0], k0y = Y2[K0];     // placed by a rail of an earlier phase
+                speed(K1, ux, uy); V = uy;
+                const double k1y = Y[K1] + dt * V;
+                speed(K2, ux, uy); V = uy;
+                const double k2y = rr[13] ? dchain : Y[K2] + dt * V;
+                if (lane == 0) { key(1, K1, k0x, k1y); key(2, K2, k0x, k2y); key(3, K3, k0x, Y[K3]); }
+                for (int j = lane; j < M2 - 1; j += kWave) put(P[j + 1], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
+                for (int j = lane; j < M3 - 1; j += kWave) put(P[M2 + j + 1], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
+                for (int j = lane; j < M4; j += kWave) put(P[M2 + M3 + j + 1], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
             }
-        };
-        if (kind == 0) {                                     // A (Setka.cpp:4331-4457)
-            double k0x, k0y, k1x, k1y, k2x, k2y;
-            speed(K0, ux, uy); V = ux * cs + uy * sn; k0x = X[K0] + dt * V * cs; k0y = Y[K0] + dt * V * sn;
-            speed(K1, ux, uy); V = ux * cs + uy * sn; k1x = X[K1] + dt * V * cs; k1y = Y[K1] + dt * V * sn;
-            speed(K2, ux, uy); V = ux * cs + uy * sn; k2x = X[K2] + dt * V * cs; k2y = Y[K2] + dt * V * sn;
-            if (lane == 0) { put(K0, k0x, k0y); put(K1, k1x, k1y); put(K2, k2x, k2y); put(K3, X[K3], Y[K3]); }
-            __builtin_amdgcn_s_waitcnt(0);                   // the key points' stores before the loops'
-            const double R2 = sqrt(sq(k0x) + sq(k0y));
-            inner_blocks(R2);
-            const double R3 = sqrt(sq(k1x) + sq(k1y));
-            for (int j = lane; j < M2 - 1; j += kWave) {
-                const double r = R2 + (R3 - R2) * (j + 1) / (M2);
-                put(P[M1 + j], r * cs, r * sn);
-            }
-            const double R4 = sqrt(sq(k2x) + sq(k2y));
-            for (int j = lane; j < M3 - 1; j += kWave) {
-                const double r = R3 + (R4 - R3) * (j + 1) / (M3);
-                put(P[M1 + M2 + j], r * cs, r * sn);
-            }
-            for (int j = lane; j < M4; j += kWave) {
-                const double r = R4 + fout[j] * (R5 - R4);
-                put(P[M1 + M2 + M3 + j], r * cs, r * sn);
-            }
-        } else if (kind == 1) {                              // B (Setka.cpp:4468-4593)
-            speed(K0, ux, uy); V = ux * cs + uy * sn;
-            const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
-            speed(K1, ux, uy); V = ux * 0.0 + uy * 1.0;
-            const double k1y = Y[K1] + dt * V;
-            speed(K2, ux, uy); V = ux * 0.0 + uy * 1.0;
-            const double k2y = Y[K2] + dt * V;
-            if (lane == 0) { put(K0, k0x, k0y); put(K1, k0x, k1y); put(K2, k0x, k2y); put(K3, k0x, Y[K3]); }
-            __builtin_amdgcn_s_waitcnt(0);
-            inner_blocks(sqrt(sq(k0x) + sq(k0y)));
-            for (int j = lane; j < M2 - 1; j += kWave) put(P[M1 + j], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
-            for (int j = lane; j < M3 - 1; j += kWave) put(P[M1 + M2 + j], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
-            for (int j = lane; j < M4; j += kWave) put(P[M1 + M2 + M3 + j], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
-        } else if (kind == 2) {                              // C (Setka.cpp:4596-4670)
-            speed(K0, ux, uy); V = ux * cs + uy * sn;
-            const double k0x = X[K0] + dt * V * cs, k0y = Y[K0] + dt * V * sn;
-            if (lane == 0) { put(K0, k0x, k0y); put(K1, t.left, Y[K1]); }
-            __builtin_amdgcn_s_waitcnt(0);
-            inner_blocks(sqrt(sq(k0x) + sq(k0y)));
-            const double R2 = k0x, R3 = t.left;
-            const double xg = pow(R3 / (R2), 1.0 / (M2));
-            for (int j = M1 + lane; j < nall; j += kWave) put(P[j], R2 * pow(xg, (double)(j + 1 - M1)), k0y);
-        } else {                                             // D (Setka.cpp:4672-4759)
-            __builtin_amdgcn_s_waitcnt(0);                   // key point 0 was placed by an earlier rail
-            __threadfence();
-            const double k0x = X2[K0], k0y = Y2[K0];
-            speed(K1, ux, uy); V = uy;
-            const double k1y = Y[K1] + dt * V;
-            speed(K2, ux, uy); V = uy;
-            double k2y = Y[K2] + dt * V;
-            if (dk >= t.d_chain) k2y = dprev;
-            dprev = k2y;
-            dk++;
-            if (lane == 0) { put(K1, k0x, k1y); put(K2, k0x, k2y); put(K3, k0x, Y[K3]); }
-            __builtin_amdgcn_s_waitcnt(0);
-            for (int j = lane; j < M2 - 1; j += kWave) put(P[j + 1], k0x, k0y + (k1y - k0y) * (j + 1) / (M2));
-            for (int j = lane; j < M3 - 1; j += kWave) put(P[M2 + j + 1], k0x, k1y + (k2y - k1y) * (j + 1) / (M3));
-            for (int j = lane; j < M4; j += kWave) put(P[M2 + M3 + j + 1], k0x, k2y + (R5 - k2y) * (j + 1) / (M4));
         }
-        __builtin_amdgcn_s_waitcnt(0);                       // this rail's stores land before the next rail's
+        __builtin_amdgcn_s_waitcnt(0);                       // the phase's stores land, and are seen,
+        __threadfence();                                     // before the next phase reads or overwrites
+        __syncthreads();
     }
 }
 
@@ -775,7 +803,42 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
             lslot[(size_t)g * 2 + e] = node_slot[p];
         }
     }
-    if ((int)lnode.size() > kMaxLineNodes) { err = "more line nodes than the tracking kernel holds"; return GEL_E_ARG; }
+    // each line node's contributions in its line's face order, and the averaging loops'
+    // nodes: the contact line's A nodes (its own loop, then each shock's loop over the
+    // shock's face count), or each line its own (average_lines)
+    std::vector<int> sum_off{ 0 }, sums, avg_off{ 0 }, avgs;
+    for (int line = 0, base = 0; line < 3; line++) {
+        const int n = r->n_line[line];
+        std::map<int, std::vector<int>> per;             // line slot -> 2 face + role
+        std::vector<int> order;
+        for (int j = 0; j < n; j++)
+            for (int e = 0; e < 2; e++) {
+                const int sl = lslot[(size_t)(base + j) * 2 + e];
+                if (!per.count(sl)) order.push_back(sl);
+                per[sl].push_back(2 * (base + j) + e);
+            }
+        for (int sl : order) {
+            if (per[sl].size() > 2) { err = "a line node ends more than two of its line's faces"; return GEL_E_ARG; }
+            sums.push_back(sl);
+            sums.push_back(per[sl][0]);
+            sums.push_back(per[sl].size() > 1 ? per[sl][1] : -1);
+        }
+        sum_off.push_back((int)sums.size() / 3);
+        const int avg = (line == 0 || tr->average_lines) ? base : 0;
+        std::vector<char> seen(lnode.size() + 1, 0);
+        for (int j = 0; j < n; j++) {
+            const int sa = lslot[(size_t)(avg + j) * 2];
+            if (seen[sa]) { err = "a line node starts two faces of one averaging loop"; return GEL_E_ARG; }
+            seen[sa] = 1;
+            avgs.push_back(sa);
+        }
+        avg_off.push_back((int)avgs.size());
+        base += n;
+    }
+    if ((int)lnode.size() > kMaxLineNodes || nlf > kMaxLineFaces) {
+        err = "more line faces or nodes than the tracking kernel holds";
+        return GEL_E_ARG;
+    }
     // rails
     const int nr = r->n_rails;
     const double dd = tr->s_outer / std::sqrt(1.0 - 1.0 / tr->p_outer);
@@ -799,7 +862,13 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
         for (int k = o0; k < o1; k++)
             if (r->all[k] < 0 || r->all[k] >= nn) { err = "rail point out of range"; return GEL_E_ARG; }
         const int* P = r->all + o0;
+        {
+            std::vector<int> u(P, P + nall);
+            std::sort(u.begin(), u.end());
+            if (std::adjacent_find(u.begin(), u.end()) != u.end()) { err = "a rail lists a point twice"; return GEL_E_ARG; }
+        }
         std::vector<int> wrote;                          // the rail's assignments, for the checks
+        size_t nloop = 0;                                // of them, its loops'
         if (kind <= 2) {
             if (M1 - 1 < n_inner || M1 < 1) { err = "rails: M1 - 1 must be at least n_inner"; return GEL_E_ARG; }
             for (int j = 0; j < M1 - 1; j++) wrote.push_back(P[j]);
@@ -809,10 +878,12 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
             for (int j = 0; j < M2 - 1; j++) wrote.push_back(P[M1 + j]);
             for (int j = 0; j < M3 - 1; j++) wrote.push_back(P[M1 + M2 + j]);
             for (int j = 0; j < M4; j++) wrote.push_back(P[M1 + M2 + M3 + j]);
+            nloop = wrote.size();
             for (int k = 0; k < 4; k++) wrote.push_back(K[k]);
         } else if (kind == 2) {
             if (M2 < 1 || nall < M1) { err = "C rails: M2 >= 1, All_point at least M1"; return GEL_E_ARG; }
             for (int j = M1; j < nall; j++) wrote.push_back(P[j]);
+            nloop = wrote.size();
             wrote.push_back(K[0]); wrote.push_back(K[1]);
         } else {
             if (M2 < 1 || M3 < 1 || M4 < 1 || M2 + M3 + M4 >= nall || K[2] != P[M2 + M3]) {
@@ -822,6 +893,7 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
             for (int j = 0; j < M2 - 1; j++) wrote.push_back(P[j + 1]);
             for (int j = 0; j < M3 - 1; j++) wrote.push_back(P[M2 + j + 1]);
             for (int j = 0; j < M4; j++) wrote.push_back(P[M2 + M3 + j + 1]);
+            nloop = wrote.size();
             wrote.push_back(K[1]); wrote.push_back(K[2]); wrote.push_back(K[3]);
         }
         wrotes.push_back(wrote);
@@ -829,6 +901,10 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
         rr[0] = kind; rr[1] = M1; rr[2] = M2; rr[3] = M3; rr[4] = M4;
         rr[5] = (int)allp.size(); rr[6] = nall;
         for (int k = 0; k < 4; k++) rr[7 + k] = k < nkey ? K[k] : K[0];
+        int kmask = 0;
+        for (int k = 0; k < nkey; k++)
+            if (std::find(wrote.begin(), wrote.begin() + nloop, K[k]) == wrote.begin() + nloop) kmask |= 1 << k;
+        rr[14] = kmask;
         allp.insert(allp.end(), P, P + nall);
         // host libm, the calls the reference makes at run time (Setka.cpp:4335-4450)
         rcs[2 * q] = std::cos(r->s[q]);
@@ -858,9 +934,32 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
     std::vector<int> freep;
     for (int p = 0; p < nn; p++)
         if (!any[p]) freep.push_back(p);
+    // D rails from d_chain on copy the outer-shock position of D rail d_chain - 1
+    int d_chain_rail = -1;
+    for (int q = 0, dk = 0; q < nr; q++) {
+        if (r->kind[q] != 3) continue;
+        if (dk == r->d_chain - 1) d_chain_rail = q;
+        rail[(size_t)q * kRailInts + 13] = dk >= r->d_chain ? 1 : 0;
+        dk++;
+    }
+    // phases: runs of consecutive rails none of which writes a point another of the run
+    // writes or reads (a D rail reads its key point 0)
+    std::vector<int> phase{ 0 }, wstamp(nn, -1), rstamp(nn, -1);
+    for (int q = 0; q < nr; q++) {
+        const int ph = (int)phase.size() - 1;
+        bool clash = false;
+        for (int p : wrotes[q]) clash = clash || wstamp[p] == ph || rstamp[p] == ph;
+        if (r->kind[q] == 3) clash = clash || wstamp[r->key[4 * q]] == ph;
+        if (clash) phase.push_back(q);
+        const int cur = (int)phase.size() - 1;
+        for (int p : wrotes[q]) wstamp[p] = cur;
+        if (r->kind[q] == 3) rstamp[r->key[4 * q]] = cur;
+    }
+    phase.push_back(nr);
     if (fpow.empty()) fpow.push_back(0.0);
     // one allocation: ints, then doubles, then the face contributions
-    const size_t ni_ = lf.size() + lslot.size() + node_slot.size() + rail.size() + allp.size() + freep.size();
+    const size_t ni_ = lf.size() + lslot.size() + node_slot.size() + rail.size() + allp.size() + freep.size() +
+                       phase.size() + sum_off.size() + sums.size() + avg_off.size() + avgs.size();
     const size_t offd = ((ni_ * 4 + 255) / 256) * 256;
     const size_t nd_ = rcs.size() + fin.size() + fpow.size();
     const size_t total = offd + (nd_ + (size_t)nlf * 6) * 8;
@@ -876,7 +975,10 @@ int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* tr, const gel_mesh_rai
     };
     MeshTrk t{};
     t.lf = puti(lf); t.lslot = puti(lslot); t.node_slot = puti(node_slot); t.rail = puti(rail);
-    t.allp = puti(allp); t.freep = puti(freep);
+    t.allp = puti(allp); t.freep = puti(freep); t.phase = puti(phase);
+    t.sum_off = puti(sum_off); t.sums = puti(sums); t.avg_off = puti(avg_off); t.avgs = puti(avgs);
+    t.nphase = (int)phase.size() - 1;
+    t.d_chain_rail = d_chain_rail;
     at = offd;
     auto putd = [&](const std::vector<double>& v) {
         const double* p = (const double*)((char*)base + at);
@@ -914,7 +1016,7 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
         a.x2 = c->d_x2;
         a.y2 = c->d_y2;
         hipLaunchKernelGGL(mtrk_face_kernel, dim3((unsigned)((t.nlf + 3) / 4)), dim3(256), 0, s, d, t, a);
-        hipLaunchKernelGGL(mtrk_move_kernel, dim3(1), dim3(kWave), 0, s, t, a, c->d_x2, c->d_y2);
+        hipLaunchKernelGGL(mtrk_move_kernel, dim3(1), dim3(kMoveThreads), 0, s, t, a, c->d_x2, c->d_y2);
         moving = true;
     }
     // grids of a multiple of eight work-groups, so that each XCD takes a contiguous
