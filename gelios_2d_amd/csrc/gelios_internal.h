@@ -100,6 +100,16 @@ struct MeshBuf;                   // device face tables of a mesh context (mesh.
 int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err);
 int gel_mesh_set_tracking(gel_ctx* c, const gel_tracking* t, const gel_mesh_rails* r, std::string& err);
 void gel_mesh_free(gel_ctx* c);
+// The mesh tables the Monte Carlo walks (device pointers), each face slot's way on
+// (code: the neighbour cell, -1 out of the grid, -2 across the inner sphere, -3 an Axis
+// slot, skipped) and its boundary face lists as (A, B, cell) triples: outer (entry)
+// faces and inner-sphere faces.
+struct MeshMcView {
+    const int *coff, *corners, *soff, *sA, *sB;
+    const int8_t* ctype;
+};
+void gel_mesh_mc_view(const gel_ctx* c, MeshMcView* v, std::vector<int>& code, std::vector<int>& outer,
+                      std::vector<int>& inner);
 namespace gel {
 // One step of the general-mesh sweep (mesh.hip); moving: next node positions in a.x2, a.y2.
 hipError_t gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving);

@@ -39,6 +39,7 @@ constexpr int kMaxGenerations = 4096;             // bound on the secondary gene
 constexpr int kMaxTries = 100000;                 // bound on the sampler's rejections (a proposal is
                                                   // accepted with probability of order one)
 
+// i, j: the cell on a polar grid; on a general mesh i is the cell and j is 0
 struct Atom {
     double p[3], v[3], mu;
     int i, j, area, s[3];
@@ -62,6 +63,19 @@ struct Args {
     unsigned long long* nout;
     long long cap;
     unsigned long long* cnt;          // [kCnt]
+    // a general mesh (the reference's own grid): the cells' corners and face slots in
+    // Cell::Grans order with each slot's way on (scode: the next cell, -1 out of the grid,
+    // -2 across the inner sphere, -3 an Axis slot), and the boundary faces the atoms
+    // enter by (outer) and cross the inner sphere to (inner), as (A, B, cell) triples
+    const int* __restrict__ coff;
+    const int* __restrict__ corners;
+    const int* __restrict__ soff;
+    const int* __restrict__ sA;
+    const int* __restrict__ sB;
+    const int* __restrict__ scode;
+    const int* __restrict__ bout;
+    const int* __restrict__ bin;
+    int nbout, nbin;
 };
 
 // ------------------------------------------------------------------- sensor ----
@@ -369,6 +383,34 @@ __device__ double row_hit(const Args& a, int q, const double* p, const double* v
     return best;
 }
 
+// Way out of mesh cell cc: the nearest crossing over its face slots, Axis faces skipped
+// (the reference's loop over now->Grans, Setka.cpp:12893-12960); slot = the slot crossed.
+__device__ double mesh_exit(const Args& a, int cc, const double* p, const double* v, double tmin, int& slot)
+{
+    double best = -1.0;
+    slot = -1;
+    for (int s = a.soff[cc]; s < a.soff[cc + 1]; s++) {
+        if (a.scode[s] == -3) continue;
+        const int A = a.sA[s], B = a.sB[s];
+        const double t = hit(p, v, a.x[A], a.y[A], a.x[B], a.y[B], tmin);
+        if (t > 0.0 && (best < 0.0 || t < best)) { best = t; slot = s; }
+    }
+    return best;
+}
+
+// first crossing with a list of n (A, B, cell) faces; jj = the list entry, -1 if none
+__device__ double list_hit(const Args& a, const int* L, int n, const double* p, const double* v, double tmin, int& jj)
+{
+    double best = -1.0;
+    jj = -1;
+    for (int c = 0; c < n; c++) {
+        const int A = L[3 * c], B = L[3 * c + 1];
+        const double t = hit(p, v, a.x[A], a.y[A], a.x[B], a.y[B], tmin);
+        if (t > 0.0 && (best < 0.0 || t < best)) { best = t; jj = c; }
+    }
+    return best;
+}
+
 // the plasma velocity of cell k at the point (y, z): (u, v cos, v sin) of its angle about the axis
 __device__ __forceinline__ void plasma_at(const Args& a, long long k, double y, double z, double* U, double& ca,
                                           double& sa)
@@ -390,6 +432,7 @@ __device__ __forceinline__ void tally(double* p, double v) { unsafeAtomicAdd(p, 
 // exchange point from the truncated exponential, the weight before it and the rest after
 // it tallied with their midpoint directions, the source integrals at the exchange point,
 // the secondary (population of the cell, kernel velocity, roulette at wmin) queued.
+template <bool MESH>
 __device__ void trace(const Args& a, Atom at)
 {
     const long long nc = a.nc;
@@ -397,13 +440,21 @@ __device__ void trace(const Args& a, Atom at)
     unsigned long long segs = 0, lost = 0;
     int seg = 0;
     for (; seg < a.max_segments; seg++) {
-        const long long k = (long long)at.i * a.nj + at.j;
+        const long long k = MESH ? (long long)at.i : (long long)at.i * a.nj + at.j;
         const double vn = sqrt(at.v[0] * at.v[0] + at.v[1] * at.v[1] + at.v[2] * at.v[2]);
-        const int n00 = at.i * (a.nj + 1) + at.j, n11 = n00 + a.nj + 2;
+        int n00, n11;   // a diagonal of the cell: its corners 0 and 2
+        if constexpr (MESH) {
+            n00 = a.corners[a.coff[k]];
+            n11 = a.corners[a.coff[k] + 2];
+        } else {
+            n00 = at.i * (a.nj + 1) + at.j;
+            n11 = n00 + a.nj + 2;
+        }
         const double dX = a.x[n11] - a.x[n00], dY = a.y[n11] - a.y[n00];
         const double tmin = 1e-9 * sqrt(dX * dX + dY * dY) / vn;
         int face;
-        const double t_exit = cell_exit(a, at.i, at.j, at.p, at.v, tmin, face);
+        const double t_exit = MESH ? mesh_exit(a, at.i, at.p, at.v, tmin, face)
+                                   : cell_exit(a, at.i, at.j, at.p, at.v, tmin, face);
         if (face < 0) { lost++; break; }
         segs++;
         const double ro = a.pl[k], cp = a.pl[nc + k];
@@ -513,7 +564,21 @@ __device__ void trace(const Args& a, Atom at)
         }
 #pragma unroll
         for (int d = 0; d < 3; d++) at.p[d] += t_exit * at.v[d];
-        if (face == 1) {
+        if constexpr (MESH) {
+            const int to = a.scode[face];
+            if (to >= 0) {
+                at.i = to;
+            } else if (to == -2) {                 // across the inner sphere
+                int jj;
+                const double th = list_hit(a, a.bin, a.nbin, at.p, at.v, tmin, jj);
+                if (jj < 0) { lost++; break; }
+#pragma unroll
+                for (int d = 0; d < 3; d++) at.p[d] += th * at.v[d];
+                at.i = a.bin[3 * jj + 2];
+            } else {
+                break;                             // out of the grid
+            }
+        } else if (face == 1) {
             if (at.i == a.ni - 1) break;           // out of the grid
             at.i++;
         } else if (face == 0) {
@@ -542,6 +607,7 @@ __device__ void trace(const Args& a, Atom at)
 // Primary number kk: velocity from the speed-weighted drifting Maxwellian, entry point
 // uniform on the disc across it of the sphere of radius Rs around the grid, then in a
 // straight line to the grid's outer surface.  False if it misses the grid.
+template <bool MESH>
 __device__ bool primary(const Args& a, long long kk, Atom& at)
 {
     seed_primary(a.seed, kk, at.s);
@@ -569,7 +635,7 @@ __device__ bool primary(const Args& a, long long kk, Atom& at)
 #pragma unroll
     for (int d = 0; d < 3; d++) p[d] = c * e1[d] + s * e2[d] - back * e[d];
     int jj;
-    const double t = row_hit(a, a.ni, p, v, 0.0, jj);
+    const double t = MESH ? list_hit(a, a.bout, a.nbout, p, v, 0.0, jj) : row_hit(a, a.ni, p, v, 0.0, jj);
     if (jj < 0) return false;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -577,27 +643,29 @@ __device__ bool primary(const Args& a, long long kk, Atom& at)
         at.v[d] = v[d];
     }
     at.mu = 1.0;
-    at.i = a.ni - 1;
-    at.j = jj;
+    at.i = MESH ? a.bout[3 * jj + 2] : a.ni - 1;
+    at.j = MESH ? 0 : jj;
     at.area = 3;
     return true;
 }
 
+template <bool MESH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) primary_kernel(Args a, long long k0, long long n)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     Atom at;
-    if (!primary(a, k0 + t, at)) return;
+    if (!primary<MESH>(a, k0 + t, at)) return;
     atomicAdd(a.cnt, 1ull);
-    trace(a, at);
+    trace<MESH>(a, at);
 }
 
+template <bool MESH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) trace_kernel(Args a)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.nin) return;
-    trace(a, a.qin[t]);
+    trace<MESH>(a, a.qin[t]);
 }
 
 // rho, cp = sqrt(p / rho), u, v, p of the plasma; bad counts cells with rho <= 0 or p <= 0
@@ -618,19 +686,31 @@ __global__ void __launch_bounds__(256) plasma_kernel(const TS* __restrict__ st, 
 
 // The post-processing of MK_start_new (Setka.cpp:12241-12449): moments per population and
 // of all, sources per unit volume, the closure with the new moments and k = I / M.
+template <bool MESH>
 __global__ void __launch_bounds__(256) finalize_kernel(Args a, double Phi, double n_p, long long nprim, double* out)
 {
     const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long nc = a.nc;
     if (k >= nc) return;
-    const int i = (int)(k / a.nj), j = (int)(k % a.nj), w = a.nj + 1;
-    const int q[4] = { i * w + j, (i + 1) * w + j, (i + 1) * w + j + 1, i * w + j + 1 };
     double A = 0.0, G = 0.0;
-    for (int m = 0; m < 4; m++) {   // Cell::Get_Volume_rotate(360) (Cell.cpp:188-206)
-        const int p0 = q[m], p1 = q[(m + 1) & 3];
-        const double cr = a.x[p0] * a.y[p1] - a.y[p0] * a.x[p1];
-        A = A + cr;
-        G = G + (a.y[p0] + a.y[p1]) * cr;
+    if constexpr (MESH) {   // Cell::Get_Volume_rotate(360) (Cell.cpp:188-206) over the contour
+        const int* q = a.corners + a.coff[k];
+        const int n = a.coff[k + 1] - a.coff[k];
+        for (int m = 0; m < n; m++) {
+            const int p0 = q[m], p1 = q[m + 1 == n ? 0 : m + 1];
+            const double cr = a.x[p0] * a.y[p1] - a.y[p0] * a.x[p1];
+            A = A + cr;
+            G = G + (a.y[p0] + a.y[p1]) * cr;
+        }
+    } else {
+        const int i = (int)(k / a.nj), j = (int)(k % a.nj), w = a.nj + 1;
+        const int q[4] = { i * w + j, (i + 1) * w + j, (i + 1) * w + j + 1, i * w + j + 1 };
+        for (int m = 0; m < 4; m++) {   // Cell::Get_Volume_rotate(360) (Cell.cpp:188-206)
+            const int p0 = q[m], p1 = q[(m + 1) & 3];
+            const double cr = a.x[p0] * a.y[p1] - a.y[p0] * a.x[p1];
+            A = A + cr;
+            G = G + (a.y[p0] + a.y[p1]) * cr;
+        }
     }
     A = A * 0.5;
     const double vol = fabs(A) * (1.0 / (6.0 * A)) * G * kPiM * 360.0 / 180.0;
@@ -856,7 +936,7 @@ int gel_mc_exchange(int n, const double* in, const int* seeds, int reps, double*
 int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st)
 {
     if (!c || !p) return gel_set_error(GEL_E_ARG, "null argument");
-    if (c->mesh) return gel_set_error(GEL_E_UNSUPPORTED, "the Monte Carlo runs on polar grids");
+    const bool mesh = c->mesh != nullptr;
     if (!c->motion.empty() || c->tracking)
         return gel_set_error(GEL_E_UNSUPPORTED, "the Monte Carlo runs on a grid at rest (no motion, no tracking)");
     if (!c->uploaded) return gel_set_error(GEL_E_STATE, "no state uploaded");
@@ -873,10 +953,16 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
     double Rs = 0.0;
     for (size_t k = 0; k < c->nn; k++) Rs = std::fmax(Rs, std::sqrt(hx[k] * hx[k] + hy[k] * hy[k]));
     Rs *= 1.0 + 1e-12;
+    MeshMcView mv{};
+    std::vector<int> scode, bout, bin;
+    if (mesh) {
+        gel_mesh_mc_view(c, &mv, scode, bout, bin);
+        if (bout.empty()) return gel_set_error(GEL_E_ARG, "the mesh has no outer boundary faces to enter by");
+    }
     std::vector<int8_t> pop(nc, 1);
     if (c->has_types) {   // population of a secondary born in a cell (Setka.cpp:13384-13398)
         std::vector<int8_t> ct(nc);
-        MC_HIP(hipMemcpy(ct.data(), c->d_ctype, nc, hipMemcpyDeviceToHost));
+        MC_HIP(hipMemcpy(ct.data(), mesh ? mv.ctype : c->d_ctype, nc, hipMemcpyDeviceToHost));
         for (long long k = 0; k < nc; k++)
             pop[k] = ct[k] == GEL_C_5 ? 3 : ct[k] == GEL_C_4 ? 2 : (ct[k] == GEL_C_1 || ct[k] == GEL_C_CENTR) ? 0 : 1;
     }
@@ -888,11 +974,18 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
     int8_t* dpop;
     mc::Atom *qa, *qb;
     unsigned long long *cnt, *nout;
+    int *dcode = nullptr, *dbout = nullptr, *dbin = nullptr;
     if (!b.get(&pl, 5 * (size_t)nc) || !b.get(&tab, 9 * (size_t)mc::kTabN) || !b.get(&tal, mc::kTal * (size_t)nc) ||
         !b.get(&dout, GEL_MC_NFIELDS * (size_t)nc) || !b.get(&dpop, (size_t)nc) || !b.get(&qa, (size_t)cap) ||
-        !b.get(&qb, (size_t)cap) || !b.get(&cnt, mc::kCnt + 1) || !b.get(&nout, 1))
+        !b.get(&qb, (size_t)cap) || !b.get(&cnt, mc::kCnt + 1) || !b.get(&nout, 1) ||
+        (mesh && (!b.get(&dcode, scode.size()) || !b.get(&dbout, bout.size()) || !b.get(&dbin, bin.size() + 1))))
         return gel_set_error(GEL_E_NOMEM, "Monte Carlo buffers");
     MC_HIP(hipMemcpyAsync(dpop, pop.data(), nc, hipMemcpyHostToDevice, s));
+    if (mesh) {
+        MC_HIP(hipMemcpyAsync(dcode, scode.data(), scode.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        MC_HIP(hipMemcpyAsync(dbout, bout.data(), bout.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        if (!bin.empty()) MC_HIP(hipMemcpyAsync(dbin, bin.data(), bin.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    }
     MC_HIP(hipMemsetAsync(cnt, 0, (mc::kCnt + 1) * sizeof(unsigned long long), s));
     MC_HIP(hipMemsetAsync(tal, 0, mc::kTal * (size_t)nc * sizeof(double), s));
     const dim3 gc((unsigned)((nc + 255) / 256));
@@ -929,6 +1022,18 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
     a.cap = cap;
     a.cnt = cnt;
     a.nout = nout;
+    if (mesh) {
+        a.ni = 0;
+        a.nj = 0;
+        a.coff = mv.coff; a.corners = mv.corners; a.soff = mv.soff; a.sA = mv.sA; a.sB = mv.sB;
+        a.scode = dcode;
+        a.bout = dbout;
+        a.nbout = (int)(bout.size() / 3);
+        a.bin = dbin;
+        a.nbin = (int)(bin.size() / 3);
+    }
+    const auto primaries = mesh ? mc::primary_kernel<true> : mc::primary_kernel<false>;
+    const auto tracer = mesh ? mc::trace_kernel<true> : mc::trace_kernel<false>;
     hipEvent_t e0, e1;
     MC_HIP(hipEventCreate(&e0));
     MC_HIP(hipEventCreate(&e1));
@@ -940,7 +1045,7 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
         mc::Atom *qin = qa, *qo = qb;
         a.qout = qo;
         if (hipMemsetAsync(nout, 0, sizeof(unsigned long long), s) != hipSuccess) { rc = GEL_E_HIP; break; }
-        hipLaunchKernelGGL(mc::primary_kernel, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, a, k0, nb);
+        hipLaunchKernelGGL(primaries, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s, a, k0, nb);
         int g = 0;
         for (;;) {
             unsigned long long n = 0;
@@ -953,7 +1058,7 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
             a.nin = std::min((long long)n, cap);
             a.qout = qo;
             if (hipMemsetAsync(nout, 0, sizeof(unsigned long long), s) != hipSuccess) { rc = GEL_E_HIP; break; }
-            hipLaunchKernelGGL(mc::trace_kernel, dim3((unsigned)((a.nin + 255) / 256)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL(tracer, dim3((unsigned)((a.nin + 255) / 256)), dim3(256), 0, s, a);
         }
         gens = std::max(gens, g);
     }
@@ -972,7 +1077,8 @@ int gel_mc_run(gel_ctx* c, const gel_mc_params* p, double* out, gel_mc_stats* st
         const double e = std::exp(-u * u / (cp * cp)), f = std::erf(u / cp);
         mean_speed[0] = e * cp / mc::kSqrtPiR + (u + cp * cp / (2.0 * u)) * f;
     }
-    hipLaunchKernelGGL(mc::finalize_kernel, gc, dim3(256), 0, s, a, Phi * mean_speed[0], p->n_p_lism, p->n_primary, dout);
+    hipLaunchKernelGGL(mesh ? mc::finalize_kernel<true> : mc::finalize_kernel<false>, gc, dim3(256), 0, s, a,
+                       Phi * mean_speed[0], p->n_p_lism, p->n_primary, dout);
     MC_HIP(hipGetLastError());
     if (c->prm.source_mode == GEL_SRC_MK1 || c->prm.source_mode == GEL_SRC_HYBRID) {
         const size_t n = (size_t)GEL_MK1_NFIELDS * c->nc;

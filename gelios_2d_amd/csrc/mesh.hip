@@ -611,6 +611,9 @@ struct MeshBuf {
     void* base = nullptr;
     // host copies of the face tables (fitted-surface lines are matched against them)
     std::vector<int> hA, hB, hown, hnbr, hdown, hup;
+    // the face slots as given (the Monte Carlo walks them): offsets per cell, Gran::A, B,
+    // type and Sosed
+    std::vector<int> hsoff, hsA, hsB, hstype, hsnbr;
     // fitted-surface tracking (gel_set_mesh_tracking)
     MeshTrk trk{};
     void* trk_base = nullptr;
@@ -743,9 +746,45 @@ int gel_mesh_build(gel_ctx* c, const gel_mesh* g, std::string& err)
         return GEL_E_HIP;
     }
     mb->hA = fA; mb->hB = fB; mb->hown = fown; mb->hnbr = fnbr; mb->hdown = fdown; mb->hup = fup;
+    mb->hsoff.assign(g->face_offset, g->face_offset + nc + 1);
+    mb->hsA = sA;
+    mb->hsB = sB;
+    mb->hstype.assign(g->face_type, g->face_type + nslot);
+    mb->hsnbr.assign(g->face_sosed, g->face_sosed + nslot);
     c->mesh = mb;
     c->mesh_faces = nf;
     return GEL_OK;
+}
+
+// The mesh tables the Monte Carlo walks: device pointers to the corners and face slots,
+// and on the host each slot's way on (code) and the two boundary lists as (A, B, cell)
+// triples.  A slot with a neighbour (Sosed, whatever its type: the inner sphere of the
+// reference's grid has the central cells behind it) leads to it; an Axis slot is
+// skipped (-3); an inner-sphere slot with no neighbour is crossed in a straight line to
+// another (-2, the inner list); any other boundary slot leaves the grid (-1, the outer
+// list, where primaries enter).
+void gel_mesh_mc_view(const gel_ctx* c, MeshMcView* v, std::vector<int>& code, std::vector<int>& outer,
+                      std::vector<int>& inner)
+{
+    const MeshBuf* mb = c->mesh;
+    const MeshDev& d = mb->dev;
+    v->coff = d.coff; v->corners = d.corners; v->soff = d.soff; v->sA = d.sA; v->sB = d.sB;
+    v->ctype = d.ctype;
+    const std::vector<int>&soff = mb->hsoff, &sA = mb->hsA, &sB = mb->hsB;
+    code.assign(mb->hstype.size(), -1);
+    outer.clear();
+    inner.clear();
+    for (int cc = 0; cc + 1 < (int)soff.size(); cc++)
+        for (int k = soff[cc]; k < soff[cc + 1]; k++) {
+            const int t = mb->hstype[k], n = mb->hsnbr[k];
+            if (t == GEL_F_AXIS) { code[k] = -3; continue; }
+            if (n >= 0) { code[k] = n; continue; }
+            code[k] = t == GEL_F_INNER_SPHERE ? -2 : -1;
+            std::vector<int>& L = t == GEL_F_INNER_SPHERE ? inner : outer;
+            L.push_back(sA[k]);
+            L.push_back(sB[k]);
+            L.push_back(cc);
+        }
 }
 
 // Tables of fitted-surface tracking on a mesh context (gel_set_mesh_tracking).  The

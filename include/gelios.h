@@ -519,10 +519,14 @@ typedef struct gel_mc_stats {
  *   25..28  F_n, F_u, F_v, F_T: moments of all populations together. */
 #define GEL_MC_NFIELDS 29
 
-/* Run the Monte Carlo on a context's grid and plasma state (polar grids, static, gas or
- * MHD: rho, p, u, v are read).  out (NULL: none) receives GEL_MC_NFIELDS x ncells; a
- * GEL_SRC_MK1 context also gets fields 0..18 as its source inputs, as from
- * gel_set_sources. */
+/* Run the Monte Carlo on a context's grid and plasma state (a polar grid, gas or MHD, or
+ * a general mesh -- the reference's own grid -- at rest: rho, p, u, v are read).  On a
+ * mesh the atoms walk each cell's face slots as Setka::Fly_exchenge_Imit_Korol walks
+ * Cell::Grans (Setka.cpp:12893-12960): Axis faces skipped, a face with a Sosed leads to
+ * it, an Inner_sphere face with none is crossed in a straight line to another, any other
+ * boundary face leaves the grid (primaries enter by those).  out (NULL: none) receives
+ * GEL_MC_NFIELDS x ncells; a GEL_SRC_MK1 context also gets fields 0..18 as its source
+ * inputs, as from gel_set_sources. */
 int gel_mc_run(gel_ctx* ctx, const gel_mc_params* p, double* out, gel_mc_stats* st);
 
 /* Point-wise pieces of the Monte Carlo on the device, for checks against the

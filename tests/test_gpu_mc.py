@@ -149,3 +149,106 @@ def test_device_mc_feeds_the_sweep():
     a.Go_5_komponent__MK2(w.steps)
     b.Go_5_komponent__MK2(w.steps)
     np.testing.assert_array_equal(a.download(), b.download())
+
+
+# ---------------------------------------------------------------- general mesh ----
+
+def _block_ctx(prim=None, source_mode=None):
+    """A context on the reference's own block grid (the Setka(N1..M4) checkpoint), at rest."""
+    from test_oracle_block import alpha, block
+    from gelios_2d_amd.mesh import mesh_from_alpha
+    d = block("b")
+    m = mesh_from_alpha(alpha(d))
+    prm = reference_params()
+    if source_mode is not None:
+        prm.source_mode = source_mode
+    s = Setka.from_mesh(m, prm)
+    s.upload(d["prim0"] if prim is None else prim(m.ncells))
+    return s, m
+
+
+def _mesh_volumes(m):
+    vol = np.zeros(m.ncells)
+    for c in range(m.ncells):
+        q = m.corners[m.coff[c]:m.coff[c + 1]]
+        px, py = m.x[q], m.y[q]
+        qx, qy = np.roll(px, -1), np.roll(py, -1)
+        vol[c] = abs(((py + qy) * (px * qy - py * qx)).sum()) / 6.0 * 2 * np.pi
+    return vol
+
+
+@pytest.mark.parametrize("kind", ["hot", "equilibrium"])
+def test_mesh_tracer_equals_polar_tracer(kind):
+    """The polar grid given as a general mesh (its face slots in Cell::Grans order, nodes
+    as each cell sees them): the mesh walk traces the polar walk's atoms, which the oracle
+    pins atom for atom -- the same counts, the tallies equal up to the order of the atomic
+    sums and the last bits of face crossings solved from the other end of the face."""
+    from gelios_2d_amd.mesh import mesh_from_polar
+    g = polar_grid(8, 6, 1.0, 10.0)
+    prim = _plasma(g.ni * g.nj, kind)
+    out_p, st_p = mc.run_mc(_ctx(g, prim), _p(20000, 5))
+    s = Setka.from_mesh(mesh_from_polar(g), reference_params())
+    s.upload(prim)
+    out_m, st_m = mc.run_mc(s, _p(20000, 5))
+    print(kind, st_m, st_p)
+    for k in ("primaries_in", "atoms", "segments", "lost", "generations"):
+        assert st_m[k] == st_p[k], k
+    assert st_m["lost"] == 0 and st_m["dropped"] == 0
+    scale = np.abs(out_p).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(out_m - out_p) / scale
+    assert err.max() < 1e-9, (np.argmax(err.max(axis=1)), err.max())
+
+
+def _shells(m, vol):
+    cx, cy = m.cell_centers()
+    r = np.hypot(cx, cy)
+    edges = [0.0, 0.5, 1.0, 2.0, 3.0, 4.0, 5.0, 7.0]
+    return [(r >= lo) & (r < hi) for lo, hi in zip(edges[:-1], edges[1:])]
+
+
+@pytest.mark.parametrize("kind,n,tol", [("vacuum", 16_000_000, 0.01), ("equilibrium", 4_000_000, 0.02)])
+def test_mesh_tracer_keeps_the_inflow_on_the_block_grid(kind, n, tol):
+    """Liouville on the reference's own grid (600 cells: the central cells behind the inner
+    sphere, the fitted-surface rows, the cylinder and back plane): in vacuum, and with
+    exchange against a plasma that moves and is as hot as the inflow (secondaries drawn
+    from the same Maxwellian), the hydrogen keeps the inflow's density, velocity and
+    temperature in every shell of cells."""
+    s, m = _block_ctx(lambda nc: _plasma(nc, kind))
+    out, st = mc.run_mc(s, _p(n, 3))
+    print(kind, st)
+    assert st["lost"] == 0 and st["dropped"] == 0
+    if kind == "vacuum":
+        assert st["atoms"] == st["primaries_in"]
+    else:
+        assert st["generations"] > 3
+    vol = _mesh_volumes(m)
+    nn, u, v, T = out[25], out[26], out[27], out[28]
+    assert abs(np.average(nn, weights=vol) - 1.0) < tol / 4
+    for sh in _shells(m, vol):
+        w = vol[sh]
+        means = [np.average(f[sh], weights=w) for f in (nn, u, v, T)]
+        print(f"shell {sh.sum():4d} cells, volume {w.sum():9.3f}: n {means[0]:.4f} u {means[1]:.4f} "
+              f"v {means[2]:+.4f} T {means[3]:.4f}", flush=True)
+        assert abs(means[0] - 1.0) < tol
+        assert abs(means[1] - V_INF) < tol
+        assert abs(means[2]) < tol
+        assert abs(means[3] - 1.0) < 1.5 * tol
+
+
+def test_mesh_mc_feeds_the_block_grid_sweep():
+    """The Monte Carlo on the block grid's plasma installs its source inputs in a
+    GEL_SRC_MK1 mesh context: stepping it equals stepping a twin given the same fields."""
+    a, m = _block_ctx(source_mode=GEL_SRC_MK1)
+    a.set_sources(np.ones((19, m.ncells)))
+    out, st = mc.run_mc(a, _p(400_000, 9))
+    print(st)
+    assert st["lost"] == 0 and st["dropped"] == 0
+    assert np.all(np.isfinite(out))
+    assert np.all((out[16:19] >= 0.33) & (out[16:19] <= 3.0))
+    b, _ = _block_ctx(source_mode=GEL_SRC_MK1)
+    b.set_sources(out[:19])
+    a.Go_5_komponent__MK2(20)
+    b.Go_5_komponent__MK2(20)
+    ra = a.download()
+    assert np.all(np.isfinite(ra))
+    np.testing.assert_array_equal(ra, b.download())
