@@ -48,6 +48,8 @@ Workloads
   mc-200x100       the hydrogen Monte Carlo (gel_mc_run) on the hllc-200x100 heliosphere's
                    plasma: a step is one run of 4e6 primary atoms with their secondaries;
                    the line's value is cell segments flown per second (its own metric)
+  mc-block-731     the same on the reference's own block grid and its checkpoint's plasma
+                   (the atoms walking each cell's face slots)
 """
 from __future__ import annotations
 
@@ -468,12 +470,35 @@ def cpu_baseline_workload(w, budget_s=12.0):
 MC_BYTES_PER_SEGMENT = 40 + 64 + 112
 
 
+def block_731_workload():
+    """The reference's block grid Setka(6,3,4,8,24,6,8,8) and its state, read from its
+    checkpoint (tests/golden/sweep_block.npz)."""
+    from gelios_2d_amd import configs
+    from gelios_2d_amd.checkpoint import read_alpha
+    from gelios_2d_amd.mesh import mesh_from_alpha, state_from_alpha
+    from gelios_2d_amd.params import reference_params
+    import tempfile
+    import numpy as np
+    d = np.load(os.path.join(ROOT, "tests", "golden", "sweep_block.npz"))
+    with tempfile.NamedTemporaryFile("wb", suffix=".txt") as f:
+        f.write(d["b_ref_text"].tobytes())
+        f.flush()
+        alpha = read_alpha(f.name)
+    m = mesh_from_alpha(alpha)
+    w = configs.Workload("reference-block-grid-731-cells-hll-fp64", m, reference_params(), state_from_alpha(alpha),
+                         200, None)
+    w.mesh = m
+    return w
+
+
 def run_mc_workload(w, K, W, no_cpu, n_primary=4_000_000):
-    """K timed runs of the Monte Carlo after W warm-up runs of a tenth of the atoms."""
+    """K timed runs of the Monte Carlo after W warm-up runs of a tenth of the atoms; on a
+    general mesh when the workload carries one."""
     from gelios_2d_amd.mc import run_mc
     from gelios_2d_amd.params import mc_reference_params
     from gelios_2d_amd.setka import Setka
-    s = Setka(w.grid, w.params)
+    mesh = getattr(w, "mesh", None)
+    s = Setka.from_mesh(mesh, w.params) if mesh is not None else Setka(w.grid, w.params)
     s.upload(w.prim0)
     for k in range(max(W, 1)):
         run_mc(s, mc_reference_params(n_primary // 10, seed=100 + k))
@@ -506,8 +531,11 @@ def run_mc_workload(w, K, W, no_cpu, n_primary=4_000_000):
         n_cpu, elapsed, segs_cpu = 20000, 0.0, 0
         while elapsed < 8.0:
             t = time.perf_counter()
-            _, st = ora.mc_run(w.grid, w.prim0, ora.mc_population(w.grid.cell_type, w.grid.ncells),
-                               mc_reference_params(n_cpu, seed=7))
+            pop = ora.mc_population(w.grid.cell_type, w.grid.ncells)
+            if mesh is not None:
+                _, st = ora.mc_run_mesh(mesh, w.prim0, pop, mc_reference_params(n_cpu, seed=7))
+            else:
+                _, st = ora.mc_run(w.grid, w.prim0, pop, mc_reference_params(n_cpu, seed=7))
             elapsed = time.perf_counter() - t
             segs_cpu = st["segments"]
             if elapsed < 8.0:
@@ -643,6 +671,10 @@ def main():
         r = run_mc_workload(configs.hllc_gas_200x100(), args.steps if args.steps is not None else 3, W, args.no_cpu)
         print(json.dumps(r), flush=True)
         return
+    elif wl == "mc-block-731":
+        r = run_mc_workload(block_731_workload(), args.steps if args.steps is not None else 3, W, args.no_cpu)
+        print(json.dumps(r), flush=True)
+        return
     else:
         if wl == "hllc-200x100":
             w = configs.hllc_gas_200x100()
@@ -659,20 +691,7 @@ def main():
             if wl == "mesh-800x400":
                 w.mesh = mesh_from_polar(w.grid)
         elif wl == "block-731":
-            from gelios_2d_amd.checkpoint import read_alpha
-            from gelios_2d_amd.mesh import mesh_from_alpha, state_from_alpha
-            from gelios_2d_amd.params import reference_params
-            import tempfile
-            import numpy as np
-            d = np.load(os.path.join(ROOT, "tests", "golden", "sweep_block.npz"))
-            with tempfile.NamedTemporaryFile("wb", suffix=".txt") as f:
-                f.write(d["b_ref_text"].tobytes())
-                f.flush()
-                alpha = read_alpha(f.name)
-            m = mesh_from_alpha(alpha)
-            w = configs.Workload("reference-block-grid-731-cells-hll-fp64", m, reference_params(),
-                                 state_from_alpha(alpha), 200, None)
-            w.mesh = m
+            w = block_731_workload()
         elif wl == "block-731-tracked":
             import numpy as np
             from gelios_2d_amd.mesh import MeshRails, mesh_from_slots

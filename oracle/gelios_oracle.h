@@ -150,6 +150,13 @@ double ora_mc_inflow(double v_inf);
  * cell; out = GEL_MC_NFIELDS x ni*nj as gel_mc_run writes it. */
 int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* prim, const int8_t* pop,
                const gel_mc_params* prm, double* out, gel_mc_stats* st);
+/* The same on a general mesh (the reference's own grid), the atoms walking each cell's
+ * face slots (Setka.cpp:12893-12960): coff/corners the contours, soff/sA/sB the slots in
+ * Cell::Grans order, scode each slot's way on (the next cell; -1 out of the grid, where
+ * primaries enter; -2 across the inner sphere; -3 an Axis slot, skipped). */
+int ora_mc_run_mesh(int nc, int nn, const double* x, const double* y, const int* coff, const int* corners,
+                    const int* soff, const int* sA, const int* sB, const int* scode, const double* prim,
+                    const int8_t* pop, const gel_mc_params* prm, double* out, gel_mc_stats* st);
 
 #ifdef __cplusplus
 }

@@ -337,6 +337,12 @@ typedef struct {
     long long nstack, cap;
     gel_mc_stats* st;
     double wmin;
+    /* a general mesh: corners and face slots per cell, each slot's way on (the next
+     * cell, -1 out of the grid, -2 across the inner sphere, -3 an Axis slot, skipped),
+     * the boundary lists as (A, B, cell) triples (outer: entry; inner: inner sphere) */
+    int mesh;
+    const int *coff, *corners, *soff, *sA, *sB, *scode;
+    int *bout, *bin, nbout, nbin;
 } mc_run;
 
 static inline int mc_nd(const mc_run* R, int q, int c) { return q * (R->nj + 1) + c; }
@@ -378,6 +384,34 @@ static double mc_row_hit(const mc_run* R, int q, const double p[3], const double
     return best;
 }
 
+/* mesh cell c: the nearest crossing over its face slots in Cell::Grans order, Axis
+ * slots skipped (Setka.cpp:12893-12960); *slot = the slot crossed */
+static double mc_mesh_exit(const mc_run* R, int c, const double p[3], const double v[3], double tmin, int* slot)
+{
+    double best = -1.0;
+    *slot = -1;
+    for (int s = R->soff[c]; s < R->soff[c + 1]; s++) {
+        if (R->scode[s] == -3) continue;
+        const int A = R->sA[s], B = R->sB[s];
+        const double t = mc_hit(p, v, R->x[A], R->y[A], R->x[B], R->y[B], tmin);
+        if (t > 0.0 && (best < 0.0 || t < best)) { best = t; *slot = s; }
+    }
+    return best;
+}
+
+/* first crossing with n (A, B, cell) faces; *jj = the entry, -1 if none */
+static double mc_list_hit(const mc_run* R, const int* L, int n, const double p[3], const double v[3], double tmin,
+                          int* jj)
+{
+    double best = -1.0;
+    *jj = -1;
+    for (int c = 0; c < n; c++) {
+        const double t = mc_hit(p, v, R->x[L[3 * c]], R->y[L[3 * c]], R->x[L[3 * c + 1]], R->y[L[3 * c + 1]], tmin);
+        if (t > 0.0 && (best < 0.0 || t < best)) { best = t; *jj = c; }
+    }
+    return best;
+}
+
 static void mc_push(mc_run* R, const mc_atom* a)
 {
     if (R->nstack == R->cap) {
@@ -412,15 +446,18 @@ static void mc_trace(mc_run* R, mc_atom a)
     double* T = R->tal;
     R->st->atoms++;
     for (int seg = 0; seg < P->max_segments; seg++) {
-        const size_t k = (size_t)a.i * R->nj + a.j;
+        const size_t k = R->mesh ? (size_t)a.i : (size_t)a.i * R->nj + a.j;
         const double vn = sqrt(a.v[0] * a.v[0] + a.v[1] * a.v[1] + a.v[2] * a.v[2]);
         const double* X = R->x;
         const double* Y = R->y;
-        const int n00 = mc_nd(R, a.i, a.j), n11 = mc_nd(R, a.i + 1, a.j + 1);
+        /* a diagonal of the cell: corners 0 and 2 */
+        const int n00 = R->mesh ? R->corners[R->coff[k]] : mc_nd(R, a.i, a.j);
+        const int n11 = R->mesh ? R->corners[R->coff[k] + 2] : mc_nd(R, a.i + 1, a.j + 1);
         const double L = sqrt((X[n11] - X[n00]) * (X[n11] - X[n00]) + (Y[n11] - Y[n00]) * (Y[n11] - Y[n00]));
         const double tmin = 1e-9 * L / vn;
         int face;
-        const double t_exit = mc_exit(R, a.i, a.j, a.p, a.v, tmin, &face);
+        const double t_exit = R->mesh ? mc_mesh_exit(R, a.i, a.p, a.v, tmin, &face)
+                                      : mc_exit(R, a.i, a.j, a.p, a.v, tmin, &face);
         if (face < 0) { R->st->lost++; return; }
         R->st->segments++;
         const double ro = R->prim[k], pr = R->prim[nc + k];
@@ -533,7 +570,20 @@ static void mc_trace(mc_run* R, mc_atom a)
             continue;
         }
         for (int d = 0; d < 3; d++) a.p[d] += t_exit * a.v[d];
-        if (face == 1) {
+        if (R->mesh) {
+            const int to = R->scode[face];
+            if (to >= 0) {
+                a.i = to;
+            } else if (to == -2) {                     /* through the inner sphere */
+                int jj;
+                const double th = mc_list_hit(R, R->bin, R->nbin, a.p, a.v, tmin, &jj);
+                if (jj < 0) { R->st->lost++; return; }
+                for (int d = 0; d < 3; d++) a.p[d] += th * a.v[d];
+                a.i = R->bin[3 * jj + 2];
+            } else {
+                return;                                /* out of the grid */
+            }
+        } else if (face == 1) {
             if (a.i == R->ni - 1) return;              /* out of the grid */
             a.i++;
         } else if (face == 0) {
@@ -583,12 +633,12 @@ static int mc_primary(mc_run* R, long long kk, double Rs, mc_atom* a)
     double p[3];
     for (int d = 0; d < 3; d++) p[d] = c * e1[d] + s * e2[d] - back * e[d];
     int jj;
-    const double t = mc_row_hit(R, R->ni, p, v, 0.0, &jj);
+    const double t = R->mesh ? mc_list_hit(R, R->bout, R->nbout, p, v, 0.0, &jj) : mc_row_hit(R, R->ni, p, v, 0.0, &jj);
     if (jj < 0) return 0;
     for (int d = 0; d < 3; d++) { a->p[d] = p[d] + t * v[d]; a->v[d] = v[d]; }
     a->mu = 1.0;
-    a->i = R->ni - 1;
-    a->j = jj;
+    a->i = R->mesh ? R->bout[3 * jj + 2] : R->ni - 1;
+    a->j = R->mesh ? 0 : jj;
     a->area = 3;
     return 1;
 }
@@ -599,6 +649,8 @@ double ora_mc_inflow(double v_inf)
     ora_mc_velosity(fabs(v_inf), 1.0, o);
     return o[0];
 }
+
+static void mc_run_all(mc_run* RR, int nn, double* out);
 
 int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* prim, const int8_t* pop,
                const gel_mc_params* prm, double* out, gel_mc_stats* st)
@@ -616,10 +668,57 @@ int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* p
     R.prm = prm;
     R.st = st;
     R.wmin = prm->w_min;
+    mc_run_all(&R, (ni + 1) * (nj + 1), out);
+    return 0;
+}
+
+/* The tracer on a general mesh: cell c's corners are corners[coff[c] .. coff[c + 1]),
+ * its face slots soff[c] .. soff[c + 1]) in Cell::Grans order with nodes sA, sB as the
+ * cell sees them and their way on scode (the next cell, -1 out of the grid, -2 across
+ * the inner sphere, -3 an Axis slot).  Primaries enter by the -1 slots. */
+int ora_mc_run_mesh(int nc, int nn, const double* x, const double* y, const int* coff, const int* corners,
+                    const int* soff, const int* sA, const int* sB, const int* scode, const double* prim,
+                    const int8_t* pop, const gel_mc_params* prm, double* out, gel_mc_stats* st)
+{
+    mc_run R;
+    memset(&R, 0, sizeof(R));
+    memset(st, 0, sizeof(*st));
+    R.x = x;
+    R.y = y;
+    R.prim = prim;
+    R.pop = pop;
+    R.nc = (size_t)nc;
+    R.prm = prm;
+    R.st = st;
+    R.wmin = prm->w_min;
+    R.mesh = 1;
+    R.coff = coff; R.corners = corners; R.soff = soff; R.sA = sA; R.sB = sB; R.scode = scode;
+    const int ns = soff[nc];
+    R.bout = (int*)malloc((3 * (size_t)ns + 1) * sizeof(int));
+    R.bin = (int*)malloc((3 * (size_t)ns + 1) * sizeof(int));
+    for (int c = 0; c < nc; c++)
+        for (int s = soff[c]; s < soff[c + 1]; s++) {
+            if (scode[s] != -1 && scode[s] != -2) continue;
+            int* L = scode[s] == -1 ? R.bout + 3 * R.nbout++ : R.bin + 3 * R.nbin++;
+            L[0] = sA[s]; L[1] = sB[s]; L[2] = c;
+        }
+    mc_run_all(&R, nn, out);
+    free(R.bout);
+    free(R.bin);
+    return 0;
+}
+
+static void mc_run_all(mc_run* RR, int nn, double* out)
+{
+    mc_run R = *RR;
+    const gel_mc_params* prm = R.prm;
+    gel_mc_stats* st = R.st;
+    const double *x = R.x, *y = R.y, *prim = R.prim;
+    const int nj = R.nj;
     const size_t nc = R.nc;
     R.tal = (double*)calloc(22 * nc, sizeof(double));
     double Rs = 0.0;
-    for (int k = 0; k < (ni + 1) * (nj + 1); k++) Rs = fmax(Rs, sqrt(x[k] * x[k] + y[k] * y[k]));
+    for (int k = 0; k < nn; k++) Rs = fmax(Rs, sqrt(x[k] * x[k] + y[k] * y[k]));
     Rs *= 1.0 + 1e-12;
     for (long long kk = 0; kk < prm->n_primary; kk++) {
         mc_atom a;
@@ -635,9 +734,25 @@ int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* p
     const double Phi = MC_PI * Rs * Rs * ora_mc_inflow(prm->v_inf);
     double* T = R.tal;
     for (size_t k = 0; k < nc; k++) {
-        const int i = (int)(k / nj), j = (int)(k % nj);
-        const int q[4] = { i * (nj + 1) + j, (i + 1) * (nj + 1) + j, (i + 1) * (nj + 1) + j + 1, i * (nj + 1) + j + 1 };
-        const double no = (double)prm->n_primary * ora_cell_volume_rotate(x, y, q, 360.0);
+        double vol;
+        if (R.mesh) {   /* Cell::Get_Volume_rotate(360) (Cell.cpp:188-206) over the contour */
+            const int* q = R.corners + R.coff[k];
+            const int n = R.coff[k + 1] - R.coff[k];
+            double A = 0.0, G = 0.0;
+            for (int m = 0; m < n; m++) {
+                const int a = q[m], b = q[m + 1 == n ? 0 : m + 1];
+                const double cr = x[a] * y[b] - y[a] * x[b];
+                A = A + cr;
+                G = G + (y[a] + y[b]) * cr;
+            }
+            A = A * 0.5;
+            vol = fabs(A) * (1.0 / (6.0 * A)) * G * MC_PI * 360.0 / 180.0;
+        } else {
+            const int i = (int)(k / nj), j = (int)(k % nj);
+            const int q[4] = { i * (nj + 1) + j, (i + 1) * (nj + 1) + j, (i + 1) * (nj + 1) + j + 1, i * (nj + 1) + j + 1 };
+            vol = ora_cell_volume_rotate(x, y, q, 360.0);
+        }
+        const double no = (double)prm->n_primary * vol;
         double Hn[4], Hu[4], Hv[4], HT[4];
         double Fn = 0.0, Fu = 0.0, Fv = 0.0, FT = 0.0;
         for (int h = 0; h < 4; h++) {
@@ -715,5 +830,4 @@ int ora_mc_run(int ni, int nj, const double* x, const double* y, const double* p
     }
     free(R.tal);
     free(R.stack);
-    return 0;
 }

@@ -114,6 +114,10 @@ def lib():
         L.ora_mc_run.restype = ctypes.c_int
         L.ora_mc_run.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int8),
                                  ctypes.POINTER(GelMcParams), _dp, ctypes.POINTER(GelMcStats)]
+        L.ora_mc_run_mesh.restype = ctypes.c_int
+        L.ora_mc_run_mesh.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _ip, _ip, _ip, _ip, _ip, _ip, _dp,
+                                      ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(GelMcParams), _dp,
+                                      ctypes.POINTER(GelMcStats)]
         _lib = L
     return _lib
 
@@ -364,6 +368,46 @@ def mc_run(g: PolarGrid, prim, pop, prm: GelMcParams):
     st = GelMcStats()
     lib().ora_mc_run(g.ni, g.nj, _d(x), _d(y), _d(prim), pop.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
                      ctypes.byref(prm), _d(out), ctypes.byref(st))
+    return out, st.as_dict()
+
+
+def mc_mesh_slots(mesh):
+    """The face slots of a general mesh (gelios_2d_amd.mesh.Mesh) as the tracer walks them,
+    compacted in Cell::Grans order: (soff, sA, sB, scode).  scode is the way on: Sosed
+    where the slot has one (any type: the block grid's Inner_sphere faces lead to the
+    central cells), -3 for Axis (the reference's tracer skips them, Setka.cpp:12897),
+    -2 for an Inner_sphere slot with no Sosed (crossed in a straight line), -1 for any
+    other boundary (the atom leaves; primaries enter there)."""
+    nc = len(mesh.nface)
+    maxf = len(mesh.fa) // nc
+    keep = (np.arange(maxf)[None, :] < np.asarray(mesh.nface)[:, None]).ravel()
+    sA = np.ascontiguousarray(np.asarray(mesh.fa)[keep], np.int32)
+    sB = np.ascontiguousarray(np.asarray(mesh.fb)[keep], np.int32)
+    t = np.asarray(mesh.ftype)[keep]
+    n = np.asarray(mesh.fnbr)[keep]
+    code = np.where(n >= 0, n, np.where(t == 1, -2, -1))
+    code = np.where(t == 4, -3, code).astype(np.int32)
+    soff = np.zeros(nc + 1, np.int32)
+    soff[1:] = np.cumsum(mesh.nface)
+    return soff, sA, sB, np.ascontiguousarray(code)
+
+
+def mc_run_mesh(mesh, prim, pop, prm: GelMcParams):
+    """The tracer on a general mesh; returns (out[GEL_MC_NFIELDS][ncells], stats dict)."""
+    nc = len(mesh.nface)
+    soff, sA, sB, code = mc_mesh_slots(mesh)
+    x = np.ascontiguousarray(mesh.x, np.float64)
+    y = np.ascontiguousarray(mesh.y, np.float64)
+    coff = np.ascontiguousarray(mesh.coff, np.int32)
+    corners = np.ascontiguousarray(mesh.corners, np.int32)
+    prim = np.ascontiguousarray(prim, np.float64)
+    pop = np.ascontiguousarray(pop, np.int8)
+    out = np.zeros((GEL_MC_NFIELDS, nc))
+    st = GelMcStats()
+    ip = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))   # noqa: E731
+    lib().ora_mc_run_mesh(nc, len(x), _d(x), _d(y), ip(coff), ip(corners), ip(soff), ip(sA), ip(sB), ip(code),
+                          _d(prim), pop.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)), ctypes.byref(prm), _d(out),
+                          ctypes.byref(st))
     return out, st.as_dict()
 
 

@@ -199,6 +199,23 @@ def test_mesh_tracer_equals_polar_tracer(kind):
     assert err.max() < 1e-9, (np.argmax(err.max(axis=1)), err.max())
 
 
+@pytest.mark.parametrize("kind", ["hot", "equilibrium"])
+def test_mesh_tracer_matches_oracle_on_the_block_grid(kind):
+    """The reference's own grid: the device walks the oracle's atoms one for one through
+    the block grid's cells (the central cells behind the inner sphere included) -- equal
+    counts, tallies within 1e-9."""
+    s, m = _block_ctx(lambda nc: _plasma(nc, kind))
+    out_d, st_d = mc.run_mc(s, _p(20000, 5))
+    out_o, st_o = ora.mc_run_mesh(m, _plasma(m.ncells, kind), ora.mc_population(m.cell_type, m.ncells), _p(20000, 5))
+    print(kind, st_d, st_o)
+    for k in ("primaries_in", "atoms", "segments", "lost"):
+        assert st_d[k] == st_o[k], k
+    assert st_d["lost"] == 0 and st_d["dropped"] == 0 and st_d["atoms"] > st_d["primaries_in"]
+    scale = np.abs(out_o).max(axis=1, keepdims=True) + 1e-300
+    err = np.abs(out_d - out_o) / scale
+    assert err.max() < 1e-9, (np.argmax(err.max(axis=1)), err.max())
+
+
 def _shells(m, vol):
     cx, cy = m.cell_centers()
     r = np.hypot(cx, cy)

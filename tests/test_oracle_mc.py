@@ -170,3 +170,76 @@ def test_rate_and_sampled_sources_agree():
     k = out[16:19]
     assert np.all((k >= 0.33) & (k <= 3.0))
     assert np.all(np.isfinite(out))
+
+
+# ------------------------------------------------- the walk on a general mesh ----
+
+def _hot(nc):
+    p = np.zeros((5, nc))
+    p[0], p[1], p[2], p[3] = 0.3, 0.6, 0.5, 0.2
+    return p
+
+
+def test_mesh_walk_equals_polar_walk():
+    """The polar grid given as a general mesh (face slots in Cell::Grans order, every face's
+    nodes as its cell sees them): the walk over each cell's slots, with Axis slots skipped,
+    a Sosed followed, the inner sphere crossed and the outer faces left, traces the same
+    atoms as the polar walk -- equal counts, the tallies to the last bits of crossings
+    solved from the other end of a face."""
+    from gelios_2d_amd.mesh import mesh_from_polar
+    g = polar_grid(8, 6, 1.0, 10.0)
+    m = mesh_from_polar(g)
+    prim = _hot(g.ncells)
+    pop = ora.mc_population(g.cell_type, g.ncells)
+    p = mc_reference_params(20000, seed=5)
+    p.max_segments = 100_000
+    out_p, st_p = ora.mc_run(g, prim, pop, p)
+    out_m, st_m = ora.mc_run_mesh(m, prim, pop, p)
+    for k in ("primaries_in", "atoms", "segments", "lost"):
+        assert st_m[k] == st_p[k], k
+    assert st_m["lost"] == 0 and st_m["atoms"] > 2 * st_m["primaries_in"]
+    scale = np.abs(out_p).max(axis=1, keepdims=True) + 1e-300
+    assert (np.abs(out_m - out_p) / scale).max() < 1e-11
+
+
+def test_mesh_slots_of_the_block_grid():
+    """The reference's block grid: its Inner_sphere faces have the central cells behind
+    them (followed, not crossed), the entry faces are the Extern, Input and Upper_wall
+    faces, and the Axis faces are skipped."""
+    from test_oracle_block import alpha, block
+    from gelios_2d_amd.mesh import mesh_from_alpha
+    m = mesh_from_alpha(alpha(block("b")))
+    soff, sA, sB, code = ora.mc_mesh_slots(m)
+    assert soff[-1] == len(code) == int(np.sum(m.nface))
+    keep = (np.arange(len(m.fa) // m.ncells)[None, :] < m.nface[:, None]).ravel()
+    t = m.ftype[keep]
+    assert np.all(code[t == 4] == -3)
+    assert np.all(code[t == 1] >= 0)
+    assert np.all(code[(t == 2) | (t == 3) | (t == 5)] == -1)
+    assert np.all(code[t == 0] >= 0) and not np.any(code == -2)
+
+
+def test_mesh_walk_keeps_the_inflow_on_the_block_grid():
+    """Liouville on the reference's own grid, in vacuum: every primary leaves, none is
+    lost, and the density, velocity and temperature of the inflow hold in the volume
+    mean and in the outer shells (2e5 primaries)."""
+    from test_oracle_block import alpha, block
+    from gelios_2d_amd.mesh import mesh_from_alpha
+    m = mesh_from_alpha(alpha(block("b")))
+    prim = np.zeros((5, m.ncells))
+    prim[0] = prim[1] = 1e-200
+    out, st = ora.mc_run_mesh(m, prim, ora.mc_population(m.cell_type, m.ncells), mc_reference_params(200_000, seed=3))
+    assert st["lost"] == 0 and st["atoms"] == st["primaries_in"] > 150_000
+    vol = np.zeros(m.ncells)
+    for c in range(m.ncells):
+        q = m.corners[m.coff[c]:m.coff[c + 1]]
+        px, py = m.x[q], m.y[q]
+        qx, qy = np.roll(px, -1), np.roll(py, -1)
+        vol[c] = abs(((py + qy) * (px * qy - py * qx)).sum()) / 6.0 * 2 * np.pi
+    cx, cy = m.cell_centers()
+    outer = np.hypot(cx, cy) > 2.0
+    n, u, T = out[25], out[26], out[28]
+    assert abs(np.average(n, weights=vol) - 1.0) < 0.01
+    assert abs(np.average(n[outer], weights=vol[outer]) - 1.0) < 0.01
+    assert abs(np.average(u[outer], weights=vol[outer]) - V_INF) < 0.02
+    assert abs(np.average(T[outer], weights=vol[outer]) - 1.0) < 0.02
