@@ -37,7 +37,7 @@ Workloads
   mesh-800x400     the gas heliosphere (HLLC, twenty rows of C_1 wind cells, FP64, 200
                    steps) given as a general mesh: the general-mesh kernels (face table,
                    per-face and per-cell launches); structured-800x400 runs the same on
-                   the structured marching kernel
+                   the structured marching kernel; mesh-800x400-fp32 the mesh in FP32 bulk
   block-731        the reference's own block grid (Setka(6,3,4,8,24,6,8,8), 731 cells, read
                    from its checkpoint in tests/golden/sweep_block.npz), HLL, the
                    reference's constants, 200 steps
@@ -684,11 +684,16 @@ def main():
             w = configs.large()
         elif wl == "plasma-4h-800x400":
             w = configs.multifluid_heliosphere(800, 400, 200, seed=9, name="plasma-4h-hllc-fp64-800x400")
-        elif wl in ("mesh-800x400", "structured-800x400"):
+        elif wl in ("mesh-800x400", "mesh-800x400-fp32", "structured-800x400"):
             from gelios_2d_amd.mesh import mesh_from_polar
-            w = configs.heliosphere(800, 400, 200, seed=1, flux=1, name=f"hllc-fp64-c1-800x400-{wl.split('-')[0]}")
+            from gelios_2d_amd.params import GEL_PREC_FP32_BULK
+            fp32 = wl.endswith("fp32")
+            w = configs.heliosphere(800, 400, 200, seed=1, flux=1,
+                                    name=f"hllc-{'fp32-bulk' if fp32 else 'fp64'}-c1-800x400-{wl.split('-')[0]}")
             w.grid.cell_type[: 20 * w.grid.nj] = 0                  # twenty rows of C_1 wind cells
-            if wl == "mesh-800x400":
+            if fp32:
+                w.params.precision = GEL_PREC_FP32_BULK
+            if wl.startswith("mesh"):
                 w.mesh = mesh_from_polar(w.grid)
         elif wl == "block-731":
             w = block_731_workload()
@@ -713,8 +718,9 @@ def main():
             raise SystemExit(f"unknown workload {wl}")
         K = args.steps if args.steps is not None else w.steps
         log(f"bench: {w.name}, {w.grid.ncells} cells, warmup {W}, steps {K}")
-        main_w, main_r = w, run_gpu(w, K, W, "fp64-plain")
-        precision, dtype = "fp64-plain", "f64"
+        fp32 = w.params.precision != 0
+        main_w, main_r = w, run_gpu(w, K, W, "fp32-bulk" if fp32 else "fp64-plain")
+        precision, dtype = ("fp32-bulk", "f32") if fp32 else ("fp64-plain", "f64")
         config = None
 
     line = {

@@ -275,8 +275,8 @@ int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* g, const gel_params* prm)
     for (int k = 0; k < g->n_cells; k++)
         if (g->corner_offset[k + 1] < g->corner_offset[k] || g->face_offset[k + 1] < g->face_offset[k])
             return fail(GEL_E_ARG, "offsets decrease");
-    if (prm->physics != GEL_PHYS_GAS || prm->precision != GEL_PREC_FP64)
-        return fail(GEL_E_UNSUPPORTED, "general meshes: gas physics in FP64");
+    if (prm->physics != GEL_PHYS_GAS || (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK))
+        return fail(GEL_E_UNSUPPORTED, "general meshes: gas physics, FP64 or FP32 bulk");
     if (prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC) return fail(GEL_E_UNSUPPORTED, "flux");
     if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1)
         return fail(GEL_E_UNSUPPORTED, "general meshes: no source or the MK1 source");
@@ -286,7 +286,7 @@ int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* g, const gel_params* prm)
     c->nc = (size_t)g->n_cells;
     c->nn = (size_t)g->n_nodes;
     c->nv = 5;
-    c->ts = 8;
+    c->ts = prm->precision == GEL_PREC_FP32_BULK ? 4 : 8;
     c->prm = *prm;
     c->has_types = true;
     auto bad = [&](hipError_t e, const char* what) {

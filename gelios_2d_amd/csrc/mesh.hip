@@ -140,6 +140,9 @@ __global__ void __launch_bounds__(256) mesh_geo_kernel(MeshDev m, const double* 
 
 // A stencil cell as the face functions of sweep_kernel.h see it; rp (the centroid
 // radius to the power 2 gamma) only where a C_1 side's radial reconstruction reads it.
+// TS: the state's storage type (FP32-bulk contexts store it in FP32; the cell's values
+// are widened, the reconstruction runs in FP64).
+template <class TS = double>
 __device__ __forceinline__ CellT<GasPhys, double> mesh_cell(const MeshDev& m, const SweepArgs& a, int c, bool rp)
 {
     CellT<GasPhys, double> k;
@@ -153,9 +156,9 @@ __device__ __forceinline__ CellT<GasPhys, double> mesh_cell(const MeshDev& m, co
         return k;
     }
     const size_t nc = m.nc;
-    const double* now = (const double*)a.now;
+    const TS* now = (const TS*)a.now;
 #pragma unroll
-    for (int v = 0; v < 5; v++) k.s[v] = now[v * nc + c];
+    for (int v = 0; v < 5; v++) k.s[v] = (double)now[v * nc + c];
     k.cx = m.cgeo[c];
     k.cy = m.cgeo[nc + c];
     k.type = m.ctype[c];
@@ -186,8 +189,11 @@ __device__ __forceinline__ void mesh_side(const CellT<GasPhys, double>& M, const
 // Face f: both sides' values, the flux, flux x area, the wave speed and the face
 // centre into fout; returns the numeric-failure count.  WAVE: the whole wavefront
 // evaluates this one face (a fitted-surface face, whose exact solver's bisection then
-// spreads its midpoints over the lanes), lane 0 stores.
-template <int FX, bool WAVE>
+// spreads its midpoints over the lanes), lane 0 stores.  TS = float (FP32 bulk): the
+// state read from FP32 storage and, on faces other than the fitted-surface and
+// nul_potok ones, the Riemann solver in FP32 on the FP64 area-weighted normal and
+// swept volume of face_geom_a (supersonic faces the FP64 upwind flux, as on polar grids).
+template <int FX, bool WAVE, class TS = double>
 __device__ __forceinline__ int mesh_face(const MeshDev& m, const SweepArgs& a, int f)
 {
     int b = 0;
@@ -197,24 +203,33 @@ __device__ __forceinline__ int mesh_face(const MeshDev& m, const SweepArgs& a, i
     const int A = m.fA[f], B = m.fB[f];
     const Face g = face_geom<true, double>(a.x[A], a.y[A], a.x[B], a.y[B], a.x2[A], a.y2[A], a.x2[B], a.y2[B], dt);
     const bool c1 = m.ctype[M] == GEL_C_1 || (N >= 0 && m.ctype[N] == GEL_C_1);
-    const CellT<GasPhys, double> Mc = mesh_cell(m, a, M, c1), Dc = mesh_cell(m, a, m.fdown[f], c1);
+    const CellT<GasPhys, double> Mc = mesh_cell<TS>(m, a, M, c1), Dc = mesh_cell<TS>(m, a, m.fdown[f], c1);
     const bool god = (m.fsurf[f] & 1) != 0, np = (m.fsurf[f] & 2) != 0;
     double L[5], R[5], P[5];
     if (t == GEL_F_USUALY) {
-        const CellT<GasPhys, double> Nc = mesh_cell(m, a, N, c1), Uc = mesh_cell(m, a, m.fup[f], c1);
+        const CellT<GasPhys, double> Nc = mesh_cell<TS>(m, a, N, c1), Uc = mesh_cell<TS>(m, a, m.fup[f], c1);
         mesh_side(Mc, Nc, Dc, m.caxis[M] != 0, god, g, gam, L, b);
         mesh_side(Nc, Mc, Uc, m.caxis[N] != 0, god, g, gam, R, b);
     } else {
         boundary_states<GasPhys, double, double, true>(a, t, Mc, Dc, g, L, R);
     }
-    double uu;
-    if (god) uu = surface_flux<FX == GEL_FLUX_HLLC, WAVE>(L, R, g, gam, P, b);
-    else if (FX == GEL_FLUX_HLLC && np) uu = nul_potok_flux(L, R, g, gam, P);
-    else uu = face_flux<GasPhys, double, FX>(L, R, g, gam, P);
+    double uu, S = g.S;
+    if (god) {
+        uu = surface_flux<FX == GEL_FLUX_HLLC, WAVE>(L, R, g, gam, P, b);
+    } else if (FX == GEL_FLUX_HLLC && np) {
+        uu = nul_potok_flux(L, R, g, gam, P);
+    } else if constexpr (sizeof(TS) == 4) {
+        const Face ga = face_geom_a<true>(a.x[A], a.y[A], a.x[B], a.y[B], a.x2[A], a.y2[A], a.x2[B], a.y2[B],
+                                          1.0 / dt);
+        uu = face_flux<GasPhys, float, FX>(L, R, ga, gam, P);   // already times the area
+        S = 1.0;
+    } else {
+        uu = face_flux<GasPhys, double, FX>(L, R, g, gam, P);
+    }
     if (WAVE && (threadIdx.x & (kWave - 1)) != 0) return 0;
     const size_t nf = m.nf;
 #pragma unroll
-    for (int k = 0; k < 5; k++) m.fout[k * nf + f] = P[k] * g.S;
+    for (int k = 0; k < 5; k++) m.fout[k * nf + f] = P[k] * S;
     m.fout[5 * nf + f] = uu;
     m.fout[6 * nf + f] = g.fx;
     m.fout[7 * nf + f] = g.fy;
@@ -224,23 +239,23 @@ __device__ __forceinline__ int mesh_face(const MeshDev& m, const SweepArgs& a, i
 // Work-groups [0, nfb) take 256 faces each, one per lane, skipping the fitted-surface
 // faces; the work-groups after them take four fitted-surface faces each, one per
 // wavefront.  nfb is a multiple of eight (xcd_item).
-template <int FX>
+template <int FX, class TS>
 __global__ void __launch_bounds__(256) mesh_face_kernel(MeshDev m, SweepArgs a, int nfb)
 {
     int b = 0;
     if ((int)blockIdx.x < nfb) {
         const int f = xcd_item(blockIdx.x, nfb) * blockDim.x + threadIdx.x;
-        if (f < m.nf && !(m.fsurf[f] & 1)) b = mesh_face<FX, false>(m, a, f);
+        if (f < m.nf && !(m.fsurf[f] & 1)) b = mesh_face<FX, false, TS>(m, a, f);
     } else {
         const int k = ((int)blockIdx.x - nfb) * 4 + (int)(threadIdx.x / kWave);
-        if (k < m.ng) b = mesh_face<FX, true>(m, a, m.gface[k]);
+        if (k < m.ng) b = mesh_face<FX, true, TS>(m, a, m.gface[k]);
     }
     const int s_bad = wave_sum(b);
     if ((threadIdx.x & (kWave - 1)) == 0 && s_bad)
         __hip_atomic_fetch_add(gp(a.counters) + 2, (unsigned long long)s_bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int SRC>
+template <int SRC, class TS>
 __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
 {
     const int c = xcd_item(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -269,19 +284,19 @@ __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
                 time = mn(time, cfl_time<double>(a.prm.cfl, cx, cy, g, m.fout[5 * nf + f]));
             }
             acc.tmin = fmin(acc.tmin, time);
-            const double* now = (const double*)a.now;
-            double* nxt = (double*)a.next;
+            const TS* now = (const TS*)a.now;
+            TS* nxt = (TS*)a.next;
             double s[5];
 #pragma unroll
-            for (int v = 0; v < 5; v++) s[v] = now[v * nc + c];
+            for (int v = 0; v < 5; v++) s[v] = (double)now[v * nc + c];
             if (m.ctype[c] == GEL_C_CENTR) {                // Setka.cpp:10461-10468
 #pragma unroll
-                for (int v = 0; v < 5; v++) nxt[v * nc + c] = s[v];
+                for (int v = 0; v < 5; v++) nxt[v * nc + c] = now[v * nc + c];
             } else {
                 double q1 = 0.0, q2 = 0.0, q3 = 0.0;
                 if constexpr (SRC == GEL_SRC_MK1) {
-                    const double* sp = (const double*)a.src + c;
-                    auto fk = [&](int k) { return sp[(size_t)k * nc]; };
+                    const TS* sp = (const TS*)a.src + c;
+                    auto fk = [&](int k) { return (double)sp[(size_t)k * nc]; };
                     source_mk1<decltype(fk), double>(a.prm, fk, s[2], s[3], s[0], s[1], q1, q2, q3);
                 }
                 const double dt = dt_read(a);
@@ -290,7 +305,7 @@ __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
                                             m.cgeo[3 * nc + c], dt, a.prm.gamma, 0.0, q1, q2, q3, 0.0, false, o,
                                             acc.n_rho, acc.n_p);
 #pragma unroll
-                for (int v = 0; v < 5; v++) nxt[v * nc + c] = o[v];
+                for (int v = 0; v < 5; v++) nxt[v * nc + c] = (TS)o[v];
             }
         }
     }
@@ -1067,9 +1082,19 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
     }
     const int nfb = (int)groups(d.nf);
     const dim3 gf((unsigned)(nfb + (d.ng + 3) / 4)), gc(groups(d.nc)), blk(256);
-    if (c->prm.flux == GEL_FLUX_HLLC) hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLLC>, gf, blk, 0, s, d, a, nfb);
-    else hipLaunchKernelGGL(mesh_face_kernel<GEL_FLUX_HLL>, gf, blk, 0, s, d, a, nfb);
-    if (c->prm.source_mode == GEL_SRC_MK1) hipLaunchKernelGGL(mesh_cell_kernel<GEL_SRC_MK1>, gc, blk, 0, s, d, a);
-    else hipLaunchKernelGGL(mesh_cell_kernel<GEL_SRC_NONE>, gc, blk, 0, s, d, a);
+    const bool hllc = c->prm.flux == GEL_FLUX_HLLC, mk1 = c->prm.source_mode == GEL_SRC_MK1;
+    using FaceK = void (*)(MeshDev, SweepArgs, int);
+    using CellK = void (*)(MeshDev, SweepArgs);
+    FaceK fk;
+    CellK ck;
+    if (c->ts == 4) {   // FP32 bulk
+        fk = hllc ? mesh_face_kernel<GEL_FLUX_HLLC, float> : mesh_face_kernel<GEL_FLUX_HLL, float>;
+        ck = mk1 ? mesh_cell_kernel<GEL_SRC_MK1, float> : mesh_cell_kernel<GEL_SRC_NONE, float>;
+    } else {
+        fk = hllc ? mesh_face_kernel<GEL_FLUX_HLLC, double> : mesh_face_kernel<GEL_FLUX_HLL, double>;
+        ck = mk1 ? mesh_cell_kernel<GEL_SRC_MK1, double> : mesh_cell_kernel<GEL_SRC_NONE, double>;
+    }
+    hipLaunchKernelGGL(fk, gf, blk, 0, s, d, a, nfb);
+    hipLaunchKernelGGL(ck, gc, blk, 0, s, d, a);
     return hipGetLastError();
 }

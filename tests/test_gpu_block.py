@@ -108,6 +108,53 @@ def test_mesh_path_equals_structured_sweep(moving):
     b.close()
 
 
+def _fp32_vs_fp64(m, prm, prim, steps):
+    from gelios_2d_amd.params import GEL_PREC_FP32_BULK
+    from gelios_2d_amd.setka import Setka
+    from gelios_2d_amd.state import l1_rel, linf_rel
+    p32 = prm.copy()
+    p32.precision = GEL_PREC_FP32_BULK
+    a, b = Setka.from_mesh(m, prm), Setka.from_mesh(m, p32)
+    for s in (a, b):
+        s.upload(prim)
+        s.Go_5_komponent__MK2(steps)
+    sa, sb = a.sync(), b.sync()
+    ra, rb = a.download(), b.download()
+    a.close()
+    b.close()
+    assert np.all(np.isfinite(rb))
+    l1 = [l1_rel(rb, ra, k) for k in (0, 1)]
+    li = [linf_rel(rb, ra, k) for k in (0, 1)]
+    print(f"FP32 bulk vs FP64 on the mesh, {steps} steps: L1 rho {l1[0]:.2e} p {l1[1]:.2e}; "
+          f"Linf {li[0]:.2e} {li[1]:.2e}; dt {sb.dt_next:.9e} / {sa.dt_next:.9e}", flush=True)
+    return l1, sa, sb
+
+
+def test_mesh_fp32_bulk_heliosphere():
+    """FP32 bulk on a general mesh (the 200 x 100 heliosphere with C_1 wind rows, HLLC):
+    FP32 state and Riemann solver, FP64 geometry, accumulation and update, within the
+    FP32-bulk tolerance of the FP64 mesh sweep (L1 <= 1e-4 per variable)."""
+    from gelios_2d_amd.configs import heliosphere
+    from gelios_2d_amd.mesh import mesh_from_polar
+    w = heliosphere(200, 100, 200, seed=1, flux=GEL_FLUX_HLLC)
+    w.grid.cell_type[: 3 * w.grid.nj] = 0
+    l1, sa, sb = _fp32_vs_fp64(mesh_from_polar(w.grid), w.params, w.prim0, w.steps)
+    assert max(l1) <= 1e-4
+    assert abs(sb.dt_next / sa.dt_next - 1.0) < 1e-4
+    assert sb.n_numeric == 0
+
+
+def test_mesh_fp32_bulk_block_grid():
+    """FP32 bulk on the reference's own block grid (the fitted-surface faces keep the
+    FP64 exact solver, on the FP32-stored state): within L1 1e-4 of the FP64 mesh sweep."""
+    d = block("b")
+    a = alpha(d)
+    m = mesh_from_alpha(a)
+    l1, sa, sb = _fp32_vs_fp64(m, reference_params(), state_from_alpha(a), 200)
+    assert max(l1) <= 1e-4
+    assert abs(sb.dt_next / sa.dt_next - 1.0) < 1e-4
+
+
 def _tracked_block(tag):
     from conftest import golden
     from gelios_2d_amd.mesh import MeshRails, mesh_from_slots
