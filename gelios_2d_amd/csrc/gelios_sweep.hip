@@ -278,8 +278,8 @@ int gel_ctx_create_mesh(gel_ctx** out, const gel_mesh* g, const gel_params* prm)
     if (prm->physics != GEL_PHYS_GAS || (prm->precision != GEL_PREC_FP64 && prm->precision != GEL_PREC_FP32_BULK))
         return fail(GEL_E_UNSUPPORTED, "general meshes: gas physics, FP64 or FP32 bulk");
     if (prm->flux != GEL_FLUX_HLL && prm->flux != GEL_FLUX_HLLC) return fail(GEL_E_UNSUPPORTED, "flux");
-    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1)
-        return fail(GEL_E_UNSUPPORTED, "general meshes: no source or the MK1 source");
+    if (prm->source_mode != GEL_SRC_NONE && prm->source_mode != GEL_SRC_MK1 && prm->source_mode != GEL_SRC_TABLE)
+        return fail(GEL_E_UNSUPPORTED, "general meshes: no source, the MK1 source or a source table");
     if (!(prm->dt0 > 0.0) || !(prm->cfl > 0.0) || !(prm->gamma > 1.0)) return fail(GEL_E_ARG, "dt0, cfl, gamma");
     gel_ctx* c = new (std::nothrow) gel_ctx();
     if (!c) return fail(GEL_E_NOMEM, "host allocation");
@@ -404,7 +404,6 @@ int gel_set_sources(gel_ctx* c, const double* f)
 
 int gel_set_source_table(gel_ctx* c, const double* q)
 {
-    if (c && c->mesh) return fail(GEL_E_UNSUPPORTED, "source tables: polar grids");
     if (!c || !q) return fail(GEL_E_ARG, "null argument");
     if (c->prm.physics == GEL_PHYS_GAS4H) return fail(GEL_E_UNSUPPORTED, "the hydrogen fluids are part of the state");
     // stored in the state's type, like the moments; ordered after the work queued on

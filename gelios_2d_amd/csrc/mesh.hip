@@ -293,17 +293,24 @@ __global__ void __launch_bounds__(256) mesh_cell_kernel(MeshDev m, SweepArgs a)
 #pragma unroll
                 for (int v = 0; v < 5; v++) nxt[v * nc + c] = now[v * nc + c];
             } else {
-                double q1 = 0.0, q2 = 0.0, q3 = 0.0;
+                double qm = 0.0, q1 = 0.0, q2 = 0.0, q3 = 0.0, qQ = 0.0;
                 if constexpr (SRC == GEL_SRC_MK1) {
                     const TS* sp = (const TS*)a.src + c;
                     auto fk = [&](int k) { return (double)sp[(size_t)k * nc]; };
                     source_mk1<decltype(fk), double>(a.prm, fk, s[2], s[3], s[0], s[1], q1, q2, q3);
+                } else if constexpr (SRC == GEL_SRC_TABLE) {   // mass, momentum x, y, energy, tracer
+                    const TS* qt = (const TS*)a.qtab + c;
+                    qm = (double)qt[0];
+                    q1 = (double)qt[nc];
+                    q2 = (double)qt[2 * nc];
+                    q3 = (double)qt[3 * nc];
+                    qQ = (double)qt[4 * nc];
                 }
                 const double dt = dt_read(a);
                 double o[5];
                 cell_update<GasPhys, false>(s, Pot, m.cgeo[4 * nc + c], m.cgeo[5 * nc + c], m.cgeo[2 * nc + c],
-                                            m.cgeo[3 * nc + c], dt, a.prm.gamma, 0.0, q1, q2, q3, 0.0, false, o,
-                                            acc.n_rho, acc.n_p);
+                                            m.cgeo[3 * nc + c], dt, a.prm.gamma, qm, q1, q2, q3, qQ,
+                                            SRC == GEL_SRC_TABLE, o, acc.n_rho, acc.n_p);
 #pragma unroll
                 for (int v = 0; v < 5; v++) nxt[v * nc + c] = (TS)o[v];
             }
@@ -785,7 +792,9 @@ void gel_mesh_mc_view(const gel_ctx* c, MeshMcView* v, std::vector<int>& code, s
     const MeshDev& d = mb->dev;
     v->coff = d.coff; v->corners = d.corners; v->soff = d.soff; v->sA = d.sA; v->sB = d.sB;
     v->ctype = d.ctype;
-    const std::vector<int>&soff = mb->hsoff, &sA = mb->hsA, &sB = mb->hsB;
+    const std::vector<int>& soff = mb->hsoff;
+    const std::vector<int>& sA = mb->hsA;
+    const std::vector<int>& sB = mb->hsB;
     code.assign(mb->hstype.size(), -1);
     outer.clear();
     inner.clear();
@@ -1082,17 +1091,22 @@ hipError_t gel::gel_mesh_step(gel_ctx* c, SweepArgs& a, bool moving)
     }
     const int nfb = (int)groups(d.nf);
     const dim3 gf((unsigned)(nfb + (d.ng + 3) / 4)), gc(groups(d.nc)), blk(256);
-    const bool hllc = c->prm.flux == GEL_FLUX_HLLC, mk1 = c->prm.source_mode == GEL_SRC_MK1;
+    const bool hllc = c->prm.flux == GEL_FLUX_HLLC;
+    const int src = c->prm.source_mode;
     using FaceK = void (*)(MeshDev, SweepArgs, int);
     using CellK = void (*)(MeshDev, SweepArgs);
     FaceK fk;
     CellK ck;
     if (c->ts == 4) {   // FP32 bulk
         fk = hllc ? mesh_face_kernel<GEL_FLUX_HLLC, float> : mesh_face_kernel<GEL_FLUX_HLL, float>;
-        ck = mk1 ? mesh_cell_kernel<GEL_SRC_MK1, float> : mesh_cell_kernel<GEL_SRC_NONE, float>;
+        ck = src == GEL_SRC_MK1     ? mesh_cell_kernel<GEL_SRC_MK1, float>
+             : src == GEL_SRC_TABLE ? mesh_cell_kernel<GEL_SRC_TABLE, float>
+                                    : mesh_cell_kernel<GEL_SRC_NONE, float>;
     } else {
         fk = hllc ? mesh_face_kernel<GEL_FLUX_HLLC, double> : mesh_face_kernel<GEL_FLUX_HLL, double>;
-        ck = mk1 ? mesh_cell_kernel<GEL_SRC_MK1, double> : mesh_cell_kernel<GEL_SRC_NONE, double>;
+        ck = src == GEL_SRC_MK1     ? mesh_cell_kernel<GEL_SRC_MK1, double>
+             : src == GEL_SRC_TABLE ? mesh_cell_kernel<GEL_SRC_TABLE, double>
+                                    : mesh_cell_kernel<GEL_SRC_NONE, double>;
     }
     hipLaunchKernelGGL(fk, gf, blk, 0, s, d, a, nfb);
     hipLaunchKernelGGL(ck, gc, blk, 0, s, d, a);

@@ -185,7 +185,7 @@ int gel_ctx_create(gel_ctx** out, const gel_grid* grid, const gel_params* prm);
  * them.  point_type: 0, 1 P_Contact, 2 P_Inner_shock, 3 P_Outer_shock (NULL: none); a
  * face between two nodes of one fitted surface takes the exact solver, a face with one
  * such node the nul_potok flux (Setka.cpp:10287-10370).  The mesh context runs the
- * gas sweep (HLL / HLLC, GEL_SRC_NONE or GEL_SRC_MK1) in FP64 or in FP32 bulk (state
+ * gas sweep (HLL / HLLC; GEL_SRC_NONE, GEL_SRC_MK1 or GEL_SRC_TABLE) in FP64 or in FP32 bulk (state
  * stored in FP32, the Riemann solver of the faces other than the fitted-surface and
  * nul_potok ones in FP32; reconstruction, geometry and update in FP64; fitted-surface
  * tracking needs FP64) with gel_step (static) and
