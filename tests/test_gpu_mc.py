@@ -153,11 +153,11 @@ def test_device_mc_feeds_the_sweep():
 
 # ---------------------------------------------------------------- general mesh ----
 
-def _block_ctx(prim=None, source_mode=None):
+def _block_ctx(prim=None, source_mode=None, tag="b"):
     """A context on the reference's own block grid (the Setka(N1..M4) checkpoint), at rest."""
     from test_oracle_block import alpha, block
     from gelios_2d_amd.mesh import mesh_from_alpha
-    d = block("b")
+    d = block(tag)
     m = mesh_from_alpha(alpha(d))
     prm = reference_params()
     if source_mode is not None:
@@ -199,12 +199,12 @@ def test_mesh_tracer_equals_polar_tracer(kind):
     assert err.max() < 1e-9, (np.argmax(err.max(axis=1)), err.max())
 
 
-@pytest.mark.parametrize("kind", ["hot", "equilibrium"])
-def test_mesh_tracer_matches_oracle_on_the_block_grid(kind):
-    """The reference's own grid: the device walks the oracle's atoms one for one through
-    the block grid's cells (the central cells behind the inner sphere included) -- equal
-    counts, tallies within 1e-9."""
-    s, m = _block_ctx(lambda nc: _plasma(nc, kind))
+@pytest.mark.parametrize("kind,tag", [("hot", "b"), ("equilibrium", "b"), ("hot", "a")])
+def test_mesh_tracer_matches_oracle_on_the_block_grid(kind, tag):
+    """The reference's own grids (both block-grid checkpoints): the device walks the
+    oracle's atoms one for one through the cells (the central cells behind the inner
+    sphere included) -- equal counts, tallies within 1e-9."""
+    s, m = _block_ctx(lambda nc: _plasma(nc, kind), tag=tag)
     out_d, st_d = mc.run_mc(s, _p(20000, 5))
     out_o, st_o = ora.mc_run_mesh(m, _plasma(m.ncells, kind), ora.mc_population(m.cell_type, m.ncells), _p(20000, 5))
     print(kind, st_d, st_o)
